@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the DB-operator hot path on one MI355X.
+
+Headline workload: stable LSD radix sort of 2^28 (uint32 key, uint32 payload)
+pairs with keys uniform on [0, 2^32).  One step = one sort of the whole
+batch, inputs already resident in HBM.  The JSON line also carries the other
+operators' rows/s at their configured sizes, the dominant kernel's roofline
+(hipEvent time on the stream it runs on), and the CPU restatement timed on
+a bounded sample on this host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--only sort]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12
+METRIC = "input rows/s & %HBM-peak per operator; served queries/s + p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_steps(ctx, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    ctx.sync()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.set_profiling(False)
+    return (t1 - t0) / steps
+
+
+def kernel_avg_ms(ctx, name):
+    n, ms = ctx.kernel_stat(name)
+    return (ms / n if n else 0.0), n
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    from the summary committed under profiles/ (None when absent)."""
+    p = ROOT / "profiles" / f"{name}_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+# ---------------- operators ----------------
+
+def bench_sort(ctx, steps, warmup, n=1 << 28, zipf=False):
+    from pimdal_amd import operators as ops, datagen
+    keys, vals = datagen.sort_workload(ctx, n, zipf=zipf)
+    ok = ctx.empty(n, "uint32")
+    ov = ctx.empty(n, "uint32")
+    step = lambda: ops.sort_pairs(ctx, keys, vals, 0, 32, out_keys=ok, out_vals=ov)
+    t = timed_steps(ctx, step, steps, warmup)
+    pass_ms, launches = kernel_avg_ms(ctx, "radix_onesweep")
+    hist_ms, _ = kernel_avg_ms(ctx, "radix_hist")
+    for a in (keys, vals, ok, ov):
+        a.free()
+    return {
+        "n": n, "s_per_step": t, "rows_per_s": n / t,
+        "alg_bytes": 68 * n, "alg_GBps": 68 * n / t / 1e9, "hbm_frac": 68 * n / t / HBM_PEAK,
+        "keys_payload_GBps": 8 * n / t / 1e9,
+        "pass_ms": pass_ms, "pass_launches": launches, "hist_ms": hist_ms,
+        "pass_alg_bytes": 16 * n,
+    }
+
+
+def bench_select(ctx, steps, warmup, n=1 << 27):
+    from pimdal_amd import operators as ops, datagen
+    key, pay = datagen.select_workload(ctx, n)
+    out = {}
+    res = {}
+
+    def step():
+        res["r"] = ops.select(ctx, key, pay, 0, datagen.SELECT_THRESHOLD, sync=False)
+
+    t = timed_steps(ctx, step, steps, warmup)
+    cnt = int(res["r"][2].numpy()[0])
+    ms, _ = kernel_avg_ms(ctx, "select")
+    alg = 8 * n + 8 * cnt
+    out.update(n=n, selected=cnt, s_per_step=t, rows_per_s=n / t, alg_bytes=alg,
+               kernel_ms=ms, alg_GBps=alg / (ms / 1e3) / 1e9 if ms else None,
+               hbm_frac=alg / (ms / 1e3) / HBM_PEAK if ms else None)
+    key.free()
+    pay.free()
+    return out
+
+
+def cpu_baseline_sort(sample_n=1 << 24, reps=3):
+    """The CPU restatement (oracle/) of the stable pair sort, all host threads."""
+    import numpy as np
+    from oracle import orc
+    k, v = orc.sort_workload(sample_n)
+    orc.sort_pairs(k[:1024], v[:1024])
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        orc.sort_pairs(k, v)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    return {"value": sample_n / t, "unit": "rows/s", "cores": orc.num_threads(), "kind": "port",
+            "sample": f"2^{int(np.log2(sample_n))} uniform (u32,u32) pairs, stable sort, median of {reps}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--only", default="")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(each GPU box has one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import pimdal_amd
+    ctx = pimdal_amd.Context(0)
+    ops_out = {}
+    log("bench: sort 2^28 uniform")
+    srt = bench_sort(ctx, args.steps, args.warmup)
+    if args.only in ("", "all"):
+        log("bench: sort 2^28 zipf")
+        ops_out["sort_zipf_2^28"] = bench_sort(ctx, max(3, args.steps // 2), 1, zipf=True)
+        log("bench: select 2^27")
+        ops_out["select_2^27"] = bench_select(ctx, args.steps, args.warmup)
+    cpu = None
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline_sort()
+    pass_s = srt["pass_ms"] / 1e3
+    achieved = srt["pass_alg_bytes"] / pass_s if pass_s else None
+    line = {
+        "metric": METRIC,
+        "value": srt["rows_per_s"],
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": srt["s_per_step"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "stable LSD radix sort of 2^28 (u32 key, u32 payload) pairs, keys uniform on [0,2^32)",
+                   "rows": srt["n"], "passes": 4, "digit_bits": 8,
+                   "alg_bytes_per_row": 68, "sort_alg_GBps": srt["alg_GBps"], "sort_hbm_frac": srt["hbm_frac"],
+                   "keys_payload_GBps": srt["keys_payload_GBps"]},
+        "roofline": {"kernel": "radix_onesweep", "bound": "hbm",
+                     "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK if achieved else None,
+                     "traffic": load_traffic("radix_onesweep_2p28"),
+                     "alg_bytes_per_launch": srt["pass_alg_bytes"], "avg_launch_ms": srt["pass_ms"]},
+        "cpu_baseline": cpu,
+        "operators": ops_out,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

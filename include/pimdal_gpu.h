@@ -1,0 +1,155 @@
+/*
+ * pimdal_gpu.h — C ABI of the MI355X (gfx950) DB-operator library.
+ *
+ * Drop-in for PIMDAL's operator entry points on one GPU.  The reference
+ * exposes its operators as C++ functions over arrow::Table
+ * (reference/db_operators/select.cpp:77, sort.cpp:42, hash_join.cpp:159,
+ * sort_merge_join.cpp:105, aggregate.cpp:95) and as DPU kernels over MRAM
+ * argument structs (pimdal/select/sel.h:7-21, pimdal/sort/sort.h:7-25,
+ * pimdal/join/hash_join.h:6-82, pimdal/join/sort_merge.h:6-23,
+ * pimdal/aggregate/aggregate.h:7-25), with Arrow columns moved to the
+ * devices by pimdal/support/transfer_helper.h:87-283.  This header replaces
+ * all three with plain pointers and sizes:
+ *
+ *   - columns move in and out through the Arrow C Data Interface
+ *     (pimdal_col_import / pimdal_col_export) or raw host buffers;
+ *   - every operator has a column-level form (library allocates the
+ *     outputs, like the reference's functions returning a new Table) and a
+ *     device-pointer form (`*_dev`: caller passes device buffers, the form
+ *     the DPU kernels' argument structs correspond to).
+ *
+ * Conventions: every function returns 0 (PIMDAL_OK) or a negative
+ * PIMDAL_E_* code; pimdal_last_error() gives a thread-local message.  No C++
+ * exception crosses this boundary.  Work is issued on the context's stream;
+ * functions that return a count or allocate outputs synchronise it.  A
+ * context is used by one host thread at a time.
+ */
+#ifndef PIMDAL_GPU_H
+#define PIMDAL_GPU_H
+
+#include <stdint.h>
+#include "pimdal_arrow_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIMDAL_OK 0
+#define PIMDAL_E_INVALID (-1)      /* bad argument */
+#define PIMDAL_E_UNSUPPORTED (-2)  /* e.g. nullable or multi-chunk Arrow input */
+#define PIMDAL_E_NOMEM (-3)        /* HBM budget or device allocation exhausted */
+#define PIMDAL_E_HIP (-4)          /* HIP runtime error */
+#define PIMDAL_E_CAPACITY (-5)     /* output larger than the caller's buffer */
+#define PIMDAL_E_INTERNAL (-6)
+
+typedef struct pimdal_ctx pimdal_ctx;
+typedef struct pimdal_col pimdal_col;
+
+typedef enum {
+    PIMDAL_U8 = 1,
+    PIMDAL_I32 = 2,
+    PIMDAL_U32 = 3,
+    PIMDAL_I64 = 4,
+    PIMDAL_U64 = 5,
+    PIMDAL_I128 = 6  /* Arrow decimal128(38, 0), little-endian two's complement */
+} pimdal_type;
+
+/* ---------------- context ----------------
+ * Replaces dpu_alloc / dpu_load and the global transfer contexts
+ * (pimdal/support/transfer_helper.h:20-22; pimdal/main/select/host/
+ * host_select_sync.cpp:42-45). */
+const char* pimdal_last_error(void);
+int pimdal_version(void);
+int pimdal_ctx_create(int device, uint64_t hbm_budget_bytes, pimdal_ctx** out);
+int pimdal_ctx_destroy(pimdal_ctx* ctx);
+int pimdal_ctx_sync(pimdal_ctx* ctx);
+/* The context's hipStream_t (default priority). */
+int pimdal_ctx_stream(pimdal_ctx* ctx, void** stream);
+/* Budget accounting: bytes live, reserved (live + cached), peak, budget. */
+int pimdal_ctx_mem_stats(pimdal_ctx* ctx, uint64_t* live, uint64_t* reserved, uint64_t* peak, uint64_t* budget);
+int pimdal_ctx_set_budget(pimdal_ctx* ctx, uint64_t hbm_budget_bytes);
+/* Per-kernel hipEvent timing on the stream each kernel ran on. */
+int pimdal_ctx_set_profiling(pimdal_ctx* ctx, int on);
+int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
+int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
+/* Tuning knobs (radix bits, fan-out, tile shapes); names in DESIGN.md. */
+int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
+int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
+
+/* ---------------- device memory ---------------- */
+int pimdal_dev_alloc(pimdal_ctx* ctx, uint64_t bytes, void** dptr);
+int pimdal_dev_free(pimdal_ctx* ctx, void* dptr);
+/* Host <-> device copies through pinned staging; synchronous. */
+int pimdal_memcpy_h2d(pimdal_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int pimdal_memcpy_d2h(pimdal_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---------------- columns ----------------
+ * Replaces dist_table / get_buf (pimdal/support/transfer_helper.h:207-236):
+ * a column is one device-resident, non-nullable, fixed-width array. */
+int pimdal_col_alloc(pimdal_ctx* ctx, pimdal_type type, uint64_t length, pimdal_col** out);
+int pimdal_col_from_host(pimdal_ctx* ctx, pimdal_type type, const void* data, uint64_t length, pimdal_col** out);
+/* Wraps caller-owned device memory (not freed by pimdal_col_free). */
+int pimdal_col_wrap(pimdal_ctx* ctx, pimdal_type type, void* dptr, uint64_t length, pimdal_col** out);
+int pimdal_col_to_host(pimdal_ctx* ctx, const pimdal_col* col, void* dst, uint64_t dst_bytes);
+int pimdal_col_info(const pimdal_col* col, pimdal_type* type, uint64_t* length, void** dptr);
+int pimdal_col_free(pimdal_ctx* ctx, pimdal_col* col);
+/* Arrow C Data Interface.  Import borrows the array until it returns and
+ * rejects nullable (null_count != 0 with a validity buffer), non-primitive
+ * or unsupported types with PIMDAL_E_UNSUPPORTED.  Export allocates host
+ * buffers owned by `out`; the caller calls out->release and
+ * out_schema->release. */
+int pimdal_col_import(pimdal_ctx* ctx, const struct ArrowArray* array, const struct ArrowSchema* schema, pimdal_col** out);
+int pimdal_col_export(pimdal_ctx* ctx, const pimdal_col* col, struct ArrowArray* out, struct ArrowSchema* out_schema);
+
+/* ---------------- selection ----------------
+ * reference/db_operators/select.cpp:77 select(table, column) and
+ * pimdal/select/sel.h:21 sel_kernel.  Keeps rows with lo <= key < hi (the
+ * predicate form of select.cpp:33-36, bounds as arguments), order
+ * preserving.  Outputs the payload and the row index of every kept row;
+ * pass the key column as payload to get the reference's compacted keys.
+ * key type: PIMDAL_U32 or PIMDAL_I32 (bounds are read as that type).
+ * out_payload / out_rowid may be NULL to skip them. */
+int pimdal_select(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* payload, int64_t lo, int64_t hi,
+                  pimdal_col** out_payload, pimdal_col** out_rowid);
+/* Device form: out buffers hold n rows; *d_count (device) receives the
+ * count.  Asynchronous. */
+int pimdal_select_dev(pimdal_ctx* ctx, pimdal_type key_type, const void* key, const uint32_t* payload, uint64_t n,
+                      int64_t lo, int64_t hi, uint32_t* out_payload, uint32_t* out_rowid, uint64_t* d_count);
+
+/* ---------------- sort ----------------
+ * reference/db_operators/sort.cpp:42 sort(table, column) and
+ * pimdal/sort/sort.h:20-25 sort_kernel.  Stable LSD radix sort of
+ * (uint32 key, payload) pairs; payload width 4 (PIMDAL_U32/I32) or 8
+ * (PIMDAL_U64/I64) bytes, or NULL for keys only.  Bits that are equal in
+ * every key are not sorted on (detected on the device). */
+int pimdal_sort_pairs(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* payload,
+                      pimdal_col** out_key, pimdal_col** out_payload);
+/* Device form.  Sorts bits [begin_bit, end_bit) of the key; pass 0, 32 for
+ * the whole key, or begin_bit = end_bit = -1 to detect the varying bits
+ * (one device->host read).  val_width 0, 4 or 8.  Asynchronous unless
+ * detecting bits. */
+int pimdal_sort_pairs_dev(pimdal_ctx* ctx, const uint32_t* keys_in, const void* vals_in, uint32_t val_width,
+                          uint64_t n, uint32_t* keys_out, void* vals_out, int begin_bit, int end_bit);
+
+/* ---------------- synthetic data (device generators) ----------------
+ * Counter-based generators, identical to include/pimdal_gen.h on the host;
+ * they replace reference/db_operators/shared.cpp:9-60. */
+int pimdal_gen_u32(pimdal_ctx* ctx, uint64_t seed, uint32_t column, uint64_t n, uint32_t* d_out);
+int pimdal_gen_range(pimdal_ctx* ctx, pimdal_type type, uint64_t seed, uint32_t column, uint64_t n,
+                     int64_t lo, int64_t hi, void* d_out);
+int pimdal_gen_perm(pimdal_ctx* ctx, uint64_t seed, uint32_t column, uint64_t m, uint64_t offset, uint64_t n,
+                    uint32_t* d_out);
+/* d_out[i] = perm_m(d_in[i] + delta) */
+int pimdal_perm_apply(pimdal_ctx* ctx, uint64_t seed, uint32_t column, uint64_t m, uint64_t n, int64_t delta,
+                      const uint32_t* d_in, uint32_t* d_out);
+int pimdal_gen_iota(pimdal_ctx* ctx, uint64_t n, uint32_t start, uint32_t* d_out);
+int pimdal_gen_zipf(pimdal_ctx* ctx, uint64_t seed, uint32_t column, uint64_t m, double s, uint64_t n,
+                    uint32_t* d_out_rank);
+int pimdal_fmix32(pimdal_ctx* ctx, uint64_t n, const uint32_t* d_in, uint32_t* d_out);
+int pimdal_gen_orderkey(pimdal_ctx* ctx, uint64_t seed, uint64_t n, uint64_t groups, int32_t* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIMDAL_GPU_H */

@@ -1,0 +1,133 @@
+// pd_device.hpp — device-side building blocks shared by the gfx950 kernels:
+// wave64 lane masks and scans, agent-scope status words for decoupled
+// look-back, and the reference hash functions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PD_WAVE 64
+
+namespace pd {
+
+__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+
+// Lanes strictly below this one.
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    return (1ull << lane()) - 1ull;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t popc64(uint64_t x) { return (uint32_t)__popcll(x); }
+
+// ---- agent-scope atomics (visible across the 8 XCDs' L2s) ----
+template <typename T>
+__device__ __forceinline__ T ld_acquire(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_relaxed(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_release(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_relaxed(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- wave64 inclusive scan of uint32 via DPP-friendly shuffles ----
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t l = lane();
+#pragma unroll
+    for (int d = 1; d < PD_WAVE; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, PD_WAVE);
+        if (l >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+    const uint32_t l = lane();
+#pragma unroll
+    for (int d = 1; d < PD_WAVE; d <<= 1) {
+        uint64_t t = __shfl_up(v, d, PD_WAVE);
+        if (l >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = PD_WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, PD_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+    for (int d = PD_WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, PD_WAVE);
+    return v;
+}
+
+// ---- single-value decoupled look-back (one running total per tile) ----
+// Status word: bits 63..62 flag (0 empty, 1 tile aggregate, 2 inclusive
+// prefix), bits 61..0 value.  One wave scans back over up to 64
+// predecessors per step.
+constexpr uint64_t LB_FLAG_AGG = 1ull << 62;
+constexpr uint64_t LB_FLAG_PRE = 2ull << 62;
+constexpr uint64_t LB_VALUE = (1ull << 62) - 1;
+
+// Called by every lane of ONE wave of the tile's block.  Publishes the tile
+// aggregate, accumulates the exclusive prefix and publishes the inclusive
+// prefix.  Returns the exclusive prefix (same value in all lanes).
+__device__ __forceinline__ uint64_t lookback_single(uint64_t* status, uint64_t tile, uint64_t agg) {
+    if (tile == 0) {
+        if (lane() == 0) st_relaxed(&status[0], LB_FLAG_PRE | agg);
+        return 0;
+    }
+    if (lane() == 0) st_relaxed(&status[tile], LB_FLAG_AGG | agg);
+    uint64_t excl = 0;
+    int64_t base = (int64_t)tile - 1;  // window covers [base - 63, base]
+    while (true) {
+        const int64_t idx = base - (int64_t)lane();
+        uint64_t s = (idx >= 0) ? ld_relaxed(&status[idx]) : LB_FLAG_PRE;  // before tile 0: prefix 0
+        // Wait until every lane sees a non-empty status.
+        while (ballot((s >> 62) == 0)) {
+            if ((s >> 62) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                s = ld_relaxed(&status[idx]);
+            }
+        }
+        const uint64_t pre_mask = ballot((s >> 62) == 2);
+        uint64_t v = s & LB_VALUE;
+        if (pre_mask) {
+            const uint32_t first = __builtin_ctzll(pre_mask);  // nearest inclusive prefix
+            if (lane() > first) v = 0;
+            excl += wave_sum64(v);
+            break;
+        }
+        excl += wave_sum64(v);
+        base -= PD_WAVE;
+    }
+    if (lane() == 0) st_relaxed(&status[tile], LB_FLAG_PRE | (excl + agg));
+    return excl;
+}
+
+// ---- hashing ----
+// reference/db_operators/hash_join.cpp:20-25: low 32 bits of the
+// byte-swapped 64-bit product with 0x9E3779B97F4A7C15.
+__device__ __forceinline__ uint32_t ref_hash(uint32_t x) {
+    uint64_t p = (uint64_t)x * 11400714785074694791ull;
+    return (uint32_t)__builtin_bswap64(p);
+}
+
+// Bucket inside a partition's table: bits of the same product that the
+// partition number does not use.
+__device__ __forceinline__ uint32_t slot_hash(uint32_t x) {
+    uint64_t p = (uint64_t)x * 11400714785074694791ull;
+    return (uint32_t)(p >> 16) ^ (uint32_t)(p >> 40);
+}
+
+}  // namespace pd

@@ -1,0 +1,157 @@
+// pd_runtime.hpp — host runtime shared by the operators: the context (device,
+// stream, caching device allocator under an HBM budget, pinned staging,
+// per-kernel hipEvent timing) and error propagation to the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+#include "../../include/pimdal_gpu.h"
+
+namespace pd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define PD_HIP(x) ::pd::hip_check((x), #x, __FILE__, __LINE__)
+#define PD_REQUIRE(cond, code, msg) do { if (!(cond)) ::pd::fail((code), (msg)); } while (0)
+
+void set_last_error(const std::string& m);
+
+// Caching device allocator.  Blocks are rounded to 256 B (small) or 2 MiB
+// (large) and kept per rounded size after free; a cached block is reused on
+// the stream it was freed on, or on any stream once its free-event has
+// completed.  Reserved bytes (live + cached) never exceed the budget: cached
+// blocks are released first, then the request fails with PIMDAL_E_NOMEM.
+class DevicePool {
+public:
+    void init(int device, uint64_t budget);
+    void* alloc(uint64_t bytes, hipStream_t stream);
+    void free(void* p, hipStream_t stream);
+    void release_cached();
+    void destroy();
+    uint64_t live() const { return live_; }
+    uint64_t reserved() const { return reserved_; }
+    uint64_t peak() const { return peak_; }
+    uint64_t budget() const { return budget_; }
+    void set_budget(uint64_t b) { budget_ = b; }
+private:
+    struct Block { void* p; uint64_t size; hipStream_t stream; hipEvent_t ev; };
+    static uint64_t round_size(uint64_t b);
+    int device_ = 0;
+    uint64_t budget_ = 0, live_ = 0, reserved_ = 0, peak_ = 0;
+    std::multimap<uint64_t, Block> cached_;
+    std::map<void*, uint64_t> live_blocks_;
+    std::mutex mu_;
+};
+
+struct KernelStat { uint64_t launches = 0; double total_ms = 0.0; };
+
+}  // namespace pd
+
+struct pimdal_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pd::DevicePool pool;
+    // pinned staging for host <-> device column copies (double buffered)
+    void* pinned[2] = {nullptr, nullptr};
+    hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+    uint64_t pinned_bytes = 0;
+    // profiling: hipEvent pairs per kernel name on the stream it ran on
+    bool profiling = false;
+    struct Pending { std::string name; hipEvent_t e0, e1; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, pd::KernelStat> stats;
+    // device properties
+    int cu_count = 256;
+    int xcd_count = 8;
+    // tuning parameters (autotuner writes these)
+    std::map<std::string, int64_t> tune;
+    int64_t tune_get(const std::string& k, int64_t dflt) const {
+        auto it = tune.find(k);
+        return it == tune.end() ? dflt : it->second;
+    }
+};
+
+struct pimdal_col {
+    pimdal_type type;
+    uint64_t length;
+    uint64_t capacity;
+    void* data;      // device memory
+    bool owned;      // allocated from the context pool
+};
+
+namespace pd {
+
+uint32_t type_width(pimdal_type t);
+
+// Temporary device buffer from the context pool, freed on scope exit
+// (stream ordered on the context stream).
+class Scratch {
+public:
+    Scratch(pimdal_ctx* c, uint64_t bytes, hipStream_t s = nullptr)
+        : c_(c), s_(s ? s : c->stream), p_(bytes ? c->pool.alloc(bytes, s ? s : c->stream) : nullptr) {}
+    ~Scratch() { if (p_) c_->pool.free(p_, s_); }
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p_); }
+    void* get() const { return p_; }
+private:
+    pimdal_ctx* c_;
+    hipStream_t s_;
+    void* p_;
+};
+
+// Kernel launch with optional hipEvent timing on the launch stream.
+hipEvent_t take_event(pimdal_ctx* c);
+void record_launch(pimdal_ctx* c, const char* name, hipEvent_t e0, hipStream_t s);
+
+template <typename... KArgs, typename... Args>
+void launch(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+            uint32_t shmem, hipStream_t s, Args... args) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    hipEvent_t e0 = nullptr;
+    if (c->profiling) {
+        e0 = take_event(c);
+        PD_HIP(hipEventRecord(e0, s));
+    }
+    hipLaunchKernelGGL(kernel, grid, block, shmem, s, static_cast<KArgs>(args)...);
+    PD_HIP(hipGetLastError());
+    if (c->profiling) record_launch(c, name, e0, s);
+}
+
+// Fill device memory with a small kernel (graph-capture safe, unlike
+// hipMemsetAsync).
+void fill_u32(pimdal_ctx* c, void* p, uint32_t value, uint64_t count, hipStream_t s);
+void fill_u64(pimdal_ctx* c, void* p, uint64_t value, uint64_t count, hipStream_t s);
+void copy_d2d(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes, hipStream_t s);
+
+uint64_t read_u64(pimdal_ctx* c, const void* dev, hipStream_t s);
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+}  // namespace pd
+
+// C-ABI boundary: run `body`, translate exceptions into return codes.
+#define PD_API_BEGIN try {
+#define PD_API_END                                                       \
+    return PIMDAL_OK;                                                    \
+    } catch (const pd::Error& e) {                                       \
+        pd::set_last_error(e.what());                                    \
+        return e.code;                                                   \
+    } catch (const std::bad_alloc&) {                                    \
+        pd::set_last_error("host allocation failed");                    \
+        return PIMDAL_E_NOMEM;                                           \
+    } catch (const std::exception& e) {                                  \
+        pd::set_last_error(e.what());                                    \
+        return PIMDAL_E_INTERNAL;                                        \
+    }

@@ -1,0 +1,365 @@
+// radix_sort.hip — stable LSD radix sort of (uint32 key, payload) pairs,
+// onesweep style.
+//
+// Reference semantics: reference/db_operators/sort.cpp:30-40 (sort of the key
+// column) and pimdal/sort/sort_keyval.c:114-179 with sort_keyval_func.c:
+// 255-364 (value-range partition plus per-split quicksort of (key, row id)
+// pairs, unstable).  Here the order is the unique stable order by key.
+//
+// Kernels:
+//  k_rs_hist      one read of the keys: LDS digit histograms for every pass
+//                 plus the AND/OR of all keys (bits equal in every key need
+//                 no pass);
+//  k_rs_scan      exclusive scan of each pass's 256-bin histogram;
+//  k_rs_onesweep  one pass per 8-bit digit: a block takes the next 8192-key
+//                 tile from an atomic counter, ranks its keys stably with
+//                 per-wave LDS match masks (ds_or of lane bits, read back,
+//                 popcount below the lane), publishes its per-digit counts,
+//                 resolves each digit's global offset by decoupled look-back
+//                 over earlier tiles, reorders the tile in LDS by digit and
+//                 writes keys and payloads out in digit runs.
+#include "pd_runtime.hpp"
+#include "pd_device.hpp"
+#include "pd_ops.hpp"
+
+namespace pd {
+
+constexpr int RS_BLOCK = 512;
+constexpr int RS_WAVES = RS_BLOCK / PD_WAVE;   // 8
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;   // 8192
+constexpr int RS_BINS = 256;
+constexpr uint32_t RS_FLAG_AGG = 1u << 30;
+constexpr uint32_t RS_FLAG_PRE = 2u << 30;
+constexpr uint32_t RS_VALUE = (1u << 30) - 1;
+
+struct RsPassDesc {
+    uint32_t shift[4];
+    uint32_t mask[4];
+    uint32_t npass;
+};
+
+// ---- histogram of every pass + AND/OR of the keys ----
+__global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n, RsPassDesc pd,
+                                                 uint32_t* __restrict__ g_hist, uint32_t* __restrict__ g_andor) {
+    __shared__ uint32_t h[4][RS_BINS];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < 4 * RS_BINS; i += blockDim.x) (&h[0][0])[i] = 0;
+    __syncthreads();
+    uint32_t a = 0xFFFFFFFFu, o = 0;
+    const uint64_t nvec = ((uintptr_t)keys % 16 == 0) ? n / 4 : 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + t; v < nvec; v += stride) {
+        const uint4 q = reinterpret_cast<const uint4*>(keys)[v];
+        const uint32_t k4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            a &= k4[e];
+            o |= k4[e];
+            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(k4[e] >> pd.shift[p]) & pd.mask[p]], 1u);
+        }
+    }
+    for (uint64_t i = nvec * 4 + blockIdx.x * (uint64_t)blockDim.x + t; i < n; i += stride) {
+        const uint32_t k = keys[i];
+        a &= k;
+        o |= k;
+        for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(k >> pd.shift[p]) & pd.mask[p]], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < pd.npass * RS_BINS; i += blockDim.x) {
+        const uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&g_hist[i], v);
+    }
+    // wave AND/OR, one atomic per wave
+#pragma unroll
+    for (int d = PD_WAVE / 2; d > 0; d >>= 1) {
+        a &= __shfl_xor(a, d, PD_WAVE);
+        o |= __shfl_xor(o, d, PD_WAVE);
+    }
+    if (pd::lane() == 0) {
+        atomicAnd(&g_andor[0], a);
+        atomicOr(&g_andor[1], o);
+    }
+}
+
+// ---- exclusive scan of each pass's histogram (one block per pass) ----
+__global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict__ g_hist, uint32_t* __restrict__ g_offs) {
+    __shared__ uint32_t s_w[RS_BINS / PD_WAVE];
+    const uint32_t t = threadIdx.x, p = blockIdx.x;
+    const uint32_t v = g_hist[p * RS_BINS + t];
+    const uint32_t inc = pd::wave_incl_scan(v);
+    if (pd::lane() == PD_WAVE - 1) s_w[t / PD_WAVE] = inc;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t i = 0; i < t / PD_WAVE; i++) add += s_w[i];
+    g_offs[p * RS_BINS + t] = add + inc - v;
+}
+
+// ---- one onesweep pass ----
+template <typename V, bool HAS_VALS>
+__global__ __launch_bounds__(RS_BLOCK, 2) void k_rs_onesweep(const uint32_t* __restrict__ kin, const V* __restrict__ vin,
+                                                             uint32_t* __restrict__ kout, V* __restrict__ vout,
+                                                             uint64_t n, uint32_t shift, uint32_t mask,
+                                                             const uint32_t* __restrict__ offs,
+                                                             uint32_t* __restrict__ status,
+                                                             uint32_t* __restrict__ tile_ctr) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_match[RS_WAVES][RS_BINS];
+    __shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+    __shared__ uint32_t s_local[RS_BINS];
+    __shared__ uint32_t s_gbase[RS_BINS];
+    __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
+    __shared__ uint8_t s_dig[RS_TILE];
+    __shared__ __align__(16) uint32_t s_buf[RS_TILE * (HAS_VALS ? sizeof(V) / 4 : 1)];
+
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
+        s_match[w][i] = 0;
+        s_cnt[w][i] = 0;
+    }
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    const uint64_t tile_base = tile * RS_TILE;
+    const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * RS_ITEMS);
+    const uint32_t tile_count = (uint32_t)min((uint64_t)RS_TILE, n - tile_base);
+
+    // ---- load keys (and payloads), wave-striped: idx = wave_base + i*64 + lane ----
+    uint32_t key[RS_ITEMS];
+    V val[HAS_VALS ? RS_ITEMS : 1];
+    uint32_t dig[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
+        const bool ok = idx < n;
+        key[i] = ok ? kin[idx] : 0xFFFFFFFFu;
+        if constexpr (HAS_VALS) val[i] = ok ? vin[idx] : V(0);
+        dig[i] = ok ? ((key[i] >> shift) & mask) : (RS_BINS - 1);
+    }
+
+    // ---- stable rank inside the wave: match mask per item ----
+    const uint64_t lt = pd::lanemask_lt();
+    uint32_t rank[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const uint32_t d = dig[i];
+        atomicOr((unsigned long long*)&s_match[w][d], (unsigned long long)(1ull << l));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const uint64_t m = __hip_atomic_load(&s_match[w][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const uint32_t below = pd::popc64(m & lt);
+        const uint32_t old = s_cnt[w][d];
+        rank[i] = old + below;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (below == 0) {
+            s_cnt[w][d] = old + pd::popc64(m);
+            __hip_atomic_store(&s_match[w][d], (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    __syncthreads();
+
+    // ---- per digit: exclusive prefix over waves, tile count, publish ----
+    uint32_t cnt_d = 0, inc_d = 0;
+    if (t < RS_BINS) {
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ww++) {
+            const uint32_t c = s_cnt[ww][t];
+            s_cnt[ww][t] = cnt_d;
+            cnt_d += c;
+        }
+        pd::st_relaxed(&status[tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
+        // block exclusive scan of cnt_d over the 256 digits (waves 0..3)
+        inc_d = pd::wave_incl_scan(cnt_d);
+        if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
+    }
+    __syncthreads();
+    if (t < RS_BINS) {
+        uint32_t add = 0;
+        for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
+        s_local[t] = add + inc_d - cnt_d;
+    }
+    __syncthreads();
+
+    // ---- reorder the tile in LDS by digit ----
+    uint32_t pos[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const uint32_t d = dig[i];
+        pos[i] = s_local[d] + s_cnt[w][d] + rank[i];
+        s_buf[pos[i]] = key[i];
+        s_dig[pos[i]] = (uint8_t)d;
+    }
+
+    // ---- global offset of each digit: decoupled look-back ----
+    if (t < RS_BINS) {
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int64_t j = (int64_t)tile - 1;
+            while (j >= 0) {
+                uint32_t s = pd::ld_relaxed(&status[(uint64_t)j * RS_BINS + t]);
+                if ((s >> 30) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & RS_VALUE;
+                if ((s >> 30) == 2) break;
+                j--;
+            }
+            pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt_d));
+        }
+        s_gbase[t] = offs[t] + excl - s_local[t];
+    }
+    __syncthreads();
+
+    // ---- write keys in digit runs ----
+#pragma unroll 4
+    for (uint32_t j = t; j < tile_count; j += RS_BLOCK) {
+        const uint64_t dst = (uint64_t)s_gbase[s_dig[j]] + j;
+        if (dst < n) kout[dst] = s_buf[j];
+    }
+    if constexpr (HAS_VALS) {
+        __syncthreads();
+        V* s_val = reinterpret_cast<V*>(s_buf);
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; i++) s_val[pos[i]] = val[i];
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t j = t; j < tile_count; j += RS_BLOCK) {
+            const uint64_t dst = (uint64_t)s_gbase[s_dig[j]] + j;
+            if (dst < n) vout[dst] = s_val[j];
+        }
+    }
+}
+
+template <typename V, bool HV>
+static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
+                       const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
+    const uint64_t ntiles = ceil_div(n, RS_TILE);
+    const uint32_t P = pdsc.npass;
+    Scratch tk(c, P > 1 ? n * 4 : 0, s);
+    Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
+    Scratch st(c, ntiles * RS_BINS * 4, s);
+    const uint32_t* ksrc = kin;
+    const V* vsrc = vin;
+    for (uint32_t j = 0; j < P; j++) {
+        const bool to_out = ((P - 1 - j) % 2) == 0;
+        uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
+        V* vdst = to_out ? vout : tv.as<V>();
+        fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
+        launch(c, "radix_onesweep", k_rs_onesweep<V, HV>, dim3((uint32_t)ntiles), dim3(RS_BLOCK), 0, s, ksrc, vsrc,
+               kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j);
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+}
+
+void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t* kout,
+                void* vout, int begin_bit, int end_bit, hipStream_t s) {
+    PD_REQUIRE(vw == 0 || vw == 4 || vw == 8, PIMDAL_E_UNSUPPORTED, "payload width must be 0, 4 or 8 bytes");
+    PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "radix sort supports up to 2^30-1 pairs per call");
+    PD_REQUIRE(n == 0 || (kin && kout), PIMDAL_E_INVALID, "key pointer is NULL");
+    PD_REQUIRE(vw == 0 || n == 0 || (vin && vout), PIMDAL_E_INVALID, "payload pointer is NULL");
+    PD_REQUIRE(n == 0 || kin != kout, PIMDAL_E_INVALID, "in-place sort is not supported (keys_in == keys_out)");
+    PD_REQUIRE(vw == 0 || n == 0 || vin != vout, PIMDAL_E_INVALID, "in-place sort is not supported (vals_in == vals_out)");
+    const bool detect = begin_bit < 0;
+    PD_REQUIRE(detect || (begin_bit >= 0 && begin_bit <= end_bit && end_bit <= 32), PIMDAL_E_INVALID,
+               "bad bit range");
+    if (n == 0) return;
+
+    // small scratch: hist[4*256] | offs[4*256] | andor[2] | ctr[4]
+    Scratch small(c, (4 * RS_BINS * 2 + 8) * 4, s);
+    uint32_t* hist = small.as<uint32_t>();
+    uint32_t* offs = hist + 4 * RS_BINS;
+    uint32_t* andor = offs + 4 * RS_BINS;
+    uint32_t* ctrs = andor + 2;
+    fill_u32(c, hist, 0, 4 * RS_BINS * 2, s);
+    fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
+    fill_u32(c, andor + 1, 0, 1 + 4, s);
+
+    RsPassDesc all{};
+    if (detect) {
+        all.npass = 4;
+        for (int p = 0; p < 4; p++) { all.shift[p] = 8 * p; all.mask[p] = 0xFF; }
+    } else {
+        all.npass = 0;
+        for (int b = begin_bit; b < end_bit; b += 8) {
+            all.shift[all.npass] = b;
+            all.mask[all.npass] = (uint32_t)((1ull << std::min(8, end_bit - b)) - 1);
+            all.npass++;
+        }
+    }
+    const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
+    if (all.npass)
+        launch(c, "radix_hist", k_rs_hist, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
+
+    RsPassDesc run = all;
+    if (detect) {
+        uint32_t ao[2];
+        PD_HIP(hipMemcpyAsync(ao, andor, 8, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        const uint32_t varying = ao[0] ^ ao[1];
+        // keep only digits with a varying bit; move their histograms down
+        run.npass = 0;
+        for (int p = 0; p < 4; p++) {
+            if ((varying >> (8 * p)) & 0xFF) {
+                if (run.npass != (uint32_t)p)
+                    copy_d2d(c, hist + run.npass * RS_BINS, hist + p * RS_BINS, RS_BINS * 4, s);
+                run.shift[run.npass] = 8 * p;
+                run.mask[run.npass] = 0xFF;
+                run.npass++;
+            }
+        }
+    }
+    if (run.npass == 0) {
+        copy_d2d(c, kout, kin, n * 4, s);
+        if (vw) copy_d2d(c, vout, vin, n * vw, s);
+        return;
+    }
+    launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
+    if (vw == 0)
+        run_passes<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
+    else if (vw == 4)
+        run_passes<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
+    else
+        run_passes<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
+}
+
+}  // namespace pd
+
+using namespace pd;
+
+extern "C" int pimdal_sort_pairs_dev(pimdal_ctx* c, const uint32_t* keys_in, const void* vals_in, uint32_t val_width,
+                                     uint64_t n, uint32_t* keys_out, void* vals_out, int begin_bit, int end_bit) {
+    PD_API_BEGIN
+    PD_REQUIRE(c, PIMDAL_E_INVALID, "ctx is NULL");
+    sort_pairs(c, keys_in, vals_in, val_width, n, keys_out, vals_out, begin_bit, end_bit, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_sort_pairs(pimdal_ctx* c, const pimdal_col* key, const pimdal_col* payload,
+                                 pimdal_col** out_key, pimdal_col** out_payload) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && key && out_key, PIMDAL_E_INVALID, "NULL argument");
+    PD_REQUIRE(key->type == PIMDAL_U32, PIMDAL_E_UNSUPPORTED, "sort key must be uint32");
+    PD_REQUIRE(!payload || payload->length == key->length, PIMDAL_E_INVALID, "payload length differs");
+    PD_REQUIRE(!payload || out_payload, PIMDAL_E_INVALID, "out_payload is NULL");
+    const uint64_t n = key->length;
+    const uint32_t vw = payload ? type_width(payload->type) : 0;
+    PD_REQUIRE(vw == 0 || vw == 4 || vw == 8, PIMDAL_E_UNSUPPORTED, "payload must be 4 or 8 bytes wide");
+    pimdal_col* ok = nullptr;
+    pimdal_col* ov = nullptr;
+    try {
+        ok = new pimdal_col{PIMDAL_U32, n, n, c->pool.alloc(std::max<uint64_t>(n, 1) * 4, c->stream), true};
+        if (payload) ov = new pimdal_col{payload->type, n, n, c->pool.alloc(std::max<uint64_t>(n, 1) * vw, c->stream), true};
+        sort_pairs(c, (const uint32_t*)key->data, payload ? payload->data : nullptr, vw, n, (uint32_t*)ok->data,
+                   ov ? ov->data : nullptr, -1, -1, c->stream);
+        PD_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        if (ok) { c->pool.free(ok->data, c->stream); delete ok; }
+        if (ov) { c->pool.free(ov->data, c->stream); delete ov; }
+        throw;
+    }
+    *out_key = ok;
+    if (out_payload) *out_payload = ov;
+    PD_API_END
+}

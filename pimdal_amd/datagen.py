@@ -1,0 +1,105 @@
+"""Synthetic workloads generated on the device (counter-based, seed-fixed).
+
+Each recipe produces the columns of one benchmark workload directly in HBM
+with the generators of include/pimdal_gen.h, so the same (seed, column, row)
+gives the same value as the host restatement.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import call
+from .device import Context, DeviceArray
+
+COL = dict(KEY=1, PAYLOAD=2, BUILD_PERM=3, PROBE_FK=4, BUILD_PAY=5, PROBE_PAY=6, ZIPF=7, QTY=12, PRICE=13,
+           DISC=14, TAX=15, DATE=16, FLAG1=17, FLAG2=18)
+
+SEEDS = {"select": 0x5EED0001, "sort": 0x5EED0002, "join": 0x5EED0003, "groupby": 0x5EED0004, "serve": 0x5EED0005}
+
+SELECT_THRESHOLD = 429496730  # ceil(0.1 * 2^32)
+
+
+def u32(ctx: Context, seed: int, col: int, n: int) -> DeviceArray:
+    d = ctx.empty(n, np.uint32)
+    call("pimdal_gen_u32", ctx.handle, seed, col, n, d.vp)
+    return d
+
+
+def rng_range(ctx: Context, dtype, seed: int, col: int, n: int, lo: int, hi: int) -> DeviceArray:
+    from .device import pimdal_type
+    d = ctx.empty(n, dtype)
+    call("pimdal_gen_range", ctx.handle, pimdal_type(dtype), seed, col, n, lo, hi, d.vp)
+    return d
+
+
+def iota(ctx: Context, n: int) -> DeviceArray:
+    """Row ids 0..n-1 (the identity permutation restricted to n)."""
+    d = ctx.empty(n, np.uint32)
+    call("pimdal_gen_iota", ctx.handle, n, 0, d.vp)
+    return d
+
+
+def perm(ctx: Context, seed: int, col: int, m: int, n: int, offset: int = 0) -> DeviceArray:
+    d = ctx.empty(n, np.uint32)
+    call("pimdal_gen_perm", ctx.handle, seed, col, m, offset, n, d.vp)
+    return d
+
+
+def zipf_ranks(ctx: Context, seed: int, col: int, m: int, s: float, n: int) -> DeviceArray:
+    d = ctx.empty(n, np.uint32)
+    call("pimdal_gen_zipf", ctx.handle, seed, col, m, float(s), n, d.vp)
+    return d
+
+
+# ---------------- workload recipes ----------------
+
+def select_workload(ctx: Context, n: int, seed: int = SEEDS["select"]):
+    """uint32 key uniform on [0, 2^32) and uint32 payload."""
+    return u32(ctx, seed, COL["KEY"], n), u32(ctx, seed, COL["PAYLOAD"], n)
+
+
+def sort_workload(ctx: Context, n: int, zipf: bool = False, seed: int = SEEDS["sort"], distinct: int = 1 << 24):
+    """Keys uniform u32, or fmix32(Zipf(1.0) rank over `distinct` values); payload = row id."""
+    if zipf:
+        r = zipf_ranks(ctx, seed, COL["ZIPF"], distinct, 1.0, n)
+        k = ctx.empty(n, np.uint32)
+        call("pimdal_fmix32", ctx.handle, n, r.vp, k.vp)
+        r.free()
+    else:
+        k = u32(ctx, seed, COL["KEY"], n)
+    return k, iota(ctx, n)
+
+
+def join_workload(ctx: Context, nr: int, ns: int, zipf: bool = False, seed: int = SEEDS["join"]):
+    """Build: unique keys perm(nr); probe: FK uniform over [0, nr) or Zipf(1.0)
+    ranks mapped through the build permutation; int32 payloads."""
+    rk = perm(ctx, seed, COL["BUILD_PERM"], nr, nr)
+    rp = u32(ctx, seed, COL["BUILD_PAY"], nr)
+    if zipf:
+        ranks = zipf_ranks(ctx, seed, COL["ZIPF"], nr, 1.0, ns)
+        # rank r in [1, nr] -> build key perm(r - 1)
+        sk = ctx.empty(ns, np.uint32)
+        call("pimdal_perm_apply", ctx.handle, seed, COL["BUILD_PERM"], nr, ns, -1, ranks.vp, sk.vp)
+        ranks.free()
+    else:
+        sk = rng_range(ctx, np.uint32, seed, COL["PROBE_FK"], ns, 0, nr - 1)
+    sp = u32(ctx, seed, COL["PROBE_PAY"], ns)
+    return rk, rp, sk, sp
+
+
+def fact_table(ctx: Context, n: int, seed: int = SEEDS["groupby"], groups: int | None = None):
+    """The config-4 fact table: dict of device columns."""
+    if groups is None:
+        groups = max(1, n // 4)
+    ok = ctx.empty(n, np.int32)
+    call("pimdal_gen_orderkey", ctx.handle, seed, n, groups, ok.vp)
+    return {
+        "orderkey": ok,
+        "quantity": rng_range(ctx, np.int64, seed, COL["QTY"], n, 1, 50),
+        "price": rng_range(ctx, np.int64, seed, COL["PRICE"], n, 90000, 10500000),
+        "discount": rng_range(ctx, np.int64, seed, COL["DISC"], n, 0, 10),
+        "tax": rng_range(ctx, np.int64, seed, COL["TAX"], n, 0, 8),
+        "date": rng_range(ctx, np.int32, seed, COL["DATE"], n, 0, 2556),
+        "flag1": rng_range(ctx, np.uint8, seed, COL["FLAG1"], n, 0, 2),
+        "flag2": rng_range(ctx, np.uint8, seed, COL["FLAG2"], n, 0, 1),
+    }

@@ -1,0 +1,59 @@
+"""Selection on the GPU vs the CPU restatement: bit-exact payloads and row ids."""
+import numpy as np
+import pytest
+
+from pimdal_amd import operators as ops, datagen
+
+pytestmark = pytest.mark.gpu
+T = datagen.SELECT_THRESHOLD
+
+
+def run(ctx, k, p, lo, hi):
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    op, orow, cnt = ops.select(ctx, dk, dp, lo, hi)
+    return op.numpy(cnt), orow.numpy(cnt)
+
+
+def test_config1_select_2_20(ctx, orc):
+    k, p = orc.select_workload(1 << 20)
+    gp, gr = run(ctx, k, p, 0, T)
+    ep, er = orc.select(k, p, 0, T)
+    assert np.array_equal(gp, ep) and np.array_equal(gr, er)
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 255, 4095, 4096, 4097, 100_003])
+@pytest.mark.parametrize("lo,hi", [(0, T), (0, 0), (0, 1 << 32), (1 << 31, 3 << 30)])
+def test_select_edge_sizes(ctx, orc, n, lo, hi):
+    k, p = orc.select_workload(n, seed=500 + n)
+    gp, gr = run(ctx, k, p, lo, hi)
+    ep, er = orc.select(k, p, lo, min(hi, (1 << 32) - 1)) if hi <= (1 << 32) - 1 else (p, np.arange(n, dtype=np.uint32))
+    assert np.array_equal(gp, ep) and np.array_equal(gr, er)
+
+
+def test_select_int32_reference_predicate(ctx, orc):
+    k = orc.gen_range(np.int32, 8, 1, 300_001, -40, 50)
+    p = orc.gen_u32(8, 2, 300_001)
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    op, orow, cnt = ops.select(ctx, dk, dp, 10, 30)
+    ep, er = orc.select(k, p, 10, 30)
+    assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
+
+
+def test_select_unaligned_input(ctx, orc):
+    k, p = orc.select_workload(50_001, seed=3)
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    vk, vp = dk.view(1, 50_000), dp.view(3, 49_998)
+    vk = dk.view(1, 49_998)
+    op, orow, cnt = ops.select(ctx, vk, vp, 0, T)
+    ep, er = orc.select(k[1:49_999], p[3:50_001], 0, T)
+    assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
+
+
+def test_select_2_27_full(ctx, orc):
+    n = 1 << 27
+    dk, dp = datagen.select_workload(ctx, n)
+    op, orow, cnt = ops.select(ctx, dk, dp, 0, T)
+    k, p = orc.select_workload(n)
+    ep, er = orc.select(k, p, 0, T)
+    assert cnt == len(ep)
+    assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)

@@ -1,0 +1,87 @@
+"""Radix sort on the GPU vs the CPU restatement (stable pair sort)."""
+import numpy as np
+import pytest
+
+from pimdal_amd import operators as ops, datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_sort(ctx, k, v, begin=-1, end=-1):
+    dk = ctx.to_device(k)
+    dv = ctx.to_device(v) if v is not None else None
+    ok, ov = ops.sort_pairs(ctx, dk, dv, begin, end)
+    return ok.numpy(), (ov.numpy() if ov is not None else None)
+
+
+@pytest.mark.parametrize("n", [1, 2, 100, 8191, 8192, 8193, 1 << 20])
+def test_sort_uniform_pairs(ctx, orc, n):
+    k, v = orc.sort_workload(n, seed=900 + n)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_zipf_pairs_stable(ctx, orc):
+    k, v = orc.sort_workload(1 << 21, zipf=True)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_keys_only(ctx, orc):
+    k, _ = orc.sort_workload(300_000, seed=4)
+    gk, _ = gpu_sort(ctx, k, None)
+    assert np.array_equal(gk, orc.sort_keys(k))
+
+
+def test_sort_u64_payload(ctx, orc):
+    k = orc.gen_range(np.uint32, 6, 1, 200_000, 0, 1000)
+    v = orc.gen_range(np.int64, 6, 2, 200_000, -(1 << 40), 1 << 40)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v.view(np.uint64))
+    assert np.array_equal(gk, ek) and np.array_equal(gv.view(np.uint64), ev)
+
+
+def test_sort_constant_and_narrow_keys(ctx, orc):
+    n = 100_000
+    v = np.arange(n, dtype=np.uint32)
+    k = np.full(n, 7, np.uint32)                          # no varying bit: pure copy
+    gk, gv = gpu_sort(ctx, k, v)
+    assert np.array_equal(gk, k) and np.array_equal(gv, v)
+    k = orc.gen_range(np.uint32, 2, 1, n, 0, (1 << 24) - 1)   # 24-bit keys: 3 passes
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+@pytest.mark.parametrize("begin,end", [(0, 32), (0, 24), (8, 16), (4, 13)])
+def test_sort_explicit_bit_range(ctx, orc, begin, end):
+    k, v = orc.sort_workload(70_000, seed=11)
+    gk, gv = gpu_sort(ctx, k, v, begin, end)
+    mk = (k >> np.uint32(begin)) & np.uint32((1 << (end - begin)) - 1)
+    order = np.argsort(mk, kind="stable")
+    assert np.array_equal(gv, v[order]) and np.array_equal(gk, k[order])
+
+
+@pytest.mark.parametrize("zipf", [False, True])
+def test_sort_2_28_properties(ctx, zipf):
+    """Full size: sorted, stable, a permutation, and out_key[i] == in_key[out_row[i]]."""
+    n = 1 << 28
+    dk, dv = datagen.sort_workload(ctx, n, zipf=zipf)
+    ok, ov = ops.sort_pairs(ctx, dk, dv)
+    k = dk.numpy()
+    dk.free()
+    dv.free()
+    gk = ok.numpy()
+    ok.free()
+    gv = ov.numpy()
+    ov.free()
+    assert np.all(gk[:-1] <= gk[1:])
+    seen = np.zeros(n, np.bool_)
+    seen[gv] = True
+    assert seen.all()
+    del seen
+    assert np.array_equal(k[gv], gk)
+    ties = gk[:-1] == gk[1:]
+    assert np.all(gv[:-1][ties] < gv[1:][ties])

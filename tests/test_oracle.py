@@ -1,0 +1,213 @@
+"""The CPU restatement (oracle/) pinned against closed forms, brute force and
+the mathematics of each operator.  The reference ships no golden vectors
+or tests for this path, so each check is chosen so that a plausible slip
+(dropped tail, wrong index, unstable tie order, lost duplicate, overflow)
+fails it.  Runs on the CPU."""
+import numpy as np
+import pytest
+
+from tests.util import canon
+
+
+# ---------------- selection ----------------
+
+@pytest.mark.parametrize("n", [0, 1, 7999, 8000, 8001, 16000, 100003])
+def test_select_matches_brute_force(orc, n):
+    k, p = orc.select_workload(n, seed=1234 + n)
+    lo, hi = 100_000_000, 1_500_000_000
+    op, orow = orc.select(k, p, lo, hi)
+    idx = [i for i in range(n) if lo <= int(k[i]) < hi] if n < 20000 else np.nonzero((k >= lo) & (k < hi))[0]
+    assert list(orow) == list(idx)          # tail rows included, ascending order
+    assert np.array_equal(op, p[np.asarray(idx, dtype=np.int64)])
+
+
+def test_select_reference_predicate_int32(orc):
+    # reference/db_operators/select.cpp:33-36 predicate 10 <= x < 30 on int32 keys in [1, 50]
+    k = orc.gen_range(np.int32, 7, 1, 50_000, 1, 50)
+    op, orow = orc.select(k, k.view(np.uint32), 10, 30)
+    ref = k[(k >= 10) & (k < 30)]
+    assert np.array_equal(op.view(np.int32), ref)
+    assert np.all(np.diff(orow.astype(np.int64)) > 0)
+
+
+def test_select_config1_selectivity(orc):
+    k, p = orc.select_workload(1 << 20)
+    op, orow = orc.select(k, p, 0, 429496730)
+    assert abs(len(op) / (1 << 20) - 0.1) < 0.003
+    assert len(op) == int(np.count_nonzero(k < 429496730))
+
+
+# ---------------- sort ----------------
+
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 65537])
+def test_sort_pairs_is_the_stable_order(orc, n):
+    k = orc.gen_range(np.uint32, 99, 1, n, 0, 300)  # many ties
+    v = orc.gen_u32(99, 2, n)
+    ok, ov = orc.sort_pairs(k, v)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(ok, k[order]) and np.array_equal(ov, v[order])
+
+
+def test_sort_ties_keep_row_order(orc):
+    k, rid = orc.sort_workload(200_000, zipf=True, distinct=1000)
+    ok, ov = orc.sort_pairs(k, rid)
+    assert np.all(ok[:-1] <= ok[1:])
+    same = ok[:-1] == ok[1:]
+    assert np.all(ov[:-1][same] < ov[1:][same])
+    assert np.array_equal(np.sort(ov), rid)   # a permutation of the rows
+
+
+def test_sort_keys_only_matches_reference_sort(orc):
+    k = orc.gen_u32(5, 1, 100_000)
+    assert np.array_equal(orc.sort_keys(k), np.sort(k))
+    ok, _ = orc.sort_pairs(k)
+    assert np.array_equal(ok, np.sort(k))
+
+
+def test_sort_u64_payload(orc):
+    k = orc.gen_range(np.uint32, 3, 1, 5000, 0, 50)
+    v = orc.gen_range(np.int64, 3, 2, 5000, -(1 << 40), 1 << 40).view(np.uint64)
+    ok, ov = orc.sort_pairs(k, v)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(ov, v[order])
+
+
+# ---------------- joins ----------------
+
+def _closed_form_pkfk(rk, rp, sk, sp):
+    inv = np.empty(rk.size, np.int64)
+    inv[rk] = np.arange(rk.size)
+    return sp, rp[inv[sk]]
+
+
+@pytest.mark.parametrize("zipf", [False, True])
+def test_hash_join_pk_fk_closed_form(orc, zipf):
+    rk, rp, sk, sp = orc.join_workload(1 << 12, 1 << 15, zipf=zipf, seed=77)
+    op, ob = orc.hash_join(rk, rp, sk, sp, part_n=64)
+    ep, eb = _closed_form_pkfk(rk, rp, sk, sp)
+    assert len(op) == len(sk)
+    assert all(np.array_equal(a, b) for a, b in zip(canon(op, ob), canon(ep, eb)))
+
+
+def _brute_join(rk, rp, sk, sp):
+    d = {}
+    for k, p in zip(rk.tolist(), rp.tolist()):
+        d.setdefault(k, []).append(p)
+    o1, o2 = [], []
+    for k, p in zip(sk.tolist(), sp.tolist()):
+        for q in d.get(k, []):
+            o1.append(p)
+            o2.append(q)
+    return np.array(o1, np.uint32), np.array(o2, np.uint32)
+
+
+def test_hash_join_duplicates_and_misses(orc):
+    rk = orc.gen_range(np.uint32, 11, 1, 3000, 0, 999)   # duplicate build keys
+    rp = orc.gen_u32(11, 2, 3000)
+    sk = orc.gen_range(np.uint32, 11, 3, 5000, 0, 1999)  # half the probes miss
+    sp = orc.gen_u32(11, 4, 5000)
+    op, ob = orc.hash_join(rk, rp, sk, sp, part_n=16)
+    ep, eb = _brute_join(rk, rp, sk, sp)
+    assert all(np.array_equal(a, b) for a, b in zip(canon(op, ob), canon(ep, eb)))
+
+
+def test_hash_join_partition_hash_is_reference_hash(orc):
+    # reference/db_operators/hash_join.cpp:20-25 on hand-computed values
+    C = 11400714785074694791
+    for x in (0, 1, 2, 12345, 2**31, 2**32 - 1):
+        p = (x * C) % 2**64
+        assert orc.hash32(x) == int.from_bytes(p.to_bytes(8, "little")[::-1], "little") % 2**32
+
+
+def test_sort_merge_join_equals_hash_join_and_is_key_ordered(orc):
+    rk = orc.gen_range(np.uint32, 21, 1, 4000, 0, 2999)
+    rp = orc.gen_u32(21, 2, 4000)
+    sk = orc.gen_range(np.uint32, 21, 3, 9000, 0, 3999)
+    sp = orc.gen_u32(21, 4, 9000)
+    mp, mb = orc.sort_merge_join(rk, rp, sk, sp)
+    hp, hb = orc.hash_join(rk, rp, sk, sp, part_n=32)
+    assert all(np.array_equal(a, b) for a, b in zip(canon(mp, mb), canon(hp, hb)))
+    # SMJ emits in (key, probe row, build row) order
+    ep, eb = _brute_join(rk, rp, sk, sp)
+    order_s = np.argsort(sk, kind="stable")
+    exp_p, exp_b = [], []
+    d = {}
+    ro = np.argsort(rk, kind="stable")
+    for i in ro:
+        d.setdefault(int(rk[i]), []).append(int(rp[i]))
+    for j in order_s:
+        for q in d.get(int(sk[j]), []):
+            exp_p.append(int(sp[j]))
+            exp_b.append(q)
+    assert mp.tolist() == exp_p and mb.tolist() == exp_b
+
+
+# ---------------- aggregation ----------------
+
+def test_hash_aggr_matches_bincount(orc):
+    k = orc.gen_range(np.uint32, 31, 1, 200_000, 1, 50)       # reference/db_operators/aggregate.cpp:33-36 shape
+    v = orc.gen_range(np.int64, 31, 2, 200_000, 1, 10)
+    ok, osum, ocnt = orc.hash_aggr(k, v)
+    assert np.array_equal(ok, np.arange(1, 51, dtype=np.uint32))
+    assert np.array_equal(osum, np.bincount(k, weights=v, minlength=51)[1:].astype(np.int64))
+    assert np.array_equal(ocnt, np.bincount(k, minlength=51)[1:])
+    sk, ss, sc = orc.sort_aggr(k, v)
+    assert np.array_equal(sk, ok) and np.array_equal(ss, osum) and np.array_equal(sc, ocnt)
+
+
+def test_hash_aggr_high_cardinality_and_negative(orc):
+    k = orc.gen_u32(41, 1, 50_000) % np.uint32(20011)
+    v = orc.gen_range(np.int64, 41, 2, 50_000, -(1 << 45), 1 << 45)
+    ok, osum, ocnt = orc.hash_aggr(k, v)
+    u, inv = np.unique(k, return_inverse=True)
+    s = np.zeros(u.size, dtype=object)
+    for i, x in zip(inv.tolist(), v.tolist()):
+        s[i] += x
+    assert np.array_equal(ok, u) and [int(x) for x in osum] == list(s)
+    assert np.array_equal(ocnt, np.bincount(inv))
+
+
+def _q1_brute(t, cutoff, by_key):
+    groups = {}
+    for i in range(t["date"].size):
+        if t["date"][i] > cutoff:
+            continue
+        g = int(t["orderkey"][i]) if by_key else int(t["flag1"][i]) * 256 + int(t["flag2"][i])
+        a = groups.setdefault(g, [0, 0, 0, 0, 0, 0])
+        q, p, d, x = (int(t[c][i]) for c in ("quantity", "price", "discount", "tax"))
+        dp = p * (100 - d)
+        a[0] += q; a[1] += p; a[2] += d; a[3] += dp; a[4] += dp * (100 + x); a[5] += 1
+    return groups
+
+
+@pytest.mark.parametrize("by_key", [False, True])
+def test_q1_matches_python_integers(orc, by_key):
+    t = orc.fact_table(20_000, seed=123, groups=3000)
+    cutoff = 2505
+    out = orc.q1_key(t, cutoff) if by_key else orc.q1_flags(t, cutoff)
+    exp = _q1_brute(t, cutoff, by_key)
+    keys = sorted(exp)
+    assert list(out["key"]) == keys
+    for j, g in enumerate(keys):
+        e = exp[g]
+        got = [int(out["sum_qty"][j]), int(out["sum_price"][j]), int(out["sum_disc"][j]),
+               int(out["sum_disc_price"][j]), out["sum_charge"][j], int(out["count"][j])]
+        assert got == e
+
+
+def test_q1_int128_charge_beyond_int64(orc):
+    n = 64
+    t = {"date": np.zeros(n, np.int32), "flag1": np.zeros(n, np.uint8), "flag2": np.zeros(n, np.uint8),
+         "quantity": np.ones(n, np.int64), "price": np.full(n, 90_000_000_000_000_000, np.int64),
+         "discount": np.zeros(n, np.int64), "tax": np.full(n, 8, np.int64)}
+    out = orc.q1_flags(t, 0)
+    assert out["sum_charge"][0] == n * 90_000_000_000_000_000 * 100 * 108
+    assert out["sum_charge"][0] > 2**63
+
+
+def test_q1_config_flags_six_groups(orc):
+    t = orc.fact_table(100_000)
+    out = orc.q1_flags(t, 2505)
+    assert list(out["key"]) == [0, 1, 256, 257, 512, 513]
+    assert int(out["count"].sum()) == int(np.count_nonzero(t["date"] <= 2505))
+    assert int(out["sum_qty"].sum()) == int(t["quantity"][t["date"] <= 2505].sum())
