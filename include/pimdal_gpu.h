@@ -131,6 +131,76 @@ int pimdal_sort_pairs(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* 
 int pimdal_sort_pairs_dev(pimdal_ctx* ctx, const uint32_t* keys_in, const void* vals_in, uint32_t val_width,
                           uint64_t n, uint32_t* keys_out, void* vals_out, int begin_bit, int end_bit);
 
+/* ---------------- joins ----------------
+ * Hash join: reference/db_operators/hash_join.cpp:159 hash_join(inner,
+ * outer, column) and pimdal/join/hash_join.h:62-82 (hash_kernel,
+ * part_kernel, merge_kernel).  Sort-merge join: reference/db_operators/
+ * sort_merge_join.cpp:105 sort_merge_join(inner, outer, column) and
+ * pimdal/join/sort_merge.h:23 merge_kernel.  Equi-join on uint32 keys
+ * (int32 keys reinterpreted: equality is the same).  Build = inner,
+ * probe = outer.  Every matching pair is emitted as (probe payload, build
+ * payload); duplicate build keys are allowed.  A NULL payload means the
+ * key itself.  Hash-join order is unspecified; sort-merge order is
+ * (key, probe row, build row).
+ * Device form: out buffers hold out_capacity rows; *out_count (host)
+ * receives the number of pairs even when it exceeds out_capacity (then
+ * PIMDAL_E_CAPACITY is returned and the first out_capacity pairs are
+ * written).  Synchronous. */
+int pimdal_hash_join(pimdal_ctx* ctx, const pimdal_col* build_key, const pimdal_col* build_payload,
+                     const pimdal_col* probe_key, const pimdal_col* probe_payload,
+                     pimdal_col** out_probe_payload, pimdal_col** out_build_payload);
+int pimdal_hash_join_dev(pimdal_ctx* ctx, const uint32_t* build_key, const uint32_t* build_payload, uint64_t n_build,
+                         const uint32_t* probe_key, const uint32_t* probe_payload, uint64_t n_probe,
+                         uint32_t* out_probe_payload, uint32_t* out_build_payload, uint64_t out_capacity,
+                         uint64_t* out_count);
+int pimdal_sort_merge_join(pimdal_ctx* ctx, const pimdal_col* build_key, const pimdal_col* build_payload,
+                           const pimdal_col* probe_key, const pimdal_col* probe_payload,
+                           pimdal_col** out_probe_payload, pimdal_col** out_build_payload);
+int pimdal_sort_merge_join_dev(pimdal_ctx* ctx, const uint32_t* build_key, const uint32_t* build_payload,
+                               uint64_t n_build, const uint32_t* probe_key, const uint32_t* probe_payload,
+                               uint64_t n_probe, uint32_t* out_probe_payload, uint32_t* out_build_payload,
+                               uint64_t out_capacity, uint64_t* out_count);
+
+/* ---------------- aggregation ----------------
+ * Hash aggregation: reference/db_operators/aggregate.cpp:95 hash_aggr(table,
+ * column) and pimdal/aggregate/aggregate.h:25 group_kernel (hash variant,
+ * pimdal/aggregate/aggregate_hash.c:123-193).  Sort aggregation: the sort
+ * variant of group_kernel (pimdal/aggregate/aggregate.c:82-158).  Group by a
+ * uint32 key; per group SUM(val) (exact int64; the reference sums into
+ * int32) and COUNT.  Hash output order is unspecified; sort output is in
+ * ascending key order.  Device form: outputs hold `capacity` groups;
+ * *n_groups (host) receives the count.  Synchronous. */
+int pimdal_hash_aggr_dev(pimdal_ctx* ctx, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
+                         int64_t* out_sum, int64_t* out_count, uint64_t capacity, uint64_t* n_groups);
+int pimdal_sort_aggr_dev(pimdal_ctx* ctx, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
+                         int64_t* out_sum, int64_t* out_count, uint64_t capacity, uint64_t* n_groups);
+int pimdal_hash_aggr(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* val, pimdal_col** out_key,
+                     pimdal_col** out_sum, pimdal_col** out_count);
+int pimdal_sort_aggr(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* val, pimdal_col** out_key,
+                     pimdal_col** out_sum, pimdal_col** out_count);
+
+/* Fused filter + group-by with the TPC-H Q1 measures:
+ * pimdal/main/tpc_h/query1/dpu/kernel_q1_1.c:59-228 (filter on the date,
+ * gather, disc_price = price*(100-disc), charge = disc_price*(100+tax)),
+ * kernel_q1_2.c:37-46 (per-group sums, count) and host_q1.cpp:233-330.
+ * Keeps rows with date <= cutoff.  Groups by (flag1, flag2) when key is
+ * NULL (group key = flag1*256 + flag2; output in ascending key order), or
+ * by the int32 key column (unspecified order).  Sums are exact: int64, and
+ * sum_charge as int128 (two int64 words per group, low word first). */
+typedef struct {
+    int32_t* key;
+    int64_t* sum_qty;
+    int64_t* sum_price;
+    int64_t* sum_disc;
+    int64_t* sum_disc_price;
+    int64_t* sum_charge; /* 2 words per group */
+    int64_t* count;
+} pimdal_q1_result;
+int pimdal_filter_groupby_q1_dev(pimdal_ctx* ctx, const int32_t* date, const uint8_t* flag1, const uint8_t* flag2,
+                                 const int32_t* key, const int64_t* quantity, const int64_t* price,
+                                 const int64_t* discount, const int64_t* tax, uint64_t n, int32_t cutoff,
+                                 const pimdal_q1_result* out, uint64_t capacity, uint64_t* n_groups);
+
 /* ---------------- synthetic data (device generators) ----------------
  * Counter-based generators, identical to include/pimdal_gen.h on the host;
  * they replace reference/db_operators/shared.cpp:9-60. */

@@ -68,6 +68,15 @@ SIGNATURES: dict[str, list] = {
     "pimdal_select_dev": [c_ctx_p, C.c_int, vp, vp, u64, i64, i64, vp, vp, vp],
     "pimdal_sort_pairs": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_sort_pairs_dev": [c_ctx_p, vp, vp, u32, u64, vp, vp, C.c_int, C.c_int],
+    "pimdal_hash_join": [c_ctx_p, c_col_p, c_col_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p)],
+    "pimdal_hash_join_dev": [c_ctx_p, vp, vp, u64, vp, vp, u64, vp, vp, u64, C.POINTER(u64)],
+    "pimdal_sort_merge_join": [c_ctx_p, c_col_p, c_col_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p)],
+    "pimdal_sort_merge_join_dev": [c_ctx_p, vp, vp, u64, vp, vp, u64, vp, vp, u64, C.POINTER(u64)],
+    "pimdal_hash_aggr_dev": [c_ctx_p, vp, vp, u64, vp, vp, vp, u64, C.POINTER(u64)],
+    "pimdal_sort_aggr_dev": [c_ctx_p, vp, vp, u64, vp, vp, vp, u64, C.POINTER(u64)],
+    "pimdal_hash_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],
+    "pimdal_sort_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],
+    "pimdal_filter_groupby_q1_dev": [c_ctx_p, vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, u64, C.POINTER(u64)],
     "pimdal_gen_u32": [c_ctx_p, u64, u32, u64, vp],
     "pimdal_gen_range": [c_ctx_p, C.c_int, u64, u32, u64, i64, i64, vp],
     "pimdal_gen_perm": [c_ctx_p, u64, u32, u64, u64, u64, vp],

@@ -1,0 +1,891 @@
+// aggregate.hip — hash aggregation, fused filter + group-by with the TPC-H Q1
+// measures, and sort-based aggregation.
+//
+// Reference semantics:
+//  * reference/db_operators/aggregate.cpp:40-151 — per-thread hash maps of
+//    key -> SUM(val), merged under a critical section, gathered to a table;
+//  * pimdal/aggregate/aggregate_hash.c:43-193 — shared WRAM hash table with
+//    bucket locks and spill rounds, aggregate applied per element;
+//  * pimdal/aggregate/aggregate.c:31-158 — run-length reduce of sorted input;
+//  * pimdal/main/tpc_h/query1/dpu/kernel_q1_1.c:138-162 and kernel_q1_2.c:
+//    37-46 — Q1 filter, disc_price = price*(100-disc), charge =
+//    disc_price*(100+tax), count, per-group sums.
+// Every sum is exact: 64-bit integer adds, the charge sum in 128 bits.
+//
+// Kernels:
+//  k_flag_domain    presence bitmaps of the two flag columns (one read of
+//                   2 B/row) so group codes can be dense;
+//  k_q1_dense<G>    filter + Q1 measures accumulated in registers for G <= 8
+//                   dense groups (no atomics), one partial row per block;
+//  k_q1_final       deterministic 128-bit reduction of the block partials;
+//  k_gb_hist / k_gb_scatter   radix partition of the filtered, projected
+//                   rows by the reference hash of the group key;
+//  k_gb_lds<Agg>    one partition per block: LDS open-addressing table
+//                   (64-bit CAS on (occupied, key) words, so every key value
+//                   is legal and no lane ever waits on another), LDS atomic
+//                   adds for the measures, then the groups are emitted;
+//  k_gb_global<Agg> partitions too large or too diverse for LDS: chunks of
+//                   rows insert into a per-partition table in HBM with global
+//                   atomics; k_gb_emit_global emits them.
+#include <algorithm>
+
+#include "pd_runtime.hpp"
+#include "pd_device.hpp"
+#include "pd_ops.hpp"
+
+namespace pd {
+
+// ===================== dense (flag) group-by, Q1 measures =====================
+
+constexpr int QD_BLOCK = 256;
+constexpr int Q1_NF = 7;  // qty, price, disc, disc_price, charge_lo, charge_hi, count
+
+__global__ __launch_bounds__(256) void k_flag_domain(const uint8_t* __restrict__ f1, const uint8_t* __restrict__ f2,
+                                                     uint64_t n, uint32_t* __restrict__ bits /*[16]*/) {
+    __shared__ uint32_t s[16];
+    if (threadIdx.x < 16) s[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t m1[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = f1[i], b = f2[i];
+        m1[a >> 5] |= 1u << (a & 31);
+        m2[b >> 5] |= 1u << (b & 31);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        if (m1[j]) atomicOr(&s[j], m1[j]);
+        if (m2[j]) atomicOr(&s[8 + j], m2[j]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && s[threadIdx.x]) atomicOr(&bits[threadIdx.x], s[threadIdx.x]);
+}
+
+// Builds the code tables from the presence bitmaps: code = rank among the
+// present values.  meta[0] = n1, meta[1] = n2.
+__global__ void k_flag_lut(const uint32_t* __restrict__ bits, uint8_t* __restrict__ lut /*[512]*/,
+                           uint8_t* __restrict__ inv /*[512]*/, uint32_t* __restrict__ meta) {
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 2; c++) {
+        uint32_t r = 0;
+        for (int v = 0; v < 256; v++) {
+            const bool present = (bits[c * 8 + (v >> 5)] >> (v & 31)) & 1;
+            lut[c * 256 + v] = present ? (uint8_t)r : 0;
+            if (present) { inv[c * 256 + r] = (uint8_t)v; r++; }
+        }
+        meta[c] = r;
+    }
+}
+
+struct Q1Acc {
+    int64_t q, p, d, dp, cnt;
+    unsigned __int128 ch;
+};
+
+template <int G>
+__global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict__ date, const uint8_t* __restrict__ f1,
+                                                       const uint8_t* __restrict__ f2, const int64_t* __restrict__ qty,
+                                                       const int64_t* __restrict__ price,
+                                                       const int64_t* __restrict__ disc,
+                                                       const int64_t* __restrict__ tax, uint64_t n, int32_t cutoff,
+                                                       const uint8_t* __restrict__ lut,
+                                                       const uint32_t* __restrict__ meta,
+                                                       uint64_t* __restrict__ partials) {
+    __shared__ uint8_t s_lut[512];
+    __shared__ uint64_t s_red[QD_BLOCK / PD_WAVE][G][8];
+    for (uint32_t i = threadIdx.x; i < 512; i += QD_BLOCK) s_lut[i] = lut[i];
+    __syncthreads();
+    const uint32_t n2 = meta[1];
+    if (meta[0] * n2 > (uint32_t)G) return;  // another instantiation handles it
+    Q1Acc acc[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[g] = Q1Acc{0, 0, 0, 0, 0, 0};
+    const uint64_t stride = (uint64_t)gridDim.x * QD_BLOCK;
+    for (uint64_t i = blockIdx.x * (uint64_t)QD_BLOCK + threadIdx.x; i < n; i += stride) {
+        if (date[i] > cutoff) continue;
+        const uint32_t code = (uint32_t)s_lut[f1[i]] * n2 + s_lut[256 + f2[i]];
+        const int64_t q = qty[i], p = price[i], d = disc[i], x = tax[i];
+        const int64_t dp = p * (100 - d);
+        const __int128 ch = (__int128)dp * (100 + x);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const bool on = code == (uint32_t)g;
+            acc[g].q += on ? q : 0;
+            acc[g].p += on ? p : 0;
+            acc[g].d += on ? d : 0;
+            acc[g].dp += on ? dp : 0;
+            acc[g].ch += on ? (unsigned __int128)ch : 0;
+            acc[g].cnt += on ? 1 : 0;
+        }
+    }
+    // block reduction: wave sums (64-bit lanes; the 128-bit charge as two
+    // words with the carry folded in through a 64-bit high-part sum)
+    const uint32_t w = threadIdx.x / PD_WAVE, l = pd::lane();
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        uint64_t v[8] = {(uint64_t)acc[g].q, (uint64_t)acc[g].p, (uint64_t)acc[g].d, (uint64_t)acc[g].dp,
+                         (uint64_t)acc[g].ch, (uint64_t)(acc[g].ch >> 64), (uint64_t)acc[g].cnt, 0};
+        // split the low charge word into 32-bit halves so a 64-bit wave sum cannot lose carries
+        const uint64_t lo32 = v[4] & 0xFFFFFFFFull, hi32 = v[4] >> 32;
+        v[4] = pd::wave_sum64(lo32);
+        v[7] = pd::wave_sum64(hi32);
+#pragma unroll
+        for (int f = 0; f < 8; f++)
+            if (f != 4 && f != 7) v[f] = pd::wave_sum64(v[f]);
+        if (l == 0)
+#pragma unroll
+            for (int f = 0; f < 8; f++) s_red[w][g][f] = v[f];
+    }
+    __syncthreads();
+    if (threadIdx.x < G * 8) {
+        const uint32_t g = threadIdx.x / 8, f = threadIdx.x % 8;
+        uint64_t s = 0;
+        for (int ww = 0; ww < QD_BLOCK / PD_WAVE; ww++) s += s_red[ww][g][f];
+        partials[((uint64_t)blockIdx.x * G + g) * 8 + f] = s;
+    }
+}
+
+// partial layout per (block, group): q, p, d, dp, sum(lo32 of charge lo), sum(charge hi),
+// cnt, sum(hi32 of charge lo)
+__global__ void k_q1_final(const uint64_t* __restrict__ partials, uint32_t nblocks, uint32_t G,
+                           const uint32_t* __restrict__ meta, const uint8_t* __restrict__ inv,
+                           int32_t* __restrict__ okey, int64_t* __restrict__ sq, int64_t* __restrict__ sp,
+                           int64_t* __restrict__ sd, int64_t* __restrict__ sdp, int64_t* __restrict__ sch,
+                           int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups) {
+    __shared__ int32_t s_keys[64];
+    __shared__ unsigned __int128 s_v[64][7];
+    const uint32_t g = threadIdx.x;
+    const uint32_t n1 = meta[0], n2 = meta[1];
+    if (g < G && g < n1 * n2) {
+        unsigned __int128 q = 0, p = 0, d = 0, dp = 0, chl = 0, chh = 0, c = 0, chm = 0;
+        for (uint32_t b = 0; b < nblocks; b++) {
+            const uint64_t* r = partials + ((uint64_t)b * G + g) * 8;
+            q += (__int128)(int64_t)r[0];
+            p += (__int128)(int64_t)r[1];
+            d += (__int128)(int64_t)r[2];
+            dp += (__int128)(int64_t)r[3];
+            chl += r[4];
+            chh += (__int128)(int64_t)r[5];
+            c += r[6];
+            chm += r[7];
+        }
+        const unsigned __int128 ch = chl + (chm << 32) + (chh << 64);
+        s_keys[g] = (int32_t)inv[g / n2] * 256 + (int32_t)inv[256 + g % n2];
+        s_v[g][0] = q; s_v[g][1] = p; s_v[g][2] = d; s_v[g][3] = dp; s_v[g][4] = ch; s_v[g][5] = c;
+    }
+    __syncthreads();
+    if (g == 0) {
+        uint64_t k = 0;
+        for (uint32_t i = 0; i < G && i < n1 * n2; i++) {
+            if (s_v[i][5] == 0) continue;
+            okey[k] = s_keys[i];
+            sq[k] = (int64_t)s_v[i][0];
+            sp[k] = (int64_t)s_v[i][1];
+            sd[k] = (int64_t)s_v[i][2];
+            sdp[k] = (int64_t)s_v[i][3];
+            sch[2 * k] = (int64_t)(uint64_t)s_v[i][4];
+            sch[2 * k + 1] = (int64_t)(uint64_t)(s_v[i][4] >> 64);
+            cnt[k] = (int64_t)s_v[i][5];
+            k++;
+        }
+        *ngroups = k;
+    }
+}
+
+// ===================== general hash group-by =====================
+
+constexpr int GB_BLOCK = 512;
+constexpr uint64_t GB_OCC = 1ull << 32;
+
+// Measure policies.  NI input words per row (after projection), NF output
+// fields (u64 each).  fold() turns a projected row into field increments.
+struct AggSum {
+    static constexpr int NI = 1;   // val
+    static constexpr int NF = 2;   // sum, count
+    static constexpr int TS = 4096;
+    __device__ static void fold(const int64_t* in, uint64_t* f) {
+        f[0] = (uint64_t)in[0];
+        f[1] = 1;
+    }
+};
+
+struct AggQ1 {
+    static constexpr int NI = 4;   // qty, price, disc, tax
+    static constexpr int NF = 7;   // qty, price, disc, disc_price, charge_lo, charge_hi, count
+    static constexpr int TS = 2048;
+    __device__ static void fold(const int64_t* in, uint64_t* f) {
+        const int64_t dp = in[1] * (100 - in[2]);
+        const __int128 ch = (__int128)dp * (100 + in[3]);
+        f[0] = (uint64_t)in[0];
+        f[1] = (uint64_t)in[1];
+        f[2] = (uint64_t)in[2];
+        f[3] = (uint64_t)dp;
+        f[4] = (uint64_t)ch;
+        f[5] = (uint64_t)(ch >> 64);
+        f[6] = 1;
+    }
+};
+
+// Field add with the 128-bit pair (4, 5) handled by carry for AggQ1.
+template <typename Agg>
+__device__ __forceinline__ void add_fields(uint64_t* const* col, uint64_t slot, const uint64_t* f) {
+    if constexpr (Agg::NF == 7) {
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            if (j == 4) {
+                const uint64_t old = atomicAdd((unsigned long long*)&col[4][slot], (unsigned long long)f[4]);
+                const uint64_t carry = (old + f[4]) < old ? 1 : 0;
+                const uint64_t hi = f[5] + carry;
+                if (hi) atomicAdd((unsigned long long*)&col[5][slot], (unsigned long long)hi);
+            } else if (j != 5) {
+                if (f[j]) atomicAdd((unsigned long long*)&col[j][slot], (unsigned long long)f[j]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++)
+            if (f[j]) atomicAdd((unsigned long long*)&col[j][slot], (unsigned long long)f[j]);
+    }
+}
+
+// Projected row source: key plus NI int64 inputs, with an optional filter.
+struct GbSource {
+    const uint32_t* key;      // group key (uint32 view); NULL -> flag pair key
+    const uint8_t* f1;
+    const uint8_t* f2;
+    const int32_t* date;      // NULL -> no filter
+    int32_t cutoff;
+    const int64_t* in[4];
+    __device__ bool keep(uint64_t i) const { return !date || date[i] <= cutoff; }
+    __device__ uint32_t k(uint64_t i) const { return key ? key[i] : ((uint32_t)f1[i] << 8 | f2[i]); }
+};
+
+template <int NI>
+struct GbRow {
+    uint32_t key, pad;
+    int64_t v[NI];
+};
+
+__global__ __launch_bounds__(512) void k_gb_hist(GbSource src, uint64_t n, uint64_t chunk, uint32_t F,
+                                                 uint32_t* __restrict__ H) {
+    extern __shared__ uint32_t h[];
+    const uint32_t t = threadIdx.x, b = blockIdx.x, G = gridDim.x;
+    for (uint32_t f = t; f < F; f += 512) h[f] = 0;
+    __syncthreads();
+    const uint64_t start = (uint64_t)b * chunk, end = min(n, start + chunk);
+    for (uint64_t i = start + t; i < end; i += 512)
+        if (src.keep(i)) atomicAdd(&h[pd::ref_hash(src.k(i)) & (F - 1)], 1u);
+    __syncthreads();
+    for (uint32_t f = t; f < F; f += 512) H[(uint64_t)f * G + b] = h[f];
+}
+
+template <int NI>
+__global__ __launch_bounds__(512) void k_gb_scatter(GbSource src, uint64_t n, uint64_t chunk, uint32_t F,
+                                                    const uint32_t* __restrict__ Hs, GbRow<NI>* __restrict__ out,
+                                                    uint64_t cap) {
+    extern __shared__ uint32_t ctr[];
+    const uint32_t t = threadIdx.x, b = blockIdx.x, G = gridDim.x;
+    for (uint32_t f = t; f < F; f += 512) ctr[f] = Hs[(uint64_t)f * G + b];
+    __syncthreads();
+    const uint64_t start = (uint64_t)b * chunk, end = min(n, start + chunk);
+    for (uint64_t i = start + t; i < end; i += 512) {
+        if (!src.keep(i)) continue;
+        GbRow<NI> r;
+        r.key = src.k(i);
+        r.pad = 0;
+#pragma unroll
+        for (int j = 0; j < NI; j++) r.v[j] = src.in[j][i];
+        const uint32_t pos = atomicAdd(&ctr[pd::ref_hash(r.key) & (F - 1)], 1u);
+        if (pos < cap) out[pos] = r;
+    }
+}
+
+// gb meta: [0] n_out (u64), [1] any partition sent to the global path
+template <typename Agg>
+__global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(const GbRow<Agg::NI>* __restrict__ rows,
+                                                     const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
+                                                     uint32_t max_rows, uint32_t* __restrict__ to_global,
+                                                     uint32_t* __restrict__ out_key, uint64_t* __restrict__ out_f,
+                                                     uint64_t out_stride, uint64_t* __restrict__ meta) {
+    constexpr int TS = Agg::TS;
+    __shared__ uint64_t s_key[TS];
+    __shared__ uint64_t s_f[Agg::NF][TS];
+    __shared__ uint32_t s_used, s_over;
+    __shared__ uint64_t s_base;
+    const uint32_t p = blockIdx.x, t = threadIdx.x;
+    const uint64_t r0 = H[(uint64_t)p * G], r1 = H[(uint64_t)(p + 1) * G];
+    if (r0 == r1) return;
+    if (r1 - r0 > max_rows) {
+        if (t == 0) { to_global[p] = 1; atomicOr((uint32_t*)&meta[1], 1u); }
+        return;
+    }
+    for (uint32_t i = t; i < TS; i += GB_BLOCK) {
+        s_key[i] = 0;
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++) s_f[j][i] = 0;
+    }
+    if (t == 0) { s_used = 0; s_over = 0; }
+    __syncthreads();
+    uint64_t* cols[Agg::NF];
+#pragma unroll
+    for (int j = 0; j < Agg::NF; j++) cols[j] = s_f[j];
+    const uint32_t limit = TS * 3 / 4;
+    for (uint64_t i = r0 + t; i < r1; i += GB_BLOCK) {
+        const GbRow<Agg::NI> r = rows[i];
+        uint64_t f[Agg::NF];
+        Agg::fold(r.v, f);
+        const uint64_t want = GB_OCC | r.key;
+        uint32_t h = pd::slot_hash(r.key) & (TS - 1);
+        uint32_t probes = 0;
+        while (true) {
+            const uint64_t cur = s_key[h];
+            if (cur == want) break;
+            if (cur == 0) {
+                const uint64_t prev = atomicCAS((unsigned long long*)&s_key[h], 0ull, (unsigned long long)want);
+                if (prev == 0) {
+                    if (atomicAdd(&s_used, 1u) >= limit) s_over = 1;
+                    break;
+                }
+                if (prev == want) break;
+            }
+            h = (h + 1) & (TS - 1);
+            if (++probes >= TS) { s_over = 1; break; }
+        }
+        if (s_over) break;
+        add_fields<Agg>(cols, h, f);
+    }
+    __syncthreads();
+    if (s_over) {
+        if (t == 0) { to_global[p] = 1; atomicOr((uint32_t*)&meta[1], 1u); }
+        return;
+    }
+    if (t == 0) s_base = atomicAdd((unsigned long long*)&meta[0], (unsigned long long)s_used);
+    __syncthreads();
+    // emit the occupied slots (compaction through an LDS counter)
+    if (t == 0) s_used = 0;
+    __syncthreads();
+    for (uint32_t i = t; i < TS; i += GB_BLOCK) {
+        const uint64_t kw = s_key[i];
+        if (!kw) continue;
+        const uint64_t o = s_base + atomicAdd(&s_used, 1u);
+        out_key[o] = (uint32_t)kw;
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++) out_f[(uint64_t)j * out_stride + o] = s_f[j][i];
+    }
+}
+
+// work items over partitions routed to HBM tables
+__global__ void k_gb_chunks(const uint32_t* __restrict__ H, uint32_t G, uint32_t F, uint32_t CH,
+                            const uint32_t* __restrict__ to_global, uint32_t* __restrict__ nch) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < F; p += gridDim.x * blockDim.x) {
+        const uint32_t len = H[(uint64_t)(p + 1) * G] - H[(uint64_t)p * G];
+        nch[p] = to_global[p] ? (len + CH - 1) / CH : 0;
+    }
+}
+
+// Table for partition p occupies slots [2*start_p, 2*end_p) of the region:
+// capacity twice its row count, so it never fills.
+template <typename Agg>
+__global__ __launch_bounds__(256) void k_gb_global(const GbRow<Agg::NI>* __restrict__ rows,
+                                                   const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
+                                                   uint32_t CH, const uint32_t* __restrict__ wstart,
+                                                   uint64_t* __restrict__ gkey, uint64_t* __restrict__ gf,
+                                                   uint64_t gstride) {
+    __shared__ uint32_t s_p;
+    const uint32_t wi = blockIdx.x;
+    if (wi >= wstart[F]) return;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = F;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (wstart[mid] <= wi) lo = mid; else hi = mid;
+        }
+        s_p = lo;
+    }
+    __syncthreads();
+    const uint32_t p = s_p;
+    const uint64_t r0 = H[(uint64_t)p * G], r1 = H[(uint64_t)(p + 1) * G];
+    const uint64_t c0 = r0 + (uint64_t)(wi - wstart[p]) * CH, c1 = min(r1, c0 + CH);
+    const uint64_t base = 2 * r0, cap = 2 * (r1 - r0);
+    uint64_t* cols[Agg::NF];
+#pragma unroll
+    for (int j = 0; j < Agg::NF; j++) cols[j] = gf + (uint64_t)j * gstride + base;
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += 256) {
+        const GbRow<Agg::NI> r = rows[i];
+        uint64_t f[Agg::NF];
+        Agg::fold(r.v, f);
+        const uint64_t want = GB_OCC | r.key;
+        uint64_t h = pd::slot_hash(r.key) % cap;
+        while (true) {
+            const uint64_t cur = pd::ld_relaxed(&gkey[base + h]);
+            if (cur == want) break;
+            if (cur == 0) {
+                const uint64_t prev = atomicCAS((unsigned long long*)&gkey[base + h], 0ull, (unsigned long long)want);
+                if (prev == 0 || prev == want) break;
+            }
+            h = (h + 1 == cap) ? 0 : h + 1;
+        }
+        add_fields<Agg>(cols, h, f);
+    }
+}
+
+template <typename Agg>
+__global__ void k_gb_emit_global(const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
+                                 const uint32_t* __restrict__ to_global, const uint64_t* __restrict__ gkey,
+                                 const uint64_t* __restrict__ gf, uint64_t gstride, uint32_t* __restrict__ out_key,
+                                 uint64_t* __restrict__ out_f, uint64_t out_stride, uint64_t* __restrict__ meta) {
+    const uint64_t n2 = 2 * (uint64_t)H[(uint64_t)F * G];
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n2; s += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t kw = gkey[s];
+        if (!kw) continue;
+        const uint64_t o = atomicAdd((unsigned long long*)&meta[0], 1ull);
+        out_key[o] = (uint32_t)kw;
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++) out_f[(uint64_t)j * out_stride + o] = gf[(uint64_t)j * gstride + s];
+    }
+}
+
+}  // namespace pd
+
+namespace pd {
+
+// ---- low-cardinality path: per-block LDS tables over large row ranges,
+// merged into one small HBM table.  meta[2] = aborted (a block's table or
+// the merged table filled up). ----
+constexpr uint32_t GB_DIRECT_SLOTS = 65536;
+
+template <typename Agg>
+__global__ __launch_bounds__(GB_BLOCK) void k_gb_direct(GbSource src, uint64_t n, uint64_t chunk,
+                                                        uint64_t* __restrict__ gkey, uint64_t* __restrict__ gf,
+                                                        uint64_t* __restrict__ meta) {
+    constexpr int TS = Agg::TS;
+    __shared__ uint64_t s_key[TS];
+    __shared__ uint64_t s_f[Agg::NF][TS];
+    __shared__ uint32_t s_used, s_over;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < TS; i += GB_BLOCK) {
+        s_key[i] = 0;
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++) s_f[j][i] = 0;
+    }
+    if (t == 0) { s_used = 0; s_over = 0; }
+    __syncthreads();
+    uint64_t* cols[Agg::NF];
+#pragma unroll
+    for (int j = 0; j < Agg::NF; j++) cols[j] = s_f[j];
+    const uint32_t limit = TS / 2;
+    const uint64_t start = (uint64_t)blockIdx.x * chunk, end = min(n, start + chunk);
+    uint32_t iter = 0;
+    for (uint64_t i0 = start; i0 < end; i0 += GB_BLOCK) {
+        // every 64 rounds, stop early if another block gave up
+        if ((iter++ & 63) == 0) {
+            if (t == 0 && pd::ld_relaxed(&meta[2])) s_over = 1;
+            __syncthreads();
+            if (s_over) return;
+        }
+        const uint64_t i = i0 + t;
+        if (i < end && src.keep(i)) {
+            int64_t in[Agg::NI];
+#pragma unroll
+            for (int j = 0; j < Agg::NI; j++) in[j] = src.in[j][i];
+            uint64_t f[Agg::NF];
+            Agg::fold(in, f);
+            const uint32_t key = src.k(i);
+            const uint64_t want = GB_OCC | key;
+            uint32_t h = pd::slot_hash(key) & (TS - 1);
+            bool ok = true;
+            while (true) {
+                const uint64_t cur = s_key[h];
+                if (cur == want) break;
+                if (cur == 0) {
+                    const uint64_t prev = atomicCAS((unsigned long long*)&s_key[h], 0ull, (unsigned long long)want);
+                    if (prev == 0) {
+                        if (atomicAdd(&s_used, 1u) >= limit) s_over = 1;
+                        break;
+                    }
+                    if (prev == want) break;
+                }
+                h = (h + 1) & (TS - 1);
+                if (s_over) { ok = false; break; }
+            }
+            if (ok) add_fields<Agg>(cols, h, f);
+        }
+        __syncthreads();
+        if (s_over) {
+            if (t == 0) pd::st_relaxed(&meta[2], (uint64_t)1);
+            return;
+        }
+    }
+    __syncthreads();
+    // merge into the HBM table
+    uint64_t* gcols[Agg::NF];
+#pragma unroll
+    for (int j = 0; j < Agg::NF; j++) gcols[j] = gf + (uint64_t)j * GB_DIRECT_SLOTS;
+    for (uint32_t i = t; i < TS; i += GB_BLOCK) {
+        const uint64_t kw = s_key[i];
+        if (!kw) continue;
+        uint32_t h = pd::slot_hash((uint32_t)kw) & (GB_DIRECT_SLOTS - 1);
+        uint32_t probes = 0;
+        bool ok = true;
+        while (true) {
+            const uint64_t cur = pd::ld_relaxed(&gkey[h]);
+            if (cur == kw) break;
+            if (cur == 0) {
+                const uint64_t prev = atomicCAS((unsigned long long*)&gkey[h], 0ull, (unsigned long long)kw);
+                if (prev == 0) {
+                    if (atomicAdd((unsigned long long*)&meta[3], 1ull) >= GB_DIRECT_SLOTS / 2)
+                        pd::st_relaxed(&meta[2], (uint64_t)1);
+                    break;
+                }
+                if (prev == kw) break;
+            }
+            h = (h + 1) & (GB_DIRECT_SLOTS - 1);
+            if (++probes >= GB_DIRECT_SLOTS) { pd::st_relaxed(&meta[2], (uint64_t)1); ok = false; break; }
+        }
+        if (!ok) continue;
+        uint64_t f[Agg::NF];
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++) f[j] = s_f[j][i];
+        add_fields<Agg>(gcols, h, f);
+    }
+}
+
+template <typename Agg>
+__global__ void k_gb_direct_emit(const uint64_t* __restrict__ gkey, const uint64_t* __restrict__ gf,
+                                 uint32_t* __restrict__ out_key, uint64_t* __restrict__ out_f, uint64_t out_stride,
+                                 uint64_t* __restrict__ meta) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < GB_DIRECT_SLOTS; s += gridDim.x * blockDim.x) {
+        const uint64_t kw = gkey[s];
+        if (!kw) continue;
+        const uint64_t o = atomicAdd((unsigned long long*)&meta[0], 1ull);
+        out_key[o] = (uint32_t)kw;
+#pragma unroll
+        for (int j = 0; j < Agg::NF; j++) out_f[(uint64_t)j * out_stride + o] = gf[(uint64_t)j * GB_DIRECT_SLOTS + s];
+    }
+}
+
+static uint32_t gb_next_pow2(uint64_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Group-by driver.  Output: keys in out_key, field j of group g at
+// out_f[j * cap + g]; returns the number of groups (unspecified order).
+template <typename Agg>
+uint64_t group_by(pimdal_ctx* c, const GbSource& src, uint64_t n, uint32_t* out_key, uint64_t* out_f, uint64_t cap,
+                  hipStream_t s) {
+    PD_REQUIRE(n < (1ull << 31), PIMDAL_E_INVALID, "group-by supports up to 2^31-1 rows per call");
+    Scratch meta(c, 8 * 4, s);
+    fill_u64(c, meta.get(), 0, 4, s);
+    if (n == 0) return 0;
+    const int64_t hint = c->tune_get("gb_groups_hint", 0);
+    const bool try_direct = hint == 0 || hint <= (int64_t)(Agg::TS / 4);
+    if (try_direct) {
+        Scratch gk(c, 8ull * GB_DIRECT_SLOTS, s), gfv(c, 8ull * GB_DIRECT_SLOTS * Agg::NF, s);
+        fill_u64(c, gk.get(), 0, GB_DIRECT_SLOTS, s);
+        fill_u64(c, gfv.get(), 0, (uint64_t)GB_DIRECT_SLOTS * Agg::NF, s);
+        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 4096), (uint64_t)c->cu_count * 2));
+        const uint64_t chunk = ceil_div(n, G);
+        launch(c, "gb_direct", k_gb_direct<Agg>, dim3(G), dim3(GB_BLOCK), 0, s, src, n, chunk, gk.as<uint64_t>(),
+               gfv.as<uint64_t>(), meta.as<uint64_t>());
+        uint64_t m[4];
+        PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        if (!m[2]) {
+            PD_REQUIRE(m[3] <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+            launch(c, "gb_direct_emit", k_gb_direct_emit<Agg>, dim3(64), dim3(256), 0, s,
+                   (const uint64_t*)gk.as<uint64_t>(), (const uint64_t*)gfv.as<uint64_t>(), out_key, out_f, cap,
+                   meta.as<uint64_t>());
+            return m[3];
+        }
+        fill_u64(c, meta.get(), 0, 4, s);
+    }
+    // partitioned path
+    const int64_t rows_target = c->tune_get("gb_rows_per_part", (int64_t)Agg::TS * 2);
+    const uint32_t F = std::min<uint32_t>(16384, gb_next_pow2(std::max<uint64_t>(1, ceil_div(n, (uint64_t)rows_target))));
+    const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 16384), (uint64_t)c->cu_count * 2));
+    const uint64_t chunk = ceil_div(n, G);
+    Scratch H(c, ((uint64_t)F * G + 1) * 4, s);
+    launch(c, "gb_hist", k_gb_hist, dim3(G), dim3(512), F * 4, s, src, n, chunk, F, H.as<uint32_t>());
+    Scratch tot(c, 8, s);
+    exclusive_scan_u32(c, H.as<uint32_t>(), H.as<uint32_t>(), (uint64_t)F * G, tot.as<uint64_t>(), s);
+    const uint64_t nsel = read_u64(c, tot.get(), s);
+    Scratch rows(c, std::max<uint64_t>(nsel, 1) * sizeof(GbRow<Agg::NI>), s);
+    launch(c, "gb_scatter", k_gb_scatter<Agg::NI>, dim3(G), dim3(512), F * 4, s, src, n, chunk, F,
+           (const uint32_t*)H.as<uint32_t>(), rows.as<GbRow<Agg::NI>>(), nsel);
+    Scratch tog(c, (uint64_t)F * 4, s);
+    fill_u32(c, tog.get(), 0, F, s);
+    const uint32_t max_rows = (uint32_t)c->tune_get("gb_lds_max_rows", 65536);
+    // the emit writes at most one group per row, so a selected-row-sized
+    // staging area bounds the output before the capacity check
+    const bool direct_out = cap >= nsel;
+    Scratch sk(c, direct_out ? 0 : nsel * 4, s), sf(c, direct_out ? 0 : nsel * 8 * Agg::NF, s);
+    uint32_t* ok = direct_out ? out_key : sk.as<uint32_t>();
+    uint64_t* of = direct_out ? out_f : sf.as<uint64_t>();
+    const uint64_t ostride = direct_out ? cap : nsel;
+    launch(c, "gb_lds", k_gb_lds<Agg>, dim3(F), dim3(GB_BLOCK), 0, s, (const GbRow<Agg::NI>*)rows.as<GbRow<Agg::NI>>(),
+           (const uint32_t*)H.as<uint32_t>(), G, F, max_rows, tog.as<uint32_t>(), ok, of, ostride, meta.as<uint64_t>());
+    uint64_t m[4];
+    PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    if (m[1]) {
+        const uint32_t CH = 16384;
+        Scratch wst(c, ((uint64_t)F + 1) * 4, s);
+        launch(c, "gb_chunks", k_gb_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s,
+               (const uint32_t*)H.as<uint32_t>(), G, F, CH, (const uint32_t*)tog.as<uint32_t>(), wst.as<uint32_t>());
+        exclusive_scan_u32(c, wst.as<uint32_t>(), wst.as<uint32_t>(), F, nullptr, s);
+        const uint64_t slots = 2 * nsel;
+        Scratch gk(c, slots * 8, s), gfv(c, slots * 8 * Agg::NF, s);
+        fill_u64(c, gk.get(), 0, slots, s);
+        fill_u64(c, gfv.get(), 0, slots * Agg::NF, s);
+        const uint64_t max_wi = F + ceil_div(nsel, CH);
+        launch(c, "gb_global", k_gb_global<Agg>, dim3((uint32_t)max_wi), dim3(256), 0, s,
+               (const GbRow<Agg::NI>*)rows.as<GbRow<Agg::NI>>(), (const uint32_t*)H.as<uint32_t>(), G, F, CH,
+               (const uint32_t*)wst.as<uint32_t>(), gk.as<uint64_t>(), gfv.as<uint64_t>(), slots);
+        launch(c, "gb_emit_global", k_gb_emit_global<Agg>, dim3((uint32_t)std::min<uint64_t>(ceil_div(slots, 256), 4096)),
+               dim3(256), 0, s, (const uint32_t*)H.as<uint32_t>(), G, F, (const uint32_t*)tog.as<uint32_t>(),
+               (const uint64_t*)gk.as<uint64_t>(), (const uint64_t*)gfv.as<uint64_t>(), slots, ok, of, ostride,
+               meta.as<uint64_t>());
+        PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+    }
+    const uint64_t groups = m[0];
+    PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    if (!direct_out) {
+        copy_d2d(c, out_key, ok, groups * 4, s);
+        for (int j = 0; j < Agg::NF; j++) copy_d2d(c, out_f + (uint64_t)j * cap, of + (uint64_t)j * nsel, groups * 8, s);
+        PD_HIP(hipStreamSynchronize(s));
+    }
+    return groups;
+}
+
+// ---- reshape engine output (u64 fields) into the Q1 result arrays ----
+__global__ void k_q1_unpack(uint64_t g, const uint32_t* __restrict__ key, const uint64_t* __restrict__ f,
+                            uint64_t stride, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
+                            int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
+                            int64_t* __restrict__ sch, int64_t* __restrict__ cnt) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < g; i += (uint64_t)gridDim.x * blockDim.x) {
+        okey[i] = (int32_t)key[i];
+        sq[i] = (int64_t)f[i];
+        sp[i] = (int64_t)f[stride + i];
+        sd[i] = (int64_t)f[2 * stride + i];
+        sdp[i] = (int64_t)f[3 * stride + i];
+        sch[2 * i] = (int64_t)f[4 * stride + i];
+        sch[2 * i + 1] = (int64_t)f[5 * stride + i];
+        cnt[i] = (int64_t)f[6 * stride + i];
+    }
+}
+
+uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1, const uint8_t* f2,
+                           const int32_t* key, const int64_t* qty, const int64_t* price, const int64_t* disc,
+                           const int64_t* tax, uint64_t n, int32_t cutoff, const pimdal_q1_result& out,
+                           uint64_t cap, hipStream_t s) {
+    PD_REQUIRE(date && qty && price && disc && tax && (key || (f1 && f2)), PIMDAL_E_INVALID, "missing input column");
+    PD_REQUIRE(out.key && out.sum_qty && out.sum_price && out.sum_disc && out.sum_disc_price && out.sum_charge &&
+                   out.count, PIMDAL_E_INVALID, "missing output column");
+    if (n == 0) return 0;
+    if (!key) {
+        // dense path: flag domains -> codes -> register accumulators
+        Scratch small(c, 64 + 8 + 512 + 512 + 16, s);
+        uint32_t* bits = small.as<uint32_t>();
+        uint32_t* meta = bits + 16;
+        uint8_t* lut = reinterpret_cast<uint8_t*>(meta + 2);
+        uint8_t* inv = lut + 512;
+        uint64_t* ng = reinterpret_cast<uint64_t*>(inv + 512);
+        fill_u32(c, bits, 0, 16 + 2, s);
+        const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 8), (uint64_t)c->cu_count * 4));
+        launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
+        launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut, inv, meta);
+        uint32_t nm[2];
+        PD_HIP(hipMemcpyAsync(nm, meta, 8, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        const uint32_t groups = nm[0] * nm[1];
+        if (groups <= 8) {
+            const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, QD_BLOCK * 16), (uint64_t)c->cu_count * 8));
+            Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s);
+            launch(c, "q1_dense", k_q1_dense<8>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc, tax, n,
+                   cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
+            launch(c, "q1_final", k_q1_final, dim3(1), dim3(64), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, 8u,
+                   (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price, out.sum_disc,
+                   out.sum_disc_price, out.sum_charge, out.count, ng);
+            const uint64_t k = read_u64(c, ng, s);
+            PD_REQUIRE(k <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+            return k;
+        }
+    }
+    GbSource src{};
+    src.key = reinterpret_cast<const uint32_t*>(key);
+    src.f1 = f1;
+    src.f2 = f2;
+    src.date = date;
+    src.cutoff = cutoff;
+    src.in[0] = qty; src.in[1] = price; src.in[2] = disc; src.in[3] = tax;
+    const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
+    Scratch k32(c, stage * 4, s), fs(c, stage * 8 * AggQ1::NF, s);
+    const uint64_t g = group_by<AggQ1>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
+    PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    launch(c, "q1_unpack", k_q1_unpack, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(g, 256), 4096))),
+           dim3(256), 0, s, g, (const uint32_t*)k32.as<uint32_t>(), (const uint64_t*)fs.as<uint64_t>(), stage,
+           out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count);
+    PD_HIP(hipStreamSynchronize(s));
+    return g;
+}
+
+uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
+                   int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
+    PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
+    if (n == 0) return 0;
+    GbSource src{};
+    src.key = key;
+    src.in[0] = val;
+    const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
+    Scratch k32(c, stage * 4, s), fs(c, stage * 16, s);
+    const uint64_t g = group_by<AggSum>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
+    PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    copy_d2d(c, out_key, k32.get(), g * 4, s);
+    copy_d2d(c, out_sum, fs.get(), g * 8, s);
+    copy_d2d(c, out_cnt, fs.as<uint64_t>() + stage, g * 8, s);
+    PD_HIP(hipStreamSynchronize(s));
+    return g;
+}
+
+// ===================== sort-based aggregation =====================
+
+__global__ void k_run_heads(const uint32_t* __restrict__ k, uint64_t n, uint32_t* __restrict__ head) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
+}
+
+// gid = exclusive scan of heads: row i belongs to group gid[i] + head[i] - 1.
+// Consecutive lanes of a wave hold consecutive rows, so each wave first
+// reduces its runs with a segmented scan and only the last lane of each run
+// adds to HBM.
+__global__ __launch_bounds__(256) void k_run_reduce(const uint32_t* __restrict__ k, const int64_t* __restrict__ v,
+                                                    uint64_t n, const uint32_t* __restrict__ head,
+                                                    const uint32_t* __restrict__ gid, uint32_t* __restrict__ okey,
+                                                    int64_t* __restrict__ osum, int64_t* __restrict__ ocnt) {
+    const uint32_t l = pd::lane();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x; i0 < n; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        const bool in = i < n;
+        const uint32_t g = in ? gid[i] + head[i] - 1 : 0xFFFFFFFFu;
+        int64_t s = in ? v[i] : 0;
+        uint64_t c = in ? 1 : 0;
+        if (in && head[i]) okey[g] = k[i];
+        // segmented inclusive scan over equal g
+#pragma unroll
+        for (int d = 1; d < PD_WAVE; d <<= 1) {
+            const int64_t ts = __shfl_up(s, d, PD_WAVE);
+            const uint64_t tc = __shfl_up(c, d, PD_WAVE);
+            const uint32_t tg = __shfl_up(g, d, PD_WAVE);
+            if (l >= (uint32_t)d && tg == g) { s += ts; c += tc; }
+        }
+        const uint32_t ng = __shfl_down(g, 1, PD_WAVE);
+        const bool last = in && (l == PD_WAVE - 1 || ng != g || i + 1 >= n);
+        if (last) {
+            atomicAdd((unsigned long long*)&osum[g], (unsigned long long)s);
+            atomicAdd((unsigned long long*)&ocnt[g], (unsigned long long)c);
+        }
+    }
+}
+
+uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
+                   int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
+    PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
+    if (n == 0) return 0;
+    Scratch sk(c, n * 4, s), sv(c, n * 8, s), head(c, n * 4, s), gid(c, (n + 1) * 4, s), tot(c, 8, s);
+    sort_pairs(c, key, val, 8, n, sk.as<uint32_t>(), sv.get(), -1, -1, s);
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), (uint64_t)c->cu_count * 16));
+    launch(c, "agg_heads", k_run_heads, dim3(g), dim3(256), 0, s, (const uint32_t*)sk.as<uint32_t>(), n, head.as<uint32_t>());
+    exclusive_scan_u32(c, head.as<uint32_t>(), gid.as<uint32_t>(), n, tot.as<uint64_t>(), s);
+    const uint64_t groups = read_u64(c, tot.get(), s);
+    PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    fill_u64(c, out_sum, 0, groups, s);
+    fill_u64(c, out_cnt, 0, groups, s);
+    launch(c, "agg_run_reduce", k_run_reduce, dim3(g), dim3(256), 0, s, (const uint32_t*)sk.as<uint32_t>(),
+           (const int64_t*)sv.as<int64_t>(), n, (const uint32_t*)head.as<uint32_t>(), (const uint32_t*)gid.as<uint32_t>(),
+           out_key, out_sum, out_cnt);
+    PD_HIP(hipStreamSynchronize(s));
+    return groups;
+}
+
+}  // namespace pd
+
+using namespace pd;
+
+extern "C" int pimdal_hash_aggr_dev(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n,
+                                    uint32_t* out_key, int64_t* out_sum, int64_t* out_count, uint64_t capacity,
+                                    uint64_t* n_groups) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    *n_groups = hash_aggr(c, key, val, n, out_key, out_sum, out_count, capacity, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_sort_aggr_dev(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n,
+                                    uint32_t* out_key, int64_t* out_sum, int64_t* out_count, uint64_t capacity,
+                                    uint64_t* n_groups) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    *n_groups = sort_aggr(c, key, val, n, out_key, out_sum, out_count, capacity, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_filter_groupby_q1_dev(pimdal_ctx* c, const int32_t* date, const uint8_t* flag1,
+                                            const uint8_t* flag2, const int32_t* key, const int64_t* quantity,
+                                            const int64_t* price, const int64_t* discount, const int64_t* tax,
+                                            uint64_t n, int32_t cutoff, const pimdal_q1_result* out,
+                                            uint64_t capacity, uint64_t* n_groups) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && out && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    *n_groups = filter_groupby_q1(c, date, flag1, flag2, key, quantity, price, discount, tax, n, cutoff, *out,
+                                  capacity, c->stream);
+    PD_API_END
+}
+
+// ---------------- column-level aggregation ----------------
+
+namespace {
+
+using AggFn = uint64_t (*)(pimdal_ctx*, const uint32_t*, const int64_t*, uint64_t, uint32_t*, int64_t*, int64_t*,
+                           uint64_t, hipStream_t);
+
+int aggr_cols(AggFn fn, pimdal_ctx* c, const pimdal_col* key, const pimdal_col* val, pimdal_col** ok,
+              pimdal_col** os, pimdal_col** oc) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && key && val && ok && os && oc, PIMDAL_E_INVALID, "NULL argument");
+    PD_REQUIRE(type_width(key->type) == 4, PIMDAL_E_UNSUPPORTED, "group key must be 32-bit");
+    PD_REQUIRE(val->type == PIMDAL_I64 || val->type == PIMDAL_U64, PIMDAL_E_UNSUPPORTED, "values must be 64-bit");
+    PD_REQUIRE(key->length == val->length, PIMDAL_E_INVALID, "column lengths differ");
+    const uint64_t n = key->length, cap = std::max<uint64_t>(n, 1);
+    pimdal_col *a = nullptr, *b = nullptr, *d = nullptr;
+    try {
+        a = new pimdal_col{key->type, 0, cap, c->pool.alloc(cap * 4, c->stream), true};
+        b = new pimdal_col{PIMDAL_I64, 0, cap, c->pool.alloc(cap * 8, c->stream), true};
+        d = new pimdal_col{PIMDAL_I64, 0, cap, c->pool.alloc(cap * 8, c->stream), true};
+        const uint64_t g = fn(c, (const uint32_t*)key->data, (const int64_t*)val->data, n, (uint32_t*)a->data,
+                              (int64_t*)b->data, (int64_t*)d->data, cap, c->stream);
+        a->length = b->length = d->length = g;
+    } catch (...) {
+        for (pimdal_col* x : {a, b, d})
+            if (x) { c->pool.free(x->data, c->stream); delete x; }
+        throw;
+    }
+    *ok = a;
+    *os = b;
+    *oc = d;
+    PD_API_END
+}
+
+}  // namespace
+
+extern "C" int pimdal_hash_aggr(pimdal_ctx* c, const pimdal_col* key, const pimdal_col* val, pimdal_col** ok,
+                                pimdal_col** os, pimdal_col** oc) {
+    return aggr_cols(pd::hash_aggr, c, key, val, ok, os, oc);
+}
+
+extern "C" int pimdal_sort_aggr(pimdal_ctx* c, const pimdal_col* key, const pimdal_col* val, pimdal_col** ok,
+                                pimdal_col** os, pimdal_col** oc) {
+    return aggr_cols(pd::sort_aggr, c, key, val, ok, os, oc);
+}

@@ -11,4 +11,28 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
 void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t* kout,
                 void* vout, int begin_bit, int end_bit, hipStream_t s);
 
+void exclusive_scan_u32(pimdal_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* total_dev,
+                        hipStream_t s);
+
+void radix_partition(pimdal_ctx* c, const uint32_t* keys, const uint32_t* pay, uint64_t n, uint32_t F, uint2* out,
+                     uint32_t* H, uint32_t G, hipStream_t s);
+
+uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
+                   const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap, hipStream_t s);
+
+uint64_t sort_merge_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
+                         const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap,
+                         hipStream_t s);
+
+uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
+                   int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s);
+
+uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
+                   int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s);
+
+uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1, const uint8_t* f2,
+                           const int32_t* key, const int64_t* qty, const int64_t* price, const int64_t* disc,
+                           const int64_t* tax, uint64_t n, int32_t cutoff, const pimdal_q1_result& out,
+                           uint64_t cap, hipStream_t s);
+
 }  // namespace pd

@@ -54,3 +54,92 @@ def sort_pairs(ctx: Context, keys: DeviceArray, vals: DeviceArray | None, begin_
     call("pimdal_sort_pairs_dev", ctx.handle, keys.vp, vals.vp if vals is not None else None, vw, n, ok.vp,
          ov.vp if ov is not None else None, begin_bit, end_bit)
     return ok, ov
+
+
+def _join(name: str, ctx: Context, bk, bp, pk, pp, capacity: int | None = None):
+    cap = pk.n if capacity is None else int(capacity)
+    op = ctx.empty(max(cap, 1), np.uint32)
+    ob = ctx.empty(max(cap, 1), np.uint32)
+    cnt = C.c_uint64()
+    rc = getattr(_lib.load(), name)(ctx.handle, bk.vp, bp.vp if bp is not None else None, bk.n, pk.vp,
+                                    pp.vp if pp is not None else None, pk.n, op.vp, ob.vp, cap, C.byref(cnt))
+    if rc == -5 and capacity is None:  # more pairs than probe rows: size exactly and rerun
+        op.free(); ob.free()
+        return _join(name, ctx, bk, bp, pk, pp, cnt.value)
+    _lib.check(rc)
+    return op, ob, cnt.value
+
+
+def hash_join(ctx: Context, build_key, build_payload, probe_key, probe_payload, capacity: int | None = None):
+    """(probe payload, build payload) of every matching pair, unordered.
+
+    reference/db_operators/hash_join.cpp:159-219; pimdal/join/hash_join.c:40-430.
+    """
+    return _join("pimdal_hash_join_dev", ctx, build_key, build_payload, probe_key, probe_payload, capacity)
+
+
+def sort_merge_join(ctx: Context, build_key, build_payload, probe_key, probe_payload, capacity: int | None = None):
+    """Every matching pair in (key, probe row, build row) order.
+
+    reference/db_operators/sort_merge_join.cpp:48-156; pimdal/join/sort_merge.c:54-152.
+    """
+    return _join("pimdal_sort_merge_join_dev", ctx, build_key, build_payload, probe_key, probe_payload, capacity)
+
+
+def _aggr(name: str, ctx: Context, key, val, capacity: int | None = None):
+    cap = max(key.n if capacity is None else int(capacity), 1)
+    ok = ctx.empty(cap, np.uint32)
+    os_ = ctx.empty(cap, np.int64)
+    oc = ctx.empty(cap, np.int64)
+    g = C.c_uint64()
+    call(name, ctx.handle, key.vp, val.vp, key.n, ok.vp, os_.vp, oc.vp, cap, C.byref(g))
+    return ok, os_, oc, g.value
+
+
+def hash_aggr(ctx: Context, key, val, capacity: int | None = None):
+    """(key, SUM(val), COUNT) per group, unordered.
+
+    reference/db_operators/aggregate.cpp:40-151; pimdal/aggregate/aggregate_hash.c:43-193.
+    """
+    return _aggr("pimdal_hash_aggr_dev", ctx, key, val, capacity)
+
+
+def sort_aggr(ctx: Context, key, val, capacity: int | None = None):
+    """(key, SUM(val), COUNT) per group in ascending key order.
+
+    pimdal/aggregate/aggregate.c:31-158.
+    """
+    return _aggr("pimdal_sort_aggr_dev", ctx, key, val, capacity)
+
+
+class _Q1Result(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in ("key", "sum_qty", "sum_price", "sum_disc", "sum_disc_price",
+                                          "sum_charge", "count")]
+
+
+Q1_FIELDS = ("key", "sum_qty", "sum_price", "sum_disc", "sum_disc_price", "sum_charge", "count")
+
+
+def filter_groupby_q1(ctx: Context, t: dict, cutoff: int, by_key: bool = False, capacity: int | None = None,
+                      to_host: bool = True):
+    """Fused date <= cutoff filter and group-by with the TPC-H Q1 measures.
+
+    Groups by (flag1, flag2) or, with by_key, by t["orderkey"].
+    pimdal/main/tpc_h/query1/dpu/kernel_q1_1.c:59-228, kernel_q1_2.c:37-46.
+    """
+    n = t["date"].n
+    cap = max(1, (64 if not by_key else n) if capacity is None else int(capacity))
+    out = {f: ctx.empty(cap * (2 if f == "sum_charge" else 1), np.int32 if f == "key" else np.int64)
+           for f in Q1_FIELDS}
+    res = _Q1Result(*[out[f].ptr for f in Q1_FIELDS])
+    g = C.c_uint64()
+    call("pimdal_filter_groupby_q1_dev", ctx.handle, t["date"].vp, t["flag1"].vp, t["flag2"].vp,
+         t["orderkey"].vp if by_key else None, t["quantity"].vp, t["price"].vp, t["discount"].vp, t["tax"].vp,
+         n, int(cutoff), C.byref(res), cap, C.byref(g))
+    k = g.value
+    if not to_host:
+        return out, k
+    h = {f: out[f].numpy(2 * k if f == "sum_charge" else k) for f in Q1_FIELDS}
+    ch = h["sum_charge"].reshape(k, 2)
+    h["sum_charge"] = [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in ch]
+    return h

@@ -1,0 +1,105 @@
+"""Hash / sort aggregation and the fused Q1 group-by on the GPU vs the CPU restatement."""
+import numpy as np
+import pytest
+
+from pimdal_amd import operators as ops, datagen
+from tests.util import canon
+
+pytestmark = pytest.mark.gpu
+
+
+def agg(ctx, fn, k, v):
+    ok, os_, oc, g = fn(ctx, ctx.to_device(k), ctx.to_device(v))
+    return canon(ok.numpy(g), os_.numpy(g), oc.numpy(g))
+
+
+@pytest.mark.parametrize("which", ["hash", "sort"])
+@pytest.mark.parametrize("n,groups", [(1, 1), (1000, 50), (500_000, 50), (300_001, 20_000), (200_000, 200_000),
+                                      (400_000, 1 << 32)])
+def test_aggr_matches_oracle(ctx, orc, which, n, groups):
+    k = orc.gen_range(np.uint32, n, 1, n, 0, groups - 1)
+    v = orc.gen_range(np.int64, n, 2, n, -(1 << 40), 1 << 40)
+    fn = ops.hash_aggr if which == "hash" else ops.sort_aggr
+    got = agg(ctx, fn, k, v)
+    exp = orc.hash_aggr(k, v)
+    assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+
+
+def test_aggr_reference_shape(ctx, orc):
+    # reference/db_operators/aggregate.cpp:33-36: keys on [1, 50], values on [1, 10]
+    k = orc.gen_range(np.uint32, 1, 1, 2_000_000, 1, 50)
+    v = orc.gen_range(np.int64, 1, 2, 2_000_000, 1, 10)
+    for fn in (ops.hash_aggr, ops.sort_aggr):
+        got = agg(ctx, fn, k, v)
+        exp = orc.hash_aggr(k, v)
+        assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+
+
+def test_sort_aggr_zipf_skew(ctx, orc):
+    k, _ = orc.sort_workload(1 << 20, zipf=True, distinct=1000)
+    v = orc.gen_range(np.int64, 5, 2, 1 << 20, 0, 1000)
+    for fn in (ops.sort_aggr, ops.hash_aggr):
+        got = agg(ctx, fn, k, v)
+        exp = orc.hash_aggr(k, v)
+        assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+
+
+def _cmp_q1(got, exp, ordered):
+    keys = list(got["key"])
+    if not ordered:
+        order = np.argsort(got["key"], kind="stable")
+        got = {f: (np.asarray(v)[order] if f != "sum_charge" else [v[i] for i in order]) for f, v in got.items()}
+    assert list(got["key"]) == list(exp["key"])
+    for f in ("sum_qty", "sum_price", "sum_disc", "sum_disc_price", "count"):
+        assert np.array_equal(np.asarray(got[f]), np.asarray(exp[f])), f
+    assert list(got["sum_charge"]) == list(exp["sum_charge"])
+
+
+@pytest.mark.parametrize("n", [1, 1000, 1 << 20])
+def test_q1_flags_dense(ctx, orc, n):
+    t = orc.fact_table(n)
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505)
+    exp = orc.q1_flags(t, 2505)
+    _cmp_q1(got, exp, ordered=True)
+
+
+def test_q1_flags_many_groups_general_path(ctx, orc):
+    n = 200_000
+    t = orc.fact_table(n)
+    t["flag1"] = orc.gen_range(np.uint8, 3, 17, n, 60, 90)   # 31 x 2 groups: beyond the dense path
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2000, capacity=4096)
+    exp = orc.q1_flags(t, 2000)
+    _cmp_q1(got, exp, ordered=False)
+
+
+def test_q1_int128_charge(ctx, orc):
+    n = 4096
+    t = {"date": np.zeros(n, np.int32), "flag1": np.zeros(n, np.uint8), "flag2": np.ones(n, np.uint8),
+         "quantity": np.ones(n, np.int64), "price": np.full(n, 90_000_000_000_000_000, np.int64),
+         "discount": np.zeros(n, np.int64), "tax": np.full(n, 8, np.int64),
+         "orderkey": np.full(n, 5, np.int32)}
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 0)
+    assert got["sum_charge"][0] == n * 90_000_000_000_000_000 * 100 * 108
+    got = ops.filter_groupby_q1(ctx, d, 0, by_key=True)
+    assert got["sum_charge"][0] == n * 90_000_000_000_000_000 * 100 * 108
+
+
+@pytest.mark.parametrize("n,groups", [(100_000, 25_000), (1 << 21, 1 << 19)])
+def test_q1_by_orderkey(ctx, orc, n, groups):
+    t = orc.fact_table(n, groups=groups)
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
+    exp = orc.q1_key(t, 2505)
+    _cmp_q1(got, exp, ordered=False)
+
+
+def test_q1_config4_full_size(ctx, orc):
+    """Config 4 at full size (6e7 rows): both variants vs the CPU restatement."""
+    n = 60_000_000
+    d = datagen.fact_table(ctx, n)
+    t = {c: v.numpy() for c, v in d.items()}
+    _cmp_q1(ops.filter_groupby_q1(ctx, d, 2505), orc.q1_flags(t, 2505), ordered=True)
+    _cmp_q1(ops.filter_groupby_q1(ctx, d, 2505, by_key=True), orc.q1_key(t, 2505), ordered=False)
