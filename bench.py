@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only", default="")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (the profiler's byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -138,6 +140,13 @@ def main():
     import pimdal_amd
     ctx = pimdal_amd.Context(0)
     ops_out = {}
+    if args.calibrate:
+        import ctypes as C
+        from pimdal_amd._lib import call
+        a, b = ctx.empty(1 << 28, "uint32"), ctx.empty(1 << 28, "uint32")
+        call("pimdal_dev_copy", ctx.handle, b.vp, a.vp, 1 << 30)
+        ctx.sync()
+        a.free(); b.free()
     log("bench: sort 2^28 uniform")
     srt = bench_sort(ctx, args.steps, args.warmup)
     if args.only in ("", "all"):

@@ -79,6 +79,8 @@ int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
 /* ---------------- device memory ---------------- */
 int pimdal_dev_alloc(pimdal_ctx* ctx, uint64_t bytes, void** dptr);
 int pimdal_dev_free(pimdal_ctx* ctx, void* dptr);
+/* Device-to-device copy on the context stream; asynchronous. */
+int pimdal_dev_copy(pimdal_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 /* Host <-> device copies through pinned staging; synchronous. */
 int pimdal_memcpy_h2d(pimdal_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int pimdal_memcpy_d2h(pimdal_ctx* ctx, void* dst, const void* src, uint64_t bytes);

@@ -54,6 +54,7 @@ SIGNATURES: dict[str, list] = {
     "pimdal_ctx_get_tune": [c_ctx_p, C.c_char_p, C.POINTER(i64)],
     "pimdal_dev_alloc": [c_ctx_p, u64, C.POINTER(vp)],
     "pimdal_dev_free": [c_ctx_p, vp],
+    "pimdal_dev_copy": [c_ctx_p, vp, vp, u64],
     "pimdal_memcpy_h2d": [c_ctx_p, vp, vp, u64],
     "pimdal_memcpy_d2h": [c_ctx_p, vp, vp, u64],
     "pimdal_col_alloc": [c_ctx_p, C.c_int, u64, C.POINTER(c_col_p)],

@@ -493,6 +493,7 @@ __global__ __launch_bounds__(GB_BLOCK) void k_gb_direct(GbSource src, uint64_t n
             const uint64_t want = GB_OCC | key;
             uint32_t h = pd::slot_hash(key) & (TS - 1);
             bool ok = true;
+            uint32_t probes = 0;
             while (true) {
                 const uint64_t cur = s_key[h];
                 if (cur == want) break;
@@ -505,7 +506,12 @@ __global__ __launch_bounds__(GB_BLOCK) void k_gb_direct(GbSource src, uint64_t n
                     if (prev == want) break;
                 }
                 h = (h + 1) & (TS - 1);
-                if (s_over) { ok = false; break; }
+                if (++probes >= TS ||
+                    __hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    s_over = 1;
+                    ok = false;
+                    break;
+                }
             }
             if (ok) add_fields<Agg>(cols, h, f);
         }

@@ -391,6 +391,13 @@ extern "C" int pimdal_dev_free(pimdal_ctx* c, void* dptr) {
     PD_API_END
 }
 
+extern "C" int pimdal_dev_copy(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && (bytes == 0 || (dst && src)), PIMDAL_E_INVALID, "NULL argument");
+    copy_d2d(c, dst, src, bytes, c->stream);
+    PD_API_END
+}
+
 extern "C" int pimdal_memcpy_h2d(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes) {
     PD_API_BEGIN
     PD_REQUIRE(c && (bytes == 0 || (dst && src)), PIMDAL_E_INVALID, "NULL argument");
