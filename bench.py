@@ -106,6 +106,31 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
+def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, rate_qps=1e6, slots=4):
+    """Config 5: mixed query stream over resident tables, batched vs one at a time."""
+    import numpy as np
+    from pimdal_amd import datagen, serving
+    fact = datagen.fact_table(ctx, n_fact, groups=n_dim)
+    dim = datagen.dim_table(ctx, n_dim)
+    srv = serving.Server(ctx, fact, dim, slots=slots, arena_bytes=1 << 30)
+    wq = serving.gen_queries(warm, n_fact, rate_qps, seed=0x5EED0105)
+    srv.run(wq, serving.BATCHED)
+    srv.run(wq[:50], serving.SEQUENTIAL)
+    q = serving.gen_queries(nq, n_fact, rate_qps, seed=0x5EED0005)
+    r_seq, st_seq = srv.run(q, serving.SEQUENTIAL)
+    r_bat, st_bat = srv.run(q, serving.BATCHED)
+    same = bool(np.array_equal(r_seq["count"], r_bat["count"]) and np.array_equal(r_seq["checksum"], r_bat["checksum"]))
+    srv.close()
+    for d in (fact, dim):
+        for a in d.values():
+            a.free()
+    return {"queries": nq, "arrival_rate_qps": rate_qps, "slots": slots,
+            "batched": {k: st_bat[k] for k in ("qps", "p50_us", "p99_us", "mean_us", "batches", "max_inflight")},
+            "sequential": {k: st_seq[k] for k in ("qps", "p50_us", "p99_us", "mean_us")},
+            "qps_speedup": st_bat["qps"] / st_seq["qps"] if st_seq["qps"] else None,
+            "results_identical": same}
+
+
 def cpu_baseline_sort(sample_n=1 << 24, reps=3):
     """The CPU restatement (oracle/) of the stable pair sort, all host threads."""
     import numpy as np
@@ -129,6 +154,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only", default="")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], help="tuning knob key=value (repeatable)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device-to-device copy (the profiler's byte-counter reference)")
     args = ap.parse_args()
@@ -139,6 +165,9 @@ def main():
 
     import pimdal_amd
     ctx = pimdal_amd.Context(0)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        ctx.set_tune(k, int(v))
     ops_out = {}
     if args.calibrate:
         import ctypes as C
@@ -154,6 +183,9 @@ def main():
         ops_out["sort_zipf_2^28"] = bench_sort(ctx, max(3, args.steps // 2), 1, zipf=True)
         log("bench: select 2^27")
         ops_out["select_2^27"] = bench_select(ctx, args.steps, args.warmup)
+    if args.only in ("", "all", "serve"):
+        log("bench: serving mixed stream")
+        ops_out["serve_config5"] = bench_serve(ctx)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")

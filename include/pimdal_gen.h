@@ -108,7 +108,22 @@ enum {
     PD_COL_BUILD_PAY = 5, PD_COL_PROBE_PAY = 6, PD_COL_ZIPF = 7,
     PD_COL_ORDERKEY_ROWS = 10, PD_COL_ORDERKEY_VALS = 11, PD_COL_QTY = 12,
     PD_COL_PRICE = 13, PD_COL_DISC = 14, PD_COL_TAX = 15, PD_COL_DATE = 16,
-    PD_COL_FLAG1 = 17, PD_COL_FLAG2 = 18
+    PD_COL_FLAG1 = 17, PD_COL_FLAG2 = 18, PD_COL_DIM_PERM = 19, PD_COL_DIM_DATE = 20, PD_COL_DIM_CAT = 21
 };
+
+/* Order-independent result checksum terms of the serving layer: a query's
+ * checksum is the 64-bit wrapping sum of one term per result row. */
+#define PD_CK_SELECT 0x53454C4543540001ULL
+#define PD_CK_JOIN 0x4A4F494E00000002ULL
+#define PD_CK_GROUP 0x47524F5550000003ULL
+PD_GEN_FN uint64_t pd_ck_select(uint64_t fact_row) { return pd_mix64(fact_row ^ PD_CK_SELECT); }
+PD_GEN_FN uint64_t pd_ck_join(uint64_t fact_row, uint64_t dim_row) {
+    return pd_mix64(((fact_row << 32) | dim_row) ^ PD_CK_JOIN);
+}
+PD_GEN_FN uint64_t pd_ck_group(int64_t key, const int64_t* v /* qty, price, disc, disc_price, charge lo, charge hi, count */) {
+    uint64_t h = pd_mix64((uint64_t)key ^ PD_CK_GROUP);
+    for (int i = 0; i < 7; i++) h = pd_mix64(h ^ (uint64_t)v[i]);
+    return h;
+}
 
 #endif /* PIMDAL_GEN_H */

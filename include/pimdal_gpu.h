@@ -109,7 +109,8 @@ int pimdal_col_export(pimdal_ctx* ctx, const pimdal_col* col, struct ArrowArray*
  * predicate form of select.cpp:33-36, bounds as arguments), order
  * preserving.  Outputs the payload and the row index of every kept row;
  * pass the key column as payload to get the reference's compacted keys.
- * key type: PIMDAL_U32 or PIMDAL_I32 (bounds are read as that type).
+ * key type: PIMDAL_U32, PIMDAL_I32 or PIMDAL_I64 (compared as signed 64-bit
+ * against lo/hi); the payload is 4 bytes wide.
  * out_payload / out_rowid may be NULL to skip them. */
 int pimdal_select(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* payload, int64_t lo, int64_t hi,
                   pimdal_col** out_payload, pimdal_col** out_rowid);
@@ -203,6 +204,68 @@ int pimdal_filter_groupby_q1_dev(pimdal_ctx* ctx, const int32_t* date, const uin
                                  const int64_t* discount, const int64_t* tax, uint64_t n, int32_t cutoff,
                                  const pimdal_q1_result* out, uint64_t capacity, uint64_t* n_groups);
 
+/* ---------------- serving ----------------
+ * No counterpart in the reference (its drivers run one operator per
+ * binary, pimdal/main/<operator>/host): a request loop over tables kept resident in
+ * HBM.  Queries of three kinds run over a slice [offset, offset + rows) of
+ * the fact table:
+ *   PIMDAL_Q_SELECT  rows with price < param                -> fact row ids
+ *   PIMDAL_Q_JOIN    slice orderkey = dim key, dim date < param -> (fact row, dim row)
+ *   PIMDAL_Q_GROUPBY date <= param, group by (flag1, flag2), Q1 measures
+ * Each result is reported as a row count and an order-independent checksum
+ * (include/pimdal_gen.h: pd_ck_select / pd_ck_join / pd_ck_group).
+ * PIMDAL_SERVE_SEQUENTIAL runs one query at a time through the plain
+ * operators; PIMDAL_SERVE_BATCHED coalesces arrived queries into batches on
+ * `slots` streams (continuous batching, one hipGraph replay per batch). */
+typedef struct pimdal_server pimdal_server;
+
+enum { PIMDAL_Q_SELECT = 1, PIMDAL_Q_JOIN = 2, PIMDAL_Q_GROUPBY = 3 };
+enum { PIMDAL_SERVE_SEQUENTIAL = 0, PIMDAL_SERVE_BATCHED = 1 };
+
+typedef struct {
+    const int32_t* orderkey;
+    const int64_t* quantity;
+    const int64_t* price;
+    const int64_t* discount;
+    const int64_t* tax;
+    const int32_t* date;
+    const uint8_t* flag1;
+    const uint8_t* flag2;
+    uint64_t n_fact;
+    const int32_t* dim_key;  /* unique */
+    const int32_t* dim_date;
+    const uint8_t* dim_category;
+    uint64_t n_dim;
+} pimdal_tables;
+
+typedef struct {
+    int32_t kind;
+    uint32_t reserved;
+    uint64_t offset;
+    uint64_t rows;
+    int64_t param;
+    double arrival_us;  /* open-loop arrival, microseconds after the run starts; non-decreasing */
+} pimdal_query;
+
+typedef struct {
+    uint64_t count;
+    uint64_t checksum;
+    double start_us, end_us, latency_us;  /* latency = end - arrival */
+} pimdal_query_result;
+
+typedef struct {
+    double makespan_us, qps, p50_us, p99_us, mean_us;
+    uint64_t batches, max_inflight;
+} pimdal_serve_stats;
+
+/* Builds the resident dimension-key index and `slots` stream slots, each with
+ * an output arena of slot_arena_bytes (0: 1 GiB) charged to the budget. */
+int pimdal_server_create(pimdal_ctx* ctx, const pimdal_tables* tables, int slots, uint64_t slot_arena_bytes,
+                         pimdal_server** out);
+int pimdal_server_run(pimdal_server* server, const pimdal_query* queries, uint64_t n, int mode,
+                      pimdal_query_result* results, pimdal_serve_stats* stats);
+int pimdal_server_destroy(pimdal_server* server);
+
 /* ---------------- synthetic data (device generators) ----------------
  * Counter-based generators, identical to include/pimdal_gen.h on the host;
  * they replace reference/db_operators/shared.cpp:9-60. */
@@ -219,6 +282,9 @@ int pimdal_gen_zipf(pimdal_ctx* ctx, uint64_t seed, uint32_t column, uint64_t m,
                     uint32_t* d_out_rank);
 int pimdal_fmix32(pimdal_ctx* ctx, uint64_t n, const uint32_t* d_in, uint32_t* d_out);
 int pimdal_gen_orderkey(pimdal_ctx* ctx, uint64_t seed, uint64_t n, uint64_t groups, int32_t* d_out);
+/* Dimension keys: the `groups` distinct fact orderkey values, each once, in a
+ * seeded order. */
+int pimdal_gen_dim_keys(pimdal_ctx* ctx, uint64_t seed, uint64_t groups, int32_t* d_out);
 
 #ifdef __cplusplus
 }

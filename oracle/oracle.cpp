@@ -176,6 +176,11 @@ extern "C" uint64_t orc_select_i32(const int32_t* key, const uint32_t* pay, uint
     return select_impl<int32_t>(key, pay, n, lo, hi, out_pay, out_rid);
 }
 
+extern "C" uint64_t orc_select_i64(const int64_t* key, const uint32_t* pay, uint64_t n, int64_t lo, int64_t hi,
+                        uint32_t* out_pay, uint32_t* out_rid) {
+    return select_impl<int64_t>(key, pay, n, lo, hi, out_pay, out_rid);
+}
+
 /* ------------------------------------------------------------------ */
 /* Sort: reference/db_operators/sort.cpp:30-40 (copy, then             */
 /* __gnu_parallel::sort).  Made stable for (key, payload) pairs by    */

@@ -30,6 +30,7 @@ _SIG = {
     "orc_hash32": (u32, [u32]),
     "orc_select_u32": (u64, [vp, vp, u64, u32, u32, vp, vp]),
     "orc_select_i32": (u64, [vp, vp, u64, i32, i32, vp, vp]),
+    "orc_select_i64": (u64, [vp, vp, u64, i64, i64, vp, vp]),
     "orc_sort_pairs_u32": (None, [vp, vp, u64, vp, vp]),
     "orc_sort_pairs_u32_u64": (None, [vp, vp, u64, vp, vp]),
     "orc_sort_keys_u32": (None, [vp, u64, vp]),
@@ -145,6 +146,8 @@ def select(key, pay, lo, hi):
         k = lib().orc_select_u32(_p(key), _p(pay), n, lo, hi, _p(op), _p(orow))
     elif key.dtype == np.int32:
         k = lib().orc_select_i32(_p(key), _p(pay), n, lo, hi, _p(op), _p(orow))
+    elif key.dtype == np.int64:
+        k = lib().orc_select_i64(_p(key), _p(pay), n, lo, hi, _p(op), _p(orow))
     else:
         raise TypeError(key.dtype)
     return op[:k], orow[:k]

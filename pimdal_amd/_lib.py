@@ -78,6 +78,10 @@ SIGNATURES: dict[str, list] = {
     "pimdal_hash_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_sort_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_filter_groupby_q1_dev": [c_ctx_p, vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, u64, C.POINTER(u64)],
+    "pimdal_server_create": [c_ctx_p, vp, C.c_int, u64, C.POINTER(vp)],
+    "pimdal_server_run": [vp, vp, u64, C.c_int, vp, vp],
+    "pimdal_server_destroy": [vp],
+    "pimdal_gen_dim_keys": [c_ctx_p, u64, u64, vp],
     "pimdal_gen_u32": [c_ctx_p, u64, u32, u64, vp],
     "pimdal_gen_range": [c_ctx_p, C.c_int, u64, u32, u64, i64, i64, vp],
     "pimdal_gen_perm": [c_ctx_p, u64, u32, u64, u64, u64, vp],

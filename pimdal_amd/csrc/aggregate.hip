@@ -737,6 +737,19 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
     return g;
 }
 
+void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n, uint8_t* lut_dev, uint32_t meta[2],
+                uint8_t* inv_host, hipStream_t s) {
+    Scratch small(c, 64 + 8, s);
+    uint32_t* bits = small.as<uint32_t>();
+    fill_u32(c, bits, 0, 18, s);
+    const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 8), (uint64_t)c->cu_count * 4));
+    if (n) launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
+    launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut_dev, lut_dev + 512, bits + 16);
+    PD_HIP(hipMemcpyAsync(meta, bits + 16, 8, hipMemcpyDeviceToHost, s));
+    if (inv_host) PD_HIP(hipMemcpyAsync(inv_host, lut_dev + 512, 512, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+}
+
 uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
                    int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");

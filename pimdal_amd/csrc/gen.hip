@@ -138,3 +138,18 @@ extern "C" int pimdal_gen_orderkey(pimdal_ctx* c, uint64_t seed, uint64_t n, uin
            pd_stream(seed, PD_COL_ORDERKEY_VALS), n, groups, out);
     PD_API_END
 }
+
+namespace pd {
+__global__ void k_gen_dim_keys(uint64_t sd, uint64_t sv, uint64_t groups, int32_t* out) {
+    PD_GRID_STRIDE(i, groups) out[i] = (int32_t)pd_perm(sv, 1ull << 31, pd_perm(sd, groups, i));
+}
+}  // namespace pd
+
+extern "C" int pimdal_gen_dim_keys(pimdal_ctx* c, uint64_t seed, uint64_t groups, int32_t* out) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && (out || !groups), PIMDAL_E_INVALID, "NULL argument");
+    PD_REQUIRE(groups >= 1 && groups <= (1ull << 31), PIMDAL_E_INVALID, "bad group count");
+    launch(c, "gen", k_gen_dim_keys, gen_grid(c, groups), dim3(256), 0, c->stream, pd_stream(seed, PD_COL_DIM_PERM),
+           pd_stream(seed, PD_COL_ORDERKEY_VALS), groups, out);
+    PD_API_END
+}

@@ -35,4 +35,9 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
                            const int64_t* tax, uint64_t n, int32_t cutoff, const pimdal_q1_result& out,
                            uint64_t cap, hipStream_t s);
 
+// Dense codes of two byte columns: lut_dev[0..512) = code of each byte value
+// (flag1 then flag2), lut_dev[512..1024) = inverse; meta = {n1, n2}.
+void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n, uint8_t* lut_dev, uint32_t meta[2],
+                uint8_t* inv_host, hipStream_t s);
+
 }  // namespace pd

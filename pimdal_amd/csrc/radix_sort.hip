@@ -24,10 +24,7 @@
 
 namespace pd {
 
-constexpr int RS_BLOCK = 512;
-constexpr int RS_WAVES = RS_BLOCK / PD_WAVE;   // 8
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;   // 8192
+constexpr int RS_ITEMS = 16;                   // keys per thread; tile = block * 16
 constexpr int RS_BINS = 256;
 constexpr uint32_t RS_FLAG_AGG = 1u << 30;
 constexpr uint32_t RS_FLAG_PRE = 2u << 30;
@@ -96,21 +93,26 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 }
 
 // ---- one onesweep pass ----
-template <typename V, bool HAS_VALS>
-__global__ __launch_bounds__(RS_BLOCK, 2) void k_rs_onesweep(const uint32_t* __restrict__ kin, const V* __restrict__ vin,
-                                                             uint32_t* __restrict__ kout, V* __restrict__ vout,
-                                                             uint64_t n, uint32_t shift, uint32_t mask,
-                                                             const uint32_t* __restrict__ offs,
-                                                             uint32_t* __restrict__ status,
-                                                             uint32_t* __restrict__ tile_ctr) {
+// Compiler-only barrier: LDS instructions of one wave execute in issue order,
+// so the ranking steps below need ordering against code motion, not waits.
+#define PD_COMPILER_BARRIER() __asm__ volatile("" ::: "memory")
+
+template <typename V, bool HAS_VALS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
+    const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
+    uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ tile_ctr) {
+    constexpr int WAVES = BLOCK / PD_WAVE;
+    constexpr int TILE = BLOCK * RS_ITEMS;
+    constexpr int BUF_WORDS = TILE * (HAS_VALS ? sizeof(V) / 4 : 1);
+    static_assert(BUF_WORDS * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the reorder buffer");
     __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_match[RS_WAVES][RS_BINS];
-    __shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
-    __shared__ uint32_t s_local[RS_BINS];
-    __shared__ uint32_t s_gbase[RS_BINS];
+    // the per-wave match masks are dead once ranking ends: the reorder buffer reuses them
+    __shared__ __align__(16) uint32_t s_buf[BUF_WORDS];
+    __shared__ uint16_t s_cnt[WAVES][RS_BINS];
+    __shared__ uint32_t s_off[RS_BINS];   // tile-local digit start, then global base minus it
     __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
-    __shared__ uint8_t s_dig[RS_TILE];
-    __shared__ __align__(16) uint32_t s_buf[RS_TILE * (HAS_VALS ? sizeof(V) / 4 : 1)];
+    auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_buf);
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
@@ -120,102 +122,119 @@ __global__ __launch_bounds__(RS_BLOCK, 2) void k_rs_onesweep(const uint32_t* __r
     }
     __syncthreads();
     const uint64_t tile = s_tile;
-    const uint64_t tile_base = tile * RS_TILE;
+    const uint64_t tile_base = tile * TILE;
     const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * RS_ITEMS);
-    const uint32_t tile_count = (uint32_t)min((uint64_t)RS_TILE, n - tile_base);
+    const uint32_t tile_count = (uint32_t)min((uint64_t)TILE, n - tile_base);
 
-    // ---- load keys (and payloads), wave-striped: idx = wave_base + i*64 + lane ----
+    // ---- load keys and payloads, wave-striped: idx = wave_base + i*64 + lane ----
+    // Rows past n get the largest digit, so they rank after every real row
+    // of the last tile and land past n (never written).
     uint32_t key[RS_ITEMS];
     V val[HAS_VALS ? RS_ITEMS : 1];
-    uint32_t dig[RS_ITEMS];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
         const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
         const bool ok = idx < n;
         key[i] = ok ? kin[idx] : 0xFFFFFFFFu;
         if constexpr (HAS_VALS) val[i] = ok ? vin[idx] : V(0);
-        dig[i] = ok ? ((key[i] >> shift) & mask) : (RS_BINS - 1);
     }
+    auto digit_of = [&](int i) -> uint32_t {
+        const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
+        return idx < n ? ((key[i] >> shift) & mask) : (RS_BINS - 1);
+    };
 
     // ---- stable rank inside the wave: match mask per item ----
     const uint64_t lt = pd::lanemask_lt();
-    uint32_t rank[RS_ITEMS];
+    uint32_t pos[RS_ITEMS];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
-        const uint32_t d = dig[i];
+        const uint32_t d = digit_of(i);
         atomicOr((unsigned long long*)&s_match[w][d], (unsigned long long)(1ull << l));
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const uint64_t m = __hip_atomic_load(&s_match[w][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        PD_COMPILER_BARRIER();
+        const uint64_t m = *(volatile uint64_t*)&s_match[w][d];
+        const uint32_t old = *(volatile uint16_t*)&s_cnt[w][d];
         const uint32_t below = pd::popc64(m & lt);
-        const uint32_t old = s_cnt[w][d];
-        rank[i] = old + below;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        pos[i] = old + below;
+        PD_COMPILER_BARRIER();
         if (below == 0) {
-            s_cnt[w][d] = old + pd::popc64(m);
-            __hip_atomic_store(&s_match[w][d], (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            *(volatile uint16_t*)&s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
+            *(volatile uint64_t*)&s_match[w][d] = 0;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        PD_COMPILER_BARRIER();
     }
     __syncthreads();
 
     // ---- per digit: exclusive prefix over waves, tile count, publish ----
+    // (digit t is handled by thread t; a 256-thread block covers all digits)
     uint32_t cnt_d = 0, inc_d = 0;
     if (t < RS_BINS) {
 #pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ww++) {
+        for (int ww = 0; ww < WAVES; ww++) {
             const uint32_t c = s_cnt[ww][t];
-            s_cnt[ww][t] = cnt_d;
+            s_cnt[ww][t] = (uint16_t)cnt_d;
             cnt_d += c;
         }
         pd::st_relaxed(&status[tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
-        // block exclusive scan of cnt_d over the 256 digits (waves 0..3)
         inc_d = pd::wave_incl_scan(cnt_d);
         if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
     }
     __syncthreads();
+    uint32_t local_d = 0;
     if (t < RS_BINS) {
         uint32_t add = 0;
         for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
-        s_local[t] = add + inc_d - cnt_d;
+        local_d = add + inc_d - cnt_d;
+        s_off[t] = local_d;
     }
     __syncthreads();
 
-    // ---- reorder the tile in LDS by digit ----
-    uint32_t pos[RS_ITEMS];
+    // ---- reorder the tile's keys in LDS by digit ----
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
-        const uint32_t d = dig[i];
-        pos[i] = s_local[d] + s_cnt[w][d] + rank[i];
+        const uint32_t d = digit_of(i);
+        pos[i] += s_off[d] + s_cnt[w][d];
         s_buf[pos[i]] = key[i];
-        s_dig[pos[i]] = (uint8_t)d;
     }
 
-    // ---- global offset of each digit: decoupled look-back ----
+    // ---- global offset of each digit: decoupled look-back, 4 tiles per round trip ----
     if (t < RS_BINS) {
         uint32_t excl = 0;
         if (tile > 0) {
             int64_t j = (int64_t)tile - 1;
-            while (j >= 0) {
-                uint32_t s = pd::ld_relaxed(&status[(uint64_t)j * RS_BINS + t]);
-                if ((s >> 30) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+            bool done = false;
+            while (!done) {
+                uint32_t st[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    st[q] = (j - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
+                bool stall = false;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (done || stall) continue;
+                    if ((st[q] >> 30) == 0) { stall = true; continue; }
+                    excl += st[q] & RS_VALUE;
+                    if ((st[q] >> 30) == 2) done = true;
+                    else j--;
                 }
-                excl += s & RS_VALUE;
-                if ((s >> 30) == 2) break;
-                j--;
+                if (stall) __builtin_amdgcn_s_sleep(1);
             }
             pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt_d));
         }
-        s_gbase[t] = offs[t] + excl - s_local[t];
+        s_off[t] = offs[t] + excl - local_d;
     }
     __syncthreads();
 
-    // ---- write keys in digit runs ----
-#pragma unroll 4
-    for (uint32_t j = t; j < tile_count; j += RS_BLOCK) {
-        const uint64_t dst = (uint64_t)s_gbase[s_dig[j]] + j;
-        if (dst < n) kout[dst] = s_buf[j];
+    // ---- write keys in digit runs; remember each slot's digit for the payloads ----
+    uint32_t wd[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const uint32_t j = t + k * BLOCK;
+        const uint32_t kk = s_buf[j];
+        // slots past tile_count hold padding rows (largest digit, placed last)
+        const uint32_t d = (j < tile_count) ? ((kk >> shift) & mask) : (RS_BINS - 1);
+        wd[k] = d;
+        const uint64_t dst = (uint64_t)s_off[d] + j;
+        if (j < tile_count && dst < n) kout[dst] = kk;
     }
     if constexpr (HAS_VALS) {
         __syncthreads();
@@ -223,10 +242,11 @@ __global__ __launch_bounds__(RS_BLOCK, 2) void k_rs_onesweep(const uint32_t* __r
 #pragma unroll
         for (int i = 0; i < RS_ITEMS; i++) s_val[pos[i]] = val[i];
         __syncthreads();
-#pragma unroll 4
-        for (uint32_t j = t; j < tile_count; j += RS_BLOCK) {
-            const uint64_t dst = (uint64_t)s_gbase[s_dig[j]] + j;
-            if (dst < n) vout[dst] = s_val[j];
+#pragma unroll
+        for (int k = 0; k < RS_ITEMS; k++) {
+            const uint32_t j = t + k * BLOCK;
+            const uint64_t dst = (uint64_t)s_off[wd[k]] + j;
+            if (j < tile_count && dst < n) vout[dst] = s_val[j];
         }
     }
 }
@@ -234,7 +254,8 @@ __global__ __launch_bounds__(RS_BLOCK, 2) void k_rs_onesweep(const uint32_t* __r
 template <typename V, bool HV>
 static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
                        const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
-    const uint64_t ntiles = ceil_div(n, RS_TILE);
+    const int block = (int)c->tune_get("sort_block", 256) == 512 ? 512 : 256;
+    const uint64_t ntiles = ceil_div(n, (uint64_t)block * RS_ITEMS);
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
     Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
@@ -246,8 +267,12 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
         V* vdst = to_out ? vout : tv.as<V>();
         fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
-        launch(c, "radix_onesweep", k_rs_onesweep<V, HV>, dim3((uint32_t)ntiles), dim3(RS_BLOCK), 0, s, ksrc, vsrc,
-               kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j);
+        if (block == 512)
+            launch(c, "radix_onesweep", k_rs_onesweep<V, HV, 512>, dim3((uint32_t)ntiles), dim3(512), 0, s, ksrc,
+                   vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j);
+        else
+            launch(c, "radix_onesweep", k_rs_onesweep<V, HV, 256>, dim3((uint32_t)ntiles), dim3(256), 0, s, ksrc,
+                   vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j);
         ksrc = kdst;
         vsrc = vdst;
     }

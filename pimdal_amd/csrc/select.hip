@@ -7,8 +7,8 @@
 // with 16-byte loads, ranks kept rows with wave64 ballots and popcounts,
 // takes its output offset from a decoupled look-back over tile totals
 // (tile ids come from an atomic counter, so every predecessor tile is
-// already resident and the wait always ends), stages the kept (payload,
-// row id) pairs in LDS and writes them out with coalesced stores.
+// already resident and the wait always ends), and writes the kept (payload,
+// row id) pairs from registers into their contiguous output range.
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
 #include "pd_ops.hpp"
@@ -29,8 +29,6 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select(const K* __restrict__ key,
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_tot[SEL_BLOCK / PD_WAVE];
     __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_pay[SEL_TILE];
-    __shared__ uint32_t s_rid[SEL_TILE];
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
@@ -39,22 +37,39 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select(const K* __restrict__ key,
     const uint64_t tile_base = tile * SEL_TILE;
     const uint64_t wave_base = tile_base + (uint64_t)w * SEL_WAVE_ROWS;
 
-    // ---- predicate: 4 bits per group, row = wave_base + g*256 + l*4 + e ----
+    // ---- load keys and payloads together (row = wave_base + g*256 + l*4 + e) ----
+    K k[SEL_GROUPS][4];
+    uint32_t p[SEL_GROUPS][4];
+#pragma unroll
+    for (int g = 0; g < SEL_GROUPS; g++) {
+        const uint64_t r0 = wave_base + (uint64_t)g * 256 + l * 4;
+        if (VEC && r0 + 3 < n) {
+            if constexpr (sizeof(K) == 8) {
+                const uint4 a = *reinterpret_cast<const uint4*>(key + r0);
+                const uint4 b = *reinterpret_cast<const uint4*>(key + r0 + 2);
+                k[g][0] = (K)(((uint64_t)a.y << 32) | a.x); k[g][1] = (K)(((uint64_t)a.w << 32) | a.z);
+                k[g][2] = (K)(((uint64_t)b.y << 32) | b.x); k[g][3] = (K)(((uint64_t)b.w << 32) | b.z);
+            } else {
+                const uint4 v = *reinterpret_cast<const uint4*>(key + r0);
+                k[g][0] = (K)v.x; k[g][1] = (K)v.y; k[g][2] = (K)v.z; k[g][3] = (K)v.w;
+            }
+            const uint4 q = *reinterpret_cast<const uint4*>(pay + r0);
+            p[g][0] = q.x; p[g][1] = q.y; p[g][2] = q.z; p[g][3] = q.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                k[g][e] = (r0 + e < n) ? key[r0 + e] : (K)0;
+                p[g][e] = (r0 + e < n) ? pay[r0 + e] : 0u;
+            }
+        }
+    }
     uint32_t bits = 0;
 #pragma unroll
     for (int g = 0; g < SEL_GROUPS; g++) {
         const uint64_t r0 = wave_base + (uint64_t)g * 256 + l * 4;
-        K k[4];
-        if (VEC && r0 + 3 < n) {
-            const uint4 v = *reinterpret_cast<const uint4*>(key + r0);
-            k[0] = (K)v.x; k[1] = (K)v.y; k[2] = (K)v.z; k[3] = (K)v.w;
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; e++) k[e] = (r0 + e < n) ? key[r0 + e] : (K)0;
-        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const bool in = (r0 + e < n) && ((int64_t)k[e] >= lo) && ((int64_t)k[e] < hi);
+            const bool in = (r0 + e < n) && ((int64_t)k[g][e] >= lo) && ((int64_t)k[g][e] < hi);
             bits |= (uint32_t)in << (g * 4 + e);
         }
     }
@@ -93,48 +108,37 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select(const K* __restrict__ key,
             if (tile == ntiles - 1) *d_count = excl + tile_total;
         }
     }
+    __syncthreads();
 
-    // ---- stage kept rows in LDS, in row order ----
+    // ---- write kept rows straight from registers, in row order ----
+    const uint64_t base = s_excl + wave_off;
 #pragma unroll
     for (int g = 0; g < SEL_GROUPS; g++) {
         const uint32_t gb = (bits >> (g * 4)) & 0xF;
-        if (!pd::ballot(gb != 0)) continue;
+        uint64_t o = base + group_off[g];
         const uint64_t r0 = wave_base + (uint64_t)g * 256 + l * 4;
-        uint32_t p[4];
-        if (VEC && r0 + 3 < n) {
-            const uint4 v = *reinterpret_cast<const uint4*>(pay + r0);
-            p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w;
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; e++) p[e] = (r0 + e < n) ? pay[r0 + e] : 0u;
-        }
-        uint32_t pos = wave_off + group_off[g];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             if (gb & (1u << e)) {
-                s_pay[pos] = p[e];
-                s_rid[pos] = (uint32_t)(r0 + e);
-                pos++;
+                if (o < n) {
+                    if (out_pay) out_pay[o] = p[g][e];
+                    if (out_rid) out_rid[o] = (uint32_t)(r0 + e);
+                }
+                o++;
             }
         }
-    }
-    __syncthreads();
-
-    // ---- coalesced write-out ----
-    const uint64_t excl = s_excl;
-    for (uint32_t j = t; j < tile_total; j += SEL_BLOCK) {
-        if (excl + j >= n) break;
-        if (out_pay) out_pay[excl + j] = s_pay[j];
-        if (out_rid) out_rid[excl + j] = s_rid[j];
     }
 }
 
 void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
                   int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s) {
-    PD_REQUIRE(kt == PIMDAL_U32 || kt == PIMDAL_I32, PIMDAL_E_UNSUPPORTED, "select key must be uint32 or int32");
+    PD_REQUIRE(kt == PIMDAL_U32 || kt == PIMDAL_I32 || kt == PIMDAL_I64, PIMDAL_E_UNSUPPORTED,
+               "select key must be uint32, int32 or int64");
+    PD_REQUIRE(kt != PIMDAL_I64 || pay || !out_pay, PIMDAL_E_INVALID, "int64 keys need a 32-bit payload column");
     PD_REQUIRE(n < (1ull << 32), PIMDAL_E_INVALID, "select supports up to 2^32-1 rows (uint32 row ids)");
     PD_REQUIRE(key || n == 0, PIMDAL_E_INVALID, "key is NULL");
     PD_REQUIRE(!out_pay || pay, PIMDAL_E_INVALID, "payload is NULL but out_payload requested");
+    // without a payload column the key words stand in (read, never written out)
     if (!pay) pay = reinterpret_cast<const uint32_t*>(key);
     if (n == 0) {
         fill_u64(c, d_count, 0, 1, s);
@@ -151,10 +155,14 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
         auto* kp = reinterpret_cast<const uint32_t*>(key);
         if (vec) launch(c, "select", k_select<uint32_t, true>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
         else     launch(c, "select", k_select<uint32_t, false>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
-    } else {
+    } else if (kt == PIMDAL_I32) {
         auto* kp = reinterpret_cast<const int32_t*>(key);
         if (vec) launch(c, "select", k_select<int32_t, true>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
         else     launch(c, "select", k_select<int32_t, false>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
+    } else {
+        auto* kp = reinterpret_cast<const int64_t*>(key);
+        if (vec) launch(c, "select", k_select<int64_t, true>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
+        else     launch(c, "select", k_select<int64_t, false>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
     }
 }
 

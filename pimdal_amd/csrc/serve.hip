@@ -1,0 +1,714 @@
+// serve.hip — serving loop over device-resident tables.
+//
+// Two ways to run a stream of queries over the resident fact and dimension
+// tables:
+//  * mode PIMDAL_SERVE_SEQUENTIAL — one query at a time through the plain
+//    operator entry points (the drop-in path): select_range over the price
+//    column; for a join, a select of the qualifying dimension rows followed
+//    by hash_join against the fact slice; the fused Q1 group-by over the
+//    slice.  Each query completes before the next starts.
+//  * mode PIMDAL_SERVE_BATCHED — continuous batching: arrivals queue up
+//    while earlier batches run; whenever one of `slots` stream slots is free
+//    and its output arena can hold them, every arrived query (FIFO) is
+//    coalesced into one batch.  A batch is one hipGraph replay per slot:
+//    descriptor copy -> reset -> persistent select / join / group-by kernels
+//    that walk a device work list of (query, tile) items -> group-by
+//    finalisation -> result copy.  Joins probe a dimension-key index built
+//    once at server creation (the table stays resident, so the index does
+//    too).  Slots run on their own streams, so batches overlap on the GPU.
+// Both modes report per query the result count and an order-independent
+// checksum (include/pimdal_gen.h), so results can be compared across modes
+// and with the CPU restatement.
+#include <string.h>
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "pd_runtime.hpp"
+#include "pd_device.hpp"
+#include "pd_ops.hpp"
+#include "../../include/pimdal_gen.h"
+
+namespace pd {
+
+constexpr int SV_MAXQ = 512;          // queries per kind per batch
+constexpr int SV_BLOCK = 256;
+constexpr int SV_TILE = 4096;         // rows per select / join work item
+constexpr uint32_t SV_GB_CHUNK = 1u << 18;  // rows per group-by work item
+constexpr int SV_G = 8;               // dense group codes in served group-bys
+
+struct SvQuery {
+    int64_t param;
+    uint64_t offset, rows;
+    uint64_t item0;      // first work item of this query within its kind
+    uint64_t out_off;    // output offset in the arena (elements of the kind's output type)
+    uint64_t status_off; // first look-back word of this query
+    uint32_t qid;        // index in the batch's result array
+    uint32_t pad;
+};
+
+struct SvKind {
+    uint32_t nq, pad;
+    uint64_t items;      // total work items of this kind
+    SvQuery q[SV_MAXQ];
+};
+
+struct SvBatch {
+    SvKind sel, join, gb;
+    uint64_t status_words;
+};
+
+struct SvResult {
+    uint64_t count, checksum;
+};
+
+struct SvTables {
+    const int32_t* orderkey;
+    const int64_t* qty;
+    const int64_t* price;
+    const int64_t* disc;
+    const int64_t* tax;
+    const int32_t* date;
+    const uint8_t* flag1;
+    const uint8_t* flag2;
+    uint64_t n_fact;
+    const int32_t* dim_date;
+    const uint64_t* dim_index;  // (dim row + 1) << 32 | key, 0 = empty
+    uint64_t dim_mask;
+    const uint8_t* lut;         // flag codes: [0,256) flag1, [256,512) flag2
+    uint32_t n2;
+};
+
+__device__ __forceinline__ uint32_t sv_find_query(const SvKind& k, uint64_t wi) {
+    uint32_t lo = 0, hi = k.nq;  // last q with item0 <= wi
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (k.q[mid].item0 <= wi) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t sv_dim_lookup(const SvTables& T, int32_t key) {
+    uint64_t h = pd_mix64((uint32_t)key) & T.dim_mask;
+    while (true) {
+        const uint64_t s = T.dim_index[h];
+        if (s == 0) return 0xFFFFFFFFu;
+        if ((uint32_t)s == (uint32_t)key) return (uint32_t)(s >> 32) - 1;
+        h = (h + 1) & T.dim_mask;
+    }
+}
+
+__global__ void k_sv_reset(const SvBatch* __restrict__ b, uint64_t* __restrict__ status, SvResult* __restrict__ res,
+                           uint32_t nres, uint32_t* __restrict__ ctr) {
+    const uint64_t n = b->status_words;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        status[i] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nres; i += gridDim.x * blockDim.x) res[i] = SvResult{0, 0};
+    if (blockIdx.x == 0 && threadIdx.x < 4) ctr[threadIdx.x] = 0;
+}
+
+// Persistent select / join over (query, 4096-row tile) work items.  Per
+// query: decoupled look-back over its own tiles (items are taken in order,
+// so a predecessor tile is always held by a running block), order-
+// preserving output, checksum added atomically (a sum, so the order of the
+// additions does not matter).
+template <bool JOIN>
+__global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict__ b, SvTables T,
+                                                      void* __restrict__ arena, uint64_t* __restrict__ status,
+                                                      SvResult* __restrict__ res, uint32_t* __restrict__ ctr) {
+    __shared__ uint64_t s_wi;
+    __shared__ uint32_t s_wave_tot[SV_BLOCK / PD_WAVE];
+    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_ck[SV_BLOCK / PD_WAVE];
+    const SvKind& K = JOIN ? b->join : b->sel;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    constexpr int RPT = SV_TILE / SV_BLOCK;  // 16 rows per thread: row = tile_base + i*256 + t
+    while (true) {
+        if (t == 0) s_wi = atomicAdd(&ctr[JOIN ? 1 : 0], 1u);
+        __syncthreads();
+        const uint64_t wi = s_wi;
+        if (wi >= K.items) break;
+        const uint32_t qi = sv_find_query(K, wi);
+        const SvQuery& q = K.q[qi];
+        const uint64_t tile = wi - q.item0;
+        const uint64_t ntl = (q.rows + SV_TILE - 1) / SV_TILE;
+        const uint64_t r0 = q.offset + tile * SV_TILE;
+        const uint64_t rend = q.offset + q.rows;
+        uint32_t hit = 0;
+        uint32_t dimrow[JOIN ? RPT : 1];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+            bool m = false;
+            if (r < rend) {
+                if constexpr (JOIN) {
+                    const uint32_t d = sv_dim_lookup(T, T.orderkey[r]);
+                    m = d != 0xFFFFFFFFu && (int64_t)T.dim_date[d] < q.param;
+                    dimrow[i] = d;
+                } else {
+                    m = T.price[r] < q.param;
+                }
+            }
+            hit |= (uint32_t)m << i;
+        }
+        // order (i, wave, lane): per item i, counts per wave
+        __shared__ uint32_t s_cnt[RPT][SV_BLOCK / PD_WAVE];
+        uint64_t ck = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const bool m = (hit >> i) & 1;
+            const uint64_t bal = pd::ballot(m);
+            if (l == 0) s_cnt[i][w] = pd::popc64(bal);
+            if (m) {
+                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+                if constexpr (JOIN) ck += pd_ck_join(r, dimrow[i]);
+                else ck += pd_ck_select(r);
+            }
+        }
+        ck = pd::wave_sum64(ck);
+        if (l == 0) s_ck[w] = ck;
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t v = (&s_cnt[0][0])[l];
+            const uint32_t inc = pd::wave_incl_scan(v);
+            (&s_cnt[0][0])[l] = inc - v;
+            if (l == PD_WAVE - 1) s_wave_tot[0] = inc;
+        }
+        __syncthreads();
+        const uint32_t total = s_wave_tot[0];
+        if (w == 0) {
+            const uint64_t e = pd::lookback_single(status + q.status_off, tile, total);
+            if (l == 0) {
+                s_excl = e;
+                uint64_t cks = 0;
+                for (int ww = 0; ww < SV_BLOCK / PD_WAVE; ww++) cks += s_ck[ww];
+                if (cks) atomicAdd((unsigned long long*)&res[q.qid].checksum, (unsigned long long)cks);
+                if (tile == ntl - 1) res[q.qid].count = e + total;
+            }
+        }
+        __syncthreads();
+        const uint64_t excl = s_excl;
+        const uint64_t lt = pd::lanemask_lt();
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const bool m = (hit >> i) & 1;
+            const uint64_t bal = pd::ballot(m);
+            if (m) {
+                const uint64_t o = excl + s_cnt[i][w] + pd::popc64(bal & lt);
+                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+                if (o < q.rows) {
+                    if constexpr (JOIN)
+                        reinterpret_cast<uint2*>(arena)[q.out_off + o] = make_uint2((uint32_t)r, dimrow[i]);
+                    else
+                        reinterpret_cast<uint32_t*>(arena)[q.out_off + o] = (uint32_t)r;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Persistent dense group-by over (query, 2^18-row chunk) items: Q1 measures
+// in registers for up to 8 flag codes, one partial row per item.
+__global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restrict__ b, SvTables T,
+                                                         uint64_t* __restrict__ partials, uint32_t* __restrict__ ctr) {
+    __shared__ uint64_t s_wi;
+    __shared__ uint64_t s_red[SV_BLOCK / PD_WAVE][SV_G][8];
+    const SvKind& K = b->gb;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    while (true) {
+        if (t == 0) s_wi = atomicAdd(&ctr[2], 1u);
+        __syncthreads();
+        const uint64_t wi = s_wi;
+        if (wi >= K.items) break;
+        const SvQuery& q = K.q[sv_find_query(K, wi)];
+        const uint64_t c0 = q.offset + (wi - q.item0) * SV_GB_CHUNK;
+        const uint64_t c1 = min(q.offset + q.rows, c0 + SV_GB_CHUNK);
+        int64_t aq[SV_G], ap[SV_G], ad[SV_G], adp[SV_G], an[SV_G];
+        unsigned __int128 ach[SV_G];
+#pragma unroll
+        for (int g = 0; g < SV_G; g++) { aq[g] = ap[g] = ad[g] = adp[g] = an[g] = 0; ach[g] = 0; }
+        for (uint64_t r = c0 + t; r < c1; r += SV_BLOCK) {
+            if ((int64_t)T.date[r] > q.param) continue;
+            const uint32_t code = (uint32_t)T.lut[T.flag1[r]] * T.n2 + T.lut[256 + T.flag2[r]];
+            const int64_t qq = T.qty[r], p = T.price[r], d = T.disc[r], x = T.tax[r];
+            const int64_t dp = p * (100 - d);
+            const __int128 ch = (__int128)dp * (100 + x);
+#pragma unroll
+            for (int g = 0; g < SV_G; g++) {
+                const bool on = code == (uint32_t)g;
+                aq[g] += on ? qq : 0;
+                ap[g] += on ? p : 0;
+                ad[g] += on ? d : 0;
+                adp[g] += on ? dp : 0;
+                ach[g] += on ? (unsigned __int128)ch : 0;
+                an[g] += on ? 1 : 0;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < SV_G; g++) {
+            const uint64_t lo = (uint64_t)ach[g];
+            uint64_t v[8] = {(uint64_t)aq[g], (uint64_t)ap[g], (uint64_t)ad[g], (uint64_t)adp[g],
+                             lo & 0xFFFFFFFFull, (uint64_t)(ach[g] >> 64), (uint64_t)an[g], lo >> 32};
+#pragma unroll
+            for (int f = 0; f < 8; f++) v[f] = pd::wave_sum64(v[f]);
+            if (l == 0)
+#pragma unroll
+                for (int f = 0; f < 8; f++) s_red[w][g][f] = v[f];
+        }
+        __syncthreads();
+        if (t < SV_G * 8) {
+            const uint32_t g = t / 8, f = t % 8;
+            uint64_t s = 0;
+            for (int ww = 0; ww < SV_BLOCK / PD_WAVE; ww++) s += s_red[ww][g][f];
+            partials[(wi * SV_G + g) * 8 + f] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// One block per group-by query: 128-bit reduction of its items' partials,
+// then count and checksum of the non-empty groups (keys flag1*256+flag2).
+__global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint64_t* __restrict__ partials,
+                                   const uint8_t* __restrict__ inv, uint32_t n1, uint32_t n2,
+                                   SvResult* __restrict__ res) {
+    const SvKind& K = b->gb;
+    if (blockIdx.x >= K.nq) return;
+    const SvQuery& q = K.q[blockIdx.x];
+    const uint64_t items = (q.rows + SV_GB_CHUNK - 1) / SV_GB_CHUNK;
+    __shared__ uint64_t s_ck[SV_G];
+    __shared__ uint32_t s_n[SV_G];
+    const uint32_t g = threadIdx.x;
+    if (g < SV_G) {
+        s_ck[g] = 0;
+        s_n[g] = 0;
+        if (g < n1 * n2) {
+            unsigned __int128 a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (uint64_t it = 0; it < items; it++) {
+                const uint64_t* r = partials + ((q.item0 + it) * SV_G + g) * 8;
+#pragma unroll
+                for (int f = 0; f < 8; f++) {
+                    if (f == 4 || f == 6 || f == 7) a[f] += r[f];
+                    else a[f] += (__int128)(int64_t)r[f];
+                }
+            }
+            if (a[6]) {
+                const unsigned __int128 ch = a[4] + (a[7] << 32) + (a[5] << 64);
+                const int64_t v[7] = {(int64_t)a[0], (int64_t)a[1], (int64_t)a[2], (int64_t)a[3],
+                                      (int64_t)(uint64_t)ch, (int64_t)(uint64_t)(ch >> 64), (int64_t)a[6]};
+                const int64_t key = (int64_t)inv[g / n2] * 256 + inv[256 + g % n2];
+                s_ck[g] = pd_ck_group(key, v);
+                s_n[g] = 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (g == 0) {
+        uint64_t ck = 0, n = 0;
+        for (int i = 0; i < SV_G; i++) { ck += s_ck[i]; n += s_n[i]; }
+        res[q.qid].count = n;
+        res[q.qid].checksum = ck;
+    }
+}
+
+__global__ void k_dim_index_build(const int32_t* __restrict__ key, uint64_t n, uint64_t* __restrict__ idx,
+                                  uint64_t mask, uint32_t* __restrict__ dup) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t want = ((i + 1) << 32) | (uint32_t)key[i];
+        uint64_t h = pd_mix64((uint32_t)key[i]) & mask;
+        while (true) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&idx[h], 0ull, (unsigned long long)want);
+            if (prev == 0) break;
+            if ((uint32_t)prev == (uint32_t)key[i]) { atomicOr(dup, 1u); break; }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+// checksum of select / join outputs of the sequential path
+__global__ void k_ck_rows(const uint32_t* __restrict__ rows, uint64_t n, uint64_t base, uint64_t* __restrict__ out) {
+    uint64_t s = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        s += pd_ck_select(base + rows[i]);
+    s = pd::wave_sum64(s);
+    if (pd::lane() == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+}
+
+__global__ void k_ck_pairs(const uint32_t* __restrict__ fact, const uint32_t* __restrict__ dim, uint64_t n,
+                           uint64_t* __restrict__ out) {
+    uint64_t s = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        s += pd_ck_join(fact[i], dim[i]);
+    s = pd::wave_sum64(s);
+    if (pd::lane() == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+}
+
+__global__ void k_iota64(uint32_t* __restrict__ out, uint64_t n, uint64_t start) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)(start + i);
+}
+
+}  // namespace pd
+
+using namespace pd;
+
+struct pimdal_server {
+    pimdal_ctx* ctx = nullptr;
+    pimdal_tables tab{};
+    SvTables T{};
+    void* dim_index = nullptr;
+    void* lut = nullptr;      // device [512] codes + [512] inverse
+    uint8_t h_inv[512];
+    uint32_t n1 = 0, n2 = 0;
+    int nslots = 0;
+    uint64_t arena_bytes = 0, status_words = 0, partial_items = 0;
+    struct Slot {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        SvBatch* h_batch = nullptr;   // pinned
+        SvBatch* d_batch = nullptr;
+        SvResult* h_res = nullptr;    // pinned
+        SvResult* d_res = nullptr;
+        void* arena = nullptr;
+        uint64_t* status = nullptr;
+        uint64_t* partials = nullptr;
+        uint32_t* ctr = nullptr;
+        hipGraphExec_t graph = nullptr;
+        std::vector<uint32_t> qids;
+        bool busy = false;
+    };
+    std::vector<Slot> slots;
+};
+
+namespace {
+
+void sv_capture(pimdal_server* S, pimdal_server::Slot& sl) {
+    pimdal_ctx* c = S->ctx;
+    const uint32_t nres = 3 * SV_MAXQ;
+    const uint32_t pgrid = (uint32_t)c->cu_count * 2;
+    hipGraph_t g;
+    PD_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
+    PD_HIP(hipMemcpyAsync(sl.d_batch, sl.h_batch, sizeof(SvBatch), hipMemcpyHostToDevice, sl.s));
+    hipLaunchKernelGGL(k_sv_reset, dim3(pgrid), dim3(256), 0, sl.s, (const SvBatch*)sl.d_batch, sl.status, sl.d_res,
+                       nres, sl.ctr);
+    hipLaunchKernelGGL(k_sv_scan<false>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
+                       sl.arena, sl.status, sl.d_res, sl.ctr);
+    hipLaunchKernelGGL(k_sv_scan<true>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
+                       sl.arena, sl.status, sl.d_res, sl.ctr);
+    hipLaunchKernelGGL(k_sv_groupby, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
+                       sl.partials, sl.ctr);
+    hipLaunchKernelGGL(k_sv_groupby_final, dim3(SV_MAXQ), dim3(64), 0, sl.s, (const SvBatch*)sl.d_batch,
+                       (const uint64_t*)sl.partials, (const uint8_t*)S->lut + 512, S->n1, S->n2, sl.d_res);
+    PD_HIP(hipMemcpyAsync(sl.h_res, sl.d_res, sizeof(SvResult) * nres, hipMemcpyDeviceToHost, sl.s));
+    PD_HIP(hipStreamEndCapture(sl.s, &g));
+    PD_HIP(hipGraphInstantiate(&sl.graph, g, nullptr, nullptr, 0));
+    PD_HIP(hipGraphDestroy(g));
+}
+
+double now_us(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct Footprint {
+    uint64_t arena_bytes, status_words, items;
+};
+
+Footprint footprint(const pimdal_query& q) {
+    const uint64_t tiles = (q.rows + SV_TILE - 1) / SV_TILE;
+    if (q.kind == PIMDAL_Q_SELECT) return {q.rows * 4, tiles, 0};
+    if (q.kind == PIMDAL_Q_JOIN) return {q.rows * 8, tiles, 0};
+    return {0, 0, (q.rows + SV_GB_CHUNK - 1) / SV_GB_CHUNK};
+}
+
+// ---- the plain, one-query-at-a-time path (drop-in operator calls) ----
+void run_plain(pimdal_server* S, const pimdal_query& q, pimdal_query_result& r) {
+    pimdal_ctx* c = S->ctx;
+    hipStream_t s = c->stream;
+    const pimdal_tables& t = S->tab;
+    if (q.kind == PIMDAL_Q_SELECT) {
+        Scratch rows(c, std::max<uint64_t>(q.rows, 1) * 4), cnt(c, 16);
+        fill_u64(c, cnt.get(), 0, 2, s);
+        select_range(c, PIMDAL_I64, t.price + q.offset, nullptr, q.rows, INT64_MIN, q.param, nullptr,
+                     rows.as<uint32_t>(), cnt.as<uint64_t>(), s);
+        const uint64_t k = read_u64(c, cnt.get(), s);
+        if (k) launch(c, "sv_ck", k_ck_rows, dim3((uint32_t)std::min<uint64_t>(ceil_div(k, 256), 1024)), dim3(256), 0, s,
+                      (const uint32_t*)rows.as<uint32_t>(), k, q.offset, cnt.as<uint64_t>() + 1);
+        r.count = k;
+        r.checksum = read_u64(c, cnt.as<uint64_t>() + 1, s);
+    } else if (q.kind == PIMDAL_Q_JOIN) {
+        // qualifying dimension rows: (key, row) with date < param
+        Scratch dk(c, t.n_dim * 4), dr(c, t.n_dim * 4), cnt(c, 16);
+        fill_u64(c, cnt.get(), 0, 2, s);
+        select_range(c, PIMDAL_I32, t.dim_date, reinterpret_cast<const uint32_t*>(t.dim_key), t.n_dim, INT64_MIN,
+                     q.param, dk.as<uint32_t>(), dr.as<uint32_t>(), cnt.as<uint64_t>(), s);
+        const uint64_t nd = read_u64(c, cnt.get(), s);
+        Scratch frow(c, std::max<uint64_t>(q.rows, 1) * 4), op(c, std::max<uint64_t>(q.rows, 1) * 4),
+            ob(c, std::max<uint64_t>(q.rows, 1) * 4);
+        launch(c, "sv_iota", k_iota64, dim3((uint32_t)std::min<uint64_t>(ceil_div(q.rows, 256), 4096)), dim3(256), 0, s,
+               frow.as<uint32_t>(), q.rows, q.offset);
+        const uint64_t k = hash_join(c, dk.as<uint32_t>(), dr.as<uint32_t>(), nd,
+                                     reinterpret_cast<const uint32_t*>(t.orderkey + q.offset), frow.as<uint32_t>(),
+                                     q.rows, op.as<uint32_t>(), ob.as<uint32_t>(), q.rows, s);
+        if (k) launch(c, "sv_ck", k_ck_pairs, dim3((uint32_t)std::min<uint64_t>(ceil_div(k, 256), 1024)), dim3(256), 0, s,
+                      (const uint32_t*)op.as<uint32_t>(), (const uint32_t*)ob.as<uint32_t>(), k, cnt.as<uint64_t>() + 1);
+        r.count = k;
+        r.checksum = read_u64(c, cnt.as<uint64_t>() + 1, s);
+    } else {
+        Scratch o(c, 64 * (4 + 8 * 6 + 16));
+        int64_t* base = o.as<int64_t>();
+        pimdal_q1_result out{reinterpret_cast<int32_t*>(base), base + 64, base + 128, base + 192, base + 256,
+                             base + 320, base + 448};
+        const uint64_t g = filter_groupby_q1(c, t.date + q.offset, t.flag1 + q.offset, t.flag2 + q.offset, nullptr,
+                                             t.quantity + q.offset, t.price + q.offset, t.discount + q.offset,
+                                             t.tax + q.offset, q.rows, (int32_t)std::max<int64_t>(
+                                                 std::min<int64_t>(q.param, INT32_MAX), INT32_MIN),
+                                             out, 64, s);
+        std::vector<int64_t> h(64 * 8);
+        PD_HIP(hipMemcpyAsync(h.data(), base, 64 * 8 * 8, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        const int32_t* keys = reinterpret_cast<const int32_t*>(h.data());
+        uint64_t ck = 0;
+        for (uint64_t i = 0; i < g; i++) {
+            const int64_t v[7] = {h[64 + i], h[128 + i], h[192 + i], h[256 + i], h[320 + 2 * i], h[320 + 2 * i + 1],
+                                  h[448 + i]};
+            ck += pd_ck_group(keys[i], v);
+        }
+        r.count = g;
+        r.checksum = ck;
+    }
+}
+
+}  // namespace
+
+extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, int slots, uint64_t slot_arena_bytes,
+                                    pimdal_server** out) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && tables && out, PIMDAL_E_INVALID, "NULL argument");
+    PD_REQUIRE(slots >= 1 && slots <= 16, PIMDAL_E_INVALID, "slots must be in [1, 16]");
+    const pimdal_tables& t = *tables;
+    PD_REQUIRE(t.orderkey && t.quantity && t.price && t.discount && t.tax && t.date && t.flag1 && t.flag2 && t.n_fact,
+               PIMDAL_E_INVALID, "missing fact column");
+    PD_REQUIRE(t.dim_key && t.dim_date && t.n_dim, PIMDAL_E_INVALID, "missing dimension column");
+    PD_REQUIRE(t.n_fact < (1ull << 32) && t.n_dim < (1ull << 31), PIMDAL_E_INVALID, "tables too large for 32-bit row ids");
+    auto* S = new pimdal_server();
+    try {
+        S->ctx = c;
+        S->tab = t;
+        hipStream_t s = c->stream;
+        // resident dimension-key index (load <= 0.5)
+        uint64_t cap = 1;
+        while (cap < 2 * t.n_dim) cap <<= 1;
+        S->dim_index = c->pool.alloc(cap * 8, s);
+        fill_u64(c, S->dim_index, 0, cap, s);
+        Scratch dup(c, 4, s);
+        fill_u32(c, dup.get(), 0, 1, s);
+        launch(c, "sv_dim_index", k_dim_index_build, dim3((uint32_t)std::min<uint64_t>(ceil_div(t.n_dim, 256), 8192)),
+               dim3(256), 0, s, t.dim_key, t.n_dim, (uint64_t*)S->dim_index, cap - 1, dup.as<uint32_t>());
+        uint32_t hd = 0;
+        PD_HIP(hipMemcpyAsync(&hd, dup.get(), 4, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        PD_REQUIRE(hd == 0, PIMDAL_E_INVALID, "dimension keys must be unique");
+        // flag domains of the fact table -> dense group codes (fixed while the table is resident)
+        S->lut = c->pool.alloc(1024, s);
+        uint32_t meta[2];
+        flag_codes(c, t.flag1, t.flag2, t.n_fact, (uint8_t*)S->lut, meta, S->h_inv, s);
+        S->n1 = meta[0];
+        S->n2 = meta[1];
+        S->T = SvTables{t.orderkey, t.quantity, t.price, t.discount, t.tax, t.date, t.flag1, t.flag2, t.n_fact,
+                        t.dim_date, (const uint64_t*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2};
+        // slots
+        S->nslots = slots;
+        S->arena_bytes = slot_arena_bytes ? slot_arena_bytes : (1ull << 30);
+        S->status_words = S->arena_bytes / 4 / SV_TILE + 4 * SV_MAXQ;
+        S->partial_items = S->arena_bytes / 4 / SV_GB_CHUNK + 2 * SV_MAXQ;
+        S->slots.resize(slots);
+        for (auto& sl : S->slots) {
+            PD_HIP(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+            PD_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+            PD_HIP(hipHostMalloc((void**)&sl.h_batch, sizeof(SvBatch), hipHostMallocDefault));
+            PD_HIP(hipHostMalloc((void**)&sl.h_res, sizeof(SvResult) * 3 * SV_MAXQ, hipHostMallocDefault));
+            memset(sl.h_batch, 0, sizeof(SvBatch));
+            sl.d_batch = (SvBatch*)c->pool.alloc(sizeof(SvBatch), s);
+            sl.d_res = (SvResult*)c->pool.alloc(sizeof(SvResult) * 3 * SV_MAXQ, s);
+            sl.arena = c->pool.alloc(S->arena_bytes, s);
+            sl.status = (uint64_t*)c->pool.alloc(S->status_words * 8, s);
+            sl.partials = (uint64_t*)c->pool.alloc(S->partial_items * SV_G * 8 * 8, s);
+            sl.ctr = (uint32_t*)c->pool.alloc(16, s);
+        }
+        PD_HIP(hipStreamSynchronize(s));
+        for (auto& sl : S->slots) sv_capture(S, sl);
+    } catch (...) {
+        pimdal_server_destroy(S);
+        throw;
+    }
+    *out = S;
+    PD_API_END
+}
+
+extern "C" int pimdal_server_destroy(pimdal_server* S) {
+    PD_API_BEGIN
+    if (!S) return PIMDAL_OK;
+    pimdal_ctx* c = S->ctx;
+    for (auto& sl : S->slots) {
+        if (sl.s) (void)hipStreamSynchronize(sl.s);
+        if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.h_batch) (void)hipHostFree(sl.h_batch);
+        if (sl.h_res) (void)hipHostFree(sl.h_res);
+        for (void* p : {(void*)sl.d_batch, (void*)sl.d_res, sl.arena, (void*)sl.status, (void*)sl.partials, (void*)sl.ctr})
+            if (p) c->pool.free(p, c->stream);
+        if (sl.s) (void)hipStreamDestroy(sl.s);
+    }
+    if (S->dim_index) c->pool.free(S->dim_index, c->stream);
+    if (S->lut) c->pool.free(S->lut, c->stream);
+    delete S;
+    PD_API_END
+}
+
+extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint64_t nq, int mode,
+                                 pimdal_query_result* res, pimdal_serve_stats* stats) {
+    PD_API_BEGIN
+    PD_REQUIRE(S && (qs || !nq) && (res || !nq), PIMDAL_E_INVALID, "NULL argument");
+    PD_REQUIRE(mode == PIMDAL_SERVE_SEQUENTIAL || mode == PIMDAL_SERVE_BATCHED, PIMDAL_E_INVALID, "unknown mode");
+    for (uint64_t i = 0; i < nq; i++) {
+        const pimdal_query& q = qs[i];
+        PD_REQUIRE(q.kind >= PIMDAL_Q_SELECT && q.kind <= PIMDAL_Q_GROUPBY, PIMDAL_E_INVALID, "unknown query kind");
+        PD_REQUIRE(q.rows >= 1 && q.offset + q.rows <= S->tab.n_fact, PIMDAL_E_INVALID, "query slice out of range");
+        PD_REQUIRE(i == 0 || qs[i - 1].arrival_us <= q.arrival_us, PIMDAL_E_INVALID, "queries must be in arrival order");
+        const Footprint f = footprint(q);
+        PD_REQUIRE(f.arena_bytes <= S->arena_bytes / 2 && f.status_words + SV_MAXQ <= S->status_words &&
+                       f.items + SV_MAXQ <= S->partial_items,
+                   PIMDAL_E_NOMEM, "query larger than a slot can hold (raise slot_arena_bytes)");
+    }
+    pimdal_ctx* c = S->ctx;
+    PD_HIP(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t batches = 0, max_inflight = 0;
+    if (mode == PIMDAL_SERVE_SEQUENTIAL) {
+        for (uint64_t i = 0; i < nq; i++) {
+            while (now_us(t0) < qs[i].arrival_us) std::this_thread::yield();
+            res[i].start_us = now_us(t0);
+            run_plain(S, qs[i], res[i]);
+            res[i].end_us = now_us(t0);
+            res[i].latency_us = res[i].end_us - qs[i].arrival_us;
+        }
+        batches = nq;
+        max_inflight = nq ? 1 : 0;
+    } else {
+        const bool dense_gb = S->n1 * S->n2 <= (uint32_t)SV_G;
+        uint64_t next = 0, done = 0, inflight = 0;
+        std::deque<uint64_t> ready;
+        while (done < nq) {
+            bool progress = false;
+            const double now = now_us(t0);
+            while (next < nq && qs[next].arrival_us <= now) ready.push_back(next++);
+            // retire finished batches
+            for (auto& sl : S->slots) {
+                if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
+                const double t_end = now_us(t0);
+                for (size_t j = 0; j < sl.qids.size(); j++) {
+                    pimdal_query_result& r = res[sl.qids[j]];
+                    r.count = sl.h_res[j].count;
+                    r.checksum = sl.h_res[j].checksum;
+                    r.end_us = t_end;
+                    r.latency_us = t_end - qs[sl.qids[j]].arrival_us;
+                }
+                done += sl.qids.size();
+                inflight -= sl.qids.size();
+                sl.qids.clear();
+                sl.busy = false;
+                progress = true;
+            }
+            // admit arrived queries into free slots
+            for (auto& sl : S->slots) {
+                if (sl.busy || ready.empty()) continue;
+                SvBatch& B = *sl.h_batch;
+                B.sel.nq = B.join.nq = B.gb.nq = 0;
+                B.sel.items = B.join.items = B.gb.items = 0;
+                uint64_t sel_bytes = 0, join_bytes = 0, status = 0, gb_items = 0;
+                const uint64_t half = S->arena_bytes / 2;  // select outputs | join outputs
+                while (!ready.empty() && sl.qids.size() < (size_t)(3 * SV_MAXQ)) {
+                    const uint64_t qi = ready.front();
+                    const pimdal_query& q = qs[qi];
+                    if (q.kind == PIMDAL_Q_GROUPBY && !dense_gb) {
+                        // more than 8 flag groups: served by the plain operator, inline
+                        ready.pop_front();
+                        res[qi].start_us = now_us(t0);
+                        run_plain(S, q, res[qi]);
+                        res[qi].end_us = now_us(t0);
+                        res[qi].latency_us = res[qi].end_us - q.arrival_us;
+                        done++;
+                        progress = true;
+                        continue;
+                    }
+                    const Footprint f = footprint(q);
+                    SvKind& K = q.kind == PIMDAL_Q_SELECT ? B.sel : q.kind == PIMDAL_Q_JOIN ? B.join : B.gb;
+                    const bool fits = K.nq < (uint32_t)SV_MAXQ && status + f.status_words <= S->status_words &&
+                                      gb_items + f.items <= S->partial_items &&
+                                      (q.kind != PIMDAL_Q_SELECT || sel_bytes + f.arena_bytes <= half) &&
+                                      (q.kind != PIMDAL_Q_JOIN || join_bytes + f.arena_bytes <= half);
+                    if (!fits) break;  // FIFO: the rest waits for the next free slot
+                    ready.pop_front();
+                    SvQuery& d = K.q[K.nq++];
+                    d.param = q.param;
+                    d.offset = q.offset;
+                    d.rows = q.rows;
+                    d.qid = (uint32_t)sl.qids.size();
+                    d.pad = 0;
+                    if (q.kind == PIMDAL_Q_GROUPBY) {
+                        d.item0 = K.items;
+                        d.out_off = 0;
+                        d.status_off = 0;
+                        K.items += f.items;
+                        gb_items += f.items;
+                    } else {
+                        d.item0 = K.items;
+                        d.status_off = status;
+                        K.items += f.status_words;
+                        status += f.status_words;
+                        if (q.kind == PIMDAL_Q_SELECT) {
+                            d.out_off = sel_bytes / 4;
+                            sel_bytes += f.arena_bytes;
+                        } else {
+                            d.out_off = (half + join_bytes) / 8;
+                            join_bytes += f.arena_bytes;
+                        }
+                    }
+                    sl.qids.push_back((uint32_t)qi);
+                    res[qi].start_us = now_us(t0);
+                }
+                if (sl.qids.empty()) continue;
+                B.status_words = status;
+                PD_HIP(hipGraphLaunch(sl.graph, sl.s));
+                PD_HIP(hipEventRecord(sl.ev, sl.s));
+                sl.busy = true;
+                batches++;
+                inflight += sl.qids.size();
+                max_inflight = std::max(max_inflight, inflight);
+                progress = true;
+            }
+            if (!progress) std::this_thread::yield();
+        }
+    }
+    if (stats) {
+        std::vector<double> lat(nq);
+        double first = nq ? qs[0].arrival_us : 0, last = 0, sum = 0;
+        for (uint64_t i = 0; i < nq; i++) {
+            lat[i] = res[i].latency_us;
+            last = std::max(last, res[i].end_us);
+            sum += lat[i];
+        }
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) { return nq ? lat[std::min<uint64_t>(nq - 1, (uint64_t)(p * (nq - 1) + 0.5))] : 0.0; };
+        stats->makespan_us = last - first;
+        stats->qps = stats->makespan_us > 0 ? nq / (stats->makespan_us * 1e-6) : 0.0;
+        stats->p50_us = pct(0.50);
+        stats->p99_us = pct(0.99);
+        stats->mean_us = nq ? sum / nq : 0.0;
+        stats->batches = batches;
+        stats->max_inflight = max_inflight;
+    }
+    PD_API_END
+}

@@ -103,3 +103,12 @@ def fact_table(ctx: Context, n: int, seed: int = SEEDS["groupby"], groups: int |
         "flag1": rng_range(ctx, np.uint8, seed, COL["FLAG1"], n, 0, 2),
         "flag2": rng_range(ctx, np.uint8, seed, COL["FLAG2"], n, 0, 1),
     }
+
+
+def dim_table(ctx: Context, n_dim: int, seed: int = SEEDS["groupby"]):
+    """Dimension table: unique int32 key (the fact orderkey values, each once),
+    int32 date on [0, 2557), uint8 category on [0, 25)."""
+    k = ctx.empty(n_dim, np.int32)
+    call("pimdal_gen_dim_keys", ctx.handle, seed, n_dim, k.vp)
+    return {"key": k, "date": rng_range(ctx, np.int32, seed, 20, n_dim, 0, 2556),
+            "category": rng_range(ctx, np.uint8, seed, 21, n_dim, 0, 24)}

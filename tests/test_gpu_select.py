@@ -57,3 +57,14 @@ def test_select_2_27_full(ctx, orc):
     ep, er = orc.select(k, p, 0, T)
     assert cnt == len(ep)
     assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
+
+
+@pytest.mark.parametrize("n", [1, 4097, 300_003])
+def test_select_int64_keys(ctx, orc, n):
+    k = orc.gen_range(np.int64, 6, 1, n, 90000, 10500000)   # the fact table's price column shape
+    p = orc.gen_u32(6, 2, n)
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    for lo, hi in ((0, 1_000_000), (-(1 << 62), 5_000_000), (10500000, 1 << 40)):
+        op, orow, cnt = ops.select(ctx, dk, dp, lo, hi)
+        ep, er = orc.select(k, p, lo, hi)
+        assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)

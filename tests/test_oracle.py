@@ -211,3 +211,11 @@ def test_q1_config_flags_six_groups(orc):
     assert list(out["key"]) == [0, 1, 256, 257, 512, 513]
     assert int(out["count"].sum()) == int(np.count_nonzero(t["date"] <= 2505))
     assert int(out["sum_qty"].sum()) == int(t["quantity"][t["date"] <= 2505].sum())
+
+
+def test_select_int64_keys(orc):
+    k = orc.gen_range(np.int64, 5, 1, 30_001, -(1 << 40), 1 << 40)
+    p = orc.gen_u32(5, 2, 30_001)
+    op, orow = orc.select(k, p, -5, 1 << 39)
+    idx = [i for i in range(k.size) if -5 <= int(k[i]) < (1 << 39)]
+    assert list(orow) == idx and np.array_equal(op, p[idx])
