@@ -97,22 +97,25 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 // so the ranking steps below need ordering against code motion, not waits.
 #define PD_COMPILER_BARRIER() __asm__ volatile("" ::: "memory")
 
-template <typename V, bool HAS_VALS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
+// 4 waves per SIMD: at most 128 VGPRs (second launch-bounds argument is the
+// minimum number of waves per execution unit).
+template <typename V, bool HAS_VALS, int BLOCK, int LBW>
+__global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
     uint32_t* __restrict__ tile_ctr) {
     constexpr int WAVES = BLOCK / PD_WAVE;
     constexpr int TILE = BLOCK * RS_ITEMS;
-    constexpr int BUF_WORDS = TILE * (HAS_VALS ? sizeof(V) / 4 : 1);
-    static_assert(BUF_WORDS * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the reorder buffer");
+    static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
     __shared__ uint32_t s_tile;
-    // the per-wave match masks are dead once ranking ends: the reorder buffer reuses them
-    __shared__ __align__(16) uint32_t s_buf[BUF_WORDS];
+    // keys reordered by digit; the per-wave match masks live here during ranking
+    __shared__ __align__(16) uint32_t s_keys[TILE];
+    __shared__ __align__(16) V s_vals[HAS_VALS ? TILE : 1];
     __shared__ uint16_t s_cnt[WAVES][RS_BINS];
-    __shared__ uint32_t s_off[RS_BINS];   // tile-local digit start, then global base minus it
+    __shared__ uint32_t s_off[RS_BINS];   // tile-local digit start
+    __shared__ uint32_t s_gbase[RS_BINS]; // global position of the digit's run minus its local start
     __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
-    auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_buf);
+    auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
@@ -126,17 +129,14 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * RS_ITEMS);
     const uint32_t tile_count = (uint32_t)min((uint64_t)TILE, n - tile_base);
 
-    // ---- load keys and payloads, wave-striped: idx = wave_base + i*64 + lane ----
+    // ---- load keys, wave-striped: idx = wave_base + i*64 + lane ----
     // Rows past n get the largest digit, so they rank after every real row
     // of the last tile and land past n (never written).
     uint32_t key[RS_ITEMS];
-    V val[HAS_VALS ? RS_ITEMS : 1];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
         const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
-        const bool ok = idx < n;
-        key[i] = ok ? kin[idx] : 0xFFFFFFFFu;
-        if constexpr (HAS_VALS) val[i] = ok ? vin[idx] : V(0);
+        key[i] = idx < n ? kin[idx] : 0xFFFFFFFFu;
     }
     auto digit_of = [&](int i) -> uint32_t {
         const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
@@ -164,9 +164,11 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     }
     __syncthreads();
 
-    // ---- per digit: exclusive prefix over waves, tile count, publish ----
-    // (digit t is handled by thread t; a 256-thread block covers all digits)
-    uint32_t cnt_d = 0, inc_d = 0;
+    // ---- per digit (thread t = digit t): prefix over waves, tile count, publish
+    // the aggregate, then resolve the global offset by look-back right away,
+    // so the inclusive prefix is published as early as possible (successor
+    // tiles stop their walks at the first inclusive prefix they see) ----
+    uint32_t cnt_d = 0, inc_d = 0, excl = 0;
     if (t < RS_BINS) {
 #pragma unroll
         for (int ww = 0; ww < WAVES; ww++) {
@@ -174,42 +176,20 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
             s_cnt[ww][t] = (uint16_t)cnt_d;
             cnt_d += c;
         }
-        pd::st_relaxed(&status[tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
-        inc_d = pd::wave_incl_scan(cnt_d);
-        if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
-    }
-    __syncthreads();
-    uint32_t local_d = 0;
-    if (t < RS_BINS) {
-        uint32_t add = 0;
-        for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
-        local_d = add + inc_d - cnt_d;
-        s_off[t] = local_d;
-    }
-    __syncthreads();
-
-    // ---- reorder the tile's keys in LDS by digit ----
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++) {
-        const uint32_t d = digit_of(i);
-        pos[i] += s_off[d] + s_cnt[w][d];
-        s_buf[pos[i]] = key[i];
-    }
-
-    // ---- global offset of each digit: decoupled look-back, 4 tiles per round trip ----
-    if (t < RS_BINS) {
-        uint32_t excl = 0;
-        if (tile > 0) {
+        if (tile == 0) {
+            pd::st_relaxed(&status[t], RS_FLAG_PRE | cnt_d);
+        } else {
+            pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_AGG | cnt_d);
             int64_t j = (int64_t)tile - 1;
             bool done = false;
             while (!done) {
-                uint32_t st[4];
+                uint32_t st[LBW];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int q = 0; q < LBW; q++)
                     st[q] = (j - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
                 bool stall = false;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < LBW; q++) {
                     if (done || stall) continue;
                     if ((st[q] >> 30) == 0) { stall = true; continue; }
                     excl += st[q] & RS_VALUE;
@@ -220,33 +200,47 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
             }
             pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt_d));
         }
-        s_off[t] = offs[t] + excl - local_d;
+        inc_d = pd::wave_incl_scan(cnt_d);
+        if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
+    }
+    __syncthreads();
+    if (t < RS_BINS) {
+        uint32_t add = 0;
+        for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
+        const uint32_t local_d = add + inc_d - cnt_d;
+        s_off[t] = local_d;
+        s_gbase[t] = offs[t] + excl - local_d;
     }
     __syncthreads();
 
-    // ---- write keys in digit runs; remember each slot's digit for the payloads ----
-    uint32_t wd[RS_ITEMS];
+    // ---- reorder the tile by digit in LDS: keys, then payloads (loaded now
+    // that their positions are known, so they never occupy registers during
+    // ranking and look-back) ----
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; k++) {
-        const uint32_t j = t + k * BLOCK;
-        const uint32_t kk = s_buf[j];
-        // slots past tile_count hold padding rows (largest digit, placed last)
-        const uint32_t d = (j < tile_count) ? ((kk >> shift) & mask) : (RS_BINS - 1);
-        wd[k] = d;
-        const uint64_t dst = (uint64_t)s_off[d] + j;
-        if (j < tile_count && dst < n) kout[dst] = kk;
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const uint32_t d = digit_of(i);
+        pos[i] += s_off[d] + s_cnt[w][d];
+        s_keys[pos[i]] = key[i];
     }
     if constexpr (HAS_VALS) {
-        __syncthreads();
-        V* s_val = reinterpret_cast<V*>(s_buf);
 #pragma unroll
-        for (int i = 0; i < RS_ITEMS; i++) s_val[pos[i]] = val[i];
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < RS_ITEMS; k++) {
-            const uint32_t j = t + k * BLOCK;
-            const uint64_t dst = (uint64_t)s_off[wd[k]] + j;
-            if (j < tile_count && dst < n) vout[dst] = s_val[j];
+        for (int i = 0; i < RS_ITEMS; i++) {
+            const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
+            s_vals[pos[i]] = idx < n ? vin[idx] : V(0);
+        }
+    }
+    __syncthreads();
+
+    // ---- write keys and payloads in digit runs ----
+#pragma unroll 4
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const uint32_t j = t + k * BLOCK;
+        if (j >= tile_count) break;   // slots past tile_count hold padding rows
+        const uint32_t kk = s_keys[j];
+        const uint64_t dst = (uint64_t)s_gbase[(kk >> shift) & mask] + j;
+        if (dst < n) {
+            kout[dst] = kk;
+            if constexpr (HAS_VALS) vout[dst] = s_vals[j];
         }
     }
 }
@@ -254,7 +248,8 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
 template <typename V, bool HV>
 static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
                        const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
-    const int block = (int)c->tune_get("sort_block", 256) == 512 ? 512 : 256;
+    const int block = (int)c->tune_get("sort_block", 512) == 256 ? 256 : 512;
+    const int lbw = (int)c->tune_get("sort_lookback", 8) >= 16 ? 16 : 8;
     const uint64_t ntiles = ceil_div(n, (uint64_t)block * RS_ITEMS);
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
@@ -267,12 +262,14 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
         V* vdst = to_out ? vout : tv.as<V>();
         fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
-        if (block == 512)
-            launch(c, "radix_onesweep", k_rs_onesweep<V, HV, 512>, dim3((uint32_t)ntiles), dim3(512), 0, s, ksrc,
-                   vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j);
-        else
-            launch(c, "radix_onesweep", k_rs_onesweep<V, HV, 256>, dim3((uint32_t)ntiles), dim3(256), 0, s, ksrc,
-                   vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j);
+#define PD_RS_LAUNCH(B, W)                                                                                       \
+    launch(c, "radix_onesweep", k_rs_onesweep<V, HV, B, W>, dim3((uint32_t)ntiles), dim3(B), 0, s, ksrc, vsrc, kdst, \
+           vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
+        if (block == 512 && lbw == 16) PD_RS_LAUNCH(512, 16);
+        else if (block == 512) PD_RS_LAUNCH(512, 8);
+        else if (lbw == 16) PD_RS_LAUNCH(256, 16);
+        else PD_RS_LAUNCH(256, 8);
+#undef PD_RS_LAUNCH
         ksrc = kdst;
         vsrc = vdst;
     }
