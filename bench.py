@@ -106,6 +106,67 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
+KERNELS = ("part_hist", "part_scatter", "scan", "hj_chunks", "hj_join", "hj_compact", "radix_hist", "radix_scan",
+           "radix_onesweep", "smj_dupcheck", "smj_merge", "q1_domain", "q1_lut", "q1_dense", "q1_final", "gb_direct",
+           "gb_hist", "gb_scatter", "gb_lds", "gb_global", "q1_unpack", "fill_u32", "fill_u64")
+
+
+def kernel_breakdown(ctx, steps):
+    out = {}
+    for k in KERNELS:
+        n, ms = ctx.kernel_stat(k)
+        if n:
+            out[k] = {"launches_per_step": n / steps, "ms_per_step": ms / steps}
+    return out
+
+
+def bench_join(ctx, steps, warmup, which, zipf, nr=1 << 24, ns=1 << 27):
+    from pimdal_amd import operators as ops, datagen
+    rk, rp, sk, sp = datagen.join_workload(ctx, nr, ns, zipf=zipf)
+    op = ctx.empty(ns, "uint32")
+    ob = ctx.empty(ns, "uint32")
+    fn = ops.hash_join if which == "hash" else ops.sort_merge_join
+    import ctypes as C
+    from pimdal_amd import _lib
+    name = "pimdal_hash_join_dev" if which == "hash" else "pimdal_sort_merge_join_dev"
+    cnt = C.c_uint64()
+
+    def step():
+        _lib.check(getattr(_lib.load(), name)(ctx.handle, rk.vp, rp.vp, nr, sk.vp, sp.vp, ns, op.vp, ob.vp, ns,
+                                              C.byref(cnt)))
+
+    t = timed_steps(ctx, step, steps, warmup)
+    alg = (24 * nr + 32 * ns) if which == "hash" else (60 * nr + 68 * ns)
+    res = {"build_rows": nr, "probe_rows": ns, "pairs": cnt.value, "s_per_step": t, "rows_per_s": (nr + ns) / t,
+           "alg_bytes": alg, "alg_GBps": alg / t / 1e9, "hbm_frac": alg / t / HBM_PEAK,
+           "kernels": kernel_breakdown(ctx, steps)}
+    for a in (rk, rp, sk, sp, op, ob):
+        a.free()
+    return res
+
+
+def bench_q1(ctx, steps, warmup, by_key, n=60_000_000):
+    from pimdal_amd import operators as ops, datagen
+    t = datagen.fact_table(ctx, n, groups=15_000_000)
+    out = {}
+
+    def step():
+        out["r"] = ops.filter_groupby_q1(ctx, t, 2505, by_key=by_key, to_host=False)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    groups = out["r"][1]
+    nsel = None
+    if by_key:
+        alg = 42 * n + 88 * int(0.98 * n) + 52 * groups
+    else:
+        alg = 38 * n
+    res = {"rows": n, "groups": groups, "s_per_step": tm, "rows_per_s": n / tm, "alg_bytes": alg,
+           "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
+    for a in t.values():
+        a.free()
+    return res
+
+
 def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, rate_qps=1e6, slots=4):
     """Config 5: mixed query stream over resident tables, batched vs one at a time."""
     import numpy as np
@@ -181,8 +242,18 @@ def main():
     if args.only in ("", "all"):
         log("bench: sort 2^28 zipf")
         ops_out["sort_zipf_2^28"] = bench_sort(ctx, max(3, args.steps // 2), 1, zipf=True)
+    if args.only in ("", "all", "sortselect"):
         log("bench: select 2^27")
         ops_out["select_2^27"] = bench_select(ctx, args.steps, args.warmup)
+    if args.only in ("", "all", "join"):
+        for which in ("hash", "smj"):
+            for z in (False, True):
+                log(f"bench: {which} join 2^24 x 2^27 zipf={z}")
+                ops_out[f"{which}_join{'_zipf' if z else ''}"] = bench_join(ctx, max(3, args.steps // 2), 1, which, z)
+    if args.only in ("", "all", "groupby"):
+        for by_key in (False, True):
+            log(f"bench: filter+group-by 6e7 by_key={by_key}")
+            ops_out["q1_orderkey" if by_key else "q1_flags"] = bench_q1(ctx, max(3, args.steps // 2), 1, by_key)
     if args.only in ("", "all", "serve"):
         log("bench: serving mixed stream")
         ops_out["serve_config5"] = bench_serve(ctx)

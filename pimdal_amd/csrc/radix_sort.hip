@@ -98,14 +98,16 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 #define PD_COMPILER_BARRIER() __asm__ volatile("" ::: "memory")
 
 // 4 waves per SIMD: at most 128 VGPRs (second launch-bounds argument is the
-// minimum number of waves per execution unit).
-template <typename V, bool HAS_VALS, int BLOCK, int LBW>
+// minimum number of waves per execution unit).  EARLY: resolve the global
+// offsets (look-back) before the LDS reorder instead of after it.
+template <typename V, bool HAS_VALS, int BLOCK, bool EARLY>
 __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
     uint32_t* __restrict__ tile_ctr) {
     constexpr int WAVES = BLOCK / PD_WAVE;
     constexpr int TILE = BLOCK * RS_ITEMS;
+    constexpr int LBW = 4;
     static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
     __shared__ uint32_t s_tile;
     // keys reordered by digit; the per-wave match masks live here during ranking
@@ -129,14 +131,17 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * RS_ITEMS);
     const uint32_t tile_count = (uint32_t)min((uint64_t)TILE, n - tile_base);
 
-    // ---- load keys, wave-striped: idx = wave_base + i*64 + lane ----
+    // ---- load keys and payloads, wave-striped: idx = wave_base + i*64 + lane ----
     // Rows past n get the largest digit, so they rank after every real row
     // of the last tile and land past n (never written).
     uint32_t key[RS_ITEMS];
+    V val[HAS_VALS ? RS_ITEMS : 1];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
         const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
-        key[i] = idx < n ? kin[idx] : 0xFFFFFFFFu;
+        const bool ok = idx < n;
+        key[i] = ok ? kin[idx] : 0xFFFFFFFFu;
+        if constexpr (HAS_VALS) val[i] = ok ? vin[idx] : V(0);
     }
     auto digit_of = [&](int i) -> uint32_t {
         const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
@@ -164,10 +169,33 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     }
     __syncthreads();
 
-    // ---- per digit (thread t = digit t): prefix over waves, tile count, publish
-    // the aggregate, then resolve the global offset by look-back right away,
-    // so the inclusive prefix is published as early as possible (successor
-    // tiles stop their walks at the first inclusive prefix they see) ----
+    // per-digit look-back (thread t = digit t): exclusive prefix of this
+    // digit over all earlier tiles, LBW statuses per round trip
+    auto lookback = [&](uint32_t cnt) -> uint32_t {
+        uint32_t excl = 0;
+        int64_t j = (int64_t)tile - 1;
+        bool done = false;
+        while (!done) {
+            uint32_t st[LBW];
+#pragma unroll
+            for (int q = 0; q < LBW; q++)
+                st[q] = (j - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
+            bool stall = false;
+#pragma unroll
+            for (int q = 0; q < LBW; q++) {
+                if (done || stall) continue;
+                if ((st[q] >> 30) == 0) { stall = true; continue; }
+                excl += st[q] & RS_VALUE;
+                if ((st[q] >> 30) == 2) done = true;
+                else j--;
+            }
+            if (stall) __builtin_amdgcn_s_sleep(1);
+        }
+        pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt));
+        return excl;
+    };
+
+    // ---- per digit: prefix over waves, tile count, publish the aggregate ----
     uint32_t cnt_d = 0, inc_d = 0, excl = 0;
     if (t < RS_BINS) {
 #pragma unroll
@@ -176,58 +204,33 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
             s_cnt[ww][t] = (uint16_t)cnt_d;
             cnt_d += c;
         }
-        if (tile == 0) {
-            pd::st_relaxed(&status[t], RS_FLAG_PRE | cnt_d);
-        } else {
-            pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_AGG | cnt_d);
-            int64_t j = (int64_t)tile - 1;
-            bool done = false;
-            while (!done) {
-                uint32_t st[LBW];
-#pragma unroll
-                for (int q = 0; q < LBW; q++)
-                    st[q] = (j - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
-                bool stall = false;
-#pragma unroll
-                for (int q = 0; q < LBW; q++) {
-                    if (done || stall) continue;
-                    if ((st[q] >> 30) == 0) { stall = true; continue; }
-                    excl += st[q] & RS_VALUE;
-                    if ((st[q] >> 30) == 2) done = true;
-                    else j--;
-                }
-                if (stall) __builtin_amdgcn_s_sleep(1);
-            }
-            pd::st_relaxed(&status[tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt_d));
-        }
+        pd::st_relaxed(&status[tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
+        if (EARLY && tile > 0) excl = lookback(cnt_d);
         inc_d = pd::wave_incl_scan(cnt_d);
         if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
     }
     __syncthreads();
+    uint32_t local_d = 0;
     if (t < RS_BINS) {
         uint32_t add = 0;
         for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
-        const uint32_t local_d = add + inc_d - cnt_d;
+        local_d = add + inc_d - cnt_d;
         s_off[t] = local_d;
-        s_gbase[t] = offs[t] + excl - local_d;
+        if (EARLY) s_gbase[t] = offs[t] + excl - local_d;
     }
     __syncthreads();
 
-    // ---- reorder the tile by digit in LDS: keys, then payloads (loaded now
-    // that their positions are known, so they never occupy registers during
-    // ranking and look-back) ----
+    // ---- reorder the tile by digit in LDS (keys and payloads) ----
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
         const uint32_t d = digit_of(i);
-        pos[i] += s_off[d] + s_cnt[w][d];
-        s_keys[pos[i]] = key[i];
+        const uint32_t p = pos[i] + s_off[d] + s_cnt[w][d];
+        s_keys[p] = key[i];
+        if constexpr (HAS_VALS) s_vals[p] = val[i];
     }
-    if constexpr (HAS_VALS) {
-#pragma unroll
-        for (int i = 0; i < RS_ITEMS; i++) {
-            const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
-            s_vals[pos[i]] = idx < n ? vin[idx] : V(0);
-        }
+    if (!EARLY && t < RS_BINS) {
+        if (tile > 0) excl = lookback(cnt_d);
+        s_gbase[t] = offs[t] + excl - local_d;
     }
     __syncthreads();
 
@@ -249,7 +252,7 @@ template <typename V, bool HV>
 static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
                        const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
     const int block = (int)c->tune_get("sort_block", 512) == 256 ? 256 : 512;
-    const int lbw = (int)c->tune_get("sort_lookback", 8) >= 16 ? 16 : 8;
+    const bool early = c->tune_get("sort_early_lookback", 0) != 0;
     const uint64_t ntiles = ceil_div(n, (uint64_t)block * RS_ITEMS);
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
@@ -262,13 +265,13 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
         V* vdst = to_out ? vout : tv.as<V>();
         fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
-#define PD_RS_LAUNCH(B, W)                                                                                       \
-    launch(c, "radix_onesweep", k_rs_onesweep<V, HV, B, W>, dim3((uint32_t)ntiles), dim3(B), 0, s, ksrc, vsrc, kdst, \
+#define PD_RS_LAUNCH(B, E)                                                                                       \
+    launch(c, "radix_onesweep", k_rs_onesweep<V, HV, B, E>, dim3((uint32_t)ntiles), dim3(B), 0, s, ksrc, vsrc, kdst, \
            vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
-        if (block == 512 && lbw == 16) PD_RS_LAUNCH(512, 16);
-        else if (block == 512) PD_RS_LAUNCH(512, 8);
-        else if (lbw == 16) PD_RS_LAUNCH(256, 16);
-        else PD_RS_LAUNCH(256, 8);
+        if (block == 512 && early) PD_RS_LAUNCH(512, true);
+        else if (block == 512) PD_RS_LAUNCH(512, false);
+        else if (early) PD_RS_LAUNCH(256, true);
+        else PD_RS_LAUNCH(256, false);
 #undef PD_RS_LAUNCH
         ksrc = kdst;
         vsrc = vdst;

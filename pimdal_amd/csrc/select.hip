@@ -9,6 +9,8 @@
 // (tile ids come from an atomic counter, so every predecessor tile is
 // already resident and the wait always ends), and writes the kept (payload,
 // row id) pairs from registers into their contiguous output range.
+#include <algorithm>
+
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
 #include "pd_ops.hpp"
@@ -31,9 +33,13 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select(const K* __restrict__ key,
     __shared__ uint64_t s_excl;
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    // persistent: each block takes tiles in counter order until none are left
+    // (a tile's predecessors were taken earlier, by running blocks)
+    while (true) {
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
     __syncthreads();
     const uint64_t tile = s_tile;
+    if (tile >= ntiles) break;
     const uint64_t tile_base = tile * SEL_TILE;
     const uint64_t wave_base = tile_base + (uint64_t)w * SEL_WAVE_ROWS;
 
@@ -128,6 +134,8 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select(const K* __restrict__ key,
             }
         }
     }
+    __syncthreads();  // shared tile id / totals are rewritten by the next iteration
+    }
 }
 
 void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
@@ -150,7 +158,8 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
     uint32_t* ctr = reinterpret_cast<uint32_t*>(status + ntiles);
     fill_u64(c, status, 0, ntiles + 1, s);
     const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)pay % 16 == 0);
-    dim3 grid((uint32_t)ntiles), block(SEL_BLOCK);
+    dim3 grid((uint32_t)std::min<uint64_t>(ntiles, (uint64_t)c->cu_count * (uint64_t)c->tune_get("select_blocks_per_cu", 8))),
+        block(SEL_BLOCK);
     if (kt == PIMDAL_U32) {
         auto* kp = reinterpret_cast<const uint32_t*>(key);
         if (vec) launch(c, "select", k_select<uint32_t, true>, grid, block, 0, s, kp, pay, n, lo, hi, out_pay, out_rid, status, ctr, d_count, ntiles);
