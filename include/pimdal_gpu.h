@@ -75,6 +75,13 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
 /* Tuning knobs (radix bits, fan-out, tile shapes); names in DESIGN.md. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
+/* Autotuner: times candidate knob values for `op` ("sort", "select",
+ * "hash_join", "groupby") on synthetic device data of n rows, keeps the
+ * fastest in the context, and appends them to cache_path (may be NULL).
+ * No counterpart in the reference (its sizes and fan-outs are compile-time
+ * CMake definitions, pimdal/main/CMakeLists.txt:11-19). */
+int pimdal_autotune(pimdal_ctx* ctx, const char* op, uint64_t n, const char* cache_path);
+int pimdal_ctx_load_tuning(pimdal_ctx* ctx, const char* cache_path);
 
 /* ---------------- device memory ---------------- */
 int pimdal_dev_alloc(pimdal_ctx* ctx, uint64_t bytes, void** dptr);

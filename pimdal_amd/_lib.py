@@ -52,6 +52,8 @@ SIGNATURES: dict[str, list] = {
     "pimdal_ctx_reset_stats": [c_ctx_p],
     "pimdal_ctx_set_tune": [c_ctx_p, C.c_char_p, i64],
     "pimdal_ctx_get_tune": [c_ctx_p, C.c_char_p, C.POINTER(i64)],
+    "pimdal_autotune": [c_ctx_p, C.c_char_p, u64, C.c_char_p],
+    "pimdal_ctx_load_tuning": [c_ctx_p, C.c_char_p],
     "pimdal_dev_alloc": [c_ctx_p, u64, C.POINTER(vp)],
     "pimdal_dev_free": [c_ctx_p, vp],
     "pimdal_dev_copy": [c_ctx_p, vp, vp, u64],

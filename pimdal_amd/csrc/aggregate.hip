@@ -81,7 +81,7 @@ struct Q1Acc {
     unsigned __int128 ch;
 };
 
-template <int G>
+template <int G, bool VEC>
 __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict__ date, const uint8_t* __restrict__ f1,
                                                        const uint8_t* __restrict__ f2, const int64_t* __restrict__ qty,
                                                        const int64_t* __restrict__ price,
@@ -95,15 +95,13 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
     for (uint32_t i = threadIdx.x; i < 512; i += QD_BLOCK) s_lut[i] = lut[i];
     __syncthreads();
     const uint32_t n2 = meta[1];
-    if (meta[0] * n2 > (uint32_t)G) return;  // another instantiation handles it
+    if (meta[0] * n2 > (uint32_t)G) return;  // the general path handles it
     Q1Acc acc[G];
 #pragma unroll
     for (int g = 0; g < G; g++) acc[g] = Q1Acc{0, 0, 0, 0, 0, 0};
-    const uint64_t stride = (uint64_t)gridDim.x * QD_BLOCK;
-    for (uint64_t i = blockIdx.x * (uint64_t)QD_BLOCK + threadIdx.x; i < n; i += stride) {
-        if (date[i] > cutoff) continue;
-        const uint32_t code = (uint32_t)s_lut[f1[i]] * n2 + s_lut[256 + f2[i]];
-        const int64_t q = qty[i], p = price[i], d = disc[i], x = tax[i];
+    auto add_row = [&](int32_t dt, uint32_t a, uint32_t b, int64_t q, int64_t p, int64_t d, int64_t x) {
+        if (dt > cutoff) return;
+        const uint32_t code = (uint32_t)s_lut[a] * n2 + s_lut[256 + b];
         const int64_t dp = p * (100 - d);
         const __int128 ch = (__int128)dp * (100 + x);
 #pragma unroll
@@ -116,21 +114,40 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
             acc[g].ch += on ? (unsigned __int128)ch : 0;
             acc[g].cnt += on ? 1 : 0;
         }
+    };
+    const uint64_t stride = (uint64_t)gridDim.x * QD_BLOCK;
+    if (VEC) {
+        // four consecutive rows per thread per step, 16-byte loads
+        const uint64_t nq = n / 4;
+        for (uint64_t v = blockIdx.x * (uint64_t)QD_BLOCK + threadIdx.x; v < nq; v += stride) {
+            const int4 dt = reinterpret_cast<const int4*>(date)[v];
+            const uint32_t fa = reinterpret_cast<const uint32_t*>(f1)[v];
+            const uint32_t fb = reinterpret_cast<const uint32_t*>(f2)[v];
+            const longlong2 q0 = reinterpret_cast<const longlong2*>(qty)[2 * v], q1 = reinterpret_cast<const longlong2*>(qty)[2 * v + 1];
+            const longlong2 p0 = reinterpret_cast<const longlong2*>(price)[2 * v], p1 = reinterpret_cast<const longlong2*>(price)[2 * v + 1];
+            const longlong2 d0 = reinterpret_cast<const longlong2*>(disc)[2 * v], d1 = reinterpret_cast<const longlong2*>(disc)[2 * v + 1];
+            const longlong2 x0 = reinterpret_cast<const longlong2*>(tax)[2 * v], x1 = reinterpret_cast<const longlong2*>(tax)[2 * v + 1];
+            add_row(dt.x, fa & 0xFF, fb & 0xFF, q0.x, p0.x, d0.x, x0.x);
+            add_row(dt.y, (fa >> 8) & 0xFF, (fb >> 8) & 0xFF, q0.y, p0.y, d0.y, x0.y);
+            add_row(dt.z, (fa >> 16) & 0xFF, (fb >> 16) & 0xFF, q1.x, p1.x, d1.x, x1.x);
+            add_row(dt.w, fa >> 24, fb >> 24, q1.y, p1.y, d1.y, x1.y);
+        }
+        for (uint64_t i = nq * 4 + blockIdx.x * (uint64_t)QD_BLOCK + threadIdx.x; i < n; i += stride)
+            add_row(date[i], f1[i], f2[i], qty[i], price[i], disc[i], tax[i]);
+    } else {
+        for (uint64_t i = blockIdx.x * (uint64_t)QD_BLOCK + threadIdx.x; i < n; i += stride)
+            add_row(date[i], f1[i], f2[i], qty[i], price[i], disc[i], tax[i]);
     }
-    // block reduction: wave sums (64-bit lanes; the 128-bit charge as two
-    // words with the carry folded in through a 64-bit high-part sum)
+    // block reduction: wave sums of 64-bit words; the low charge word is
+    // split into 32-bit halves so the 64-bit sums cannot drop carries
     const uint32_t w = threadIdx.x / PD_WAVE, l = pd::lane();
 #pragma unroll
     for (int g = 0; g < G; g++) {
         uint64_t v[8] = {(uint64_t)acc[g].q, (uint64_t)acc[g].p, (uint64_t)acc[g].d, (uint64_t)acc[g].dp,
-                         (uint64_t)acc[g].ch, (uint64_t)(acc[g].ch >> 64), (uint64_t)acc[g].cnt, 0};
-        // split the low charge word into 32-bit halves so a 64-bit wave sum cannot lose carries
-        const uint64_t lo32 = v[4] & 0xFFFFFFFFull, hi32 = v[4] >> 32;
-        v[4] = pd::wave_sum64(lo32);
-        v[7] = pd::wave_sum64(hi32);
+                         (uint64_t)acc[g].ch & 0xFFFFFFFFull, (uint64_t)(acc[g].ch >> 64), (uint64_t)acc[g].cnt,
+                         ((uint64_t)acc[g].ch) >> 32};
 #pragma unroll
-        for (int f = 0; f < 8; f++)
-            if (f != 4 && f != 7) v[f] = pd::wave_sum64(v[f]);
+        for (int f = 0; f < 8; f++) v[f] = pd::wave_sum64(v[f]);
         if (l == 0)
 #pragma unroll
             for (int f = 0; f < 8; f++) s_red[w][g][f] = v[f];
@@ -144,51 +161,56 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
     }
 }
 
-// partial layout per (block, group): q, p, d, dp, sum(lo32 of charge lo), sum(charge hi),
-// cnt, sum(hi32 of charge lo)
-__global__ void k_q1_final(const uint64_t* __restrict__ partials, uint32_t nblocks, uint32_t G,
-                           const uint32_t* __restrict__ meta, const uint8_t* __restrict__ inv,
-                           int32_t* __restrict__ okey, int64_t* __restrict__ sq, int64_t* __restrict__ sp,
-                           int64_t* __restrict__ sd, int64_t* __restrict__ sdp, int64_t* __restrict__ sch,
-                           int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups) {
-    __shared__ int32_t s_keys[64];
-    __shared__ unsigned __int128 s_v[64][7];
-    const uint32_t g = threadIdx.x;
-    const uint32_t n1 = meta[0], n2 = meta[1];
-    if (g < G && g < n1 * n2) {
-        unsigned __int128 q = 0, p = 0, d = 0, dp = 0, chl = 0, chh = 0, c = 0, chm = 0;
-        for (uint32_t b = 0; b < nblocks; b++) {
-            const uint64_t* r = partials + ((uint64_t)b * G + g) * 8;
-            q += (__int128)(int64_t)r[0];
-            p += (__int128)(int64_t)r[1];
-            d += (__int128)(int64_t)r[2];
-            dp += (__int128)(int64_t)r[3];
-            chl += r[4];
-            chh += (__int128)(int64_t)r[5];
-            c += r[6];
-            chm += r[7];
+// partial layout per (block, group): q, p, d, dp, sum(low 32 bits of the charge's
+// low word), sum(charge high word), count, sum(high 32 bits of the charge's low word)
+// One block per group code: 128-bit sums of the block partials.
+__global__ __launch_bounds__(256) void k_q1_reduce(const uint64_t* __restrict__ partials, uint32_t nblocks, uint32_t G,
+                                                   unsigned __int128* __restrict__ sums /*[G][6]*/) {
+    __shared__ unsigned __int128 s_v[256][6];
+    const uint32_t g = blockIdx.x, t = threadIdx.x;
+    unsigned __int128 a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = t; b < nblocks; b += 256) {
+        const uint64_t* r = partials + ((uint64_t)b * G + g) * 8;
+#pragma unroll
+        for (int f = 0; f < 8; f++) {
+            if (f == 4 || f == 6 || f == 7) a[f] += r[f];
+            else a[f] += (unsigned __int128)(__int128)(int64_t)r[f];
         }
-        const unsigned __int128 ch = chl + (chm << 32) + (chh << 64);
-        s_keys[g] = (int32_t)inv[g / n2] * 256 + (int32_t)inv[256 + g % n2];
-        s_v[g][0] = q; s_v[g][1] = p; s_v[g][2] = d; s_v[g][3] = dp; s_v[g][4] = ch; s_v[g][5] = c;
     }
+    s_v[t][0] = a[0]; s_v[t][1] = a[1]; s_v[t][2] = a[2]; s_v[t][3] = a[3];
+    s_v[t][4] = a[4] + (a[7] << 32) + (a[5] << 64);
+    s_v[t][5] = a[6];
     __syncthreads();
-    if (g == 0) {
-        uint64_t k = 0;
-        for (uint32_t i = 0; i < G && i < n1 * n2; i++) {
-            if (s_v[i][5] == 0) continue;
-            okey[k] = s_keys[i];
-            sq[k] = (int64_t)s_v[i][0];
-            sp[k] = (int64_t)s_v[i][1];
-            sd[k] = (int64_t)s_v[i][2];
-            sdp[k] = (int64_t)s_v[i][3];
-            sch[2 * k] = (int64_t)(uint64_t)s_v[i][4];
-            sch[2 * k + 1] = (int64_t)(uint64_t)(s_v[i][4] >> 64);
-            cnt[k] = (int64_t)s_v[i][5];
-            k++;
-        }
-        *ngroups = k;
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int f = 0; f < 6; f++) s_v[t][f] += s_v[t + w][f];
+        __syncthreads();
     }
+    if (t < 6) sums[g * 6 + t] = s_v[0][t];
+}
+
+__global__ void k_q1_final(const unsigned __int128* __restrict__ sums, uint32_t G, const uint32_t* __restrict__ meta,
+                           const uint8_t* __restrict__ inv, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
+                           int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
+                           int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups) {
+    if (threadIdx.x != 0) return;
+    const uint32_t n1 = meta[0], n2 = meta[1];
+    uint64_t k = 0;
+    for (uint32_t g = 0; g < G && g < n1 * n2; g++) {
+        const unsigned __int128* v = sums + g * 6;
+        if (v[5] == 0) continue;
+        okey[k] = (int32_t)inv[g / n2] * 256 + (int32_t)inv[256 + g % n2];
+        sq[k] = (int64_t)v[0];
+        sp[k] = (int64_t)v[1];
+        sd[k] = (int64_t)v[2];
+        sdp[k] = (int64_t)v[3];
+        sch[2 * k] = (int64_t)(uint64_t)v[4];
+        sch[2 * k + 1] = (int64_t)(uint64_t)(v[4] >> 64);
+        cnt[k] = (int64_t)v[5];
+        k++;
+    }
+    *ngroups = k;
 }
 
 // ===================== general hash group-by =====================
@@ -708,11 +730,19 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         const uint32_t groups = nm[0] * nm[1];
         if (groups <= 8) {
             const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, QD_BLOCK * 16), (uint64_t)c->cu_count * 8));
-            Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s);
-            launch(c, "q1_dense", k_q1_dense<8>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc, tax, n,
-                   cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
-            launch(c, "q1_final", k_q1_final, dim3(1), dim3(64), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, 8u,
-                   (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price, out.sum_disc,
+            Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s), sums(c, 8 * 6 * 16, s);
+            const bool vec = ((uintptr_t)date | (uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)qty | (uintptr_t)price |
+                              (uintptr_t)disc | (uintptr_t)tax) % 16 == 0;
+            if (vec)
+                launch(c, "q1_dense", k_q1_dense<8, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc,
+                       tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
+            else
+                launch(c, "q1_dense", k_q1_dense<8, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price,
+                       disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
+            launch(c, "q1_reduce", k_q1_reduce, dim3(8), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, 8u,
+                   sums.as<unsigned __int128>());
+            launch(c, "q1_final", k_q1_final, dim3(1), dim3(64), 0, s, (const unsigned __int128*)sums.as<unsigned __int128>(),
+                   8u, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price, out.sum_disc,
                    out.sum_disc_price, out.sum_charge, out.count, ng);
             const uint64_t k = read_u64(c, ng, s);
             PD_REQUIRE(k <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");

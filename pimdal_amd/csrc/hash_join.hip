@@ -275,6 +275,10 @@ __global__ __launch_bounds__(256) void k_hj_compact(const uint32_t* __restrict__
     }
 }
 
+uint64_t hash_join_global(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
+                          const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap,
+                          hipStream_t s);
+
 static uint32_t next_pow2(uint64_t x) {
     uint32_t p = 1;
     while (p < x) p <<= 1;
@@ -288,6 +292,8 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
     PD_REQUIRE(nb < (1ull << 32) && np < (1ull << 32), PIMDAL_E_INVALID, "join inputs must have < 2^32 rows");
     PD_REQUIRE((bk || !nb) && (pk || !np), PIMDAL_E_INVALID, "key pointer is NULL");
     if (nb == 0 || np == 0) return 0;
+    if (c->tune_get("hj_mode", 1) == 1)
+        return hash_join_global(c, bk, bp, nb, pk, pp, np, out_p, out_b, cap, s);
     const int64_t tuned = c->tune_get("hj_fanout", 0);
     uint32_t F = tuned > 0 ? next_pow2((uint64_t)tuned)
                            : next_pow2(std::max<uint64_t>(1, ceil_div(nb, (uint64_t)c->tune_get("hj_rows_per_part", 4096))));
@@ -390,3 +396,213 @@ extern "C" int pimdal_hash_join_dev(pimdal_ctx* c, const uint32_t* build_key, co
     PD_REQUIRE(k <= out_capacity, PIMDAL_E_CAPACITY, "join output larger than out_capacity (count returned)");
     PD_API_END
 }
+
+// ===================== non-partitioned mode (global table) =====================
+// The build side goes into one open-addressing table in HBM (slot = key |
+// payload << 32, occupancy in a separate bitmap so every key and payload
+// value is legal); the probe side streams through it once.  At the
+// benchmark size the table (2^25 slots, 256 MB) and bitmap (4 MB) are what
+// the Infinity Cache and the L2s hold, so probes mostly avoid HBM.  Output
+// is compacted per 4096-row probe tile with a decoupled look-back, so it
+// comes out in probe order.
+
+namespace pd {
+
+constexpr int HG_BLOCK = 256;
+constexpr int HG_ITEMS = 16;
+constexpr int HG_TILE = HG_BLOCK * HG_ITEMS;
+
+__device__ __forceinline__ uint64_t hg_slot(uint32_t key, uint64_t mask) { return pd_mix64_dev(key) & mask; }
+
+__global__ void k_hg_build(const uint32_t* __restrict__ bk, const uint32_t* __restrict__ bp, uint64_t nb,
+                           uint64_t* __restrict__ tab, uint32_t* __restrict__ occ, uint64_t mask) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = bk[i];
+        const uint32_t p = bp ? bp[i] : k;
+        uint64_t h = hg_slot(k, mask);
+        while (true) {
+            const uint32_t bit = 1u << (h & 31);
+            if (!(atomicOr(&occ[h >> 5], bit) & bit)) {
+                tab[h] = (uint64_t)k | ((uint64_t)p << 32);
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+// MODE 0: emit the first match of each probe row (and flag rows with more
+// than one match); MODE 1: count matches per tile; MODE 2: emit every match
+// at the tile's offset (offsets from a scan of MODE 1 counts).
+template <int MODE>
+__global__ __launch_bounds__(HG_BLOCK) void k_hg_probe(const uint32_t* __restrict__ pk, const uint32_t* __restrict__ pp,
+                                                       uint64_t np, const uint64_t* __restrict__ tab,
+                                                       const uint32_t* __restrict__ occ, uint64_t mask,
+                                                       uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
+                                                       uint64_t cap, uint64_t* __restrict__ status,
+                                                       uint32_t* __restrict__ tile_ctr, uint64_t ntiles,
+                                                       uint32_t* __restrict__ tile_cnt,
+                                                       const uint32_t* __restrict__ tile_off,
+                                                       uint32_t* __restrict__ flags) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_cnt[HG_ITEMS][HG_BLOCK / PD_WAVE];
+    __shared__ uint32_t s_total;
+    __shared__ uint64_t s_excl;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    while (true) {
+        if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        if (tile >= ntiles) break;
+        const uint64_t base = tile * HG_TILE;
+        // probe rows base + i*BLOCK + t (row order = (i, wave, lane))
+        uint32_t key[HG_ITEMS], m[HG_ITEMS], first[HG_ITEMS];
+        bool dup = false;
+#pragma unroll
+        for (int i = 0; i < HG_ITEMS; i++) {
+            const uint64_t j = base + (uint64_t)i * HG_BLOCK + t;
+            m[i] = 0;
+            first[i] = 0;
+            key[i] = 0;
+            if (j < np) {
+                const uint32_t k = pk[j];
+                key[i] = k;
+                uint64_t h = hg_slot(k, mask);
+                while (occ[h >> 5] & (1u << (h & 31))) {
+                    const uint64_t e = tab[h];
+                    if ((uint32_t)e == k) {
+                        if (!m[i]) first[i] = (uint32_t)(e >> 32);
+                        m[i]++;
+                        if (MODE == 0 && m[i] > 1) break;
+                    }
+                    h = (h + 1) & mask;
+                }
+                if (MODE == 0) { dup |= m[i] > 1; m[i] = m[i] ? 1 : 0; }
+            }
+        }
+        if (MODE == 0 && dup) atomicOr(&flags[0], 1u);
+        if (MODE == 1) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int i = 0; i < HG_ITEMS; i++) c += m[i];
+            c = pd::wave_sum(c);
+            if (l == 0) s_cnt[0][w] = c;
+            __syncthreads();
+            if (t == 0) {
+                uint32_t tot = 0;
+                for (int ww = 0; ww < HG_BLOCK / PD_WAVE; ww++) tot += s_cnt[0][ww];
+                tile_cnt[tile] = tot;
+            }
+            __syncthreads();
+            continue;
+        }
+        // per (item, wave) counts -> exclusive scan in (item, wave, lane) order
+#pragma unroll
+        for (int i = 0; i < HG_ITEMS; i++) {
+            const uint32_t c = pd::wave_sum(m[i]);
+            if (l == 0) s_cnt[i][w] = c;
+        }
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t v = (&s_cnt[0][0])[l];
+            const uint32_t inc = pd::wave_incl_scan(v);
+            (&s_cnt[0][0])[l] = inc - v;
+            if (l == PD_WAVE - 1) s_total = inc;
+        }
+        __syncthreads();
+        if (MODE == 0) {
+            if (w == 0) {
+                const uint64_t e = pd::lookback_single(status, tile, s_total);
+                if (l == 0) {
+                    s_excl = e;
+                    if (tile == ntiles - 1) atomicAdd(reinterpret_cast<unsigned long long*>(flags + 2), (unsigned long long)(e + s_total));
+                }
+            }
+        } else if (t == 0) {
+            s_excl = tile_off[tile];
+        }
+        __syncthreads();
+        const uint64_t excl = s_excl;
+        // inclusive-within-lane prefix of m over lanes below, per item
+#pragma unroll
+        for (int i = 0; i < HG_ITEMS; i++) {
+            const uint32_t below = pd::wave_incl_scan(m[i]) - m[i];
+            if (!m[i]) continue;
+            uint64_t o = excl + s_cnt[i][w] + below;
+            const uint64_t j = base + (uint64_t)i * HG_BLOCK + t;
+            const uint32_t pay = pp ? pp[j] : key[i];
+            if (MODE == 0) {
+                if (o < cap) { out_p[o] = pay; out_b[o] = first[i]; }
+            } else {
+                uint64_t h = hg_slot(key[i], mask);
+                while (occ[h >> 5] & (1u << (h & 31))) {
+                    const uint64_t e = tab[h];
+                    if ((uint32_t)e == key[i]) {
+                        if (o < cap) { out_p[o] = pay; out_b[o] = (uint32_t)(e >> 32); }
+                        o++;
+                    }
+                    h = (h + 1) & mask;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+uint64_t hash_join_global(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
+                          const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap,
+                          hipStream_t s) {
+    uint64_t slots = 1;
+    while (slots < 2 * nb) slots <<= 1;
+    const uint64_t mask = slots - 1;
+    Scratch tab(c, slots * 8, s), occ(c, std::max<uint64_t>(slots / 8, 4), s);
+    fill_u32(c, occ.get(), 0, std::max<uint64_t>(slots / 32, 1), s);
+    launch(c, "hg_build", k_hg_build, dim3((uint32_t)std::min<uint64_t>(ceil_div(nb, 256), 16384)), dim3(256), 0, s, bk,
+           bp, nb, tab.as<uint64_t>(), occ.as<uint32_t>(), mask);
+    const uint64_t ntiles = ceil_div(np, HG_TILE);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)c->cu_count * 8);
+    Scratch st(c, 8 * (ntiles + 1), s), flags(c, 16, s);
+    fill_u64(c, st.get(), 0, ntiles + 1, s);
+    fill_u32(c, flags.get(), 0, 4, s);
+    const bool direct = cap >= np && out_p && out_b;
+    Scratch tp(c, direct ? 0 : np * 4, s), tb(c, direct ? 0 : np * 4, s);
+    uint32_t* p0 = direct ? out_p : tp.as<uint32_t>();
+    uint32_t* b0 = direct ? out_b : tb.as<uint32_t>();
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(st.as<uint64_t>() + ntiles);
+    launch(c, "hg_probe", k_hg_probe<0>, dim3(grid), dim3(HG_BLOCK), 0, s, pk, pp, np,
+           (const uint64_t*)tab.as<uint64_t>(), (const uint32_t*)occ.as<uint32_t>(), mask, p0, b0, (uint64_t)np,
+           st.as<uint64_t>(), ctr, ntiles, (uint32_t*)nullptr, (const uint32_t*)nullptr, flags.as<uint32_t>());
+    uint32_t fl[4];
+    PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
+    if (!fl[0]) {
+        if (!direct && out_p && out_b && cap) {
+            copy_d2d(c, out_p, p0, std::min(total, cap) * 4, s);
+            copy_d2d(c, out_b, b0, std::min(total, cap) * 4, s);
+            PD_HIP(hipStreamSynchronize(s));
+        }
+        return total;
+    }
+    // duplicate build keys: count per tile, scan, write every pair
+    Scratch tc(c, (ntiles + 1) * 4, s), to(c, (ntiles + 1) * 4, s), tot(c, 8, s);
+    fill_u32(c, ctr, 0, 1, s);
+    launch(c, "hg_probe_count", k_hg_probe<1>, dim3(grid), dim3(HG_BLOCK), 0, s, pk, pp, np,
+           (const uint64_t*)tab.as<uint64_t>(), (const uint32_t*)occ.as<uint32_t>(), mask, (uint32_t*)nullptr,
+           (uint32_t*)nullptr, (uint64_t)0, st.as<uint64_t>(), ctr, ntiles, tc.as<uint32_t>(), (const uint32_t*)nullptr,
+           flags.as<uint32_t>());
+    exclusive_scan_u32(c, tc.as<uint32_t>(), to.as<uint32_t>(), ntiles, tot.as<uint64_t>(), s);
+    total = read_u64(c, tot.get(), s);
+    PD_REQUIRE(total < (1ull << 32), PIMDAL_E_UNSUPPORTED, "join output exceeds 2^32-1 pairs");
+    if (out_p && out_b && cap) {
+        fill_u32(c, ctr, 0, 1, s);
+        launch(c, "hg_probe_write", k_hg_probe<2>, dim3(grid), dim3(HG_BLOCK), 0, s, pk, pp, np,
+               (const uint64_t*)tab.as<uint64_t>(), (const uint32_t*)occ.as<uint32_t>(), mask, out_p, out_b, cap,
+               st.as<uint64_t>(), ctr, ntiles, (uint32_t*)nullptr, (const uint32_t*)to.as<uint32_t>(),
+               flags.as<uint32_t>());
+        PD_HIP(hipStreamSynchronize(s));
+    }
+    return total;
+}
+
+}  // namespace pd

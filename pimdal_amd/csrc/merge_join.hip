@@ -56,9 +56,10 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
                                                          const uint32_t* __restrict__ sp, uint64_t ns,
                                                          uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
                                                          uint64_t* __restrict__ status, uint32_t* __restrict__ tile_ctr,
-                                                         uint64_t ntiles, uint64_t* __restrict__ d_count) {
+                                                         uint64_t ntiles, uint64_t* __restrict__ d_count,
+                                                         const uint64_t* __restrict__ ranges) {
     __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_lo, s_hi, s_excl;
+    __shared__ uint64_t s_excl;
     __shared__ uint32_t s_rk[SM_RCAP];
     __shared__ uint32_t s_cnt[SM_ITEMS][SM_WAVES];
     __shared__ uint32_t s_total;
@@ -71,12 +72,7 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
     const uint64_t tile = s_tile;
     const uint64_t base = tile * SM_TILE;
     const uint64_t end = min(ns, base + SM_TILE);
-    if (t == 0) {
-        s_lo = lower_bound_u32(rk, 0, nr, sk[base]);
-        s_hi = upper_bound_u32(rk, s_lo, nr, sk[end - 1]);
-    }
-    __syncthreads();
-    const uint64_t lo = s_lo, hi = s_hi;
+    const uint64_t lo = ranges[2 * tile], hi = ranges[2 * tile + 1];
     const bool cached = (hi - lo) <= SM_RCAP;
     if (cached)
         for (uint64_t i = lo + t; i < hi; i += SM_BLOCK) s_rk[i - lo] = rk[i];
@@ -150,6 +146,18 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
     }
 }
 
+// Build-side range each probe tile can match: [lower_bound(first key),
+// upper_bound(last key)).  One thread per tile, all tiles in parallel.
+__global__ void k_smj_tile_ranges(const uint32_t* __restrict__ rk, uint64_t nr, const uint32_t* __restrict__ sk,
+                                  uint64_t ns, uint64_t ntiles, uint64_t* __restrict__ ranges) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < ntiles; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = t * SM_TILE, e = min(ns, b + SM_TILE);
+        const uint64_t lo = lower_bound_u32(rk, 0, nr, sk[b]);
+        ranges[2 * t] = lo;
+        ranges[2 * t + 1] = upper_bound_u32(rk, lo, nr, sk[e - 1]);
+    }
+}
+
 // per probe row: first matching build row and number of matches
 __global__ void k_smj_ranges(const uint32_t* __restrict__ rk, uint64_t nr, const uint32_t* __restrict__ sk, uint64_t ns,
                              uint32_t* __restrict__ first, uint32_t* __restrict__ cnt) {
@@ -207,11 +215,15 @@ uint64_t sort_merge_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, 
         Scratch tp(c, direct ? 0 : np * 4, s), tb(c, direct ? 0 : np * 4, s);
         uint32_t* op = direct ? out_p : tp.as<uint32_t>();
         uint32_t* ob = direct ? out_b : tb.as<uint32_t>();
+        Scratch rng(c, ntiles * 16, s);
+        launch(c, "smj_ranges", k_smj_tile_ranges, dim3((uint32_t)ceil_div(ntiles, 256)), dim3(256), 0, s,
+               (const uint32_t*)rk.as<uint32_t>(), nb, (const uint32_t*)sk.as<uint32_t>(), np, ntiles,
+               rng.as<uint64_t>());
         launch(c, "smj_merge", k_smj_unique, dim3((uint32_t)ntiles), dim3(SM_BLOCK), 0, s,
                (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(), nb,
                (const uint32_t*)sk.as<uint32_t>(), (const uint32_t*)spay.as<uint32_t>(), np, op, ob,
                st.as<uint64_t>(), reinterpret_cast<uint32_t*>(st.as<uint64_t>() + ntiles), ntiles,
-               small.as<uint64_t>() + 1);
+               small.as<uint64_t>() + 1, (const uint64_t*)rng.as<uint64_t>());
         uint64_t total = 0;
         PD_HIP(hipMemcpyAsync(&total, small.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));

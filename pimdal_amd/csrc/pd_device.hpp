@@ -123,6 +123,14 @@ __device__ __forceinline__ uint32_t ref_hash(uint32_t x) {
     return (uint32_t)__builtin_bswap64(p);
 }
 
+// SplitMix64 finaliser (device copy; the host definition is in pimdal_gen.h)
+__device__ __forceinline__ uint64_t pd_mix64_dev(uint64_t x) {
+    x ^= x >> 30; x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27; x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return x;
+}
+
 // Bucket inside a partition's table: bits of the same product that the
 // partition number does not use.
 __device__ __forceinline__ uint32_t slot_hash(uint32_t x) {

@@ -144,11 +144,11 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
             bool m = false;
             if (r < rend) {
                 if constexpr (JOIN) {
-                    const uint32_t d = sv_dim_lookup(T, __builtin_nontemporal_load(&T.orderkey[r]));
+                    const uint32_t d = sv_dim_lookup(T, T.orderkey[r]);
                     m = d != 0xFFFFFFFFu && (int64_t)T.dim_date[d] < q.param;
                     dimrow[i] = d;
                 } else {
-                    m = __builtin_nontemporal_load(&T.price[r]) < q.param;
+                    m = T.price[r] < q.param;
                 }
             }
             hit |= (uint32_t)m << i;
@@ -497,11 +497,12 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         S->ctx = c;
         S->tab = t;
         hipStream_t s = c->stream;
-        // resident dimension-key index: power-of-two slots at load <= 0.9, so
-        // index + date column stay small enough for the 256 MB Infinity Cache
-        // at the configured dimension size (1.5e7 keys -> 2^24 slots, 128 MB)
+        // resident dimension-key index: power-of-two slots at load <= 0.5.
+        // Higher loads lengthen linear-probe chains, and a wave waits for its
+        // slowest lane, so the tail of the chain length sets the probe cost.
+        const double load = (double)c->tune_get("serve_index_load_pct", 50) / 100.0;
         uint64_t cap = 1;
-        while ((double)cap * 0.9 < (double)t.n_dim) cap <<= 1;
+        while ((double)cap * load < (double)t.n_dim) cap <<= 1;
         S->dim_index = c->pool.alloc(cap * 8, s);
         fill_u64(c, S->dim_index, 0, cap, s);
         Scratch dup(c, 4, s);

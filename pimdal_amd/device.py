@@ -127,6 +127,13 @@ class Context:
     def set_tune(self, key: str, value: int) -> None:
         call("pimdal_ctx_set_tune", self.handle, key.encode(), int(value))
 
+    def autotune(self, op: str, n: int, cache_path: str | None = None) -> None:
+        """Pick the fastest knob values for `op` at size n (pimdal_autotune)."""
+        call("pimdal_autotune", self.handle, op.encode(), int(n), cache_path.encode() if cache_path else None)
+
+    def load_tuning(self, cache_path: str) -> None:
+        call("pimdal_ctx_load_tuning", self.handle, cache_path.encode())
+
     def get_tune(self, key: str) -> int:
         v = C.c_int64()
         call("pimdal_ctx_get_tune", self.handle, key.encode(), C.byref(v))

@@ -85,3 +85,15 @@ def test_sort_2_28_properties(ctx, zipf):
     assert np.array_equal(k[gv], gk)
     ties = gk[:-1] == gk[1:]
     assert np.all(gv[:-1][ties] < gv[1:][ties])
+
+
+def test_autotune_sets_knobs_and_results_stay_exact(ctx, orc, tmp_path):
+    cache = str(tmp_path / "tune.txt")
+    ctx.autotune("sort", 1 << 22, cache)
+    assert ctx.get_tune("sort_block") in (256, 512)
+    ctx.load_tuning(cache)
+    k, v = orc.sort_workload(1 << 20, seed=5)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+    assert open(cache).read().split()[0] == "sort"
