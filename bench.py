@@ -106,9 +106,12 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
-KERNELS = ("part_hist", "part_scatter", "scan", "hj_chunks", "hj_join", "hj_compact", "radix_hist", "radix_scan",
-           "radix_onesweep", "smj_dupcheck", "smj_merge", "q1_domain", "q1_lut", "q1_dense", "q1_final", "gb_direct",
-           "gb_hist", "gb_scatter", "gb_lds", "gb_global", "q1_unpack", "fill_u32", "fill_u64")
+KERNELS = ("part_hist", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
+           "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
+           "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
+           "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
+           "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
+           "select", "fill_u32", "fill_u64")
 
 
 def kernel_breakdown(ctx, steps):
@@ -168,7 +171,7 @@ def bench_q1(ctx, steps, warmup, by_key, n=60_000_000):
 
 
 def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, rate_qps=1e6, slots=4):
-    """Config 5: mixed query stream over resident tables, batched vs one at a time."""
+    """Mixed select/join/group-by query stream over resident tables (6e7-row fact table, 1.5e7-row dimension table), batched vs one at a time."""
     import numpy as np
     from pimdal_amd import datagen, serving
     fact = datagen.fact_table(ctx, n_fact, groups=n_dim)
@@ -220,8 +223,8 @@ def main():
                     help="also run one 1 GiB device-to-device copy (the profiler's byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(each GPU box has one MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path; "
+              "run with --gpus 1 (how the operators would shard is documented with the design)", file=sys.stderr)
         sys.exit(2)
 
     import pimdal_amd
@@ -250,13 +253,20 @@ def main():
             for z in (False, True):
                 log(f"bench: {which} join 2^24 x 2^27 zipf={z}")
                 ops_out[f"{which}_join{'_zipf' if z else ''}"] = bench_join(ctx, max(3, args.steps // 2), 1, which, z)
+        # small build side: the global-table mode against the partitioned one
+        for mode in (1, 0):
+            log(f"bench: hash join 2^20 x 2^27 hj_mode={mode}")
+            ctx.set_tune("hj_mode", mode)
+            ops_out[f"hash_join_build2^20_mode{mode}"] = bench_join(ctx, max(3, args.steps // 2), 1, "hash", False,
+                                                                    nr=1 << 20)
+        ctx.set_tune("hj_mode", -1)
     if args.only in ("", "all", "groupby"):
         for by_key in (False, True):
             log(f"bench: filter+group-by 6e7 by_key={by_key}")
             ops_out["q1_orderkey" if by_key else "q1_flags"] = bench_q1(ctx, max(3, args.steps // 2), 1, by_key)
     if args.only in ("", "all", "serve"):
         log("bench: serving mixed stream")
-        ops_out["serve_config5"] = bench_serve(ctx)
+        ops_out["serve_mixed_stream"] = bench_serve(ctx)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")

@@ -72,7 +72,10 @@ int pimdal_ctx_set_budget(pimdal_ctx* ctx, uint64_t hbm_budget_bytes);
 int pimdal_ctx_set_profiling(pimdal_ctx* ctx, int on);
 int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
-/* Tuning knobs (radix bits, fan-out, tile shapes); names in DESIGN.md. */
+/* Tuning knobs by name: sort_block, sort_early_lookback, select_blocks_per_cu,
+ * select_lookback_width (1, 2 or 4 status words per lane), hj_mode (0 partitioned, 1 global table), hj_fanout, hj_rows_per_part,
+ * hj_chunk, gb_groups_hint, gb_rows_per_part, gb_lds_max_rows,
+ * serve_index_load_pct.  Unknown keys are stored and ignored. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
 /* Autotuner: times candidate knob values for `op` ("sort", "select",

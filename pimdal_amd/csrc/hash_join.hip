@@ -11,10 +11,12 @@
 //
 // Plan on gfx950:
 //  1. radix-partition both relations by the reference hash
-//     (hash_join.cpp:20-25) into F partitions: per-block LDS histograms
-//     (k_part_hist), one device scan over [partition][block], then a
-//     scatter that re-reads the rows and claims slots with LDS counters
-//     (k_part_scatter), writing (key, payload) pairs;
+//     (hash_join.cpp:20-25) into F partitions with the onesweep passes of
+//     the radix sort run over hash digits (hash_partition): each pass
+//     reorders an 8192-row tile by digit in LDS and writes digit runs, so
+//     HBM sees long sequential runs instead of one 8-byte write per row
+//     into F x blocks separate streams; partition bounds come from a
+//     binary search over the partitioned keys;
 //  2. split every probe partition into chunks of at most CH rows; each
 //     chunk is one work item, so a hot partition under Zipf skew becomes
 //     many blocks instead of one straggler;
@@ -33,65 +35,9 @@
 
 namespace pd {
 
-constexpr int PT_BLOCK = 512;
 constexpr int HJ_BLOCK = 512;
 constexpr int HJ_TS = 8192;           // LDS table slots (8 B each)
-constexpr int HJ_ROWS = 4;            // probe rows per thread per round
-
-// ---------------- radix partition of (key, payload) rows ----------------
-
-__global__ __launch_bounds__(PT_BLOCK) void k_part_hist(const uint32_t* __restrict__ keys, uint64_t n, uint64_t chunk,
-                                                        uint32_t F, uint32_t* __restrict__ H) {
-    extern __shared__ uint32_t h[];
-    const uint32_t t = threadIdx.x, b = blockIdx.x, G = gridDim.x;
-    for (uint32_t f = t; f < F; f += PT_BLOCK) h[f] = 0;
-    __syncthreads();
-    const uint64_t start = (uint64_t)b * chunk, end = min(n, start + chunk);
-    const uint32_t fm = F - 1;
-    for (uint64_t i = start + t; i < end; i += PT_BLOCK) atomicAdd(&h[pd::ref_hash(keys[i]) & fm], 1u);
-    __syncthreads();
-    for (uint32_t f = t; f < F; f += PT_BLOCK) H[(uint64_t)f * G + b] = h[f];
-}
-
-__global__ __launch_bounds__(PT_BLOCK) void k_part_scatter(const uint32_t* __restrict__ keys,
-                                                           const uint32_t* __restrict__ pay, uint64_t n,
-                                                           uint64_t chunk, uint32_t F,
-                                                           const uint32_t* __restrict__ Hs, uint2* __restrict__ out) {
-    extern __shared__ uint32_t ctr[];
-    const uint32_t t = threadIdx.x, b = blockIdx.x, G = gridDim.x;
-    for (uint32_t f = t; f < F; f += PT_BLOCK) ctr[f] = Hs[(uint64_t)f * G + b];
-    __syncthreads();
-    const uint64_t start = (uint64_t)b * chunk, end = min(n, start + chunk);
-    const uint32_t fm = F - 1;
-    for (uint64_t i = start + t; i < end; i += PT_BLOCK) {
-        const uint32_t k = keys[i];
-        const uint32_t p = pay ? pay[i] : k;
-        const uint32_t pos = atomicAdd(&ctr[pd::ref_hash(k) & fm], 1u);
-        if (pos < n) out[pos] = make_uint2(k, p);
-    }
-}
-
-// Partitioned relation: rows in `rows`, partition f spans
-// [H[f*G], H[(f+1)*G]) where H is the scanned [partition][block] array.
-struct Parted {
-    uint2* rows;
-    uint32_t* H;
-    uint32_t G;
-};
-
-static uint32_t part_blocks(pimdal_ctx* c, uint64_t n) {
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 16384), (uint64_t)c->cu_count * 2));
-}
-
-void radix_partition(pimdal_ctx* c, const uint32_t* keys, const uint32_t* pay, uint64_t n, uint32_t F, uint2* out,
-                     uint32_t* H, uint32_t G, hipStream_t s) {
-    const uint64_t chunk = ceil_div(n, G);
-    const uint32_t shm = F * 4;
-    launch(c, "part_hist", k_part_hist, dim3(G), dim3(PT_BLOCK), shm, s, keys, n, chunk, F, H);
-    exclusive_scan_u32(c, H, H, (uint64_t)F * G, nullptr, s);
-    launch(c, "part_scatter", k_part_scatter, dim3(G), dim3(PT_BLOCK), shm, s, keys, pay, n, chunk, F,
-           (const uint32_t*)H, out);
-}
+constexpr int HJ_ROWS = 16;           // probe rows per thread per round
 
 // ---------------- work items over probe partitions ----------------
 
@@ -109,8 +55,10 @@ __global__ void k_hj_chunks(const uint32_t* __restrict__ HS, uint32_t GS, uint32
 enum { HJ_UNIQUE = 0, HJ_COUNT = 1, HJ_WRITE = 2 };
 
 template <int PHASE>
-__global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint2* __restrict__ R, const uint32_t* __restrict__ HR,
-                                                         uint32_t GR, const uint2* __restrict__ S,
+__global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restrict__ Rk,
+                                                         const uint32_t* __restrict__ Rp, const uint32_t* __restrict__ HR,
+                                                         uint32_t GR, const uint32_t* __restrict__ Sk,
+                                                         const uint32_t* __restrict__ Sp,
                                                          const uint32_t* __restrict__ HS, uint32_t GS, uint32_t F,
                                                          uint32_t CH, const uint32_t* __restrict__ wstart,
                                                          uint32_t* __restrict__ wcount,
@@ -152,7 +100,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint2* __restrict
 
     // ---- build ----
     for (uint64_t i = r0 + t; i < r1; i += HJ_BLOCK) {
-        const uint2 kv = R[i];
+        const uint2 kv = make_uint2(Rk[i], Rp[i]);
         uint32_t h = pd::slot_hash(kv.x) & (HJ_TS - 1);
         while (true) {
             const uint32_t bit = 1u << (h & 31);
@@ -173,19 +121,23 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint2* __restrict
     for (uint64_t rb = s0; rb < s1; rb += (uint64_t)HJ_BLOCK * HJ_ROWS) {
         uint32_t key[HJ_ROWS], pay[HJ_ROWS], first[HJ_ROWS], m[HJ_ROWS];
         uint32_t cnt = 0;
+        // all loads of the round first, so they are in flight together
+#pragma unroll
+        for (int k = 0; k < HJ_ROWS; k++) {
+            const uint64_t i = rb + (uint64_t)k * HJ_BLOCK + t;
+            key[k] = i < s1 ? Sk[i] : 0u;
+            pay[k] = i < s1 ? Sp[i] : 0u;
+        }
 #pragma unroll
         for (int k = 0; k < HJ_ROWS; k++) {
             const uint64_t i = rb + (uint64_t)k * HJ_BLOCK + t;
             m[k] = 0;
             first[k] = 0;
             if (i < s1) {
-                const uint2 kv = S[i];
-                key[k] = kv.x;
-                pay[k] = kv.y;
-                uint32_t h = pd::slot_hash(kv.x) & (HJ_TS - 1);
+                uint32_t h = pd::slot_hash(key[k]) & (HJ_TS - 1);
                 while (occ[h >> 5] & (1u << (h & 31))) {
                     const uint2 e = tab[h];
-                    if (e.x == kv.x) {
+                    if (e.x == key[k]) {
                         if (m[k] == 0) first[k] = e.y;
                         m[k]++;
                     }
@@ -292,14 +244,20 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
     PD_REQUIRE(nb < (1ull << 32) && np < (1ull << 32), PIMDAL_E_INVALID, "join inputs must have < 2^32 rows");
     PD_REQUIRE((bk || !nb) && (pk || !np), PIMDAL_E_INVALID, "key pointer is NULL");
     if (nb == 0 || np == 0) return 0;
-    if (c->tune_get("hj_mode", 1) == 1)
+    // hj_mode: 1 = one global table (small build sides, whose table the
+    // caches hold), 0 = radix-partitioned LDS tables; default by build size
+    const int64_t mode = c->tune_get("hj_mode", -1);
+    if (mode == 1 || (mode < 0 && nb <= (uint64_t)c->tune_get("hj_global_max_build", 1 << 20)))
         return hash_join_global(c, bk, bp, nb, pk, pp, np, out_p, out_b, cap, s);
+    PD_REQUIRE(nb < (1ull << 30) && np < (1ull << 30), PIMDAL_E_INVALID,
+               "partitioned hash join supports up to 2^30-1 rows per side");
     const int64_t tuned = c->tune_get("hj_fanout", 0);
     uint32_t F = tuned > 0 ? next_pow2((uint64_t)tuned)
                            : next_pow2(std::max<uint64_t>(1, ceil_div(nb, (uint64_t)c->tune_get("hj_rows_per_part", 4096))));
+    F = std::max<uint32_t>(F, 2);
     const uint32_t CH = (uint32_t)c->tune_get("hj_chunk", 32768);
-    const uint32_t GR = part_blocks(c, nb), GS = part_blocks(c, np);
-    Scratch rrows(c, nb * 8, s), srows(c, np * 8, s);
+    constexpr uint32_t GR = 1, GS = 1;   // partition f = [H[f], H[f + 1])
+    Scratch rk(c, nb * 4, s), rp(c, nb * 4, s), sk(c, np * 4, s), sp(c, np * 4, s);
     Scratch flags(c, 16, s);
     const bool direct = cap >= np && out_p && out_b;
     Scratch tmp_p(c, direct ? 0 : np * 4, s), tmp_b(c, direct ? 0 : np * 4, s);
@@ -309,9 +267,10 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         PD_REQUIRE(F <= 16384, PIMDAL_E_UNSUPPORTED,
                    "hash join: a build partition does not fit the LDS table (heavy duplicate build keys); "
                    "use the sort-merge join");
-        Scratch HR(c, ((uint64_t)F * GR + 1) * 4, s), HS(c, ((uint64_t)F * GS + 1) * 4, s);
-        radix_partition(c, bk, bp, nb, F, rrows.as<uint2>(), HR.as<uint32_t>(), GR, s);
-        radix_partition(c, pk, pp, np, F, srows.as<uint2>(), HS.as<uint32_t>(), GS, s);
+        const uint32_t bits = (uint32_t)__builtin_ctz(F);
+        Scratch HR(c, ((uint64_t)F + 1) * 4, s), HS(c, ((uint64_t)F + 1) * 4, s);
+        hash_partition(c, bk, bp ? bp : bk, nb, bits, rk.as<uint32_t>(), rp.as<uint32_t>(), HR.as<uint32_t>(), s);
+        hash_partition(c, pk, pp ? pp : pk, np, bits, sk.as<uint32_t>(), sp.as<uint32_t>(), HS.as<uint32_t>(), s);
         Scratch wst(c, ((uint64_t)F + 1) * 4, s);
         launch(c, "hj_chunks", k_hj_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s,
                (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, wst.as<uint32_t>());
@@ -321,8 +280,9 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         fill_u32(c, flags.get(), 0, 4, s);
         fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
         launch(c, "hj_join", k_hj_join<HJ_UNIQUE>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
-               (const uint2*)rrows.as<uint2>(), (const uint32_t*)HR.as<uint32_t>(), GR,
-               (const uint2*)srows.as<uint2>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
+               (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
+               (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
+               (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
                (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), (const uint32_t*)nullptr, p0, b0,
                (uint64_t)np, flags.as<uint32_t>());
         uint32_t fl[4];
@@ -338,8 +298,9 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
             fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
             fill_u32(c, flags.get(), 0, 4, s);
             launch(c, "hj_join_count", k_hj_join<HJ_COUNT>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
-                   (const uint2*)rrows.as<uint2>(), (const uint32_t*)HR.as<uint32_t>(), GR,
-                   (const uint2*)srows.as<uint2>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
+                   (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
+                   (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
+                   (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
                    (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), (const uint32_t*)nullptr,
                    (uint32_t*)nullptr, (uint32_t*)nullptr, (uint64_t)0, flags.as<uint32_t>());
             exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
@@ -349,8 +310,9 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
             PD_REQUIRE(total < (1ull << 32), PIMDAL_E_UNSUPPORTED, "join output exceeds 2^32-1 pairs");
             if (out_p && out_b && cap)
                 launch(c, "hj_join_write", k_hj_join<HJ_WRITE>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
-                       (const uint2*)rrows.as<uint2>(), (const uint32_t*)HR.as<uint32_t>(), GR,
-                       (const uint2*)srows.as<uint2>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
+                       (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
+                       (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
+                       (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
                        (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(),
                        (const uint32_t*)woff.as<uint32_t>(), out_p, out_b, cap, flags.as<uint32_t>());
             return total;
@@ -401,8 +363,9 @@ extern "C" int pimdal_hash_join_dev(pimdal_ctx* c, const uint32_t* build_key, co
 // The build side goes into one open-addressing table in HBM (slot = key |
 // payload << 32, occupancy in a separate bitmap so every key and payload
 // value is legal); the probe side streams through it once.  At the
-// benchmark size the table (2^25 slots, 256 MB) and bitmap (4 MB) are what
-// the Infinity Cache and the L2s hold, so probes mostly avoid HBM.  Output
+// It pays for small build sides only, whose table the Infinity Cache and the
+// L2s hold: a 2^24-row build (2^25 slots, 256 MB) sends nearly every probe
+// to HBM for a whole cache line, well behind the partitioned mode.  Output
 // is compacted per 4096-row probe tile with a decoupled look-back, so it
 // comes out in probe order.
 
@@ -512,7 +475,7 @@ __global__ __launch_bounds__(HG_BLOCK) void k_hg_probe(const uint32_t* __restric
         __syncthreads();
         if (MODE == 0) {
             if (w == 0) {
-                const uint64_t e = pd::lookback_single(status, tile, s_total);
+                const uint64_t e = pd::lookback_single<4>(status, tile, s_total);
                 if (l == 0) {
                     s_excl = e;
                     if (tile == ntiles - 1) atomicAdd(reinterpret_cast<unsigned long long*>(flags + 2), (unsigned long long)(e + s_total));

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
     __syncthreads();
     const uint32_t total = s_total;
     if (w == 0) {
-        const uint64_t e = pd::lookback_single(status, tile, total);
+        const uint64_t e = pd::lookback_single<4>(status, tile, total);
         if (l == 0) {
             s_excl = e;
             if (tile == ntiles - 1) *d_count = e + total;

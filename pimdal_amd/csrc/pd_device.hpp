@@ -82,6 +82,13 @@ constexpr uint64_t LB_VALUE = (1ull << 62) - 1;
 // Called by every lane of ONE wave of the tile's block.  Publishes the tile
 // aggregate, accumulates the exclusive prefix and publishes the inclusive
 // prefix.  Returns the exclusive prefix (same value in all lanes).
+//
+// Each step reads a window of 64*W predecessors (lane l reads tiles
+// base - l*W - j, j < W, all loads in flight together).  When tiles finish
+// faster than one coherent load round trip per 64*W tiles, the distance to
+// the nearest published inclusive prefix grows without bound, so W sets the
+// tile rate the chain can sustain.
+template <int W = 1>
 __device__ __forceinline__ uint64_t lookback_single(uint64_t* status, uint64_t tile, uint64_t agg) {
     if (tile == 0) {
         if (lane() == 0) st_relaxed(&status[0], LB_FLAG_PRE | agg);
@@ -89,27 +96,50 @@ __device__ __forceinline__ uint64_t lookback_single(uint64_t* status, uint64_t t
     }
     if (lane() == 0) st_relaxed(&status[tile], LB_FLAG_AGG | agg);
     uint64_t excl = 0;
-    int64_t base = (int64_t)tile - 1;  // window covers [base - 63, base]
+    int64_t base = (int64_t)tile - 1;  // window covers [base - 64*W + 1, base]
     while (true) {
-        const int64_t idx = base - (int64_t)lane();
-        uint64_t s = (idx >= 0) ? ld_relaxed(&status[idx]) : LB_FLAG_PRE;  // before tile 0: prefix 0
-        // Wait until every lane sees a non-empty status.
-        while (ballot((s >> 62) == 0)) {
-            if ((s >> 62) == 0) {
+        uint64_t s[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const int64_t idx = base - (int64_t)lane() * W - j;
+            s[j] = (idx >= 0) ? ld_relaxed(&status[idx]) : LB_FLAG_PRE;  // before tile 0: prefix 0
+        }
+        // Wait until every status in the window is non-empty (all are
+        // predecessors, already running, so each publishes soon).
+        while (true) {
+            bool empty = false;
+#pragma unroll
+            for (int j = 0; j < W; j++) empty |= (s[j] >> 62) == 0;
+            if (!ballot(empty)) break;
+            if (empty) {
                 __builtin_amdgcn_s_sleep(1);
-                s = ld_relaxed(&status[idx]);
+#pragma unroll
+                for (int j = 0; j < W; j++) {
+                    if ((s[j] >> 62) == 0) s[j] = ld_relaxed(&status[base - (int64_t)lane() * W - j]);
+                }
             }
         }
-        const uint64_t pre_mask = ballot((s >> 62) == 2);
-        uint64_t v = s & LB_VALUE;
+        // nearest inclusive prefix: first lane holding one, first j in that lane
+        int first_j = W;
+#pragma unroll
+        for (int j = W - 1; j >= 0; j--) {
+            if ((s[j] >> 62) == 2) first_j = j;
+        }
+        const uint64_t pre_mask = ballot(first_j < W);
+        uint64_t v = 0;
         if (pre_mask) {
-            const uint32_t first = __builtin_ctzll(pre_mask);  // nearest inclusive prefix
-            if (lane() > first) v = 0;
+            const uint32_t fl = __builtin_ctzll(pre_mask);
+#pragma unroll
+            for (int j = 0; j < W; j++) {
+                if (lane() < fl || (lane() == fl && j <= first_j)) v += s[j] & LB_VALUE;
+            }
             excl += wave_sum64(v);
             break;
         }
+#pragma unroll
+        for (int j = 0; j < W; j++) v += s[j] & LB_VALUE;
         excl += wave_sum64(v);
-        base -= PD_WAVE;
+        base -= (int64_t)PD_WAVE * W;
     }
     if (lane() == 0) st_relaxed(&status[tile], LB_FLAG_PRE | (excl + agg));
     return excl;

@@ -36,7 +36,16 @@ struct RsPassDesc {
     uint32_t npass;
 };
 
+// Digit source: the key itself, or (HASH) the reference join hash of the
+// key, so the same passes radix-partition rows by hash bits.
+template <bool HASH>
+__device__ __forceinline__ uint32_t rs_src(uint32_t k) {
+    if constexpr (HASH) return pd::ref_hash(k);
+    else return k;
+}
+
 // ---- histogram of every pass + AND/OR of the keys ----
+template <bool HASH>
 __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n, RsPassDesc pd,
                                                  uint32_t* __restrict__ g_hist, uint32_t* __restrict__ g_andor) {
     __shared__ uint32_t h[4][RS_BINS];
@@ -53,14 +62,16 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
         for (int e = 0; e < 4; e++) {
             a &= k4[e];
             o |= k4[e];
-            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(k4[e] >> pd.shift[p]) & pd.mask[p]], 1u);
+            const uint32_t src = rs_src<HASH>(k4[e]);
+            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
         }
     }
     for (uint64_t i = nvec * 4 + blockIdx.x * (uint64_t)blockDim.x + t; i < n; i += stride) {
         const uint32_t k = keys[i];
         a &= k;
         o |= k;
-        for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(k >> pd.shift[p]) & pd.mask[p]], 1u);
+        const uint32_t src = rs_src<HASH>(k);
+        for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
     }
     __syncthreads();
     for (uint32_t i = t; i < pd.npass * RS_BINS; i += blockDim.x) {
@@ -100,7 +111,7 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 // 4 waves per SIMD: at most 128 VGPRs (second launch-bounds argument is the
 // minimum number of waves per execution unit).  EARLY: resolve the global
 // offsets (look-back) before the LDS reorder instead of after it.
-template <typename V, bool HAS_VALS, int BLOCK, bool EARLY>
+template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH>
 __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
@@ -145,7 +156,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     }
     auto digit_of = [&](int i) -> uint32_t {
         const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
-        return idx < n ? ((key[i] >> shift) & mask) : (RS_BINS - 1);
+        return idx < n ? ((rs_src<HASH>(key[i]) >> shift) & mask) : (RS_BINS - 1);
     };
 
     // ---- stable rank inside the wave: match mask per item ----
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
         const uint32_t j = t + k * BLOCK;
         if (j >= tile_count) break;   // slots past tile_count hold padding rows
         const uint32_t kk = s_keys[j];
-        const uint64_t dst = (uint64_t)s_gbase[(kk >> shift) & mask] + j;
+        const uint64_t dst = (uint64_t)s_gbase[(rs_src<HASH>(kk) >> shift) & mask] + j;
         if (dst < n) {
             kout[dst] = kk;
             if constexpr (HAS_VALS) vout[dst] = s_vals[j];
@@ -248,7 +259,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     }
 }
 
-template <typename V, bool HV>
+template <typename V, bool HV, bool HASH = false>
 static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
                        const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
     const int block = (int)c->tune_get("sort_block", 512) == 256 ? 256 : 512;
@@ -266,7 +277,8 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         V* vdst = to_out ? vout : tv.as<V>();
         fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
 #define PD_RS_LAUNCH(B, E)                                                                                       \
-    launch(c, "radix_onesweep", k_rs_onesweep<V, HV, B, E>, dim3((uint32_t)ntiles), dim3(B), 0, s, ksrc, vsrc, kdst, \
+    launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, E, HASH>, dim3((uint32_t)ntiles),  \
+           dim3(B), 0, s, ksrc, vsrc, kdst, \
            vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
         if (block == 512 && early) PD_RS_LAUNCH(512, true);
         else if (block == 512) PD_RS_LAUNCH(512, false);
@@ -315,7 +327,7 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
     }
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
     if (all.npass)
-        launch(c, "radix_hist", k_rs_hist, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
+        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
 
     RsPassDesc run = all;
     if (detect) {
@@ -347,6 +359,54 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         run_passes<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
     else
         run_passes<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
+}
+
+// Partition bounds of rows stably sorted by ref_hash(key) & (F - 1):
+// bounds[f] = first row whose partition is >= f (binary search), bounds[F] = n.
+__global__ void k_part_bounds(const uint32_t* __restrict__ k, uint64_t n, uint32_t F, uint32_t* __restrict__ bounds) {
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f <= F; f += gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = n;   // answer in [lo, hi]
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            if ((pd::ref_hash(k[mid]) & (F - 1)) < f) lo = mid + 1; else hi = mid;
+        }
+        bounds[f] = (uint32_t)(f == F ? n : lo);
+    }
+}
+
+void hash_partition(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uint64_t n, uint32_t bits,
+                    uint32_t* kout, uint32_t* vout, uint32_t* bounds, hipStream_t s) {
+    PD_REQUIRE(bits >= 1 && bits <= 24, PIMDAL_E_INVALID, "partition bits must be in [1, 24]");
+    PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "hash partition supports up to 2^30-1 rows per call");
+    const uint32_t F = 1u << bits;
+    if (n == 0) {
+        fill_u32(c, bounds, 0, F + 1, s);
+        return;
+    }
+    Scratch small(c, (4 * RS_BINS * 2 + 8) * 4, s);
+    uint32_t* hist = small.as<uint32_t>();
+    uint32_t* offs = hist + 4 * RS_BINS;
+    uint32_t* andor = offs + 4 * RS_BINS;
+    uint32_t* ctrs = andor + 2;
+    fill_u32(c, hist, 0, 4 * RS_BINS * 2 + 8, s);
+    // passes low digit first, widths as equal as possible (longer write runs)
+    RsPassDesc run{};
+    run.npass = (bits + 7) / 8;
+    for (uint32_t p = 0, b = 0; p < run.npass; p++) {
+        const uint32_t w = (bits - b) / (run.npass - p);
+        run.shift[p] = b;
+        run.mask[p] = (1u << w) - 1;
+        b += w;
+    }
+    const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
+    launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
+    launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
+    if (vin)
+        run_passes<uint32_t, true, true>(c, kin, vin, n, kout, vout, run, offs, ctrs, s);
+    else
+        run_passes<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
+    launch(c, "part_bounds", k_part_bounds, dim3((uint32_t)std::min<uint64_t>(ceil_div(F + 1, 256), 1024)), dim3(256), 0,
+           s, (const uint32_t*)kout, n, F, bounds);
 }
 
 }  // namespace pd

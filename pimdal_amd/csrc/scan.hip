@@ -42,7 +42,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t* __restri
         tot += s_wsum[i];
     }
     if (w == 0) {
-        const uint64_t e = pd::lookback_single(status, tile, tot);
+        const uint64_t e = pd::lookback_single<4>(status, tile, tot);
         if (l == 0) {
             s_excl = e;
             if (tile == ntiles - 1 && total) *total = e + tot;

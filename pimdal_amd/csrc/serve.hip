@@ -179,7 +179,7 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
         __syncthreads();
         const uint32_t total = s_wave_tot[0];
         if (w == 0) {
-            const uint64_t e = pd::lookback_single(status + q.status_off, tile, total);
+            const uint64_t e = pd::lookback_single<4>(status + q.status_off, tile, total);
             if (l == 0) {
                 s_excl = e;
                 uint64_t cks = 0;

@@ -46,7 +46,7 @@ def test_zipf_frequencies(orc):
     assert abs(f1 - 1 / H) < 0.005 and abs(f2 - 0.5 / H) < 0.004
 
 
-def test_zipf_top_key_share_config2(orc):
+def test_zipf_top_key_share_2_24_distinct(orc):
     # hottest of 2^24 values holds about 1/H(2^24) = 5.8 % of rows
     r = orc.gen_zipf(1, 7, 1 << 24, 1.0, 200_000)
     assert abs(np.count_nonzero(r == 1) / 200_000 - 0.0581) < 0.004

@@ -96,8 +96,8 @@ def test_q1_by_orderkey(ctx, orc, n, groups):
     _cmp_q1(got, exp, ordered=False)
 
 
-def test_q1_config4_full_size(ctx, orc):
-    """Config 4 at full size (6e7 rows): both variants vs the CPU restatement."""
+def test_q1_6e7_rows_full_size(ctx, orc):
+    """Q1 at full size (6e7 rows): both variants vs the CPU restatement."""
     n = 60_000_000
     d = datagen.fact_table(ctx, n)
     t = {c: v.numpy() for c, v in d.items()}

@@ -14,13 +14,31 @@ def run(ctx, fn, rk, rp, sk, sp):
     return op.numpy(k), ob.numpy(k)
 
 
+HJ_MODE = {"hash": -1, "hash_part": 0, "hash_global": 1}
+
+
+def join_fn(ctx, which):
+    """hash = mode picked by build size (hj_mode -1), hash_part = radix-
+    partitioned LDS tables (0), hash_global = one global table (1),
+    smj = sort-merge join."""
+    if which == "smj":
+        return ops.sort_merge_join
+
+    def fn(c, *args):
+        ctx.set_tune("hj_mode", HJ_MODE[which])
+        try:
+            return ops.hash_join(c, *args)
+        finally:
+            ctx.set_tune("hj_mode", -1)
+    return fn
+
+
 @pytest.mark.parametrize("zipf", [False, True])
-@pytest.mark.parametrize("which", ["hash", "smj"])
+@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_global", "smj"])
 def test_join_pk_fk_2_20(ctx, orc, zipf, which):
     rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 20, zipf=zipf)
-    fn = ops.hash_join if which == "hash" else ops.sort_merge_join
-    gp, gb = run(ctx, fn, rk, rp, sk, sp)
-    ep, eb = orc.hash_join(rk, rp, sk, sp) if which == "hash" else orc.sort_merge_join(rk, rp, sk, sp)
+    gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
+    ep, eb = orc.sort_merge_join(rk, rp, sk, sp) if which == "smj" else orc.hash_join(rk, rp, sk, sp)
     assert len(gp) == len(sk)
     if which == "smj":   # fully ordered output: bit-exact including order
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
@@ -28,7 +46,7 @@ def test_join_pk_fk_2_20(ctx, orc, zipf, which):
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash", "smj"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_global", "smj"])
 @pytest.mark.parametrize("nb,ns,kr,ks", [(1, 1, 1, 1), (3000, 5000, 1000, 2000), (100, 70_000, 60, 200),
                                          (50_000, 3, 10, 10), (4097, 8193, 1 << 31, 1 << 31)])
 def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
@@ -36,8 +54,7 @@ def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
     rp = orc.gen_u32(nb, 2, nb)
     sk = orc.gen_range(np.uint32, nb + ns, 3, ns, 0, ks - 1)
     sp = orc.gen_u32(ns, 4, ns)
-    fn = ops.hash_join if which == "hash" else ops.sort_merge_join
-    gp, gb = run(ctx, fn, rk, rp, sk, sp)
+    gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
     ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
     if which == "smj":
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
@@ -50,14 +67,14 @@ def test_join_full_keys_and_max_value(ctx, orc):
     rp = np.arange(4, dtype=np.uint32)
     sk = np.array([0xFFFFFFFF, 0, 0, 5, 0x80000000], np.uint32)
     sp = np.arange(10, 15, dtype=np.uint32)
-    for fn in (ops.hash_join, ops.sort_merge_join):
-        gp, gb = run(ctx, fn, rk, rp, sk, sp)
+    for which in ("hash_part", "hash_global", "smj"):
+        gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
         assert same_multiset((gp, gb), (np.array([10, 11, 12, 14], np.uint32), np.array([1, 0, 0, 3], np.uint32)))
 
 
 @pytest.mark.parametrize("zipf", [False, True])
-def test_join_config3_full_size_properties(ctx, zipf):
-    """Config 3 at full size: 2^24 build, 2^27 probe.  Every probe row matches
+def test_join_2p24_build_2p27_probe_properties(ctx, zipf):
+    """Full size: 2^24 build rows, 2^27 probe rows.  Every probe row matches
     exactly one build row, so the output must be, as a multiset, exactly
     {(probe payload[j], build payload[pos(probe key[j])])}.  Checked via
     per-row closed form on the device outputs of both joins."""
@@ -68,8 +85,8 @@ def test_join_config3_full_size_properties(ctx, zipf):
     inv[hk] = np.arange(nr)
     exp_b = hp[inv[sks]]
     del inv
-    for fn in (ops.hash_join, ops.sort_merge_join):
-        op, ob, k = fn(ctx, rk, rp, sk, sp)
+    for which in ("hash_part", "hash_global", "smj"):
+        op, ob, k = join_fn(ctx, which)(ctx, rk, rp, sk, sp)
         assert k == ns
         gp, gb = op.numpy(k), ob.numpy(k)
         op.free(); ob.free()

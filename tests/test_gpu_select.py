@@ -14,7 +14,7 @@ def run(ctx, k, p, lo, hi):
     return op.numpy(cnt), orow.numpy(cnt)
 
 
-def test_config1_select_2_20(ctx, orc):
+def test_select_2_20_uniform_10pct(ctx, orc):
     k, p = orc.select_workload(1 << 20)
     gp, gr = run(ctx, k, p, 0, T)
     ep, er = orc.select(k, p, 0, T)
@@ -68,3 +68,39 @@ def test_select_int64_keys(ctx, orc, n):
         op, orow, cnt = ops.select(ctx, dk, dp, lo, hi)
         ep, er = orc.select(k, p, lo, hi)
         assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
+
+
+def brute_select(k, p, lo, hi):
+    """lo <= key < hi in exact integer arithmetic (bounds may lie outside the
+    key type, which the C ABI's int64 bounds allow)."""
+    ko = k.astype(object)
+    rows = np.nonzero((ko >= lo) & (ko < hi))[0].astype(np.uint32)
+    return p[rows], rows
+
+
+@pytest.mark.parametrize("lo,hi", [(-(1 << 40), 1 << 40), (-(1 << 31), -(1 << 31) + 1), ((1 << 31) - 1, 1 << 31),
+                                   (-5, 5), (7, 7), (10, 3), (-(1 << 63), -(1 << 31)), (0, 1 << 62)])
+def test_select_int32_full_domain_bounds(ctx, orc, lo, hi):
+    k = orc.gen_u32(9, 1, 200_003).view(np.int32).copy()
+    k[:4] = [-(1 << 31), (1 << 31) - 1, 0, -1]
+    p = orc.gen_u32(9, 2, 200_003)
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    op, orow, cnt = ops.select(ctx, dk, dp, lo, hi)
+    ep, er = brute_select(k, p, lo, hi)
+    assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
+
+
+@pytest.mark.parametrize("lo,hi", [(-(1 << 63), (1 << 63) - 1), (-(1 << 63), 0), (0, (1 << 63) - 1), (-1, 1),
+                                   (1 << 62, -(1 << 62))])
+def test_select_int64_full_domain_bounds(ctx, orc, lo, hi):
+    n = 100_001
+    k = ((orc.gen_u32(10, 1, n).astype(np.uint64) << np.uint64(32)) | orc.gen_u32(10, 2, n)).view(np.int64).copy()
+    k[:4] = [-(1 << 63), (1 << 63) - 1, 0, -1]
+    p = orc.gen_u32(10, 3, n)
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    op, orow, cnt = ops.select(ctx, dk, dp, lo, hi)
+    ep, er = brute_select(k, p, lo, hi)
+    assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
+    if -(1 << 63) <= lo and hi <= (1 << 63) - 1:   # inside int64: the oracle agrees too
+        op2, or2 = orc.select(k, p, lo, hi)
+        assert np.array_equal(or2, er)

@@ -30,7 +30,7 @@ def test_select_reference_predicate_int32(orc):
     assert np.all(np.diff(orow.astype(np.int64)) > 0)
 
 
-def test_select_config1_selectivity(orc):
+def test_select_2_20_selectivity_10pct(orc):
     k, p = orc.select_workload(1 << 20)
     op, orow = orc.select(k, p, 0, 429496730)
     assert abs(len(op) / (1 << 20) - 0.1) < 0.003
