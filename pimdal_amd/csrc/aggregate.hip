@@ -287,6 +287,107 @@ struct GbRow {
     int64_t v[NI];
 };
 
+// Row sources of the per-partition kernels: load(i) gives the group key and
+// the NI int64 inputs of partitioned row i.
+template <int NI>
+struct RowsAoS {   // GbRow array written by the scatter partitioner
+    static constexpr int kNI = NI;
+    const GbRow<NI>* rows;
+    __device__ void load(uint64_t i, uint32_t& key, int64_t* in) const {
+        const GbRow<NI> r = rows[i];
+        key = r.key;
+#pragma unroll
+        for (int j = 0; j < NI; j++) in[j] = r.v[j];
+    }
+};
+
+struct RowsSum {   // key column + int64 value column (hash aggregation)
+    static constexpr int kNI = 1;
+    const uint32_t* key;
+    const int64_t* val;
+    __device__ void load(uint64_t i, uint32_t& k, int64_t* in) const {
+        k = key[i];
+        in[0] = val[i];
+    }
+};
+
+// Q1 measures packed into one word: price (int32) in bits 0-31, quantity
+// (int16) in 32-47, discount (int8) in 48-55, tax (int8) in 56-63.
+__device__ __forceinline__ uint64_t q1_pack(int64_t qty, int64_t price, int64_t disc, int64_t tax) {
+    return (uint64_t)(uint32_t)(int32_t)price | ((uint64_t)(uint16_t)(int16_t)qty << 32) |
+           ((uint64_t)(uint8_t)(int8_t)disc << 48) | ((uint64_t)(uint8_t)(int8_t)tax << 56);
+}
+__device__ __forceinline__ bool q1_packable(int64_t qty, int64_t price, int64_t disc, int64_t tax) {
+    return qty == (int16_t)qty && price == (int32_t)price && disc == (int8_t)disc && tax == (int8_t)tax;
+}
+
+struct RowsQ1Packed {
+    static constexpr int kNI = 4;
+    const uint32_t* key;
+    const uint64_t* w;
+    __device__ void load(uint64_t i, uint32_t& k, int64_t* in) const {
+        k = key[i];
+        const uint64_t x = w[i];
+        in[0] = (int16_t)(x >> 32);
+        in[1] = (int32_t)(uint32_t)x;
+        in[2] = (int8_t)(x >> 48);
+        in[3] = (int8_t)(x >> 56);
+    }
+};
+
+// Filter + pack for the Q1 group-by on a key column: rows with
+// date <= cutoff become (key, packed measures), compacted with one counter
+// reservation per wave and 1024 rows (output order is irrelevant to the
+// aggregation).  A kept row whose measures exceed the packed widths sets
+// *overflow; the caller then runs the unpacked path instead.
+constexpr int QP_ITEMS = 16;
+__global__ __launch_bounds__(256) void k_q1_pack(const int32_t* __restrict__ date, const uint32_t* __restrict__ key,
+                                                 const int64_t* __restrict__ qty, const int64_t* __restrict__ price,
+                                                 const int64_t* __restrict__ disc, const int64_t* __restrict__ tax,
+                                                 uint64_t n, int32_t cutoff, uint32_t* __restrict__ okey,
+                                                 uint64_t* __restrict__ ow, unsigned long long* __restrict__ count,
+                                                 uint32_t* __restrict__ overflow) {
+    const uint32_t l = pd::lane();
+    const uint64_t lt = pd::lanemask_lt();
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / PD_WAVE);
+    const uint64_t wid = (uint64_t)blockIdx.x * (blockDim.x / PD_WAVE) + threadIdx.x / PD_WAVE;
+    bool bad = false;
+    for (uint64_t base = wid * (PD_WAVE * QP_ITEMS); base < n; base += waves * (PD_WAVE * QP_ITEMS)) {
+        uint64_t mask[QP_ITEMS];
+        uint32_t k[QP_ITEMS];
+        uint64_t w[QP_ITEMS];
+        uint32_t total = 0;
+#pragma unroll
+        for (int j = 0; j < QP_ITEMS; j++) {
+            const uint64_t i = base + (uint64_t)j * PD_WAVE + l;
+            const bool keep = i < n && date[i] <= cutoff;
+            mask[j] = pd::ballot(keep);
+            total += pd::popc64(mask[j]);
+            k[j] = 0;
+            w[j] = 0;
+            if (keep) {
+                const int64_t q = qty[i], p = price[i], d = disc[i], t = tax[i];
+                bad |= !q1_packable(q, p, d, t);
+                k[j] = key[i];
+                w[j] = q1_pack(q, p, d, t);
+            }
+        }
+        uint64_t o = 0;
+        if (l == 0 && total) o = atomicAdd(count, (unsigned long long)total);
+        o = __shfl(o, 0, PD_WAVE);
+#pragma unroll
+        for (int j = 0; j < QP_ITEMS; j++) {
+            if (mask[j] >> l & 1) {
+                const uint64_t dst = o + pd::popc64(mask[j] & lt);
+                okey[dst] = k[j];
+                ow[dst] = w[j];
+            }
+            o += pd::popc64(mask[j]);
+        }
+    }
+    if (pd::ballot(bad) && l == 0) atomicOr(overflow, 1u);
+}
+
 __global__ __launch_bounds__(512) void k_gb_hist(GbSource src, uint64_t n, uint64_t chunk, uint32_t F,
                                                  uint32_t* __restrict__ H) {
     extern __shared__ uint32_t h[];
@@ -322,9 +423,8 @@ __global__ __launch_bounds__(512) void k_gb_scatter(GbSource src, uint64_t n, ui
 }
 
 // gb meta: [0] n_out (u64), [1] any partition sent to the global path
-template <typename Agg>
-__global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(const GbRow<Agg::NI>* __restrict__ rows,
-                                                     const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
+template <typename Agg, typename Src>
+__global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(Src src, const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
                                                      uint32_t max_rows, uint32_t* __restrict__ to_global,
                                                      uint32_t* __restrict__ out_key, uint64_t* __restrict__ out_f,
                                                      uint64_t out_stride, uint64_t* __restrict__ meta) {
@@ -351,29 +451,43 @@ __global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(const GbRow<Agg::NI>* __res
 #pragma unroll
     for (int j = 0; j < Agg::NF; j++) cols[j] = s_f[j];
     const uint32_t limit = TS * 3 / 4;
-    for (uint64_t i = r0 + t; i < r1; i += GB_BLOCK) {
-        const GbRow<Agg::NI> r = rows[i];
-        uint64_t f[Agg::NF];
-        Agg::fold(r.v, f);
-        const uint64_t want = GB_OCC | r.key;
-        uint32_t h = pd::slot_hash(r.key) & (TS - 1);
-        uint32_t probes = 0;
-        while (true) {
-            const uint64_t cur = s_key[h];
-            if (cur == want) break;
-            if (cur == 0) {
-                const uint64_t prev = atomicCAS((unsigned long long*)&s_key[h], 0ull, (unsigned long long)want);
-                if (prev == 0) {
-                    if (atomicAdd(&s_used, 1u) >= limit) s_over = 1;
-                    break;
-                }
-                if (prev == want) break;
-            }
-            h = (h + 1) & (TS - 1);
-            if (++probes >= TS) { s_over = 1; break; }
+    constexpr int R = 4;   // rows per thread per round, loaded together
+    for (uint64_t rb = r0; rb < r1; rb += (uint64_t)GB_BLOCK * R) {
+        uint32_t key[R];
+        int64_t in[R][Agg::NI];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const uint64_t i = rb + (uint64_t)q * GB_BLOCK + t;
+            if (i < r1) src.load(i, key[q], in[q]);
         }
-        if (s_over) break;
-        add_fields<Agg>(cols, h, f);
+        bool over = false;
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const uint64_t i = rb + (uint64_t)q * GB_BLOCK + t;
+            if (i >= r1 || over) continue;
+            uint64_t f[Agg::NF];
+            Agg::fold(in[q], f);
+            const uint64_t want = GB_OCC | key[q];
+            uint32_t h = pd::slot_hash(key[q]) & (TS - 1);
+            uint32_t probes = 0;
+            while (true) {
+                const uint64_t cur = s_key[h];
+                if (cur == want) break;
+                if (cur == 0) {
+                    const uint64_t prev = atomicCAS((unsigned long long*)&s_key[h], 0ull, (unsigned long long)want);
+                    if (prev == 0) {
+                        if (atomicAdd(&s_used, 1u) >= limit) s_over = 1;
+                        break;
+                    }
+                    if (prev == want) break;
+                }
+                h = (h + 1) & (TS - 1);
+                if (++probes >= TS) { s_over = 1; over = true; break; }
+            }
+            if (over) continue;
+            add_fields<Agg>(cols, h, f);
+        }
+        if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
     }
     __syncthreads();
     if (s_over) {
@@ -406,9 +520,8 @@ __global__ void k_gb_chunks(const uint32_t* __restrict__ H, uint32_t G, uint32_t
 
 // Table for partition p occupies slots [2*start_p, 2*end_p) of the region:
 // capacity twice its row count, so it never fills.
-template <typename Agg>
-__global__ __launch_bounds__(256) void k_gb_global(const GbRow<Agg::NI>* __restrict__ rows,
-                                                   const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
+template <typename Agg, typename Src>
+__global__ __launch_bounds__(256) void k_gb_global(Src src, const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
                                                    uint32_t CH, const uint32_t* __restrict__ wstart,
                                                    uint64_t* __restrict__ gkey, uint64_t* __restrict__ gf,
                                                    uint64_t gstride) {
@@ -432,11 +545,13 @@ __global__ __launch_bounds__(256) void k_gb_global(const GbRow<Agg::NI>* __restr
 #pragma unroll
     for (int j = 0; j < Agg::NF; j++) cols[j] = gf + (uint64_t)j * gstride + base;
     for (uint64_t i = c0 + threadIdx.x; i < c1; i += 256) {
-        const GbRow<Agg::NI> r = rows[i];
+        uint32_t key;
+        int64_t in[Agg::NI];
+        src.load(i, key, in);
         uint64_t f[Agg::NF];
-        Agg::fold(r.v, f);
-        const uint64_t want = GB_OCC | r.key;
-        uint64_t h = pd::slot_hash(r.key) % cap;
+        Agg::fold(in, f);
+        const uint64_t want = GB_OCC | key;
+        uint64_t h = pd::slot_hash(key) % cap;
         while (true) {
             const uint64_t cur = pd::ld_relaxed(&gkey[base + h]);
             if (cur == want) break;
@@ -597,37 +712,121 @@ static uint32_t gb_next_pow2(uint64_t x) {
     return p;
 }
 
-// Group-by driver.  Output: keys in out_key, field j of group g at
-// out_f[j * cap + g]; returns the number of groups (unspecified order).
+// Per-partition aggregation of partitioned rows (partition p spans
+// [H[p*G], H[(p+1)*G])): one LDS table per partition, partitions that do
+// not fit go to HBM tables.  Output as group_by; returns the group count.
+template <typename Agg, typename Src>
+uint64_t aggregate_parts(pimdal_ctx* c, Src src, const uint32_t* H, uint32_t G, uint32_t F, uint64_t nsel,
+                         uint32_t* out_key, uint64_t* out_f, uint64_t cap, hipStream_t s) {
+    Scratch meta(c, 8 * 4, s);
+    fill_u64(c, meta.get(), 0, 4, s);
+    Scratch tog(c, (uint64_t)F * 4, s);
+    fill_u32(c, tog.get(), 0, F, s);
+    const uint32_t max_rows = (uint32_t)c->tune_get("gb_lds_max_rows", 65536);
+    // the emit writes at most one group per row, so a selected-row-sized
+    // staging area bounds the output before the capacity check
+    const bool direct_out = cap >= nsel;
+    Scratch sk(c, direct_out ? 0 : nsel * 4, s), sf(c, direct_out ? 0 : nsel * 8 * Agg::NF, s);
+    uint32_t* ok = direct_out ? out_key : sk.as<uint32_t>();
+    uint64_t* of = direct_out ? out_f : sf.as<uint64_t>();
+    const uint64_t ostride = direct_out ? cap : nsel;
+    launch(c, "gb_lds", k_gb_lds<Agg, Src>, dim3(F), dim3(GB_BLOCK), 0, s, src, H, G, F, max_rows, tog.as<uint32_t>(),
+           ok, of, ostride, meta.as<uint64_t>());
+    uint64_t m[4];
+    PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    if (m[1]) {
+        const uint32_t CH = 16384;
+        Scratch wst(c, ((uint64_t)F + 1) * 4, s);
+        launch(c, "gb_chunks", k_gb_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s, H, G, F,
+               CH, (const uint32_t*)tog.as<uint32_t>(), wst.as<uint32_t>());
+        exclusive_scan_u32(c, wst.as<uint32_t>(), wst.as<uint32_t>(), F, nullptr, s);
+        const uint64_t slots = 2 * nsel;
+        Scratch gk(c, slots * 8, s), gfv(c, slots * 8 * Agg::NF, s);
+        fill_u64(c, gk.get(), 0, slots, s);
+        fill_u64(c, gfv.get(), 0, slots * Agg::NF, s);
+        const uint64_t max_wi = F + ceil_div(nsel, CH);
+        launch(c, "gb_global", k_gb_global<Agg, Src>, dim3((uint32_t)max_wi), dim3(256), 0, s, src, H, G, F, CH,
+               (const uint32_t*)wst.as<uint32_t>(), gk.as<uint64_t>(), gfv.as<uint64_t>(), slots);
+        launch(c, "gb_emit_global", k_gb_emit_global<Agg>, dim3((uint32_t)std::min<uint64_t>(ceil_div(slots, 256), 4096)),
+               dim3(256), 0, s, H, G, F, (const uint32_t*)tog.as<uint32_t>(), (const uint64_t*)gk.as<uint64_t>(),
+               (const uint64_t*)gfv.as<uint64_t>(), slots, ok, of, ostride, meta.as<uint64_t>());
+        PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+    }
+    const uint64_t groups = m[0];
+    PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    if (!direct_out) {
+        copy_d2d(c, out_key, ok, groups * 4, s);
+        for (int j = 0; j < Agg::NF; j++) copy_d2d(c, out_f + (uint64_t)j * cap, of + (uint64_t)j * nsel, groups * 8, s);
+        PD_HIP(hipStreamSynchronize(s));
+    }
+    return groups;
+}
+
+// Partition count for n rows: about rows_target rows per partition.
+static uint32_t gb_part_bits(pimdal_ctx* c, uint64_t n, int64_t dflt_rows) {
+    const int64_t rows_target = c->tune_get("gb_rows_per_part", dflt_rows);
+    uint32_t bits = 1;
+    while (bits < 16 && ((uint64_t)1 << bits) * (uint64_t)rows_target < n) bits++;
+    return bits;
+}
+
+// Group-by over rows already projected to (key column, Src): partitioned
+// by hash digits with the onesweep passes (long sequential runs on the
+// write side), then aggregate_parts.
+template <typename Agg, typename Src, typename MakeSrc>
+uint64_t group_by_onesweep(pimdal_ctx* c, const uint32_t* key, const void* payload, uint32_t vw, uint64_t n,
+                           MakeSrc make_src, uint32_t* out_key, uint64_t* out_f, uint64_t cap, hipStream_t s) {
+    if (n == 0) return 0;
+    const uint32_t bits = gb_part_bits(c, n, (int64_t)Agg::TS * 2);
+    const uint32_t F = 1u << bits;
+    Scratch pk(c, n * 4, s), pv(c, n * vw, s), bounds(c, ((uint64_t)F + 1) * 4, s);
+    hash_partition(c, key, payload, vw, n, bits, pk.as<uint32_t>(), pv.get(), bounds.as<uint32_t>(), s);
+    return aggregate_parts<Agg>(c, make_src(pk.as<uint32_t>(), pv.get()), bounds.as<uint32_t>(), 1u, F, n, out_key,
+                                out_f, cap, s);
+}
+
+// Low-cardinality attempt: per-block LDS tables merged into one small HBM
+// table.  Returns false (nothing written) when a table filled up; skipped
+// when the group-count hint says the groups cannot fit.
+template <typename Agg>
+bool group_by_direct(pimdal_ctx* c, const GbSource& src, uint64_t n, uint32_t* out_key, uint64_t* out_f, uint64_t cap,
+                     uint64_t* groups, hipStream_t s) {
+    const int64_t hint = c->tune_get("gb_groups_hint", 0);
+    if (!(hint == 0 || hint <= (int64_t)(Agg::TS / 4))) return false;
+    Scratch meta(c, 8 * 4, s);
+    fill_u64(c, meta.get(), 0, 4, s);
+    Scratch gk(c, 8ull * GB_DIRECT_SLOTS, s), gfv(c, 8ull * GB_DIRECT_SLOTS * Agg::NF, s);
+    fill_u64(c, gk.get(), 0, GB_DIRECT_SLOTS, s);
+    fill_u64(c, gfv.get(), 0, (uint64_t)GB_DIRECT_SLOTS * Agg::NF, s);
+    const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 4096), (uint64_t)c->cu_count * 2));
+    const uint64_t chunk = ceil_div(n, G);
+    launch(c, "gb_direct", k_gb_direct<Agg>, dim3(G), dim3(GB_BLOCK), 0, s, src, n, chunk, gk.as<uint64_t>(),
+           gfv.as<uint64_t>(), meta.as<uint64_t>());
+    uint64_t m[4];
+    PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    if (m[2]) return false;
+    PD_REQUIRE(m[3] <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    launch(c, "gb_direct_emit", k_gb_direct_emit<Agg>, dim3(64), dim3(256), 0, s, (const uint64_t*)gk.as<uint64_t>(),
+           (const uint64_t*)gfv.as<uint64_t>(), out_key, out_f, cap, meta.as<uint64_t>());
+    PD_HIP(hipStreamSynchronize(s));
+    *groups = m[3];
+    return true;
+}
+
+// Group-by driver over a projected row source (general path: filter in the
+// source, rows scattered into partitions as GbRow).  Output: keys in
+// out_key, field j of group g at out_f[j * cap + g]; returns the number of
+// groups (unspecified order).
 template <typename Agg>
 uint64_t group_by(pimdal_ctx* c, const GbSource& src, uint64_t n, uint32_t* out_key, uint64_t* out_f, uint64_t cap,
                   hipStream_t s) {
     PD_REQUIRE(n < (1ull << 31), PIMDAL_E_INVALID, "group-by supports up to 2^31-1 rows per call");
-    Scratch meta(c, 8 * 4, s);
-    fill_u64(c, meta.get(), 0, 4, s);
     if (n == 0) return 0;
-    const int64_t hint = c->tune_get("gb_groups_hint", 0);
-    const bool try_direct = hint == 0 || hint <= (int64_t)(Agg::TS / 4);
-    if (try_direct) {
-        Scratch gk(c, 8ull * GB_DIRECT_SLOTS, s), gfv(c, 8ull * GB_DIRECT_SLOTS * Agg::NF, s);
-        fill_u64(c, gk.get(), 0, GB_DIRECT_SLOTS, s);
-        fill_u64(c, gfv.get(), 0, (uint64_t)GB_DIRECT_SLOTS * Agg::NF, s);
-        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 4096), (uint64_t)c->cu_count * 2));
-        const uint64_t chunk = ceil_div(n, G);
-        launch(c, "gb_direct", k_gb_direct<Agg>, dim3(G), dim3(GB_BLOCK), 0, s, src, n, chunk, gk.as<uint64_t>(),
-               gfv.as<uint64_t>(), meta.as<uint64_t>());
-        uint64_t m[4];
-        PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
-        if (!m[2]) {
-            PD_REQUIRE(m[3] <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-            launch(c, "gb_direct_emit", k_gb_direct_emit<Agg>, dim3(64), dim3(256), 0, s,
-                   (const uint64_t*)gk.as<uint64_t>(), (const uint64_t*)gfv.as<uint64_t>(), out_key, out_f, cap,
-                   meta.as<uint64_t>());
-            return m[3];
-        }
-        fill_u64(c, meta.get(), 0, 4, s);
-    }
+    uint64_t groups = 0;
+    if (group_by_direct<Agg>(c, src, n, out_key, out_f, cap, &groups, s)) return groups;
     // partitioned path
     const int64_t rows_target = c->tune_get("gb_rows_per_part", (int64_t)Agg::TS * 2);
     const uint32_t F = std::min<uint32_t>(16384, gb_next_pow2(std::max<uint64_t>(1, ceil_div(n, (uint64_t)rows_target))));
@@ -641,50 +840,8 @@ uint64_t group_by(pimdal_ctx* c, const GbSource& src, uint64_t n, uint32_t* out_
     Scratch rows(c, std::max<uint64_t>(nsel, 1) * sizeof(GbRow<Agg::NI>), s);
     launch(c, "gb_scatter", k_gb_scatter<Agg::NI>, dim3(G), dim3(512), F * 4, s, src, n, chunk, F,
            (const uint32_t*)H.as<uint32_t>(), rows.as<GbRow<Agg::NI>>(), nsel);
-    Scratch tog(c, (uint64_t)F * 4, s);
-    fill_u32(c, tog.get(), 0, F, s);
-    const uint32_t max_rows = (uint32_t)c->tune_get("gb_lds_max_rows", 65536);
-    // the emit writes at most one group per row, so a selected-row-sized
-    // staging area bounds the output before the capacity check
-    const bool direct_out = cap >= nsel;
-    Scratch sk(c, direct_out ? 0 : nsel * 4, s), sf(c, direct_out ? 0 : nsel * 8 * Agg::NF, s);
-    uint32_t* ok = direct_out ? out_key : sk.as<uint32_t>();
-    uint64_t* of = direct_out ? out_f : sf.as<uint64_t>();
-    const uint64_t ostride = direct_out ? cap : nsel;
-    launch(c, "gb_lds", k_gb_lds<Agg>, dim3(F), dim3(GB_BLOCK), 0, s, (const GbRow<Agg::NI>*)rows.as<GbRow<Agg::NI>>(),
-           (const uint32_t*)H.as<uint32_t>(), G, F, max_rows, tog.as<uint32_t>(), ok, of, ostride, meta.as<uint64_t>());
-    uint64_t m[4];
-    PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
-    if (m[1]) {
-        const uint32_t CH = 16384;
-        Scratch wst(c, ((uint64_t)F + 1) * 4, s);
-        launch(c, "gb_chunks", k_gb_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s,
-               (const uint32_t*)H.as<uint32_t>(), G, F, CH, (const uint32_t*)tog.as<uint32_t>(), wst.as<uint32_t>());
-        exclusive_scan_u32(c, wst.as<uint32_t>(), wst.as<uint32_t>(), F, nullptr, s);
-        const uint64_t slots = 2 * nsel;
-        Scratch gk(c, slots * 8, s), gfv(c, slots * 8 * Agg::NF, s);
-        fill_u64(c, gk.get(), 0, slots, s);
-        fill_u64(c, gfv.get(), 0, slots * Agg::NF, s);
-        const uint64_t max_wi = F + ceil_div(nsel, CH);
-        launch(c, "gb_global", k_gb_global<Agg>, dim3((uint32_t)max_wi), dim3(256), 0, s,
-               (const GbRow<Agg::NI>*)rows.as<GbRow<Agg::NI>>(), (const uint32_t*)H.as<uint32_t>(), G, F, CH,
-               (const uint32_t*)wst.as<uint32_t>(), gk.as<uint64_t>(), gfv.as<uint64_t>(), slots);
-        launch(c, "gb_emit_global", k_gb_emit_global<Agg>, dim3((uint32_t)std::min<uint64_t>(ceil_div(slots, 256), 4096)),
-               dim3(256), 0, s, (const uint32_t*)H.as<uint32_t>(), G, F, (const uint32_t*)tog.as<uint32_t>(),
-               (const uint64_t*)gk.as<uint64_t>(), (const uint64_t*)gfv.as<uint64_t>(), slots, ok, of, ostride,
-               meta.as<uint64_t>());
-        PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
-    }
-    const uint64_t groups = m[0];
-    PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-    if (!direct_out) {
-        copy_d2d(c, out_key, ok, groups * 4, s);
-        for (int j = 0; j < Agg::NF; j++) copy_d2d(c, out_f + (uint64_t)j * cap, of + (uint64_t)j * nsel, groups * 8, s);
-        PD_HIP(hipStreamSynchronize(s));
-    }
-    return groups;
+    return aggregate_parts<Agg>(c, RowsAoS<Agg::NI>{rows.as<GbRow<Agg::NI>>()}, H.as<uint32_t>(), G, F, nsel, out_key,
+                                out_f, cap, s);
 }
 
 // ---- reshape engine output (u64 fields) into the Q1 result arrays ----
@@ -756,9 +913,32 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
     src.date = date;
     src.cutoff = cutoff;
     src.in[0] = qty; src.in[1] = price; src.in[2] = disc; src.in[3] = tax;
+    PD_REQUIRE(n < (1ull << 31), PIMDAL_E_INVALID, "group-by supports up to 2^31-1 rows per call");
     const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
     Scratch k32(c, stage * 4, s), fs(c, stage * 8 * AggQ1::NF, s);
-    const uint64_t g = group_by<AggQ1>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
+    uint64_t g = 0;
+    bool done = false;
+    if (key && c->tune_get("gb_packed", 1)) {
+        // filter + pack to (key, 8-byte measures), then onesweep partitions;
+        // measures that do not fit the packed widths take the general path
+        Scratch pk(c, n * 4, s), pw(c, n * 8, s), ctl(c, 16, s);
+        fill_u64(c, ctl.get(), 0, 2, s);
+        const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * QP_ITEMS), (uint64_t)c->cu_count * 8));
+        launch(c, "q1_pack", k_q1_pack, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key), qty,
+               price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ctl.as<unsigned long long>(),
+               reinterpret_cast<uint32_t*>(ctl.as<uint64_t>() + 1));
+        uint64_t cv[2];
+        PD_HIP(hipMemcpyAsync(cv, ctl.get(), 16, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        if (!(uint32_t)cv[1]) {
+            g = group_by_onesweep<AggQ1, RowsQ1Packed>(
+                c, pk.as<uint32_t>(), pw.get(), 8, cv[0],
+                [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; }, k32.as<uint32_t>(),
+                fs.as<uint64_t>(), stage, s);
+            done = true;
+        }
+    }
+    if (!done) g = group_by<AggQ1>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
     PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     launch(c, "q1_unpack", k_q1_unpack, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(g, 256), 4096))),
            dim3(256), 0, s, g, (const uint32_t*)k32.as<uint32_t>(), (const uint64_t*)fs.as<uint64_t>(), stage,
@@ -784,12 +964,19 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
                    int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
     if (n == 0) return 0;
+    PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "hash aggregation supports up to 2^30-1 rows per call");
     GbSource src{};
     src.key = key;
     src.in[0] = val;
     const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
     Scratch k32(c, stage * 4, s), fs(c, stage * 16, s);
-    const uint64_t g = group_by<AggSum>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
+    uint64_t g = 0;
+    // few groups: per-block LDS tables; otherwise onesweep partitions of
+    // (key, value) by hash digits and one LDS table per partition
+    if (!group_by_direct<AggSum>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, &g, s))
+        g = group_by_onesweep<AggSum, RowsSum>(
+            c, key, val, 8, n, [](uint32_t* k, void* v) { return RowsSum{k, (const int64_t*)v}; }, k32.as<uint32_t>(),
+            fs.as<uint64_t>(), stage, s);
     PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     copy_d2d(c, out_key, k32.get(), g * 4, s);
     copy_d2d(c, out_sum, fs.get(), g * 8, s);

@@ -269,8 +269,8 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
                    "use the sort-merge join");
         const uint32_t bits = (uint32_t)__builtin_ctz(F);
         Scratch HR(c, ((uint64_t)F + 1) * 4, s), HS(c, ((uint64_t)F + 1) * 4, s);
-        hash_partition(c, bk, bp ? bp : bk, nb, bits, rk.as<uint32_t>(), rp.as<uint32_t>(), HR.as<uint32_t>(), s);
-        hash_partition(c, pk, pp ? pp : pk, np, bits, sk.as<uint32_t>(), sp.as<uint32_t>(), HS.as<uint32_t>(), s);
+        hash_partition(c, bk, bp ? bp : bk, 4, nb, bits, rk.as<uint32_t>(), rp.get(), HR.as<uint32_t>(), s);
+        hash_partition(c, pk, pp ? pp : pk, 4, np, bits, sk.as<uint32_t>(), sp.get(), HS.as<uint32_t>(), s);
         Scratch wst(c, ((uint64_t)F + 1) * 4, s);
         launch(c, "hj_chunks", k_hj_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s,
                (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, wst.as<uint32_t>());

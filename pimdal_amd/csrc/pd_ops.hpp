@@ -15,10 +15,10 @@ void exclusive_scan_u32(pimdal_ctx* c, const uint32_t* in, uint32_t* out, uint64
                         hipStream_t s);
 
 // Stable partition of (key, payload) rows by ref_hash(key) & (2^bits - 1)
-// (onesweep passes over hash digits); bounds[f] .. bounds[f + 1] is
-// partition f, bounds has 2^bits + 1 entries.
-void hash_partition(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uint64_t n, uint32_t bits,
-                    uint32_t* kout, uint32_t* vout, uint32_t* bounds, hipStream_t s);
+// (onesweep passes over hash digits); payloads are vw = 0, 4 or 8 bytes;
+// bounds[f] .. bounds[f + 1] is partition f, bounds has 2^bits + 1 entries.
+void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t bits,
+                    uint32_t* kout, void* vout, uint32_t* bounds, hipStream_t s);
 
 uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
                    const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap, hipStream_t s);

@@ -374,8 +374,9 @@ __global__ void k_part_bounds(const uint32_t* __restrict__ k, uint64_t n, uint32
     }
 }
 
-void hash_partition(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uint64_t n, uint32_t bits,
-                    uint32_t* kout, uint32_t* vout, uint32_t* bounds, hipStream_t s) {
+void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t bits,
+                    uint32_t* kout, void* vout, uint32_t* bounds, hipStream_t s) {
+    PD_REQUIRE(vw == 0 || vw == 4 || vw == 8, PIMDAL_E_UNSUPPORTED, "partition payload must be 0, 4 or 8 bytes");
     PD_REQUIRE(bits >= 1 && bits <= 24, PIMDAL_E_INVALID, "partition bits must be in [1, 24]");
     PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "hash partition supports up to 2^30-1 rows per call");
     const uint32_t F = 1u << bits;
@@ -401,8 +402,10 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uin
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
     launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
     launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
-    if (vin)
-        run_passes<uint32_t, true, true>(c, kin, vin, n, kout, vout, run, offs, ctrs, s);
+    if (vw == 4)
+        run_passes<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
+    else if (vw == 8)
+        run_passes<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
     else
         run_passes<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
     launch(c, "part_bounds", k_part_bounds, dim3((uint32_t)std::min<uint64_t>(ceil_div(F + 1, 256), 1024)), dim3(256), 0,

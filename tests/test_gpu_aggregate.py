@@ -103,3 +103,40 @@ def test_q1_6e7_rows_full_size(ctx, orc):
     t = {c: v.numpy() for c, v in d.items()}
     _cmp_q1(ops.filter_groupby_q1(ctx, d, 2505), orc.q1_flags(t, 2505), ordered=True)
     _cmp_q1(ops.filter_groupby_q1(ctx, d, 2505, by_key=True), orc.q1_key(t, 2505), ordered=False)
+
+
+def test_q1_by_orderkey_signed_packed_fields(ctx, orc):
+    """Negative measures that fit the packed widths (int16 quantity, int32
+    price, int8 discount and tax) must sign-extend back exactly."""
+    n = 150_000
+    t = orc.fact_table(n, groups=30_000)
+    t["quantity"] = orc.gen_range(np.int64, 21, 12, n, -32768, 32767)
+    t["price"] = orc.gen_range(np.int64, 21, 13, n, -(1 << 31), (1 << 31) - 1)
+    t["discount"] = orc.gen_range(np.int64, 21, 14, n, -128, 127)
+    t["tax"] = orc.gen_range(np.int64, 21, 15, n, -128, 127)
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
+    _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
+@pytest.mark.parametrize("col,val", [("quantity", 32768), ("price", 1 << 31), ("discount", -129), ("tax", 128)])
+def test_q1_by_orderkey_one_row_beyond_packed_width(ctx, orc, col, val):
+    """One kept row outside the packed widths sends the whole call down the
+    unpacked path; the result must not change."""
+    n = 120_000
+    t = orc.fact_table(n, groups=40_000)
+    t["date"][777] = 0
+    t[col][777] = val
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
+    _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
+def test_q1_by_orderkey_few_hot_keys(ctx, orc):
+    """Few groups with many rows each: partitions larger than an LDS table's
+    row limit go to the HBM tables."""
+    n = 1 << 20
+    t = orc.fact_table(n, groups=7)
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
+    _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
