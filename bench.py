@@ -111,7 +111,7 @@ KERNELS = ("part_hist", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
-           "select", "fill_u32", "fill_u64")
+           "select", "select_count", "select_write", "q1_pack", "fill_u32", "fill_u64")
 
 
 def kernel_breakdown(ctx, steps):

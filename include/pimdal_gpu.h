@@ -73,9 +73,12 @@ int pimdal_ctx_set_profiling(pimdal_ctx* ctx, int on);
 int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
 /* Tuning knobs by name: sort_block, sort_early_lookback, select_blocks_per_cu,
- * select_lookback_width (1, 2 or 4 status words per lane), hj_mode (0 partitioned, 1 global table), hj_fanout, hj_rows_per_part,
- * hj_chunk, gb_groups_hint, gb_rows_per_part, gb_lds_max_rows,
- * serve_index_load_pct.  Unknown keys are stored and ignored. */
+ * select_lookback_width (1, 2 or 4 status words per lane), select_mode (0 one
+ * pass with decoupled look-back, 1 count pass + write pass), hj_mode (-1 by
+ * build size, 0 partitioned, 1 global table), hj_global_max_build,
+ * hj_fanout, hj_rows_per_part, hj_chunk, gb_groups_hint, gb_rows_per_part,
+ * gb_lds_max_rows, gb_packed, serve_index_load_pct.  Unknown keys are stored
+ * and ignored. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
 /* Autotuner: times candidate knob values for `op` ("sort", "select",

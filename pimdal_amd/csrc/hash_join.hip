@@ -247,7 +247,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
     // hj_mode: 1 = one global table (small build sides, whose table the
     // caches hold), 0 = radix-partitioned LDS tables; default by build size
     const int64_t mode = c->tune_get("hj_mode", -1);
-    if (mode == 1 || (mode < 0 && nb <= (uint64_t)c->tune_get("hj_global_max_build", 1 << 20)))
+    if (mode == 1 || (mode < 0 && nb <= (uint64_t)c->tune_get("hj_global_max_build", 1 << 16)))
         return hash_join_global(c, bk, bp, nb, pk, pp, np, out_p, out_b, cap, s);
     PD_REQUIRE(nb < (1ull << 30) && np < (1ull << 30), PIMDAL_E_INVALID,
                "partitioned hash join supports up to 2^30-1 rows per side");

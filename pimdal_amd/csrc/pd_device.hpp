@@ -73,8 +73,8 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 
 // ---- single-value decoupled look-back (one running total per tile) ----
 // Status word: bits 63..62 flag (0 empty, 1 tile aggregate, 2 inclusive
-// prefix), bits 61..0 value.  One wave scans back over up to 64
-// predecessors per step.
+// prefix), bits 61..0 value.  One wave scans back over 64*W predecessors
+// per step.
 constexpr uint64_t LB_FLAG_AGG = 1ull << 62;
 constexpr uint64_t LB_FLAG_PRE = 2ull << 62;
 constexpr uint64_t LB_VALUE = (1ull << 62) - 1;
@@ -84,17 +84,30 @@ constexpr uint64_t LB_VALUE = (1ull << 62) - 1;
 // prefix.  Returns the exclusive prefix (same value in all lanes).
 //
 // Each step reads a window of 64*W predecessors (lane l reads tiles
-// base - l*W - j, j < W, all loads in flight together).  When tiles finish
-// faster than one coherent load round trip per 64*W tiles, the distance to
-// the nearest published inclusive prefix grows without bound, so W sets the
-// tile rate the chain can sustain.
+// base - l*W - j, j < W, all loads in flight together), so a tile far from
+// the nearest published inclusive prefix sums 64*W aggregates per coherent
+// round trip.
+//
+// Split form: lb_publish_agg as soon as the tile's total is known (tile 0
+// publishes its inclusive prefix directly), lb_resolve later.
+__device__ __forceinline__ void lb_publish_agg(uint64_t* status, uint64_t tile, uint64_t agg) {
+    if (lane() == 0) st_relaxed(&status[tile], (tile == 0 ? LB_FLAG_PRE : LB_FLAG_AGG) | agg);
+}
+
+template <int W = 1>
+__device__ __forceinline__ uint64_t lb_resolve(uint64_t* status, uint64_t tile, uint64_t agg);
+
 template <int W = 1>
 __device__ __forceinline__ uint64_t lookback_single(uint64_t* status, uint64_t tile, uint64_t agg) {
-    if (tile == 0) {
-        if (lane() == 0) st_relaxed(&status[0], LB_FLAG_PRE | agg);
-        return 0;
-    }
-    if (lane() == 0) st_relaxed(&status[tile], LB_FLAG_AGG | agg);
+    lb_publish_agg(status, tile, agg);
+    return lb_resolve<W>(status, tile, agg);
+}
+
+// Walks back from an already published tile aggregate; publishes the
+// inclusive prefix and returns the exclusive one.
+template <int W>
+__device__ __forceinline__ uint64_t lb_resolve(uint64_t* status, uint64_t tile, uint64_t agg) {
+    if (tile == 0) return 0;
     uint64_t excl = 0;
     int64_t base = (int64_t)tile - 1;  // window covers [base - 64*W + 1, base]
     while (true) {
@@ -147,7 +160,7 @@ __device__ __forceinline__ uint64_t lookback_single(uint64_t* status, uint64_t t
 
 // ---- hashing ----
 // reference/db_operators/hash_join.cpp:20-25: low 32 bits of the
-// byte-swapped 64-bit product with 0x9E3779B97F4A7C15.
+// byte-swapped 64-bit product with 11400714785074694791 = 0x9E3779B185EBCA87.
 __device__ __forceinline__ uint32_t ref_hash(uint32_t x) {
     uint64_t p = (uint64_t)x * 11400714785074694791ull;
     return (uint32_t)__builtin_bswap64(p);

@@ -85,30 +85,87 @@ __device__ __forceinline__ bool sel_keep(K k, uint64_t ulo, uint64_t wm1) {
     }
 }
 
-// Persistent blocks take tiles in counter order, one tile ahead.  Per tile:
-// predicate + wave ranks, kept rows staged compacted in the wave's LDS slice,
-// then the next tile's loads are issued (into the registers just freed)
-// before the look-back, so HBM reads stay in flight across it; finally each
-// wave writes its kept rows with coalesced stores.  Forward progress: the
-// lowest unpublished tile is always some block's current tile (a block only
-// moves to its prefetched tile after publishing the current one), and all of
-// its predecessors are published.
+// Predicate bits of one wave's 16 rows per lane, ranks within the wave
+// (packed 16-bit group counts, one 64-bit wave scan), kept rows staged
+// compacted in the wave's LDS slice.  Returns the wave's kept-row count.
+template <typename K>
+__device__ __forceinline__ uint32_t sel_count_stage(const K (&k)[SEL_GROUPS][4], const uint32_t (&p)[SEL_GROUPS][4],
+                                                    uint32_t valid, uint64_t ulo, uint64_t wm1,
+                                                    uint32_t* __restrict__ s_pay, uint16_t* __restrict__ s_idx) {
+    const uint32_t l4 = pd::lane() * 4;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int g = 0; g < SEL_GROUPS; g++)
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            bits |= (uint32_t)((g * 256 + l4 + e < valid) && sel_keep<K>(k[g][e], ulo, wm1)) << (g * 4 + e);
+    uint64_t packed = 0;
+#pragma unroll
+    for (int g = 0; g < SEL_GROUPS; g++) packed |= (uint64_t)__popc((bits >> (g * 4)) & 0xF) << (16 * g);
+    const uint64_t incl = pd::wave_incl_scan64(packed);
+    const uint64_t tot = __shfl(incl, PD_WAVE - 1, PD_WAVE);
+    uint32_t wave_total = 0;
+#pragma unroll
+    for (int g = 0; g < SEL_GROUPS; g++) {
+        const uint32_t gb = (bits >> (g * 4)) & 0xF;
+        uint32_t pos = wave_total + ((uint32_t)(incl >> (16 * g)) & 0xFFFF) - __popc(gb);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if (gb & (1u << e)) {
+                s_pay[pos] = p[g][e];
+                s_idx[pos] = (uint16_t)(g * 256 + l4 + e);
+                pos++;
+            }
+        }
+        wave_total += (uint32_t)(tot >> (16 * g)) & 0xFFFF;
+    }
+    return wave_total;
+}
+
+// Coalesced write of a wave's staged rows to [obase, obase + wt).
+__device__ __forceinline__ void sel_write_staged(const uint32_t* __restrict__ s_pay, const uint16_t* __restrict__ s_idx,
+                                                 uint32_t wt, uint64_t obase, uint32_t row0,
+                                                 uint32_t* __restrict__ out_pay, uint32_t* __restrict__ out_rid) {
+    const uint32_t l = pd::lane();
+    if (out_pay) {
+        uint32_t* op = out_pay + obase;
+        for (uint32_t i = l; i < wt; i += PD_WAVE) op[i] = s_pay[i];
+    }
+    if (out_rid) {
+        uint32_t* orr = out_rid + obase;
+        for (uint32_t i = l; i < wt; i += PD_WAVE) orr[i] = row0 + s_idx[i];
+    }
+}
+
+__device__ __forceinline__ uint64_t sel_uniform64(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+// Single pass.  Persistent blocks take tiles in counter order.  A tile's
+// aggregate is published as soon as the tile is counted; its look-back is
+// resolved one step later, while the block's next tile is loading:
+//   take B -> load B -> resolve A (look-back) -> write A -> count B,
+//   publish B's aggregate -> A = B.
+// The current tile waits in LDS (kept rows compacted per wave) while the
+// next one is in registers.  Forward progress: tile ids are taken in
+// counter order, and a block publishes the aggregate of every tile it holds
+// before it waits on any later look-back, so every predecessor of a tile in
+// look-back has published or is about to.
 template <typename K, bool VEC, int LBW>
-__global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select(const K* __restrict__ key, const uint32_t* __restrict__ pay,
-                                                      uint64_t n, uint64_t ulo, uint64_t wm1,
-                                                      uint32_t* __restrict__ out_pay, uint32_t* __restrict__ out_rid,
-                                                      uint64_t* __restrict__ status, uint32_t* __restrict__ tile_ctr,
-                                                      uint64_t* __restrict__ d_count, uint64_t ntiles) {
+__global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select(
+    const K* __restrict__ key, const uint32_t* __restrict__ pay, uint64_t n, uint64_t ulo, uint64_t wm1,
+    uint32_t* __restrict__ out_pay, uint32_t* __restrict__ out_rid, uint64_t* __restrict__ status,
+    uint32_t* __restrict__ tile_ctr, uint64_t* __restrict__ d_count, uint64_t ntiles) {
     constexpr int WAVES = SEL_BLOCK / PD_WAVE;
-    __shared__ uint32_t s_tile[2];
+    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_tot[WAVES];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_pay[WAVES][SEL_WAVE_ROWS];
     __shared__ uint16_t s_idx[WAVES][SEL_WAVE_ROWS];
 
-    const uint32_t t = threadIdx.x, l = pd::lane(), l4 = l * 4;
+    const uint32_t t = threadIdx.x, l = pd::lane();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / PD_WAVE);
-    // rows of this wave in tile `tl` that exist (0..SEL_WAVE_ROWS), wave-uniform
     auto wave_rows = [&](uint32_t tl) -> uint32_t {
         const uint64_t wb = (uint64_t)tl * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
         return wb >= n ? 0u : (uint32_t)min((uint64_t)SEL_WAVE_ROWS, n - wb);
@@ -117,90 +174,163 @@ __global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select(co
         const uint64_t wb = (uint64_t)tl * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
         sel_load<K, VEC>(key + wb, pay + wb, wave_rows(tl), k, p);
     };
-    if (t == 0) s_tile[0] = atomicAdd(tile_ctr, 1u);
-    __syncthreads();
-    uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile[0]);
-    if (tile >= ntiles) return;
     K k[SEL_GROUPS][4];
     uint32_t p[SEL_GROUPS][4];
+
+    // ---- prologue: first tile counted, staged and published ----
+    if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+    if (tile >= ntiles) return;
     load(tile, k, p);
-    uint32_t buf = 0;
+    uint32_t wt = sel_count_stage<K>(k, p, wave_rows(tile), ulo, wm1, s_pay[w], s_idx[w]);
+    if (l == 0) s_wave_tot[w] = wt;
+    __syncthreads();
+    uint32_t woff = 0, ttot = 0;
+#pragma unroll
+    for (int i = 0; i < WAVES; i++) {
+        const uint32_t v = s_wave_tot[i];
+        woff += (i < (int)w) ? v : 0;
+        ttot += v;
+    }
+    if (w == 0) pd::lb_publish_agg(status, tile, ttot);
 
     while (true) {
-        if (t == 0) s_tile[buf ^ 1] = atomicAdd(tile_ctr, 1u);
-        const uint32_t valid = wave_rows(tile);
-        uint32_t bits = 0;
-#pragma unroll
-        for (int g = 0; g < SEL_GROUPS; g++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                bits |= (uint32_t)((g * 256 + l4 + e < valid) && sel_keep<K>(k[g][e], ulo, wm1)) << (g * 4 + e);
-
-        // ---- wave ranks: per group inclusive scan of packed 16-bit counts ----
-        uint64_t packed = 0;
-#pragma unroll
-        for (int g = 0; g < SEL_GROUPS; g++) packed |= (uint64_t)__popc((bits >> (g * 4)) & 0xF) << (16 * g);
-        const uint64_t incl = pd::wave_incl_scan64(packed);
-        const uint64_t tot = __shfl(incl, PD_WAVE - 1, PD_WAVE);
-        uint32_t wave_total = 0;
-#pragma unroll
-        for (int g = 0; g < SEL_GROUPS; g++) {
-            const uint32_t gb = (bits >> (g * 4)) & 0xF;
-            uint32_t pos = wave_total + ((uint32_t)(incl >> (16 * g)) & 0xFFFF) - __popc(gb);
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if (gb & (1u << e)) {
-                    s_pay[w][pos] = p[g][e];
-                    s_idx[w][pos] = (uint16_t)(g * 256 + l4 + e);
-                    pos++;
-                }
-            }
-            wave_total += (uint32_t)(tot >> (16 * g)) & 0xFFFF;
-        }
-        if (l == 0) s_wave_tot[w] = wave_total;
+        __syncthreads();   // every wave has read s_tile and s_wave_tot
+        if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
         __syncthreads();
-        const uint32_t next = __builtin_amdgcn_readfirstlane(s_tile[buf ^ 1]);
+        const uint32_t next = __builtin_amdgcn_readfirstlane(s_tile);
         if (next < ntiles) load(next, k, p);
 
-        uint32_t wave_off = 0, tile_total = 0;
+        if (w == 0) {
+            const uint64_t excl = pd::lb_resolve<LBW>(status, tile, ttot);
+            if (l == 0) {
+                s_excl = excl;
+                if (tile == ntiles - 1) *d_count = excl + ttot;
+            }
+        }
+        __syncthreads();
+        sel_write_staged(s_pay[w], s_idx[w], wt, sel_uniform64(s_excl + woff), tile * SEL_TILE + w * SEL_WAVE_ROWS,
+                         out_pay, out_rid);
+        if (next >= ntiles) break;
+
+        // the wave's own LDS slice is free again (its writes were issued first)
+        wt = sel_count_stage<K>(k, p, wave_rows(next), ulo, wm1, s_pay[w], s_idx[w]);
+        if (l == 0) s_wave_tot[w] = wt;
+        __syncthreads();
+        woff = 0;
+        ttot = 0;
 #pragma unroll
         for (int i = 0; i < WAVES; i++) {
             const uint32_t v = s_wave_tot[i];
-            wave_off += (i < (int)w) ? v : 0;
-            tile_total += v;
+            woff += (i < (int)w) ? v : 0;
+            ttot += v;
         }
-
-        // ---- tile offset: one wave does the look-back ----
-        if (w == 0) {
-            const uint64_t excl = pd::lookback_single<LBW>(status, tile, tile_total);
-            if (l == 0) {
-                s_excl = excl;
-                if (tile == ntiles - 1) *d_count = excl + tile_total;
-            }
-        }
-        __syncthreads();
-
-        // ---- coalesced writes of the wave's kept rows, in row order ----
-        const uint64_t ob = s_excl + wave_off;
-        const uint64_t obase = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ob >> 32)) << 32) |
-                               __builtin_amdgcn_readfirstlane((uint32_t)ob);
-        const uint32_t row0 = tile * SEL_TILE + w * SEL_WAVE_ROWS;   // n < 2^32
-        const uint32_t wt = __builtin_amdgcn_readfirstlane(wave_total);
-        if (out_pay) {
-            uint32_t* op = out_pay + obase;
-            for (uint32_t i = l; i < wt; i += PD_WAVE) op[i] = s_pay[w][i];
-        }
-        if (out_rid) {
-            uint32_t* orr = out_rid + obase;
-            for (uint32_t i = l; i < wt; i += PD_WAVE) orr[i] = row0 + s_idx[w][i];
-        }
-        // No barrier needed before the next tile: each wave restages only its
-        // own LDS slice (after its own writes), s_wave_tot is rewritten after
-        // every wave passed the second barrier, and s_excl after the first
-        // barrier of the next tile.
-        if (next >= ntiles) break;
         tile = next;
-        buf ^= 1;
+        if (w == 0) pd::lb_publish_agg(status, tile, ttot);
+    }
+}
+
+// Two passes (select_mode 1): per-block counts over contiguous row chunks,
+// a scan of the counts, then a write pass over the same chunks.
+constexpr int SEL2_CHUNK_TILES = 8;   // 4096-row tiles per block chunk
+
+template <typename K, bool VEC>
+__global__ __launch_bounds__(SEL_BLOCK) void k_select_count(const K* __restrict__ key, uint64_t n, uint64_t ulo,
+                                                            uint64_t wm1, uint32_t* __restrict__ counts) {
+    constexpr int WAVES = SEL_BLOCK / PD_WAVE;
+    __shared__ uint32_t s_w[WAVES];
+    const uint32_t t = threadIdx.x, l4 = pd::lane() * 4;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    uint32_t cnt = 0;
+    for (uint32_t j = 0; j < SEL2_CHUNK_TILES; j++) {
+        const uint64_t wb = ((uint64_t)blockIdx.x * SEL2_CHUNK_TILES + j) * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
+        const uint32_t valid = wb >= n ? 0u : (uint32_t)min((uint64_t)SEL_WAVE_ROWS, n - wb);
+        if (!valid) break;
+        const __amdgpu_buffer_rsrc_t rk = sel_rsrc(key + wb, valid * (uint32_t)sizeof(K));
+#pragma unroll
+        for (int g = 0; g < SEL_GROUPS; g++) {
+            const uint32_t r0 = g * 256 + l4;
+            K kk[4];
+            if constexpr (VEC && sizeof(K) == 4) {
+                const uint4 v = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rk, r0 * 4, 0, 0));
+                kk[0] = (K)v.x; kk[1] = (K)v.y; kk[2] = (K)v.z; kk[3] = (K)v.w;
+            } else if constexpr (VEC) {
+                const uint4 a = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rk, r0 * 8, 0, 0));
+                const uint4 b = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rk, r0 * 8 + 16, 0, 0));
+                kk[0] = (K)(((uint64_t)a.y << 32) | a.x); kk[1] = (K)(((uint64_t)a.w << 32) | a.z);
+                kk[2] = (K)(((uint64_t)b.y << 32) | b.x); kk[3] = (K)(((uint64_t)b.w << 32) | b.z);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if constexpr (sizeof(K) == 8) {
+                        const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(rk, (r0 + e) * 8, 0, 0);
+                        const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(rk, (r0 + e) * 8 + 4, 0, 0);
+                        kk[e] = (K)(((uint64_t)hi << 32) | lo);
+                    } else {
+                        kk[e] = (K)__builtin_amdgcn_raw_buffer_load_b32(rk, (r0 + e) * 4, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) cnt += (r0 + e < valid) && sel_keep<K>(kk[e], ulo, wm1);
+        }
+    }
+    cnt = pd::wave_sum(cnt);
+    if (pd::lane() == 0) s_w[w] = cnt;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < WAVES; i++) s += s_w[i];
+        counts[blockIdx.x] = s;
+    }
+}
+
+template <typename K, bool VEC>
+__global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select_write(
+    const K* __restrict__ key, const uint32_t* __restrict__ pay, uint64_t n, uint64_t ulo, uint64_t wm1,
+    const uint32_t* __restrict__ offs, uint32_t* __restrict__ out_pay, uint32_t* __restrict__ out_rid) {
+    constexpr int WAVES = SEL_BLOCK / PD_WAVE;
+    __shared__ uint32_t s_wave_tot[WAVES];
+    __shared__ uint32_t s_pay[WAVES][SEL_WAVE_ROWS];
+    __shared__ uint16_t s_idx[WAVES][SEL_WAVE_ROWS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint64_t tile0 = (uint64_t)blockIdx.x * SEL2_CHUNK_TILES;
+    uint64_t run = offs[blockIdx.x];
+    auto wave_rows = [&](uint64_t tl) -> uint32_t {
+        const uint64_t wb = tl * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
+        return wb >= n ? 0u : (uint32_t)min((uint64_t)SEL_WAVE_ROWS, n - wb);
+    };
+    K k[SEL_GROUPS][4];
+    uint32_t p[SEL_GROUPS][4];
+    if (tile0 * SEL_TILE >= n) return;
+    {
+        const uint64_t wb = tile0 * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
+        sel_load<K, VEC>(key + wb, pay + wb, wave_rows(tile0), k, p);
+    }
+    for (uint32_t j = 0; j < SEL2_CHUNK_TILES; j++) {
+        const uint64_t tl = tile0 + j;
+        if (tl * SEL_TILE >= n) break;
+        const uint32_t wt = sel_count_stage<K>(k, p, wave_rows(tl), ulo, wm1, s_pay[w], s_idx[w]);
+        if (pd::lane() == 0) s_wave_tot[w] = wt;
+        __syncthreads();
+        const uint64_t nt = tl + 1;
+        if (j + 1 < SEL2_CHUNK_TILES && nt * SEL_TILE < n) {
+            const uint64_t wb = nt * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
+            sel_load<K, VEC>(key + wb, pay + wb, wave_rows(nt), k, p);
+        }
+        uint32_t woff = 0, ttot = 0;
+#pragma unroll
+        for (int i = 0; i < WAVES; i++) {
+            const uint32_t v = s_wave_tot[i];
+            woff += (i < (int)w) ? v : 0;
+            ttot += v;
+        }
+        sel_write_staged(s_pay[w], s_idx[w], wt, sel_uniform64(run + woff), (uint32_t)(tl * SEL_TILE) + w * SEL_WAVE_ROWS,
+                         out_pay, out_rid);
+        run += ttot;
+        __syncthreads();   // s_wave_tot is rewritten by the next tile
     }
 }
 
@@ -234,11 +364,28 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
         return;
     }
     const uint64_t ntiles = ceil_div(n, SEL_TILE);
+    const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)pay % 16 == 0);
+    if (c->tune_get("select_mode", 0) == 1) {
+        // two passes: counts per block chunk, scan, write
+        const uint64_t nb = ceil_div(ntiles, (uint64_t)SEL2_CHUNK_TILES);
+        Scratch cnt(c, 4 * (nb + 1), s);
+        auto go2 = [&](auto kp) {
+            using K = std::remove_const_t<std::remove_pointer_t<decltype(kp)>>;
+            if (vec) launch(c, "select_count", k_select_count<K, true>, dim3((uint32_t)nb), dim3(SEL_BLOCK), 0, s, kp, n, ulo, wm1, cnt.as<uint32_t>());
+            else     launch(c, "select_count", k_select_count<K, false>, dim3((uint32_t)nb), dim3(SEL_BLOCK), 0, s, kp, n, ulo, wm1, cnt.as<uint32_t>());
+            exclusive_scan_u32(c, cnt.as<uint32_t>(), cnt.as<uint32_t>(), nb, d_count, s);
+            if (vec) launch(c, "select_write", k_select_write<K, true>, dim3((uint32_t)nb), dim3(SEL_BLOCK), 0, s, kp, pay, n, ulo, wm1, (const uint32_t*)cnt.as<uint32_t>(), out_pay, out_rid);
+            else     launch(c, "select_write", k_select_write<K, false>, dim3((uint32_t)nb), dim3(SEL_BLOCK), 0, s, kp, pay, n, ulo, wm1, (const uint32_t*)cnt.as<uint32_t>(), out_pay, out_rid);
+        };
+        if (kt == PIMDAL_U32) go2(reinterpret_cast<const uint32_t*>(key));
+        else if (kt == PIMDAL_I32) go2(reinterpret_cast<const int32_t*>(key));
+        else go2(reinterpret_cast<const int64_t*>(key));
+        return;
+    }
     Scratch scratch(c, 8 * (ntiles + 1), s);
     uint64_t* status = scratch.as<uint64_t>();
     uint32_t* ctr = reinterpret_cast<uint32_t*>(status + ntiles);
     fill_u64(c, status, 0, ntiles + 1, s);
-    const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)pay % 16 == 0);
     dim3 grid((uint32_t)std::min<uint64_t>(ntiles, (uint64_t)c->cu_count * (uint64_t)c->tune_get("select_blocks_per_cu", 5))),
         block(SEL_BLOCK);
     const int64_t lbw = c->tune_get("select_lookback_width", 4);

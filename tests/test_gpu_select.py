@@ -14,6 +14,15 @@ def run(ctx, k, p, lo, hi):
     return op.numpy(cnt), orow.numpy(cnt)
 
 
+@pytest.fixture(params=[0, 1], ids=["one_pass", "two_pass"], autouse=True)
+def select_mode(request, ctx):
+    """Every select test runs in both modes: one pass with decoupled
+    look-back (0) and count pass + write pass (1)."""
+    ctx.set_tune("select_mode", request.param)
+    yield request.param
+    ctx.set_tune("select_mode", 0)
+
+
 def test_select_2_20_uniform_10pct(ctx, orc):
     k, p = orc.select_workload(1 << 20)
     gp, gr = run(ctx, k, p, 0, T)
