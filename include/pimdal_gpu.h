@@ -73,7 +73,8 @@ int pimdal_ctx_set_profiling(pimdal_ctx* ctx, int on);
 int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
 /* Tuning knobs by name: sort_block, sort_early_lookback, select_blocks_per_cu,
- * select_lookback_width (1, 2 or 4 status words per lane), select_mode (0 one
+ * select_block (256, 512, 1024: 1024 rows per wave per tile),
+ * select_lookback_width (1 or 4 status words per lane), select_mode (0 one
  * pass with decoupled look-back, 1 count pass + write pass), hj_mode (-1 by
  * build size, 0 partitioned, 1 global table), hj_global_max_build,
  * hj_fanout, hj_rows_per_part, hj_chunk, gb_groups_hint, gb_rows_per_part,

@@ -293,11 +293,12 @@ template <int NI>
 struct RowsAoS {   // GbRow array written by the scatter partitioner
     static constexpr int kNI = NI;
     const GbRow<NI>* rows;
-    __device__ void load(uint64_t i, uint32_t& key, int64_t* in) const {
+    __device__ bool load(uint64_t i, uint32_t& key, int64_t* in) const {
         const GbRow<NI> r = rows[i];
         key = r.key;
 #pragma unroll
         for (int j = 0; j < NI; j++) in[j] = r.v[j];
+        return true;
     }
 };
 
@@ -305,88 +306,122 @@ struct RowsSum {   // key column + int64 value column (hash aggregation)
     static constexpr int kNI = 1;
     const uint32_t* key;
     const int64_t* val;
-    __device__ void load(uint64_t i, uint32_t& k, int64_t* in) const {
+    __device__ bool load(uint64_t i, uint32_t& k, int64_t* in) const {
         k = key[i];
         in[0] = val[i];
+        return true;
     }
 };
 
 // Q1 measures packed into one word: price (int32) in bits 0-31, quantity
-// (int16) in 32-47, discount (int8) in 48-55, tax (int8) in 56-63.
+// (int16) in 32-47, discount (int8) in 48-55, tax (int8) in 56-63.  The
+// quantity -32768 is reserved: it marks a row the date filter dropped.
+constexpr uint64_t Q1_SKIP = (uint64_t)0x8000 << 32;
 __device__ __forceinline__ uint64_t q1_pack(int64_t qty, int64_t price, int64_t disc, int64_t tax) {
     return (uint64_t)(uint32_t)(int32_t)price | ((uint64_t)(uint16_t)(int16_t)qty << 32) |
            ((uint64_t)(uint8_t)(int8_t)disc << 48) | ((uint64_t)(uint8_t)(int8_t)tax << 56);
 }
 __device__ __forceinline__ bool q1_packable(int64_t qty, int64_t price, int64_t disc, int64_t tax) {
-    return qty == (int16_t)qty && price == (int32_t)price && disc == (int8_t)disc && tax == (int8_t)tax;
+    return qty == (int16_t)qty && qty != -32768 && price == (int32_t)price && disc == (int8_t)disc &&
+           tax == (int8_t)tax;
 }
 
 struct RowsQ1Packed {
     static constexpr int kNI = 4;
     const uint32_t* key;
     const uint64_t* w;
-    __device__ void load(uint64_t i, uint32_t& k, int64_t* in) const {
+    __device__ bool load(uint64_t i, uint32_t& k, int64_t* in) const {
         k = key[i];
         const uint64_t x = w[i];
         in[0] = (int16_t)(x >> 32);
         in[1] = (int32_t)(uint32_t)x;
         in[2] = (int8_t)(x >> 48);
         in[3] = (int8_t)(x >> 56);
+        return ((x >> 32) & 0xFFFF) != 0x8000;
     }
 };
 
-// Filter + pack for the Q1 group-by on a key column: rows with
-// date <= cutoff become (key, packed measures), compacted with one counter
-// reservation per wave and 1024 rows (output order is irrelevant to the
-// aggregation).  A kept row whose measures exceed the packed widths sets
-// *overflow; the caller then runs the unpacked path instead.
-constexpr int QP_ITEMS = 16;
+// Filter + pack for the Q1 group-by on a key column, a pure map (no
+// compaction, so no shared counter): row i becomes (key, packed measures),
+// or (key, Q1_SKIP) when date > cutoff.  A kept row whose measures do not
+// fit the packed widths sets *overflow; the caller then runs the unpacked
+// path instead.  Four rows per thread per step (16-byte loads and stores).
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_q1_pack(const int32_t* __restrict__ date, const uint32_t* __restrict__ key,
                                                  const int64_t* __restrict__ qty, const int64_t* __restrict__ price,
                                                  const int64_t* __restrict__ disc, const int64_t* __restrict__ tax,
                                                  uint64_t n, int32_t cutoff, uint32_t* __restrict__ okey,
-                                                 uint64_t* __restrict__ ow, unsigned long long* __restrict__ count,
-                                                 uint32_t* __restrict__ overflow) {
-    const uint32_t l = pd::lane();
-    const uint64_t lt = pd::lanemask_lt();
-    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / PD_WAVE);
-    const uint64_t wid = (uint64_t)blockIdx.x * (blockDim.x / PD_WAVE) + threadIdx.x / PD_WAVE;
+                                                 uint64_t* __restrict__ ow, uint32_t* __restrict__ overflow) {
     bool bad = false;
-    for (uint64_t base = wid * (PD_WAVE * QP_ITEMS); base < n; base += waves * (PD_WAVE * QP_ITEMS)) {
-        uint64_t mask[QP_ITEMS];
-        uint32_t k[QP_ITEMS];
-        uint64_t w[QP_ITEMS];
-        uint32_t total = 0;
-#pragma unroll
-        for (int j = 0; j < QP_ITEMS; j++) {
-            const uint64_t i = base + (uint64_t)j * PD_WAVE + l;
-            const bool keep = i < n && date[i] <= cutoff;
-            mask[j] = pd::ballot(keep);
-            total += pd::popc64(mask[j]);
-            k[j] = 0;
-            w[j] = 0;
-            if (keep) {
-                const int64_t q = qty[i], p = price[i], d = disc[i], t = tax[i];
-                bad |= !q1_packable(q, p, d, t);
-                k[j] = key[i];
-                w[j] = q1_pack(q, p, d, t);
-            }
-        }
-        uint64_t o = 0;
-        if (l == 0 && total) o = atomicAdd(count, (unsigned long long)total);
-        o = __shfl(o, 0, PD_WAVE);
-#pragma unroll
-        for (int j = 0; j < QP_ITEMS; j++) {
-            if (mask[j] >> l & 1) {
-                const uint64_t dst = o + pd::popc64(mask[j] & lt);
-                okey[dst] = k[j];
-                ow[dst] = w[j];
-            }
-            o += pd::popc64(mask[j]);
-        }
+    auto row = [&](int32_t d, int64_t q, int64_t p, int64_t ds, int64_t tx) -> uint64_t {
+        if (d > cutoff) return Q1_SKIP;
+        bad |= !q1_packable(q, p, ds, tx);
+        return q1_pack(q, p, ds, tx);
+    };
+    const uint64_t n4 = VEC ? n / 4 : 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n4; v += stride) {
+        const uint64_t i = v * 4;
+        const int4 d = *reinterpret_cast<const int4*>(date + i);
+        const uint4 k = *reinterpret_cast<const uint4*>(key + i);
+        const longlong2 q0 = *reinterpret_cast<const longlong2*>(qty + i), q1 = *reinterpret_cast<const longlong2*>(qty + i + 2);
+        const longlong2 p0 = *reinterpret_cast<const longlong2*>(price + i), p1 = *reinterpret_cast<const longlong2*>(price + i + 2);
+        const longlong2 s0 = *reinterpret_cast<const longlong2*>(disc + i), s1 = *reinterpret_cast<const longlong2*>(disc + i + 2);
+        const longlong2 t0 = *reinterpret_cast<const longlong2*>(tax + i), t1 = *reinterpret_cast<const longlong2*>(tax + i + 2);
+        ulonglong2 w0, w1;
+        w0.x = row(d.x, q0.x, p0.x, s0.x, t0.x);
+        w0.y = row(d.y, q0.y, p0.y, s0.y, t0.y);
+        w1.x = row(d.z, q1.x, p1.x, s1.x, t1.x);
+        w1.y = row(d.w, q1.y, p1.y, s1.y, t1.y);
+        *reinterpret_cast<uint4*>(okey + i) = k;
+        *reinterpret_cast<ulonglong2*>(ow + i) = w0;
+        *reinterpret_cast<ulonglong2*>(ow + i + 2) = w1;
     }
-    if (pd::ballot(bad) && l == 0) atomicOr(overflow, 1u);
+    for (uint64_t i = n4 * 4 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        okey[i] = key[i];
+        ow[i] = row(date[i], qty[i], price[i], disc[i], tax[i]);
+    }
+    if (pd::ballot(bad) && pd::lane() == 0) atomicOr(overflow, 1u);
 }
+
+// Output policies of the per-partition kernels: put(o, key, fields).
+struct OutFields {   // key array + NF u64 field columns of stride `stride`
+    uint32_t* key;
+    uint64_t* f;
+    uint64_t stride;
+    template <int NF>
+    __device__ void put(uint64_t o, uint32_t k, const uint64_t (&v)[NF]) const {
+        key[o] = k;
+#pragma unroll
+        for (int j = 0; j < NF; j++) f[(uint64_t)j * stride + o] = v[j];
+    }
+};
+
+struct OutSum {      // hash aggregation: key, sum, count arrays
+    uint32_t* key;
+    int64_t* sum;
+    int64_t* cnt;
+    __device__ void put(uint64_t o, uint32_t k, const uint64_t (&v)[2]) const {
+        key[o] = k;
+        sum[o] = (int64_t)v[0];
+        cnt[o] = (int64_t)v[1];
+    }
+};
+
+struct OutQ1 {       // the Q1 result arrays (pimdal_q1_result)
+    int32_t* key;
+    int64_t *sq, *sp, *sd, *sdp, *sch, *cnt;
+    __device__ void put(uint64_t o, uint32_t k, const uint64_t (&v)[7]) const {
+        key[o] = (int32_t)k;
+        sq[o] = (int64_t)v[0];
+        sp[o] = (int64_t)v[1];
+        sd[o] = (int64_t)v[2];
+        sdp[o] = (int64_t)v[3];
+        sch[2 * o] = (int64_t)v[4];
+        sch[2 * o + 1] = (int64_t)v[5];
+        cnt[o] = (int64_t)v[6];
+    }
+};
 
 __global__ __launch_bounds__(512) void k_gb_hist(GbSource src, uint64_t n, uint64_t chunk, uint32_t F,
                                                  uint32_t* __restrict__ H) {
@@ -422,12 +457,15 @@ __global__ __launch_bounds__(512) void k_gb_scatter(GbSource src, uint64_t n, ui
     }
 }
 
-// gb meta: [0] n_out (u64), [1] any partition sent to the global path
-template <typename Agg, typename Src>
-__global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(Src src, const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
-                                                     uint32_t max_rows, uint32_t* __restrict__ to_global,
-                                                     uint32_t* __restrict__ out_key, uint64_t* __restrict__ out_f,
-                                                     uint64_t out_stride, uint64_t* __restrict__ meta) {
+// gb meta: [0] n_out (u64), [1] any partition sent to the global path.
+// One 1024-thread block per partition: the LDS table takes most of a CU's
+// LDS, so the block is as wide as allowed to keep 16 waves per CU.
+constexpr int GB_LDS_BLOCK = 1024;
+template <typename Agg, typename Src, typename Out>
+__global__ __launch_bounds__(GB_LDS_BLOCK) void k_gb_lds(Src src, const uint32_t* __restrict__ H, uint32_t G,
+                                                         uint32_t F, uint32_t max_rows,
+                                                         uint32_t* __restrict__ to_global, Out out,
+                                                         uint64_t* __restrict__ meta) {
     constexpr int TS = Agg::TS;
     __shared__ uint64_t s_key[TS];
     __shared__ uint64_t s_f[Agg::NF][TS];
@@ -440,7 +478,7 @@ __global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(Src src, const uint32_t* __
         if (t == 0) { to_global[p] = 1; atomicOr((uint32_t*)&meta[1], 1u); }
         return;
     }
-    for (uint32_t i = t; i < TS; i += GB_BLOCK) {
+    for (uint32_t i = t; i < TS; i += GB_LDS_BLOCK) {
         s_key[i] = 0;
 #pragma unroll
         for (int j = 0; j < Agg::NF; j++) s_f[j][i] = 0;
@@ -452,19 +490,19 @@ __global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(Src src, const uint32_t* __
     for (int j = 0; j < Agg::NF; j++) cols[j] = s_f[j];
     const uint32_t limit = TS * 3 / 4;
     constexpr int R = 4;   // rows per thread per round, loaded together
-    for (uint64_t rb = r0; rb < r1; rb += (uint64_t)GB_BLOCK * R) {
+    for (uint64_t rb = r0; rb < r1; rb += (uint64_t)GB_LDS_BLOCK * R) {
         uint32_t key[R];
         int64_t in[R][Agg::NI];
+        bool ok[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
-            const uint64_t i = rb + (uint64_t)q * GB_BLOCK + t;
-            if (i < r1) src.load(i, key[q], in[q]);
+            const uint64_t i = rb + (uint64_t)q * GB_LDS_BLOCK + t;
+            ok[q] = i < r1 && src.load(i, key[q], in[q]);
         }
         bool over = false;
 #pragma unroll
         for (int q = 0; q < R; q++) {
-            const uint64_t i = rb + (uint64_t)q * GB_BLOCK + t;
-            if (i >= r1 || over) continue;
+            if (!ok[q] || over) continue;
             uint64_t f[Agg::NF];
             Agg::fold(in[q], f);
             const uint64_t want = GB_OCC | key[q];
@@ -499,13 +537,14 @@ __global__ __launch_bounds__(GB_BLOCK) void k_gb_lds(Src src, const uint32_t* __
     // emit the occupied slots (compaction through an LDS counter)
     if (t == 0) s_used = 0;
     __syncthreads();
-    for (uint32_t i = t; i < TS; i += GB_BLOCK) {
+    for (uint32_t i = t; i < TS; i += GB_LDS_BLOCK) {
         const uint64_t kw = s_key[i];
         if (!kw) continue;
         const uint64_t o = s_base + atomicAdd(&s_used, 1u);
-        out_key[o] = (uint32_t)kw;
+        uint64_t v[Agg::NF];
 #pragma unroll
-        for (int j = 0; j < Agg::NF; j++) out_f[(uint64_t)j * out_stride + o] = s_f[j][i];
+        for (int j = 0; j < Agg::NF; j++) v[j] = s_f[j][i];
+        out.put(o, (uint32_t)kw, v);
     }
 }
 
@@ -547,7 +586,7 @@ __global__ __launch_bounds__(256) void k_gb_global(Src src, const uint32_t* __re
     for (uint64_t i = c0 + threadIdx.x; i < c1; i += 256) {
         uint32_t key;
         int64_t in[Agg::NI];
-        src.load(i, key, in);
+        if (!src.load(i, key, in)) continue;
         uint64_t f[Agg::NF];
         Agg::fold(in, f);
         const uint64_t want = GB_OCC | key;
@@ -565,19 +604,20 @@ __global__ __launch_bounds__(256) void k_gb_global(Src src, const uint32_t* __re
     }
 }
 
-template <typename Agg>
+template <typename Agg, typename Out>
 __global__ void k_gb_emit_global(const uint32_t* __restrict__ H, uint32_t G, uint32_t F,
                                  const uint32_t* __restrict__ to_global, const uint64_t* __restrict__ gkey,
-                                 const uint64_t* __restrict__ gf, uint64_t gstride, uint32_t* __restrict__ out_key,
-                                 uint64_t* __restrict__ out_f, uint64_t out_stride, uint64_t* __restrict__ meta) {
+                                 const uint64_t* __restrict__ gf, uint64_t gstride, Out out,
+                                 uint64_t* __restrict__ meta) {
     const uint64_t n2 = 2 * (uint64_t)H[(uint64_t)F * G];
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n2; s += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t kw = gkey[s];
         if (!kw) continue;
         const uint64_t o = atomicAdd((unsigned long long*)&meta[0], 1ull);
-        out_key[o] = (uint32_t)kw;
+        uint64_t v[Agg::NF];
 #pragma unroll
-        for (int j = 0; j < Agg::NF; j++) out_f[(uint64_t)j * out_stride + o] = gf[(uint64_t)j * gstride + s];
+        for (int j = 0; j < Agg::NF; j++) v[j] = gf[(uint64_t)j * gstride + s];
+        out.put(o, (uint32_t)kw, v);
     }
 }
 
@@ -712,55 +752,89 @@ static uint32_t gb_next_pow2(uint64_t x) {
     return p;
 }
 
+// k_q1_unpack (below) reshapes staged engine fields into the Q1 arrays.
+__global__ void k_q1_unpack(uint64_t g, const uint32_t* __restrict__ key, const uint64_t* __restrict__ f,
+                            uint64_t stride, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
+                            int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
+                            int64_t* __restrict__ sch, int64_t* __restrict__ cnt);
+
+// Copies `groups` staged rows (OutFields layout, stride `stride`) to the
+// caller's output.
+template <int NF>
+void finish_staged(pimdal_ctx* c, const OutFields& dst, const uint32_t* sk, const uint64_t* sf, uint64_t stride,
+                   uint64_t groups, hipStream_t s) {
+    copy_d2d(c, dst.key, sk, groups * 4, s);
+    for (int j = 0; j < NF; j++) copy_d2d(c, dst.f + (uint64_t)j * dst.stride, sf + (uint64_t)j * stride, groups * 8, s);
+}
+template <int NF>
+void finish_staged(pimdal_ctx* c, const OutSum& dst, const uint32_t* sk, const uint64_t* sf, uint64_t stride,
+                   uint64_t groups, hipStream_t s) {
+    copy_d2d(c, dst.key, sk, groups * 4, s);
+    copy_d2d(c, dst.sum, sf, groups * 8, s);
+    copy_d2d(c, dst.cnt, sf + stride, groups * 8, s);
+}
+template <int NF>
+void finish_staged(pimdal_ctx* c, const OutQ1& dst, const uint32_t* sk, const uint64_t* sf, uint64_t stride,
+                   uint64_t groups, hipStream_t s) {
+    if (groups)
+        launch(c, "q1_unpack", k_q1_unpack, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(groups, 256), 4096))),
+               dim3(256), 0, s, groups, sk, sf, stride, dst.key, dst.sq, dst.sp, dst.sd, dst.sdp, dst.sch, dst.cnt);
+}
+
 // Per-partition aggregation of partitioned rows (partition p spans
 // [H[p*G], H[(p+1)*G])): one LDS table per partition, partitions that do
-// not fit go to HBM tables.  Output as group_by; returns the group count.
-template <typename Agg, typename Src>
-uint64_t aggregate_parts(pimdal_ctx* c, Src src, const uint32_t* H, uint32_t G, uint32_t F, uint64_t nsel,
-                         uint32_t* out_key, uint64_t* out_f, uint64_t cap, hipStream_t s) {
+// not fit go to HBM tables.  Groups are written through `out` when it can
+// hold one group per row (cap >= nrows), else staged and copied once the
+// count is known.  Returns the group count.
+template <typename Agg, typename Src, typename Out>
+uint64_t aggregate_parts(pimdal_ctx* c, Src src, const uint32_t* H, uint32_t G, uint32_t F, uint64_t nrows,
+                         const Out& out, uint64_t cap, hipStream_t s) {
     Scratch meta(c, 8 * 4, s);
     fill_u64(c, meta.get(), 0, 4, s);
     Scratch tog(c, (uint64_t)F * 4, s);
     fill_u32(c, tog.get(), 0, F, s);
     const uint32_t max_rows = (uint32_t)c->tune_get("gb_lds_max_rows", 65536);
-    // the emit writes at most one group per row, so a selected-row-sized
-    // staging area bounds the output before the capacity check
-    const bool direct_out = cap >= nsel;
-    Scratch sk(c, direct_out ? 0 : nsel * 4, s), sf(c, direct_out ? 0 : nsel * 8 * Agg::NF, s);
-    uint32_t* ok = direct_out ? out_key : sk.as<uint32_t>();
-    uint64_t* of = direct_out ? out_f : sf.as<uint64_t>();
-    const uint64_t ostride = direct_out ? cap : nsel;
-    launch(c, "gb_lds", k_gb_lds<Agg, Src>, dim3(F), dim3(GB_BLOCK), 0, s, src, H, G, F, max_rows, tog.as<uint32_t>(),
-           ok, of, ostride, meta.as<uint64_t>());
-    uint64_t m[4];
-    PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
-    if (m[1]) {
-        const uint32_t CH = 16384;
-        Scratch wst(c, ((uint64_t)F + 1) * 4, s);
-        launch(c, "gb_chunks", k_gb_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s, H, G, F,
-               CH, (const uint32_t*)tog.as<uint32_t>(), wst.as<uint32_t>());
-        exclusive_scan_u32(c, wst.as<uint32_t>(), wst.as<uint32_t>(), F, nullptr, s);
-        const uint64_t slots = 2 * nsel;
-        Scratch gk(c, slots * 8, s), gfv(c, slots * 8 * Agg::NF, s);
-        fill_u64(c, gk.get(), 0, slots, s);
-        fill_u64(c, gfv.get(), 0, slots * Agg::NF, s);
-        const uint64_t max_wi = F + ceil_div(nsel, CH);
-        launch(c, "gb_global", k_gb_global<Agg, Src>, dim3((uint32_t)max_wi), dim3(256), 0, s, src, H, G, F, CH,
-               (const uint32_t*)wst.as<uint32_t>(), gk.as<uint64_t>(), gfv.as<uint64_t>(), slots);
-        launch(c, "gb_emit_global", k_gb_emit_global<Agg>, dim3((uint32_t)std::min<uint64_t>(ceil_div(slots, 256), 4096)),
-               dim3(256), 0, s, H, G, F, (const uint32_t*)tog.as<uint32_t>(), (const uint64_t*)gk.as<uint64_t>(),
-               (const uint64_t*)gfv.as<uint64_t>(), slots, ok, of, ostride, meta.as<uint64_t>());
+    const bool direct_out = cap >= nrows;
+    Scratch sk(c, direct_out ? 0 : nrows * 4, s), sf(c, direct_out ? 0 : nrows * 8 * Agg::NF, s);
+    auto run = [&](const auto& o) {
+        using O = std::decay_t<decltype(o)>;
+        launch(c, "gb_lds", k_gb_lds<Agg, Src, O>, dim3(F), dim3(GB_LDS_BLOCK), 0, s, src, H, G, F, max_rows,
+               tog.as<uint32_t>(), o, meta.as<uint64_t>());
+        uint64_t m[4];
         PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
+        if (m[1]) {
+            const uint32_t CH = 16384;
+            Scratch wst(c, ((uint64_t)F + 1) * 4, s);
+            launch(c, "gb_chunks", k_gb_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s, H, G,
+                   F, CH, (const uint32_t*)tog.as<uint32_t>(), wst.as<uint32_t>());
+            exclusive_scan_u32(c, wst.as<uint32_t>(), wst.as<uint32_t>(), F, nullptr, s);
+            const uint64_t slots = 2 * nrows;
+            Scratch gk(c, slots * 8, s), gfv(c, slots * 8 * Agg::NF, s);
+            fill_u64(c, gk.get(), 0, slots, s);
+            fill_u64(c, gfv.get(), 0, slots * Agg::NF, s);
+            const uint64_t max_wi = F + ceil_div(nrows, CH);
+            launch(c, "gb_global", k_gb_global<Agg, Src>, dim3((uint32_t)max_wi), dim3(256), 0, s, src, H, G, F, CH,
+                   (const uint32_t*)wst.as<uint32_t>(), gk.as<uint64_t>(), gfv.as<uint64_t>(), slots);
+            launch(c, "gb_emit_global", k_gb_emit_global<Agg, O>,
+                   dim3((uint32_t)std::min<uint64_t>(ceil_div(slots, 256), 4096)), dim3(256), 0, s, H, G, F,
+                   (const uint32_t*)tog.as<uint32_t>(), (const uint64_t*)gk.as<uint64_t>(),
+                   (const uint64_t*)gfv.as<uint64_t>(), slots, o, meta.as<uint64_t>());
+            PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
+            PD_HIP(hipStreamSynchronize(s));
+        }
+        return m[0];
+    };
+    if (direct_out) {
+        const uint64_t groups = run(out);
+        PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+        return groups;
     }
-    const uint64_t groups = m[0];
+    const OutFields stage{sk.as<uint32_t>(), sf.as<uint64_t>(), nrows};
+    const uint64_t groups = run(stage);
     PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-    if (!direct_out) {
-        copy_d2d(c, out_key, ok, groups * 4, s);
-        for (int j = 0; j < Agg::NF; j++) copy_d2d(c, out_f + (uint64_t)j * cap, of + (uint64_t)j * nsel, groups * 8, s);
-        PD_HIP(hipStreamSynchronize(s));
-    }
+    finish_staged<Agg::NF>(c, out, sk.as<uint32_t>(), sf.as<uint64_t>(), nrows, groups, s);
+    PD_HIP(hipStreamSynchronize(s));
     return groups;
 }
 
@@ -775,16 +849,15 @@ static uint32_t gb_part_bits(pimdal_ctx* c, uint64_t n, int64_t dflt_rows) {
 // Group-by over rows already projected to (key column, Src): partitioned
 // by hash digits with the onesweep passes (long sequential runs on the
 // write side), then aggregate_parts.
-template <typename Agg, typename Src, typename MakeSrc>
+template <typename Agg, typename Src, typename MakeSrc, typename Out>
 uint64_t group_by_onesweep(pimdal_ctx* c, const uint32_t* key, const void* payload, uint32_t vw, uint64_t n,
-                           MakeSrc make_src, uint32_t* out_key, uint64_t* out_f, uint64_t cap, hipStream_t s) {
+                           MakeSrc make_src, const Out& out, uint64_t cap, hipStream_t s) {
     if (n == 0) return 0;
     const uint32_t bits = gb_part_bits(c, n, (int64_t)Agg::TS * 2);
     const uint32_t F = 1u << bits;
     Scratch pk(c, n * 4, s), pv(c, n * vw, s), bounds(c, ((uint64_t)F + 1) * 4, s);
     hash_partition(c, key, payload, vw, n, bits, pk.as<uint32_t>(), pv.get(), bounds.as<uint32_t>(), s);
-    return aggregate_parts<Agg>(c, make_src(pk.as<uint32_t>(), pv.get()), bounds.as<uint32_t>(), 1u, F, n, out_key,
-                                out_f, cap, s);
+    return aggregate_parts<Agg>(c, make_src(pk.as<uint32_t>(), pv.get()), bounds.as<uint32_t>(), 1u, F, n, out, cap, s);
 }
 
 // Low-cardinality attempt: per-block LDS tables merged into one small HBM
@@ -840,8 +913,8 @@ uint64_t group_by(pimdal_ctx* c, const GbSource& src, uint64_t n, uint32_t* out_
     Scratch rows(c, std::max<uint64_t>(nsel, 1) * sizeof(GbRow<Agg::NI>), s);
     launch(c, "gb_scatter", k_gb_scatter<Agg::NI>, dim3(G), dim3(512), F * 4, s, src, n, chunk, F,
            (const uint32_t*)H.as<uint32_t>(), rows.as<GbRow<Agg::NI>>(), nsel);
-    return aggregate_parts<Agg>(c, RowsAoS<Agg::NI>{rows.as<GbRow<Agg::NI>>()}, H.as<uint32_t>(), G, F, nsel, out_key,
-                                out_f, cap, s);
+    return aggregate_parts<Agg>(c, RowsAoS<Agg::NI>{rows.as<GbRow<Agg::NI>>()}, H.as<uint32_t>(), G, F, nsel,
+                                OutFields{out_key, out_f, cap}, cap, s);
 }
 
 // ---- reshape engine output (u64 fields) into the Q1 result arrays ----
@@ -913,32 +986,38 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
     src.date = date;
     src.cutoff = cutoff;
     src.in[0] = qty; src.in[1] = price; src.in[2] = disc; src.in[3] = tax;
-    PD_REQUIRE(n < (1ull << 31), PIMDAL_E_INVALID, "group-by supports up to 2^31-1 rows per call");
-    const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
-    Scratch k32(c, stage * 4, s), fs(c, stage * 8 * AggQ1::NF, s);
-    uint64_t g = 0;
-    bool done = false;
+    PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "group-by supports up to 2^30-1 rows per call");
     if (key && c->tune_get("gb_packed", 1)) {
-        // filter + pack to (key, 8-byte measures), then onesweep partitions;
-        // measures that do not fit the packed widths take the general path
-        Scratch pk(c, n * 4, s), pw(c, n * 8, s), ctl(c, 16, s);
-        fill_u64(c, ctl.get(), 0, 2, s);
-        const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * QP_ITEMS), (uint64_t)c->cu_count * 8));
-        launch(c, "q1_pack", k_q1_pack, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key), qty,
-               price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ctl.as<unsigned long long>(),
-               reinterpret_cast<uint32_t*>(ctl.as<uint64_t>() + 1));
-        uint64_t cv[2];
-        PD_HIP(hipMemcpyAsync(cv, ctl.get(), 16, hipMemcpyDeviceToHost, s));
+        // filter + pack to (key, 8-byte measures) in place of the rows, then
+        // onesweep partitions and per-partition LDS tables writing the Q1
+        // arrays directly; measures beyond the packed widths take the
+        // general path below
+        Scratch pk(c, n * 4, s), pw(c, n * 8, s), ovf(c, 8, s);
+        fill_u32(c, ovf.get(), 0, 1, s);
+        const bool vec = ((uintptr_t)date | (uintptr_t)key | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc |
+                          (uintptr_t)tax) % 16 == 0;
+        const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 4), (uint64_t)c->cu_count * 8));
+        if (vec)
+            launch(c, "q1_pack", k_q1_pack<true>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
+                   qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
+        else
+            launch(c, "q1_pack", k_q1_pack<false>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
+                   qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
+        uint32_t overflow = 0;
+        PD_HIP(hipMemcpyAsync(&overflow, ovf.get(), 4, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
-        if (!(uint32_t)cv[1]) {
-            g = group_by_onesweep<AggQ1, RowsQ1Packed>(
-                c, pk.as<uint32_t>(), pw.get(), 8, cv[0],
-                [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; }, k32.as<uint32_t>(),
-                fs.as<uint64_t>(), stage, s);
-            done = true;
+        if (!overflow) {
+            const OutQ1 o{out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count};
+            const uint64_t g = group_by_onesweep<AggQ1, RowsQ1Packed>(
+                c, pk.as<uint32_t>(), pw.get(), 8, n, [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; },
+                o, cap, s);
+            PD_HIP(hipStreamSynchronize(s));
+            return g;
         }
     }
-    if (!done) g = group_by<AggQ1>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
+    const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
+    Scratch k32(c, stage * 4, s), fs(c, stage * 8 * AggQ1::NF, s);
+    const uint64_t g = group_by<AggQ1>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, s);
     PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     launch(c, "q1_unpack", k_q1_unpack, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(g, 256), 4096))),
            dim3(256), 0, s, g, (const uint32_t*)k32.as<uint32_t>(), (const uint64_t*)fs.as<uint64_t>(), stage,
@@ -968,19 +1047,28 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
     GbSource src{};
     src.key = key;
     src.in[0] = val;
-    const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
-    Scratch k32(c, stage * 4, s), fs(c, stage * 16, s);
     uint64_t g = 0;
-    // few groups: per-block LDS tables; otherwise onesweep partitions of
-    // (key, value) by hash digits and one LDS table per partition
-    if (!group_by_direct<AggSum>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, &g, s))
-        g = group_by_onesweep<AggSum, RowsSum>(
-            c, key, val, 8, n, [](uint32_t* k, void* v) { return RowsSum{k, (const int64_t*)v}; }, k32.as<uint32_t>(),
-            fs.as<uint64_t>(), stage, s);
-    PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-    copy_d2d(c, out_key, k32.get(), g * 4, s);
-    copy_d2d(c, out_sum, fs.get(), g * 8, s);
-    copy_d2d(c, out_cnt, fs.as<uint64_t>() + stage, g * 8, s);
+    {
+        // few groups: per-block LDS tables merged in one small HBM table
+        const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
+        const int64_t hint = c->tune_get("gb_groups_hint", 0);
+        if (hint == 0 || hint <= (int64_t)(AggSum::TS / 4)) {
+            Scratch k32(c, stage * 4, s), fs(c, stage * 16, s);
+            if (group_by_direct<AggSum>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, &g, s)) {
+                PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+                copy_d2d(c, out_key, k32.get(), g * 4, s);
+                copy_d2d(c, out_sum, fs.get(), g * 8, s);
+                copy_d2d(c, out_cnt, fs.as<uint64_t>() + stage, g * 8, s);
+                PD_HIP(hipStreamSynchronize(s));
+                return g;
+            }
+        }
+    }
+    // otherwise onesweep partitions of (key, value) by hash digits and one
+    // LDS table per partition, writing the outputs directly
+    g = group_by_onesweep<AggSum, RowsSum>(
+        c, key, val, 8, n, [](uint32_t* k, void* v) { return RowsSum{k, (const int64_t*)v}; },
+        OutSum{out_key, out_sum, out_cnt}, cap, s);
     PD_HIP(hipStreamSynchronize(s));
     return g;
 }

@@ -95,7 +95,7 @@ extern "C" int pimdal_autotune(pimdal_ctx* c, const char* op, uint64_t n, const 
         Scratch k(c, n * 4), p(c, n * 4), op_(c, n * 4), orow(c, n * 4), cnt(c, 8);
         fill(c, k.get(), n, 3, 0);
         fill(c, p.get(), n, 4, 0);
-        knobs = {{"select_mode", {0, 1}}, {"select_blocks_per_cu", {4, 5, 8}}, {"select_lookback_width", {1, 2, 4}}};
+        knobs = {{"select_mode", {0, 1}}, {"select_block", {256, 512, 1024}}, {"select_lookback_width", {1, 4}}};
         tune(c, knobs, [&] {
             select_range(c, PIMDAL_U32, k.get(), p.as<uint32_t>(), n, 0, 429496730, op_.as<uint32_t>(),
                          orow.as<uint32_t>(), cnt.as<uint64_t>(), s);

@@ -68,6 +68,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
     __shared__ uint2 tab[HJ_TS];
     __shared__ uint32_t occ[HJ_TS / 32];
     __shared__ uint32_t s_wsum[HJ_BLOCK / PD_WAVE];
+    __shared__ uint32_t s_kc[HJ_ROWS * (HJ_BLOCK / PD_WAVE)];
     __shared__ uint32_t s_f, s_bad;
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
@@ -150,6 +151,40 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
             } else {
                 cnt += m[k];
             }
+        }
+        if constexpr (PHASE == HJ_UNIQUE) {
+            // at most one match per row: output slots in row order
+            // (row = rb + k * HJ_BLOCK + wave * 64 + lane), so every store
+            // instruction writes consecutive slots
+            constexpr int WAVES = HJ_BLOCK / PD_WAVE;
+            static_assert(HJ_ROWS * WAVES == 2 * PD_WAVE, "one wave scans the (row group, wave) counts");
+#pragma unroll
+            for (int k = 0; k < HJ_ROWS; k++) {
+                const uint64_t b = pd::ballot(m[k] != 0);
+                if (l == 0) s_kc[k * WAVES + w] = pd::popc64(b);
+            }
+            __syncthreads();
+            if (w == 0) {
+                const uint32_t a0 = s_kc[2 * l], a1 = s_kc[2 * l + 1];
+                const uint32_t inc = pd::wave_incl_scan(a0 + a1);
+                s_kc[2 * l] = inc - a0 - a1;
+                s_kc[2 * l + 1] = inc - a1;
+                if (l == PD_WAVE - 1) s_wsum[0] = inc;
+            }
+            __syncthreads();
+            const uint32_t tot = s_wsum[0];
+            const uint64_t lt = pd::lanemask_lt();
+#pragma unroll
+            for (int k = 0; k < HJ_ROWS; k++) {
+                const uint64_t b = pd::ballot(m[k] != 0);
+                if (m[k]) {
+                    const uint64_t o = wbase + written + s_kc[k * WAVES + w] + pd::popc64(b & lt);
+                    if (o < out_cap) { out_p[o] = pay[k]; out_b[o] = first[k]; }
+                }
+            }
+            written += tot;
+            __syncthreads();   // s_kc / s_wsum are rewritten by the next round
+            continue;
         }
         // block exclusive scan of cnt
         const uint32_t inc = pd::wave_incl_scan(cnt);

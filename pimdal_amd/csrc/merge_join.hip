@@ -13,8 +13,8 @@
 // build keys are unique, one kernel merges: each 4096-row tile of the
 // sorted probe side finds the build range it can match with two binary
 // searches, caches that range's keys in LDS, searches each probe key there,
-// and compacts the matches with a decoupled look-back, so the output is in
-// probe order.  With repeated build keys: per probe row the match range
+// and writes the matches at offsets from a decoupled look-back, so the
+// output is in probe order.  With repeated build keys: per probe row the match range
 // [lower, upper), a device scan of the range lengths, then a write pass.
 #include <algorithm>
 
@@ -28,7 +28,7 @@ constexpr int SM_BLOCK = 256;
 constexpr int SM_ITEMS = 16;
 constexpr int SM_TILE = SM_BLOCK * SM_ITEMS;  // 4096
 constexpr int SM_WAVES = SM_BLOCK / PD_WAVE;
-constexpr uint32_t SM_RCAP = 8192;            // build keys cached per tile
+constexpr uint32_t SM_RCAP = 2048;            // build keys cached per tile
 
 __device__ __forceinline__ uint64_t lower_bound_u32(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t x) {
     while (lo < hi) {
@@ -51,6 +51,13 @@ __global__ void k_adjacent_dup(const uint32_t* __restrict__ k, uint64_t n, uint3
         if (k[i] == k[i - 1]) { *flag = 1; return; }
 }
 
+// Unique build keys: one 4096-row tile of the sorted probe side per block.
+// Probe keys and payloads are loaded while the tile's build-key range is
+// cached in LDS; each key is searched there (binary search in HBM when the
+// range exceeds the cache); build payloads are gathered before the
+// look-back; matches are written straight from registers in
+// (item, wave, lane) = probe-row order, so every store instruction writes
+// consecutive output slots.
 __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rp,
                                                          uint64_t nr, const uint32_t* __restrict__ sk,
                                                          const uint32_t* __restrict__ sp, uint64_t ns,
@@ -62,9 +69,6 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_rk[SM_RCAP];
     __shared__ uint32_t s_cnt[SM_ITEMS][SM_WAVES];
-    __shared__ uint32_t s_total;
-    __shared__ uint32_t s_op[SM_TILE];
-    __shared__ uint32_t s_ob[SM_TILE];
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
@@ -72,36 +76,42 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
     const uint64_t tile = s_tile;
     const uint64_t base = tile * SM_TILE;
     const uint64_t end = min(ns, base + SM_TILE);
+    // probe rows base + i*BLOCK + t: row order is (i, wave, lane)
+    uint32_t key[SM_ITEMS], pay[SM_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SM_ITEMS; i++) {
+        const uint64_t j = base + (uint64_t)i * SM_BLOCK + t;
+        key[i] = j < end ? sk[j] : 0u;
+        pay[i] = j < end ? sp[j] : 0u;
+    }
     const uint64_t lo = ranges[2 * tile], hi = ranges[2 * tile + 1];
     const bool cached = (hi - lo) <= SM_RCAP;
     if (cached)
         for (uint64_t i = lo + t; i < hi; i += SM_BLOCK) s_rk[i - lo] = rk[i];
     __syncthreads();
 
-    // probe rows base + i*BLOCK + t: row order is (i, wave, lane)
-    uint32_t pos[SM_ITEMS];
+    uint32_t bval[SM_ITEMS];
     uint32_t hit = 0;
 #pragma unroll
     for (int i = 0; i < SM_ITEMS; i++) {
         const uint64_t j = base + (uint64_t)i * SM_BLOCK + t;
         bool m = false;
-        pos[i] = 0;
+        bval[i] = 0;
         if (j < end) {
-            const uint32_t key = sk[j];
             uint64_t p;
             if (cached) {
                 uint32_t a = 0, b = (uint32_t)(hi - lo);
                 while (a < b) {
                     const uint32_t mid = (a + b) >> 1;
-                    if (s_rk[mid] < key) a = mid + 1; else b = mid;
+                    if (s_rk[mid] < key[i]) a = mid + 1; else b = mid;
                 }
-                m = a < (uint32_t)(hi - lo) && s_rk[a] == key;
+                m = a < (uint32_t)(hi - lo) && s_rk[a] == key[i];
                 p = lo + a;
             } else {
-                p = lower_bound_u32(rk, lo, hi, key);
-                m = p < hi && rk[p] == key;
+                p = lower_bound_u32(rk, lo, hi, key[i]);
+                m = p < hi && rk[p] == key[i];
             }
-            pos[i] = (uint32_t)p;
+            if (m) bval[i] = rp[p];
         }
         const uint64_t bal = pd::ballot(m);
         hit |= (uint32_t)m << i;
@@ -113,35 +123,26 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
         const uint32_t v = (&s_cnt[0][0])[l];
         const uint32_t inc = pd::wave_incl_scan(v);
         (&s_cnt[0][0])[l] = inc - v;
-        if (l == PD_WAVE - 1) s_total = inc;
-    }
-    __syncthreads();
-    const uint32_t total = s_total;
-    if (w == 0) {
+        const uint32_t total = __shfl(inc, PD_WAVE - 1, PD_WAVE);
         const uint64_t e = pd::lookback_single<4>(status, tile, total);
         if (l == 0) {
             s_excl = e;
             if (tile == ntiles - 1) *d_count = e + total;
         }
     }
+    __syncthreads();
+    const uint64_t excl = s_excl;
     const uint64_t lt = pd::lanemask_lt();
 #pragma unroll
     for (int i = 0; i < SM_ITEMS; i++) {
         const bool m = (hit >> i) & 1;
         const uint64_t bal = pd::ballot(m);
         if (m) {
-            const uint32_t o = s_cnt[i][w] + pd::popc64(bal & lt);
-            const uint64_t j = base + (uint64_t)i * SM_BLOCK + t;
-            s_op[o] = sp[j];
-            s_ob[o] = rp[pos[i]];
-        }
-    }
-    __syncthreads();
-    const uint64_t excl = s_excl;
-    for (uint32_t j = t; j < total; j += SM_BLOCK) {
-        if (excl + j < ns) {
-            out_p[excl + j] = s_op[j];
-            out_b[excl + j] = s_ob[j];
+            const uint64_t o = excl + s_cnt[i][w] + pd::popc64(bal & lt);
+            if (o < ns) {
+                out_p[o] = pay[i];
+                out_b[o] = bval[i];
+            }
         }
     }
 }

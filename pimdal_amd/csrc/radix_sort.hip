@@ -45,13 +45,19 @@ __device__ __forceinline__ uint32_t rs_src(uint32_t k) {
 }
 
 // ---- histogram of every pass + AND/OR of the keys ----
+// Lanes spread their LDS atomics over RS_HCOPY copies of the histograms
+// (copy = lane mod RS_HCOPY), so lanes of one instruction that hit the same
+// bin are fewer (few-bin digits would otherwise serialize on one address).
+constexpr int RS_HCOPY = 4;
+
 template <bool HASH>
 __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n, RsPassDesc pd,
                                                  uint32_t* __restrict__ g_hist, uint32_t* __restrict__ g_andor) {
-    __shared__ uint32_t h[4][RS_BINS];
+    __shared__ uint32_t h[RS_HCOPY][4][RS_BINS];
     const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < 4 * RS_BINS; i += blockDim.x) (&h[0][0])[i] = 0;
+    for (uint32_t i = t; i < RS_HCOPY * 4 * RS_BINS; i += blockDim.x) (&h[0][0][0])[i] = 0;
     __syncthreads();
+    auto& hc = h[pd::lane() % RS_HCOPY];
     uint32_t a = 0xFFFFFFFFu, o = 0;
     const uint64_t nvec = ((uintptr_t)keys % 16 == 0) ? n / 4 : 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -63,7 +69,7 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
             a &= k4[e];
             o |= k4[e];
             const uint32_t src = rs_src<HASH>(k4[e]);
-            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
+            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&hc[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
         }
     }
     for (uint64_t i = nvec * 4 + blockIdx.x * (uint64_t)blockDim.x + t; i < n; i += stride) {
@@ -71,11 +77,13 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
         a &= k;
         o |= k;
         const uint32_t src = rs_src<HASH>(k);
-        for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
+        for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&hc[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
     }
     __syncthreads();
     for (uint32_t i = t; i < pd.npass * RS_BINS; i += blockDim.x) {
-        const uint32_t v = (&h[0][0])[i];
+        uint32_t v = 0;
+#pragma unroll
+        for (int cpy = 0; cpy < RS_HCOPY; cpy++) v += (&h[cpy][0][0])[i];
         if (v) atomicAdd(&g_hist[i], v);
     }
     // wave AND/OR, one atomic per wave

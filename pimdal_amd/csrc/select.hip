@@ -152,12 +152,13 @@ __device__ __forceinline__ uint64_t sel_uniform64(uint64_t v) {
 // counter order, and a block publishes the aggregate of every tile it holds
 // before it waits on any later look-back, so every predecessor of a tile in
 // look-back has published or is about to.
-template <typename K, bool VEC, int LBW>
-__global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select(
+template <typename K, bool VEC, int LBW, int BLOCK>
+__global__ __launch_bounds__(BLOCK, (sizeof(K) == 8 || BLOCK == 1024) ? 4 : 5) void k_select(
     const K* __restrict__ key, const uint32_t* __restrict__ pay, uint64_t n, uint64_t ulo, uint64_t wm1,
     uint32_t* __restrict__ out_pay, uint32_t* __restrict__ out_rid, uint64_t* __restrict__ status,
     uint32_t* __restrict__ tile_ctr, uint64_t* __restrict__ d_count, uint64_t ntiles) {
-    constexpr int WAVES = SEL_BLOCK / PD_WAVE;
+    constexpr int WAVES = BLOCK / PD_WAVE;
+    constexpr int TILE = WAVES * SEL_WAVE_ROWS;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_tot[WAVES];
     __shared__ uint64_t s_excl;
@@ -167,11 +168,11 @@ __global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select(
     const uint32_t t = threadIdx.x, l = pd::lane();
     const uint32_t w = __builtin_amdgcn_readfirstlane(t / PD_WAVE);
     auto wave_rows = [&](uint32_t tl) -> uint32_t {
-        const uint64_t wb = (uint64_t)tl * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
+        const uint64_t wb = (uint64_t)tl * TILE + (uint64_t)w * SEL_WAVE_ROWS;
         return wb >= n ? 0u : (uint32_t)min((uint64_t)SEL_WAVE_ROWS, n - wb);
     };
     auto load = [&](uint32_t tl, K (&k)[SEL_GROUPS][4], uint32_t (&p)[SEL_GROUPS][4]) {
-        const uint64_t wb = (uint64_t)tl * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
+        const uint64_t wb = (uint64_t)tl * TILE + (uint64_t)w * SEL_WAVE_ROWS;
         sel_load<K, VEC>(key + wb, pay + wb, wave_rows(tl), k, p);
     };
     K k[SEL_GROUPS][4];
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select(
             }
         }
         __syncthreads();
-        sel_write_staged(s_pay[w], s_idx[w], wt, sel_uniform64(s_excl + woff), tile * SEL_TILE + w * SEL_WAVE_ROWS,
+        sel_write_staged(s_pay[w], s_idx[w], wt, sel_uniform64(s_excl + woff), tile * TILE + w * SEL_WAVE_ROWS,
                          out_pay, out_rid);
         if (next >= ntiles) break;
 
@@ -382,23 +383,25 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
         else go2(reinterpret_cast<const int64_t*>(key));
         return;
     }
-    Scratch scratch(c, 8 * (ntiles + 1), s);
+    // one pass: tile = block size / 64 waves x 1024 rows
+    const int64_t bsel = c->tune_get("select_block", 256);
+    const int BLK = bsel >= 1024 ? 1024 : bsel >= 512 ? 512 : 256;
+    const uint64_t ntl = ceil_div(n, (uint64_t)(BLK / PD_WAVE) * SEL_WAVE_ROWS);
+    Scratch scratch(c, 8 * (ntl + 1), s);
     uint64_t* status = scratch.as<uint64_t>();
-    uint32_t* ctr = reinterpret_cast<uint32_t*>(status + ntiles);
-    fill_u64(c, status, 0, ntiles + 1, s);
-    dim3 grid((uint32_t)std::min<uint64_t>(ntiles, (uint64_t)c->cu_count * (uint64_t)c->tune_get("select_blocks_per_cu", 5))),
-        block(SEL_BLOCK);
-    const int64_t lbw = c->tune_get("select_lookback_width", 4);
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(status + ntl);
+    fill_u64(c, status, 0, ntl + 1, s);
+    const int64_t per_cu = c->tune_get("select_blocks_per_cu", BLK == 256 ? 5 : BLK == 512 ? 3 : 1);
+    dim3 grid((uint32_t)std::min<uint64_t>(ntl, (uint64_t)c->cu_count * (uint64_t)per_cu));
+    const int64_t lbw = c->tune_get("select_lookback_width", 1);
     auto go = [&](auto kp) {
         using K = std::remove_const_t<std::remove_pointer_t<decltype(kp)>>;
-#define PD_SEL_LAUNCH(V, W) launch(c, "select", k_select<K, V, W>, grid, block, 0, s, kp, pay, n, ulo, wm1, out_pay, out_rid, status, ctr, d_count, ntiles)
-        if (vec) {
-            if (lbw >= 4) PD_SEL_LAUNCH(true, 4);
-            else if (lbw >= 2) PD_SEL_LAUNCH(true, 2);
-            else PD_SEL_LAUNCH(true, 1);
-        } else {
-            PD_SEL_LAUNCH(false, 4);
-        }
+#define PD_SEL_LAUNCH(V, W, B) launch(c, "select", k_select<K, V, W, B>, grid, dim3(B), 0, s, kp, pay, n, ulo, wm1, out_pay, out_rid, status, ctr, d_count, ntl)
+        if (!vec) PD_SEL_LAUNCH(false, 1, 256);
+        else if (BLK == 1024) PD_SEL_LAUNCH(true, 1, 1024);
+        else if (BLK == 512) PD_SEL_LAUNCH(true, 1, 512);
+        else if (lbw >= 4) PD_SEL_LAUNCH(true, 4, 256);
+        else PD_SEL_LAUNCH(true, 1, 256);
 #undef PD_SEL_LAUNCH
     };
     if (kt == PIMDAL_U32) go(reinterpret_cast<const uint32_t*>(key));

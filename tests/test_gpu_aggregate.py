@@ -119,10 +119,12 @@ def test_q1_by_orderkey_signed_packed_fields(ctx, orc):
     _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
 
 
-@pytest.mark.parametrize("col,val", [("quantity", 32768), ("price", 1 << 31), ("discount", -129), ("tax", 128)])
+@pytest.mark.parametrize("col,val", [("quantity", 32768), ("quantity", -32768), ("price", 1 << 31), ("discount", -129),
+                                     ("tax", 128)])
 def test_q1_by_orderkey_one_row_beyond_packed_width(ctx, orc, col, val):
-    """One kept row outside the packed widths sends the whole call down the
-    unpacked path; the result must not change."""
+    """One kept row outside the packed widths (or with the reserved quantity
+    -32768) sends the whole call down the unpacked path; the result must not
+    change."""
     n = 120_000
     t = orc.fact_table(n, groups=40_000)
     t["date"][777] = 0

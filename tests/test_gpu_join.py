@@ -105,3 +105,20 @@ def test_join_2p24_build_2p27_probe_properties(ctx, zipf, which):
     got = np.sort((op.numpy(k).astype(np.uint64) << np.uint64(32)) | ob.numpy(k).astype(np.uint64))
     op.free(); ob.free()
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("which", ["hash", "smj"])
+def test_join_probe_tile_spans_wide_build_range(ctx, orc, which):
+    """Unique build keys, few probe rows spread over all of them: one sorted
+    probe tile's build range exceeds the merge's LDS key cache."""
+    nb, ns = 100_000, 3_000
+    rk = orc.gen_perm(31, 3, nb, nb)
+    rp = orc.gen_u32(31, 5, nb)
+    sk = orc.gen_range(np.uint32, 31, 4, ns, 0, nb + 500)   # some probe keys miss
+    sp = orc.gen_u32(31, 6, ns)
+    gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
+    ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
+    if which == "smj":
+        assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
+    else:
+        assert same_multiset((gp, gb), (ep, eb))

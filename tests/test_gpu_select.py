@@ -14,13 +14,18 @@ def run(ctx, k, p, lo, hi):
     return op.numpy(cnt), orow.numpy(cnt)
 
 
-@pytest.fixture(params=[0, 1], ids=["one_pass", "two_pass"], autouse=True)
+@pytest.fixture(params=[(0, 256), (0, 1024), (1, 256)], ids=["one_pass", "one_pass_16k_tiles", "two_pass"],
+                autouse=True)
 def select_mode(request, ctx):
-    """Every select test runs in both modes: one pass with decoupled
-    look-back (0) and count pass + write pass (1)."""
-    ctx.set_tune("select_mode", request.param)
+    """Every select test runs in each mode: one pass with decoupled look-back
+    over 4096-row tiles and over 16384-row tiles, and count pass + write
+    pass."""
+    mode, block = request.param
+    ctx.set_tune("select_mode", mode)
+    ctx.set_tune("select_block", block)
     yield request.param
     ctx.set_tune("select_mode", 0)
+    ctx.set_tune("select_block", 256)
 
 
 def test_select_2_20_uniform_10pct(ctx, orc):
