@@ -75,7 +75,7 @@ struct SvTables {
     const uint8_t* flag2;
     uint64_t n_fact;
     const int32_t* dim_date;
-    const uint64_t* dim_index;  // (dim row + 1) << 32 | key, 0 = empty
+    const uint4* dim_index;     // (key, dim row + 1, dim date, 0); row 0 = empty slot
     uint64_t dim_mask;
     const uint8_t* lut;         // flag codes: [0,256) flag1, [256,512) flag2
     uint32_t n2;
@@ -90,12 +90,17 @@ __device__ __forceinline__ uint32_t sv_find_query(const SvKind& k, uint64_t wi) 
     return lo;
 }
 
-__device__ __forceinline__ uint32_t sv_dim_lookup(const SvTables& T, int32_t key) {
+// One 16-byte slot read per probe: the slot carries the dimension row's
+// date, so the join predicate needs no second random read.
+__device__ __forceinline__ uint32_t sv_dim_lookup(const SvTables& T, int32_t key, int32_t* date) {
     uint64_t h = pd_mix64((uint32_t)key) & T.dim_mask;
     while (true) {
-        const uint64_t s = T.dim_index[h];
-        if (s == 0) return 0xFFFFFFFFu;
-        if ((uint32_t)s == (uint32_t)key) return (uint32_t)(s >> 32) - 1;
+        const uint4 e = T.dim_index[h];
+        if (e.y == 0) return 0xFFFFFFFFu;
+        if (e.x == (uint32_t)key) {
+            *date = (int32_t)e.z;
+            return e.y - 1;
+        }
         h = (h + 1) & T.dim_mask;
     }
 }
@@ -144,8 +149,9 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
             bool m = false;
             if (r < rend) {
                 if constexpr (JOIN) {
-                    const uint32_t d = sv_dim_lookup(T, T.orderkey[r]);
-                    m = d != 0xFFFFFFFFu && (int64_t)T.dim_date[d] < q.param;
+                    int32_t dd = 0;
+                    const uint32_t d = sv_dim_lookup(T, T.orderkey[r], &dd);
+                    m = d != 0xFFFFFFFFu && (int64_t)dd < q.param;
                     dimrow[i] = d;
                 } else {
                     m = T.price[r] < q.param;
@@ -230,10 +236,9 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restri
         unsigned __int128 ach[SV_G];
 #pragma unroll
         for (int g = 0; g < SV_G; g++) { aq[g] = ap[g] = ad[g] = adp[g] = an[g] = 0; ach[g] = 0; }
-        for (uint64_t r = c0 + t; r < c1; r += SV_BLOCK) {
-            if ((int64_t)T.date[r] > q.param) continue;
-            const uint32_t code = (uint32_t)T.lut[T.flag1[r]] * T.n2 + T.lut[256 + T.flag2[r]];
-            const int64_t qq = T.qty[r], p = T.price[r], d = T.disc[r], x = T.tax[r];
+        auto add_row = [&](int32_t dt, uint32_t a, uint32_t b, int64_t qq, int64_t p, int64_t d, int64_t x) {
+            if ((int64_t)dt > q.param) return;
+            const uint32_t code = (uint32_t)T.lut[a] * T.n2 + T.lut[256 + b];
             const int64_t dp = p * (100 - d);
             const __int128 ch = (__int128)dp * (100 + x);
 #pragma unroll
@@ -246,7 +251,29 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restri
                 ach[g] += on ? (unsigned __int128)ch : 0;
                 an[g] += on ? 1 : 0;
             }
+        };
+        auto scalar = [&](uint64_t r) {
+            add_row(T.date[r], T.flag1[r], T.flag2[r], T.qty[r], T.price[r], T.disc[r], T.tax[r]);
+        };
+        // rows [c0, a0) and [a1, c1) one at a time; the 4-aligned middle four
+        // rows per thread per step with 16-byte loads (the columns are
+        // 256-byte aligned allocations)
+        const uint64_t a0 = min(c1, (c0 + 3) & ~3ull), a1 = max(a0, c1 & ~3ull);
+        for (uint64_t r = c0 + t; r < a0; r += SV_BLOCK) scalar(r);
+        for (uint64_t v = a0 / 4 + t; v < a1 / 4; v += SV_BLOCK) {
+            const int4 dt = reinterpret_cast<const int4*>(T.date)[v];
+            const uint32_t fa = reinterpret_cast<const uint32_t*>(T.flag1)[v];
+            const uint32_t fb = reinterpret_cast<const uint32_t*>(T.flag2)[v];
+            const longlong2 q0 = reinterpret_cast<const longlong2*>(T.qty)[2 * v], q1 = reinterpret_cast<const longlong2*>(T.qty)[2 * v + 1];
+            const longlong2 p0 = reinterpret_cast<const longlong2*>(T.price)[2 * v], p1 = reinterpret_cast<const longlong2*>(T.price)[2 * v + 1];
+            const longlong2 d0 = reinterpret_cast<const longlong2*>(T.disc)[2 * v], d1 = reinterpret_cast<const longlong2*>(T.disc)[2 * v + 1];
+            const longlong2 x0 = reinterpret_cast<const longlong2*>(T.tax)[2 * v], x1 = reinterpret_cast<const longlong2*>(T.tax)[2 * v + 1];
+            add_row(dt.x, fa & 0xFF, fb & 0xFF, q0.x, p0.x, d0.x, x0.x);
+            add_row(dt.y, (fa >> 8) & 0xFF, (fb >> 8) & 0xFF, q0.y, p0.y, d0.y, x0.y);
+            add_row(dt.z, (fa >> 16) & 0xFF, (fb >> 16) & 0xFF, q1.x, p1.x, d1.x, x1.x);
+            add_row(dt.w, fa >> 24, fb >> 24, q1.y, p1.y, d1.y, x1.y);
         }
+        for (uint64_t r = a1 + t; r < c1; r += SV_BLOCK) scalar(r);
 #pragma unroll
         for (int g = 0; g < SV_G; g++) {
             const uint64_t lo = (uint64_t)ach[g];
@@ -313,14 +340,19 @@ __global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint64_t
     }
 }
 
-__global__ void k_dim_index_build(const int32_t* __restrict__ key, uint64_t n, uint64_t* __restrict__ idx,
-                                  uint64_t mask, uint32_t* __restrict__ dup) {
+__global__ void k_dim_index_build(const int32_t* __restrict__ key, const int32_t* __restrict__ date, uint64_t n,
+                                  uint4* __restrict__ idx, uint64_t mask, uint32_t* __restrict__ dup) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        // claim a slot with its first 8 bytes (key, row + 1), then add the date
         const uint64_t want = ((i + 1) << 32) | (uint32_t)key[i];
         uint64_t h = pd_mix64((uint32_t)key[i]) & mask;
         while (true) {
-            const uint64_t prev = atomicCAS((unsigned long long*)&idx[h], 0ull, (unsigned long long)want);
-            if (prev == 0) break;
+            unsigned long long* slot = reinterpret_cast<unsigned long long*>(&idx[h]);
+            const uint64_t prev = atomicCAS(slot, 0ull, (unsigned long long)want);
+            if (prev == 0) {
+                idx[h].z = (uint32_t)date[i];
+                break;
+            }
             if ((uint32_t)prev == (uint32_t)key[i]) { atomicOr(dup, 1u); break; }
             h = (h + 1) & mask;
         }
@@ -503,12 +535,12 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         const double load = (double)c->tune_get("serve_index_load_pct", 50) / 100.0;
         uint64_t cap = 1;
         while ((double)cap * load < (double)t.n_dim) cap <<= 1;
-        S->dim_index = c->pool.alloc(cap * 8, s);
-        fill_u64(c, S->dim_index, 0, cap, s);
+        S->dim_index = c->pool.alloc(cap * 16, s);
+        fill_u64(c, S->dim_index, 0, 2 * cap, s);
         Scratch dup(c, 4, s);
         fill_u32(c, dup.get(), 0, 1, s);
         launch(c, "sv_dim_index", k_dim_index_build, dim3((uint32_t)std::min<uint64_t>(ceil_div(t.n_dim, 256), 8192)),
-               dim3(256), 0, s, t.dim_key, t.n_dim, (uint64_t*)S->dim_index, cap - 1, dup.as<uint32_t>());
+               dim3(256), 0, s, t.dim_key, t.dim_date, t.n_dim, (uint4*)S->dim_index, cap - 1, dup.as<uint32_t>());
         uint32_t hd = 0;
         PD_HIP(hipMemcpyAsync(&hd, dup.get(), 4, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
@@ -520,7 +552,7 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         S->n1 = meta[0];
         S->n2 = meta[1];
         S->T = SvTables{t.orderkey, t.quantity, t.price, t.discount, t.tax, t.date, t.flag1, t.flag2, t.n_fact,
-                        t.dim_date, (const uint64_t*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2};
+                        t.dim_date, (const uint4*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2};
         // slots
         S->nslots = slots;
         S->arena_bytes = slot_arena_bytes ? slot_arena_bytes : (1ull << 30);
