@@ -119,7 +119,8 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 // 4 waves per SIMD: at most 128 VGPRs (second launch-bounds argument is the
 // minimum number of waves per execution unit).  EARLY: resolve the global
 // offsets (look-back) before the LDS reorder instead of after it.
-template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH>
+// BALLOT: rank with per-bit ballots instead of LDS match masks.
+template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false>
 __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
@@ -167,9 +168,30 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
         return idx < n ? ((rs_src<HASH>(key[i]) >> shift) & mask) : (RS_BINS - 1);
     };
 
-    // ---- stable rank inside the wave: match mask per item ----
+    // ---- stable rank inside the wave ----
     const uint64_t lt = pd::lanemask_lt();
     uint32_t pos[RS_ITEMS];
+    if constexpr (BALLOT) {
+        // lanes holding the same digit = AND over the digit's bits of the
+        // bit's ballot (or its complement); the lowest such lane bumps the
+        // wave's running count of the digit
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; i++) {
+            const uint32_t d = digit_of(i);
+            uint64_t m = ~0ull;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const uint64_t bb = pd::ballot((d >> b) & 1);
+                m &= ((d >> b) & 1) ? bb : ~bb;
+            }
+            const uint32_t below = pd::popc64(m & lt);
+            const uint32_t old = *(volatile uint16_t*)&s_cnt[w][d];
+            pos[i] = old + below;
+            PD_COMPILER_BARRIER();
+            if (below == 0) *(volatile uint16_t*)&s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
+            PD_COMPILER_BARRIER();
+        }
+    } else
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
         const uint32_t d = digit_of(i);
@@ -272,6 +294,7 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
                        const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
     const int block = (int)c->tune_get("sort_block", 512) == 256 ? 256 : 512;
     const bool early = c->tune_get("sort_early_lookback", 0) != 0;
+    const bool ballot = c->tune_get("sort_rank", 0) == 1;
     const uint64_t ntiles = ceil_div(n, (uint64_t)block * RS_ITEMS);
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
@@ -288,11 +311,17 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, E, HASH>, dim3((uint32_t)ntiles),  \
            dim3(B), 0, s, ksrc, vsrc, kdst, \
            vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
-        if (block == 512 && early) PD_RS_LAUNCH(512, true);
+#define PD_RS_LAUNCH_B(B)                                                                                          \
+    launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, false, HASH, true>, dim3((uint32_t)ntiles),  \
+           dim3(B), 0, s, ksrc, vsrc, kdst, \
+           vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
+        if (block == 512 && ballot && !early) PD_RS_LAUNCH_B(512);
+        else if (block == 512 && early) PD_RS_LAUNCH(512, true);
         else if (block == 512) PD_RS_LAUNCH(512, false);
         else if (early) PD_RS_LAUNCH(256, true);
         else PD_RS_LAUNCH(256, false);
 #undef PD_RS_LAUNCH
+#undef PD_RS_LAUNCH_B
         ksrc = kdst;
         vsrc = vdst;
     }
@@ -369,6 +398,57 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         run_passes<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
 }
 
+// Histogram of the whole partition id ref_hash(key) & (F - 1) for F <= 2^14
+// (one LDS atomic per key over F bins, so lanes of an instruction rarely
+// collide).  Each block writes its F counts as one row of g_blk; the fold
+// sums the rows per bin and accumulates the per-pass digit histograms.
+constexpr uint32_t RS_FULL_MAX_BITS = 14;
+__global__ __launch_bounds__(512) void k_part_hist_full(const uint32_t* __restrict__ keys, uint64_t n, uint32_t F,
+                                                        uint32_t* __restrict__ g_blk) {
+    __shared__ uint32_t h[1u << RS_FULL_MAX_BITS];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < F; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t r0 = min(n, (uint64_t)blockIdx.x * chunk), r1 = min(n, r0 + chunk);
+    // 16-byte loads over the 4-aligned middle of the block's rows
+    const uint64_t a0 = min(r1, (r0 + 3) & ~3ull), a1 = max(a0, r1 & ~3ull);
+    const bool vec = ((uintptr_t)keys % 16) == 0;
+    const uint64_t v1 = vec ? a1 / 4 : a0 / 4;
+    for (uint64_t i = r0 + t; i < (vec ? a0 : r0); i += blockDim.x) atomicAdd(&h[pd::ref_hash(keys[i]) & (F - 1)], 1u);
+    for (uint64_t v = a0 / 4 + t; v < v1; v += blockDim.x) {
+        const uint4 q = reinterpret_cast<const uint4*>(keys)[v];
+        atomicAdd(&h[pd::ref_hash(q.x) & (F - 1)], 1u);
+        atomicAdd(&h[pd::ref_hash(q.y) & (F - 1)], 1u);
+        atomicAdd(&h[pd::ref_hash(q.z) & (F - 1)], 1u);
+        atomicAdd(&h[pd::ref_hash(q.w) & (F - 1)], 1u);
+    }
+    for (uint64_t i = (vec ? a1 : r0) + t; i < r1; i += blockDim.x) atomicAdd(&h[pd::ref_hash(keys[i]) & (F - 1)], 1u);
+    __syncthreads();
+    for (uint32_t i = t; i < F; i += blockDim.x) g_blk[(uint64_t)blockIdx.x * F + i] = h[i];
+}
+
+// One thread per bin: sum of the G block rows, added into the per-pass
+// digit histograms (LDS first, then one global atomic per digit per block).
+__global__ __launch_bounds__(256) void k_part_hist_fold(const uint32_t* __restrict__ g_blk, uint32_t G, uint32_t F,
+                                                        RsPassDesc pd, uint32_t* __restrict__ g_hist) {
+    __shared__ uint32_t h[4][RS_BINS];
+    for (uint32_t i = threadIdx.x; i < 4 * RS_BINS; i += blockDim.x) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < F) {
+        uint32_t v = 0;
+        for (uint32_t g = 0; g < G; g++) v += g_blk[(uint64_t)g * F + b];
+        if (v)
+            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(b >> pd.shift[p]) & pd.mask[p]], v);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < pd.npass * RS_BINS; i += blockDim.x) {
+        const uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&g_hist[i], v);
+    }
+}
+
 // Partition bounds of rows stably sorted by ref_hash(key) & (F - 1):
 // bounds[f] = first row whose partition is >= f (binary search), bounds[F] = n.
 __global__ void k_part_bounds(const uint32_t* __restrict__ k, uint64_t n, uint32_t F, uint32_t* __restrict__ bounds) {
@@ -408,7 +488,15 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
         b += w;
     }
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
-    launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
+    if (bits <= RS_FULL_MAX_BITS) {
+        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 16384), (uint64_t)c->cu_count * 2));
+        Scratch blk(c, (uint64_t)G * F * 4, s);
+        launch(c, "part_hist", k_part_hist_full, dim3(G), dim3(512), 0, s, kin, n, F, blk.as<uint32_t>());
+        launch(c, "part_hist_fold", k_part_hist_fold, dim3(ceil_div(F, 256)), dim3(256), 0, s,
+               (const uint32_t*)blk.as<uint32_t>(), G, F, run, hist);
+    } else {
+        launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
+    }
     launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
     if (vw == 4)
         run_passes<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);

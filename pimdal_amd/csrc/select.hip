@@ -384,7 +384,7 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
         return;
     }
     // one pass: tile = block size / 64 waves x 1024 rows
-    const int64_t bsel = c->tune_get("select_block", 256);
+    const int64_t bsel = c->tune_get("select_block", 1024);
     // unaligned columns take the 256-thread kernel with dword loads
     const int BLK = !vec ? 256 : bsel >= 1024 ? 1024 : bsel >= 512 ? 512 : 256;
     const uint64_t ntl = ceil_div(n, (uint64_t)(BLK / PD_WAVE) * SEL_WAVE_ROWS);

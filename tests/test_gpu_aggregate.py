@@ -142,3 +142,19 @@ def test_q1_by_orderkey_few_hot_keys(ctx, orc):
     d = {c: ctx.to_device(v) for c, v in t.items()}
     got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
     _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
+def test_hash_aggr_many_partitions(orc):
+    """2^16 partitions (beyond the full partition-id histogram's 2^14): the
+    per-digit histogram path of the hashed partitioning.  Own context, so
+    the knobs do not leak into other tests."""
+    import pimdal_amd
+    n = 1 << 20
+    k = orc.gen_range(np.uint32, 41, 1, n, 0, n // 3)
+    v = orc.gen_range(np.int64, 41, 2, n, -(1 << 40), 1 << 40)
+    with pimdal_amd.Context(0) as c2:
+        c2.set_tune("gb_rows_per_part", 16)
+        c2.set_tune("gb_groups_hint", n // 3)
+        got = agg(c2, ops.hash_aggr, k, v)
+    exp = orc.hash_aggr(k, v)
+    assert all(np.array_equal(a, b) for a, b in zip(got, exp))

@@ -25,7 +25,7 @@ def select_mode(request, ctx):
     ctx.set_tune("select_block", block)
     yield request.param
     ctx.set_tune("select_mode", 0)
-    ctx.set_tune("select_block", 256)
+    ctx.set_tune("select_block", 1024)
 
 
 def test_select_2_20_uniform_10pct(ctx, orc):

@@ -97,3 +97,47 @@ def test_autotune_sets_knobs_and_results_stay_exact(ctx, orc, tmp_path):
     ek, ev = orc.sort_pairs(k, v)
     assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
     assert open(cache).read().split()[0] == "sort"
+
+
+@pytest.fixture
+def ballot_rank(ctx):
+    """Onesweep ranking by per-bit ballots (sort_rank 1) instead of LDS
+    match masks."""
+    ctx.set_tune("sort_rank", 1)
+    yield
+    ctx.set_tune("sort_rank", 0)
+
+
+@pytest.mark.parametrize("n", [1, 8193, 1 << 20])
+def test_sort_ballot_rank_uniform(ctx, orc, ballot_rank, n):
+    k, v = orc.sort_workload(n, seed=70 + n)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_ballot_rank_zipf_and_partial_bits(ctx, orc, ballot_rank):
+    k, v = orc.sort_workload(1 << 21, zipf=True)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+    k, v = orc.sort_workload(70_000, seed=12)
+    gk, gv = gpu_sort(ctx, k, v, 4, 13)
+    mk = (k >> np.uint32(4)) & np.uint32((1 << 9) - 1)
+    order = np.argsort(mk, kind="stable")
+    assert np.array_equal(gv, v[order]) and np.array_equal(gk, k[order])
+
+
+def test_sort_ballot_rank_u64_payload_and_hash_join(ctx, orc, ballot_rank):
+    k = orc.gen_range(np.uint32, 6, 1, 200_000, 0, 1000)
+    v = orc.gen_range(np.int64, 6, 2, 200_000, -(1 << 40), 1 << 40)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v.view(np.uint64))
+    assert np.array_equal(gk, ek) and np.array_equal(gv.view(np.uint64), ev)
+    # the hashed partition passes of the hash join run the same kernel
+    rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 19)   # above the global-table size: partitioned
+    op, ob, cnt = ops.hash_join(ctx, *[ctx.to_device(a) for a in (rk, rp, sk, sp)])
+    ep, eb = orc.hash_join(rk, rp, sk, sp)
+    got = np.sort((op.numpy(cnt).astype(np.uint64) << np.uint64(32)) | ob.numpy(cnt))
+    exp = np.sort((ep.astype(np.uint64) << np.uint64(32)) | eb)
+    assert np.array_equal(got, exp)
