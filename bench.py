@@ -106,7 +106,7 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
-KERNELS = ("part_hist", "part_hist_fold", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
+KERNELS = ("part_hist", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
            "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",

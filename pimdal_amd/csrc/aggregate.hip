@@ -28,6 +28,7 @@
 //                   rows insert into a per-partition table in HBM with global
 //                   atomics; k_gb_emit_global emits them.
 #include <algorithm>
+#include <type_traits>
 
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
@@ -40,21 +41,55 @@ namespace pd {
 constexpr int QD_BLOCK = 256;
 constexpr int Q1_NF = 7;  // qty, price, disc, disc_price, charge_lo, charge_hi, count
 
+// Presence bitmaps of the byte values of two flag columns (bits[0..8) for
+// flag1, bits[8..16) for flag2).  16 bytes per lane per load; the 256-bit
+// masks live in registers, set with unrolled selects (no dynamic register
+// indexing).
+__device__ __forceinline__ void flag_mark(uint64_t (&m)[4], uint32_t v) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) m[j] |= ((v >> 6) == (uint32_t)j) ? (1ull << (v & 63)) : 0ull;
+}
+
 __global__ __launch_bounds__(256) void k_flag_domain(const uint8_t* __restrict__ f1, const uint8_t* __restrict__ f2,
                                                      uint64_t n, uint32_t* __restrict__ bits /*[16]*/) {
     __shared__ uint32_t s[16];
     if (threadIdx.x < 16) s[threadIdx.x] = 0;
     __syncthreads();
-    uint32_t m1[8] = {0, 0, 0, 0, 0, 0, 0, 0}, m2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t a = f1[i], b = f2[i];
-        m1[a >> 5] |= 1u << (a & 31);
-        m2[b >> 5] |= 1u << (b & 31);
-    }
+    uint64_t m1[4] = {0, 0, 0, 0}, m2[4] = {0, 0, 0, 0};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const bool vec = (((uintptr_t)f1 | (uintptr_t)f2) % 16) == 0;
+    const uint64_t n16 = vec ? n / 16 : 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n16; v += stride) {
+        const uint4 a = reinterpret_cast<const uint4*>(f1)[v];
+        const uint4 b = reinterpret_cast<const uint4*>(f2)[v];
+        const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wb[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        if (m1[j]) atomicOr(&s[j], m1[j]);
-        if (m2[j]) atomicOr(&s[8 + j], m2[j]);
+        for (int k = 0; k < 4; k++) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                flag_mark(m1, (wa[k] >> (8 * e)) & 0xFF);
+                flag_mark(m2, (wb[k] >> (8 * e)) & 0xFF);
+            }
+        }
+    }
+    for (uint64_t i = n16 * 16 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        flag_mark(m1, f1[i]);
+        flag_mark(m2, f2[i]);
+    }
+    // wave OR, then one LDS atomic per word per wave
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        for (int d = PD_WAVE / 2; d > 0; d >>= 1) {
+            m1[j] |= __shfl_xor(m1[j], d, PD_WAVE);
+            m2[j] |= __shfl_xor(m2[j], d, PD_WAVE);
+        }
+    }
+    if (pd::lane() == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (m1[j]) { atomicOr(&s[2 * j], (uint32_t)m1[j]); atomicOr(&s[2 * j + 1], (uint32_t)(m1[j] >> 32)); }
+            if (m2[j]) { atomicOr(&s[8 + 2 * j], (uint32_t)m2[j]); atomicOr(&s[8 + 2 * j + 1], (uint32_t)(m2[j] >> 32)); }
+        }
     }
     __syncthreads();
     if (threadIdx.x < 16 && s[threadIdx.x]) atomicOr(&bits[threadIdx.x], s[threadIdx.x]);
@@ -951,7 +986,7 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         uint8_t* inv = lut + 512;
         uint64_t* ng = reinterpret_cast<uint64_t*>(inv + 512);
         fill_u32(c, bits, 0, 16 + 2, s);
-        const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 8), (uint64_t)c->cu_count * 4));
+        const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64), (uint64_t)c->cu_count * 4));
         launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
         launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut, inv, meta);
         uint32_t nm[2];
@@ -963,16 +998,26 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
             Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s), sums(c, 8 * 6 * 16, s);
             const bool vec = ((uintptr_t)date | (uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)qty | (uintptr_t)price |
                               (uintptr_t)disc | (uintptr_t)tax) % 16 == 0;
-            if (vec)
-                launch(c, "q1_dense", k_q1_dense<8, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc,
-                       tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
-            else
-                launch(c, "q1_dense", k_q1_dense<8, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price,
-                       disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
-            launch(c, "q1_reduce", k_q1_reduce, dim3(8), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, 8u,
+            // accumulators for exactly as many codes as the data has (rounded
+            // up to even): fewer selects per row and fewer registers
+            const uint32_t Gt = groups <= 2 ? 2 : groups <= 4 ? 4 : groups <= 6 ? 6 : 8;
+            auto dense = [&](auto gtag) {
+                constexpr int GG = decltype(gtag)::value;
+                if (vec)
+                    launch(c, "q1_dense", k_q1_dense<GG, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price,
+                           disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
+                else
+                    launch(c, "q1_dense", k_q1_dense<GG, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty,
+                           price, disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
+            };
+            if (Gt == 2) dense(std::integral_constant<int, 2>{});
+            else if (Gt == 4) dense(std::integral_constant<int, 4>{});
+            else if (Gt == 6) dense(std::integral_constant<int, 6>{});
+            else dense(std::integral_constant<int, 8>{});
+            launch(c, "q1_reduce", k_q1_reduce, dim3(Gt), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, Gt,
                    sums.as<unsigned __int128>());
             launch(c, "q1_final", k_q1_final, dim3(1), dim3(64), 0, s, (const unsigned __int128*)sums.as<unsigned __int128>(),
-                   8u, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price, out.sum_disc,
+                   Gt, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price, out.sum_disc,
                    out.sum_disc_price, out.sum_charge, out.count, ng);
             const uint64_t k = read_u64(c, ng, s);
             PD_REQUIRE(k <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
@@ -1031,7 +1076,7 @@ void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n,
     Scratch small(c, 64 + 8, s);
     uint32_t* bits = small.as<uint32_t>();
     fill_u32(c, bits, 0, 18, s);
-    const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 8), (uint64_t)c->cu_count * 4));
+    const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64), (uint64_t)c->cu_count * 4));
     if (n) launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
     launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut_dev, lut_dev + 512, bits + 16);
     PD_HIP(hipMemcpyAsync(meta, bits + 16, 8, hipMemcpyDeviceToHost, s));

@@ -400,14 +400,16 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
 
 // Histogram of the whole partition id ref_hash(key) & (F - 1) for F <= 2^14
 // (one LDS atomic per key over F bins, so lanes of an instruction rarely
-// collide).  Each block writes its F counts as one row of g_blk; the fold
-// sums the rows per bin and accumulates the per-pass digit histograms.
+// collide), folded in the same block into the per-pass digit histograms,
+// which are added to the global ones (npass x 256 atomics per block).
 constexpr uint32_t RS_FULL_MAX_BITS = 14;
 __global__ __launch_bounds__(512) void k_part_hist_full(const uint32_t* __restrict__ keys, uint64_t n, uint32_t F,
-                                                        uint32_t* __restrict__ g_blk) {
+                                                        RsPassDesc pd, uint32_t* __restrict__ g_hist) {
     __shared__ uint32_t h[1u << RS_FULL_MAX_BITS];
+    __shared__ uint32_t hd[4][RS_BINS];
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < F; i += blockDim.x) h[i] = 0;
+    for (uint32_t i = t; i < 4 * RS_BINS; i += blockDim.x) (&hd[0][0])[i] = 0;
     __syncthreads();
     const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint64_t r0 = min(n, (uint64_t)blockIdx.x * chunk), r1 = min(n, r0 + chunk);
@@ -425,26 +427,14 @@ __global__ __launch_bounds__(512) void k_part_hist_full(const uint32_t* __restri
     }
     for (uint64_t i = (vec ? a1 : r0) + t; i < r1; i += blockDim.x) atomicAdd(&h[pd::ref_hash(keys[i]) & (F - 1)], 1u);
     __syncthreads();
-    for (uint32_t i = t; i < F; i += blockDim.x) g_blk[(uint64_t)blockIdx.x * F + i] = h[i];
-}
-
-// One thread per bin: sum of the G block rows, added into the per-pass
-// digit histograms (LDS first, then one global atomic per digit per block).
-__global__ __launch_bounds__(256) void k_part_hist_fold(const uint32_t* __restrict__ g_blk, uint32_t G, uint32_t F,
-                                                        RsPassDesc pd, uint32_t* __restrict__ g_hist) {
-    __shared__ uint32_t h[4][RS_BINS];
-    for (uint32_t i = threadIdx.x; i < 4 * RS_BINS; i += blockDim.x) (&h[0][0])[i] = 0;
-    __syncthreads();
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < F) {
-        uint32_t v = 0;
-        for (uint32_t g = 0; g < G; g++) v += g_blk[(uint64_t)g * F + b];
+    for (uint32_t b = t; b < F; b += blockDim.x) {
+        const uint32_t v = h[b];
         if (v)
-            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&h[p][(b >> pd.shift[p]) & pd.mask[p]], v);
+            for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&hd[p][(b >> pd.shift[p]) & pd.mask[p]], v);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < pd.npass * RS_BINS; i += blockDim.x) {
-        const uint32_t v = (&h[0][0])[i];
+    for (uint32_t i = t; i < pd.npass * RS_BINS; i += blockDim.x) {
+        const uint32_t v = (&hd[0][0])[i];
         if (v) atomicAdd(&g_hist[i], v);
     }
 }
@@ -490,10 +480,7 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
     if (bits <= RS_FULL_MAX_BITS) {
         const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 16384), (uint64_t)c->cu_count * 2));
-        Scratch blk(c, (uint64_t)G * F * 4, s);
-        launch(c, "part_hist", k_part_hist_full, dim3(G), dim3(512), 0, s, kin, n, F, blk.as<uint32_t>());
-        launch(c, "part_hist_fold", k_part_hist_fold, dim3(ceil_div(F, 256)), dim3(256), 0, s,
-               (const uint32_t*)blk.as<uint32_t>(), G, F, run, hist);
+        launch(c, "part_hist", k_part_hist_full, dim3(G), dim3(512), 0, s, kin, n, F, run, hist);
     } else {
         launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
     }

@@ -158,3 +158,15 @@ def test_hash_aggr_many_partitions(orc):
         got = agg(c2, ops.hash_aggr, k, v)
     exp = orc.hash_aggr(k, v)
     assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+
+
+@pytest.mark.parametrize("n1,n2", [(2, 2), (4, 2), (1, 1)])
+def test_q1_flags_dense_group_counts(ctx, orc, n1, n2):
+    """The dense path instantiated for 2, 4 and 8 group codes."""
+    n = 300_001
+    t = orc.fact_table(n)
+    t["flag1"] = (orc.gen_range(np.uint8, 51, 17, n, 0, n1 - 1) * 7 + 65).astype(np.uint8)
+    t["flag2"] = (orc.gen_range(np.uint8, 51, 18, n, 0, n2 - 1) * 11 + 70).astype(np.uint8)
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2000)
+    _cmp_q1(got, orc.q1_flags(t, 2000), ordered=True)

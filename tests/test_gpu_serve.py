@@ -56,3 +56,37 @@ def test_admission_rejects_query_larger_than_a_slot(ctx, tables):
     with pytest.raises(PimdalError, match="larger than a slot"):
         srv.run(q, serving.BATCHED)
     srv.close()
+
+
+def test_dimension_keys_must_be_unique(ctx, tables):
+    fact, dim, hf, hd, _ = tables
+    k = hd["key"].copy()
+    k[5] = k[6]
+    dup = dict(dim)
+    dup["key"] = ctx.to_device(k)
+    with pytest.raises(PimdalError, match="unique"):
+        serving.Server(ctx, fact, dup, slots=1, arena_bytes=16 << 20)
+
+
+def test_join_probes_that_miss(ctx, orc, tables):
+    """Half the dimension keys replaced by values no fact row holds: those
+    fact rows find no dimension row (the probe walks to a bucket that never
+    overflowed), in both modes and in the host restatement."""
+    fact, dim, hf, hd, _ = tables
+    k = hd["key"].copy()
+    k[::2] = -1 - np.arange(k[::2].size, dtype=np.int32)
+    d2 = dict(dim)
+    d2["key"] = ctx.to_device(k)
+    hd2 = dict(hd)
+    hd2["key"] = k
+    order = np.argsort(k, kind="stable")
+    ds2 = (order, k[order])
+    srv = serving.Server(ctx, fact, d2, slots=2, arena_bytes=256 << 20)
+    q = serving.gen_queries(60, N_FACT, rate_qps=0, min_log2=10, max_log2=18, seed=23)
+    q["kind"] = serving.JOIN
+    r_seq, _ = srv.run(q, serving.SEQUENTIAL)
+    r_bat, _ = srv.run(q, serving.BATCHED)
+    assert np.array_equal(r_seq["count"], r_bat["count"]) and np.array_equal(r_seq["checksum"], r_bat["checksum"])
+    for i in range(len(q)):
+        assert (int(r_bat["count"][i]), int(r_bat["checksum"][i])) == expected(q[i], hf, hd2, orc, ds2), i
+    srv.close()
