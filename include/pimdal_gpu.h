@@ -76,7 +76,8 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  * select_block (256, 512, 1024: 1024 rows per wave per tile),
  * select_lookback_width (1 or 4 status words per lane), select_mode (0 one
  * pass with decoupled look-back, 1 count pass + write pass), hj_mode (-1 by
- * build size, 0 partitioned, 1 global table), hj_global_max_build,
+ * build size, 0 partitioned, 1 global table), hj_global_max_build, hj_block
+ * (probe block: 1024 threads x 8 rows or 512 x 16),
  * hj_fanout, hj_rows_per_part, hj_chunk, gb_groups_hint, gb_rows_per_part,
  * gb_lds_max_rows, gb_packed, serve_index_load_pct.  Unknown keys are stored
  * and ignored. */

@@ -107,7 +107,8 @@ extern "C" int pimdal_autotune(pimdal_ctx* c, const char* op, uint64_t n, const 
         fill(c, bp.get(), nb, 6, 0);
         fill(c, pk.get(), n, 7, 0);
         fill(c, pp.get(), n, 8, 0);
-        knobs = {{"hj_mode", {0, 1}}, {"hj_rows_per_part", {2048, 4096}}, {"hj_chunk", {16384, 32768, 65536}}};
+        knobs = {{"hj_mode", {0, 1}}, {"hj_block", {512, 1024}}, {"hj_rows_per_part", {2048, 4096}},
+                 {"hj_chunk", {16384, 32768, 65536, 131072}}};
         tune(c, knobs, [&] {
             hash_join(c, bk.as<uint32_t>(), bp.as<uint32_t>(), nb, pk.as<uint32_t>(), pp.as<uint32_t>(), n,
                       o1.as<uint32_t>(), o2.as<uint32_t>(), n, s);

@@ -35,7 +35,7 @@
 
 namespace pd {
 
-constexpr int HJ_BLOCK = 512;
+constexpr int HJ_BLOCK = 512;         // count/write phases; the unique phase takes its shape as parameters
 constexpr int HJ_TS = 8192;           // LDS table slots (8 B each)
 constexpr int HJ_ROWS = 16;           // probe rows per thread per round
 
@@ -54,8 +54,9 @@ __global__ void k_hj_chunks(const uint32_t* __restrict__ HS, uint32_t GS, uint32
 // flags[2..3]: total emitted pairs (uint64)
 enum { HJ_UNIQUE = 0, HJ_COUNT = 1, HJ_WRITE = 2 };
 
-template <int PHASE>
-__global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restrict__ Rk,
+// BLOCK threads x ROWS probe rows per thread per round (BLOCK / 64 * ROWS == 128).
+template <int PHASE, int BLOCK = HJ_BLOCK, int ROWS = HJ_ROWS>
+__global__ __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 2) void k_hj_join(const uint32_t* __restrict__ Rk,
                                                          const uint32_t* __restrict__ Rp, const uint32_t* __restrict__ HR,
                                                          uint32_t GR, const uint32_t* __restrict__ Sk,
                                                          const uint32_t* __restrict__ Sp,
@@ -67,8 +68,8 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
                                                          uint64_t out_cap, uint32_t* __restrict__ flags) {
     __shared__ uint2 tab[HJ_TS];
     __shared__ uint32_t occ[HJ_TS / 32];
-    __shared__ uint32_t s_wsum[HJ_BLOCK / PD_WAVE];
-    __shared__ uint32_t s_kc[HJ_ROWS * (HJ_BLOCK / PD_WAVE)];
+    __shared__ uint32_t s_wsum[BLOCK / PD_WAVE];
+    __shared__ uint32_t s_kc[ROWS * (BLOCK / PD_WAVE)];
     __shared__ uint32_t s_f, s_bad;
 
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
         s_f = lo;
         s_bad = 0;
     }
-    for (uint32_t i = t; i < HJ_TS / 32; i += HJ_BLOCK) occ[i] = 0;
+    for (uint32_t i = t; i < HJ_TS / 32; i += BLOCK) occ[i] = 0;
     __syncthreads();
     const uint32_t f = s_f;
     const uint32_t ck = wi - wstart[f];
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
     }
 
     // ---- build ----
-    for (uint64_t i = r0 + t; i < r1; i += HJ_BLOCK) {
+    for (uint64_t i = r0 + t; i < r1; i += BLOCK) {
         const uint2 kv = make_uint2(Rk[i], Rp[i]);
         uint32_t h = pd::slot_hash(kv.x) & (HJ_TS - 1);
         while (true) {
@@ -115,23 +116,23 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
     }
     __syncthreads();
 
-    // ---- probe in rounds of HJ_BLOCK * HJ_ROWS rows ----
+    // ---- probe in rounds of BLOCK * ROWS rows ----
     uint64_t written = 0;        // block-uniform running output count
     const uint64_t wbase = (PHASE == HJ_WRITE) ? woff[wi] : s0;
     bool dup = false;
-    for (uint64_t rb = s0; rb < s1; rb += (uint64_t)HJ_BLOCK * HJ_ROWS) {
-        uint32_t key[HJ_ROWS], pay[HJ_ROWS], first[HJ_ROWS], m[HJ_ROWS];
+    for (uint64_t rb = s0; rb < s1; rb += (uint64_t)BLOCK * ROWS) {
+        uint32_t key[ROWS], pay[ROWS], first[ROWS], m[ROWS];
         uint32_t cnt = 0;
         // all loads of the round first, so they are in flight together
 #pragma unroll
-        for (int k = 0; k < HJ_ROWS; k++) {
-            const uint64_t i = rb + (uint64_t)k * HJ_BLOCK + t;
+        for (int k = 0; k < ROWS; k++) {
+            const uint64_t i = rb + (uint64_t)k * BLOCK + t;
             key[k] = i < s1 ? Sk[i] : 0u;
             pay[k] = i < s1 ? Sp[i] : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < HJ_ROWS; k++) {
-            const uint64_t i = rb + (uint64_t)k * HJ_BLOCK + t;
+        for (int k = 0; k < ROWS; k++) {
+            const uint64_t i = rb + (uint64_t)k * BLOCK + t;
             m[k] = 0;
             first[k] = 0;
             if (i < s1) {
@@ -154,12 +155,12 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
         }
         if constexpr (PHASE == HJ_UNIQUE) {
             // at most one match per row: output slots in row order
-            // (row = rb + k * HJ_BLOCK + wave * 64 + lane), so every store
+            // (row = rb + k * BLOCK + wave * 64 + lane), so every store
             // instruction writes consecutive slots
-            constexpr int WAVES = HJ_BLOCK / PD_WAVE;
-            static_assert(HJ_ROWS * WAVES == 2 * PD_WAVE, "one wave scans the (row group, wave) counts");
+            constexpr int WAVES = BLOCK / PD_WAVE;
+            static_assert(ROWS * WAVES == 2 * PD_WAVE, "one wave scans the (row group, wave) counts");
 #pragma unroll
-            for (int k = 0; k < HJ_ROWS; k++) {
+            for (int k = 0; k < ROWS; k++) {
                 const uint64_t b = pd::ballot(m[k] != 0);
                 if (l == 0) s_kc[k * WAVES + w] = pd::popc64(b);
             }
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
             const uint32_t tot = s_wsum[0];
             const uint64_t lt = pd::lanemask_lt();
 #pragma unroll
-            for (int k = 0; k < HJ_ROWS; k++) {
+            for (int k = 0; k < ROWS; k++) {
                 const uint64_t b = pd::ballot(m[k] != 0);
                 if (m[k]) {
                     const uint64_t o = wbase + written + s_kc[k * WAVES + w] + pd::popc64(b & lt);
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
         __syncthreads();
         uint32_t pre = 0, tot = 0;
 #pragma unroll
-        for (int i = 0; i < HJ_BLOCK / PD_WAVE; i++) {
+        for (int i = 0; i < BLOCK / PD_WAVE; i++) {
             const uint32_t v = s_wsum[i];
             pre += (i < (int)w) ? v : 0;
             tot += v;
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(HJ_BLOCK, 2) void k_hj_join(const uint32_t* __restr
         if (PHASE != HJ_COUNT) {
             uint64_t o = wbase + written + pre + inc - cnt;
 #pragma unroll
-            for (int k = 0; k < HJ_ROWS; k++) {
+            for (int k = 0; k < ROWS; k++) {
                 if (!m[k]) continue;
                 if (PHASE == HJ_UNIQUE) {
                     if (o < out_cap) { out_p[o] = pay[k]; out_b[o] = first[k]; }
@@ -290,7 +291,16 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
     uint32_t F = tuned > 0 ? next_pow2((uint64_t)tuned)
                            : next_pow2(std::max<uint64_t>(1, ceil_div(nb, (uint64_t)c->tune_get("hj_rows_per_part", 4096))));
     F = std::max<uint32_t>(F, 2);
-    const uint32_t CH = (uint32_t)c->tune_get("hj_chunk", 32768);
+    // probe rows per work item: 1.5x the average probe partition, so a
+    // typical partition is one work item (one LDS table build) while a
+    // skewed one is split over several blocks; at most np / (4 x CUs) rows,
+    // so there are always enough work items to fill the chip
+    const int64_t ch_tuned = c->tune_get("hj_chunk", 0);
+    uint32_t CH = (uint32_t)ch_tuned;
+    if (ch_tuned <= 0) {
+        CH = std::max<uint32_t>(8192, next_pow2(ceil_div(np, (uint64_t)F) * 3 / 2));
+        while (CH > 8192 && (uint64_t)CH * 4 * (uint64_t)c->cu_count > np) CH >>= 1;
+    }
     constexpr uint32_t GR = 1, GS = 1;   // partition f = [H[f], H[f + 1])
     Scratch rk(c, nb * 4, s), rp(c, nb * 4, s), sk(c, np * 4, s), sp(c, np * 4, s);
     Scratch flags(c, 16, s);
@@ -314,12 +324,15 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         Scratch wcnt(c, (max_wi + 1) * 4, s), woff(c, (max_wi + 1) * 4, s);
         fill_u32(c, flags.get(), 0, 4, s);
         fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
-        launch(c, "hj_join", k_hj_join<HJ_UNIQUE>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
-               (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
-               (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
-               (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
-               (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), (const uint32_t*)nullptr, p0, b0,
-               (uint64_t)np, flags.as<uint32_t>());
+#define PD_HJ_UNIQUE(B, R)                                                                                     \
+    launch(c, "hj_join", k_hj_join<HJ_UNIQUE, B, R>, dim3((uint32_t)max_wi), dim3(B), 0, s,                        \
+           (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(), (const uint32_t*)HR.as<uint32_t>(), \
+           GR, (const uint32_t*)sk.as<uint32_t>(), (const uint32_t*)sp.as<uint32_t>(),                             \
+           (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), \
+           (const uint32_t*)nullptr, p0, b0, (uint64_t)np, flags.as<uint32_t>())
+        if (c->tune_get("hj_block", 1024) == 1024) PD_HJ_UNIQUE(1024, 8);
+        else PD_HJ_UNIQUE(512, 16);
+#undef PD_HJ_UNIQUE
         uint32_t fl[4];
         PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));

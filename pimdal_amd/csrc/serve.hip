@@ -75,8 +75,8 @@ struct SvTables {
     const uint8_t* flag2;
     uint64_t n_fact;
     const int32_t* dim_date;
-    const uint32_t* dim_bkt;    // 64-byte buckets: 5 slots (key, dim row, dim date), word 15 = claims
-    uint64_t dim_nb;            // bucket count
+    const uint4* dim_index;     // (key, dim row + 1, dim date, 0); row 0 = empty slot
+    uint64_t dim_mask;
     const uint8_t* lut;         // flag codes: [0,256) flag1, [256,512) flag2
     uint32_t n2;
 };
@@ -90,37 +90,9 @@ __device__ __forceinline__ uint32_t sv_find_query(const SvKind& k, uint64_t wi) 
     return lo;
 }
 
-// Dimension-key index: 64-byte buckets of five 12-byte slots (key, dim row,
-// dim date) plus a claim counter in word 15, at about 0.9 load, so the
-// index (about 213 MB for 1.5e7 keys) stays resident in the 256 MB
-// Infinity Cache.  A probe reads one aligned 64-byte line (four 16-byte
-// loads); the date comes with the slot, so the join predicate needs no
-// second random read.  A bucket whose counter passed 5 overflowed into the
-// next bucket, so the probe continues there.
-constexpr uint32_t SV_BKT_SLOTS = 5;
-__device__ __forceinline__ uint64_t sv_bucket(uint32_t key, uint64_t nb) {
-    return ((pd_mix64(key) >> 32) * nb) >> 32;   // multiply-shift range reduction
-}
-
-__device__ __forceinline__ uint32_t sv_dim_lookup(const SvTables& T, int32_t key, int32_t* date) {
-    uint64_t b = sv_bucket((uint32_t)key, T.dim_nb);
-    while (true) {
-        const uint4* bp = reinterpret_cast<const uint4*>(T.dim_bkt + b * 16);
-        const uint4 q0 = bp[0], q1 = bp[1], q2 = bp[2], q3 = bp[3];
-        const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-        const uint32_t cnt = w[15];
-#pragma unroll
-        for (uint32_t sl = 0; sl < SV_BKT_SLOTS; sl++) {
-            if (sl < cnt && w[3 * sl] == (uint32_t)key) {
-                *date = (int32_t)w[3 * sl + 2];
-                return w[3 * sl + 1];
-            }
-        }
-        if (cnt <= SV_BKT_SLOTS) return 0xFFFFFFFFu;
-        b = (b + 1 == T.dim_nb) ? 0 : b + 1;
-    }
-}
+// Index slots are 16 bytes (key, dim row + 1, dim date, 0), row 0 = empty,
+// linear probing at load <= 0.5: one slot read per probe, and the date comes
+// with the slot, so the join predicate needs no second random read.
 
 __global__ void k_sv_reset(const SvBatch* __restrict__ b, uint64_t* __restrict__ status, SvResult* __restrict__ res,
                            uint32_t nres, uint32_t* __restrict__ ctr) {
@@ -160,21 +132,46 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
         const uint64_t rend = q.offset + q.rows;
         uint32_t hit = 0;
         uint32_t dimrow[JOIN ? RPT : 1];
+        if constexpr (JOIN) {
+            // all keys, then all first-slot probes in flight together; the
+            // rare longer chains are walked afterwards
+            int32_t key[RPT];
 #pragma unroll
-        for (int i = 0; i < RPT; i++) {
-            const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
-            bool m = false;
-            if (r < rend) {
-                if constexpr (JOIN) {
-                    int32_t dd = 0;
-                    const uint32_t d = sv_dim_lookup(T, T.orderkey[r], &dd);
-                    m = d != 0xFFFFFFFFu && (int64_t)dd < q.param;
-                    dimrow[i] = d;
-                } else {
-                    m = T.price[r] < q.param;
-                }
+            for (int i = 0; i < RPT; i++) {
+                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+                key[i] = r < rend ? T.orderkey[r] : 0;
             }
-            hit |= (uint32_t)m << i;
+            uint4 e[RPT];
+#pragma unroll
+            for (int i = 0; i < RPT; i++) e[i] = T.dim_index[pd_mix64((uint32_t)key[i]) & T.dim_mask];
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+                bool m = false;
+                dimrow[i] = 0xFFFFFFFFu;
+                if (r < rend) {
+                    uint4 x = e[i];
+                    if (x.y != 0 && x.x != (uint32_t)key[i]) {
+                        uint64_t hh = pd_mix64((uint32_t)key[i]) & T.dim_mask;
+                        do {
+                            hh = (hh + 1) & T.dim_mask;
+                            x = T.dim_index[hh];
+                        } while (x.y != 0 && x.x != (uint32_t)key[i]);
+                    }
+                    if (x.y != 0) {
+                        dimrow[i] = x.y - 1;
+                        m = (int64_t)(int32_t)x.z < q.param;
+                    }
+                }
+                hit |= (uint32_t)m << i;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+                const bool m = r < rend && T.price[r] < q.param;
+                hit |= (uint32_t)m << i;
+            }
         }
         // order (i, wave, lane): per item i, counts per wave
         __shared__ uint32_t s_cnt[RPT][SV_BLOCK / PD_WAVE];
@@ -358,29 +355,21 @@ __global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint64_t
 }
 
 __global__ void k_dim_index_build(const int32_t* __restrict__ key, const int32_t* __restrict__ date, uint64_t n,
-                                  uint32_t* __restrict__ bkt, uint64_t nb) {
+                                  uint4* __restrict__ idx, uint64_t mask, uint32_t* __restrict__ dup) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t b = sv_bucket((uint32_t)key[i], nb);
+        // claim a slot with its first 8 bytes (key, row + 1), then add the date
+        const uint64_t want = ((i + 1) << 32) | (uint32_t)key[i];
+        uint64_t h = pd_mix64((uint32_t)key[i]) & mask;
         while (true) {
-            uint32_t* B = bkt + b * 16;
-            const uint32_t sl = atomicAdd(&B[15], 1u);
-            if (sl < SV_BKT_SLOTS) {
-                B[3 * sl] = (uint32_t)key[i];
-                B[3 * sl + 1] = (uint32_t)i;
-                B[3 * sl + 2] = (uint32_t)date[i];
+            unsigned long long* slot = reinterpret_cast<unsigned long long*>(&idx[h]);
+            const uint64_t prev = atomicCAS(slot, 0ull, (unsigned long long)want);
+            if (prev == 0) {
+                idx[h].z = (uint32_t)date[i];
                 break;
             }
-            b = (b + 1 == nb) ? 0 : b + 1;
+            if ((uint32_t)prev == (uint32_t)key[i]) { atomicOr(dup, 1u); break; }
+            h = (h + 1) & mask;
         }
-    }
-}
-
-// Every dimension row must find itself: a repeated key makes one of its
-// rows find the other first.
-__global__ void k_dim_index_check(const int32_t* __restrict__ key, uint64_t n, SvTables T, uint32_t* __restrict__ bad) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        int32_t d = 0;
-        if (sv_dim_lookup(T, key[i], &d) != (uint32_t)i) atomicOr(bad, 1u);
     }
 }
 
@@ -554,21 +543,18 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         S->ctx = c;
         S->tab = t;
         hipStream_t s = c->stream;
-        // resident dimension-key index: 64-byte buckets at about 0.9 load
-        const double load = std::min(0.95, std::max(0.1, (double)c->tune_get("serve_index_load_pct", 90) / 100.0));
-        const uint64_t nb = std::max<uint64_t>(1, (uint64_t)((double)t.n_dim / (SV_BKT_SLOTS * load)) + 1);
-        PD_REQUIRE(nb < (1ull << 32) && t.n_dim < (1ull << 32), PIMDAL_E_INVALID, "dimension table too large");
-        S->dim_index = c->pool.alloc(nb * 64, s);
-        fill_u64(c, S->dim_index, 0, nb * 8, s);
+        // resident dimension-key index: power-of-two slots at load <= 0.5.
+        // Higher loads lengthen linear-probe chains, and a wave waits for its
+        // slowest lane, so the tail of the chain length sets the probe cost.
+        const double load = (double)c->tune_get("serve_index_load_pct", 50) / 100.0;
+        uint64_t cap = 1;
+        while ((double)cap * load < (double)t.n_dim) cap <<= 1;
+        S->dim_index = c->pool.alloc(cap * 16, s);
+        fill_u64(c, S->dim_index, 0, 2 * cap, s);
         Scratch dup(c, 4, s);
         fill_u32(c, dup.get(), 0, 1, s);
-        const dim3 ig((uint32_t)std::min<uint64_t>(ceil_div(t.n_dim, 256), 8192));
-        launch(c, "sv_dim_index", k_dim_index_build, ig, dim3(256), 0, s, t.dim_key, t.dim_date, t.n_dim,
-               (uint32_t*)S->dim_index, nb);
-        SvTables probe{};
-        probe.dim_bkt = (const uint32_t*)S->dim_index;
-        probe.dim_nb = nb;
-        launch(c, "sv_dim_index", k_dim_index_check, ig, dim3(256), 0, s, t.dim_key, t.n_dim, probe, dup.as<uint32_t>());
+        launch(c, "sv_dim_index", k_dim_index_build, dim3((uint32_t)std::min<uint64_t>(ceil_div(t.n_dim, 256), 8192)),
+               dim3(256), 0, s, t.dim_key, t.dim_date, t.n_dim, (uint4*)S->dim_index, cap - 1, dup.as<uint32_t>());
         uint32_t hd = 0;
         PD_HIP(hipMemcpyAsync(&hd, dup.get(), 4, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
@@ -580,7 +566,7 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         S->n1 = meta[0];
         S->n2 = meta[1];
         S->T = SvTables{t.orderkey, t.quantity, t.price, t.discount, t.tax, t.date, t.flag1, t.flag2, t.n_fact,
-                        t.dim_date, (const uint32_t*)S->dim_index, nb, (const uint8_t*)S->lut, S->n2};
+                        t.dim_date, (const uint4*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2};
         // slots
         S->nslots = slots;
         S->arena_bytes = slot_arena_bytes ? slot_arena_bytes : (1ull << 30);

@@ -14,27 +14,30 @@ def run(ctx, fn, rk, rp, sk, sp):
     return op.numpy(k), ob.numpy(k)
 
 
-HJ_MODE = {"hash": -1, "hash_part": 0, "hash_global": 1}
+HJ_MODE = {"hash": -1, "hash_part": 0, "hash_part_512": 0, "hash_global": 1}
 
 
 def join_fn(ctx, which):
     """hash = mode picked by build size (hj_mode -1), hash_part = radix-
-    partitioned LDS tables (0), hash_global = one global table (1),
+    partitioned LDS tables (0) probed by 1024-thread blocks, hash_part_512 =
+    the same probed by 512-thread blocks, hash_global = one global table (1),
     smj = sort-merge join."""
     if which == "smj":
         return ops.sort_merge_join
 
     def fn(c, *args):
         ctx.set_tune("hj_mode", HJ_MODE[which])
+        ctx.set_tune("hj_block", 512 if which == "hash_part_512" else 1024)
         try:
             return ops.hash_join(c, *args)
         finally:
             ctx.set_tune("hj_mode", -1)
+            ctx.set_tune("hj_block", 1024)
     return fn
 
 
 @pytest.mark.parametrize("zipf", [False, True])
-@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_512", "hash_global", "smj"])
 def test_join_pk_fk_2_20(ctx, orc, zipf, which):
     rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 20, zipf=zipf)
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
@@ -46,7 +49,7 @@ def test_join_pk_fk_2_20(ctx, orc, zipf, which):
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_512", "hash_global", "smj"])
 @pytest.mark.parametrize("nb,ns,kr,ks", [(1, 1, 1, 1), (3000, 5000, 1000, 2000), (100, 70_000, 60, 200),
                                          (50_000, 3, 10, 10), (4097, 8193, 1 << 31, 1 << 31)])
 def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
