@@ -267,6 +267,12 @@ def main():
     if args.only in ("", "all", "serve"):
         log("bench: serving mixed stream")
         ops_out["serve_mixed_stream"] = bench_serve(ctx)
+        # the same stream with batched joins probing the key index per row
+        # (no resident join index), to separate batching from the index
+        log("bench: serving mixed stream, probe index")
+        ctx.set_tune("serve_join_index", 0)
+        ops_out["serve_mixed_stream_probe_index"] = bench_serve(ctx)
+        ctx.set_tune("serve_join_index", 1)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -289,6 +295,7 @@ def main():
         "config": {"workload": "stable LSD radix sort of 2^28 (u32 key, u32 payload) pairs, keys uniform on [0,2^32)",
                    "rows": srt["n"], "passes": 4, "digit_bits": 8,
                    "alg_bytes_per_row": 68, "sort_alg_GBps": srt["alg_GBps"], "sort_hbm_frac": srt["hbm_frac"],
+                   "hist_ms": srt["hist_ms"],
                    "keys_payload_GBps": srt["keys_payload_GBps"]},
         "roofline": {"kernel": "radix_onesweep", "bound": "hbm",
                      "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

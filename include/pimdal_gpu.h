@@ -72,15 +72,18 @@ int pimdal_ctx_set_budget(pimdal_ctx* ctx, uint64_t hbm_budget_bytes);
 int pimdal_ctx_set_profiling(pimdal_ctx* ctx, int on);
 int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
-/* Tuning knobs by name: sort_block, sort_early_lookback, select_blocks_per_cu,
- * select_block (256, 512, 1024: 1024 rows per wave per tile),
- * select_lookback_width (1 or 4 status words per lane), select_mode (0 one
- * pass with decoupled look-back, 1 count pass + write pass), hj_mode (-1 by
- * build size, 0 partitioned, 1 global table), hj_global_max_build, hj_block
- * (probe block: 1024 threads x 8 rows or 512 x 16),
- * hj_fanout, hj_rows_per_part, hj_chunk, gb_groups_hint, gb_rows_per_part,
- * gb_lds_max_rows, gb_packed, serve_index_load_pct.  Unknown keys are stored
- * and ignored. */
+/* Tuning knobs by name (unknown keys are stored and ignored):
+ *   sort_block (256, 512), sort_early_lookback (0, 1), sort_rank (0 LDS
+ *   match masks, 1 per-bit ballots);
+ *   select_mode (0 one pass with decoupled look-back, 1 count pass + write
+ *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
+ *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;
+ *   hj_mode (-1 by build size, 0 partitioned, 1 global table),
+ *   hj_global_max_build, hj_block (probe block: 1024 threads x 8 rows or
+ *   512 x 16), hj_fanout, hj_rows_per_part, hj_chunk;
+ *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,
+ *   gb_q1_slots (1024 or 2048);
+ *   serve_index_load_pct. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
 /* Autotuner: times candidate knob values for `op` ("sort", "select",

@@ -265,10 +265,11 @@ struct AggSum {
     }
 };
 
-struct AggQ1 {
+template <int SLOTS>
+struct AggQ1T {
     static constexpr int NI = 4;   // qty, price, disc, tax
     static constexpr int NF = 7;   // qty, price, disc, disc_price, charge_lo, charge_hi, count
-    static constexpr int TS = 2048;
+    static constexpr int TS = SLOTS;
     __device__ static void fold(const int64_t* in, uint64_t* f) {
         const int64_t dp = in[1] * (100 - in[2]);
         const __int128 ch = (__int128)dp * (100 + in[3]);
@@ -281,6 +282,8 @@ struct AggQ1 {
         f[6] = 1;
     }
 };
+
+using AggQ1 = AggQ1T<2048>;
 
 // Field add with the 128-bit pair (4, 5) handled by carry for AggQ1.
 template <typename Agg>
@@ -1053,9 +1056,13 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         PD_HIP(hipStreamSynchronize(s));
         if (!overflow) {
             const OutQ1 o{out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count};
-            const uint64_t g = group_by_onesweep<AggQ1, RowsQ1Packed>(
-                c, pk.as<uint32_t>(), pw.get(), 8, n, [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; },
-                o, cap, s);
+            // LDS table per partition: 2048 slots (128 KB, one block per CU)
+            // or 1024 slots (64 KB, two blocks per CU, twice the partitions)
+            auto mk = [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; };
+            const uint64_t g =
+                c->tune_get("gb_q1_slots", 2048) == 1024
+                    ? group_by_onesweep<AggQ1T<1024>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s)
+                    : group_by_onesweep<AggQ1T<2048>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s);
             PD_HIP(hipStreamSynchronize(s));
             return g;
         }

@@ -13,9 +13,11 @@
 //    coalesced into one batch.  A batch is one hipGraph replay per slot:
 //    descriptor copy -> reset -> persistent select / join / group-by kernels
 //    that walk a device work list of (query, tile) items -> group-by
-//    finalisation -> result copy.  Joins probe a dimension-key index built
-//    once at server creation (the table stays resident, so the index does
-//    too).  Slots run on their own streams, so batches overlap on the GPU.
+//    finalisation -> result copy.  Joins read a foreign-key join index (the
+//    dimension row and date of every fact row, built once at server
+//    creation from a dimension-key index, since both tables stay resident),
+//    or with serve_join_index = 0 probe the dimension-key index per row.
+//    Slots run on their own streams, so batches overlap on the GPU.
 // Both modes report per query the result count and an order-independent
 // checksum (include/pimdal_gen.h), so results can be compared across modes
 // and with the CPU restatement.
@@ -79,6 +81,10 @@ struct SvTables {
     uint64_t dim_mask;
     const uint8_t* lut;         // flag codes: [0,256) flag1, [256,512) flag2
     uint32_t n2;
+    // foreign-key join index over the resident tables (NULL when off):
+    // dimension row of each fact row (0xFFFFFFFF = no match) and its date
+    const uint32_t* fk_row;
+    const int32_t* fk_date;
 };
 
 __device__ __forceinline__ uint32_t sv_find_query(const SvKind& k, uint64_t wi) {
@@ -133,6 +139,17 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
         uint32_t hit = 0;
         uint32_t dimrow[JOIN ? RPT : 1];
         if constexpr (JOIN) {
+          if (T.fk_row) {
+            // resident join index: the match and its date per fact row
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
+                dimrow[i] = r < rend ? T.fk_row[r] : 0xFFFFFFFFu;
+                const int32_t dd = r < rend ? T.fk_date[r] : 0;
+                const bool m = dimrow[i] != 0xFFFFFFFFu && (int64_t)dd < q.param;
+                hit |= (uint32_t)m << i;
+            }
+          } else {
             // all keys, then all first-slot probes in flight together; the
             // rare longer chains are walked afterwards
             int32_t key[RPT];
@@ -165,6 +182,7 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
                 }
                 hit |= (uint32_t)m << i;
             }
+          }
         } else {
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
@@ -373,6 +391,22 @@ __global__ void k_dim_index_build(const int32_t* __restrict__ key, const int32_t
     }
 }
 
+// Foreign-key join index: for every fact row, its dimension row (probing
+// the key index once, at server creation) and that row's date.
+__global__ void k_fk_index_build(SvTables T, uint32_t* __restrict__ fk_row, int32_t* __restrict__ fk_date) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < T.n_fact; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t key = (uint32_t)T.orderkey[r];
+        uint64_t h = pd_mix64(key) & T.dim_mask;
+        uint4 x = T.dim_index[h];
+        while (x.y != 0 && x.x != key) {
+            h = (h + 1) & T.dim_mask;
+            x = T.dim_index[h];
+        }
+        fk_row[r] = x.y != 0 ? x.y - 1 : 0xFFFFFFFFu;
+        fk_date[r] = x.y != 0 ? (int32_t)x.z : 0;
+    }
+}
+
 // checksum of select / join outputs of the sequential path
 __global__ void k_ck_rows(const uint32_t* __restrict__ rows, uint64_t n, uint64_t base, uint64_t* __restrict__ out) {
     uint64_t s = 0;
@@ -405,6 +439,8 @@ struct pimdal_server {
     pimdal_tables tab{};
     SvTables T{};
     void* dim_index = nullptr;
+    void* fk_row = nullptr;   // foreign-key join index (optional)
+    void* fk_date = nullptr;
     void* lut = nullptr;      // device [512] codes + [512] inverse
     uint8_t h_inv[512];
     uint32_t n1 = 0, n2 = 0;
@@ -566,7 +602,19 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         S->n1 = meta[0];
         S->n2 = meta[1];
         S->T = SvTables{t.orderkey, t.quantity, t.price, t.discount, t.tax, t.date, t.flag1, t.flag2, t.n_fact,
-                        t.dim_date, (const uint4*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2};
+                        t.dim_date, (const uint4*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2,
+                        nullptr, nullptr};
+        // foreign-key join index of the resident pair (fact.orderkey ->
+        // dim.key): batched joins then stream 8 bytes per fact row instead
+        // of one random index probe per row
+        if (c->tune_get("serve_join_index", 1)) {
+            S->fk_row = c->pool.alloc(t.n_fact * 4, s);
+            S->fk_date = c->pool.alloc(t.n_fact * 4, s);
+            launch(c, "sv_fk_index", k_fk_index_build, dim3((uint32_t)std::min<uint64_t>(ceil_div(t.n_fact, 256), 16384)),
+                   dim3(256), 0, s, S->T, (uint32_t*)S->fk_row, (int32_t*)S->fk_date);
+            S->T.fk_row = (const uint32_t*)S->fk_row;
+            S->T.fk_date = (const int32_t*)S->fk_date;
+        }
         // slots
         S->nslots = slots;
         S->arena_bytes = slot_arena_bytes ? slot_arena_bytes : (1ull << 30);
@@ -611,6 +659,8 @@ extern "C" int pimdal_server_destroy(pimdal_server* S) {
         if (sl.s) (void)hipStreamDestroy(sl.s);
     }
     if (S->dim_index) c->pool.free(S->dim_index, c->stream);
+    if (S->fk_row) c->pool.free(S->fk_row, c->stream);
+    if (S->fk_date) c->pool.free(S->fk_date, c->stream);
     if (S->lut) c->pool.free(S->lut, c->stream);
     delete S;
     PD_API_END

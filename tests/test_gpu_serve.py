@@ -11,6 +11,15 @@ pytestmark = pytest.mark.gpu
 N_FACT, N_DIM = 1 << 22, 1 << 20
 
 
+@pytest.fixture(params=[1, 0], ids=["join_index", "probe_index"], autouse=True)
+def join_mode(request, ctx):
+    """Batched joins read the resident foreign-key join index (1) or probe
+    the dimension-key index per row (0)."""
+    ctx.set_tune("serve_join_index", request.param)
+    yield request.param
+    ctx.set_tune("serve_join_index", 1)
+
+
 @pytest.fixture(scope="module")
 def tables(ctx):
     fact = datagen.fact_table(ctx, N_FACT, groups=N_DIM)
