@@ -276,8 +276,10 @@ typedef struct {
     uint64_t batches, max_inflight;
 } pimdal_serve_stats;
 
-/* Builds the resident dimension-key index and `slots` stream slots, each with
- * an output arena of slot_arena_bytes (0: 1 GiB) charged to the budget. */
+/* Builds the resident dimension-key index (and, unless the context's
+ * serve_join_index knob is 0, the foreign-key join index: the dimension row
+ * and date of every fact row) and `slots` stream slots, each with an output
+ * arena of slot_arena_bytes (0: 1 GiB) charged to the budget. */
 int pimdal_server_create(pimdal_ctx* ctx, const pimdal_tables* tables, int slots, uint64_t slot_arena_bytes,
                          pimdal_server** out);
 int pimdal_server_run(pimdal_server* server, const pimdal_query* queries, uint64_t n, int mode,

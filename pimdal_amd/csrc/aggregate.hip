@@ -1056,11 +1056,12 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         PD_HIP(hipStreamSynchronize(s));
         if (!overflow) {
             const OutQ1 o{out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count};
-            // LDS table per partition: 2048 slots (128 KB, one block per CU)
-            // or 1024 slots (64 KB, two blocks per CU, twice the partitions)
+            // LDS table per partition: 1024 slots (64 KB, two blocks per CU;
+            // default) or 2048 slots (128 KB, one block per CU, half the
+            // partitions)
             auto mk = [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; };
             const uint64_t g =
-                c->tune_get("gb_q1_slots", 2048) == 1024
+                c->tune_get("gb_q1_slots", 1024) == 1024
                     ? group_by_onesweep<AggQ1T<1024>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s)
                     : group_by_onesweep<AggQ1T<2048>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s);
             PD_HIP(hipStreamSynchronize(s));

@@ -48,7 +48,7 @@ __device__ __forceinline__ uint32_t rs_src(uint32_t k) {
 // Lanes spread their LDS atomics over RS_HCOPY copies of the histograms
 // (copy = lane mod RS_HCOPY), so lanes of one instruction that hit the same
 // bin are fewer (few-bin digits would otherwise serialize on one address).
-constexpr int RS_HCOPY = 4;
+constexpr int RS_HCOPY = 8;
 
 template <bool HASH>
 __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n, RsPassDesc pd,
@@ -398,14 +398,17 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         run_passes<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
 }
 
-// Histogram of the whole partition id ref_hash(key) & (F - 1) for F <= 2^14
+// Histogram of the whole partition id ref_hash(key) & (F - 1) for F <= 2^15
 // (one LDS atomic per key over F bins, so lanes of an instruction rarely
 // collide), folded in the same block into the per-pass digit histograms,
 // which are added to the global ones (npass x 256 atomics per block).
-constexpr uint32_t RS_FULL_MAX_BITS = 14;
-__global__ __launch_bounds__(512) void k_part_hist_full(const uint32_t* __restrict__ keys, uint64_t n, uint32_t F,
-                                                        RsPassDesc pd, uint32_t* __restrict__ g_hist) {
-    __shared__ uint32_t h[1u << RS_FULL_MAX_BITS];
+// MAXB = 14: 64 KB of counters, 512 threads; MAXB = 15: 128 KB, 1024
+// threads (one block per CU, 16 waves).
+constexpr uint32_t RS_FULL_MAX_BITS = 15;
+template <uint32_t MAXB, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_part_hist_full(const uint32_t* __restrict__ keys, uint64_t n, uint32_t F,
+                                                          RsPassDesc pd, uint32_t* __restrict__ g_hist) {
+    __shared__ uint32_t h[1u << MAXB];
     __shared__ uint32_t hd[4][RS_BINS];
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < F; i += blockDim.x) h[i] = 0;
@@ -479,8 +482,13 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
     }
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
     if (bits <= RS_FULL_MAX_BITS) {
-        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 16384), (uint64_t)c->cu_count * 2));
-        launch(c, "part_hist", k_part_hist_full, dim3(G), dim3(512), 0, s, kin, n, F, run, hist);
+        if (bits <= 14) {
+            const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 16384), (uint64_t)c->cu_count * 2));
+            launch(c, "part_hist", k_part_hist_full<14, 512>, dim3(G), dim3(512), 0, s, kin, n, F, run, hist);
+        } else {
+            const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 32768), (uint64_t)c->cu_count));
+            launch(c, "part_hist", k_part_hist_full<15, 1024>, dim3(G), dim3(1024), 0, s, kin, n, F, run, hist);
+        }
     } else {
         launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
     }

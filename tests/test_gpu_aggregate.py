@@ -173,13 +173,13 @@ def test_q1_flags_dense_group_counts(ctx, orc, n1, n2):
 
 
 @pytest.mark.parametrize("n,groups", [(100_000, 25_000), (1 << 21, 1 << 19)])
-def test_q1_by_orderkey_1024_slot_tables(ctx, orc, n, groups):
-    """Key group-by with 1024-slot LDS tables (two blocks per CU)."""
+def test_q1_by_orderkey_2048_slot_tables(ctx, orc, n, groups):
+    """Key group-by with 2048-slot LDS tables (one block per CU)."""
     t = orc.fact_table(n, groups=groups)
     d = {c: ctx.to_device(v) for c, v in t.items()}
-    ctx.set_tune("gb_q1_slots", 1024)
+    ctx.set_tune("gb_q1_slots", 2048)
     try:
         got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
     finally:
-        ctx.set_tune("gb_q1_slots", 2048)
+        ctx.set_tune("gb_q1_slots", 1024)
     _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
