@@ -74,7 +74,8 @@ int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launch
 int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
 /* Tuning knobs by name (unknown keys are stored and ignored):
  *   sort_block (256, 512), sort_early_lookback (0, 1), sort_rank (0 LDS
- *   match masks, 1 per-bit ballots);
+ *   match masks, 1 per-bit ballots), sort_two_phase (1: keys then 4-byte
+ *   payloads through one LDS buffer, 12 keys per thread);
  *   select_mode (0 one pass with decoupled look-back, 1 count pass + write
  *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
  *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;

@@ -25,6 +25,7 @@
 namespace pd {
 
 constexpr int RS_ITEMS = 16;                   // keys per thread; tile = block * 16
+constexpr int RS_TP_ITEMS = 12;                // keys per thread of the two-phase variant
 constexpr int RS_BINS = 256;
 constexpr uint32_t RS_FLAG_AGG = 1u << 30;
 constexpr uint32_t RS_FLAG_PRE = 2u << 30;
@@ -120,19 +121,27 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 // minimum number of waves per execution unit).  EARLY: resolve the global
 // offsets (look-back) before the LDS reorder instead of after it.
 // BALLOT: rank with per-bit ballots instead of LDS match masks.
-template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false>
-__global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
+// TP (4-byte payloads): one LDS buffer reorders the keys, then the payloads
+// (with the digit of every slot kept in a byte array), about 46 KB per
+// block instead of 70 KB, so three 512-thread blocks fit a CU; payloads are
+// loaded only after the keys are in LDS, so at most 32 data registers are
+// live (three blocks need at most 80 VGPRs).
+template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false, bool TP = false,
+          int ITEMS = RS_ITEMS>
+__global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
     uint32_t* __restrict__ tile_ctr) {
     constexpr int WAVES = BLOCK / PD_WAVE;
-    constexpr int TILE = BLOCK * RS_ITEMS;
+    constexpr int TILE = BLOCK * ITEMS;
     constexpr int LBW = 4;
     static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
+    static_assert(!TP || (HAS_VALS && sizeof(V) == 4 && !EARLY), "two-phase reorder is for 4-byte payloads");
     __shared__ uint32_t s_tile;
     // keys reordered by digit; the per-wave match masks live here during ranking
     __shared__ __align__(16) uint32_t s_keys[TILE];
-    __shared__ __align__(16) V s_vals[HAS_VALS ? TILE : 1];
+    __shared__ __align__(16) V s_vals[(HAS_VALS && !TP) ? TILE : 1];
+    __shared__ uint8_t s_dig[TP ? TILE : 1];
     __shared__ uint16_t s_cnt[WAVES][RS_BINS];
     __shared__ uint32_t s_off[RS_BINS];   // tile-local digit start
     __shared__ uint32_t s_gbase[RS_BINS]; // global position of the digit's run minus its local start
@@ -148,35 +157,38 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
     __syncthreads();
     const uint64_t tile = s_tile;
     const uint64_t tile_base = tile * TILE;
-    const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * RS_ITEMS);
+    const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * ITEMS);
     const uint32_t tile_count = (uint32_t)min((uint64_t)TILE, n - tile_base);
 
     // ---- load keys and payloads, wave-striped: idx = wave_base + i*64 + lane ----
     // Rows past n get the largest digit, so they rank after every real row
     // of the last tile and land past n (never written).
-    uint32_t key[RS_ITEMS];
-    V val[HAS_VALS ? RS_ITEMS : 1];
+    uint32_t key[ITEMS];
+    V val[HAS_VALS ? ITEMS : 1];
+    // rows of this wave that exist (wave-uniform), checked with 32-bit offsets
+    const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * ITEMS), n - wave_base) : 0));
+    const uint32_t* kw = kin + wave_base;
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++) {
-        const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
-        const bool ok = idx < n;
-        key[i] = ok ? kin[idx] : 0xFFFFFFFFu;
-        if constexpr (HAS_VALS) val[i] = ok ? vin[idx] : V(0);
+    for (int i = 0; i < ITEMS; i++) {
+        const uint32_t o = (uint32_t)i * PD_WAVE + l;
+        const bool ok = o < wrows;
+        key[i] = ok ? kw[o] : 0xFFFFFFFFu;
+        if constexpr (HAS_VALS && !TP) val[i] = ok ? vin[wave_base + o] : V(0);
     }
     auto digit_of = [&](int i) -> uint32_t {
-        const uint64_t idx = wave_base + (uint64_t)i * PD_WAVE + l;
-        return idx < n ? ((rs_src<HASH>(key[i]) >> shift) & mask) : (RS_BINS - 1);
+        return ((uint32_t)i * PD_WAVE + l) < wrows ? ((rs_src<HASH>(key[i]) >> shift) & mask) : (RS_BINS - 1);
     };
 
     // ---- stable rank inside the wave ----
     const uint64_t lt = pd::lanemask_lt();
-    uint32_t pos[RS_ITEMS];
+    uint32_t pos[ITEMS];
     if constexpr (BALLOT) {
         // lanes holding the same digit = AND over the digit's bits of the
         // bit's ballot (or its complement); the lowest such lane bumps the
         // wave's running count of the digit
 #pragma unroll
-        for (int i = 0; i < RS_ITEMS; i++) {
+        for (int i = 0; i < ITEMS; i++) {
             const uint32_t d = digit_of(i);
             uint64_t m = ~0ull;
 #pragma unroll
@@ -193,7 +205,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
         }
     } else
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint32_t d = digit_of(i);
         atomicOr((unsigned long long*)&s_match[w][d], (unsigned long long)(1ull << l));
         PD_COMPILER_BARRIER();
@@ -263,11 +275,27 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
 
     // ---- reorder the tile by digit in LDS (keys and payloads) ----
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint32_t d = digit_of(i);
         const uint32_t p = pos[i] + s_off[d] + s_cnt[w][d];
         s_keys[p] = key[i];
-        if constexpr (HAS_VALS) s_vals[p] = val[i];
+        if constexpr (TP) {
+            s_dig[p] = (uint8_t)d;
+            pos[i] = p;          // the payload's slot in the second phase
+        } else if constexpr (HAS_VALS) {
+            s_vals[p] = val[i];
+        }
+    }
+    if constexpr (TP) {
+        // payload loads go out now, behind the look-back and the key writes
+        // (kept below the reorder: the keys' registers are free by then)
+        PD_COMPILER_BARRIER();
+        const V* vw = vin + wave_base;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t o = (uint32_t)i * PD_WAVE + l;
+            val[i] = o < wrows ? vw[o] : V(0);
+        }
     }
     if (!EARLY && t < RS_BINS) {
         if (tile > 0) excl = lookback(cnt_d);
@@ -277,14 +305,29 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep(
 
     // ---- write keys and payloads in digit runs ----
 #pragma unroll 4
-    for (int k = 0; k < RS_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint32_t j = t + k * BLOCK;
         if (j >= tile_count) break;   // slots past tile_count hold padding rows
         const uint32_t kk = s_keys[j];
         const uint64_t dst = (uint64_t)s_gbase[(rs_src<HASH>(kk) >> shift) & mask] + j;
         if (dst < n) {
             kout[dst] = kk;
-            if constexpr (HAS_VALS) vout[dst] = s_vals[j];
+            if constexpr (HAS_VALS && !TP) vout[dst] = s_vals[j];
+        }
+    }
+    if constexpr (TP) {
+        // second phase: the payloads through the same buffer, same slots
+        __syncthreads();
+        V* s_v = reinterpret_cast<V*>(s_keys);
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) s_v[pos[i]] = val[i];
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t j = t + k * BLOCK;
+            if (j >= tile_count) break;
+            const uint64_t dst = (uint64_t)s_gbase[s_dig[j]] + j;
+            if (dst < n) vout[dst] = s_v[j];
         }
     }
 }
@@ -295,7 +338,9 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     const int block = (int)c->tune_get("sort_block", 512) == 256 ? 256 : 512;
     const bool early = c->tune_get("sort_early_lookback", 0) != 0;
     const bool ballot = c->tune_get("sort_rank", 0) == 1;
-    const uint64_t ntiles = ceil_div(n, (uint64_t)block * RS_ITEMS);
+    const bool two_phase = HV && sizeof(V) == 4 && c->tune_get("sort_two_phase", 0) == 1 && block == 512 && !early &&
+                           !ballot;
+    const uint64_t ntiles = ceil_div(n, (uint64_t)block * (two_phase ? RS_TP_ITEMS : RS_ITEMS));
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
     Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
@@ -315,13 +360,19 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, false, HASH, true>, dim3((uint32_t)ntiles),  \
            dim3(B), 0, s, ksrc, vsrc, kdst, \
            vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
-        if (block == 512 && ballot && !early) PD_RS_LAUNCH_B(512);
+#define PD_RS_LAUNCH_T(B)                                                                                          \
+    launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, false, HASH, false, HV && sizeof(V) == 4, RS_TP_ITEMS>, dim3((uint32_t)ntiles),  \
+           dim3(B), 0, s, ksrc, vsrc, kdst, \
+           vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
+        if (two_phase) PD_RS_LAUNCH_T(512);
+        else if (block == 512 && ballot && !early) PD_RS_LAUNCH_B(512);
         else if (block == 512 && early) PD_RS_LAUNCH(512, true);
         else if (block == 512) PD_RS_LAUNCH(512, false);
         else if (early) PD_RS_LAUNCH(256, true);
         else PD_RS_LAUNCH(256, false);
 #undef PD_RS_LAUNCH
 #undef PD_RS_LAUNCH_B
+#undef PD_RS_LAUNCH_T
         ksrc = kdst;
         vsrc = vdst;
     }

@@ -141,3 +141,38 @@ def test_sort_ballot_rank_u64_payload_and_hash_join(ctx, orc, ballot_rank):
     got = np.sort((op.numpy(cnt).astype(np.uint64) << np.uint64(32)) | ob.numpy(cnt))
     exp = np.sort((ep.astype(np.uint64) << np.uint64(32)) | eb)
     assert np.array_equal(got, exp)
+
+
+@pytest.fixture
+def two_phase(ctx):
+    """Onesweep with the two-phase LDS reorder (keys, then payloads through
+    one buffer) and 12 keys per thread (sort_two_phase 1)."""
+    ctx.set_tune("sort_two_phase", 1)
+    yield
+    ctx.set_tune("sort_two_phase", 0)
+
+
+@pytest.mark.parametrize("n", [1, 6143, 6144, 6145, 1 << 20])
+def test_sort_two_phase_uniform(ctx, orc, two_phase, n):
+    k, v = orc.sort_workload(n, seed=170 + n)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_two_phase_zipf_partial_bits_and_hash_join(ctx, orc, two_phase):
+    k, v = orc.sort_workload(1 << 21, zipf=True)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+    k, v = orc.sort_workload(70_000, seed=13)
+    gk, gv = gpu_sort(ctx, k, v, 4, 13)
+    mk = (k >> np.uint32(4)) & np.uint32((1 << 9) - 1)
+    order = np.argsort(mk, kind="stable")
+    assert np.array_equal(gv, v[order]) and np.array_equal(gk, k[order])
+    rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 19)   # partitioned: the hashed passes
+    op, ob, cnt = ops.hash_join(ctx, *[ctx.to_device(a) for a in (rk, rp, sk, sp)])
+    ep, eb = orc.hash_join(rk, rp, sk, sp)
+    got = np.sort((op.numpy(cnt).astype(np.uint64) << np.uint64(32)) | ob.numpy(cnt))
+    exp = np.sort((ep.astype(np.uint64) << np.uint64(32)) | eb)
+    assert np.array_equal(got, exp)
