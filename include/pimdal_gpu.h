@@ -73,9 +73,13 @@ int pimdal_ctx_set_profiling(pimdal_ctx* ctx, int on);
 int pimdal_ctx_kernel_stat(pimdal_ctx* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
 /* Tuning knobs by name (unknown keys are stored and ignored):
- *   sort_block (256, 512), sort_early_lookback (0, 1), sort_rank (0 LDS
- *   match masks, 1 per-bit ballots), sort_two_phase (1: keys then 4-byte
- *   payloads through one LDS buffer, 12 keys per thread);
+ *   sort_block (256, 512, 1024), sort_items (keys per thread: 16 or 8;
+ *   1024-thread blocks take 8), sort_early_lookback (0, 1), sort_rank (0 LDS
+ *   match masks, 1 per-bit ballots), sort_lbw (look-back statuses read per
+ *   round trip: 4, 8, 16, 32), sort_persistent (1: co-resident blocks loop
+ *   over tiles, loading the next tile during the current one's write
+ *   phase), sort_fused_hist (with sort_persistent: each pass counts the
+ *   next pass's digit, so only the first digit is counted up front);
  *   select_mode (0 one pass with decoupled look-back, 1 count pass + write
  *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
  *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;
@@ -87,6 +91,8 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   serve_index_load_pct. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
+/* Forgets a knob: the built-in default applies again. */
+int pimdal_ctx_clear_tune(pimdal_ctx* ctx, const char* key);
 /* Autotuner: times candidate knob values for `op` ("sort", "select",
  * "hash_join", "groupby") on synthetic device data of n rows, keeps the
  * fastest in the context, and appends them to cache_path (may be NULL).

@@ -51,6 +51,7 @@ SIGNATURES: dict[str, list] = {
     "pimdal_ctx_kernel_stat": [c_ctx_p, C.c_char_p, C.POINTER(u64), C.POINTER(dbl)],
     "pimdal_ctx_reset_stats": [c_ctx_p],
     "pimdal_ctx_set_tune": [c_ctx_p, C.c_char_p, i64],
+    "pimdal_ctx_clear_tune": [c_ctx_p, C.c_char_p],
     "pimdal_ctx_get_tune": [c_ctx_p, C.c_char_p, C.POINTER(i64)],
     "pimdal_autotune": [c_ctx_p, C.c_char_p, u64, C.c_char_p],
     "pimdal_ctx_load_tuning": [c_ctx_p, C.c_char_p],

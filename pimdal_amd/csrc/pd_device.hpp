@@ -20,6 +20,13 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ uint32_t popc64(uint64_t x) { return (uint32_t)__popcll(x); }
 
+// Buffer resource over `bytes` bytes from `base` (raw, untyped, 32-bit
+// offsets): loads past the range return 0 and stores past it are dropped by
+// the hardware range check.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
 // ---- agent-scope atomics (visible across the 8 XCDs' L2s) ----
 template <typename T>
 __device__ __forceinline__ T ld_acquire(const T* p) {

@@ -25,7 +25,6 @@
 namespace pd {
 
 constexpr int RS_ITEMS = 16;                   // keys per thread; tile = block * 16
-constexpr int RS_TP_ITEMS = 12;                // keys per thread of the two-phase variant
 constexpr int RS_BINS = 256;
 constexpr uint32_t RS_FLAG_AGG = 1u << 30;
 constexpr uint32_t RS_FLAG_PRE = 2u << 30;
@@ -62,8 +61,8 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
     uint32_t a = 0xFFFFFFFFu, o = 0;
     const uint64_t nvec = ((uintptr_t)keys % 16 == 0) ? n / 4 : 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + t; v < nvec; v += stride) {
-        const uint4 q = reinterpret_cast<const uint4*>(keys)[v];
+    const uint4* k4p = reinterpret_cast<const uint4*>(keys);
+    auto count4 = [&](const uint4 q) {
         const uint32_t k4[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -72,7 +71,17 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
             const uint32_t src = rs_src<HASH>(k4[e]);
             for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&hc[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
         }
+    };
+    // four 16-byte loads in flight per thread, then their atomics
+    uint64_t v = blockIdx.x * (uint64_t)blockDim.x + t;
+    for (; v + 3 * stride < nvec; v += 4 * stride) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = k4p[v + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++) count4(q[u]);
     }
+    for (; v < nvec; v += stride) count4(k4p[v]);
     for (uint64_t i = nvec * 4 + blockIdx.x * (uint64_t)blockDim.x + t; i < n; i += stride) {
         const uint32_t k = keys[i];
         a &= k;
@@ -115,40 +124,43 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 // ---- one onesweep pass ----
 // Compiler-only barrier: LDS instructions of one wave execute in issue order,
 // so the ranking steps below need ordering against code motion, not waits.
+// (Plain LDS accesses between these barriers, not volatile ones: the
+// compiler follows a volatile access with a wait for every outstanding
+// global load and store too, which would stall the ranking on the payload
+// loads and, in the persistent pass, on the previous tile's stores.)
 #define PD_COMPILER_BARRIER() __asm__ volatile("" ::: "memory")
 
 // 4 waves per SIMD: at most 128 VGPRs (second launch-bounds argument is the
 // minimum number of waves per execution unit).  EARLY: resolve the global
 // offsets (look-back) before the LDS reorder instead of after it.
 // BALLOT: rank with per-bit ballots instead of LDS match masks.
-// TP (4-byte payloads): one LDS buffer reorders the keys, then the payloads
-// (with the digit of every slot kept in a byte array), about 46 KB per
-// block instead of 70 KB, so three 512-thread blocks fit a CU; payloads are
-// loaded only after the keys are in LDS, so at most 32 data registers are
-// live (three blocks need at most 80 VGPRs).
-template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false, bool TP = false,
+// LBW: statuses each digit's thread reads per look-back round trip.  A
+// status written on another XCD is read from memory (the L2s are per XCD),
+// so one round trip costs about a microsecond; with tiles finishing at tens
+// per microsecond, a tile typically walks back over dozens of published
+// aggregates before it reaches an inclusive prefix, and a wide window
+// covers them in one or two round trips.
+template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false, int LBW = 4,
           int ITEMS = RS_ITEMS>
-__global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
+__global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
     uint32_t* __restrict__ tile_ctr) {
     constexpr int WAVES = BLOCK / PD_WAVE;
     constexpr int TILE = BLOCK * ITEMS;
-    constexpr int LBW = 4;
     static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
-    static_assert(!TP || (HAS_VALS && sizeof(V) == 4 && !EARLY), "two-phase reorder is for 4-byte payloads");
     __shared__ uint32_t s_tile;
     // keys reordered by digit; the per-wave match masks live here during ranking
     __shared__ __align__(16) uint32_t s_keys[TILE];
-    __shared__ __align__(16) V s_vals[(HAS_VALS && !TP) ? TILE : 1];
-    __shared__ uint8_t s_dig[TP ? TILE : 1];
+    __shared__ __align__(16) V s_vals[HAS_VALS ? TILE : 1];
     __shared__ uint16_t s_cnt[WAVES][RS_BINS];
     __shared__ uint32_t s_off[RS_BINS];   // tile-local digit start
     __shared__ uint32_t s_gbase[RS_BINS]; // global position of the digit's run minus its local start
     __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
     auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);
 
-    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint32_t t = threadIdx.x, l = pd::lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
     for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
         s_match[w][i] = 0;
@@ -168,16 +180,29 @@ __global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
     // rows of this wave that exist (wave-uniform), checked with 32-bit offsets
     const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
         (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * ITEMS), n - wave_base) : 0));
-    const uint32_t* kw = kin + wave_base;
+    // buffer loads whose range ends at the wave's last row (0 past it)
+    const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(kin + wave_base, wrows * 4u);
 #pragma unroll
-    for (int i = 0; i < ITEMS; i++) {
-        const uint32_t o = (uint32_t)i * PD_WAVE + l;
-        const bool ok = o < wrows;
-        key[i] = ok ? kw[o] : 0xFFFFFFFFu;
-        if constexpr (HAS_VALS && !TP) val[i] = ok ? vin[wave_base + o] : V(0);
+    for (int i = 0; i < ITEMS; i++)
+        key[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + (uint32_t)i * PD_WAVE) * 4u, 0, 0);
+    if constexpr (HAS_VALS) {
+        const __amdgpu_buffer_rsrc_t rv = pd::buf_rsrc(vin + wave_base, wrows * (uint32_t)sizeof(V));
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            if constexpr (sizeof(V) == 4)
+                val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, (l + (uint32_t)i * PD_WAVE) * 4u, 0, 0);
+            else
+                val[i] = __builtin_bit_cast(
+                    V, __builtin_amdgcn_raw_buffer_load_b64(rv, (l + (uint32_t)i * PD_WAVE) * 8u, 0, 0));
+        }
     }
+    // digit of item i, computed where it is used: the key goes through an
+    // opaque copy, so the compiler cannot compute all digits up front (and
+    // hold them in registers beside the keys)
     auto digit_of = [&](int i) -> uint32_t {
-        return ((uint32_t)i * PD_WAVE + l) < wrows ? ((rs_src<HASH>(key[i]) >> shift) & mask) : (RS_BINS - 1);
+        uint32_t kk = key[i];
+        asm volatile("" : "+v"(kk));
+        return ((uint32_t)i * PD_WAVE + l) < wrows ? ((rs_src<HASH>(kk) >> shift) & mask) : (RS_BINS - 1);
     };
 
     // ---- stable rank inside the wave ----
@@ -197,10 +222,10 @@ __global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
                 m &= ((d >> b) & 1) ? bb : ~bb;
             }
             const uint32_t below = pd::popc64(m & lt);
-            const uint32_t old = *(volatile uint16_t*)&s_cnt[w][d];
-            pos[i] = old + below;
+            const uint32_t old = s_cnt[w][d];
+            pos[i] = (old + below) | (d << 16);
             PD_COMPILER_BARRIER();
-            if (below == 0) *(volatile uint16_t*)&s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
+            if (below == 0) s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
             PD_COMPILER_BARRIER();
         }
     } else
@@ -209,14 +234,14 @@ __global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
         const uint32_t d = digit_of(i);
         atomicOr((unsigned long long*)&s_match[w][d], (unsigned long long)(1ull << l));
         PD_COMPILER_BARRIER();
-        const uint64_t m = *(volatile uint64_t*)&s_match[w][d];
-        const uint32_t old = *(volatile uint16_t*)&s_cnt[w][d];
+        const uint64_t m = s_match[w][d];
+        const uint32_t old = s_cnt[w][d];
         const uint32_t below = pd::popc64(m & lt);
-        pos[i] = old + below;
+        pos[i] = (old + below) | (d << 16);
         PD_COMPILER_BARRIER();
         if (below == 0) {
-            *(volatile uint16_t*)&s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
-            *(volatile uint64_t*)&s_match[w][d] = 0;
+            s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
+            s_match[w][d] = 0;
         }
         PD_COMPILER_BARRIER();
     }
@@ -276,26 +301,10 @@ __global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
     // ---- reorder the tile by digit in LDS (keys and payloads) ----
 #pragma unroll
     for (int i = 0; i < ITEMS; i++) {
-        const uint32_t d = digit_of(i);
-        const uint32_t p = pos[i] + s_off[d] + s_cnt[w][d];
+        const uint32_t d = pos[i] >> 16;
+        const uint32_t p = (pos[i] & 0xFFFFu) + s_off[d] + s_cnt[w][d];
         s_keys[p] = key[i];
-        if constexpr (TP) {
-            s_dig[p] = (uint8_t)d;
-            pos[i] = p;          // the payload's slot in the second phase
-        } else if constexpr (HAS_VALS) {
-            s_vals[p] = val[i];
-        }
-    }
-    if constexpr (TP) {
-        // payload loads go out now, behind the look-back and the key writes
-        // (kept below the reorder: the keys' registers are free by then)
-        PD_COMPILER_BARRIER();
-        const V* vw = vin + wave_base;
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t o = (uint32_t)i * PD_WAVE + l;
-            val[i] = o < wrows ? vw[o] : V(0);
-        }
+        if constexpr (HAS_VALS) s_vals[p] = val[i];
     }
     if (!EARLY && t < RS_BINS) {
         if (tile > 0) excl = lookback(cnt_d);
@@ -312,67 +321,352 @@ __global__ __launch_bounds__(BLOCK, TP ? 6 : 4) void k_rs_onesweep(
         const uint64_t dst = (uint64_t)s_gbase[(rs_src<HASH>(kk) >> shift) & mask] + j;
         if (dst < n) {
             kout[dst] = kk;
-            if constexpr (HAS_VALS && !TP) vout[dst] = s_vals[j];
+            if constexpr (HAS_VALS) vout[dst] = s_vals[j];
         }
     }
-    if constexpr (TP) {
-        // second phase: the payloads through the same buffer, same slots
-        __syncthreads();
-        V* s_v = reinterpret_cast<V*>(s_keys);
+
+}
+
+// ---- persistent onesweep pass ----
+// gridDim.x blocks (as many as are co-resident) loop over tiles taken in
+// order from tile_ctr.  Right after a tile has been reordered into LDS, the
+// block takes its next tile and loads its keys and payloads into the same
+// registers, so that load's latency overlaps this tile's look-back and
+// write phase.  A block waits (look-back) only on tiles taken before its
+// own, all of which are held by running blocks, so every wait ends.
+// NEXT: also count the digit (src >> nshift) & nmask of every row into
+// g_nhist, the next pass's histogram (complete when this pass ends), so
+// only the first pass needs a histogram read of the keys.  The counters
+// are spread over RS_NCOPY LDS copies by lane.
+constexpr int RS_NCOPY = 4;
+template <typename V, bool HAS_VALS, int BLOCK, bool HASH, bool NEXT, int LBW = 16, int ITEMS = RS_ITEMS>
+__global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep_pf(
+    const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
+    uint32_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ tile_ctr, uint32_t ntiles, uint32_t nshift, uint32_t nmask,
+    uint32_t* __restrict__ g_nhist) {
+    constexpr int WAVES = BLOCK / PD_WAVE;
+    constexpr int TILE = BLOCK * ITEMS;
+    static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
+    __shared__ uint32_t s_next;
+    __shared__ __align__(16) uint32_t s_keys[TILE];
+    __shared__ __align__(16) V s_vals[HAS_VALS ? TILE : 1];
+    __shared__ uint16_t s_cnt[WAVES][RS_BINS];
+    __shared__ uint32_t s_off[RS_BINS];
+    __shared__ uint32_t s_gbase[RS_BINS];
+    __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
+    __shared__ uint32_t s_nh[NEXT ? RS_NCOPY : 1][NEXT ? RS_BINS : 1];
+    auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);
+
+    const uint32_t t0 = threadIdx.x, l0 = pd::lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t0 / PD_WAVE);
+    if constexpr (NEXT)
+        for (uint32_t i = t0; i < RS_NCOPY * RS_BINS; i += BLOCK) (&s_nh[0][0])[i] = 0;
+    if (t0 == 0) s_next = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    uint32_t tile = s_next;
+
+    uint32_t key[ITEMS];
+    V val[HAS_VALS ? ITEMS : 1];
+    uint32_t wrows = 0;   // rows of this wave in the current tile (wave-uniform)
+    // wave-striped rows: wave_base + i*64 + lane, through buffer loads whose
+    // range ends at the wave's last row (0 past it); rows past n get the
+    // largest digit, so they rank after every real row and are never written
+    // keys of a tile: loaded one tile ahead (during the previous tile's
+    // look-back and writes); payloads: loaded when the tile's turn comes,
+    // in flight during its ranking (so only the keys stay live across the
+    // write phase)
+    auto load_keys = [&](uint32_t tl, uint32_t lane4) {
+        asm volatile("" : "+v"(lane4));   // lane * 4 + i * 256: immediate offsets, nothing hoisted
+        const uint32_t wave_base = tl * (uint32_t)TILE + w * (uint32_t)(PD_WAVE * ITEMS);
+        wrows = n > wave_base ? min((uint32_t)(PD_WAVE * ITEMS), n - wave_base) : 0u;
+        const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(kin + wave_base, wrows * 4u);
 #pragma unroll
-        for (int i = 0; i < ITEMS; i++) s_v[pos[i]] = val[i];
+        for (int i = 0; i < ITEMS; i++)
+            key[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, lane4 + (uint32_t)i * PD_WAVE * 4u, 0, 0);
+    };
+    auto load_vals = [&](uint32_t tl, uint32_t lane4) {
+        asm volatile("" : "+v"(lane4));
+        const uint32_t wave_base = tl * (uint32_t)TILE + w * (uint32_t)(PD_WAVE * ITEMS);
+        if constexpr (HAS_VALS) {
+            const __amdgpu_buffer_rsrc_t rv = pd::buf_rsrc(vin + wave_base, wrows * (uint32_t)sizeof(V));
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                if constexpr (sizeof(V) == 4)
+                    val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, lane4 + (uint32_t)i * PD_WAVE * 4u, 0, 0);
+                else
+                    val[i] = __builtin_bit_cast(
+                        V, __builtin_amdgcn_raw_buffer_load_b64(rv, 2u * lane4 + (uint32_t)i * PD_WAVE * 8u, 0, 0));
+            }
+        }
+    };
+    const __amdgpu_buffer_rsrc_t rko = pd::buf_rsrc(kout, n * 4u);
+    const __amdgpu_buffer_rsrc_t rvo = pd::buf_rsrc(vout, HAS_VALS ? n * (uint32_t)sizeof(V) : 0u);
+    if (tile < ntiles) load_keys(tile, l0 * 4u);
+
+    while (tile < ntiles) {
+        // lane id recomputed every tile by an opaque instruction pair, and
+        // the thread id from it and the wave number: nothing derived from
+        // them is hoisted out of the loop (and spilled; a scratch reload is
+        // a vector-memory access, and waiting for it waits for every
+        // earlier load and store of the wave)
+        uint32_t l;
+        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+        const uint32_t t = w * PD_WAVE + l;
+        const uint64_t lt = (1ull << l) - 1ull;
+        const uint32_t tile_count = min((uint32_t)TILE, n - tile * (uint32_t)TILE);
+        // next ticket: issued now, its value is needed only after the
+        // ranking.  The opaque zero offset keeps it a plain returning
+        // atomic (a wave-aggregated one reads its result back at once).
+        uint32_t my_next = 0;
+        if (t == 0) {
+            uint32_t z = 0;
+            asm volatile("" : "+v"(z));
+            my_next = atomicAdd(tile_ctr + z, 1u);
+        }
+        load_vals(tile, l * 4u);
+        for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
+            s_match[w][i] = 0;
+            s_cnt[w][i] = 0;
+        }
         __syncthreads();
+        auto digit_of = [&](int i) -> uint32_t {
+            uint32_t kk = key[i];
+            asm volatile("" : "+v"(kk));   // computed where used (see the single-tile pass)
+            return ((uint32_t)i * PD_WAVE + l) < wrows ? ((rs_src<HASH>(kk) >> shift) & mask) : (RS_BINS - 1);
+        };
+        // ---- stable rank inside the wave (LDS match masks) ----
+        uint32_t pos[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t d = digit_of(i);
+            atomicOr((unsigned long long*)&s_match[w][d], (unsigned long long)(1ull << l));
+            PD_COMPILER_BARRIER();
+            const uint64_t m = s_match[w][d];
+            const uint32_t old = s_cnt[w][d];
+            const uint32_t below = pd::popc64(m & lt);
+            pos[i] = (old + below) | (d << 16);
+            PD_COMPILER_BARRIER();
+            if (below == 0) {
+                s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
+                s_match[w][d] = 0;
+            }
+            PD_COMPILER_BARRIER();
+        }
+        if constexpr (NEXT) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++)
+                if (((uint32_t)i * PD_WAVE + l) < wrows)
+                    atomicAdd(&s_nh[l % RS_NCOPY][(rs_src<HASH>(key[i]) >> nshift) & nmask], 1u);
+        }
+        __syncthreads();
+
+        // ---- per digit: prefix over waves, tile count, publish the aggregate ----
+        uint32_t cnt_d = 0, inc_d = 0;
+        if (t < RS_BINS) {
+#pragma unroll
+            for (int ww = 0; ww < WAVES; ww++) {
+                const uint32_t cw = s_cnt[ww][t];
+                s_cnt[ww][t] = (uint16_t)cnt_d;
+                cnt_d += cw;
+            }
+            pd::st_relaxed(&status[(uint64_t)tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
+            inc_d = pd::wave_incl_scan(cnt_d);
+            if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
+        }
+        if (t == 0) s_next = my_next;
+        __syncthreads();
+        uint32_t local_d = 0;
+        if (t < RS_BINS) {
+            uint32_t add = 0;
+            for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
+            local_d = add + inc_d - cnt_d;
+            s_off[t] = local_d;
+        }
+        __syncthreads();
+
+        // ---- reorder the tile by digit in LDS ----
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t d = pos[i] >> 16;
+            const uint32_t p = (pos[i] & 0xFFFFu) + s_off[d] + s_cnt[w][d];
+            s_keys[p] = key[i];
+            if constexpr (HAS_VALS) s_vals[p] = val[i];
+        }
+        // ---- first look-back window (thread t = digit t), then the next
+        // tile's loads: the window is issued first, so waiting for it does
+        // not wait for the prefetch ----
+        const bool lb = t < RS_BINS && tile > 0;
+        uint32_t st[LBW];
+        if (lb) {
+#pragma unroll
+            for (int q = 0; q < LBW; q++)
+                st[q] = ((int64_t)tile - 1 - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(tile - 1 - q) * RS_BINS + t])
+                                                     : RS_FLAG_PRE;
+        }
+        PD_COMPILER_BARRIER();
+        const uint32_t next = s_next;
+        if (next < ntiles) load_keys(next, l * 4u);
+        PD_COMPILER_BARRIER();
+
+        // ---- per-digit look-back over earlier tiles ----
+        if (t < RS_BINS) {
+            uint32_t excl = 0;
+            if (lb) {
+                int64_t j = (int64_t)tile - 1;
+                bool done = false, first = true;
+                while (!done) {
+                    if (!first) {
+#pragma unroll
+                        for (int q = 0; q < LBW; q++)
+                            st[q] = (j - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
+                    }
+                    first = false;
+                    bool stall = false;
+#pragma unroll
+                    for (int q = 0; q < LBW; q++) {
+                        if (done || stall) continue;
+                        if ((st[q] >> 30) == 0) { stall = true; continue; }
+                        excl += st[q] & RS_VALUE;
+                        if ((st[q] >> 30) == 2) done = true;
+                        else j--;
+                    }
+                    if (stall) __builtin_amdgcn_s_sleep(1);
+                }
+                pd::st_relaxed(&status[(uint64_t)tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt_d));
+            }
+            s_gbase[t] = offs[t] + excl - local_d;
+        }
+        __syncthreads();
+
+        // ---- write keys and payloads in digit runs ----
 #pragma unroll 4
         for (int k = 0; k < ITEMS; k++) {
             const uint32_t j = t + k * BLOCK;
             if (j >= tile_count) break;
-            const uint64_t dst = (uint64_t)s_gbase[s_dig[j]] + j;
-            if (dst < n) vout[dst] = s_v[j];
+            const uint32_t kk = s_keys[j];
+            const uint32_t dst = s_gbase[(rs_src<HASH>(kk) >> shift) & mask] + j;
+            __builtin_amdgcn_raw_buffer_store_b32(kk, rko, dst * 4u, 0, 0);
+            if constexpr (HAS_VALS) {
+                if constexpr (sizeof(V) == 4)
+                    __builtin_amdgcn_raw_buffer_store_b32((uint32_t)s_vals[j], rvo, dst * 4u, 0, 0);
+                else
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, s_vals[j]),
+                                                          rvo, dst * 8u, 0, 0);
+            }
+        }
+        __syncthreads();   // s_keys, s_vals, s_gbase are rewritten by the next tile
+        tile = next;
+    }
+    if constexpr (NEXT) {
+        for (uint32_t i = t0; i < RS_BINS; i += BLOCK) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int cp = 0; cp < RS_NCOPY; cp++) v += s_nh[cp][i];
+            if (v) atomicAdd(&g_nhist[i], v);
         }
     }
 }
 
-template <typename V, bool HV, bool HASH = false>
-static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
-                       const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
-    const int block = (int)c->tune_get("sort_block", 512) == 256 ? 256 : 512;
-    const bool early = c->tune_get("sort_early_lookback", 0) != 0;
-    const bool ballot = c->tune_get("sort_rank", 0) == 1;
-    const bool two_phase = HV && sizeof(V) == 4 && c->tune_get("sort_two_phase", 0) == 1 && block == 512 && !early &&
-                           !ballot;
-    const uint64_t ntiles = ceil_div(n, (uint64_t)block * (two_phase ? RS_TP_ITEMS : RS_ITEMS));
+// Look-back window of the passes (knob sort_lbw: 4, 8, 16 or 32).
+static int sort_lbw(pimdal_ctx* c) {
+    const int64_t v = c->tune_get("sort_lbw", 4);
+    return v >= 32 ? 32 : v >= 16 ? 16 : v >= 8 ? 8 : 4;
+}
+
+// Co-resident blocks per CU of a kernel at a block size (at least 1).
+template <typename K>
+static uint32_t blocks_per_cu(K kernel, int block) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1) nb = 1;
+    return (uint32_t)nb;
+}
+
+// Persistent passes.  hist_fused: only pass 0's histogram is in hist; each
+// pass counts the next pass's digit and its offsets are scanned before it
+// runs.
+template <typename V, bool HV, bool HASH>
+static void run_passes_pf(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
+                          const RsPassDesc& pdsc, uint32_t* hist, uint32_t* offs, uint32_t* ctrs, bool hist_fused,
+                          hipStream_t s) {
+    constexpr int B = 512;
+    const int lbw = sort_lbw(c);
+    const uint32_t ntiles = (uint32_t)ceil_div(n, (uint64_t)B * RS_ITEMS);
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
     Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
-    Scratch st(c, ntiles * RS_BINS * 4, s);
+    Scratch st(c, (uint64_t)ntiles * RS_BINS * 4, s);
     const uint32_t* ksrc = kin;
     const V* vsrc = vin;
     for (uint32_t j = 0; j < P; j++) {
         const bool to_out = ((P - 1 - j) % 2) == 0;
         uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
         V* vdst = to_out ? vout : tv.as<V>();
+        fill_u32(c, st.get(), 0, (uint64_t)ntiles * RS_BINS, s);
+        if (hist_fused && j > 0)
+            launch(c, "radix_scan", k_rs_scan, dim3(1), dim3(RS_BINS), 0, s, (const uint32_t*)(hist + j * RS_BINS),
+                   offs + j * RS_BINS);
+        const bool next = hist_fused && j + 1 < P;
+        const uint32_t nshift = next ? pdsc.shift[j + 1] : 0, nmask = next ? pdsc.mask[j + 1] : 0;
+        uint32_t* nh = next ? hist + (j + 1) * RS_BINS : nullptr;
+        const char* name = HASH ? "part_onesweep" : "radix_onesweep";
+#define PD_PF_LAUNCH(NX, W)                                                                                          \
+    do {                                                                                                              \
+        auto k = k_rs_onesweep_pf<V, HV, B, HASH, NX, W>;                                                             \
+        const uint32_t g = std::min<uint32_t>(ntiles, (uint32_t)c->cu_count * blocks_per_cu(k, B));                   \
+        launch(c, name, k, dim3(g), dim3(B), 0, s, ksrc, vsrc, kdst, vdst, (uint32_t)n, pdsc.shift[j], pdsc.mask[j],   \
+               (const uint32_t*)(offs + j * RS_BINS), st.as<uint32_t>(), ctrs + j, ntiles, nshift, nmask, nh);         \
+    } while (0)
+        if (next) {
+            if (lbw >= 16) PD_PF_LAUNCH(true, 16); else PD_PF_LAUNCH(true, 4);
+        } else {
+            if (lbw >= 16) PD_PF_LAUNCH(false, 16); else PD_PF_LAUNCH(false, 4);
+        }
+#undef PD_PF_LAUNCH
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+}
+
+template <typename V, bool HV, bool HASH = false>
+static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
+                       const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
+    const int64_t bk = c->tune_get("sort_block", 512);
+    const int block = bk >= 1024 ? 1024 : bk <= 256 ? 256 : 512;
+    // keys per thread: 16, or 8 (1024-thread blocks always take 8, so a
+    // tile stays 8192 keys and two blocks still fit a CU's LDS)
+    const int items = (block == 1024 || c->tune_get("sort_items", 16) == 8) ? 8 : 16;
+    const bool early = c->tune_get("sort_early_lookback", 0) != 0;
+    const bool ballot = c->tune_get("sort_rank", 0) == 1;
+    const int lbw = sort_lbw(c);
+    const uint64_t ntiles = ceil_div(n, (uint64_t)block * items);
+    const uint32_t P = pdsc.npass;
+    Scratch tk(c, P > 1 ? n * 4 : 0, s);
+    Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
+    Scratch st(c, ntiles * RS_BINS * 4, s);
+    const uint32_t* ksrc = kin;
+    const V* vsrc = vin;
+    const char* name = HASH ? "part_onesweep" : "radix_onesweep";
+    for (uint32_t j = 0; j < P; j++) {
+        const bool to_out = ((P - 1 - j) % 2) == 0;
+        uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
+        V* vdst = to_out ? vout : tv.as<V>();
         fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
-#define PD_RS_LAUNCH(B, E)                                                                                       \
-    launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, E, HASH>, dim3((uint32_t)ntiles),  \
-           dim3(B), 0, s, ksrc, vsrc, kdst, \
-           vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
-#define PD_RS_LAUNCH_B(B)                                                                                          \
-    launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, false, HASH, true>, dim3((uint32_t)ntiles),  \
-           dim3(B), 0, s, ksrc, vsrc, kdst, \
-           vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
-#define PD_RS_LAUNCH_T(B)                                                                                          \
-    launch(c, HASH ? "part_onesweep" : "radix_onesweep", k_rs_onesweep<V, HV, B, false, HASH, false, HV && sizeof(V) == 4, RS_TP_ITEMS>, dim3((uint32_t)ntiles),  \
-           dim3(B), 0, s, ksrc, vsrc, kdst, \
-           vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
-        if (two_phase) PD_RS_LAUNCH_T(512);
-        else if (block == 512 && ballot && !early) PD_RS_LAUNCH_B(512);
-        else if (block == 512 && early) PD_RS_LAUNCH(512, true);
-        else if (block == 512) PD_RS_LAUNCH(512, false);
-        else if (early) PD_RS_LAUNCH(256, true);
-        else PD_RS_LAUNCH(256, false);
+#define PD_RS_LAUNCH(...)                                                                                         \
+    launch(c, name, k_rs_onesweep<__VA_ARGS__>, dim3((uint32_t)ntiles), dim3(block), 0, s, ksrc, vsrc, kdst, vdst, n, \
+           pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
+        if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
+        else if (items == 8 && block == 512) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 8);
+        else if (block == 512 && !early && !ballot) {
+            if (lbw == 32) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 32);
+            else if (lbw == 16) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 16);
+            else if (lbw == 8) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 8);
+            else PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4);
+        } else if (block == 512 && ballot && !early) PD_RS_LAUNCH(V, HV, 512, false, HASH, true, 16);
+        else if (block == 512) PD_RS_LAUNCH(V, HV, 512, true, HASH, false, 16);
+        else if (early) PD_RS_LAUNCH(V, HV, 256, true, HASH, false, 16);
+        else PD_RS_LAUNCH(V, HV, 256, false, HASH, false, 16);
 #undef PD_RS_LAUNCH
-#undef PD_RS_LAUNCH_B
-#undef PD_RS_LAUNCH_T
         ksrc = kdst;
         vsrc = vdst;
     }
@@ -414,11 +708,43 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         }
     }
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
+    // persistent passes; fused: the histogram read counts only the first
+    // digit, and each pass counts the next one
+    const bool persistent = c->tune_get("sort_persistent", 0) != 0;
+    const bool fused = persistent && c->tune_get("sort_fused_hist", 1) != 0;
+    RsPassDesc first{};
+    first.npass = all.npass ? 1 : 0;
+    first.shift[0] = all.shift[0];
+    first.mask[0] = all.mask[0];
     if (all.npass)
-        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
+        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n,
+               fused ? first : all, hist, andor);
 
     RsPassDesc run = all;
-    if (detect) {
+    if (detect && fused) {
+        uint32_t ao[2];
+        PD_HIP(hipMemcpyAsync(ao, andor, 8, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        const uint32_t varying = ao[0] ^ ao[1];
+        run.npass = 0;
+        for (int p = 0; p < 4; p++) {
+            if ((varying >> (8 * p)) & 0xFF) {
+                run.shift[run.npass] = 8 * p;
+                run.mask[run.npass] = 0xFF;
+                run.npass++;
+            }
+        }
+        if (run.npass && run.shift[0] != 0) {
+            // the lowest digit never varies: count the first varying one
+            first.npass = 1;
+            first.shift[0] = run.shift[0];
+            first.mask[0] = 0xFF;
+            fill_u32(c, hist, 0, RS_BINS, s);
+            fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
+            launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, first,
+                   hist, andor);
+        }
+    } else if (detect) {
         uint32_t ao[2];
         PD_HIP(hipMemcpyAsync(ao, andor, 8, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
@@ -440,7 +766,18 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         if (vw) copy_d2d(c, vout, vin, n * vw, s);
         return;
     }
-    launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
+    launch(c, "radix_scan", k_rs_scan, dim3(fused ? 1 : run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
+    if (persistent) {
+        if (vw == 0)
+            run_passes_pf<uint32_t, false, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, fused, s);
+        else if (vw == 4)
+            run_passes_pf<uint32_t, true, false>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs,
+                                                 ctrs, fused, s);
+        else
+            run_passes_pf<uint64_t, true, false>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs,
+                                                 ctrs, fused, s);
+        return;
+    }
     if (vw == 0)
         run_passes<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
     else if (vw == 4)
@@ -544,7 +881,16 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
         launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
     }
     launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
-    if (vw == 4)
+    if (c->tune_get("sort_persistent", 0) != 0) {
+        if (vw == 4)
+            run_passes_pf<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs,
+                                                ctrs, false, s);
+        else if (vw == 8)
+            run_passes_pf<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs,
+                                                ctrs, false, s);
+        else
+            run_passes_pf<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, false, s);
+    } else if (vw == 4)
         run_passes<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
     else if (vw == 8)
         run_passes<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);

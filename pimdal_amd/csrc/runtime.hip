@@ -375,6 +375,13 @@ extern "C" int pimdal_ctx_get_tune(pimdal_ctx* c, const char* key, int64_t* valu
     PD_API_END
 }
 
+extern "C" int pimdal_ctx_clear_tune(pimdal_ctx* c, const char* key) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && key, PIMDAL_E_INVALID, "NULL argument");
+    c->tune.erase(key);
+    PD_API_END
+}
+
 // ---------------- C ABI: memory ----------------
 
 extern "C" int pimdal_dev_alloc(pimdal_ctx* c, uint64_t bytes, void** dptr) {

@@ -127,6 +127,10 @@ class Context:
     def set_tune(self, key: str, value: int) -> None:
         call("pimdal_ctx_set_tune", self.handle, key.encode(), int(value))
 
+    def clear_tune(self, key: str) -> None:
+        """Forget a knob; the library's built-in default applies again."""
+        call("pimdal_ctx_clear_tune", self.handle, key.encode())
+
     def autotune(self, op: str, n: int, cache_path: str | None = None) -> None:
         """Pick the fastest knob values for `op` at size n (pimdal_autotune)."""
         call("pimdal_autotune", self.handle, op.encode(), int(n), cache_path.encode() if cache_path else None)

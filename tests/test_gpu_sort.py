@@ -143,24 +143,39 @@ def test_sort_ballot_rank_u64_payload_and_hash_join(ctx, orc, ballot_rank):
     assert np.array_equal(got, exp)
 
 
-@pytest.fixture
-def two_phase(ctx):
-    """Onesweep with the two-phase LDS reorder (keys, then payloads through
-    one buffer) and 12 keys per thread (sort_two_phase 1)."""
-    ctx.set_tune("sort_two_phase", 1)
-    yield
-    ctx.set_tune("sort_two_phase", 0)
+PASS_VARIANTS = {
+    "window4": {"sort_lbw": 4},
+    "window8": {"sort_lbw": 8},
+    "window32": {"sort_lbw": 32},
+    "block1024_items8": {"sort_block": 1024},
+    "block512_items8": {"sort_items": 8},
+    "persistent_fused_hist": {"sort_persistent": 1},
+    "persistent_window4": {"sort_persistent": 1, "sort_lbw": 4, "sort_fused_hist": 0},
+}
 
 
-@pytest.mark.parametrize("n", [1, 6143, 6144, 6145, 1 << 20])
-def test_sort_two_phase_uniform(ctx, orc, two_phase, n):
+@pytest.fixture(params=sorted(PASS_VARIANTS))
+def pass_variant(ctx, request):
+    """Onesweep pass variants: look-back window (statuses read per round
+    trip) and the persistent pass that loads its next tile while writing
+    the current one (with or without counting the next pass's digit)."""
+    knobs = PASS_VARIANTS[request.param]
+    for k, v in knobs.items():
+        ctx.set_tune(k, v)
+    yield request.param
+    for k in knobs:
+        ctx.clear_tune(k)
+
+
+@pytest.mark.parametrize("n", [1, 8191, 8193, 1 << 20])
+def test_sort_pass_variant_uniform(ctx, orc, pass_variant, n):
     k, v = orc.sort_workload(n, seed=170 + n)
     gk, gv = gpu_sort(ctx, k, v)
     ek, ev = orc.sort_pairs(k, v)
     assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
 
 
-def test_sort_two_phase_zipf_partial_bits_and_hash_join(ctx, orc, two_phase):
+def test_sort_pass_variant_zipf_bits_payloads_and_hash_join(ctx, orc, pass_variant):
     k, v = orc.sort_workload(1 << 21, zipf=True)
     gk, gv = gpu_sort(ctx, k, v)
     ek, ev = orc.sort_pairs(k, v)
@@ -170,6 +185,15 @@ def test_sort_two_phase_zipf_partial_bits_and_hash_join(ctx, orc, two_phase):
     mk = (k >> np.uint32(4)) & np.uint32((1 << 9) - 1)
     order = np.argsort(mk, kind="stable")
     assert np.array_equal(gv, v[order]) and np.array_equal(gk, k[order])
+    # 24-bit keys (top digit constant), keys only, 8-byte payloads
+    k = orc.gen_range(np.uint32, 2, 1, 100_000, 0, (1 << 24) - 1)
+    gk, _ = gpu_sort(ctx, k, None)
+    assert np.array_equal(gk, orc.sort_keys(k))
+    k = (orc.gen_range(np.uint32, 3, 1, 100_000, 0, 255) << np.uint32(8)).astype(np.uint32)  # lowest digit constant
+    v = orc.gen_range(np.int64, 6, 2, 100_000, -(1 << 40), 1 << 40)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v.view(np.uint64))
+    assert np.array_equal(gk, ek) and np.array_equal(gv.view(np.uint64), ev)
     rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 19)   # partitioned: the hashed passes
     op, ob, cnt = ops.hash_join(ctx, *[ctx.to_device(a) for a in (rk, rp, sk, sp)])
     ep, eb = orc.hash_join(rk, rp, sk, sp)
