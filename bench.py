@@ -74,6 +74,8 @@ def bench_sort(ctx, steps, warmup, n=1 << 28, zipf=False):
     t = timed_steps(ctx, step, steps, warmup)
     pass_ms, launches = kernel_avg_ms(ctx, "radix_onesweep")
     hist_ms, _ = kernel_avg_ms(ctx, "radix_hist")
+    count_ms, count_launches = kernel_avg_ms(ctx, "radix_count")
+    tscan_ms, _ = kernel_avg_ms(ctx, "radix_tscan")
     for a in (keys, vals, ok, ov):
         a.free()
     return {
@@ -81,6 +83,7 @@ def bench_sort(ctx, steps, warmup, n=1 << 28, zipf=False):
         "alg_bytes": 68 * n, "alg_GBps": 68 * n / t / 1e9, "hbm_frac": 68 * n / t / HBM_PEAK,
         "keys_payload_GBps": 8 * n / t / 1e9,
         "pass_ms": pass_ms, "pass_launches": launches, "hist_ms": hist_ms,
+        "count_ms": count_ms, "count_launches": count_launches, "tscan_ms": tscan_ms,
         "pass_alg_bytes": 16 * n,
     }
 
@@ -106,7 +109,7 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
-KERNELS = ("part_hist", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
+KERNELS = ("part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
            "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
@@ -295,7 +298,7 @@ def main():
         "config": {"workload": "stable LSD radix sort of 2^28 (u32 key, u32 payload) pairs, keys uniform on [0,2^32)",
                    "rows": srt["n"], "passes": 4, "digit_bits": 8,
                    "alg_bytes_per_row": 68, "sort_alg_GBps": srt["alg_GBps"], "sort_hbm_frac": srt["hbm_frac"],
-                   "hist_ms": srt["hist_ms"],
+                   "hist_ms": srt["hist_ms"], "count_ms": srt["count_ms"], "tscan_ms": srt["tscan_ms"],
                    "keys_payload_GBps": srt["keys_payload_GBps"]},
         "roofline": {"kernel": "radix_onesweep", "bound": "hbm",
                      "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -76,10 +76,10 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   sort_block (256, 512, 1024), sort_items (keys per thread: 16 or 8;
  *   1024-thread blocks take 8), sort_early_lookback (0, 1), sort_rank (0 LDS
  *   match masks, 1 per-bit ballots), sort_lbw (look-back statuses read per
- *   round trip: 4, 8, 16, 32), sort_persistent (1: co-resident blocks loop
- *   over tiles, loading the next tile during the current one's write
- *   phase), sort_fused_hist (with sort_persistent: each pass counts the
- *   next pass's digit, so only the first digit is counted up front);
+ *   round trip: 4, 8, 16, 32), sort_scheme (0 chained passes with decoupled
+ *   look-back, 1 counted passes: per-tile digit counts and a scan, then a
+ *   reorder with no look-back), sort_rts_order (tile order of counted
+ *   passes: 0 block order, 1 ticket, 2 XCD-contiguous);
  *   select_mode (0 one pass with decoupled look-back, 1 count pass + write
  *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
  *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 #include <stdexcept>
+#include <tuple>
 
 #include "../../include/pimdal_gpu.h"
 
@@ -127,6 +128,36 @@ void launch(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid
     hipLaunchKernelGGL(kernel, grid, block, shmem, s, static_cast<KArgs>(args)...);
     PD_HIP(hipGetLastError());
     if (c->profiling) record_launch(c, name, e0, s);
+}
+
+// Cooperative launch: the runtime starts the grid only if every block is
+// resident at once (or fails), so blocks may wait on one another's
+// progress.  Persistent kernels that walk tiles in a fixed order (block b
+// takes tiles b, b + grid, ...) use it instead of a tile counter: one
+// device-wide atomic per tile serializes at about 11 ns each.
+template <typename... KArgs, typename... Args>
+void launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+                 uint32_t shmem, hipStream_t s, Args... args) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    std::tuple<KArgs...> vals(static_cast<KArgs>(args)...);
+    void* ptrs[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1];
+    std::apply([&](auto&... v) { int i = 0; ((ptrs[i++] = (void*)&v), ...); }, vals);
+    hipEvent_t e0 = nullptr;
+    if (c->profiling) {
+        e0 = take_event(c);
+        PD_HIP(hipEventRecord(e0, s));
+    }
+    PD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shmem, s));
+    if (c->profiling) record_launch(c, name, e0, s);
+}
+
+// Blocks of `kernel` (at `block` threads, no dynamic LDS) that one CU holds
+// at once; at least 1.
+template <typename K>
+uint32_t blocks_per_cu(K kernel, int block) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1) nb = 1;
+    return (uint32_t)nb;
 }
 
 // Fill device memory with a small kernel (graph-capture safe, unlike

@@ -96,13 +96,24 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
         for (int cpy = 0; cpy < RS_HCOPY; cpy++) v += (&h[cpy][0][0])[i];
         if (v) atomicAdd(&g_hist[i], v);
     }
-    // wave AND/OR, one atomic per wave
+    // AND/OR over the block, then one atomic pair per block (device-wide
+    // atomics on one word serialize, about 11 ns each)
 #pragma unroll
     for (int d = PD_WAVE / 2; d > 0; d >>= 1) {
         a &= __shfl_xor(a, d, PD_WAVE);
         o |= __shfl_xor(o, d, PD_WAVE);
     }
+    __shared__ uint32_t s_ao[2][512 / PD_WAVE];
     if (pd::lane() == 0) {
+        s_ao[0][t / PD_WAVE] = a;
+        s_ao[1][t / PD_WAVE] = o;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (uint32_t i = 1; i < blockDim.x / PD_WAVE; i++) {
+            a &= s_ao[0][i];
+            o |= s_ao[1][i];
+        }
         atomicAnd(&g_andor[0], a);
         atomicOr(&g_andor[1], o);
     }
@@ -121,13 +132,154 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
     g_offs[p * RS_BINS + t] = add + inc - v;
 }
 
+// ---- reduce-then-scan pass: per-tile digit counts, then a scan over tiles ----
+// The chained (onesweep) pass makes every tile wait until its predecessors
+// have published; measured on one pass over 2^28 pairs, those waits cost
+// 0.5 of its 1.3 ms (the same pass without them: 0.83 ms), while reading the
+// keys once more to count each tile's digits up front costs 0.19 ms.  So
+// the default pass is reduce-then-scan: k_rs_up counts, k_rs_tscan turns
+// the counts into per-tile digit offsets, and the ranking/reorder kernel
+// (k_rs_onesweep with RTS) reads its offsets instead of looking back.
+
+// Digit counts of TILE-key tiles: cnt[tile][d] for d < 256 (rows past n are
+// not counted).  AO: also the AND and OR of the tile's keys, andor_t[2 tile
+// + 0/1] (bits equal in every key need no pass).
+template <bool HASH, bool AO, int TILE>
+__global__ __launch_bounds__(512) void k_rs_up(const uint32_t* __restrict__ kin, uint32_t n, uint32_t shift, uint32_t mask,
+                                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ andor_t) {
+    constexpr int Q = TILE / 512;   // keys per thread
+    __shared__ uint32_t h[RS_BINS];
+    __shared__ uint32_t s_ao[2][512 / PD_WAVE];
+    const uint32_t t = threadIdx.x, l = pd::lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    if (t < RS_BINS) h[t] = 0;
+    // wave w reads rows [base + w*TILE/8, +TILE/8), lane-contiguous 4-byte
+    // loads (any 4-byte alignment) through a range-checked buffer
+    const uint32_t wave_base = blockIdx.x * (uint32_t)TILE + w * (uint32_t)(TILE / 8);
+    const uint32_t wrows = n > wave_base ? min((uint32_t)(TILE / 8), n - wave_base) : 0u;
+    const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(kin + wave_base, wrows * 4u);
+    uint32_t q[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) q[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + (uint32_t)i * PD_WAVE) * 4u, 0, 0);
+    __syncthreads();
+    uint32_t a = 0xFFFFFFFFu, o = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        if (l + (uint32_t)i * PD_WAVE < wrows) {
+            atomicAdd(&h[(rs_src<HASH>(q[i]) >> shift) & mask], 1u);
+            if constexpr (AO) { a &= q[i]; o |= q[i]; }
+        }
+    }
+    if constexpr (AO) {
+#pragma unroll
+        for (int d = PD_WAVE / 2; d > 0; d >>= 1) {
+            a &= __shfl_xor(a, d, PD_WAVE);
+            o |= __shfl_xor(o, d, PD_WAVE);
+        }
+        if (l == 0) {
+            s_ao[0][w] = a;
+            s_ao[1][w] = o;
+        }
+    }
+    __syncthreads();
+    if (t < RS_BINS) cnt[(uint64_t)blockIdx.x * RS_BINS + t] = h[t];
+    if (AO && t == 0) {
+        for (int i = 1; i < 512 / PD_WAVE; i++) {
+            a &= s_ao[0][i];
+            o |= s_ao[1][i];
+        }
+        andor_t[2 * (uint64_t)blockIdx.x] = a;
+        andor_t[2 * (uint64_t)blockIdx.x + 1] = o;
+    }
+}
+
+// Scan over tiles, in place: cnt[tile][d] becomes the number of rows with
+// digit d in tiles before `tile`; base[d] = rows with a digit below d.  A
+// block (thread t = digit t) takes the next group of GT tiles from a
+// counter, sums its digits, and resolves the group prefix by decoupled
+// look-back over earlier groups (few: ntiles / GT).  The last group writes
+// base.  AO: the groups also fold the tiles' AND/OR into andor[0..1].
+constexpr uint32_t RS_GT = 64;
+template <bool AO>
+__global__ __launch_bounds__(RS_BINS) void k_rs_tscan(uint32_t* __restrict__ cnt, uint32_t ntiles,
+                                                      uint32_t* __restrict__ gstatus, uint32_t* __restrict__ ctr,
+                                                      uint32_t* __restrict__ base, const uint32_t* __restrict__ andor_t,
+                                                      uint32_t* __restrict__ andor) {
+    __shared__ uint32_t s_g, s_w[RS_BINS / PD_WAVE];
+    const uint32_t t = threadIdx.x, l = pd::lane();
+    if (t == 0) s_g = atomicAdd(ctr, 1u);
+    __syncthreads();
+    const uint32_t g = s_g;
+    const uint32_t ngroups = (ntiles + RS_GT - 1) / RS_GT;
+    const uint32_t t0 = g * RS_GT, t1 = min(ntiles, t0 + RS_GT);
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t i = t0; i < t1; i++) sum += cnt[(uint64_t)i * RS_BINS + t];
+    pd::st_relaxed(&gstatus[(uint64_t)g * RS_BINS + t], (g == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | sum);
+    uint32_t excl = 0;
+    if (g > 0) {
+        int64_t j = (int64_t)g - 1;
+        bool done = false;
+        while (!done) {
+            uint32_t st[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                st[q] = (j - q >= 0) ? pd::ld_relaxed(&gstatus[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
+            bool stall = false;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (done || stall) continue;
+                if ((st[q] >> 30) == 0) { stall = true; continue; }
+                excl += st[q] & RS_VALUE;
+                if ((st[q] >> 30) == 2) done = true;
+                else j--;
+            }
+            if (stall) __builtin_amdgcn_s_sleep(1);
+        }
+        pd::st_relaxed(&gstatus[(uint64_t)g * RS_BINS + t], RS_FLAG_PRE | (excl + sum));
+    }
+    uint32_t run = excl;
+    for (uint32_t i = t0; i < t1; i++) {
+        const uint64_t k = (uint64_t)i * RS_BINS + t;
+        const uint32_t c = cnt[k];
+        cnt[k] = run;
+        run += c;
+    }
+    if (g == ngroups - 1) {
+        // run = total of digit t: exclusive scan over digits
+        const uint32_t inc = pd::wave_incl_scan(run);
+        if (l == PD_WAVE - 1) s_w[t / PD_WAVE] = inc;
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t i = 0; i < t / PD_WAVE; i++) add += s_w[i];
+        base[t] = add + inc - run;
+    }
+    if constexpr (AO) {
+        // AND/OR of the group's tiles, one atomic pair per group
+        uint32_t a = 0xFFFFFFFFu, o = 0;
+        for (uint32_t i = t0 + t; i < t1; i += RS_BINS) {
+            a &= andor_t[2 * (uint64_t)i];
+            o |= andor_t[2 * (uint64_t)i + 1];
+        }
+#pragma unroll
+        for (int d = PD_WAVE / 2; d > 0; d >>= 1) {
+            a &= __shfl_xor(a, d, PD_WAVE);
+            o |= __shfl_xor(o, d, PD_WAVE);
+        }
+        if (l == 0 && (t / PD_WAVE) * PD_WAVE < t1 - t0) {
+            atomicAnd(&andor[0], a);
+            atomicOr(&andor[1], o);
+        }
+    }
+}
+
 // ---- one onesweep pass ----
 // Compiler-only barrier: LDS instructions of one wave execute in issue order,
 // so the ranking steps below need ordering against code motion, not waits.
 // (Plain LDS accesses between these barriers, not volatile ones: the
 // compiler follows a volatile access with a wait for every outstanding
 // global load and store too, which would stall the ranking on the payload
-// loads and, in the persistent pass, on the previous tile's stores.)
+// loads.)
 #define PD_COMPILER_BARRIER() __asm__ volatile("" ::: "memory")
 
 // 4 waves per SIMD: at most 128 VGPRs (second launch-bounds argument is the
@@ -140,15 +292,19 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_scan(const uint32_t* __restrict_
 // per microsecond, a tile typically walks back over dozens of published
 // aggregates before it reaches an inclusive prefix, and a wide window
 // covers them in one or two round trips.
+// RTS (reduce-then-scan): tile = blockIdx.x, no status words and no
+// look-back: `offs` is the scanned per-tile counts (k_rs_tscan) and `base`
+// the digit starts, so no tile waits on another.
 template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false, int LBW = 4,
-          int ITEMS = RS_ITEMS>
+          int ITEMS = RS_ITEMS, bool RTS = false>
 __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ tile_ctr) {
+    uint32_t* __restrict__ tile_ctr, const uint32_t* __restrict__ base, uint32_t xcd_tiles) {
     constexpr int WAVES = BLOCK / PD_WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
+    static_assert(!(RTS && (EARLY || BALLOT)), "the counted pass has no look-back");
     __shared__ uint32_t s_tile;
     // keys reordered by digit; the per-wave match masks live here during ranking
     __shared__ __align__(16) uint32_t s_keys[TILE];
@@ -161,13 +317,23 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
 
     const uint32_t t = threadIdx.x, l = pd::lane();
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
-    if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    if ((!RTS || tile_ctr) && t == 0) s_tile = atomicAdd(tile_ctr, 1u);
     for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
         s_match[w][i] = 0;
         s_cnt[w][i] = 0;
     }
     __syncthreads();
-    const uint64_t tile = s_tile;
+    // RTS tile order: ticket (tile_ctr), XCD-contiguous (xcd_tiles > 0:
+    // workgroups go to the 8 XCDs round-robin, so block b on XCD b % 8 takes
+    // tile (b % 8) * xcd_tiles + b / 8 and neighbouring tiles share an L2),
+    // or block order
+    const uint64_t tile = (!RTS || tile_ctr) ? (uint64_t)s_tile
+                          : xcd_tiles ? (uint64_t)(blockIdx.x % 8) * xcd_tiles + blockIdx.x / 8
+                                      : (uint64_t)blockIdx.x;
+    if (RTS && tile * TILE >= n) return;   // XCD order: grid rounded up to a multiple of 8
+    // RTS: this tile's run start of digit t, loaded now (used after the reorder)
+    uint32_t rts_start = 0;
+    if (RTS && t < RS_BINS) rts_start = base[t] + offs[tile * RS_BINS + t];
     const uint64_t tile_base = tile * TILE;
     const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * ITEMS);
     const uint32_t tile_count = (uint32_t)min((uint64_t)TILE, n - tile_base);
@@ -282,8 +448,10 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
             s_cnt[ww][t] = (uint16_t)cnt_d;
             cnt_d += c;
         }
-        pd::st_relaxed(&status[tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
-        if (EARLY && tile > 0) excl = lookback(cnt_d);
+        if constexpr (!RTS) {
+            pd::st_relaxed(&status[tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
+            if (EARLY && tile > 0) excl = lookback(cnt_d);
+        }
         inc_d = pd::wave_incl_scan(cnt_d);
         if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
     }
@@ -306,7 +474,8 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
         s_keys[p] = key[i];
         if constexpr (HAS_VALS) s_vals[p] = val[i];
     }
-    if (!EARLY && t < RS_BINS) {
+    if (RTS && t < RS_BINS) s_gbase[t] = rts_start - local_d;
+    if (!RTS && !EARLY && t < RS_BINS) {
         if (tile > 0) excl = lookback(cnt_d);
         s_gbase[t] = offs[t] + excl - local_d;
     }
@@ -327,305 +496,10 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
 
 }
 
-// ---- persistent onesweep pass ----
-// gridDim.x blocks (as many as are co-resident) loop over tiles taken in
-// order from tile_ctr.  Right after a tile has been reordered into LDS, the
-// block takes its next tile and loads its keys and payloads into the same
-// registers, so that load's latency overlaps this tile's look-back and
-// write phase.  A block waits (look-back) only on tiles taken before its
-// own, all of which are held by running blocks, so every wait ends.
-// NEXT: also count the digit (src >> nshift) & nmask of every row into
-// g_nhist, the next pass's histogram (complete when this pass ends), so
-// only the first pass needs a histogram read of the keys.  The counters
-// are spread over RS_NCOPY LDS copies by lane.
-constexpr int RS_NCOPY = 4;
-template <typename V, bool HAS_VALS, int BLOCK, bool HASH, bool NEXT, int LBW = 16, int ITEMS = RS_ITEMS>
-__global__ __launch_bounds__(BLOCK, 4) void k_rs_onesweep_pf(
-    const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
-    uint32_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ tile_ctr, uint32_t ntiles, uint32_t nshift, uint32_t nmask,
-    uint32_t* __restrict__ g_nhist) {
-    constexpr int WAVES = BLOCK / PD_WAVE;
-    constexpr int TILE = BLOCK * ITEMS;
-    static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
-    __shared__ uint32_t s_next;
-    __shared__ __align__(16) uint32_t s_keys[TILE];
-    __shared__ __align__(16) V s_vals[HAS_VALS ? TILE : 1];
-    __shared__ uint16_t s_cnt[WAVES][RS_BINS];
-    __shared__ uint32_t s_off[RS_BINS];
-    __shared__ uint32_t s_gbase[RS_BINS];
-    __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
-    __shared__ uint32_t s_nh[NEXT ? RS_NCOPY : 1][NEXT ? RS_BINS : 1];
-    auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);
-
-    const uint32_t t0 = threadIdx.x, l0 = pd::lane();
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t0 / PD_WAVE);
-    if constexpr (NEXT)
-        for (uint32_t i = t0; i < RS_NCOPY * RS_BINS; i += BLOCK) (&s_nh[0][0])[i] = 0;
-    if (t0 == 0) s_next = atomicAdd(tile_ctr, 1u);
-    __syncthreads();
-    uint32_t tile = s_next;
-
-    uint32_t key[ITEMS];
-    V val[HAS_VALS ? ITEMS : 1];
-    uint32_t wrows = 0;   // rows of this wave in the current tile (wave-uniform)
-    // wave-striped rows: wave_base + i*64 + lane, through buffer loads whose
-    // range ends at the wave's last row (0 past it); rows past n get the
-    // largest digit, so they rank after every real row and are never written
-    // keys of a tile: loaded one tile ahead (during the previous tile's
-    // look-back and writes); payloads: loaded when the tile's turn comes,
-    // in flight during its ranking (so only the keys stay live across the
-    // write phase)
-    auto load_keys = [&](uint32_t tl, uint32_t lane4) {
-        asm volatile("" : "+v"(lane4));   // lane * 4 + i * 256: immediate offsets, nothing hoisted
-        const uint32_t wave_base = tl * (uint32_t)TILE + w * (uint32_t)(PD_WAVE * ITEMS);
-        wrows = n > wave_base ? min((uint32_t)(PD_WAVE * ITEMS), n - wave_base) : 0u;
-        const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(kin + wave_base, wrows * 4u);
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++)
-            key[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, lane4 + (uint32_t)i * PD_WAVE * 4u, 0, 0);
-    };
-    auto load_vals = [&](uint32_t tl, uint32_t lane4) {
-        asm volatile("" : "+v"(lane4));
-        const uint32_t wave_base = tl * (uint32_t)TILE + w * (uint32_t)(PD_WAVE * ITEMS);
-        if constexpr (HAS_VALS) {
-            const __amdgpu_buffer_rsrc_t rv = pd::buf_rsrc(vin + wave_base, wrows * (uint32_t)sizeof(V));
-#pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                if constexpr (sizeof(V) == 4)
-                    val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, lane4 + (uint32_t)i * PD_WAVE * 4u, 0, 0);
-                else
-                    val[i] = __builtin_bit_cast(
-                        V, __builtin_amdgcn_raw_buffer_load_b64(rv, 2u * lane4 + (uint32_t)i * PD_WAVE * 8u, 0, 0));
-            }
-        }
-    };
-    const __amdgpu_buffer_rsrc_t rko = pd::buf_rsrc(kout, n * 4u);
-    const __amdgpu_buffer_rsrc_t rvo = pd::buf_rsrc(vout, HAS_VALS ? n * (uint32_t)sizeof(V) : 0u);
-    if (tile < ntiles) load_keys(tile, l0 * 4u);
-
-    while (tile < ntiles) {
-        // lane id recomputed every tile by an opaque instruction pair, and
-        // the thread id from it and the wave number: nothing derived from
-        // them is hoisted out of the loop (and spilled; a scratch reload is
-        // a vector-memory access, and waiting for it waits for every
-        // earlier load and store of the wave)
-        uint32_t l;
-        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
-        const uint32_t t = w * PD_WAVE + l;
-        const uint64_t lt = (1ull << l) - 1ull;
-        const uint32_t tile_count = min((uint32_t)TILE, n - tile * (uint32_t)TILE);
-        // next ticket: issued now, its value is needed only after the
-        // ranking.  The opaque zero offset keeps it a plain returning
-        // atomic (a wave-aggregated one reads its result back at once).
-        uint32_t my_next = 0;
-        if (t == 0) {
-            uint32_t z = 0;
-            asm volatile("" : "+v"(z));
-            my_next = atomicAdd(tile_ctr + z, 1u);
-        }
-        load_vals(tile, l * 4u);
-        for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
-            s_match[w][i] = 0;
-            s_cnt[w][i] = 0;
-        }
-        __syncthreads();
-        auto digit_of = [&](int i) -> uint32_t {
-            uint32_t kk = key[i];
-            asm volatile("" : "+v"(kk));   // computed where used (see the single-tile pass)
-            return ((uint32_t)i * PD_WAVE + l) < wrows ? ((rs_src<HASH>(kk) >> shift) & mask) : (RS_BINS - 1);
-        };
-        // ---- stable rank inside the wave (LDS match masks) ----
-        uint32_t pos[ITEMS];
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t d = digit_of(i);
-            atomicOr((unsigned long long*)&s_match[w][d], (unsigned long long)(1ull << l));
-            PD_COMPILER_BARRIER();
-            const uint64_t m = s_match[w][d];
-            const uint32_t old = s_cnt[w][d];
-            const uint32_t below = pd::popc64(m & lt);
-            pos[i] = (old + below) | (d << 16);
-            PD_COMPILER_BARRIER();
-            if (below == 0) {
-                s_cnt[w][d] = (uint16_t)(old + pd::popc64(m));
-                s_match[w][d] = 0;
-            }
-            PD_COMPILER_BARRIER();
-        }
-        if constexpr (NEXT) {
-#pragma unroll
-            for (int i = 0; i < ITEMS; i++)
-                if (((uint32_t)i * PD_WAVE + l) < wrows)
-                    atomicAdd(&s_nh[l % RS_NCOPY][(rs_src<HASH>(key[i]) >> nshift) & nmask], 1u);
-        }
-        __syncthreads();
-
-        // ---- per digit: prefix over waves, tile count, publish the aggregate ----
-        uint32_t cnt_d = 0, inc_d = 0;
-        if (t < RS_BINS) {
-#pragma unroll
-            for (int ww = 0; ww < WAVES; ww++) {
-                const uint32_t cw = s_cnt[ww][t];
-                s_cnt[ww][t] = (uint16_t)cnt_d;
-                cnt_d += cw;
-            }
-            pd::st_relaxed(&status[(uint64_t)tile * RS_BINS + t], (tile == 0 ? RS_FLAG_PRE : RS_FLAG_AGG) | cnt_d);
-            inc_d = pd::wave_incl_scan(cnt_d);
-            if (l == PD_WAVE - 1) s_wsum[w] = inc_d;
-        }
-        if (t == 0) s_next = my_next;
-        __syncthreads();
-        uint32_t local_d = 0;
-        if (t < RS_BINS) {
-            uint32_t add = 0;
-            for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
-            local_d = add + inc_d - cnt_d;
-            s_off[t] = local_d;
-        }
-        __syncthreads();
-
-        // ---- reorder the tile by digit in LDS ----
-#pragma unroll
-        for (int i = 0; i < ITEMS; i++) {
-            const uint32_t d = pos[i] >> 16;
-            const uint32_t p = (pos[i] & 0xFFFFu) + s_off[d] + s_cnt[w][d];
-            s_keys[p] = key[i];
-            if constexpr (HAS_VALS) s_vals[p] = val[i];
-        }
-        // ---- first look-back window (thread t = digit t), then the next
-        // tile's loads: the window is issued first, so waiting for it does
-        // not wait for the prefetch ----
-        const bool lb = t < RS_BINS && tile > 0;
-        uint32_t st[LBW];
-        if (lb) {
-#pragma unroll
-            for (int q = 0; q < LBW; q++)
-                st[q] = ((int64_t)tile - 1 - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(tile - 1 - q) * RS_BINS + t])
-                                                     : RS_FLAG_PRE;
-        }
-        PD_COMPILER_BARRIER();
-        const uint32_t next = s_next;
-        if (next < ntiles) load_keys(next, l * 4u);
-        PD_COMPILER_BARRIER();
-
-        // ---- per-digit look-back over earlier tiles ----
-        if (t < RS_BINS) {
-            uint32_t excl = 0;
-            if (lb) {
-                int64_t j = (int64_t)tile - 1;
-                bool done = false, first = true;
-                while (!done) {
-                    if (!first) {
-#pragma unroll
-                        for (int q = 0; q < LBW; q++)
-                            st[q] = (j - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j - q) * RS_BINS + t]) : RS_FLAG_PRE;
-                    }
-                    first = false;
-                    bool stall = false;
-#pragma unroll
-                    for (int q = 0; q < LBW; q++) {
-                        if (done || stall) continue;
-                        if ((st[q] >> 30) == 0) { stall = true; continue; }
-                        excl += st[q] & RS_VALUE;
-                        if ((st[q] >> 30) == 2) done = true;
-                        else j--;
-                    }
-                    if (stall) __builtin_amdgcn_s_sleep(1);
-                }
-                pd::st_relaxed(&status[(uint64_t)tile * RS_BINS + t], RS_FLAG_PRE | (excl + cnt_d));
-            }
-            s_gbase[t] = offs[t] + excl - local_d;
-        }
-        __syncthreads();
-
-        // ---- write keys and payloads in digit runs ----
-#pragma unroll 4
-        for (int k = 0; k < ITEMS; k++) {
-            const uint32_t j = t + k * BLOCK;
-            if (j >= tile_count) break;
-            const uint32_t kk = s_keys[j];
-            const uint32_t dst = s_gbase[(rs_src<HASH>(kk) >> shift) & mask] + j;
-            __builtin_amdgcn_raw_buffer_store_b32(kk, rko, dst * 4u, 0, 0);
-            if constexpr (HAS_VALS) {
-                if constexpr (sizeof(V) == 4)
-                    __builtin_amdgcn_raw_buffer_store_b32((uint32_t)s_vals[j], rvo, dst * 4u, 0, 0);
-                else
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, s_vals[j]),
-                                                          rvo, dst * 8u, 0, 0);
-            }
-        }
-        __syncthreads();   // s_keys, s_vals, s_gbase are rewritten by the next tile
-        tile = next;
-    }
-    if constexpr (NEXT) {
-        for (uint32_t i = t0; i < RS_BINS; i += BLOCK) {
-            uint32_t v = 0;
-#pragma unroll
-            for (int cp = 0; cp < RS_NCOPY; cp++) v += s_nh[cp][i];
-            if (v) atomicAdd(&g_nhist[i], v);
-        }
-    }
-}
-
 // Look-back window of the passes (knob sort_lbw: 4, 8, 16 or 32).
 static int sort_lbw(pimdal_ctx* c) {
     const int64_t v = c->tune_get("sort_lbw", 4);
     return v >= 32 ? 32 : v >= 16 ? 16 : v >= 8 ? 8 : 4;
-}
-
-// Co-resident blocks per CU of a kernel at a block size (at least 1).
-template <typename K>
-static uint32_t blocks_per_cu(K kernel, int block) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1) nb = 1;
-    return (uint32_t)nb;
-}
-
-// Persistent passes.  hist_fused: only pass 0's histogram is in hist; each
-// pass counts the next pass's digit and its offsets are scanned before it
-// runs.
-template <typename V, bool HV, bool HASH>
-static void run_passes_pf(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
-                          const RsPassDesc& pdsc, uint32_t* hist, uint32_t* offs, uint32_t* ctrs, bool hist_fused,
-                          hipStream_t s) {
-    constexpr int B = 512;
-    const int lbw = sort_lbw(c);
-    const uint32_t ntiles = (uint32_t)ceil_div(n, (uint64_t)B * RS_ITEMS);
-    const uint32_t P = pdsc.npass;
-    Scratch tk(c, P > 1 ? n * 4 : 0, s);
-    Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
-    Scratch st(c, (uint64_t)ntiles * RS_BINS * 4, s);
-    const uint32_t* ksrc = kin;
-    const V* vsrc = vin;
-    for (uint32_t j = 0; j < P; j++) {
-        const bool to_out = ((P - 1 - j) % 2) == 0;
-        uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
-        V* vdst = to_out ? vout : tv.as<V>();
-        fill_u32(c, st.get(), 0, (uint64_t)ntiles * RS_BINS, s);
-        if (hist_fused && j > 0)
-            launch(c, "radix_scan", k_rs_scan, dim3(1), dim3(RS_BINS), 0, s, (const uint32_t*)(hist + j * RS_BINS),
-                   offs + j * RS_BINS);
-        const bool next = hist_fused && j + 1 < P;
-        const uint32_t nshift = next ? pdsc.shift[j + 1] : 0, nmask = next ? pdsc.mask[j + 1] : 0;
-        uint32_t* nh = next ? hist + (j + 1) * RS_BINS : nullptr;
-        const char* name = HASH ? "part_onesweep" : "radix_onesweep";
-#define PD_PF_LAUNCH(NX, W)                                                                                          \
-    do {                                                                                                              \
-        auto k = k_rs_onesweep_pf<V, HV, B, HASH, NX, W>;                                                             \
-        const uint32_t g = std::min<uint32_t>(ntiles, (uint32_t)c->cu_count * blocks_per_cu(k, B));                   \
-        launch(c, name, k, dim3(g), dim3(B), 0, s, ksrc, vsrc, kdst, vdst, (uint32_t)n, pdsc.shift[j], pdsc.mask[j],   \
-               (const uint32_t*)(offs + j * RS_BINS), st.as<uint32_t>(), ctrs + j, ntiles, nshift, nmask, nh);         \
-    } while (0)
-        if (next) {
-            if (lbw >= 16) PD_PF_LAUNCH(true, 16); else PD_PF_LAUNCH(true, 4);
-        } else {
-            if (lbw >= 16) PD_PF_LAUNCH(false, 16); else PD_PF_LAUNCH(false, 4);
-        }
-#undef PD_PF_LAUNCH
-        ksrc = kdst;
-        vsrc = vdst;
-    }
 }
 
 template <typename V, bool HV, bool HASH = false>
@@ -654,7 +528,7 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
 #define PD_RS_LAUNCH(...)                                                                                         \
     launch(c, name, k_rs_onesweep<__VA_ARGS__>, dim3((uint32_t)ntiles), dim3(block), 0, s, ksrc, vsrc, kdst, vdst, n, \
-           pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j)
+           pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j, (const uint32_t*)nullptr, 0u)
         if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
         else if (items == 8 && block == 512) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 8);
         else if (block == 512 && !early && !ballot) {
@@ -672,6 +546,123 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     }
 }
 
+// ---- reduce-then-scan passes ----
+// Scratch of the counting passes: per-tile counts (scanned in place into
+// tile offsets), group statuses of the tile scan, group tickets (one per
+// count), digit starts, and per-tile AND/OR words with their fold.
+struct RtsBufs {
+    Scratch cnt, gst, small, ao_t;
+    uint32_t ntiles, ngroups;
+    uint32_t* ctrs;    // 8 tickets
+    uint32_t* base;    // 256
+    uint32_t* andor;   // 2
+    RtsBufs(pimdal_ctx* c, uint64_t n, bool ao, hipStream_t s)
+        : cnt(c, ceil_div(n, 8192) * RS_BINS * 4, s), gst(c, ceil_div(ceil_div(n, 8192), RS_GT) * RS_BINS * 4, s),
+          small(c, (8 + RS_BINS + 2) * 4, s), ao_t(c, ao ? ceil_div(n, 8192) * 8 : 0, s) {
+        ntiles = (uint32_t)ceil_div(n, 8192);
+        ngroups = (uint32_t)ceil_div(ntiles, RS_GT);
+        ctrs = small.as<uint32_t>();
+        base = ctrs + 8;
+        andor = base + RS_BINS;
+        fill_u32(c, ctrs, 0, 8, s);
+        fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
+        fill_u32(c, andor + 1, 0, 1, s);
+    }
+};
+
+// Counts one digit of every tile and scans the counts (ticket k of the
+// tile scan); AO: also the AND/OR of all keys into B.andor.
+template <bool HASH, bool AO>
+static void rts_count(pimdal_ctx* c, const uint32_t* keys, uint64_t n, uint32_t shift, uint32_t mask, RtsBufs& B,
+                      int k, hipStream_t s) {
+    launch(c, HASH ? "part_count" : "radix_count", k_rs_up<HASH, AO, 8192>, dim3(B.ntiles), dim3(512), 0, s, keys,
+           (uint32_t)n, shift, mask, B.cnt.as<uint32_t>(), AO ? B.ao_t.as<uint32_t>() : (uint32_t*)nullptr);
+    fill_u32(c, B.gst.get(), 0, (uint64_t)B.ngroups * RS_BINS, s);
+    launch(c, HASH ? "part_tscan" : "radix_tscan", k_rs_tscan<AO>, dim3(B.ngroups), dim3(RS_BINS), 0, s,
+           B.cnt.as<uint32_t>(), B.ntiles, B.gst.as<uint32_t>(), B.ctrs + k, B.base,
+           (const uint32_t*)(AO ? B.ao_t.as<uint32_t>() : nullptr), AO ? B.andor : (uint32_t*)nullptr);
+}
+
+// Passes over the digits of pdsc: count + scan (unless the first digit is
+// already counted in B), then rank and reorder each 8192-row tile to its
+// offsets.  The shape knobs pick 512 threads x 16 keys (default) or 1024 x 8.
+template <typename V, bool HV, bool HASH = false>
+static void run_passes_rts(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
+                           const RsPassDesc& pdsc, RtsBufs& B, bool first_counted, hipStream_t s) {
+    const bool b1024 = c->tune_get("sort_block", 512) >= 1024;
+    const uint32_t P = pdsc.npass;
+    Scratch tk(c, P > 1 ? n * 4 : 0, s);
+    Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
+    const uint32_t* ksrc = kin;
+    const V* vsrc = vin;
+    const char* name = HASH ? "part_onesweep" : "radix_onesweep";
+    for (uint32_t j = 0; j < P; j++) {
+        const bool to_out = ((P - 1 - j) % 2) == 0;
+        uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
+        V* vdst = to_out ? vout : tv.as<V>();
+        if (j > 0 || !first_counted) rts_count<HASH, false>(c, ksrc, n, pdsc.shift[j], pdsc.mask[j], B, 1 + (int)j, s);
+        const uint32_t* off = B.cnt.as<uint32_t>();
+        // tile order (knob sort_rts_order): 0 block order, 1 ticket, 2 XCD-contiguous
+        const int64_t order = c->tune_get("sort_rts_order", 2);
+        uint32_t* tctr = order == 1 ? B.ctrs + 5 + (j % 3) : nullptr;
+        if (order == 1) fill_u32(c, tctr, 0, 1, s);
+        const uint32_t xt = order == 2 ? (uint32_t)ceil_div(B.ntiles, 8) : 0u;
+        const uint32_t grid = order == 2 ? 8 * xt : B.ntiles;
+        if (b1024)
+            launch(c, name, k_rs_onesweep<V, HV, 1024, false, HASH, false, 4, 8, true>, dim3(grid), dim3(1024), 0, s,
+                   ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], off, (uint32_t*)nullptr, tctr,
+                   (const uint32_t*)B.base, xt);
+        else
+            launch(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, 4, 16, true>, dim3(grid), dim3(512), 0, s,
+                   ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], off, (uint32_t*)nullptr, tctr,
+                   (const uint32_t*)B.base, xt);
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+}
+
+// Pass scheme (knob sort_scheme): 0 = chained onesweep per range (default),
+// 1 = reduce-then-scan.  Other tile shapes than 8192 keys run chained.
+static bool use_rts(pimdal_ctx* c) {
+    if (c->tune_get("sort_scheme", 0) != 1) return false;
+    const int64_t bk = c->tune_get("sort_block", 512);
+    if (bk <= 256) return false;
+    if (bk < 1024 && c->tune_get("sort_items", 16) == 8) return false;
+    return c->tune_get("sort_early_lookback", 0) == 0 && c->tune_get("sort_rank", 0) == 0;
+}
+
+template <typename V, bool HV>
+static void sort_rts(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
+                     bool detect, const RsPassDesc& all, hipStream_t s) {
+    RtsBufs B(c, n, detect, s);
+    RsPassDesc run = all;
+    bool counted = false;
+    if (detect) {
+        // count digit 0 and fold the AND/OR of all keys (a bit equal in
+        // every key needs no pass)
+        rts_count<false, true>(c, kin, n, 0, 0xFF, B, 0, s);
+        uint32_t ao[2];
+        PD_HIP(hipMemcpyAsync(ao, B.andor, 8, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        const uint32_t varying = ao[0] ^ ao[1];
+        run.npass = 0;
+        for (int p = 0; p < 4; p++) {
+            if ((varying >> (8 * p)) & 0xFF) {
+                run.shift[run.npass] = 8 * p;
+                run.mask[run.npass] = 0xFF;
+                run.npass++;
+            }
+        }
+        counted = run.npass > 0 && run.shift[0] == 0;
+    }
+    if (run.npass == 0) {
+        copy_d2d(c, kout, kin, n * 4, s);
+        if (HV) copy_d2d(c, vout, vin, n * sizeof(V), s);
+        return;
+    }
+    run_passes_rts<V, HV>(c, kin, vin, n, kout, vout, run, B, counted, s);
+}
+
 void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t* kout,
                 void* vout, int begin_bit, int end_bit, hipStream_t s) {
     PD_REQUIRE(vw == 0 || vw == 4 || vw == 8, PIMDAL_E_UNSUPPORTED, "payload width must be 0, 4 or 8 bytes");
@@ -685,16 +676,6 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
                "bad bit range");
     if (n == 0) return;
 
-    // small scratch: hist[4*256] | offs[4*256] | andor[2] | ctr[4]
-    Scratch small(c, (4 * RS_BINS * 2 + 8) * 4, s);
-    uint32_t* hist = small.as<uint32_t>();
-    uint32_t* offs = hist + 4 * RS_BINS;
-    uint32_t* andor = offs + 4 * RS_BINS;
-    uint32_t* ctrs = andor + 2;
-    fill_u32(c, hist, 0, 4 * RS_BINS * 2, s);
-    fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
-    fill_u32(c, andor + 1, 0, 1 + 4, s);
-
     RsPassDesc all{};
     if (detect) {
         all.npass = 4;
@@ -707,44 +688,32 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
             all.npass++;
         }
     }
+    if (use_rts(c)) {
+        if (vw == 0)
+            sort_rts<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, detect, all, s);
+        else if (vw == 4)
+            sort_rts<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, detect, all, s);
+        else
+            sort_rts<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, detect, all, s);
+        return;
+    }
+
+    // chained passes: one histogram read of the keys counts every digit
+    // small scratch: hist[4*256] | offs[4*256] | andor[2] | ctr[4]
+    Scratch small(c, (4 * RS_BINS * 2 + 8) * 4, s);
+    uint32_t* hist = small.as<uint32_t>();
+    uint32_t* offs = hist + 4 * RS_BINS;
+    uint32_t* andor = offs + 4 * RS_BINS;
+    uint32_t* ctrs = andor + 2;
+    fill_u32(c, hist, 0, 4 * RS_BINS * 2, s);
+    fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
+    fill_u32(c, andor + 1, 0, 1 + 4, s);
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
-    // persistent passes; fused: the histogram read counts only the first
-    // digit, and each pass counts the next one
-    const bool persistent = c->tune_get("sort_persistent", 0) != 0;
-    const bool fused = persistent && c->tune_get("sort_fused_hist", 1) != 0;
-    RsPassDesc first{};
-    first.npass = all.npass ? 1 : 0;
-    first.shift[0] = all.shift[0];
-    first.mask[0] = all.mask[0];
     if (all.npass)
-        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n,
-               fused ? first : all, hist, andor);
+        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
 
     RsPassDesc run = all;
-    if (detect && fused) {
-        uint32_t ao[2];
-        PD_HIP(hipMemcpyAsync(ao, andor, 8, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
-        const uint32_t varying = ao[0] ^ ao[1];
-        run.npass = 0;
-        for (int p = 0; p < 4; p++) {
-            if ((varying >> (8 * p)) & 0xFF) {
-                run.shift[run.npass] = 8 * p;
-                run.mask[run.npass] = 0xFF;
-                run.npass++;
-            }
-        }
-        if (run.npass && run.shift[0] != 0) {
-            // the lowest digit never varies: count the first varying one
-            first.npass = 1;
-            first.shift[0] = run.shift[0];
-            first.mask[0] = 0xFF;
-            fill_u32(c, hist, 0, RS_BINS, s);
-            fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
-            launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, first,
-                   hist, andor);
-        }
-    } else if (detect) {
+    if (detect) {
         uint32_t ao[2];
         PD_HIP(hipMemcpyAsync(ao, andor, 8, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
@@ -766,18 +735,7 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         if (vw) copy_d2d(c, vout, vin, n * vw, s);
         return;
     }
-    launch(c, "radix_scan", k_rs_scan, dim3(fused ? 1 : run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
-    if (persistent) {
-        if (vw == 0)
-            run_passes_pf<uint32_t, false, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, fused, s);
-        else if (vw == 4)
-            run_passes_pf<uint32_t, true, false>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs,
-                                                 ctrs, fused, s);
-        else
-            run_passes_pf<uint64_t, true, false>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs,
-                                                 ctrs, fused, s);
-        return;
-    }
+    launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
     if (vw == 0)
         run_passes<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
     else if (vw == 4)
@@ -853,12 +811,6 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
         fill_u32(c, bounds, 0, F + 1, s);
         return;
     }
-    Scratch small(c, (4 * RS_BINS * 2 + 8) * 4, s);
-    uint32_t* hist = small.as<uint32_t>();
-    uint32_t* offs = hist + 4 * RS_BINS;
-    uint32_t* andor = offs + 4 * RS_BINS;
-    uint32_t* ctrs = andor + 2;
-    fill_u32(c, hist, 0, 4 * RS_BINS * 2 + 8, s);
     // passes low digit first, widths as equal as possible (longer write runs)
     RsPassDesc run{};
     run.npass = (bits + 7) / 8;
@@ -868,6 +820,25 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
         run.mask[p] = (1u << w) - 1;
         b += w;
     }
+    if (use_rts(c)) {
+        RtsBufs B(c, n, false, s);
+        if (vw == 4)
+            run_passes_rts<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, B, false, s);
+        else if (vw == 8)
+            run_passes_rts<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, B, false, s);
+        else
+            run_passes_rts<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, B, false, s);
+        launch(c, "part_bounds", k_part_bounds, dim3((uint32_t)std::min<uint64_t>(ceil_div(F + 1, 256), 1024)), dim3(256),
+               0, s, (const uint32_t*)kout, n, F, bounds);
+        return;
+    }
+    // chained passes: every pass's digit histogram from one read of the keys
+    Scratch small(c, (4 * RS_BINS * 2 + 8) * 4, s);
+    uint32_t* hist = small.as<uint32_t>();
+    uint32_t* offs = hist + 4 * RS_BINS;
+    uint32_t* andor = offs + 4 * RS_BINS;
+    uint32_t* ctrs = andor + 2;
+    fill_u32(c, hist, 0, 4 * RS_BINS * 2 + 8, s);
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
     if (bits <= RS_FULL_MAX_BITS) {
         if (bits <= 14) {
@@ -881,16 +852,7 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
         launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
     }
     launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
-    if (c->tune_get("sort_persistent", 0) != 0) {
-        if (vw == 4)
-            run_passes_pf<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs,
-                                                ctrs, false, s);
-        else if (vw == 8)
-            run_passes_pf<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs,
-                                                ctrs, false, s);
-        else
-            run_passes_pf<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, false, s);
-    } else if (vw == 4)
+    if (vw == 4)
         run_passes<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
     else if (vw == 8)
         run_passes<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);

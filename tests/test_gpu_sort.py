@@ -144,21 +144,23 @@ def test_sort_ballot_rank_u64_payload_and_hash_join(ctx, orc, ballot_rank):
 
 
 PASS_VARIANTS = {
-    "window4": {"sort_lbw": 4},
     "window8": {"sort_lbw": 8},
     "window32": {"sort_lbw": 32},
     "block1024_items8": {"sort_block": 1024},
     "block512_items8": {"sort_items": 8},
-    "persistent_fused_hist": {"sort_persistent": 1},
-    "persistent_window4": {"sort_persistent": 1, "sort_lbw": 4, "sort_fused_hist": 0},
+    "block256_early": {"sort_block": 256, "sort_early_lookback": 1},
+    "ballot_rank": {"sort_rank": 1},
+    "counted_xcd_order": {"sort_scheme": 1},
+    "counted_ticket_order": {"sort_scheme": 1, "sort_rts_order": 1},
+    "counted_block1024": {"sort_scheme": 1, "sort_block": 1024},
 }
 
 
 @pytest.fixture(params=sorted(PASS_VARIANTS))
 def pass_variant(ctx, request):
-    """Onesweep pass variants: look-back window (statuses read per round
-    trip) and the persistent pass that loads its next tile while writing
-    the current one (with or without counting the next pass's digit)."""
+    """Pass variants: the chained pass (decoupled look-back) with other
+    look-back windows, block shapes and rankings; the counted
+    (reduce-then-scan) pass in its tile orders and shapes."""
     knobs = PASS_VARIANTS[request.param]
     for k, v in knobs.items():
         ctx.set_tune(k, v)
