@@ -84,8 +84,9 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
  *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;
  *   hj_mode (-1 by build size, 0 partitioned, 1 global table),
- *   hj_global_max_build, hj_block (probe block: 1024 threads x 8 rows or
- *   512 x 16), hj_fanout, hj_rows_per_part, hj_chunk;
+ *   hj_global_max_build, hj_probe (1 direct output slots + match bitmap,
+ *   0 compacting rounds), hj_block (compacting probe block: 1024 threads x 8
+ *   rows or 512 x 16), hj_fanout, hj_rows_per_part, hj_chunk;
  *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,
  *   gb_q1_slots (1024 or 2048);
  *   serve_index_load_pct. */

@@ -231,6 +231,197 @@ __global__ __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 2) void k_hj_join(const 
     }
 }
 
+// Unique-key probe with direct output slots: a matching probe row at
+// partitioned position i writes its pair to slot i, and a bitmap records
+// which rows matched (item-local: word j of item wi covers rows s0 + 64 j ..
+// s0 + 64 j + 63, at valid[wi * vw + j]).  With every probe row matched (a
+// foreign-key join) the output is complete and dense as written; otherwise
+// k_hj_compact_bits packs it.  No block-wide scan per round (the compacting
+// probe above needs three barriers per round), and each round's keys and
+// payloads are loaded while the previous round probes.  flags as in
+// k_hj_join; wcount[wi] = rows of the item that matched.
+template <int BLOCK, int ROWS>
+__global__ __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 2) void k_hj_direct(
+    const uint32_t* __restrict__ Rk, const uint32_t* __restrict__ Rp, const uint32_t* __restrict__ HR,
+    const uint32_t* __restrict__ Sk, const uint32_t* __restrict__ Sp, const uint32_t* __restrict__ HS, uint32_t F,
+    uint32_t CH, const uint32_t* __restrict__ wstart, uint32_t* __restrict__ wcount, uint32_t* __restrict__ out_p,
+    uint32_t* __restrict__ out_b, uint64_t* __restrict__ valid, uint32_t vw, uint32_t* __restrict__ flags) {
+    constexpr int WAVES = BLOCK / PD_WAVE;
+    __shared__ uint2 tab[HJ_TS];
+    __shared__ uint32_t occ[HJ_TS / 32];
+    __shared__ uint32_t s_f, s_dup, s_cnt[WAVES];
+
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint32_t wi = blockIdx.x;
+    if (wi >= wstart[F]) return;
+    if (t == 0) {
+        uint32_t lo = 0, hi = F;   // partition: last f with wstart[f] <= wi
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (wstart[mid] <= wi) lo = mid; else hi = mid;
+        }
+        s_f = lo;
+        s_dup = 0;
+    }
+    for (uint32_t i = t; i < HJ_TS / 32; i += BLOCK) occ[i] = 0;
+    __syncthreads();
+    const uint32_t f = s_f;
+    const uint32_t ck = wi - wstart[f];
+    const uint64_t r0 = HR[f], r1 = HR[f + 1];
+    const uint64_t s0 = (uint64_t)HS[f] + (uint64_t)ck * CH, s1 = min((uint64_t)HS[f + 1], s0 + CH);
+    if (r1 - r0 > (uint64_t)HJ_TS * 7 / 8) {
+        if (t == 0) {
+            atomicOr(&flags[1], 1u);
+            wcount[wi] = 0;
+        }
+        return;
+    }
+    // first round's probe rows, in flight during the build
+    uint32_t key[ROWS], pay[ROWS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+        const uint64_t i = s0 + (uint64_t)k * BLOCK + t;
+        key[k] = i < s1 ? Sk[i] : 0u;
+        pay[k] = i < s1 ? Sp[i] : 0u;
+    }
+    // ---- build ----
+    for (uint64_t i = r0 + t; i < r1; i += BLOCK) {
+        const uint2 kv = make_uint2(Rk[i], Rp[i]);
+        uint32_t h = pd::slot_hash(kv.x) & (HJ_TS - 1);
+        while (true) {
+            const uint32_t bit = 1u << (h & 31);
+            const uint32_t prev = atomicOr(&occ[h >> 5], bit);
+            if (!(prev & bit)) {
+                tab[h] = kv;
+                break;
+            }
+            h = (h + 1) & (HJ_TS - 1);
+        }
+    }
+    __syncthreads();
+
+    // ---- probe ----
+    uint32_t matched = 0;
+    bool dup = false;
+    for (uint64_t rb = s0; rb < s1; rb += (uint64_t)BLOCK * ROWS) {
+        uint32_t nkey[ROWS], npay[ROWS];
+        const uint64_t nb = rb + (uint64_t)BLOCK * ROWS;
+#pragma unroll
+        for (int k = 0; k < ROWS; k++) {
+            const uint64_t i = nb + (uint64_t)k * BLOCK + t;
+            nkey[k] = i < s1 ? Sk[i] : 0u;
+            npay[k] = i < s1 ? Sp[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < ROWS; k++) {
+            const uint64_t i = rb + (uint64_t)k * BLOCK + t;
+            uint32_t m = 0, first = 0;
+            if (i < s1) {
+                uint32_t h = pd::slot_hash(key[k]) & (HJ_TS - 1);
+                uint32_t o = occ[h >> 5];
+                uint2 e = tab[h];
+                while (o & (1u << (h & 31))) {
+                    if (e.x == key[k]) {
+                        if (m == 0) first = e.y;
+                        m++;
+                    }
+                    h = (h + 1) & (HJ_TS - 1);
+                    o = occ[h >> 5];
+                    e = tab[h];
+                }
+            }
+            dup |= m > 1;
+            if (m) {
+                out_p[i] = pay[k];
+                out_b[i] = first;
+                matched++;
+            }
+            // rows rb - s0 + k * BLOCK + w * 64 .. + 63 of the item: one word
+            const uint64_t b = pd::ballot(m != 0);
+            if (l == 0) valid[(uint64_t)wi * vw + (rb - s0 + (uint64_t)k * BLOCK + (uint64_t)w * PD_WAVE) / PD_WAVE] = b;
+        }
+#pragma unroll
+        for (int k = 0; k < ROWS; k++) {
+            key[k] = nkey[k];
+            pay[k] = npay[k];
+        }
+    }
+    matched = pd::wave_sum(matched);
+    if (l == 0) s_cnt[w] = matched;
+    if (dup) s_dup = 1;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < WAVES; i++) tot += s_cnt[i];
+        wcount[wi] = tot;
+        if (s_dup) atomicOr(&flags[0], 1u);
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(flags + 2), (unsigned long long)tot);
+    }
+}
+
+// Packs the direct probe's output: work item wi's matched rows (bitmap
+// words valid[wi * vw ..]) go, in row order, to woff[wi] onwards.  One block
+// per work item; wave v takes words v, v + 4, ...; lane = bit.
+__global__ __launch_bounds__(256) void k_hj_compact_bits(const uint32_t* __restrict__ HS, uint32_t F, uint32_t CH,
+                                                         const uint32_t* __restrict__ wstart,
+                                                         const uint32_t* __restrict__ woff,
+                                                         const uint64_t* __restrict__ valid, uint32_t vw,
+                                                         const uint32_t* __restrict__ in_p,
+                                                         const uint32_t* __restrict__ in_b, uint32_t* __restrict__ out_p,
+                                                         uint32_t* __restrict__ out_b, uint64_t out_cap) {
+    __shared__ uint32_t s_f, s_w[4];
+    __shared__ uint32_t s_wo[1024];   // exclusive popcount prefix per word (vw <= 1024)
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint32_t wi = blockIdx.x;
+    if (wi >= wstart[F]) return;
+    if (t == 0) {
+        uint32_t lo = 0, hi = F;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (wstart[mid] <= wi) lo = mid; else hi = mid;
+        }
+        s_f = lo;
+    }
+    __syncthreads();
+    const uint32_t f = s_f;
+    const uint64_t s0 = (uint64_t)HS[f] + (uint64_t)(wi - wstart[f]) * CH;
+    const uint64_t s1 = min((uint64_t)HS[f + 1], s0 + CH);
+    const uint32_t nw = (uint32_t)((s1 - s0 + PD_WAVE - 1) / PD_WAVE);
+    const uint64_t* vb = valid + (uint64_t)wi * vw;
+    // block exclusive scan of the words' popcounts (4 words per thread)
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t j = t * 4 + q;
+        c[q] = j < nw ? pd::popc64(vb[j]) : 0u;
+        sum += c[q];
+    }
+    const uint32_t inc = pd::wave_incl_scan(sum);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (uint32_t i = 0; i < w; i++) run += s_w[i];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        s_wo[t * 4 + q] = run;
+        run += c[q];
+    }
+    __syncthreads();
+    const uint64_t base = woff[wi];
+    const uint64_t lt = pd::lanemask_lt();
+    for (uint32_t j = w; j < nw; j += 4) {
+        const uint64_t bits = vb[j];
+        if ((bits >> l) & 1) {
+            const uint64_t src = s0 + (uint64_t)j * PD_WAVE + l;
+            const uint64_t dst = base + s_wo[j] + pd::popc64(bits & lt);
+            if (dst < out_cap) {
+                out_p[dst] = in_p[src];
+                out_b[dst] = in_b[src];
+            }
+        }
+    }
+}
+
 // Moves each work item's compacted output [wbase(wi), +wcount) to woff[wi].
 __global__ __launch_bounds__(256) void k_hj_compact(const uint32_t* __restrict__ HS, uint32_t GS, uint32_t F,
                                                     uint32_t CH, const uint32_t* __restrict__ wstart,
@@ -301,6 +492,12 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         CH = std::max<uint32_t>(8192, next_pow2(ceil_div(np, (uint64_t)F) * 3 / 2));
         while (CH > 8192 && (uint64_t)CH * 4 * (uint64_t)c->cu_count > np) CH >>= 1;
     }
+    // probe (knob hj_probe): 1 = direct output slots + match bitmap (items of
+    // at most 65536 rows: the packing kernel scans 1024 bitmap words per
+    // item), 0 = compacting rounds
+    const bool direct_probe = c->tune_get("hj_probe", 1) == 1;
+    if (direct_probe) CH = std::min<uint32_t>(std::max<uint32_t>(CH, 1), 65536);
+    const uint32_t vw = (uint32_t)(ceil_div(CH, 8192) * 128);   // bitmap words per item (whole 8192-row rounds)
     constexpr uint32_t GR = 1, GS = 1;   // partition f = [H[f], H[f + 1])
     Scratch rk(c, nb * 4, s), rp(c, nb * 4, s), sk(c, np * 4, s), sp(c, np * 4, s);
     Scratch flags(c, 16, s);
@@ -324,18 +521,60 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         Scratch wcnt(c, (max_wi + 1) * 4, s), woff(c, (max_wi + 1) * 4, s);
         fill_u32(c, flags.get(), 0, 4, s);
         fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
+        Scratch valid(c, direct_probe ? max_wi * vw * 8 : 0, s);
+        if (direct_probe) {
+            launch(c, "hj_join", k_hj_direct<1024, 8>, dim3((uint32_t)max_wi), dim3(1024), 0, s,
+                   (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
+                   (const uint32_t*)HR.as<uint32_t>(), (const uint32_t*)sk.as<uint32_t>(),
+                   (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), F, CH,
+                   (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), p0, b0, valid.as<uint64_t>(), vw,
+                   flags.as<uint32_t>());
+            uint32_t fl[4];
+            PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
+            PD_HIP(hipStreamSynchronize(s));
+            if (fl[1]) {  // a build partition overflowed the table: more partitions
+                F *= 2;
+                continue;
+            }
+            if (!fl[0]) {
+                const uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
+                if (direct && total == np) return total;  // every probe row matched once: dense as written
+                exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
+                if (out_p && out_b && cap) {
+                    if (direct) {
+                        Scratch cp(c, total * 4 + 4, s), cb(c, total * 4 + 4, s);
+                        launch(c, "hj_compact", k_hj_compact_bits, dim3((uint32_t)max_wi), dim3(256), 0, s,
+                               (const uint32_t*)HS.as<uint32_t>(), F, CH, (const uint32_t*)wst.as<uint32_t>(),
+                               (const uint32_t*)woff.as<uint32_t>(), (const uint64_t*)valid.as<uint64_t>(), vw,
+                               (const uint32_t*)p0, (const uint32_t*)b0, cp.as<uint32_t>(), cb.as<uint32_t>(), total);
+                        copy_d2d(c, out_p, cp.get(), std::min(total, cap) * 4, s);
+                        copy_d2d(c, out_b, cb.get(), std::min(total, cap) * 4, s);
+                        PD_HIP(hipStreamSynchronize(s));
+                    } else {
+                        launch(c, "hj_compact", k_hj_compact_bits, dim3((uint32_t)max_wi), dim3(256), 0, s,
+                               (const uint32_t*)HS.as<uint32_t>(), F, CH, (const uint32_t*)wst.as<uint32_t>(),
+                               (const uint32_t*)woff.as<uint32_t>(), (const uint64_t*)valid.as<uint64_t>(), vw,
+                               (const uint32_t*)p0, (const uint32_t*)b0, out_p, out_b, cap);
+                    }
+                }
+                return total;
+            }
+            // duplicate build keys: the counting path below
+        }
 #define PD_HJ_UNIQUE(B, R)                                                                                     \
     launch(c, "hj_join", k_hj_join<HJ_UNIQUE, B, R>, dim3((uint32_t)max_wi), dim3(B), 0, s,                        \
            (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(), (const uint32_t*)HR.as<uint32_t>(), \
            GR, (const uint32_t*)sk.as<uint32_t>(), (const uint32_t*)sp.as<uint32_t>(),                             \
            (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), \
            (const uint32_t*)nullptr, p0, b0, (uint64_t)np, flags.as<uint32_t>())
-        if (c->tune_get("hj_block", 1024) == 1024) PD_HJ_UNIQUE(1024, 8);
-        else PD_HJ_UNIQUE(512, 16);
+        uint32_t fl[4] = {1, 0, 0, 0};   // direct probe found duplicate build keys
+        if (!direct_probe) {
+            if (c->tune_get("hj_block", 1024) == 1024) PD_HJ_UNIQUE(1024, 8);
+            else PD_HJ_UNIQUE(512, 16);
+            PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
+            PD_HIP(hipStreamSynchronize(s));
+        }
 #undef PD_HJ_UNIQUE
-        uint32_t fl[4];
-        PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
         if (fl[1]) {  // a build partition overflowed the table: more partitions
             F *= 2;
             continue;

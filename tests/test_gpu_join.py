@@ -14,30 +14,37 @@ def run(ctx, fn, rk, rp, sk, sp):
     return op.numpy(k), ob.numpy(k)
 
 
-HJ_MODE = {"hash": -1, "hash_part": 0, "hash_part_512": 0, "hash_global": 1}
+HJ_KNOBS = {
+    "hash": {"hj_mode": -1},
+    "hash_part": {"hj_mode": 0},
+    "hash_part_rounds": {"hj_mode": 0, "hj_probe": 0},
+    "hash_part_rounds_512": {"hj_mode": 0, "hj_probe": 0, "hj_block": 512},
+    "hash_global": {"hj_mode": 1},
+}
 
 
 def join_fn(ctx, which):
-    """hash = mode picked by build size (hj_mode -1), hash_part = radix-
-    partitioned LDS tables (0) probed by 1024-thread blocks, hash_part_512 =
-    the same probed by 512-thread blocks, hash_global = one global table (1),
-    smj = sort-merge join."""
+    """hash = mode picked by build size (hj_mode -1); hash_part = radix-
+    partitioned LDS tables (hj_mode 0) with the direct-slot probe; hash_part_
+    rounds(_512) = the same with the compacting probe in 1024- (512-) thread
+    blocks; hash_global = one global table (hj_mode 1); smj = sort-merge join."""
     if which == "smj":
         return ops.sort_merge_join
 
     def fn(c, *args):
-        ctx.set_tune("hj_mode", HJ_MODE[which])
-        ctx.set_tune("hj_block", 512 if which == "hash_part_512" else 1024)
+        knobs = HJ_KNOBS[which]
+        for k, v in knobs.items():
+            ctx.set_tune(k, v)
         try:
             return ops.hash_join(c, *args)
         finally:
-            ctx.set_tune("hj_mode", -1)
-            ctx.set_tune("hj_block", 1024)
+            for k in knobs:
+                ctx.clear_tune(k)
     return fn
 
 
 @pytest.mark.parametrize("zipf", [False, True])
-@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_512", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj"])
 def test_join_pk_fk_2_20(ctx, orc, zipf, which):
     rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 20, zipf=zipf)
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
@@ -49,7 +56,7 @@ def test_join_pk_fk_2_20(ctx, orc, zipf, which):
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_part_512", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj"])
 @pytest.mark.parametrize("nb,ns,kr,ks", [(1, 1, 1, 1), (3000, 5000, 1000, 2000), (100, 70_000, 60, 200),
                                          (50_000, 3, 10, 10), (4097, 8193, 1 << 31, 1 << 31)])
 def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
@@ -59,6 +66,25 @@ def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
     sp = orc.gen_u32(ns, 4, ns)
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
     ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
+    if which == "smj":
+        assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
+    else:
+        assert same_multiset((gp, gb), (ep, eb))
+
+
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_global", "smj"])
+def test_join_unique_build_half_misses(ctx, orc, which):
+    """Unique build keys (a permutation of [0, 2^18)) probed by keys uniform
+    on [0, 2^19): about half the probe rows match nothing, so the direct
+    probe's output has holes that the bitmap packing closes."""
+    nb, ns = 1 << 18, 1 << 20
+    rk = orc.gen_perm(11, 1, nb, nb)
+    rp = orc.gen_u32(nb, 2, nb)
+    sk = orc.gen_range(np.uint32, 12, 3, ns, 0, 2 * nb - 1)
+    sp = orc.gen_u32(ns, 4, ns)
+    gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
+    ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
+    assert 0.4 * ns < len(ep) < 0.6 * ns
     if which == "smj":
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
     else:
