@@ -87,6 +87,8 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   hj_global_max_build, hj_probe (1 direct output slots + match bitmap,
  *   0 compacting rounds), hj_block (compacting probe block: 1024 threads x 8
  *   rows or 512 x 16), hj_fanout, hj_rows_per_part, hj_chunk;
+ *   smj_merge (1 direct output slots + match bitmap, 0 ticketed merge with
+ *   decoupled look-back over per-tile match counts);
  *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,
  *   gb_q1_slots (1024 or 2048);
  *   serve_index_load_pct. */

@@ -104,7 +104,7 @@ __global__ void k_flag_lut(const uint32_t* __restrict__ bits, uint8_t* __restric
         uint32_t r = 0;
         for (int v = 0; v < 256; v++) {
             const bool present = (bits[c * 8 + (v >> 5)] >> (v & 31)) & 1;
-            lut[c * 256 + v] = present ? (uint8_t)r : 0;
+            lut[c * 256 + v] = present ? (uint8_t)r : (uint8_t)0xFF;   // 0xFF: not in the domain
             if (present) { inv[c * 256 + r] = (uint8_t)v; r++; }
         }
         meta[c] = r;
@@ -124,23 +124,31 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
                                                        const int64_t* __restrict__ tax, uint64_t n, int32_t cutoff,
                                                        const uint8_t* __restrict__ lut,
                                                        const uint32_t* __restrict__ meta,
-                                                       uint64_t* __restrict__ partials) {
+                                                       uint64_t* __restrict__ partials,
+                                                       uint32_t* __restrict__ miss) {
     __shared__ uint8_t s_lut[512];
+    __shared__ uint32_t s_miss;
     __shared__ uint64_t s_red[QD_BLOCK / PD_WAVE][G][8];
     for (uint32_t i = threadIdx.x; i < 512; i += QD_BLOCK) s_lut[i] = lut[i];
+    if (threadIdx.x == 0) s_miss = 0;
     __syncthreads();
     const uint32_t n2 = meta[1];
-    if (meta[0] * n2 > (uint32_t)G) return;  // the general path handles it
+    const uint32_t ngr = meta[0] * n2;
+    if (ngr > (uint32_t)G) return;  // the general path handles it
     Q1Acc acc[G];
 #pragma unroll
     for (int g = 0; g < G; g++) acc[g] = Q1Acc{0, 0, 0, 0, 0, 0};
+    bool out_of_domain = false;   // a kept row's flag byte is missing from the code table
     auto add_row = [&](int32_t dt, uint32_t a, uint32_t b, int64_t q, int64_t p, int64_t d, int64_t x) {
         if (dt > cutoff) return;
-        const uint32_t code = (uint32_t)s_lut[a] * n2 + s_lut[256 + b];
+        const uint32_t ca = s_lut[a], cb = s_lut[256 + b];
+        if (ca == 0xFFu || cb == 0xFFu) { out_of_domain = true; return; }
+        const uint32_t code = ca * n2 + cb;
         const int64_t dp = p * (100 - d);
         const __int128 ch = (__int128)dp * (100 + x);
 #pragma unroll
         for (int g = 0; g < G; g++) {
+            if ((uint32_t)g >= ngr) break;   // wave-uniform: only as many slots as codes
             const bool on = code == (uint32_t)g;
             acc[g].q += on ? q : 0;
             acc[g].p += on ? p : 0;
@@ -173,6 +181,7 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
         for (uint64_t i = blockIdx.x * (uint64_t)QD_BLOCK + threadIdx.x; i < n; i += stride)
             add_row(date[i], f1[i], f2[i], qty[i], price[i], disc[i], tax[i]);
     }
+    if (out_of_domain) s_miss = 1;
     // block reduction: wave sums of 64-bit words; the low charge word is
     // split into 32-bit halves so the 64-bit sums cannot drop carries
     const uint32_t w = threadIdx.x / PD_WAVE, l = pd::lane();
@@ -194,6 +203,7 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
         for (int ww = 0; ww < QD_BLOCK / PD_WAVE; ww++) s += s_red[ww][g][f];
         partials[((uint64_t)blockIdx.x * G + g) * 8 + f] = s;
     }
+    if (threadIdx.x == 0 && s_miss) atomicOr(miss, 1u);
 }
 
 // partial layout per (block, group): q, p, d, dp, sum(low 32 bits of the charge's
@@ -981,50 +991,72 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
                    out.count, PIMDAL_E_INVALID, "missing output column");
     if (n == 0) return 0;
     if (!key) {
-        // dense path: flag domains -> codes -> register accumulators
-        Scratch small(c, 64 + 8 + 512 + 512 + 16, s);
-        uint32_t* bits = small.as<uint32_t>();
-        uint32_t* meta = bits + 16;
-        uint8_t* lut = reinterpret_cast<uint8_t*>(meta + 2);
+        // dense path: flag codes from a 64K-row sample, then filter + Q1
+        // measures in register accumulators for up to 8 codes; a kept row
+        // whose flag byte the sample lacks sets `miss`, and the call is then
+        // redone with the codes of every row.  One readback at the end.
+        // scratch: ng u64 | miss | pad | meta[2] | bits[16] | lut[512] | inv[512]
+        Scratch small(c, 8 + 8 + 8 + 64 + 512 + 512, s);
+        uint64_t* ng = small.as<uint64_t>();
+        uint32_t* miss = reinterpret_cast<uint32_t*>(ng + 1);
+        uint32_t* meta = miss + 2;
+        uint32_t* bits = meta + 2;
+        uint8_t* lut = reinterpret_cast<uint8_t*>(bits + 16);
         uint8_t* inv = lut + 512;
-        uint64_t* ng = reinterpret_cast<uint64_t*>(inv + 512);
-        fill_u32(c, bits, 0, 16 + 2, s);
-        const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64), (uint64_t)c->cu_count * 4));
-        launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
-        launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut, inv, meta);
-        uint32_t nm[2];
-        PD_HIP(hipMemcpyAsync(nm, meta, 8, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
-        const uint32_t groups = nm[0] * nm[1];
-        if (groups <= 8) {
-            const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, QD_BLOCK * 16), (uint64_t)c->cu_count * 8));
-            Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s), sums(c, 8 * 6 * 16, s);
-            const bool vec = ((uintptr_t)date | (uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)qty | (uintptr_t)price |
-                              (uintptr_t)disc | (uintptr_t)tax) % 16 == 0;
-            // accumulators for exactly as many codes as the data has (rounded
-            // up to even): fewer selects per row and fewer registers
-            const uint32_t Gt = groups <= 2 ? 2 : groups <= 4 ? 4 : groups <= 6 ? 6 : 8;
-            auto dense = [&](auto gtag) {
-                constexpr int GG = decltype(gtag)::value;
-                if (vec)
-                    launch(c, "q1_dense", k_q1_dense<GG, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price,
-                           disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
-                else
-                    launch(c, "q1_dense", k_q1_dense<GG, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty,
-                           price, disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>());
-            };
-            if (Gt == 2) dense(std::integral_constant<int, 2>{});
-            else if (Gt == 4) dense(std::integral_constant<int, 4>{});
-            else if (Gt == 6) dense(std::integral_constant<int, 6>{});
-            else dense(std::integral_constant<int, 8>{});
-            launch(c, "q1_reduce", k_q1_reduce, dim3(Gt), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, Gt,
-                   sums.as<unsigned __int128>());
+        const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, QD_BLOCK * 16), (uint64_t)c->cu_count * 8));
+        Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s), sums(c, 8 * 6 * 16, s);
+        const bool vec = ((uintptr_t)date | (uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)qty | (uintptr_t)price |
+                          (uintptr_t)disc | (uintptr_t)tax) % 16 == 0;
+        auto codes = [&](uint64_t rows) {
+            fill_u32(c, bits, 0, 16, s);
+            const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(rows, 256 * 64), (uint64_t)c->cu_count * 4));
+            launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, rows, bits);
+            launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut, inv, meta);
+        };
+        auto dense = [&](auto gtag) {
+            constexpr int GG = decltype(gtag)::value;
+            fill_u32(c, miss, 0, 1, s);
+            if (vec)
+                launch(c, "q1_dense", k_q1_dense<GG, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc,
+                       tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
+            else
+                launch(c, "q1_dense", k_q1_dense<GG, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price,
+                       disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
+            launch(c, "q1_reduce", k_q1_reduce, dim3(GG), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb,
+                   (uint32_t)GG, sums.as<unsigned __int128>());
             launch(c, "q1_final", k_q1_final, dim3(1), dim3(64), 0, s, (const unsigned __int128*)sums.as<unsigned __int128>(),
-                   Gt, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price, out.sum_disc,
-                   out.sum_disc_price, out.sum_charge, out.count, ng);
-            const uint64_t k = read_u64(c, ng, s);
-            PD_REQUIRE(k <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-            return k;
+                   (uint32_t)GG, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price,
+                   out.sum_disc, out.sum_disc_price, out.sum_charge, out.count, ng);
+        };
+        struct { uint64_t ng; uint32_t miss, pad, n1, n2; } hv;
+        auto readback = [&]() {
+            PD_HIP(hipMemcpyAsync(&hv, ng, sizeof(hv), hipMemcpyDeviceToHost, s));
+            PD_HIP(hipStreamSynchronize(s));
+        };
+        codes(std::min<uint64_t>(n, 65536));
+        dense(std::integral_constant<int, 8>{});
+        readback();
+        bool done = hv.n1 * hv.n2 <= 8 && !hv.miss;
+        if (!done && hv.n1 * hv.n2 <= 8) {
+            // the sample missed a flag value: codes of every row
+            codes(n);
+            PD_HIP(hipMemcpyAsync(&hv.n1, meta, 8, hipMemcpyDeviceToHost, s));
+            PD_HIP(hipStreamSynchronize(s));
+            const uint32_t groups = hv.n1 * hv.n2;
+            if (groups <= 8) {
+                // accumulators for exactly as many codes as the data has
+                // (rounded up to even): fewer selects per row
+                if (groups <= 2) dense(std::integral_constant<int, 2>{});
+                else if (groups <= 4) dense(std::integral_constant<int, 4>{});
+                else if (groups <= 6) dense(std::integral_constant<int, 6>{});
+                else dense(std::integral_constant<int, 8>{});
+                readback();
+                done = true;
+            }
+        }
+        if (done) {
+            PD_REQUIRE(hv.ng <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+            return hv.ng;
         }
     }
     GbSource src{};

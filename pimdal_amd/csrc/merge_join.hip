@@ -147,6 +147,136 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_unique(const uint32_t* __restr
     }
 }
 
+// Unique build keys, direct output slots: probe row j (sorted order) writes
+// its pair to slot j when it matches, and valid[j / 64] bit j % 64 records
+// that it did; tcnt[tile] = the tile's matches.  With every probe row
+// matched (a foreign-key join) the output is complete, in probe order, as
+// written; otherwise k_smj_pack closes the holes (order kept).  No tile
+// waits on another (no ticket, no look-back).  Matching is a merge: each
+// thread takes 16 consecutive probe rows of the tile (staged in LDS, one pad
+// word per 16 so a wave's rows fall in distinct banks), binary-searches the
+// build keys for its first row and walks forward for the rest.
+constexpr uint32_t SM_MISS = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t smj_pad(uint32_t r) { return r + (r >> 4); }
+
+__global__ __launch_bounds__(SM_BLOCK) void k_smj_direct(const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rp,
+                                                         const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sp,
+                                                         uint64_t ns, uint32_t* __restrict__ out_p,
+                                                         uint32_t* __restrict__ out_b, uint64_t* __restrict__ valid,
+                                                         uint32_t* __restrict__ tcnt, const uint64_t* __restrict__ ranges) {
+    __shared__ uint32_t s_rk[SM_RCAP];
+    __shared__ uint32_t s_x[SM_TILE + SM_TILE / 16];   // probe keys, then build row (or SM_MISS) per probe row
+    __shared__ uint32_t s_w[SM_WAVES];
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint64_t tile = blockIdx.x;
+    const uint64_t base = tile * SM_TILE;
+    const uint32_t cnt_rows = (uint32_t)min((uint64_t)SM_TILE, ns - base);
+    // probe rows base + i*BLOCK + t (coalesced); keys to LDS, payloads kept
+    uint32_t pay[SM_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SM_ITEMS; i++) {
+        const uint32_t r = (uint32_t)i * SM_BLOCK + t;
+        const uint32_t k = r < cnt_rows ? sk[base + r] : 0xFFFFFFFFu;
+        pay[i] = r < cnt_rows ? sp[base + r] : 0u;
+        s_x[smj_pad(r)] = k;
+    }
+    const uint64_t lo = ranges[2 * tile], hi = ranges[2 * tile + 1];
+    const uint32_t m = (uint32_t)(hi - lo);
+    const bool cached = m <= SM_RCAP;
+    if (cached)
+        for (uint32_t i = t; i < m; i += SM_BLOCK) s_rk[i] = rk[lo + i];
+    __syncthreads();
+    auto bkey = [&](uint32_t p) -> uint32_t { return cached ? s_rk[p] : rk[lo + p]; };
+    // ---- merge: rows t*16 .. t*16+15 of the tile ----
+    {
+        const uint32_t r0 = t * SM_ITEMS;
+        uint32_t p = 0;
+        if (r0 < cnt_rows) {
+            const uint32_t k0 = s_x[smj_pad(r0)];
+            uint32_t a = 0, b = m;   // lower bound of k0 in the build range
+            while (a < b) {
+                const uint32_t mid = (a + b) >> 1;
+                if (bkey(mid) < k0) a = mid + 1; else b = mid;
+            }
+            p = a;
+        }
+#pragma unroll 4
+        for (int j = 0; j < SM_ITEMS; j++) {
+            const uint32_t r = r0 + j;
+            if (r >= cnt_rows) break;
+            const uint32_t k = s_x[smj_pad(r)];
+            while (p < m && bkey(p) < k) p++;
+            s_x[smj_pad(r)] = (p < m && bkey(p) == k) ? (uint32_t)(lo + p) : SM_MISS;
+        }
+    }
+    __syncthreads();
+    // ---- gather build payloads (all in flight together), then write the
+    // matched pairs at their rows ----
+    uint32_t bval[SM_ITEMS], hits = 0;
+#pragma unroll
+    for (int i = 0; i < SM_ITEMS; i++) {
+        const uint32_t r = (uint32_t)i * SM_BLOCK + t;
+        const uint32_t bx = r < cnt_rows ? s_x[smj_pad(r)] : SM_MISS;
+        hits |= (uint32_t)(bx != SM_MISS) << i;
+        bval[i] = bx != SM_MISS ? rp[bx] : 0u;
+    }
+    uint32_t matched = 0;
+#pragma unroll
+    for (int i = 0; i < SM_ITEMS; i++) {
+        const uint32_t r = (uint32_t)i * SM_BLOCK + t;
+        const bool hit = (hits >> i) & 1;
+        if (hit) {
+            out_p[base + r] = pay[i];
+            out_b[base + r] = bval[i];
+            matched++;
+        }
+        const uint64_t bal = pd::ballot(hit);
+        if (l == 0 && base + (uint64_t)i * SM_BLOCK + (uint64_t)w * PD_WAVE < ns)
+            valid[(base + (uint64_t)i * SM_BLOCK + (uint64_t)w * PD_WAVE) / PD_WAVE] = bal;
+    }
+    matched = pd::wave_sum(matched);
+    if (l == 0) s_w[w] = matched;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < SM_WAVES; i++) tot += s_w[i];
+        tcnt[tile] = tot;
+    }
+}
+
+// Packs the direct merge's output in probe order: tile `tile`'s matched rows
+// (bitmap words valid[tile * 64 ..]) go to toff[tile] onwards.
+__global__ __launch_bounds__(SM_BLOCK) void k_smj_pack(uint64_t ns, const uint64_t* __restrict__ valid,
+                                                       const uint32_t* __restrict__ toff,
+                                                       const uint32_t* __restrict__ in_p, const uint32_t* __restrict__ in_b,
+                                                       uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
+                                                       uint64_t cap) {
+    constexpr uint32_t WPT = SM_TILE / PD_WAVE;   // 64 bitmap words per tile
+    __shared__ uint32_t s_wo[WPT];
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint64_t tile = blockIdx.x;
+    const uint64_t w0 = tile * WPT;
+    const uint64_t nw_all = (ns + PD_WAVE - 1) / PD_WAVE;
+    if (w == 0) {
+        const uint32_t c = (w0 + l < nw_all) ? pd::popc64(valid[w0 + l]) : 0u;
+        s_wo[l] = pd::wave_incl_scan(c) - c;
+    }
+    __syncthreads();
+    const uint64_t obase = toff[tile];
+    const uint64_t lt = pd::lanemask_lt();
+    for (uint32_t j = w; j < WPT && w0 + j < nw_all; j += SM_WAVES) {
+        const uint64_t bits = valid[w0 + j];
+        if ((bits >> l) & 1) {
+            const uint64_t src = (w0 + j) * PD_WAVE + l;
+            const uint64_t dst = obase + s_wo[j] + pd::popc64(bits & lt);
+            if (dst < cap) {
+                out_p[dst] = in_p[src];
+                out_b[dst] = in_b[src];
+            }
+        }
+    }
+}
+
 // Build-side range each probe tile can match: [lower_bound(first key),
 // upper_bound(last key)).  One thread per tile, all tiles in parallel.
 __global__ void k_smj_tile_ranges(const uint32_t* __restrict__ rk, uint64_t nr, const uint32_t* __restrict__ sk,
@@ -220,6 +350,39 @@ uint64_t sort_merge_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, 
         launch(c, "smj_ranges", k_smj_tile_ranges, dim3((uint32_t)ceil_div(ntiles, 256)), dim3(256), 0, s,
                (const uint32_t*)rk.as<uint32_t>(), nb, (const uint32_t*)sk.as<uint32_t>(), np, ntiles,
                rng.as<uint64_t>());
+        // merge (knob smj_merge): 1 = direct output slots + match bitmap,
+        // 0 = compacting merge with decoupled look-back
+        if (c->tune_get("smj_merge", 1) == 1) {
+            Scratch valid(c, ceil_div(np, 64) * 8, s), tcnt(c, (ntiles + 1) * 4, s), toff(c, (ntiles + 1) * 4, s),
+                tot(c, 8, s);
+            launch(c, "smj_merge", k_smj_direct, dim3((uint32_t)ntiles), dim3(SM_BLOCK), 0, s,
+                   (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
+                   (const uint32_t*)sk.as<uint32_t>(), (const uint32_t*)spay.as<uint32_t>(), np, op, ob,
+                   valid.as<uint64_t>(), tcnt.as<uint32_t>(), (const uint64_t*)rng.as<uint64_t>());
+            exclusive_scan_u32(c, tcnt.as<uint32_t>(), toff.as<uint32_t>(), ntiles, tot.as<uint64_t>(), s);
+            const uint64_t total = read_u64(c, tot.get(), s);
+            if (total == np) {   // every probe row matched: complete and in order as written
+                if (!direct && out_p && out_b && cap) {
+                    copy_d2d(c, out_p, op, std::min(total, cap) * 4, s);
+                    copy_d2d(c, out_b, ob, std::min(total, cap) * 4, s);
+                    PD_HIP(hipStreamSynchronize(s));
+                }
+                return total;
+            }
+            if (out_p && out_b && cap) {
+                Scratch cp(c, direct ? total * 4 + 4 : 0, s), cb(c, direct ? total * 4 + 4 : 0, s);
+                launch(c, "smj_pack", k_smj_pack, dim3((uint32_t)ntiles), dim3(SM_BLOCK), 0, s, np,
+                       (const uint64_t*)valid.as<uint64_t>(), (const uint32_t*)toff.as<uint32_t>(), (const uint32_t*)op,
+                       (const uint32_t*)ob, direct ? cp.as<uint32_t>() : out_p, direct ? cb.as<uint32_t>() : out_b,
+                       direct ? total : cap);
+                if (direct) {
+                    copy_d2d(c, out_p, cp.get(), std::min(total, cap) * 4, s);
+                    copy_d2d(c, out_b, cb.get(), std::min(total, cap) * 4, s);
+                }
+                PD_HIP(hipStreamSynchronize(s));
+            }
+            return total;
+        }
         launch(c, "smj_merge", k_smj_unique, dim3((uint32_t)ntiles), dim3(SM_BLOCK), 0, s,
                (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(), nb,
                (const uint32_t*)sk.as<uint32_t>(), (const uint32_t*)spay.as<uint32_t>(), np, op, ob,

@@ -160,6 +160,28 @@ def test_hash_aggr_many_partitions(orc):
     assert all(np.array_equal(a, b) for a, b in zip(got, exp))
 
 
+@pytest.mark.parametrize("case", ["late_value", "late_values_past_8_groups", "late_value_filtered_out"])
+def test_q1_flags_values_after_the_code_sample(ctx, orc, case):
+    """The dense path takes its flag codes from the first 65536 rows; a kept
+    row with a flag value the sample lacks sends the call to the codes of
+    every row (or, past 8 groups, to the general path); a row the date
+    filter drops does not."""
+    n = 200_000
+    t = orc.fact_table(n)
+    if case == "late_value":
+        t["flag1"][150_000] = 9
+        t["date"][150_000] = 10
+    elif case == "late_values_past_8_groups":
+        t["flag1"][100_000:100_010] = np.arange(3, 13, dtype=np.uint8)
+        t["date"][100_000:100_010] = 10
+    else:
+        t["flag1"][150_000] = 9
+        t["date"][150_000] = 2556
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, capacity=4096)
+    _cmp_q1(got, orc.q1_flags(t, 2505), ordered=(case != "late_values_past_8_groups"))
+
+
 @pytest.mark.parametrize("n1,n2", [(2, 2), (4, 2), (1, 1)])
 def test_q1_flags_dense_group_counts(ctx, orc, n1, n2):
     """The dense path instantiated for 2, 4 and 8 group codes."""

@@ -20,6 +20,7 @@ HJ_KNOBS = {
     "hash_part_rounds": {"hj_mode": 0, "hj_probe": 0},
     "hash_part_rounds_512": {"hj_mode": 0, "hj_probe": 0, "hj_block": 512},
     "hash_global": {"hj_mode": 1},
+    "smj_lookback": {"smj_merge": 0},
 }
 
 
@@ -27,16 +28,19 @@ def join_fn(ctx, which):
     """hash = mode picked by build size (hj_mode -1); hash_part = radix-
     partitioned LDS tables (hj_mode 0) with the direct-slot probe; hash_part_
     rounds(_512) = the same with the compacting probe in 1024- (512-) thread
-    blocks; hash_global = one global table (hj_mode 1); smj = sort-merge join."""
+    blocks; hash_global = one global table (hj_mode 1); smj = sort-merge join
+    (direct-slot merge); smj_lookback = the merge that compacts through a
+    decoupled look-back."""
     if which == "smj":
         return ops.sort_merge_join
 
     def fn(c, *args):
         knobs = HJ_KNOBS[which]
+        call = ops.sort_merge_join if which.startswith("smj") else ops.hash_join
         for k, v in knobs.items():
             ctx.set_tune(k, v)
         try:
-            return ops.hash_join(c, *args)
+            return call(c, *args)
         finally:
             for k in knobs:
                 ctx.clear_tune(k)
@@ -44,19 +48,21 @@ def join_fn(ctx, which):
 
 
 @pytest.mark.parametrize("zipf", [False, True])
-@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj",
+                                   "smj_lookback"])
 def test_join_pk_fk_2_20(ctx, orc, zipf, which):
     rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 20, zipf=zipf)
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
-    ep, eb = orc.sort_merge_join(rk, rp, sk, sp) if which == "smj" else orc.hash_join(rk, rp, sk, sp)
+    ep, eb = orc.sort_merge_join(rk, rp, sk, sp) if which.startswith("smj") else orc.hash_join(rk, rp, sk, sp)
     assert len(gp) == len(sk)
-    if which == "smj":   # fully ordered output: bit-exact including order
+    if which.startswith("smj"):   # fully ordered output: bit-exact including order
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
     else:
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj",
+                                   "smj_lookback"])
 @pytest.mark.parametrize("nb,ns,kr,ks", [(1, 1, 1, 1), (3000, 5000, 1000, 2000), (100, 70_000, 60, 200),
                                          (50_000, 3, 10, 10), (4097, 8193, 1 << 31, 1 << 31)])
 def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
@@ -66,13 +72,13 @@ def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
     sp = orc.gen_u32(ns, 4, ns)
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
     ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
-    if which == "smj":
+    if which.startswith("smj"):
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
     else:
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_global", "smj", "smj_lookback"])
 def test_join_unique_build_half_misses(ctx, orc, which):
     """Unique build keys (a permutation of [0, 2^18)) probed by keys uniform
     on [0, 2^19): about half the probe rows match nothing, so the direct
@@ -85,7 +91,7 @@ def test_join_unique_build_half_misses(ctx, orc, which):
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
     ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
     assert 0.4 * ns < len(ep) < 0.6 * ns
-    if which == "smj":
+    if which.startswith("smj"):
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
     else:
         assert same_multiset((gp, gb), (ep, eb))
@@ -136,7 +142,7 @@ def test_join_2p24_build_2p27_probe_properties(ctx, zipf, which):
     assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("which", ["hash", "smj"])
+@pytest.mark.parametrize("which", ["hash", "smj", "smj_lookback"])
 def test_join_probe_tile_spans_wide_build_range(ctx, orc, which):
     """Unique build keys, few probe rows spread over all of them: one sorted
     probe tile's build range exceeds the merge's LDS key cache."""
@@ -147,7 +153,7 @@ def test_join_probe_tile_spans_wide_build_range(ctx, orc, which):
     sp = orc.gen_u32(31, 6, ns)
     gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
     ep, eb = orc.sort_merge_join(rk, rp, sk, sp)
-    if which == "smj":
+    if which.startswith("smj"):
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
     else:
         assert same_multiset((gp, gb), (ep, eb))
