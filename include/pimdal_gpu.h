@@ -76,7 +76,11 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   sort_block (256, 512, 1024), sort_items (keys per thread: 16 or 8;
  *   1024-thread blocks take 8), sort_early_lookback (0, 1), sort_rank (0 LDS
  *   match masks, 1 per-bit ballots), sort_lbw (look-back statuses read per
- *   round trip: 4, 8, 16, 32), sort_scheme (0 chained passes with decoupled
+ *   round trip: 4, 8, 16, 32), sort_lb_split (1: all threads of a 512-thread
+ *   block take part in the look-back, two per digit), sort_nt (non-temporal
+ *   pass loads 1, stores 2, both 3), sort_xcd_tiles (S > 0: persistent
+ *   chained passes on a co-resident grid, each XCD taking runs of S
+ *   consecutive tiles; default 16; 0: one block per tile), sort_scheme (0 chained passes with decoupled
  *   look-back, 1 counted passes: per-tile digit counts and a scan, then a
  *   reorder with no look-back), sort_rts_order (tile order of counted
  *   passes: 0 block order, 1 ticket, 2 XCD-contiguous);

@@ -89,7 +89,7 @@ extern "C" int pimdal_autotune(pimdal_ctx* c, const char* op, uint64_t n, const 
         Scratch k(c, n * 4), v(c, n * 4), ko(c, n * 4), vo(c, n * 4);
         fill(c, k.get(), n, 1, 0);
         fill(c, v.get(), n, 2, 0);
-        knobs = {{"sort_block", {256, 512}}, {"sort_early_lookback", {0, 1}}};
+        knobs = {{"sort_xcd_tiles", {16, 8, 32, 0}}, {"sort_block", {512, 256}}, {"sort_early_lookback", {0, 1}}};
         tune(c, knobs, [&] { sort_pairs(c, k.as<uint32_t>(), v.get(), 4, n, ko.as<uint32_t>(), vo.get(), 0, 32, s); }, 3);
     } else if (o == "select") {
         Scratch k(c, n * 4), p(c, n * 4), op_(c, n * 4), orow(c, n * 4), cnt(c, 8);

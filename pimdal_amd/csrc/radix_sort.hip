@@ -295,8 +295,26 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_tscan(uint32_t* __restrict__ cnt
 // RTS (reduce-then-scan): tile = blockIdx.x, no status words and no
 // look-back: `offs` is the scanned per-tile counts (k_rs_tscan) and `base`
 // the digit starts, so no tile waits on another.
+// SPLIT: every thread of the block takes part in the look-back, BLOCK / 256
+// threads per digit, each reading LBW consecutive statuses of one window, so
+// one round trip covers BLOCK / 256 * LBW tiles; the parts meet in LDS after
+// a barrier.
+// NT: bit 0 = non-temporal input loads, bit 1 = non-temporal output stores
+// (the pass touches every key and payload once).
+// XP (XCD-persistent): a co-resident grid (cooperative launch) whose blocks
+// loop over tiles.  Block b is in class x = b % 8 (workgroups go to the 8
+// XCDs round-robin, so a class is one XCD's blocks); block i of the class's
+// B blocks takes the class's tickets i, i + B, i + 2B, ..., and ticket c is
+// tile (c / S) * 8S + x * S + c % S with S = xcd_tiles.  Each XCD thus
+// processes runs of S consecutive tiles, and the partial 64-byte lines where
+// neighbouring tiles' digit runs meet are merged in one L2 before they are
+// written back (runs of 16 tiles: 1.06 ms per 2^28-pair pass against 1.26
+// ms with tiles dealt to blocks in launch order).  Every tile waits only on
+// lower tiles.  The lowest unfinished tile belongs to one block, whose
+// earlier tickets are lower tiles and so finished; with all blocks resident
+// the pass always progresses.
 template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALLOT = false, int LBW = 4,
-          int ITEMS = RS_ITEMS, bool RTS = false>
+          int ITEMS = RS_ITEMS, bool RTS = false, bool SPLIT = false, int NT = 0, bool XP = false>
 __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
@@ -305,6 +323,8 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
     static_assert(!(RTS && (EARLY || BALLOT)), "the counted pass has no look-back");
+    static_assert(!SPLIT || (!RTS && !EARLY && BLOCK % RS_BINS == 0), "split look-back: after the reorder");
+    constexpr int PARTS = SPLIT ? BLOCK / RS_BINS : 1;
     __shared__ uint32_t s_tile;
     // keys reordered by digit; the per-wave match masks live here during ranking
     __shared__ __align__(16) uint32_t s_keys[TILE];
@@ -313,11 +333,34 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     __shared__ uint32_t s_off[RS_BINS];   // tile-local digit start
     __shared__ uint32_t s_gbase[RS_BINS]; // global position of the digit's run minus its local start
     __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
+    // split look-back exchange, double-buffered by round: part sums, part
+    // tiles consumed | kind << 6, and per wave whether any digit still walks
+    __shared__ uint32_t s_lsum[SPLIT ? 2 : 1][PARTS][SPLIT ? RS_BINS : 1];
+    __shared__ uint8_t s_lpc[SPLIT ? 2 : 1][PARTS][SPLIT ? RS_BINS : 1];
+    __shared__ uint32_t s_busy[2][WAVES];
     auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);
 
-    const uint32_t t = threadIdx.x, l = pd::lane();
+    static_assert(!XP || !RTS, "XCD-persistent passes are chained");
+    // XP: block i of class x's B blocks takes the class's tickets i, i + B,
+    // i + 2B, ... (no atomics); ticket c -> tile (c / S) * 8S + x * S + c % S
+    const uint32_t xp_x = blockIdx.x % 8, xp_s = xcd_tiles;
+    auto xp_tile = [&](uint32_t cc) -> uint32_t { return (cc / xp_s) * (8 * xp_s) + xp_x * xp_s + cc % xp_s; };
+    uint32_t xp_iter = 0;
+    for (;;) {   // one tile per iteration; one iteration unless XP
+    // thread and lane ids through opaque copies: values derived from them
+    // are recomputed per tile instead of being held across the loop
+    uint32_t t = threadIdx.x, l = pd::lane();
+    if constexpr (XP) {
+        asm volatile("" : "+v"(t));
+        asm volatile("" : "+v"(l));
+    }
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
-    if ((!RTS || tile_ctr) && t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    if (t == 0) {
+        if constexpr (XP)
+            s_tile = xp_tile(blockIdx.x / 8 + xp_iter * (gridDim.x / 8));
+        else if (!RTS || tile_ctr)
+            s_tile = atomicAdd(tile_ctr, 1u);
+    }
     for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
         s_match[w][i] = 0;
         s_cnt[w][i] = 0;
@@ -331,6 +374,7 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
                           : xcd_tiles ? (uint64_t)(blockIdx.x % 8) * xcd_tiles + blockIdx.x / 8
                                       : (uint64_t)blockIdx.x;
     if (RTS && tile * TILE >= n) return;   // XCD order: grid rounded up to a multiple of 8
+    if (XP && tile * TILE >= n) break;     // this counter's tiles are done (s_tile is block-wide)
     // RTS: this tile's run start of digit t, loaded now (used after the reorder)
     uint32_t rts_start = 0;
     if (RTS && t < RS_BINS) rts_start = base[t] + offs[tile * RS_BINS + t];
@@ -350,16 +394,16 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(kin + wave_base, wrows * 4u);
 #pragma unroll
     for (int i = 0; i < ITEMS; i++)
-        key[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + (uint32_t)i * PD_WAVE) * 4u, 0, 0);
+        key[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + (uint32_t)i * PD_WAVE) * 4u, 0, (NT & 1) ? 2 : 0);
     if constexpr (HAS_VALS) {
         const __amdgpu_buffer_rsrc_t rv = pd::buf_rsrc(vin + wave_base, wrows * (uint32_t)sizeof(V));
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             if constexpr (sizeof(V) == 4)
-                val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, (l + (uint32_t)i * PD_WAVE) * 4u, 0, 0);
+                val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, (l + (uint32_t)i * PD_WAVE) * 4u, 0, (NT & 1) ? 2 : 0);
             else
                 val[i] = __builtin_bit_cast(
-                    V, __builtin_amdgcn_raw_buffer_load_b64(rv, (l + (uint32_t)i * PD_WAVE) * 8u, 0, 0));
+                    V, __builtin_amdgcn_raw_buffer_load_b64(rv, (l + (uint32_t)i * PD_WAVE) * 8u, 0, (NT & 1) ? 2 : 0));
         }
     }
     // digit of item i, computed where it is used: the key goes through an
@@ -439,6 +483,67 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
         return excl;
     };
 
+    // split look-back, called by every thread: thread t reads statuses
+    // j - (t / 256) * LBW - q of digit t % 256.  A part ends at its first
+    // inclusive prefix (counted, kind 1) or first unpublished status (not
+    // counted, kind 2); the parts are combined in window order.  Every digit's
+    // threads take the same steps, and the loop ends for the whole block
+    // once no wave has a digit still walking.
+    auto lookback_split = [&](uint32_t cnt) -> uint32_t {
+        const uint32_t d = t % RS_BINS, h = t / RS_BINS;
+        uint32_t excl = 0;
+        int64_t j = (int64_t)tile - 1;
+        bool done = tile == 0;
+        for (uint32_t r = 0;; r++) {
+            const uint32_t b = r & 1;
+            uint32_t psum = 0, pc = 0;
+            if (!done) {
+                const int64_t j0 = j - (int64_t)h * LBW;
+                uint32_t st[LBW];
+#pragma unroll
+                for (int q = 0; q < LBW; q++)
+                    st[q] = (j0 - q >= 0) ? pd::ld_relaxed(&status[(uint64_t)(j0 - q) * RS_BINS + d]) : RS_FLAG_PRE;
+                uint32_t kind = 0, used = 0;
+#pragma unroll
+                for (int q = 0; q < LBW; q++) {
+                    if (kind) continue;
+                    const uint32_t f = st[q] >> 30;
+                    if (f == 0) { kind = 2; continue; }
+                    psum += st[q] & RS_VALUE;
+                    used++;
+                    if (f == 2) kind = 1;
+                }
+                pc = used | (kind << 6);
+            }
+            s_lsum[b][h][d] = psum;
+            s_lpc[b][h][d] = (uint8_t)pc;
+            const uint64_t busy = pd::ballot(!done);
+            if (l == 0) s_busy[b][w] = busy != 0;
+            __syncthreads();
+            uint32_t any = 0;
+#pragma unroll
+            for (int ww = 0; ww < WAVES; ww++) any |= s_busy[b][ww];
+            if (!any) break;
+            if (!done) {
+                uint32_t sum = 0, used = 0, kind = 0;
+#pragma unroll
+                for (int p = 0; p < PARTS; p++) {
+                    if (kind) break;
+                    const uint32_t pcp = s_lpc[b][p][d];
+                    sum += s_lsum[b][p][d];
+                    used += pcp & 63u;
+                    kind = pcp >> 6;
+                }
+                excl += sum;
+                j -= used;
+                if (kind == 1) done = true;
+                else if (kind == 2) __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (h == 0 && tile > 0) pd::st_relaxed(&status[tile * RS_BINS + d], RS_FLAG_PRE | (excl + cnt));
+        return excl;
+    };
+
     // ---- per digit: prefix over waves, tile count, publish the aggregate ----
     uint32_t cnt_d = 0, inc_d = 0, excl = 0;
     if (t < RS_BINS) {
@@ -475,7 +580,10 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
         if constexpr (HAS_VALS) s_vals[p] = val[i];
     }
     if (RTS && t < RS_BINS) s_gbase[t] = rts_start - local_d;
-    if (!RTS && !EARLY && t < RS_BINS) {
+    if constexpr (SPLIT) {
+        excl = lookback_split(cnt_d);
+        if (t < RS_BINS) s_gbase[t] = offs[t] + excl - local_d;
+    } else if (!RTS && !EARLY && t < RS_BINS) {
         if (tile > 0) excl = lookback(cnt_d);
         s_gbase[t] = offs[t] + excl - local_d;
     }
@@ -489,9 +597,18 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
         const uint32_t kk = s_keys[j];
         const uint64_t dst = (uint64_t)s_gbase[(rs_src<HASH>(kk) >> shift) & mask] + j;
         if (dst < n) {
-            kout[dst] = kk;
-            if constexpr (HAS_VALS) vout[dst] = s_vals[j];
+            if constexpr ((NT & 2) != 0) {
+                __builtin_nontemporal_store(kk, kout + dst);
+                if constexpr (HAS_VALS) __builtin_nontemporal_store(s_vals[j], vout + dst);
+            } else {
+                kout[dst] = kk;
+                if constexpr (HAS_VALS) vout[dst] = s_vals[j];
+            }
         }
+    }
+    if constexpr (!XP) break;
+    xp_iter++;
+    __syncthreads();   // the next tile reuses the LDS buffers and s_tile
     }
 
 }
@@ -513,11 +630,28 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     const bool early = c->tune_get("sort_early_lookback", 0) != 0;
     const bool ballot = c->tune_get("sort_rank", 0) == 1;
     const int lbw = sort_lbw(c);
+    // knob sort_lb_split: 1 = two threads per digit in the look-back
+    const bool split = c->tune_get("sort_lb_split", 0) == 1;
+    // knob sort_nt: non-temporal loads (1), stores (2) or both (3)
+    const int nt = (int)(c->tune_get("sort_nt", 0) & 3);
     const uint64_t ntiles = ceil_div(n, (uint64_t)block * items);
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
     Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
     Scratch st(c, ntiles * RS_BINS * 4, s);
+    // knob sort_xcd_tiles S > 0 (default 16): XCD-persistent passes with
+    // runs of S tiles per XCD (512 x 16 shape, LDS ranking, look-back after
+    // the reorder); 0 = one block per tile in launch order
+    const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
+    const bool xp_shape = block == 512 && items == 16 && !early && !ballot && !split && nt == 0;
+    auto xs_of = [&](uint32_t) -> uint32_t { return xp_shape && xs_knob > 0 ? (uint32_t)xs_knob : 0u; };
+    bool xp = false;
+    for (uint32_t j = 0; j < pdsc.npass; j++) xp |= xs_of(j) > 0;
+    uint32_t xp_grid = 0;
+    if (xp) {
+        const uint32_t per_cu = blocks_per_cu(k_rs_onesweep<V, HV, 512, false, HASH, false, 4, 16, false, false, 0, true>, 512);
+        xp_grid = std::max<uint32_t>(8, (per_cu * c->cu_count) / 8 * 8);
+    }
     const uint32_t* ksrc = kin;
     const V* vsrc = vin;
     const char* name = HASH ? "part_onesweep" : "radix_onesweep";
@@ -529,9 +663,21 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
 #define PD_RS_LAUNCH(...)                                                                                         \
     launch(c, name, k_rs_onesweep<__VA_ARGS__>, dim3((uint32_t)ntiles), dim3(block), 0, s, ksrc, vsrc, kdst, vdst, n, \
            pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j, (const uint32_t*)nullptr, 0u)
-        if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
+        if (xs_of(j) > 0)
+            launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, 4, 16, false, false, 0, true>,
+                        dim3(xp_grid), dim3(512), 0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j],
+                        offs + j * RS_BINS, st.as<uint32_t>(), (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                        xs_of(j));
+        else if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
         else if (items == 8 && block == 512) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 8);
-        else if (block == 512 && !early && !ballot) {
+        else if (block == 512 && !early && !ballot && split) {
+            if (lbw >= 8) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 8, 16, false, true);
+            else PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 16, false, true);
+        } else if (block == 512 && !early && !ballot && nt) {
+            if (nt == 1) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 16, false, false, 1);
+            else if (nt == 2) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 16, false, false, 2);
+            else PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 16, false, false, 3);
+        } else if (block == 512 && !early && !ballot) {
             if (lbw == 32) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 32);
             else if (lbw == 16) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 16);
             else if (lbw == 8) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 8);

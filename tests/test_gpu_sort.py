@@ -146,6 +146,14 @@ def test_sort_ballot_rank_u64_payload_and_hash_join(ctx, orc, ballot_rank):
 PASS_VARIANTS = {
     "window8": {"sort_lbw": 8},
     "window32": {"sort_lbw": 32},
+    "split_lookback": {"sort_lb_split": 1},
+    "split_lookback_window8": {"sort_lb_split": 1, "sort_lbw": 8},
+    "nontemporal_loads": {"sort_nt": 1},
+    "nontemporal_loads_stores": {"sort_nt": 3},
+    "launch_order_tiles": {"sort_xcd_tiles": 0},
+    "xcd_persistent_runs1": {"sort_xcd_tiles": 1},
+    "xcd_persistent_runs5": {"sort_xcd_tiles": 5},
+    "xcd_persistent_runs16_all_digit_widths": {"sort_xcd_tiles": 16},
     "block1024_items8": {"sort_block": 1024},
     "block512_items8": {"sort_items": 8},
     "block256_early": {"sort_block": 256, "sort_early_lookback": 1},
