@@ -76,7 +76,7 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   sort_block (256, 512, 1024), sort_items (keys per thread: 16 or 8;
  *   1024-thread blocks take 8), sort_early_lookback (0, 1), sort_rank (0 LDS
  *   match masks, 1 per-bit ballots), sort_lbw (look-back statuses read per
- *   round trip: 4, 8, 16, 32), sort_lb_split (1: all threads of a 512-thread
+ *   round trip: 1, 2, 4, 8, 16, 32), sort_lb_split (1: all threads of a 512-thread
  *   block take part in the look-back, two per digit), sort_nt (non-temporal
  *   pass loads 1, stores 2, both 3), sort_xcd_tiles (S > 0: persistent
  *   chained passes on a co-resident grid, each XCD taking runs of S

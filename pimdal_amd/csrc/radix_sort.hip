@@ -287,11 +287,10 @@ __global__ __launch_bounds__(RS_BINS) void k_rs_tscan(uint32_t* __restrict__ cnt
 // offsets (look-back) before the LDS reorder instead of after it.
 // BALLOT: rank with per-bit ballots instead of LDS match masks.
 // LBW: statuses each digit's thread reads per look-back round trip.  A
-// status written on another XCD is read from memory (the L2s are per XCD),
-// so one round trip costs about a microsecond; with tiles finishing at tens
-// per microsecond, a tile typically walks back over dozens of published
-// aggregates before it reaches an inclusive prefix, and a wide window
-// covers them in one or two round trips.
+// round trip costs about a microsecond and a tile walks back over a few
+// dozen published aggregates, yet wide windows are slower: the cost of a
+// round grows with the statuses it reads, and the status reads of all tiles
+// in flight compete with the pass's own traffic.
 // RTS (reduce-then-scan): tile = blockIdx.x, no status words and no
 // look-back: `offs` is the scanned per-tile counts (k_rs_tscan) and `base`
 // the digit starts, so no tile waits on another.
@@ -613,10 +612,12 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
 
 }
 
-// Look-back window of the passes (knob sort_lbw: 4, 8, 16 or 32).
+// Look-back window of the passes (knob sort_lbw: 1, 2, 4, 8, 16 or 32;
+// default 2: wider windows read statuses the walk does not need, and the
+// status reads of ~500 tiles in flight compete with the pass's own traffic).
 static int sort_lbw(pimdal_ctx* c) {
-    const int64_t v = c->tune_get("sort_lbw", 4);
-    return v >= 32 ? 32 : v >= 16 ? 16 : v >= 8 ? 8 : 4;
+    const int64_t v = c->tune_get("sort_lbw", 2);
+    return v >= 32 ? 32 : v >= 16 ? 16 : v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1;
 }
 
 template <typename V, bool HV, bool HASH = false>
@@ -645,13 +646,10 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
     const bool xp_shape = block == 512 && items == 16 && !early && !ballot && !split && nt == 0;
     auto xs_of = [&](uint32_t) -> uint32_t { return xp_shape && xs_knob > 0 ? (uint32_t)xs_knob : 0u; };
-    bool xp = false;
-    for (uint32_t j = 0; j < pdsc.npass; j++) xp |= xs_of(j) > 0;
-    uint32_t xp_grid = 0;
-    if (xp) {
-        const uint32_t per_cu = blocks_per_cu(k_rs_onesweep<V, HV, 512, false, HASH, false, 4, 16, false, false, 0, true>, 512);
-        xp_grid = std::max<uint32_t>(8, (per_cu * c->cu_count) / 8 * 8);
-    }
+    // co-resident grid of an XCD-persistent pass: a multiple of 8 blocks
+    auto xp_grid_of = [&](auto kernel) -> uint32_t {
+        return std::max<uint32_t>(8, (blocks_per_cu(kernel, 512) * c->cu_count) / 8 * 8);
+    };
     const uint32_t* ksrc = kin;
     const V* vsrc = vin;
     const char* name = HASH ? "part_onesweep" : "radix_onesweep";
@@ -664,10 +662,18 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     launch(c, name, k_rs_onesweep<__VA_ARGS__>, dim3((uint32_t)ntiles), dim3(block), 0, s, ksrc, vsrc, kdst, vdst, n, \
            pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j, (const uint32_t*)nullptr, 0u)
         if (xs_of(j) > 0)
-            launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, 4, 16, false, false, 0, true>,
-                        dim3(xp_grid), dim3(512), 0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j],
-                        offs + j * RS_BINS, st.as<uint32_t>(), (uint32_t*)nullptr, (const uint32_t*)nullptr,
-                        xs_of(j));
+        {
+#define PD_RS_XP(W)                                                                                               \
+    launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>,                \
+                dim3(xp_grid_of(k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>)), dim3(512),  \
+                0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(),   \
+                (uint32_t*)nullptr, (const uint32_t*)nullptr, xs_of(j))
+            if (lbw == 2) PD_RS_XP(2);
+            else if (lbw == 1) PD_RS_XP(1);
+            else if (lbw == 8) PD_RS_XP(8);
+            else PD_RS_XP(4);
+#undef PD_RS_XP
+        }
         else if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
         else if (items == 8 && block == 512) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 8);
         else if (block == 512 && !early && !ballot && split) {
@@ -678,7 +684,8 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
             else if (nt == 2) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 16, false, false, 2);
             else PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 16, false, false, 3);
         } else if (block == 512 && !early && !ballot) {
-            if (lbw == 32) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 32);
+            if (lbw == 2) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 2);
+            else if (lbw == 32) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 32);
             else if (lbw == 16) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 16);
             else if (lbw == 8) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 8);
             else PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4);

@@ -144,6 +144,8 @@ def test_sort_ballot_rank_u64_payload_and_hash_join(ctx, orc, ballot_rank):
 
 
 PASS_VARIANTS = {
+    "window1": {"sort_lbw": 1},
+    "window4": {"sort_lbw": 4},
     "window8": {"sort_lbw": 8},
     "window32": {"sort_lbw": 32},
     "split_lookback": {"sort_lb_split": 1},
