@@ -114,7 +114,7 @@ KERNELS = ("part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan"
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
-           "select", "select_count", "select_write", "q1_pack", "fill_u32", "fill_u64")
+           "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64")
 
 
 def kernel_breakdown(ctx, steps):
@@ -167,6 +167,24 @@ def bench_q1(ctx, steps, warmup, by_key, n=60_000_000):
     else:
         alg = 38 * n
     res = {"rows": n, "groups": groups, "s_per_step": tm, "rows_per_s": n / tm, "alg_bytes": alg,
+           "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
+    for a in t.values():
+        a.free()
+    return res
+
+
+def bench_q6(ctx, steps, warmup, n=60_000_000):
+    """TPC-H Q6 shape over the 6e7-row fact table: one year of dates, discount 5..7, quantity < 24."""
+    from pimdal_amd import operators as ops, datagen
+    t = datagen.fact_table(ctx, n, groups=15_000_000)
+    out = {}
+
+    def step():
+        out["r"] = ops.q6(ctx, t)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    alg = 28 * n   # date 4 + quantity 8 + price 8 + discount 8 bytes per row
+    res = {"rows": n, "rows_kept": out["r"][1], "s_per_step": tm, "rows_per_s": n / tm, "alg_bytes": alg,
            "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
     for a in t.values():
         a.free()
@@ -267,6 +285,8 @@ def main():
         for by_key in (False, True):
             log(f"bench: filter+group-by 6e7 by_key={by_key}")
             ops_out["q1_orderkey" if by_key else "q1_flags"] = bench_q1(ctx, max(3, args.steps // 2), 1, by_key)
+        log("bench: TPC-H Q6 shape 6e7")
+        ops_out["q6"] = bench_q6(ctx, max(3, args.steps // 2), 1)
     if args.only in ("", "all", "serve"):
         log("bench: serving mixed stream")
         ops_out["serve_mixed_stream"] = bench_serve(ctx)

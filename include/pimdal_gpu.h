@@ -236,6 +236,19 @@ int pimdal_filter_groupby_q1_dev(pimdal_ctx* ctx, const int32_t* date, const uin
                                  const int64_t* discount, const int64_t* tax, uint64_t n, int32_t cutoff,
                                  const pimdal_q1_result* out, uint64_t capacity, uint64_t* n_groups);
 
+/* TPC-H Q6 shape: pimdal/main/tpc_h/query6/dpu/kernel_q6.c:91-141 (the
+ * predicates pred_date, pred_quantity and pred_discount applied in turn,
+ * then price * discount and an int64 sum) with the parameters of
+ * host_q6.cpp:46-49 (one year of dates, discount 6 +- 1, quantity < 24)
+ * and the sum over devices of host_q6.cpp:60-66.  Keeps rows with
+ * date_lo <= date < date_hi, quantity < qty_lt and disc_lo <= discount <=
+ * disc_hi; revenue = sum of price * discount over them, int64 modulo 2^64
+ * like the reference's int64 accumulators (the reference prints it divided
+ * by 10^4).  n_rows = rows kept.  Synchronises the stream. */
+int pimdal_q6_dev(pimdal_ctx* ctx, const int32_t* date, const int64_t* quantity, const int64_t* price,
+                  const int64_t* discount, uint64_t n, int32_t date_lo, int32_t date_hi, int64_t disc_lo,
+                  int64_t disc_hi, int64_t qty_lt, int64_t* revenue, uint64_t* n_rows);
+
 /* ---------------- serving ----------------
  * No counterpart in the reference (its drivers run one operator per
  * binary, pimdal/main/<operator>/host): a request loop over tables kept resident in

@@ -529,6 +529,37 @@ extern "C" uint64_t orc_q1_key(const int32_t* date, const int32_t* key, const in
 }
 
 /* ------------------------------------------------------------------ */
+/* TPC-H Q6 shape:                                                     */
+/*   pimdal/main/tpc_h/query6/dpu/kernel_q6.c:91-101 (pred_date:       */
+/*   start <= date < end; pred_discount: D-1 <= disc <= D+1;           */
+/*   pred_quantity: qty < Q), 113-141 (the three selections in turn,   */
+/*   then price * discount and an int64 sum) and host_q6.cpp:46-49     */
+/*   (D = 6, Q = 24, one year of dates), 60-66 (sum over devices),     */
+/*   69-96 (the Acero plan: the same filters, sum(discount * price)).  */
+/* Sums are int64 modulo 2^64, as the reference's int64 accumulators.  */
+/* Returns the number of rows kept.                                    */
+/* ------------------------------------------------------------------ */
+extern "C" uint64_t orc_q6(const int32_t* date, const int64_t* qty, const int64_t* price, const int64_t* disc,
+                           uint64_t n, int32_t date_lo, int32_t date_hi, int64_t disc_lo, int64_t disc_hi,
+                           int64_t qty_lt, int64_t* revenue) {
+    uint64_t rev = 0, cnt = 0;
+    const uint64_t nb = nblocks(n);
+    #pragma omp parallel for schedule(static) reduction(+ : rev, cnt)
+    for (uint64_t b = 0; b < nb; b++) {
+        const uint64_t e = std::min(n, (b + 1) * kBlock);
+        for (uint64_t i = b * kBlock; i < e; i++) {
+            if (!(date[i] >= date_lo && date[i] < date_hi)) continue;
+            if (!(qty[i] < qty_lt)) continue;
+            if (!(disc[i] >= disc_lo && disc[i] <= disc_hi)) continue;
+            rev += (uint64_t)disc[i] * (uint64_t)price[i];
+            cnt += 1;
+        }
+    }
+    *revenue = (int64_t)rev;
+    return cnt;
+}
+
+/* ------------------------------------------------------------------ */
 /* Sort-based aggregation: pimdal/aggregate/aggregate.c:31-158 (run-  */
 /* length reduce of key-sorted input).  Sort by key (stable), then one*/
 /* (key, sum, count) per run, ascending key.                          */

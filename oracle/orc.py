@@ -40,6 +40,7 @@ _SIG = {
     "orc_sort_aggr_u32_i64": (u64, [vp, vp, u64, vp, vp, vp, u64]),
     "orc_q1_flags": (u64, [vp, vp, vp, vp, vp, vp, vp, u64, i32, u64, vp, vp, vp, vp, vp, vp, vp]),
     "orc_q1_key": (u64, [vp, vp, vp, vp, vp, vp, u64, i32, u64, vp, vp, vp, vp, vp, vp, vp]),
+    "orc_q6": (u64, [vp, vp, vp, vp, u64, i32, i32, i64, i64, i64, vp]),
     "orc_num_threads": (C.c_int, []),
     "orc_set_threads": (None, [C.c_int]),
 }
@@ -247,6 +248,20 @@ def q1_key(t, cutoff):
                          _p(c["tax"]), n, cutoff, max(n, 1), _p(o["key"]), _p(o["sum_qty"]), _p(o["sum_price"]),
                          _p(o["sum_disc"]), _p(o["sum_disc_price"]), _p(o["sum_charge"]), _p(o["count"]))
     return _q1_trim(o, g)
+
+
+# TPC-H Q6 defaults on the fact table's date domain (days from 1992-01-01):
+# one year from 1994-01-01, discount 6 +- 1 (hundredths), quantity < 24
+Q6_DEFAULT = {"date_lo": 731, "date_hi": 1096, "disc_lo": 5, "disc_hi": 7, "qty_lt": 24}
+
+
+def q6(t, date_lo=731, date_hi=1096, disc_lo=5, disc_hi=7, qty_lt=24):
+    """(revenue, rows kept): revenue = sum(price * discount) over the kept rows, int64 modulo 2^64."""
+    c = {k: np.ascontiguousarray(t[k]) for k in ("date", "quantity", "price", "discount")}
+    rev = C.c_int64()
+    cnt = lib().orc_q6(_p(c["date"]), _p(c["quantity"]), _p(c["price"]), _p(c["discount"]), c["date"].size,
+                       date_lo, date_hi, disc_lo, disc_hi, qty_lt, C.byref(rev))
+    return rev.value, cnt
 
 
 def num_threads() -> int:

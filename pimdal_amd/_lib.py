@@ -81,6 +81,8 @@ SIGNATURES: dict[str, list] = {
     "pimdal_hash_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_sort_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_filter_groupby_q1_dev": [c_ctx_p, vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, u64, C.POINTER(u64)],
+    "pimdal_q6_dev": [c_ctx_p, vp, vp, vp, vp, u64, i32, i32, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_int64),
+                      C.POINTER(u64)],
     "pimdal_server_create": [c_ctx_p, vp, C.c_int, u64, C.POINTER(vp)],
     "pimdal_server_run": [vp, vp, u64, C.c_int, vp, vp],
     "pimdal_server_destroy": [vp],

@@ -18,6 +18,8 @@
 //  k_q1_dense<G>    filter + Q1 measures accumulated in registers for G <= 8
 //                   dense groups (no atomics), one partial row per block;
 //  k_q1_final       deterministic 128-bit reduction of the block partials;
+//  k_q6 / k_q6_final   the TPC-H Q6 filter chain and revenue sum in one
+//                   streaming pass (pimdal/main/tpc_h/query6/dpu/kernel_q6.c);
 //  k_gb_hist / k_gb_scatter   radix partition of the filtered, projected
 //                   rows by the reference hash of the group key;
 //  k_gb_lds<Agg>    one partition per block: LDS open-addressing table
@@ -982,6 +984,107 @@ __global__ void k_q1_unpack(uint64_t g, const uint32_t* __restrict__ key, const 
     }
 }
 
+// TPC-H Q6 shape (pimdal/main/tpc_h/query6/dpu/kernel_q6.c:91-141): rows
+// with date_lo <= date < date_hi, quantity < qty_lt and disc_lo <= discount
+// <= disc_hi add price * discount to the revenue.  The reference selects in
+// three passes with row-id gathers; here one streaming pass reads the four
+// columns (28 bytes per row, four rows per thread with 16-byte loads) and
+// keeps the sums in registers.  Sums are modulo 2^64 like the reference's
+// int64 accumulators.  Per-block partials: [block][0] revenue, [1] rows.
+struct Q6Args {
+    int32_t date_lo, date_hi;
+    int64_t disc_lo, disc_hi, qty_lt;
+};
+constexpr int Q6_BLOCK = 256;
+template <bool VEC>
+__global__ __launch_bounds__(Q6_BLOCK) void k_q6(const int32_t* __restrict__ date, const int64_t* __restrict__ qty,
+                                                 const int64_t* __restrict__ price, const int64_t* __restrict__ disc,
+                                                 uint64_t n, Q6Args a, uint64_t* __restrict__ partials) {
+    __shared__ uint64_t s_red[Q6_BLOCK / PD_WAVE][2];
+    uint64_t rev = 0, cnt = 0;
+    auto row = [&](int32_t dt, int64_t q, int64_t p, int64_t d) {
+        const bool keep = dt >= a.date_lo && dt < a.date_hi && q < a.qty_lt && d >= a.disc_lo && d <= a.disc_hi;
+        rev += keep ? (uint64_t)p * (uint64_t)d : 0;
+        cnt += keep ? 1 : 0;
+    };
+    const uint64_t stride = (uint64_t)gridDim.x * Q6_BLOCK;
+    if (VEC) {
+        const uint64_t nq = n / 4;
+        for (uint64_t v = blockIdx.x * (uint64_t)Q6_BLOCK + threadIdx.x; v < nq; v += stride) {
+            const int4 dt = reinterpret_cast<const int4*>(date)[v];
+            const longlong2 q0 = reinterpret_cast<const longlong2*>(qty)[2 * v], q1 = reinterpret_cast<const longlong2*>(qty)[2 * v + 1];
+            const longlong2 p0 = reinterpret_cast<const longlong2*>(price)[2 * v], p1 = reinterpret_cast<const longlong2*>(price)[2 * v + 1];
+            const longlong2 d0 = reinterpret_cast<const longlong2*>(disc)[2 * v], d1 = reinterpret_cast<const longlong2*>(disc)[2 * v + 1];
+            row(dt.x, q0.x, p0.x, d0.x);
+            row(dt.y, q0.y, p0.y, d0.y);
+            row(dt.z, q1.x, p1.x, d1.x);
+            row(dt.w, q1.y, p1.y, d1.y);
+        }
+        for (uint64_t i = nq * 4 + blockIdx.x * (uint64_t)Q6_BLOCK + threadIdx.x; i < n; i += stride)
+            row(date[i], qty[i], price[i], disc[i]);
+    } else {
+        for (uint64_t i = blockIdx.x * (uint64_t)Q6_BLOCK + threadIdx.x; i < n; i += stride)
+            row(date[i], qty[i], price[i], disc[i]);
+    }
+    rev = pd::wave_sum64(rev);
+    cnt = pd::wave_sum64(cnt);
+    const uint32_t w = threadIdx.x / PD_WAVE;
+    if (pd::lane() == 0) {
+        s_red[w][0] = rev;
+        s_red[w][1] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint64_t v = 0;
+        for (int ww = 0; ww < Q6_BLOCK / PD_WAVE; ww++) v += s_red[ww][threadIdx.x];
+        partials[2 * (uint64_t)blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+// Sums the block partials (one block; modulo 2^64): out[0] revenue, out[1] rows.
+__global__ __launch_bounds__(256) void k_q6_final(const uint64_t* __restrict__ partials, uint32_t nblocks,
+                                                  uint64_t* __restrict__ out) {
+    __shared__ uint64_t s_red[256 / PD_WAVE][2];
+    uint64_t rev = 0, cnt = 0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+        rev += partials[2 * (uint64_t)b];
+        cnt += partials[2 * (uint64_t)b + 1];
+    }
+    rev = pd::wave_sum64(rev);
+    cnt = pd::wave_sum64(cnt);
+    if (pd::lane() == 0) {
+        s_red[threadIdx.x / PD_WAVE][0] = rev;
+        s_red[threadIdx.x / PD_WAVE][1] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint64_t v = 0;
+        for (int ww = 0; ww < 256 / PD_WAVE; ww++) v += s_red[ww][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+}
+
+// Revenue and kept-row count of the Q6 filter chain; synchronises the stream.
+uint64_t q6(pimdal_ctx* c, const int32_t* date, const int64_t* qty, const int64_t* price, const int64_t* disc,
+            uint64_t n, const Q6Args& a, int64_t* revenue, hipStream_t s) {
+    PD_REQUIRE((date && qty && price && disc) || n == 0, PIMDAL_E_INVALID, "missing input column");
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, Q6_BLOCK * 4 * 4),
+                                                                           (uint64_t)c->cu_count * 8));
+    Scratch part(c, (uint64_t)nb * 16, s), res(c, 16, s);
+    const bool vec = ((uintptr_t)date | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc) % 16 == 0;
+    if (vec)
+        launch(c, "q6", k_q6<true>, dim3(nb), dim3(Q6_BLOCK), 0, s, date, qty, price, disc, n, a, part.as<uint64_t>());
+    else
+        launch(c, "q6", k_q6<false>, dim3(nb), dim3(Q6_BLOCK), 0, s, date, qty, price, disc, n, a, part.as<uint64_t>());
+    launch(c, "q6_final", k_q6_final, dim3(1), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb,
+           res.as<uint64_t>());
+    uint64_t h[2];
+    PD_HIP(hipMemcpyAsync(h, res.get(), 16, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    *revenue = (int64_t)h[0];
+    return h[1];
+}
+
 uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1, const uint8_t* f2,
                            const int32_t* key, const int64_t* qty, const int64_t* price, const int64_t* disc,
                            const int64_t* tax, uint64_t n, int32_t cutoff, const pimdal_q1_result& out,
@@ -1250,6 +1353,16 @@ extern "C" int pimdal_filter_groupby_q1_dev(pimdal_ctx* c, const int32_t* date, 
     PD_REQUIRE(c && out && n_groups, PIMDAL_E_INVALID, "NULL argument");
     *n_groups = filter_groupby_q1(c, date, flag1, flag2, key, quantity, price, discount, tax, n, cutoff, *out,
                                   capacity, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_q6_dev(pimdal_ctx* c, const int32_t* date, const int64_t* quantity, const int64_t* price,
+                             const int64_t* discount, uint64_t n, int32_t date_lo, int32_t date_hi, int64_t disc_lo,
+                             int64_t disc_hi, int64_t qty_lt, int64_t* revenue, uint64_t* n_rows) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && revenue && n_rows, PIMDAL_E_INVALID, "NULL argument");
+    const Q6Args a{date_lo, date_hi, disc_lo, disc_hi, qty_lt};
+    *n_rows = q6(c, date, quantity, price, discount, n, a, revenue, c->stream);
     PD_API_END
 }
 

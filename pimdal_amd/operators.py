@@ -112,6 +112,21 @@ def sort_aggr(ctx: Context, key, val, capacity: int | None = None):
     return _aggr("pimdal_sort_aggr_dev", ctx, key, val, capacity)
 
 
+def q6(ctx: Context, t: dict, date_lo: int = 731, date_hi: int = 1096, disc_lo: int = 5, disc_hi: int = 7,
+       qty_lt: int = 24):
+    """TPC-H Q6 shape: (revenue, rows kept), revenue = sum(price * discount)
+    over rows with date_lo <= date < date_hi, quantity < qty_lt and
+    disc_lo <= discount <= disc_hi (int64 modulo 2^64).
+    pimdal/main/tpc_h/query6/dpu/kernel_q6.c:91-141, host_q6.cpp:46-49.
+    """
+    rev = C.c_int64()
+    cnt = C.c_uint64()
+    call("pimdal_q6_dev", ctx.handle, t["date"].vp, t["quantity"].vp, t["price"].vp, t["discount"].vp,
+         t["date"].n, int(date_lo), int(date_hi), int(disc_lo), int(disc_hi), int(qty_lt), C.byref(rev),
+         C.byref(cnt))
+    return rev.value, cnt.value
+
+
 class _Q1Result(C.Structure):
     _fields_ = [(f, C.c_void_p) for f in ("key", "sum_qty", "sum_price", "sum_disc", "sum_disc_price",
                                           "sum_charge", "count")]

@@ -205,3 +205,46 @@ def test_q1_by_orderkey_2048_slot_tables(ctx, orc, n, groups):
     finally:
         ctx.set_tune("gb_q1_slots", 1024)
     _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
+# ---------------- TPC-H Q6 shape ----------------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 3, 4095, 1 << 20, (1 << 20) + 3])
+def test_q6_matches_oracle(ctx, orc, n):
+    t = orc.fact_table(n, seed=90 + n)
+    d = {c: ctx.to_device(t[c]) for c in ("date", "quantity", "price", "discount")}
+    assert ops.q6(ctx, d) == orc.q6(t)
+    # other parameters: two years, discount 0..3, quantity < 50
+    assert ops.q6(ctx, d, 365, 1096, 0, 3, 50) == orc.q6(t, 365, 1096, 0, 3, 50)
+
+
+@pytest.mark.gpu
+def test_q6_bounds_and_wrapping_sums(ctx, orc):
+    date = np.array([730, 731, 1095, 1096, 900, 900, 900, 900, 900, 900] * 7, np.int32)
+    disc = np.array([6, 6, 6, 6, 4, 5, 7, 8, 6, 6] * 7, np.int64)
+    qty = np.array([10, 10, 10, 10, 10, 10, 10, 10, 23, 24] * 7, np.int64)
+    price = np.full(70, (1 << 62) - 5, np.int64)
+    t = {"date": date, "quantity": qty, "price": price, "discount": disc}
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    assert ops.q6(ctx, d) == orc.q6(t)
+    assert ops.q6(ctx, d)[1] == 35
+
+
+@pytest.mark.gpu
+def test_q6_unaligned_columns(ctx, orc):
+    # columns starting 4 and 8 bytes past an aligned base take the scalar loads
+    n = 100_003
+    t = orc.fact_table(n + 1, seed=5)
+    d = {c: ctx.to_device(t[c]) for c in ("date", "quantity", "price", "discount")}
+    v = {c: d[c].view(1, n) for c in d}
+    sub = {c: t[c][1:] for c in d}
+    assert ops.q6(ctx, v) == orc.q6(sub)
+
+
+@pytest.mark.gpu
+def test_q6_6e7_rows_full_size(ctx, orc):
+    n = 60_000_000
+    d = datagen.fact_table(ctx, n)
+    t = {c: d[c].numpy() for c in ("date", "quantity", "price", "discount")}
+    assert ops.q6(ctx, d) == orc.q6(t)

@@ -219,3 +219,46 @@ def test_select_int64_keys(orc):
     op, orow = orc.select(k, p, -5, 1 << 39)
     idx = [i for i in range(k.size) if -5 <= int(k[i]) < (1 << 39)]
     assert list(orow) == idx and np.array_equal(op, p[idx])
+
+
+# ---------------- TPC-H Q6 shape ----------------
+
+def _q6_brute(t, lo, hi, dlo, dhi, qlt):
+    rev, cnt = 0, 0
+    for i in range(t["date"].size):
+        if lo <= int(t["date"][i]) < hi and int(t["quantity"][i]) < qlt and dlo <= int(t["discount"][i]) <= dhi:
+            rev += int(t["price"][i]) * int(t["discount"][i])
+            cnt += 1
+    rev %= 1 << 64
+    return rev - (1 << 64) if rev >= 1 << 63 else rev, cnt
+
+
+@pytest.mark.parametrize("n", [0, 1, 7999, 8001, 30_011])
+def test_q6_matches_python_integers(orc, n):
+    t = orc.fact_table(n, seed=71 + n)
+    assert orc.q6(t) == _q6_brute(t, 731, 1096, 5, 7, 24)
+
+
+def test_q6_bounds_are_half_open_date_inclusive_discount_strict_quantity(orc):
+    # one row on each side of every bound: only the rows inside all three count
+    date = np.array([730, 731, 1095, 1096, 900, 900, 900, 900, 900, 900], np.int32)
+    disc = np.array([6, 6, 6, 6, 4, 5, 7, 8, 6, 6], np.int64)
+    qty = np.array([10, 10, 10, 10, 10, 10, 10, 10, 23, 24], np.int64)
+    price = np.arange(1, 11, dtype=np.int64) * 1000
+    t = {"date": date, "quantity": qty, "price": price, "discount": disc}
+    keep = [1, 2, 5, 6, 8]
+    assert orc.q6(t) == (sum(int(price[i]) * int(disc[i]) for i in keep), len(keep))
+
+
+def test_q6_revenue_wraps_like_int64(orc):
+    n = 64
+    t = {"date": np.full(n, 800, np.int32), "quantity": np.ones(n, np.int64),
+         "price": np.full(n, (1 << 62) - 5, np.int64), "discount": np.full(n, 7, np.int64)}
+    assert orc.q6(t) == _q6_brute(t, 731, 1096, 5, 7, 24)
+
+
+def test_q6_matches_numpy_at_config_size_sample(orc):
+    t = orc.fact_table(2_000_000)
+    m = ((t["date"] >= 731) & (t["date"] < 1096) & (t["quantity"] < 24) & (t["discount"] >= 5) & (t["discount"] <= 7))
+    rev = int((t["price"][m].astype(np.uint64) * t["discount"][m].astype(np.uint64)).sum(dtype=np.uint64))
+    assert orc.q6(t) == (rev - (1 << 64) if rev >= 1 << 63 else rev, int(m.sum()))
