@@ -317,7 +317,8 @@ template <typename V, bool HAS_VALS, int BLOCK, bool EARLY, bool HASH, bool BALL
 __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     const uint32_t* __restrict__ kin, const V* __restrict__ vin, uint32_t* __restrict__ kout, V* __restrict__ vout,
     uint64_t n, uint32_t shift, uint32_t mask, const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ tile_ctr, const uint32_t* __restrict__ base, uint32_t xcd_tiles) {
+    uint32_t* __restrict__ tile_ctr, const uint32_t* __restrict__ base, uint32_t xcd_tiles,
+    uint32_t* __restrict__ status_next) {
     constexpr int WAVES = BLOCK / PD_WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(TILE * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
@@ -339,7 +340,26 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
     __shared__ uint32_t s_busy[2][WAVES];
     auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);
 
-    static_assert(!XP || !RTS, "XCD-persistent passes are chained");
+    static_assert(!XP || (!RTS && !EARLY && !SPLIT), "XCD-persistent passes are chained, look-back after the reorder");
+    // XP: `offs` is this pass's 256-bin digit histogram; each block scans it
+    // once into s_offs (the digit starts) instead of a separate scan kernel
+    __shared__ uint32_t s_offs[XP ? RS_BINS : 1];
+    if constexpr (XP) {
+        const uint32_t t = threadIdx.x, w = t / PD_WAVE;
+        uint32_t h = 0, inc = 0;
+        if (t < RS_BINS) {
+            h = offs[t];
+            inc = pd::wave_incl_scan(h);
+            if (pd::lane() == PD_WAVE - 1) s_wsum[w] = inc;
+        }
+        __syncthreads();
+        if (t < RS_BINS) {
+            uint32_t add = 0;
+            for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
+            s_offs[t] = add + inc - h;
+        }
+        // (the first tile's barriers order these writes before any read)
+    }
     // XP: block i of class x's B blocks takes the class's tickets i, i + B,
     // i + 2B, ... (no atomics); ticket c -> tile (c / S) * 8S + x * S + c % S
     const uint32_t xp_x = blockIdx.x % 8, xp_s = xcd_tiles;
@@ -374,6 +394,8 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
                                       : (uint64_t)blockIdx.x;
     if (RTS && tile * TILE >= n) return;   // XCD order: grid rounded up to a multiple of 8
     if (XP && tile * TILE >= n) break;     // this counter's tiles are done (s_tile is block-wide)
+    // the next pass's status words of this tile start at zero
+    if (status_next && t < RS_BINS) status_next[tile * RS_BINS + t] = 0;
     // RTS: this tile's run start of digit t, loaded now (used after the reorder)
     uint32_t rts_start = 0;
     if (RTS && t < RS_BINS) rts_start = base[t] + offs[tile * RS_BINS + t];
@@ -584,7 +606,7 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 8 ? 8 : 4) void k_rs_onesweep(
         if (t < RS_BINS) s_gbase[t] = offs[t] + excl - local_d;
     } else if (!RTS && !EARLY && t < RS_BINS) {
         if (tile > 0) excl = lookback(cnt_d);
-        s_gbase[t] = offs[t] + excl - local_d;
+        s_gbase[t] = (XP ? s_offs[t] : offs[t]) + excl - local_d;
     }
     __syncthreads();
 
@@ -622,7 +644,7 @@ static int sort_lbw(pimdal_ctx* c) {
 
 template <typename V, bool HV, bool HASH = false>
 static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
-                       const RsPassDesc& pdsc, const uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
+                       const RsPassDesc& pdsc, const uint32_t* hist, uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
     const int64_t bk = c->tune_get("sort_block", 512);
     const int block = bk >= 1024 ? 1024 : bk <= 256 ? 256 : 512;
     // keys per thread: 16, or 8 (1024-thread blocks always take 8, so a
@@ -639,13 +661,20 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     const uint32_t P = pdsc.npass;
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
     Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
-    Scratch st(c, ntiles * RS_BINS * 4, s);
     // knob sort_xcd_tiles S > 0 (default 16): XCD-persistent passes with
     // runs of S tiles per XCD (512 x 16 shape, LDS ranking, look-back after
     // the reorder); 0 = one block per tile in launch order
     const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
     const bool xp_shape = block == 512 && items == 16 && !early && !ballot && !split && nt == 0;
-    auto xs_of = [&](uint32_t) -> uint32_t { return xp_shape && xs_knob > 0 ? (uint32_t)xs_knob : 0u; };
+    const bool xp = xp_shape && xs_knob > 0;
+    auto xs_of = [&](uint32_t) -> uint32_t { return xp ? (uint32_t)xs_knob : 0u; };
+    // status words: XCD-persistent passes scan their digit histogram
+    // themselves and clear the next pass's statuses (two buffers in turn);
+    // the others take the scanned offsets and a cleared buffer per pass
+    const uint64_t nst = ntiles * RS_BINS;
+    Scratch st(c, (xp && P > 1 ? 2 : 1) * nst * 4, s);
+    if (xp) fill_u32(c, st.get(), 0, nst, s);
+    else launch(c, "radix_scan", k_rs_scan, dim3(P), dim3(RS_BINS), 0, s, hist, offs);
     // co-resident grid of an XCD-persistent pass: a multiple of 8 blocks
     auto xp_grid_of = [&](auto kernel) -> uint32_t {
         return std::max<uint32_t>(8, (blocks_per_cu(kernel, 512) * c->cu_count) / 8 * 8);
@@ -657,17 +686,20 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         const bool to_out = ((P - 1 - j) % 2) == 0;
         uint32_t* kdst = to_out ? kout : tk.as<uint32_t>();
         V* vdst = to_out ? vout : tv.as<V>();
-        fill_u32(c, st.get(), 0, ntiles * RS_BINS, s);
+        if (!xp) fill_u32(c, st.get(), 0, nst, s);
+        uint32_t* st_j = st.as<uint32_t>() + (xp ? (uint64_t)(j & 1) * nst : 0);
+        uint32_t* st_next = (xp && j + 1 < P) ? st.as<uint32_t>() + (uint64_t)((j + 1) & 1) * nst : nullptr;
 #define PD_RS_LAUNCH(...)                                                                                         \
     launch(c, name, k_rs_onesweep<__VA_ARGS__>, dim3((uint32_t)ntiles), dim3(block), 0, s, ksrc, vsrc, kdst, vdst, n, \
-           pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j, (const uint32_t*)nullptr, 0u)
+           pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j, (const uint32_t*)nullptr, 0u, \
+           (uint32_t*)nullptr)
         if (xs_of(j) > 0)
         {
 #define PD_RS_XP(W)                                                                                               \
     launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>,                \
                 dim3(xp_grid_of(k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>)), dim3(512),  \
-                0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(),   \
-                (uint32_t*)nullptr, (const uint32_t*)nullptr, xs_of(j))
+                0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], hist + j * RS_BINS, st_j,                \
+                (uint32_t*)nullptr, (const uint32_t*)nullptr, xs_of(j), st_next)
             if (lbw == 2) PD_RS_XP(2);
             else if (lbw == 1) PD_RS_XP(1);
             else if (lbw == 8) PD_RS_XP(8);
@@ -764,11 +796,11 @@ static void run_passes_rts(pimdal_ctx* c, const uint32_t* kin, const V* vin, uin
         if (b1024)
             launch(c, name, k_rs_onesweep<V, HV, 1024, false, HASH, false, 4, 8, true>, dim3(grid), dim3(1024), 0, s,
                    ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], off, (uint32_t*)nullptr, tctr,
-                   (const uint32_t*)B.base, xt);
+                   (const uint32_t*)B.base, xt, (uint32_t*)nullptr);
         else
             launch(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, 4, 16, true>, dim3(grid), dim3(512), 0, s,
                    ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], off, (uint32_t*)nullptr, tctr,
-                   (const uint32_t*)B.base, xt);
+                   (const uint32_t*)B.base, xt, (uint32_t*)nullptr);
         ksrc = kdst;
         vsrc = vdst;
     }
@@ -888,13 +920,12 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         if (vw) copy_d2d(c, vout, vin, n * vw, s);
         return;
     }
-    launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
     if (vw == 0)
-        run_passes<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
+        run_passes<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, s);
     else if (vw == 4)
-        run_passes<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
+        run_passes<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs, ctrs, s);
     else
-        run_passes<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
+        run_passes<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs, ctrs, s);
 }
 
 // Histogram of the whole partition id ref_hash(key) & (F - 1) for F <= 2^15
@@ -1004,13 +1035,12 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
     } else {
         launch(c, "part_hist", k_rs_hist<true>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, run, hist, andor);
     }
-    launch(c, "radix_scan", k_rs_scan, dim3(run.npass), dim3(RS_BINS), 0, s, (const uint32_t*)hist, offs);
     if (vw == 4)
-        run_passes<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, offs, ctrs, s);
+        run_passes<uint32_t, true, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs, ctrs, s);
     else if (vw == 8)
-        run_passes<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, offs, ctrs, s);
+        run_passes<uint64_t, true, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs, ctrs, s);
     else
-        run_passes<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, offs, ctrs, s);
+        run_passes<uint32_t, false, true>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, s);
     launch(c, "part_bounds", k_part_bounds, dim3((uint32_t)std::min<uint64_t>(ceil_div(F + 1, 256), 1024)), dim3(256), 0,
            s, (const uint32_t*)kout, n, F, bounds);
 }
