@@ -17,7 +17,8 @@
 //                   2 B/row) so group codes can be dense;
 //  k_q1_dense<G>    filter + Q1 measures accumulated in registers for G <= 8
 //                   dense groups (no atomics), one partial row per block;
-//  k_q1_final       deterministic 128-bit reduction of the block partials;
+//  k_q1_reduce      deterministic 128-bit reduction of the block partials;
+//                   the last block to finish writes the result rows;
 //  k_q6 / k_q6_final   the TPC-H Q6 filter chain and revenue sum in one
 //                   streaming pass (pimdal/main/tpc_h/query6/dpu/kernel_q6.c);
 //  k_gb_hist / k_gb_scatter   radix partition of the filtered, projected
@@ -110,6 +111,68 @@ __global__ void k_flag_lut(const uint32_t* __restrict__ bits, uint8_t* __restric
             if (present) { inv[c * 256 + r] = (uint8_t)v; r++; }
         }
         meta[c] = r;
+    }
+}
+
+// The code tables of a row sample in one block: presence bitmaps of the
+// first `rows` flag bytes, then lut / inv / meta as k_flag_lut builds them
+// (code = rank among the present values, in parallel: thread v of column c
+// counts the present values below v), and the dense path's miss flag and
+// reduction counter cleared (clr[0..1]).
+__global__ __launch_bounds__(1024) void k_flag_sample_codes(const uint8_t* __restrict__ f1,
+                                                            const uint8_t* __restrict__ f2, uint32_t rows,
+                                                            uint8_t* __restrict__ lut /*[512]*/,
+                                                            uint8_t* __restrict__ inv /*[512]*/,
+                                                            uint32_t* __restrict__ meta, uint32_t* __restrict__ clr) {
+    __shared__ uint32_t s[16];
+    const uint32_t t = threadIdx.x;
+    if (t < 16) s[t] = 0;
+    if (t < 2) clr[t] = 0;
+    __syncthreads();
+    uint64_t m1[4] = {0, 0, 0, 0}, m2[4] = {0, 0, 0, 0};
+    const bool vec = (((uintptr_t)f1 | (uintptr_t)f2) % 16) == 0;
+    const uint32_t n16 = vec ? rows / 16 : 0;
+    for (uint32_t v = t; v < n16; v += 1024) {
+        const uint4 a = reinterpret_cast<const uint4*>(f1)[v];
+        const uint4 b = reinterpret_cast<const uint4*>(f2)[v];
+        const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                flag_mark(m1, (wa[k] >> (8 * e)) & 0xFF);
+                flag_mark(m2, (wb[k] >> (8 * e)) & 0xFF);
+            }
+    }
+    for (uint32_t i = n16 * 16 + t; i < rows; i += 1024) {
+        flag_mark(m1, f1[i]);
+        flag_mark(m2, f2[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        for (int d = PD_WAVE / 2; d > 0; d >>= 1) {
+            m1[j] |= __shfl_xor(m1[j], d, PD_WAVE);
+            m2[j] |= __shfl_xor(m2[j], d, PD_WAVE);
+        }
+    }
+    if (pd::lane() == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (m1[j]) { atomicOr(&s[2 * j], (uint32_t)m1[j]); atomicOr(&s[2 * j + 1], (uint32_t)(m1[j] >> 32)); }
+            if (m2[j]) { atomicOr(&s[8 + 2 * j], (uint32_t)m2[j]); atomicOr(&s[8 + 2 * j + 1], (uint32_t)(m2[j] >> 32)); }
+        }
+    }
+    __syncthreads();
+    if (t < 512) {
+        const uint32_t c = t / 256, v = t % 256;
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < v / 32; j++) r += __popc(s[c * 8 + j]);
+        const uint32_t word = s[c * 8 + v / 32];
+        r += __popc(word & ((1u << (v % 32)) - 1u));
+        const bool present = (word >> (v % 32)) & 1;
+        lut[c * 256 + v] = present ? (uint8_t)r : (uint8_t)0xFF;   // 0xFF: not in the domain
+        if (present) inv[c * 256 + r] = (uint8_t)v;
+        if (v == 255) meta[c] = r + (present ? 1 : 0);
     }
 }
 
@@ -211,9 +274,24 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
 // partial layout per (block, group): q, p, d, dp, sum(low 32 bits of the charge's
 // low word), sum(charge high word), count, sum(high 32 bits of the charge's low word)
 // One block per group code: 128-bit sums of the block partials.
+__device__ void q1_final_rows(const unsigned __int128* __restrict__ sums, uint32_t G, const uint32_t* __restrict__ meta,
+                              const uint8_t* __restrict__ inv, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
+                              int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
+                              int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups);
+
+// One block per group code: 128-bit sums of the block partials.  With
+// `done` set, the last block to finish (device-scope counter, zeroed
+// beforehand) also writes the result rows (q1_final_rows).
 __global__ __launch_bounds__(256) void k_q1_reduce(const uint64_t* __restrict__ partials, uint32_t nblocks, uint32_t G,
-                                                   unsigned __int128* __restrict__ sums /*[G][6]*/) {
+                                                   unsigned __int128* __restrict__ sums /*[G][6]*/,
+                                                   uint32_t* __restrict__ done, const uint32_t* __restrict__ meta,
+                                                   const uint8_t* __restrict__ inv, int32_t* __restrict__ okey,
+                                                   int64_t* __restrict__ sq, int64_t* __restrict__ sp,
+                                                   int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
+                                                   int64_t* __restrict__ sch, int64_t* __restrict__ cnt,
+                                                   uint64_t* __restrict__ ngroups) {
     __shared__ unsigned __int128 s_v[256][6];
+    __shared__ uint32_t s_last;
     const uint32_t g = blockIdx.x, t = threadIdx.x;
     unsigned __int128 a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t b = t; b < nblocks; b += 256) {
@@ -235,13 +313,22 @@ __global__ __launch_bounds__(256) void k_q1_reduce(const uint64_t* __restrict__ 
         __syncthreads();
     }
     if (t < 6) sums[g * 6 + t] = s_v[0][t];
+    if (!done) return;
+    __threadfence();   // this block's sums before its count
+    __syncthreads();
+    if (t == 0)
+        s_last = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == G - 1;
+    __syncthreads();
+    if (s_last && t == 0) {
+        __threadfence();   // every block's sums before the rows
+        q1_final_rows(sums, G, meta, inv, okey, sq, sp, sd, sdp, sch, cnt, ngroups);
+    }
 }
 
-__global__ void k_q1_final(const unsigned __int128* __restrict__ sums, uint32_t G, const uint32_t* __restrict__ meta,
-                           const uint8_t* __restrict__ inv, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
-                           int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
-                           int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups) {
-    if (threadIdx.x != 0) return;
+__device__ void q1_final_rows(const unsigned __int128* __restrict__ sums, uint32_t G, const uint32_t* __restrict__ meta,
+                              const uint8_t* __restrict__ inv, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
+                              int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
+                              int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups) {
     const uint32_t n1 = meta[0], n2 = meta[1];
     uint64_t k = 0;
     for (uint32_t g = 0; g < G && g < n1 * n2; g++) {
@@ -1098,10 +1185,11 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         // measures in register accumulators for up to 8 codes; a kept row
         // whose flag byte the sample lacks sets `miss`, and the call is then
         // redone with the codes of every row.  One readback at the end.
-        // scratch: ng u64 | miss | pad | meta[2] | bits[16] | lut[512] | inv[512]
+        // scratch: ng u64 | miss | reduction counter | meta[2] | bits[16] | lut[512] | inv[512]
         Scratch small(c, 8 + 8 + 8 + 64 + 512 + 512, s);
         uint64_t* ng = small.as<uint64_t>();
         uint32_t* miss = reinterpret_cast<uint32_t*>(ng + 1);
+        uint32_t* done_ctr = miss + 1;
         uint32_t* meta = miss + 2;
         uint32_t* bits = meta + 2;
         uint8_t* lut = reinterpret_cast<uint8_t*>(bits + 16);
@@ -1110,15 +1198,21 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s), sums(c, 8 * 6 * 16, s);
         const bool vec = ((uintptr_t)date | (uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)qty | (uintptr_t)price |
                           (uintptr_t)disc | (uintptr_t)tax) % 16 == 0;
+        // codes of the first `rows` rows; the sample's kernel also clears the
+        // miss flag and the reduction counter, the full pass clears them apart
         auto codes = [&](uint64_t rows) {
-            fill_u32(c, bits, 0, 16, s);
+            if (rows <= 65536) {
+                launch(c, "q1_lut", k_flag_sample_codes, dim3(1), dim3(1024), 0, s, f1, f2, (uint32_t)rows, lut, inv,
+                       meta, miss);
+                return;
+            }
+            fill_u32(c, miss, 0, 4 + 16, s);   // miss, counter, meta[2] (rewritten below), bits[16]
             const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(rows, 256 * 64), (uint64_t)c->cu_count * 4));
             launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, rows, bits);
             launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut, inv, meta);
         };
         auto dense = [&](auto gtag) {
             constexpr int GG = decltype(gtag)::value;
-            fill_u32(c, miss, 0, 1, s);
             if (vec)
                 launch(c, "q1_dense", k_q1_dense<GG, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc,
                        tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
@@ -1126,10 +1220,8 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
                 launch(c, "q1_dense", k_q1_dense<GG, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price,
                        disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
             launch(c, "q1_reduce", k_q1_reduce, dim3(GG), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb,
-                   (uint32_t)GG, sums.as<unsigned __int128>());
-            launch(c, "q1_final", k_q1_final, dim3(1), dim3(64), 0, s, (const unsigned __int128*)sums.as<unsigned __int128>(),
-                   (uint32_t)GG, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty, out.sum_price,
-                   out.sum_disc, out.sum_disc_price, out.sum_charge, out.count, ng);
+                   (uint32_t)GG, sums.as<unsigned __int128>(), done_ctr, (const uint32_t*)meta, (const uint8_t*)inv,
+                   out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count, ng);
         };
         struct { uint64_t ng; uint32_t miss, pad, n1, n2; } hv;
         auto readback = [&]() {

@@ -41,12 +41,30 @@ constexpr int HJ_ROWS = 16;           // probe rows per thread per round
 
 // ---------------- work items over probe partitions ----------------
 
-__global__ void k_hj_chunks(const uint32_t* __restrict__ HS, uint32_t GS, uint32_t F, uint32_t CH,
-                            uint32_t* __restrict__ nch) {
-    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
-        const uint32_t len = HS[(uint64_t)(f + 1) * GS] - HS[(uint64_t)f * GS];
-        nch[f] = (len + CH - 1) / CH;
+// The probe's plan in one block: wst[f] = first work item of probe
+// partition f (ceil(rows / CH) items each, an exclusive scan, wst[F] =
+// total), and the flags and per-item counts cleared.  F <= 16384.
+__global__ __launch_bounds__(1024) void k_hj_plan(const uint32_t* __restrict__ HS, uint32_t F, uint32_t CH,
+                                                  uint32_t* __restrict__ wst, uint32_t* __restrict__ flags,
+                                                  uint32_t* __restrict__ wcnt, uint32_t nwcnt) {
+    __shared__ uint32_t s_w[1024 / PD_WAVE];
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE;
+    for (uint32_t i = t; i < nwcnt; i += 1024) wcnt[i] = 0;
+    if (t < 4) flags[t] = 0;
+    const uint32_t per = (F + 1023) / 1024, f0 = min(F, t * per), f1 = min(F, f0 + per);
+    auto items = [&](uint32_t f) -> uint32_t { return (HS[f + 1] - HS[f] + CH - 1) / CH; };
+    uint32_t sum = 0;
+    for (uint32_t f = f0; f < f1; f++) sum += items(f);
+    const uint32_t inc = pd::wave_incl_scan(sum);
+    if (pd::lane() == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t base = inc - sum;
+    for (uint32_t i = 0; i < w; i++) base += s_w[i];
+    for (uint32_t f = f0; f < f1; f++) {
+        wst[f] = base;
+        base += items(f);
     }
+    if (t == 1023) wst[F] = base;   // the last thread's range ends at F
 }
 
 // flags[0]: a probe row matched more than one build row
@@ -514,13 +532,10 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         hash_partition(c, bk, bp ? bp : bk, 4, nb, bits, rk.as<uint32_t>(), rp.get(), HR.as<uint32_t>(), s);
         hash_partition(c, pk, pp ? pp : pk, 4, np, bits, sk.as<uint32_t>(), sp.get(), HS.as<uint32_t>(), s);
         Scratch wst(c, ((uint64_t)F + 1) * 4, s);
-        launch(c, "hj_chunks", k_hj_chunks, dim3(std::min<uint32_t>(ceil_div(F, 256), 1024)), dim3(256), 0, s,
-               (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, wst.as<uint32_t>());
-        exclusive_scan_u32(c, wst.as<uint32_t>(), wst.as<uint32_t>(), F, nullptr, s);
         const uint64_t max_wi = F + ceil_div(np, CH);
         Scratch wcnt(c, (max_wi + 1) * 4, s), woff(c, (max_wi + 1) * 4, s);
-        fill_u32(c, flags.get(), 0, 4, s);
-        fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
+        launch(c, "hj_chunks", k_hj_plan, dim3(1), dim3(1024), 0, s, (const uint32_t*)HS.as<uint32_t>(), F, CH,
+               wst.as<uint32_t>(), flags.as<uint32_t>(), wcnt.as<uint32_t>(), (uint32_t)(max_wi + 1));
         Scratch valid(c, direct_probe ? max_wi * vw * 8 : 0, s);
         if (direct_probe) {
             launch(c, "hj_join", k_hj_direct<1024, 8>, dim3((uint32_t)max_wi), dim3(1024), 0, s,
