@@ -626,7 +626,12 @@ __global__ __launch_bounds__(GB_LDS_BLOCK) void k_gb_lds(Src src, const uint32_t
 #pragma unroll
     for (int j = 0; j < Agg::NF; j++) cols[j] = s_f[j];
     const uint32_t limit = TS * 3 / 4;
-    constexpr int R = 4;   // rows per thread per round, loaded together
+    constexpr int R = 2;   // rows per thread per round, loaded together (a typical partition: one round)
+    // Each round inserts its rows' keys, then adds their measures.  After the
+    // last round's inserts the group count is final, so the block's output
+    // range is claimed (one device atomic) there and its round trip overlaps
+    // the last additions.
+    unsigned long long base = 0;
     for (uint64_t rb = r0; rb < r1; rb += (uint64_t)GB_LDS_BLOCK * R) {
         uint32_t key[R];
         int64_t in[R][Agg::NI];
@@ -636,12 +641,11 @@ __global__ __launch_bounds__(GB_LDS_BLOCK) void k_gb_lds(Src src, const uint32_t
             const uint64_t i = rb + (uint64_t)q * GB_LDS_BLOCK + t;
             ok[q] = i < r1 && src.load(i, key[q], in[q]);
         }
-        bool over = false;
+        uint32_t slot[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
-            if (!ok[q] || over) continue;
-            uint64_t f[Agg::NF];
-            Agg::fold(in[q], f);
+            slot[q] = TS;   // no slot
+            if (!ok[q]) continue;
             const uint64_t want = GB_OCC | key[q];
             uint32_t h = pd::slot_hash(key[q]) & (TS - 1);
             uint32_t probes = 0;
@@ -657,19 +661,37 @@ __global__ __launch_bounds__(GB_LDS_BLOCK) void k_gb_lds(Src src, const uint32_t
                     if (prev == want) break;
                 }
                 h = (h + 1) & (TS - 1);
-                if (++probes >= TS) { s_over = 1; over = true; break; }
+                if (++probes >= TS) { s_over = 1; h = TS; break; }
             }
-            if (over) continue;
-            add_fields<Agg>(cols, h, f);
+            slot[q] = h;
         }
-        if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        const bool last = rb + (uint64_t)GB_LDS_BLOCK * R >= r1;
+        if (last) {
+            __syncthreads();   // every insert done: s_used and s_over are final
+            if (s_over) break;
+            if (t == 0) {
+                // opaque offset: keeps the compiler from reading the result at once
+                uint32_t z = 0;
+                asm volatile("" : "+v"(z));
+                base = atomicAdd((unsigned long long*)&meta[0] + z, (unsigned long long)s_used);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            if (slot[q] >= TS) continue;
+            uint64_t f[Agg::NF];
+            Agg::fold(in[q], f);
+            add_fields<Agg>(cols, slot[q], f);
+        }
+        // (no early exit on overflow: the barrier above needs every thread;
+        // inserts into a full table stop at their probe limit)
     }
     __syncthreads();
     if (s_over) {
         if (t == 0) { to_global[p] = 1; atomicOr((uint32_t*)&meta[1], 1u); }
         return;
     }
-    if (t == 0) s_base = atomicAdd((unsigned long long*)&meta[0], (unsigned long long)s_used);
+    if (t == 0) s_base = base;
     __syncthreads();
     // emit the occupied slots (compaction through an LDS counter)
     if (t == 0) s_used = 0;
