@@ -1300,22 +1300,22 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         else
             launch(c, "q1_pack", k_q1_pack<false>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
                    qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
+        // The overflow flag is read after the packed path has run (its group
+        // count depends on the keys only); when set, the general path below
+        // recomputes every output.
+        const OutQ1 o{out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count};
+        // LDS table per partition: 1024 slots (64 KB, two blocks per CU;
+        // default) or 2048 slots (128 KB, one block per CU, half the
+        // partitions)
+        auto mk = [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; };
+        const uint64_t g =
+            c->tune_get("gb_q1_slots", 1024) == 1024
+                ? group_by_onesweep<AggQ1T<1024>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s)
+                : group_by_onesweep<AggQ1T<2048>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s);
         uint32_t overflow = 0;
         PD_HIP(hipMemcpyAsync(&overflow, ovf.get(), 4, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
-        if (!overflow) {
-            const OutQ1 o{out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count};
-            // LDS table per partition: 1024 slots (64 KB, two blocks per CU;
-            // default) or 2048 slots (128 KB, one block per CU, half the
-            // partitions)
-            auto mk = [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; };
-            const uint64_t g =
-                c->tune_get("gb_q1_slots", 1024) == 1024
-                    ? group_by_onesweep<AggQ1T<1024>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s)
-                    : group_by_onesweep<AggQ1T<2048>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s);
-            PD_HIP(hipStreamSynchronize(s));
-            return g;
-        }
+        if (!overflow) return g;
     }
     const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
     Scratch k32(c, stage * 4, s), fs(c, stage * 8 * AggQ1::NF, s);

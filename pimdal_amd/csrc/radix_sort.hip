@@ -72,14 +72,39 @@ __global__ __launch_bounds__(512) void k_rs_hist(const uint32_t* __restrict__ ke
             for (uint32_t p = 0; p < pd.npass; p++) atomicAdd(&hc[p][(src >> pd.shift[p]) & pd.mask[p]], 1u);
         }
     };
-    // four 16-byte loads in flight per thread, then their atomics
+    // four 16-byte loads in flight per thread, then their atomics.  A digit
+    // that all 16 keys of every active lane share (a constant high digit, a
+    // narrow key range) takes one atomic for the wave instead of 16 per lane.
     uint64_t v = blockIdx.x * (uint64_t)blockDim.x + t;
     for (; v + 3 * stride < nvec; v += 4 * stride) {
         uint4 q[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) q[u] = k4p[v + u * stride];
+        uint32_t src[16], qa = 0xFFFFFFFFu, qo = 0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) count4(q[u]);
+        for (int u = 0; u < 4; u++) {
+            const uint32_t k4[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                a &= k4[e];
+                o |= k4[e];
+                src[u * 4 + e] = rs_src<HASH>(k4[e]);
+                qa &= src[u * 4 + e];
+                qo |= src[u * 4 + e];
+            }
+        }
+        const uint32_t y0 = (uint32_t)__builtin_amdgcn_readfirstlane(qa);
+        const uint32_t var = (qa ^ qo) | (qa ^ y0);   // bits that differ among this wave's keys
+        const uint64_t act = pd::ballot(true);
+        for (uint32_t p = 0; p < pd.npass; p++) {
+            if (pd::ballot(((var >> pd.shift[p]) & pd.mask[p]) != 0) == 0) {
+                if (pd::lane() == (uint32_t)__builtin_ctzll(act))
+                    atomicAdd(&h[0][p][(y0 >> pd.shift[p]) & pd.mask[p]], 16u * pd::popc64(act));
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; i++) atomicAdd(&hc[p][(src[i] >> pd.shift[p]) & pd.mask[p]], 1u);
+            }
+        }
     }
     for (; v < nvec; v += stride) count4(k4p[v]);
     for (uint64_t i = nvec * 4 + blockIdx.x * (uint64_t)blockDim.x + t; i < n; i += stride) {
