@@ -976,13 +976,23 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist_full(const uint32_t* __rest
     const bool vec = ((uintptr_t)keys % 16) == 0;
     const uint64_t v1 = vec ? a1 / 4 : a0 / 4;
     for (uint64_t i = r0 + t; i < (vec ? a0 : r0); i += blockDim.x) atomicAdd(&h[pd::ref_hash(keys[i]) & (F - 1)], 1u);
-    for (uint64_t v = a0 / 4 + t; v < v1; v += blockDim.x) {
-        const uint4 q = reinterpret_cast<const uint4*>(keys)[v];
+    auto count4 = [&](const uint4 q) {
         atomicAdd(&h[pd::ref_hash(q.x) & (F - 1)], 1u);
         atomicAdd(&h[pd::ref_hash(q.y) & (F - 1)], 1u);
         atomicAdd(&h[pd::ref_hash(q.z) & (F - 1)], 1u);
         atomicAdd(&h[pd::ref_hash(q.w) & (F - 1)], 1u);
+    };
+    // four 16-byte loads in flight per thread, then their atomics
+    const uint4* k4p = reinterpret_cast<const uint4*>(keys);
+    uint64_t v = a0 / 4 + t;
+    for (; v + 3 * (uint64_t)BLOCK < v1; v += 4 * (uint64_t)BLOCK) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = k4p[v + (uint64_t)u * BLOCK];
+#pragma unroll
+        for (int u = 0; u < 4; u++) count4(q[u]);
     }
+    for (; v < v1; v += BLOCK) count4(k4p[v]);
     for (uint64_t i = (vec ? a1 : r0) + t; i < r1; i += blockDim.x) atomicAdd(&h[pd::ref_hash(keys[i]) & (F - 1)], 1u);
     __syncthreads();
     for (uint32_t b = t; b < F; b += blockDim.x) {
