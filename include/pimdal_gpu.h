@@ -80,7 +80,10 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   block take part in the look-back, two per digit), sort_nt (non-temporal
  *   pass loads 1, stores 2, both 3), sort_xcd_tiles (S > 0: persistent
  *   chained passes on a co-resident grid, each XCD taking runs of S
- *   consecutive tiles; default 16; 0: one block per tile), sort_scheme (0 chained passes with decoupled
+ *   consecutive tiles; default 16; 0: one block per tile; if the runtime
+ *   refuses the co-resident grid, that pass and the later ones take one
+ *   block per tile; sort_xcd_refuse_pass j makes pass j's launch count as
+ *   refused, for tests), sort_scheme (0 chained passes with decoupled
  *   look-back, 1 counted passes: per-tile digit counts and a scan, then a
  *   reorder with no look-back), sort_rts_order (tile order of counted
  *   passes: 0 block order, 1 ticket, 2 XCD-contiguous);

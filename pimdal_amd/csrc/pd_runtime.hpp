@@ -135,10 +135,13 @@ void launch(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid
 // progress.  Persistent kernels that walk tiles in a fixed order (block b
 // takes tiles b, b + grid, ...) use it instead of a tile counter: one
 // device-wide atomic per tile serializes at about 11 ns each.
+// Returns false, with nothing launched, when the runtime refuses the grid
+// as too large to be resident at once (fewer CUs available than the device
+// reports, for example); other errors throw.
 template <typename... KArgs, typename... Args>
-void launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+bool launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
                  uint32_t shmem, hipStream_t s, Args... args) {
-    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return true;
     std::tuple<KArgs...> vals(static_cast<KArgs>(args)...);
     void* ptrs[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1];
     std::apply([&](auto&... v) { int i = 0; ((ptrs[i++] = (void*)&v), ...); }, vals);
@@ -147,8 +150,15 @@ void launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3
         e0 = take_event(c);
         PD_HIP(hipEventRecord(e0, s));
     }
-    PD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shmem, s));
+    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shmem, s);
+    if (e == hipErrorCooperativeLaunchTooLarge) {
+        (void)hipGetLastError();   // clear the sticky launch error
+        if (e0) c->event_pool.push_back(e0);
+        return false;
+    }
+    PD_HIP(e);
     if (c->profiling) record_launch(c, name, e0, s);
+    return true;
 }
 
 // Blocks of `kernel` (at `block` threads, no dynamic LDS) that one CU holds

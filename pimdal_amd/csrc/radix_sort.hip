@@ -691,7 +691,7 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     // the reorder); 0 = one block per tile in launch order
     const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
     const bool xp_shape = block == 512 && items == 16 && !early && !ballot && !split && nt == 0;
-    const bool xp = xp_shape && xs_knob > 0;
+    bool xp = xp_shape && xs_knob > 0;   // cleared if the runtime refuses the co-resident grid
     auto xs_of = [&](uint32_t) -> uint32_t { return xp ? (uint32_t)xs_knob : 0u; };
     // status words: XCD-persistent passes scan their digit histogram
     // themselves and clear the next pass's statuses (two buffers in turn);
@@ -714,6 +714,7 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         if (!xp) fill_u32(c, st.get(), 0, nst, s);
         uint32_t* st_j = st.as<uint32_t>() + (xp ? (uint64_t)(j & 1) * nst : 0);
         uint32_t* st_next = (xp && j + 1 < P) ? st.as<uint32_t>() + (uint64_t)((j + 1) & 1) * nst : nullptr;
+        bool refused = false;   // the co-resident grid was refused: this and later passes take one block per tile
 #define PD_RS_LAUNCH(...)                                                                                         \
     launch(c, name, k_rs_onesweep<__VA_ARGS__>, dim3((uint32_t)ntiles), dim3(block), 0, s, ksrc, vsrc, kdst, vdst, n, \
            pdsc.shift[j], pdsc.mask[j], offs + j * RS_BINS, st.as<uint32_t>(), ctrs + j, (const uint32_t*)nullptr, 0u, \
@@ -721,17 +722,28 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
         if (xs_of(j) > 0)
         {
 #define PD_RS_XP(W)                                                                                               \
-    launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>,                \
+    refused = !launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>,     \
                 dim3(xp_grid_of(k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>)), dim3(512),  \
                 0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], hist + j * RS_BINS, st_j,                \
                 (uint32_t*)nullptr, (const uint32_t*)nullptr, xs_of(j), st_next)
-            if (lbw == 2) PD_RS_XP(2);
+            // knob sort_xcd_refuse_pass j (tests): take pass j's launch as refused
+            if ((int64_t)j == c->tune_get("sort_xcd_refuse_pass", -1)) refused = true;
+            else if (lbw == 2) PD_RS_XP(2);
             else if (lbw == 1) PD_RS_XP(1);
             else if (lbw == 8) PD_RS_XP(8);
             else PD_RS_XP(4);
 #undef PD_RS_XP
+            if (refused) {
+                // scanned offsets and a cleared status buffer, as the
+                // one-block-per-tile passes take them
+                xp = false;
+                launch(c, "radix_scan", k_rs_scan, dim3(P), dim3(RS_BINS), 0, s, hist, offs);
+                fill_u32(c, st.get(), 0, nst, s);
+            }
         }
-        else if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
+        if (xs_of(j) > 0) {
+            // the XCD-persistent pass ran above
+        } else if (block == 1024) PD_RS_LAUNCH(V, HV, 1024, false, HASH, false, 4, 8);
         else if (items == 8 && block == 512) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 4, 8);
         else if (block == 512 && !early && !ballot && split) {
             if (lbw >= 8) PD_RS_LAUNCH(V, HV, 512, false, HASH, false, 8, 16, false, true);

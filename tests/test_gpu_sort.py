@@ -153,6 +153,8 @@ PASS_VARIANTS = {
     "nontemporal_loads": {"sort_nt": 1},
     "nontemporal_loads_stores": {"sort_nt": 3},
     "launch_order_tiles": {"sort_xcd_tiles": 0},
+    "persistent_launch_refused_at_pass0": {"sort_xcd_refuse_pass": 0},
+    "persistent_launch_refused_at_pass1": {"sort_xcd_refuse_pass": 1},
     "xcd_persistent_runs1": {"sort_xcd_tiles": 1},
     "xcd_persistent_runs5": {"sort_xcd_tiles": 5},
     "xcd_persistent_runs16_all_digit_widths": {"sort_xcd_tiles": 16},
