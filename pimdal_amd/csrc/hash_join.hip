@@ -302,15 +302,24 @@ __global__ __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 2) void k_hj_direct(
         key[k] = i < s1 ? Sk[i] : 0u;
         pay[k] = i < s1 ? Sp[i] : 0u;
     }
-    // ---- build ----
-    for (uint64_t i = r0 + t; i < r1; i += BLOCK) {
-        const uint2 kv = make_uint2(Rk[i], Rp[i]);
-        uint32_t h = pd::slot_hash(kv.x) & (HJ_TS - 1);
+    // ---- build: every build row of this thread loaded first, then inserted
+    // (a partition holds at most 7/8 of the table, so BR rows per thread) ----
+    constexpr int BR = (HJ_TS * 7 / 8 + BLOCK - 1) / BLOCK;
+    uint2 brow[BR];
+#pragma unroll
+    for (int k = 0; k < BR; k++) {
+        const uint64_t i = r0 + (uint64_t)k * BLOCK + t;
+        brow[k] = i < r1 ? make_uint2(Rk[i], Rp[i]) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < BR; k++) {
+        if (r0 + (uint64_t)k * BLOCK + t >= r1) break;
+        uint32_t h = pd::slot_hash(brow[k].x) & (HJ_TS - 1);
         while (true) {
             const uint32_t bit = 1u << (h & 31);
             const uint32_t prev = atomicOr(&occ[h >> 5], bit);
             if (!(prev & bit)) {
-                tab[h] = kv;
+                tab[h] = brow[k];
                 break;
             }
             h = (h + 1) & (HJ_TS - 1);
