@@ -183,8 +183,20 @@ __global__ __launch_bounds__(SM_BLOCK) void k_smj_direct(const uint32_t* __restr
     const uint64_t lo = ranges[2 * tile], hi = ranges[2 * tile + 1];
     const uint32_t m = (uint32_t)(hi - lo);
     const bool cached = m <= SM_RCAP;
-    if (cached)
-        for (uint32_t i = t; i < m; i += SM_BLOCK) s_rk[i] = rk[lo + i];
+    if (cached) {
+        // every load of the build range first, then the LDS stores
+        uint32_t bk[SM_RCAP / SM_BLOCK];
+#pragma unroll
+        for (int k = 0; k < (int)(SM_RCAP / SM_BLOCK); k++) {
+            const uint32_t i = (uint32_t)k * SM_BLOCK + t;
+            bk[k] = i < m ? rk[lo + i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < (int)(SM_RCAP / SM_BLOCK); k++) {
+            const uint32_t i = (uint32_t)k * SM_BLOCK + t;
+            if (i < m) s_rk[i] = bk[k];
+        }
+    }
     __syncthreads();
     auto bkey = [&](uint32_t p) -> uint32_t { return cached ? s_rk[p] : rk[lo + p]; };
     // ---- merge: rows t*16 .. t*16+15 of the tile ----
