@@ -960,8 +960,7 @@ uint64_t aggregate_parts(pimdal_ctx* c, Src src, const uint32_t* H, uint32_t G, 
         launch(c, "gb_lds", k_gb_lds<Agg, Src, O>, dim3(F), dim3(GB_LDS_BLOCK), 0, s, src, H, G, F, max_rows,
                tog.as<uint32_t>(), o, meta.as<uint64_t>());
         uint64_t m[4];
-        PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
+        read_small(c, m, meta.get(), 32, s);
         if (m[1]) {
             const uint32_t CH = 16384;
             Scratch wst(c, ((uint64_t)F + 1) * 4, s);
@@ -979,8 +978,7 @@ uint64_t aggregate_parts(pimdal_ctx* c, Src src, const uint32_t* H, uint32_t G, 
                    dim3((uint32_t)std::min<uint64_t>(ceil_div(slots, 256), 4096)), dim3(256), 0, s, H, G, F,
                    (const uint32_t*)tog.as<uint32_t>(), (const uint64_t*)gk.as<uint64_t>(),
                    (const uint64_t*)gfv.as<uint64_t>(), slots, o, meta.as<uint64_t>());
-            PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-            PD_HIP(hipStreamSynchronize(s));
+            read_small(c, m, meta.get(), 32, s);
         }
         return m[0];
     };
@@ -1037,8 +1035,7 @@ bool group_by_direct(pimdal_ctx* c, const GbSource& src, uint64_t n, uint32_t* o
     launch(c, "gb_direct", k_gb_direct<Agg>, dim3(G), dim3(GB_BLOCK), 0, s, src, n, chunk, gk.as<uint64_t>(),
            gfv.as<uint64_t>(), meta.as<uint64_t>());
     uint64_t m[4];
-    PD_HIP(hipMemcpyAsync(m, meta.get(), 32, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
+    read_small(c, m, meta.get(), 32, s);
     if (m[2]) return false;
     PD_REQUIRE(m[3] <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     launch(c, "gb_direct_emit", k_gb_direct_emit<Agg>, dim3(64), dim3(256), 0, s, (const uint64_t*)gk.as<uint64_t>(),
@@ -1188,8 +1185,7 @@ uint64_t q6(pimdal_ctx* c, const int32_t* date, const int64_t* qty, const int64_
     launch(c, "q6_final", k_q6_final, dim3(1), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb,
            res.as<uint64_t>());
     uint64_t h[2];
-    PD_HIP(hipMemcpyAsync(h, res.get(), 16, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
+    read_small(c, h, res.get(), 16, s);
     *revenue = (int64_t)h[0];
     return h[1];
 }
@@ -1247,8 +1243,7 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         };
         struct { uint64_t ng; uint32_t miss, pad, n1, n2; } hv;
         auto readback = [&]() {
-            PD_HIP(hipMemcpyAsync(&hv, ng, sizeof(hv), hipMemcpyDeviceToHost, s));
-            PD_HIP(hipStreamSynchronize(s));
+            read_small(c, &hv, ng, sizeof(hv), s);
         };
         codes(std::min<uint64_t>(n, 65536));
         dense(std::integral_constant<int, 8>{});
@@ -1257,8 +1252,7 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         if (!done && hv.n1 * hv.n2 <= 8) {
             // the sample missed a flag value: codes of every row
             codes(n);
-            PD_HIP(hipMemcpyAsync(&hv.n1, meta, 8, hipMemcpyDeviceToHost, s));
-            PD_HIP(hipStreamSynchronize(s));
+            read_small(c, &hv.n1, meta, 8, s);
             const uint32_t groups = hv.n1 * hv.n2;
             if (groups <= 8) {
                 // accumulators for exactly as many codes as the data has
@@ -1313,8 +1307,7 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
                 ? group_by_onesweep<AggQ1T<1024>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s)
                 : group_by_onesweep<AggQ1T<2048>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s);
         uint32_t overflow = 0;
-        PD_HIP(hipMemcpyAsync(&overflow, ovf.get(), 4, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
+        read_small(c, &overflow, ovf.get(), 4, s);
         if (!overflow) return g;
     }
     const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
@@ -1337,8 +1330,7 @@ void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n,
     if (n) launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
     launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut_dev, lut_dev + 512, bits + 16);
     PD_HIP(hipMemcpyAsync(meta, bits + 16, 8, hipMemcpyDeviceToHost, s));
-    if (inv_host) PD_HIP(hipMemcpyAsync(inv_host, lut_dev + 512, 512, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
+    if (inv_host) read_small(c, inv_host, lut_dev + 512, 512, s);
 }
 
 uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,

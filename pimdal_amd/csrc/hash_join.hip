@@ -554,8 +554,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
                    (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), p0, b0, valid.as<uint64_t>(), vw,
                    flags.as<uint32_t>());
             uint32_t fl[4];
-            PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
-            PD_HIP(hipStreamSynchronize(s));
+            read_small(c, fl, flags.get(), 16, s);
             if (fl[1]) {  // a build partition overflowed the table: more partitions
                 F *= 2;
                 continue;
@@ -595,8 +594,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         if (!direct_probe) {
             if (c->tune_get("hj_block", 1024) == 1024) PD_HJ_UNIQUE(1024, 8);
             else PD_HJ_UNIQUE(512, 16);
-            PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
-            PD_HIP(hipStreamSynchronize(s));
+            read_small(c, fl, flags.get(), 16, s);
         }
 #undef PD_HJ_UNIQUE
         if (fl[1]) {  // a build partition overflowed the table: more partitions
@@ -615,8 +613,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
                    (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), (const uint32_t*)nullptr,
                    (uint32_t*)nullptr, (uint32_t*)nullptr, (uint64_t)0, flags.as<uint32_t>());
             exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
-            PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
-            PD_HIP(hipStreamSynchronize(s));
+            read_small(c, fl, flags.get(), 16, s);
             total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
             PD_REQUIRE(total < (1ull << 32), PIMDAL_E_UNSUPPORTED, "join output exceeds 2^32-1 pairs");
             if (out_p && out_b && cap)
@@ -847,8 +844,7 @@ uint64_t hash_join_global(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp,
            (const uint64_t*)tab.as<uint64_t>(), (const uint32_t*)occ.as<uint32_t>(), mask, p0, b0, (uint64_t)np,
            st.as<uint64_t>(), ctr, ntiles, (uint32_t*)nullptr, (const uint32_t*)nullptr, flags.as<uint32_t>());
     uint32_t fl[4];
-    PD_HIP(hipMemcpyAsync(fl, flags.get(), 16, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
+    read_small(c, fl, flags.get(), 16, s);
     uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
     if (!fl[0]) {
         if (!direct && out_p && out_b && cap) {

@@ -348,8 +348,7 @@ uint64_t sort_merge_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, 
     launch(c, "smj_dupcheck", k_adjacent_dup, dim3(std::max(1u, g)), dim3(256), 0, s, (const uint32_t*)rk.as<uint32_t>(),
            nb, small.as<uint32_t>());
     uint32_t dup = 0;
-    PD_HIP(hipMemcpyAsync(&dup, small.get(), 4, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
+    read_small(c, &dup, small.get(), 4, s);
     if (!dup) {
         const uint64_t ntiles = ceil_div(np, SM_TILE);
         Scratch st(c, 8 * (ntiles + 1), s);
@@ -401,8 +400,7 @@ uint64_t sort_merge_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, 
                st.as<uint64_t>(), reinterpret_cast<uint32_t*>(st.as<uint64_t>() + ntiles), ntiles,
                small.as<uint64_t>() + 1, (const uint64_t*)rng.as<uint64_t>());
         uint64_t total = 0;
-        PD_HIP(hipMemcpyAsync(&total, small.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
+        read_small(c, &total, small.as<uint64_t>() + 1, 8, s);
         if (!direct && out_p && out_b && cap) {
             copy_d2d(c, out_p, op, std::min(total, cap) * 4, s);
             copy_d2d(c, out_b, ob, std::min(total, cap) * 4, s);

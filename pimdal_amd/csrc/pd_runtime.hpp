@@ -47,6 +47,9 @@ public:
 private:
     struct Block { void* p; uint64_t size; hipStream_t stream; hipEvent_t ev; };
     static uint64_t round_size(uint64_t b);
+    hipEvent_t take_event();            // an idle event (created when none is idle)
+    void give_event(hipEvent_t e);      // back to the idle list
+    std::vector<hipEvent_t> idle_events_;
     int device_ = 0;
     uint64_t budget_ = 0, live_ = 0, reserved_ = 0, peak_ = 0;
     std::multimap<uint64_t, Block> cached_;
@@ -66,6 +69,9 @@ struct pimdal_ctx {
     void* pinned[2] = {nullptr, nullptr};
     hipEvent_t pinned_ev[2] = {nullptr, nullptr};
     uint64_t pinned_bytes = 0;
+    // pinned words for small device-to-host reads (read_small)
+    void* readback = nullptr;
+    uint64_t readback_words[8] = {};
     // profiling: hipEvent pairs per kernel name on the stream it ran on
     bool profiling = false;
     struct Pending { std::string name; hipEvent_t e0, e1; };
@@ -177,6 +183,8 @@ void fill_u64(pimdal_ctx* c, void* p, uint64_t value, uint64_t count, hipStream_
 void copy_d2d(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes, hipStream_t s);
 
 uint64_t read_u64(pimdal_ctx* c, const void* dev, hipStream_t s);
+// Copies `bytes` (<= 64 through pinned memory) from the device and waits.
+void read_small(pimdal_ctx* c, void* dst, const void* dev, uint64_t bytes, hipStream_t s);
 
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 

@@ -864,8 +864,7 @@ static void sort_rts(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t 
         // every key needs no pass)
         rts_count<false, true>(c, kin, n, 0, 0xFF, B, 0, s);
         uint32_t ao[2];
-        PD_HIP(hipMemcpyAsync(ao, B.andor, 8, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
+        read_small(c, ao, B.andor, 8, s);
         const uint32_t varying = ao[0] ^ ao[1];
         run.npass = 0;
         for (int p = 0; p < 4; p++) {
@@ -937,8 +936,7 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
     RsPassDesc run = all;
     if (detect) {
         uint32_t ao[2];
-        PD_HIP(hipMemcpyAsync(ao, andor, 8, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
+        read_small(c, ao, andor, 8, s);
         const uint32_t varying = ao[0] ^ ao[1];
         // keep only digits with a varying bit; move their histograms down
         run.npass = 0;

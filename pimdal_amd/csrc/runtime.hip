@@ -50,6 +50,24 @@ void DevicePool::init(int device, uint64_t budget) {
     budget_ = budget;
 }
 
+// Events that mark when a freed block's last use completes are recycled:
+// creating and destroying one per free costs more host time than the
+// small operators' kernels.
+hipEvent_t DevicePool::take_event() {
+    if (!idle_events_.empty()) {
+        hipEvent_t e = idle_events_.back();
+        idle_events_.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    PD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+void DevicePool::give_event(hipEvent_t e) {
+    if (e) idle_events_.push_back(e);
+}
+
 void* DevicePool::alloc(uint64_t bytes, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(mu_);
     const uint64_t sz = round_size(bytes);
@@ -60,7 +78,7 @@ void* DevicePool::alloc(uint64_t bytes, hipStream_t stream) {
         bool ok = (b.stream == stream) || (hipEventQuery(b.ev) == hipSuccess);
         if (ok) {
             void* p = b.p;
-            if (b.ev) (void)hipEventDestroy(b.ev);
+            give_event(b.ev);
             cached_.erase(it);
             live_ += sz;
             live_blocks_[p] = sz;
@@ -73,7 +91,7 @@ void* DevicePool::alloc(uint64_t bytes, hipStream_t stream) {
         PD_HIP(hipDeviceSynchronize());
         for (auto& kv : cached_) {
             (void)hipFree(kv.second.p);
-            if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+            give_event(kv.second.ev);
             reserved_ -= kv.second.size;
         }
         cached_.clear();
@@ -102,8 +120,7 @@ void DevicePool::free(void* p, hipStream_t stream) {
     const uint64_t sz = it->second;
     live_blocks_.erase(it);
     live_ -= sz;
-    hipEvent_t ev = nullptr;
-    PD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipEvent_t ev = take_event();
     PD_HIP(hipEventRecord(ev, stream));
     cached_.emplace(sz, Block{p, sz, stream, ev});
 }
@@ -113,7 +130,7 @@ void DevicePool::release_cached() {
     (void)hipDeviceSynchronize();
     for (auto& kv : cached_) {
         (void)hipFree(kv.second.p);
-        if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+        give_event(kv.second.ev);
         reserved_ -= kv.second.size;
     }
     cached_.clear();
@@ -128,6 +145,8 @@ void DevicePool::destroy() {
     }
     live_blocks_.clear();
     live_ = 0;
+    for (hipEvent_t e : idle_events_) (void)hipEventDestroy(e);
+    idle_events_.clear();
 }
 
 // ---------------- profiling ----------------
@@ -193,11 +212,24 @@ void copy_d2d(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes, hipStre
 }
 
 uint64_t read_u64(pimdal_ctx* c, const void* dev, hipStream_t s) {
-    (void)c;
     uint64_t v = 0;
-    PD_HIP(hipMemcpyAsync(&v, dev, 8, hipMemcpyDeviceToHost, s));
-    PD_HIP(hipStreamSynchronize(s));
+    read_small(c, &v, dev, 8, s);
     return v;
+}
+
+// Small device-to-host reads (counts, flags) through a pinned word buffer
+// of the context: a copy into pageable memory goes through the runtime's
+// staging path, several times slower for a few bytes.
+void read_small(pimdal_ctx* c, void* dst, const void* dev, uint64_t bytes, hipStream_t s) {
+    if (bytes > sizeof(c->readback_words)) {
+        PD_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, s));
+        PD_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    if (!c->readback) PD_HIP(hipHostMalloc(&c->readback, sizeof(c->readback_words), hipHostMallocDefault));
+    PD_HIP(hipMemcpyAsync(c->readback, dev, bytes, hipMemcpyDeviceToHost, s));
+    PD_HIP(hipStreamSynchronize(s));
+    memcpy(dst, c->readback, bytes);
 }
 
 // ---------------- pinned staging ----------------
@@ -296,6 +328,7 @@ extern "C" int pimdal_ctx_destroy(pimdal_ctx* c) {
         if (c->pinned[i]) (void)hipHostFree(c->pinned[i]);
         if (c->pinned_ev[i]) (void)hipEventDestroy(c->pinned_ev[i]);
     }
+    if (c->readback) (void)hipHostFree(c->readback);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.e0); (void)hipEventDestroy(p.e1); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);

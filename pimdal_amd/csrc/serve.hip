@@ -592,8 +592,7 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         launch(c, "sv_dim_index", k_dim_index_build, dim3((uint32_t)std::min<uint64_t>(ceil_div(t.n_dim, 256), 8192)),
                dim3(256), 0, s, t.dim_key, t.dim_date, t.n_dim, (uint4*)S->dim_index, cap - 1, dup.as<uint32_t>());
         uint32_t hd = 0;
-        PD_HIP(hipMemcpyAsync(&hd, dup.get(), 4, hipMemcpyDeviceToHost, s));
-        PD_HIP(hipStreamSynchronize(s));
+        read_small(c, &hd, dup.get(), 4, s);
         PD_REQUIRE(hd == 0, PIMDAL_E_INVALID, "dimension keys must be unique");
         // flag domains of the fact table -> dense group codes (fixed while the table is resident)
         S->lut = c->pool.alloc(1024, s);
