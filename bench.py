@@ -32,16 +32,24 @@ def log(*a):
 
 
 def timed_steps(ctx, fn, steps, warmup):
+    """Seconds per step over `steps` uninstrumented steps, timed between two
+    device synchronisations; then the same number of steps again with
+    per-kernel hipEvent timing on, for kernel_breakdown and the roofline's
+    kernel durations (events around every launch would add host time to
+    the timed steps)."""
     for _ in range(warmup):
         fn()
     ctx.sync()
-    ctx.reset_stats()
-    ctx.set_profiling(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
     ctx.sync()
     t1 = time.perf_counter()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    for _ in range(steps):
+        fn()
+    ctx.sync()
     ctx.set_profiling(False)
     return (t1 - t0) / steps
 
