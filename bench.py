@@ -8,7 +8,8 @@ operators' rows/s at their configured sizes, the dominant kernel's roofline
 (hipEvent time on the stream it runs on), and the CPU restatement timed on
 a bounded sample on this host.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--only sort]
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--only sort|zipf|sortselect|join|groupby|serve]
 """
 from __future__ import annotations
 
@@ -271,7 +272,7 @@ def main():
         a.free(); b.free()
     log("bench: sort 2^28 uniform")
     srt = bench_sort(ctx, args.steps, args.warmup)
-    if args.only in ("", "all"):
+    if args.only in ("", "all", "zipf"):
         log("bench: sort 2^28 zipf")
         ops_out["sort_zipf_2^28"] = bench_sort(ctx, max(3, args.steps // 2), 1, zipf=True)
     if args.only in ("", "all", "sortselect"):

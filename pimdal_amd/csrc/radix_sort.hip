@@ -690,7 +690,7 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     // runs of S tiles per XCD (512 x 16 shape, LDS ranking, look-back after
     // the reorder); 0 = one block per tile in launch order
     const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
-    const bool xp_shape = block == 512 && items == 16 && !early && !ballot && !split && nt == 0;
+    const bool xp_shape = block == 512 && items == 16 && !early && !split && nt == 0;
     bool xp = xp_shape && xs_knob > 0;   // cleared if the runtime refuses the co-resident grid
     auto xs_of = [&](uint32_t) -> uint32_t { return xp ? (uint32_t)xs_knob : 0u; };
     // status words: XCD-persistent passes scan their digit histogram
@@ -721,17 +721,18 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
            (uint32_t*)nullptr)
         if (xs_of(j) > 0)
         {
-#define PD_RS_XP(W)                                                                                               \
-    refused = !launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>,     \
-                dim3(xp_grid_of(k_rs_onesweep<V, HV, 512, false, HASH, false, W, 16, false, false, 0, true>)), dim3(512),  \
+#define PD_RS_XP(W, BAL)                                                                                          \
+    refused = !launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, BAL, W, 16, false, false, 0, true>,       \
+                dim3(xp_grid_of(k_rs_onesweep<V, HV, 512, false, HASH, BAL, W, 16, false, false, 0, true>)), dim3(512),    \
                 0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], hist + j * RS_BINS, st_j,                \
                 (uint32_t*)nullptr, (const uint32_t*)nullptr, xs_of(j), st_next)
             // knob sort_xcd_refuse_pass j (tests): take pass j's launch as refused
             if ((int64_t)j == c->tune_get("sort_xcd_refuse_pass", -1)) refused = true;
-            else if (lbw == 2) PD_RS_XP(2);
-            else if (lbw == 1) PD_RS_XP(1);
-            else if (lbw == 8) PD_RS_XP(8);
-            else PD_RS_XP(4);
+            else if (ballot) PD_RS_XP(2, true);
+            else if (lbw == 2) PD_RS_XP(2, false);
+            else if (lbw == 1) PD_RS_XP(1, false);
+            else if (lbw == 8) PD_RS_XP(8, false);
+            else PD_RS_XP(4, false);
 #undef PD_RS_XP
             if (refused) {
                 // scanned offsets and a cleared status buffer, as the
