@@ -672,10 +672,10 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
                        const RsPassDesc& pdsc, const uint32_t* hist, uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
     const int64_t bk = c->tune_get("sort_block", 512);
     const int block = bk >= 1024 ? 1024 : bk <= 256 ? 256 : 512;
+    const bool early = c->tune_get("sort_early_lookback", 0) != 0;
     // keys per thread: 16, or 8 (1024-thread blocks always take 8, so a
     // tile stays 8192 keys and two blocks still fit a CU's LDS)
     const int items = (block == 1024 || c->tune_get("sort_items", 16) == 8) ? 8 : 16;
-    const bool early = c->tune_get("sort_early_lookback", 0) != 0;
     const bool ballot = c->tune_get("sort_rank", 0) == 1;
     const int lbw = sort_lbw(c);
     // knob sort_lb_split: 1 = two threads per digit in the look-back
@@ -687,10 +687,11 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     Scratch tk(c, P > 1 ? n * 4 : 0, s);
     Scratch tv(c, (HV && P > 1) ? n * sizeof(V) : 0, s);
     // knob sort_xcd_tiles S > 0 (default 16): XCD-persistent passes with
-    // runs of S tiles per XCD (512 x 16 shape, LDS ranking, look-back after
-    // the reorder); 0 = one block per tile in launch order
+    // runs of S tiles per XCD (512 x 16 and 256 x 16 shapes, look-back
+    // after the reorder); 0 = one block per tile in launch order
     const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
-    const bool xp_shape = block == 512 && items == 16 && !early && !split && nt == 0;
+    const bool xp_shape = !early && !split && nt == 0 &&
+                          ((block == 512 && items == 16) || (block == 256 && items == 16 && !ballot));
     bool xp = xp_shape && xs_knob > 0;   // cleared if the runtime refuses the co-resident grid
     auto xs_of = [&](uint32_t) -> uint32_t { return xp ? (uint32_t)xs_knob : 0u; };
     // status words: XCD-persistent passes scan their digit histogram
@@ -702,7 +703,7 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     else launch(c, "radix_scan", k_rs_scan, dim3(P), dim3(RS_BINS), 0, s, hist, offs);
     // co-resident grid of an XCD-persistent pass: a multiple of 8 blocks
     auto xp_grid_of = [&](auto kernel) -> uint32_t {
-        return std::max<uint32_t>(8, (blocks_per_cu(kernel, 512) * c->cu_count) / 8 * 8);
+        return std::max<uint32_t>(8, (blocks_per_cu(kernel, block) * c->cu_count) / 8 * 8);
     };
     const uint32_t* ksrc = kin;
     const V* vsrc = vin;
@@ -721,18 +722,19 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
            (uint32_t*)nullptr)
         if (xs_of(j) > 0)
         {
-#define PD_RS_XP(W, BAL)                                                                                          \
-    refused = !launch_coop(c, name, k_rs_onesweep<V, HV, 512, false, HASH, BAL, W, 16, false, false, 0, true>,       \
-                dim3(xp_grid_of(k_rs_onesweep<V, HV, 512, false, HASH, BAL, W, 16, false, false, 0, true>)), dim3(512),    \
+#define PD_RS_XP(B, IT, W, BAL)                                                                                   \
+    refused = !launch_coop(c, name, k_rs_onesweep<V, HV, B, false, HASH, BAL, W, IT, false, false, 0, true>,         \
+                dim3(xp_grid_of(k_rs_onesweep<V, HV, B, false, HASH, BAL, W, IT, false, false, 0, true>)), dim3(B),      \
                 0, s, ksrc, vsrc, kdst, vdst, n, pdsc.shift[j], pdsc.mask[j], hist + j * RS_BINS, st_j,                \
                 (uint32_t*)nullptr, (const uint32_t*)nullptr, xs_of(j), st_next)
             // knob sort_xcd_refuse_pass j (tests): take pass j's launch as refused
             if ((int64_t)j == c->tune_get("sort_xcd_refuse_pass", -1)) refused = true;
-            else if (ballot) PD_RS_XP(2, true);
-            else if (lbw == 2) PD_RS_XP(2, false);
-            else if (lbw == 1) PD_RS_XP(1, false);
-            else if (lbw == 8) PD_RS_XP(8, false);
-            else PD_RS_XP(4, false);
+            else if (block == 256) PD_RS_XP(256, 16, 2, false);
+            else if (ballot) PD_RS_XP(512, 16, 2, true);
+            else if (lbw == 2) PD_RS_XP(512, 16, 2, false);
+            else if (lbw == 1) PD_RS_XP(512, 16, 1, false);
+            else if (lbw == 8) PD_RS_XP(512, 16, 8, false);
+            else PD_RS_XP(512, 16, 4, false);
 #undef PD_RS_XP
             if (refused) {
                 // scanned offsets and a cleared status buffer, as the

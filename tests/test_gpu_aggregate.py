@@ -160,6 +160,34 @@ def test_hash_aggr_many_partitions(orc):
     assert all(np.array_equal(a, b) for a, b in zip(got, exp))
 
 
+def test_hash_aggr_partitions_over_the_row_limit_between_lds_ones(orc):
+    """65536 partitions of about 16 rows; the ones over a 20-row limit (about
+    one in eight) go to the HBM tables, the rest stay in LDS tables.  Own
+    context, so the knobs do not leak."""
+    import pimdal_amd
+    n = 1 << 20
+    k = orc.gen_range(np.uint32, 43, 1, n, 0, n // 2)
+    v = orc.gen_range(np.int64, 43, 2, n, -(1 << 40), 1 << 40)
+    with pimdal_amd.Context(0) as c2:
+        c2.set_tune("gb_rows_per_part", 16)
+        c2.set_tune("gb_lds_max_rows", 20)
+        c2.set_tune("gb_groups_hint", n // 2)
+        got = agg(c2, ops.hash_aggr, k, v)
+    exp = orc.hash_aggr(k, v)
+    assert all(np.array_equal(a, b) for a, b in zip(got, exp))
+
+
+def test_q1_by_orderkey_tables_overflowing_between_lds_ones(ctx, orc):
+    """2048 partitions with about 768 groups each (the overflow point of a
+    1024-slot table): about half overflow to the HBM tables, the rest are
+    aggregated in LDS."""
+    n = 1 << 22
+    t = orc.fact_table(n, groups=2048 * 768)
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
+    _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
 @pytest.mark.parametrize("case", ["late_value", "late_values_past_8_groups", "late_value_filtered_out"])
 def test_q1_flags_values_after_the_code_sample(ctx, orc, case):
     """The dense path takes its flag codes from the first 65536 rows; a kept

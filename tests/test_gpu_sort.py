@@ -161,6 +161,8 @@ PASS_VARIANTS = {
     "block1024_items8": {"sort_block": 1024},
     "block512_items8": {"sort_items": 8},
     "block256_early": {"sort_block": 256, "sort_early_lookback": 1},
+    "xcd_persistent_block256": {"sort_block": 256},
+    "block256_refused_at_pass1": {"sort_block": 256, "sort_xcd_refuse_pass": 1},
     "ballot_rank": {"sort_rank": 1},
     "counted_xcd_order": {"sort_scheme": 1},
     "counted_ticket_order": {"sort_scheme": 1, "sort_rts_order": 1},
