@@ -9,7 +9,7 @@ operators' rows/s at their configured sizes, the dominant kernel's roofline
 a bounded sample on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--only sort|zipf|sortselect|join|groupby|serve]
+                  [--only sort|zipf|sortselect|join|groupby|ingest|serve]
 """
 from __future__ import annotations
 
@@ -118,7 +118,7 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
-KERNELS = ("part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
+KERNELS = ("date32_pack", "part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
            "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
@@ -179,6 +179,26 @@ def bench_q1(ctx, steps, warmup, by_key, n=60_000_000):
            "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
     for a in t.values():
         a.free()
+    return res
+
+
+def bench_date32(ctx, steps, warmup, n=60_000_000):
+    """Arrow date32 column (days over 1992-1998) to the reference's packed dates, 6e7 values."""
+    from pimdal_amd import operators as ops
+    days = ctx.empty(n, "int32")
+    from pimdal_amd._lib import call
+    call("pimdal_gen_range", ctx.handle, 2, 0xDA7E, 1, n, 8035, 8035 + 2556, days.vp)
+    out = ctx.empty(n, "int32")
+
+    def step():
+        ops.date32_pack(ctx, days, out)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    alg = 8 * n   # 4 B read + 4 B written per value
+    res = {"rows": n, "s_per_step": tm, "rows_per_s": n / tm, "alg_bytes": alg, "alg_GBps": alg / tm / 1e9,
+           "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
+    days.free()
+    out.free()
     return res
 
 
@@ -296,6 +316,9 @@ def main():
             ops_out["q1_orderkey" if by_key else "q1_flags"] = bench_q1(ctx, max(3, args.steps // 2), 1, by_key)
         log("bench: TPC-H Q6 shape 6e7")
         ops_out["q6"] = bench_q6(ctx, max(3, args.steps // 2), 1)
+    if args.only in ("", "all", "ingest"):
+        log("bench: date32 to packed dates 6e7")
+        ops_out["date32_pack_6e7"] = bench_date32(ctx, args.steps, args.warmup)
     if args.only in ("", "all", "serve"):
         log("bench: serving mixed stream")
         ops_out["serve_mixed_stream"] = bench_serve(ctx)

@@ -136,7 +136,35 @@ int pimdal_col_free(pimdal_ctx* ctx, pimdal_col* col);
  * buffers owned by `out`; the caller calls out->release and
  * out_schema->release. */
 int pimdal_col_import(pimdal_ctx* ctx, const struct ArrowArray* array, const struct ArrowSchema* schema, pimdal_col** out);
+/* Chunked column (an arrow::ChunkedArray, as a Parquet reader returns one
+ * chunk per row group): the `nchunks` arrays, all of `schema`'s type and
+ * none with nulls, concatenated in order into one device column.  Widens
+ * the single-chunk input of the reference operators
+ * (reference/db_operators/select.cpp:22 takes chunk(0)) and of dist_table
+ * (pimdal/support/transfer_helper.h:87-283).  Every chunk is checked
+ * before anything is copied.  The Arrow date32 format ("tdD", int32 days
+ * since 1970-01-01) imports as PIMDAL_I32. */
+int pimdal_col_import_chunks(pimdal_ctx* ctx, const struct ArrowArray* const* chunks, uint32_t nchunks,
+                             const struct ArrowSchema* schema, pimdal_col** out);
 int pimdal_col_export(pimdal_ctx* ctx, const pimdal_col* col, struct ArrowArray* out, struct ArrowSchema* out_schema);
+
+/* ---------------- dates ----------------
+ * The packed date of pimdal/main/tpc_h/reader/read_table.cpp:8-16
+ * (date_to_int): day + (month << 5) + ((year - 1900) << 9), the form the
+ * TPC-H programs compare their date columns and literals in
+ * (pimdal/main/tpc_h/query6/host/host_q6.cpp:46-47, query1/host/host_q1.cpp:49).
+ * pimdal_date_pack: from numbers (month 1..12, day 1..31, as strptime's
+ * %m and %d accept them; the day is not checked against the month).
+ * pimdal_date_pack_str: from "YYYY-MM-DD" (a 1-4 digit year, 1-2 digit month
+ * and day; surrounding blanks allowed), the literal form date_to_int takes;
+ * a malformed string is PIMDAL_E_INVALID (the reference leaves the result
+ * undefined).
+ * pimdal_date32_pack_dev: a device column of Arrow date32 values (int32 days
+ * since 1970-01-01, proleptic Gregorian) to packed dates, n values,
+ * in place allowed (out == days).  Asynchronous. */
+int pimdal_date_pack(int32_t year, int32_t month, int32_t day, int32_t* out);
+int pimdal_date_pack_str(const char* yyyy_mm_dd, int32_t* out);
+int pimdal_date32_pack_dev(pimdal_ctx* ctx, const int32_t* days, uint64_t n, int32_t* out);
 
 /* ---------------- selection ----------------
  * reference/db_operators/select.cpp:77 select(table, column) and

@@ -22,6 +22,7 @@
  */
 #include <stdint.h>
 #include <string.h>
+#include <time.h>
 #include <algorithm>
 #include <vector>
 #include <unordered_map>
@@ -580,6 +581,40 @@ extern "C" uint64_t orc_sort_aggr_u32_i64(const uint32_t* key, const int64_t* va
         i = j;
     }
     return g;
+}
+
+/* ------------------------------------------------------------------ */
+/* Packed dates: pimdal/main/tpc_h/reader/read_table.cpp:8-16         */
+/* (date_to_int).  The reference fills a struct tm with strptime of a */
+/* "YYYY-MM-DD" literal and packs tm_mday + ((tm_mon + 1) << 5) +     */
+/* (tm_year << 9).  The same fields come from the C library here:     */
+/* strptime for literals, gmtime_r for date32 values (days since      */
+/* 1970-01-01), then the same packing, summed modulo 2^32 (what the   */
+/* reference's int arithmetic gives for years too far from 1900).     */
+/* ------------------------------------------------------------------ */
+
+static int32_t pack_tm(const struct tm& tm) {
+    return (int32_t)((uint32_t)tm.tm_mday + ((uint32_t)(tm.tm_mon + 1) << 5) + ((uint32_t)tm.tm_year << 9));
+}
+
+/* Returns 0 and the packed date, or -1 when strptime rejects the text. */
+extern "C" int orc_date_pack_str(const char* s, int32_t* out) {
+    struct tm tm;
+    memset(&tm, 0, sizeof tm);
+    const char* end = strptime(s, "%Y-%m-%d", &tm);
+    if (!end) return -1;
+    *out = pack_tm(tm);
+    return 0;
+}
+
+extern "C" void orc_date32_pack(const int32_t* days, uint64_t n, int32_t* out) {
+    #pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; i++) {
+        const time_t t = (time_t)days[i] * 86400;
+        struct tm tm;
+        gmtime_r(&t, &tm);
+        out[i] = pack_tm(tm);
+    }
 }
 
 extern "C" int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -41,6 +41,8 @@ _SIG = {
     "orc_q1_flags": (u64, [vp, vp, vp, vp, vp, vp, vp, u64, i32, u64, vp, vp, vp, vp, vp, vp, vp]),
     "orc_q1_key": (u64, [vp, vp, vp, vp, vp, vp, u64, i32, u64, vp, vp, vp, vp, vp, vp, vp]),
     "orc_q6": (u64, [vp, vp, vp, vp, u64, i32, i32, i64, i64, i64, vp]),
+    "orc_date_pack_str": (C.c_int, [C.c_char_p, vp]),
+    "orc_date32_pack": (None, [vp, u64, vp]),
     "orc_num_threads": (C.c_int, []),
     "orc_set_threads": (None, [C.c_int]),
 }
@@ -262,6 +264,20 @@ def q6(t, date_lo=731, date_hi=1096, disc_lo=5, disc_hi=7, qty_lt=24):
     cnt = lib().orc_q6(_p(c["date"]), _p(c["quantity"]), _p(c["price"]), _p(c["discount"]), c["date"].size,
                        date_lo, date_hi, disc_lo, disc_hi, qty_lt, C.byref(rev))
     return rev.value, cnt
+
+
+def date_pack_str(text: str):
+    """Packed date of a "YYYY-MM-DD" literal via strptime, or None when strptime rejects it."""
+    out = C.c_int32()
+    return out.value if lib().orc_date_pack_str(text.encode(), C.byref(out)) == 0 else None
+
+
+def date32_pack(days):
+    """Packed dates of date32 values (int32 days since 1970-01-01)."""
+    days = np.ascontiguousarray(days, dtype=np.int32)
+    out = np.empty(days.size, np.int32)
+    lib().orc_date32_pack(_p(days), days.size, _p(out))
+    return out
 
 
 def num_threads() -> int:

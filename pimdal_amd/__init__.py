@@ -6,6 +6,6 @@ for tests, benchmarks and Python callers.
 """
 from ._lib import PimdalError, load  # noqa: F401
 from .device import Context, DeviceArray  # noqa: F401
-from . import operators, datagen, serving  # noqa: F401
+from . import operators, datagen, serving, ingest  # noqa: F401
 
-__all__ = ["Context", "DeviceArray", "PimdalError", "load", "operators", "datagen", "serving"]
+__all__ = ["Context", "DeviceArray", "PimdalError", "load", "operators", "datagen", "serving", "ingest"]

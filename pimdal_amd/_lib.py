@@ -67,6 +67,10 @@ SIGNATURES: dict[str, list] = {
     "pimdal_col_info": [c_col_p, C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(vp)],
     "pimdal_col_free": [c_ctx_p, c_col_p],
     "pimdal_col_import": [c_ctx_p, vp, vp, C.POINTER(c_col_p)],
+    "pimdal_col_import_chunks": [c_ctx_p, vp, u32, vp, C.POINTER(c_col_p)],
+    "pimdal_date_pack": [i32, i32, i32, C.POINTER(i32)],
+    "pimdal_date_pack_str": [C.c_char_p, C.POINTER(i32)],
+    "pimdal_date32_pack_dev": [c_ctx_p, vp, u64, vp],
     "pimdal_col_export": [c_ctx_p, c_col_p, vp, vp],
     "pimdal_select": [c_ctx_p, c_col_p, c_col_p, i64, i64, C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_select_dev": [c_ctx_p, C.c_int, vp, vp, u64, i64, i64, vp, vp, vp],

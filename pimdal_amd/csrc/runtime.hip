@@ -519,6 +519,7 @@ static bool arrow_type(const char* fmt, pimdal_type* t) {
     if (!fmt) return false;
     if (!strcmp(fmt, "C")) { *t = PIMDAL_U8; return true; }
     if (!strcmp(fmt, "i")) { *t = PIMDAL_I32; return true; }
+    if (!strcmp(fmt, "tdD")) { *t = PIMDAL_I32; return true; }   // date32: int32 days since 1970-01-01
     if (!strcmp(fmt, "I")) { *t = PIMDAL_U32; return true; }
     if (!strcmp(fmt, "l")) { *t = PIMDAL_I64; return true; }
     if (!strcmp(fmt, "L")) { *t = PIMDAL_U64; return true; }
@@ -538,27 +539,61 @@ static const char* arrow_format(pimdal_type t) {
     return "";
 }
 
-extern "C" int pimdal_col_import(pimdal_ctx* c, const struct ArrowArray* a, const struct ArrowSchema* s, pimdal_col** out) {
-    PD_API_BEGIN
-    PD_REQUIRE(c && a && s && out, PIMDAL_E_INVALID, "NULL argument");
-    PD_REQUIRE(a->release && s->release, PIMDAL_E_INVALID, "released Arrow structure");
+// Type of an importable Arrow schema (primitive, fixed width, no children).
+static pimdal_type import_type(const struct ArrowSchema* s) {
+    PD_REQUIRE(s->release, PIMDAL_E_INVALID, "released Arrow schema");
     pimdal_type t;
     PD_REQUIRE(arrow_type(s->format, &t), PIMDAL_E_UNSUPPORTED,
                std::string("unsupported Arrow format '") + (s->format ? s->format : "") + "'");
-    PD_REQUIRE(a->n_children == 0 && s->n_children == 0 && !a->dictionary, PIMDAL_E_UNSUPPORTED,
+    PD_REQUIRE(s->n_children == 0 && !s->dictionary, PIMDAL_E_UNSUPPORTED,
+               "nested or dictionary arrays are not supported");
+    return t;
+}
+
+// Host address of an importable array's first value (its offset applied).
+static const char* import_values(const struct ArrowArray* a, uint32_t w) {
+    PD_REQUIRE(a, PIMDAL_E_INVALID, "NULL Arrow array");
+    PD_REQUIRE(a->release, PIMDAL_E_INVALID, "released Arrow array");
+    PD_REQUIRE(a->n_children == 0 && !a->dictionary, PIMDAL_E_UNSUPPORTED,
                "nested or dictionary arrays are not supported");
     PD_REQUIRE(a->n_buffers == 2, PIMDAL_E_UNSUPPORTED, "expected a primitive array with 2 buffers");
     PD_REQUIRE(a->null_count == 0 || (a->null_count == -1 && a->buffers[0] == nullptr),
                PIMDAL_E_UNSUPPORTED, "nullable columns are not supported (null_count != 0)");
     PD_REQUIRE(a->length >= 0 && a->offset >= 0, PIMDAL_E_INVALID, "negative length or offset");
-    const uint32_t w = type_width(t);
     const char* base = (const char*)a->buffers[1];
     PD_REQUIRE(base || a->length == 0, PIMDAL_E_INVALID, "missing data buffer");
+    return base ? base + (uint64_t)a->offset * w : nullptr;
+}
+
+extern "C" int pimdal_col_import(pimdal_ctx* c, const struct ArrowArray* a, const struct ArrowSchema* s, pimdal_col** out) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && a && s && out, PIMDAL_E_INVALID, "NULL argument");
+    return pimdal_col_import_chunks(c, &a, 1, s, out);
+    PD_API_END
+}
+
+extern "C" int pimdal_col_import_chunks(pimdal_ctx* c, const struct ArrowArray* const* chunks, uint32_t nchunks,
+                                        const struct ArrowSchema* s, pimdal_col** out) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && s && out && (chunks || nchunks == 0), PIMDAL_E_INVALID, "NULL argument");
+    const pimdal_type t = import_type(s);
+    const uint32_t w = type_width(t);
+    // every chunk is checked before anything is allocated or copied
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nchunks; i++) {
+        import_values(chunks[i], w);
+        total += (uint64_t)chunks[i]->length;
+    }
     pimdal_col* col = nullptr;
-    int rc = pimdal_col_alloc(c, t, (uint64_t)a->length, &col);
+    int rc = pimdal_col_alloc(c, t, total, &col);
     if (rc) fail(rc, pimdal_last_error());
     try {
-        stage_h2d(c, col->data, base + (uint64_t)a->offset * w, (uint64_t)a->length * w);
+        uint64_t at = 0;
+        for (uint32_t i = 0; i < nchunks; i++) {
+            const uint64_t len = (uint64_t)chunks[i]->length;
+            if (len) stage_h2d(c, (char*)col->data + at * w, import_values(chunks[i], w), len * w);
+            at += len;
+        }
     } catch (...) {
         pimdal_col_free(c, col);
         throw;

@@ -127,6 +127,32 @@ def q6(ctx: Context, t: dict, date_lo: int = 731, date_hi: int = 1096, disc_lo: 
     return rev.value, cnt.value
 
 
+def date_pack(year: int, month: int, day: int) -> int:
+    """Packed date day + (month << 5) + ((year - 1900) << 9):
+    pimdal/main/tpc_h/reader/read_table.cpp:8-16 (date_to_int)."""
+    out = C.c_int32()
+    call("pimdal_date_pack", int(year), int(month), int(day), C.byref(out))
+    return out.value
+
+
+def date_pack_str(text: str) -> int:
+    """Packed date of a "YYYY-MM-DD" literal (read_table.cpp:8-16)."""
+    out = C.c_int32()
+    call("pimdal_date_pack_str", text.encode(), C.byref(out))
+    return out.value
+
+
+def date32_pack(ctx: Context, days: DeviceArray, out: DeviceArray | None = None) -> DeviceArray:
+    """Arrow date32 values (int32 days since 1970-01-01) to packed dates on
+    the device; `out` may be `days` itself (in place)."""
+    if days.dtype != np.int32:
+        raise TypeError("date32 values are int32")
+    if out is None:
+        out = ctx.empty(days.n, np.int32)
+    call("pimdal_date32_pack_dev", ctx.handle, days.vp, days.n, out.vp)
+    return out
+
+
 class _Q1Result(C.Structure):
     _fields_ = [(f, C.c_void_p) for f in ("key", "sum_qty", "sum_price", "sum_disc", "sum_disc_price",
                                           "sum_charge", "count")]

@@ -262,3 +262,48 @@ def test_q6_matches_numpy_at_config_size_sample(orc):
     m = ((t["date"] >= 731) & (t["date"] < 1096) & (t["quantity"] < 24) & (t["discount"] >= 5) & (t["discount"] <= 7))
     rev = int((t["price"][m].astype(np.uint64) * t["discount"][m].astype(np.uint64)).sum(dtype=np.uint64))
     assert orc.q6(t) == (rev - (1 << 64) if rev >= 1 << 63 else rev, int(m.sum()))
+
+
+# ---------------- packed dates ----------------
+# pimdal/main/tpc_h/reader/read_table.cpp:8-16: the calendar fields come from
+# the C library (strptime / gmtime_r) in the oracle; Python's datetime is an
+# independent calendar.  The packing expression is the reference's own
+# (day + (month << 5) + ((year - 1900) << 9)); its decode below recovers the
+# fields, so a slip in a shift or in the month base fails.
+
+def _py_pack(d):
+    return d.day + (d.month << 5) + ((d.year - 1900) << 9)
+
+
+def _unpack(p):
+    p = int(p)
+    return (p >> 9) + 1900, (p >> 5) & 15, p & 31
+
+
+def test_date32_pack_matches_python_calendar(orc):
+    import datetime as dt
+    epoch = dt.date(1970, 1, 1)
+    lo, hi = (dt.date(1, 1, 1) - epoch).days, (dt.date(9999, 12, 31) - epoch).days
+    rng = np.random.default_rng(11)
+    days = np.concatenate([rng.integers(lo, hi + 1, 20000), np.arange(-800, 800),
+                           [lo, hi, 0, -1, 1, (dt.date(1900, 2, 28) - epoch).days, (dt.date(1900, 3, 1) - epoch).days,
+                            (dt.date(2000, 2, 29) - epoch).days, (dt.date(2100, 3, 1) - epoch).days]]).astype(np.int32)
+    got = orc.date32_pack(days)
+    for x, p in zip(days.tolist(), got.tolist()):
+        d = epoch + dt.timedelta(days=x)
+        assert p == _py_pack(d), (x, d, p)
+        assert _unpack(p) == (d.year, d.month, d.day)
+
+
+def test_date32_pack_is_order_preserving(orc):
+    # the TPC-H predicates compare packed dates: later date, larger value
+    days = np.arange(-30000, 60000, dtype=np.int32)
+    assert np.all(np.diff(orc.date32_pack(days).astype(np.int64)) > 0)
+
+
+def test_date_pack_str_reference_literals(orc):
+    # the literals of pimdal/main/tpc_h/query{1,4,5,6}/host/host_q*.cpp
+    import datetime as dt
+    for s in ["1998-09-02", "1993-07-01", "1993-10-01", "1993-08-01", "1993-11-01", "1994-01-01", "1995-01-01"]:
+        assert orc.date_pack_str(s) == _py_pack(dt.date.fromisoformat(s))
+    assert orc.date_pack_str("not a date") is None
