@@ -9,7 +9,7 @@ operators' rows/s at their configured sizes, the dominant kernel's roofline
 a bounded sample on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--only sort|zipf|sortselect|join|groupby|ingest|serve]
+                  [--only sort|zipf|sortselect|join|bloom|groupby|ingest|serve]
 """
 from __future__ import annotations
 
@@ -118,7 +118,7 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
-KERNELS = ("date32_pack", "part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
+KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
            "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
@@ -199,6 +199,37 @@ def bench_date32(ctx, steps, warmup, n=60_000_000):
            "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
     days.free()
     out.free()
+    return res
+
+
+def bench_bloom(ctx, steps, warmup, nb, part_bits, npr=1 << 27):
+    """Bloom-filter semi-join: nb build keys into 2^part_bits filters, then a
+    probe of npr keys with payloads (a quarter of them build keys)."""
+    from pimdal_amd import operators as ops
+    from pimdal_amd._lib import call
+    import ctypes as C
+    bk = ctx.empty(nb, "uint32")
+    call("pimdal_gen_u32", ctx.handle, 0xB10, 1, nb, bk.vp)
+    pk, pp = ctx.empty(npr, "uint32"), ctx.empty(npr, "uint32")
+    call("pimdal_gen_u32", ctx.handle, 0xB10, 2, npr, pk.vp)
+    call("pimdal_gen_u32", ctx.handle, 0xB10, 3, npr, pp.vp)
+    for q in range(0, npr, 4 * nb):       # every 4th block of nb probe keys repeats the build keys
+        m = min(nb, npr - q)
+        call("pimdal_dev_copy", ctx.handle, C.c_void_p(pk.ptr + 4 * q), bk.vp, 4 * m)
+    op, orow, cnt = ctx.empty(npr, "uint32"), ctx.empty(npr, "uint32"), ctx.empty(1, "uint64")
+
+    def step():
+        call("pimdal_bloom_semi_join_dev", ctx.handle, bk.vp, nb, pk.vp, pp.vp, npr, part_bits, 32, op.vp,
+             orow.vp, cnt.vp)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    kept = int(cnt.numpy()[0])
+    alg = 4 * nb + 8 * npr + 8 * kept    # build keys; probe key + payload; kept payload + row id
+    res = {"build_rows": nb, "probe_rows": npr, "filters": 1 << part_bits, "kept": kept, "s_per_step": tm,
+           "rows_per_s": (nb + npr) / tm, "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
+           "kernels": kernel_breakdown(ctx, steps)}
+    for a in (bk, pk, pp, op, orow, cnt):
+        a.free()
     return res
 
 
@@ -310,6 +341,10 @@ def main():
             ops_out[f"hash_join_build2^20_mode{mode}"] = bench_join(ctx, max(3, args.steps // 2), 1, "hash", False,
                                                                     nr=1 << 20)
         ctx.set_tune("hj_mode", -1)
+    if args.only in ("", "all", "bloom"):
+        for nb, bits in ((1 << 16, 5), (1 << 20, 9)):
+            log(f"bench: bloom semi-join 2^{nb.bit_length() - 1} x 2^27, {1 << bits} filters")
+            ops_out[f"bloom_semi_join_2^{nb.bit_length() - 1}x2^27"] = bench_bloom(ctx, args.steps, args.warmup, nb, bits)
     if args.only in ("", "all", "groupby"):
         for by_key in (False, True):
             log(f"bench: filter+group-by 6e7 by_key={by_key}")

@@ -227,6 +227,34 @@ int pimdal_sort_merge_join_dev(pimdal_ctx* ctx, const uint32_t* build_key, const
                                uint64_t n_probe, uint32_t* out_probe_payload, uint32_t* out_build_payload,
                                uint64_t out_capacity, uint64_t* out_count);
 
+/* ---------------- Bloom-filter semi-join ----------------
+ * The PIM library's filter_kernel / match_kernel (pimdal/join/hash_join.c:
+ * 117-152, 237-341; insert_bloom_filter / probe_bloom_filter,
+ * pimdal/hash/hash_steps.c:143-183; mixers pimdal/hash/hash_func.h:5-58).
+ * Keys fall into 2^part_bits partitions by bits [shift - part_bits, shift)
+ * of hash0(key) (part_kernel's MSB-first radix partition, whose `shift` is
+ * this argument; part_bits <= 16, part_bits <= shift <= 32).  Each
+ * partition has a 16384-bit filter (2048 bytes, the reference's byte
+ * layout) in which hash1..hash4(key) % 16384 are set.
+ * pimdal_bloom_build_dev: the filters of n build keys into `filters`
+ * (2^part_bits x 2048 bytes, overwritten).
+ * pimdal_bloom_probe_dev: the probe rows whose four bits are all set in
+ * their partition's filter: every row whose key is a build key, plus the
+ * filter's false positives, exactly as the reference's filters decide.
+ * Writes their payload (the key when payload is NULL) and row index in row
+ * order (the reference emits them partition by partition); *d_count
+ * (device) receives the count.  out_payload / out_rowid hold n rows or are
+ * NULL.  pimdal_bloom_semi_join_dev: both, with the filters in scratch
+ * memory.  Asynchronous. */
+int pimdal_bloom_build_dev(pimdal_ctx* ctx, const uint32_t* keys, uint64_t n, uint32_t part_bits, uint32_t shift,
+                           uint32_t* filters);
+int pimdal_bloom_probe_dev(pimdal_ctx* ctx, const uint32_t* keys, const uint32_t* payload, uint64_t n,
+                           uint32_t part_bits, uint32_t shift, const uint32_t* filters, uint32_t* out_payload,
+                           uint32_t* out_rowid, uint64_t* d_count);
+int pimdal_bloom_semi_join_dev(pimdal_ctx* ctx, const uint32_t* build_keys, uint64_t nb, const uint32_t* probe_keys,
+                               const uint32_t* probe_payload, uint64_t np, uint32_t part_bits, uint32_t shift,
+                               uint32_t* out_payload, uint32_t* out_rowid, uint64_t* d_count);
+
 /* ---------------- aggregation ----------------
  * Hash aggregation: reference/db_operators/aggregate.cpp:95 hash_aggr(table,
  * column) and pimdal/aggregate/aggregate.h:25 group_kernel (hash variant,

@@ -617,6 +617,82 @@ extern "C" void orc_date32_pack(const int32_t* days, uint64_t n, int32_t* out) {
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Bloom-filter semi-join of the PIM library: partition by hash0 bits */
+/* (pimdal/join/hash_join.c:154-235 part_kernel; bucket =            */
+/* (hash0(key) >> shift) & (buckets - 1) per level, MSB first, so the */
+/* final partition is bits [shift - part_bits, shift) of hash0),      */
+/* one 16384-bit filter per partition set by hash1..hash4 (filter_    */
+/* kernel, hash_join.c:117-152; insert_bloom_filter,                  */
+/* pimdal/hash/hash_steps.c:143-152), and a probe that keeps an       */
+/* element when all four bits are set (match_kernel, hash_join.c:237- */
+/* 341; probe_bloom_filter, hash_steps.c:161-183).  The mixers are    */
+/* pimdal/hash/hash_func.h:5-58.                                      */
+/* Parity unpinned for the mixer constants (the filter bit positions):*/
+/* no reference run checks them (the DPU sources need the UPMEM SDK); */
+/* the tests pin the filter's invariants instead.                     */
+/* ------------------------------------------------------------------ */
+
+static uint32_t pim_mix(uint32_t key, uint32_t add, int s1, int s2, int s3, int s4) {
+    key += add;
+    key += ~(key << s1);
+    key ^= (key >> s2);
+    key += (key << s3);
+    key ^= (key >> s4);
+    return key;
+}
+static uint32_t pim_hash0(uint32_t k) { return pim_mix(k, 272333, 5, 18, 11, 17); }
+static uint32_t pim_bloom_hash(int f, uint32_t k) {
+    switch (f) {
+        case 0: return pim_mix(k, 593627, 22, 26, 10, 28);   /* hash1 */
+        case 1: return pim_mix(k, 274277, 13, 20, 3, 9);     /* hash2 */
+        case 2: return pim_mix(k, 855467, 4, 21, 5, 16);     /* hash3 */
+        default: return pim_mix(k, 176521, 10, 25, 5, 24);   /* hash4 */
+    }
+}
+constexpr uint32_t kFilterBits = 16384;   /* hash_steps.c:15 FILTER_SIZE */
+
+static uint32_t pim_part(uint32_t key, uint32_t part_bits, uint32_t shift) {
+    return part_bits ? (pim_hash0(key) >> (shift - part_bits)) & ((1u << part_bits) - 1) : 0;
+}
+
+/* filters: (1 << part_bits) * 2048 bytes, written whole. */
+extern "C" void orc_bloom_build(const uint32_t* keys, uint64_t n, uint32_t part_bits, uint32_t shift,
+                                uint8_t* filters) {
+    const uint64_t fb = kFilterBits / 8;
+    memset(filters, 0, ((uint64_t)1 << part_bits) * fb);
+    for (uint64_t i = 0; i < n; i++) {
+        uint8_t* f = filters + pim_part(keys[i], part_bits, shift) * fb;
+        for (int h = 0; h < 4; h++) {
+            const uint32_t bit = pim_bloom_hash(h, keys[i]) % kFilterBits;
+            f[bit / 8] |= (uint8_t)(1u << (bit % 8));
+        }
+    }
+}
+
+/* Kept probe elements in probe order: payload and row index; returns the count. */
+extern "C" uint64_t orc_bloom_probe(const uint32_t* keys, const uint32_t* pay, uint64_t n, uint32_t part_bits,
+                                    uint32_t shift, const uint8_t* filters, uint32_t* out_pay, uint32_t* out_row) {
+    const uint64_t fb = kFilterBits / 8;
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint8_t* f = filters + pim_part(keys[i], part_bits, shift) * fb;
+        bool hit = true;
+        for (int h = 0; h < 4 && hit; h++) {
+            const uint32_t bit = pim_bloom_hash(h, keys[i]) % kFilterBits;
+            hit = (f[bit / 8] >> (bit % 8)) & 1;
+        }
+        if (hit) {
+            if (out_pay) out_pay[k] = pay ? pay[i] : keys[i];
+            if (out_row) out_row[k] = (uint32_t)i;
+            k++;
+        }
+    }
+    return k;
+}
+
+extern "C" uint32_t orc_pim_hash(int which, uint32_t key) { return which == 0 ? pim_hash0(key) : pim_bloom_hash(which - 1, key); }
+
 extern "C" int orc_num_threads(void) { return omp_get_max_threads(); }
 extern "C" void orc_set_threads(int t) { omp_set_num_threads(t); }
 

@@ -43,6 +43,9 @@ _SIG = {
     "orc_q6": (u64, [vp, vp, vp, vp, u64, i32, i32, i64, i64, i64, vp]),
     "orc_date_pack_str": (C.c_int, [C.c_char_p, vp]),
     "orc_date32_pack": (None, [vp, u64, vp]),
+    "orc_bloom_build": (None, [vp, u64, u32, u32, vp]),
+    "orc_bloom_probe": (u64, [vp, vp, u64, u32, u32, vp, vp, vp]),
+    "orc_pim_hash": (u32, [C.c_int, u32]),
     "orc_num_threads": (C.c_int, []),
     "orc_set_threads": (None, [C.c_int]),
 }
@@ -278,6 +281,29 @@ def date32_pack(days):
     out = np.empty(days.size, np.int32)
     lib().orc_date32_pack(_p(days), days.size, _p(out))
     return out
+
+
+def bloom_build(keys, part_bits, shift):
+    """Bloom filters of the build keys: (1 << part_bits) x 2048 bytes."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    out = np.empty((1 << part_bits) * 2048, np.uint8)
+    lib().orc_bloom_build(_p(keys), keys.size, part_bits, shift, _p(out))
+    return out
+
+
+def bloom_probe(keys, pay, part_bits, shift, filters):
+    """(payload, row index) of the probe rows the filters keep, in row order."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    pay = None if pay is None else np.ascontiguousarray(pay, dtype=np.uint32)
+    op = np.empty(keys.size, np.uint32)
+    orow = np.empty(keys.size, np.uint32)
+    k = lib().orc_bloom_probe(_p(keys), _p(pay), keys.size, part_bits, shift, _p(filters), _p(op), _p(orow))
+    return op[:k], orow[:k]
+
+
+def pim_hash(which, key):
+    """hash0 (which = 0) or hash1..hash4 of pimdal/hash/hash_func.h."""
+    return lib().orc_pim_hash(which, key)
 
 
 def num_threads() -> int:

@@ -80,6 +80,9 @@ SIGNATURES: dict[str, list] = {
     "pimdal_hash_join_dev": [c_ctx_p, vp, vp, u64, vp, vp, u64, vp, vp, u64, C.POINTER(u64)],
     "pimdal_sort_merge_join": [c_ctx_p, c_col_p, c_col_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_sort_merge_join_dev": [c_ctx_p, vp, vp, u64, vp, vp, u64, vp, vp, u64, C.POINTER(u64)],
+    "pimdal_bloom_build_dev": [c_ctx_p, vp, u64, u32, u32, vp],
+    "pimdal_bloom_probe_dev": [c_ctx_p, vp, vp, u64, u32, u32, vp, vp, vp, vp],
+    "pimdal_bloom_semi_join_dev": [c_ctx_p, vp, u64, vp, vp, u64, u32, u32, vp, vp, vp],
     "pimdal_hash_aggr_dev": [c_ctx_p, vp, vp, u64, vp, vp, vp, u64, C.POINTER(u64)],
     "pimdal_sort_aggr_dev": [c_ctx_p, vp, vp, u64, vp, vp, vp, u64, C.POINTER(u64)],
     "pimdal_hash_aggr": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p), C.POINTER(c_col_p)],

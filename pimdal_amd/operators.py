@@ -127,6 +127,36 @@ def q6(ctx: Context, t: dict, date_lo: int = 731, date_hi: int = 1096, disc_lo: 
     return rev.value, cnt.value
 
 
+def bloom_build(ctx: Context, keys: DeviceArray, part_bits: int, shift: int) -> DeviceArray:
+    """Bloom filters of the build keys, (1 << part_bits) x 2048 bytes (as uint32 words).
+    pimdal/join/hash_join.c:117-152 (filter_kernel)."""
+    filters = ctx.empty(512 << part_bits, np.uint32)
+    call("pimdal_bloom_build_dev", ctx.handle, keys.vp, keys.n, int(part_bits), int(shift), filters.vp)
+    return filters
+
+
+def bloom_probe(ctx: Context, keys: DeviceArray, payload: DeviceArray | None, part_bits: int, shift: int,
+                filters: DeviceArray):
+    """Probe rows the filters keep, in row order: (payload, rowid, count).
+    pimdal/join/hash_join.c:237-341 (match_kernel)."""
+    n = keys.n
+    out_p, out_r, cnt = ctx.empty(n, np.uint32), ctx.empty(n, np.uint32), ctx.empty(1, np.uint64)
+    call("pimdal_bloom_probe_dev", ctx.handle, keys.vp, payload.vp if payload is not None else None, n,
+         int(part_bits), int(shift), filters.vp, out_p.vp, out_r.vp, cnt.vp)
+    return out_p, out_r, _read_count(ctx, cnt)
+
+
+def bloom_semi_join(ctx: Context, build_keys: DeviceArray, probe_keys: DeviceArray, probe_payload: DeviceArray | None,
+                    part_bits: int, shift: int):
+    """Build the filters and probe them in one call: (payload, rowid, count)."""
+    n = probe_keys.n
+    out_p, out_r, cnt = ctx.empty(n, np.uint32), ctx.empty(n, np.uint32), ctx.empty(1, np.uint64)
+    call("pimdal_bloom_semi_join_dev", ctx.handle, build_keys.vp, build_keys.n, probe_keys.vp,
+         probe_payload.vp if probe_payload is not None else None, n, int(part_bits), int(shift), out_p.vp,
+         out_r.vp, cnt.vp)
+    return out_p, out_r, _read_count(ctx, cnt)
+
+
 def date_pack(year: int, month: int, day: int) -> int:
     """Packed date day + (month << 5) + ((year - 1900) << 9):
     pimdal/main/tpc_h/reader/read_table.cpp:8-16 (date_to_int)."""

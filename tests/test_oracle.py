@@ -307,3 +307,37 @@ def test_date_pack_str_reference_literals(orc):
     for s in ["1998-09-02", "1993-07-01", "1993-10-01", "1993-08-01", "1993-11-01", "1994-01-01", "1995-01-01"]:
         assert orc.date_pack_str(s) == _py_pack(dt.date.fromisoformat(s))
     assert orc.date_pack_str("not a date") is None
+
+
+# ---------------- Bloom-filter semi-join ----------------
+# pimdal/join/hash_join.c:117-152, 237-341; pimdal/hash/hash_steps.c:143-183.
+# No reference run pins the filter bit positions (the DPU sources need the
+# UPMEM SDK).  What the mathematics fixes is checked instead: no false
+# negatives, a false-positive rate near (1 - e^(-4n/m))^4, a build key only
+# touches its own partition's filter, and at most 4 bits per key.
+
+def test_bloom_no_false_negatives_and_fp_rate(orc):
+    rng = np.random.default_rng(21)
+    keys = np.unique(rng.integers(0, 1 << 32, 2000, dtype=np.uint64)).astype(np.uint32)
+    filt = orc.bloom_build(keys, 0, 32)
+    assert int(np.unpackbits(filt).sum()) <= 4 * keys.size
+    probe = rng.integers(0, 1 << 32, 200_000, dtype=np.uint64).astype(np.uint32)
+    probe = probe[~np.isin(probe, keys)]
+    _, rows = orc.bloom_probe(np.concatenate([keys, probe]), None, 0, 32, filt)
+    assert np.array_equal(rows[: keys.size], np.arange(keys.size))        # every build key is kept
+    fp = (rows.size - keys.size) / probe.size
+    theory = (1 - np.exp(-4 * keys.size / 16384)) ** 4
+    assert 0.5 * theory < fp < 2 * theory, (fp, theory)
+
+
+def test_bloom_partitions_are_isolated(orc):
+    part_bits, shift = 4, 20
+    rng = np.random.default_rng(22)
+    keys = rng.integers(0, 1 << 32, 50_000, dtype=np.uint64).astype(np.uint32)
+    part = np.array([(orc.pim_hash(0, int(k)) >> (shift - part_bits)) & 15 for k in keys[:4000]])
+    sel = keys[:4000][part == 7]
+    filt = orc.bloom_build(sel, part_bits, shift).reshape(16, 2048)
+    assert filt[7].any() and not np.delete(filt, 7, axis=0).any()
+    # probes that fall in another partition see an empty filter: nothing kept
+    other = keys[:4000][part != 7]
+    assert orc.bloom_probe(other, None, part_bits, shift, filt.ravel())[1].size == 0
