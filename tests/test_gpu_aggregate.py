@@ -2,6 +2,7 @@
 import numpy as np
 import pytest
 
+import pimdal_amd
 from pimdal_amd import operators as ops, datagen
 from tests.util import canon
 
@@ -132,6 +133,42 @@ def test_q1_by_orderkey_one_row_beyond_packed_width(ctx, orc, col, val):
     d = {c: ctx.to_device(v) for c, v in t.items()}
     got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
     _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
+@pytest.mark.parametrize("sign", [1, -1])
+def test_q1_by_orderkey_extreme_packed_charge_sums(ctx, orc, sign):
+    """Groups of about 60000 rows at the extremes of the packed widths: the
+    128-bit charge sums (about 6.7e15, negative for sign -1) of LDS tables,
+    carries and sign words included."""
+    n = 120_000
+    t = orc.fact_table(n, groups=2)
+    t["price"][:] = (1 << 31) - 1 if sign > 0 else -(1 << 31)
+    t["discount"][:] = -128
+    t["tax"][:] = 127
+    t["quantity"][:] = 32767
+    d = {c: ctx.to_device(v) for c, v in t.items()}
+    got = ops.filter_groupby_q1(ctx, d, 2505, by_key=True)
+    _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
+
+
+def test_q1_by_orderkey_charge_beyond_int64_in_one_group(orc):
+    """One group of 100000 extreme rows with the LDS row limit raised to
+    2^20: its charge sum (about 1.1e19) passes 2^63 inside one LDS table, so
+    the carry into the high word must be exact.  Own context, so the knob
+    does not leak."""
+    n = 100_000
+    t = orc.fact_table(n, groups=1)
+    t["price"][:] = (1 << 31) - 1
+    t["discount"][:] = -128
+    t["tax"][:] = 127
+    t["date"][:] = 0
+    with pimdal_amd.Context(0) as c2:
+        c2.set_tune("gb_lds_max_rows", 1 << 20)
+        d = {c: c2.to_device(v) for c, v in t.items()}
+        got = ops.filter_groupby_q1(c2, d, 2505, by_key=True)
+    exp = orc.q1_key(t, 2505)
+    assert exp["sum_charge"][0] > (1 << 63)
+    _cmp_q1(got, exp, ordered=False)
 
 
 def test_q1_by_orderkey_few_hot_keys(ctx, orc):
