@@ -628,9 +628,8 @@ extern "C" void orc_date32_pack(const int32_t* days, uint64_t n, int32_t* out) {
 /* element when all four bits are set (match_kernel, hash_join.c:237- */
 /* 341; probe_bloom_filter, hash_steps.c:161-183).  The mixers are    */
 /* pimdal/hash/hash_func.h:5-58.                                      */
-/* Parity unpinned for the mixer constants (the filter bit positions):*/
-/* no reference run checks them (the DPU sources need the UPMEM SDK); */
-/* the tests pin the filter's invariants instead.                     */
+/* The mixers are pinned against hash_func.h itself, compiled into   */
+/* oracle/_ref by oracle/ref.mk (tests/test_ref_pins.py).             */
 /* ------------------------------------------------------------------ */
 
 static uint32_t pim_mix(uint32_t key, uint32_t add, int s1, int s2, int s3, int s4) {

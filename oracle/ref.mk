@@ -1,7 +1,9 @@
 # Builds the reference's own CPU operators (reference/db_operators/*.cpp,
 # compiled where they lie under the reference tree) into oracle/_ref/ as
-# ctypes-loadable libraries, through oracle/ref_harness.cpp.  Test
-# infrastructure: these pin the CPU restatement in oracle.cpp.
+# ctypes-loadable libraries, through oracle/ref_harness.cpp, and the PIM
+# library's hash mixers (pimdal/hash/hash_func.h, plain C) through
+# oracle/ref_hash_harness.c.  Test infrastructure: these pin the CPU
+# restatement in oracle.cpp.
 #
 # Needs: the reference tree, pyarrow's bundled Arrow C++ (headers and
 # libarrow.so.2500) and five source files of the abseil copy vendored in the
@@ -10,6 +12,7 @@
 #
 #   make -C oracle -f ref.mk
 REF ?= /root/reference/reference/db_operators
+PIMHASH ?= /root/reference/pimdal/hash
 PYARROW ?= $(shell python3 -c "import pyarrow, os; print(os.path.dirname(pyarrow.__file__))")
 AB := $(REF)/abseil-cpp
 OUT := _ref
@@ -22,7 +25,7 @@ ABSL := $(AB)/absl/container/internal/raw_hash_set.cc $(AB)/absl/hash/internal/h
         $(AB)/absl/hash/internal/city.cc $(AB)/absl/hash/internal/low_level_hash.cc \
         $(AB)/absl/base/internal/raw_logging.cc
 
-all: $(OUT)/ref_select.so $(OUT)/ref_sort.so $(OUT)/ref_hash_join.so $(OUT)/ref_aggregate.so
+all: $(OUT)/ref_hash_func.so $(OUT)/ref_select.so $(OUT)/ref_sort.so $(OUT)/ref_hash_join.so $(OUT)/ref_aggregate.so
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -38,6 +41,9 @@ $(OUT)/ref_hash_join.so: ref_harness.cpp | $(OUT)
 
 $(OUT)/ref_aggregate.so: ref_harness.cpp | $(OUT)
 	$(CXX) $(FLAGS) -DREF_AGGREGATE -DREF_SRC='"aggregate.cpp"' ref_harness.cpp $(ABSL) -o $@ $(LIBS)
+
+$(OUT)/ref_hash_func.so: ref_hash_harness.c | $(OUT)
+	$(CC) -O2 -fPIC -shared -w -I$(PIMHASH) ref_hash_harness.c -o $@
 
 clean:
 	rm -rf $(OUT)

@@ -311,10 +311,10 @@ def test_date_pack_str_reference_literals(orc):
 
 # ---------------- Bloom-filter semi-join ----------------
 # pimdal/join/hash_join.c:117-152, 237-341; pimdal/hash/hash_steps.c:143-183.
-# No reference run pins the filter bit positions (the DPU sources need the
-# UPMEM SDK).  What the mathematics fixes is checked instead: no false
-# negatives, a false-positive rate near (1 - e^(-4n/m))^4, a build key only
-# touches its own partition's filter, and at most 4 bits per key.
+# The mixers and the filter bit positions are pinned against the reference's
+# own hash_func.h in tests/test_ref_pins.py.  Here, what the mathematics
+# fixes: no false negatives, a false-positive rate near (1 - e^(-4n/m))^4, a
+# build key only touches its own partition's filter, at most 4 bits per key.
 
 def test_bloom_no_false_negatives_and_fp_rate(orc):
     rng = np.random.default_rng(21)

@@ -1,5 +1,6 @@
 """The CPU restatement pinned against the reference's own CPU operators
-(reference/db_operators/*.cpp built into oracle/_ref by oracle/ref.mk).
+(reference/db_operators/*.cpp built into oracle/_ref by oracle/ref.mk) and
+the PIM library's hash mixers (pimdal/hash/hash_func.h, same recipe).
 
 Inputs avoid the reference's documented quirks: the row count is a multiple
 of its 8000-row block and equals its compiled BUFFER_SIZE (select.cpp:15-16
@@ -88,3 +89,38 @@ def test_hash_aggr_matches_reference(orc):
     rk, rv = canon(ok[:g], ov[:g])
     ek, es, ec = orc.hash_aggr(k, v.astype(np.int64))
     assert np.array_equal(rk, ek) and np.array_equal(rv.astype(np.int64), es)
+
+
+# ---------------- the PIM library's hash mixers (pimdal/hash/hash_func.h) ----------------
+
+def _ref_hash():
+    L = ref("hash_func")
+    L.ref_pim_hash.restype = C.c_uint32
+    L.ref_pim_hash.argtypes = [C.c_int, C.c_uint32]
+    return L
+
+
+def test_pim_hash_mixers_match_reference(orc):
+    L = _ref_hash()
+    keys = [0, 1, 2, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF] + \
+        np.random.default_rng(31).integers(0, 1 << 32, 20000, dtype=np.uint64).tolist()
+    for w in range(5):
+        for k in keys:
+            assert orc.pim_hash(w, k) == L.ref_pim_hash(w, k), (w, k)
+
+
+def test_bloom_filters_follow_reference_hashes(orc):
+    """The oracle's filters against bits placed with the reference's own
+    mixers: partition = bits [shift - part_bits, shift) of hash0, bit
+    hash_func[f](key) % 16384 of that partition's filter, byte layout of
+    insert_bloom_filter (pimdal/hash/hash_steps.c:143-152)."""
+    L = _ref_hash()
+    part_bits, shift = 3, 29
+    keys = np.random.default_rng(32).integers(0, 1 << 32, 3000, dtype=np.uint64).astype(np.uint32)
+    exp = np.zeros((1 << part_bits) * 2048, np.uint8)
+    for k in keys.tolist():
+        p = (L.ref_pim_hash(0, k) >> (shift - part_bits)) & ((1 << part_bits) - 1)
+        for f in range(1, 5):
+            bit = L.ref_pim_hash(f, k) % 16384
+            exp[p * 2048 + bit // 8] |= 1 << (bit % 8)
+    assert np.array_equal(orc.bloom_build(keys, part_bits, shift), exp)
