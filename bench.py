@@ -251,8 +251,12 @@ def bench_q6(ctx, steps, warmup, n=60_000_000):
     return res
 
 
-def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, rate_qps=1e6, slots=4):
-    """Mixed select/join/group-by query stream over resident tables (6e7-row fact table, 1.5e7-row dimension table), batched vs one at a time."""
+def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, rate_qps=1e6, slots=4,
+                sweep=(2e3, 1e4, 1e5)):
+    """Mixed select/join/group-by query stream over resident tables (6e7-row
+    fact table, 1.5e7-row dimension table), batched vs one at a time, at the
+    headline arrival rate and over a sweep of Poisson arrival rates (at most
+    about 4 s of arrivals per rate)."""
     import numpy as np
     from pimdal_amd import datagen, serving
     fact = datagen.fact_table(ctx, n_fact, groups=n_dim)
@@ -261,19 +265,25 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
     wq = serving.gen_queries(warm, n_fact, rate_qps, seed=0x5EED0105)
     srv.run(wq, serving.BATCHED)
     srv.run(wq[:50], serving.SEQUENTIAL)
-    q = serving.gen_queries(nq, n_fact, rate_qps, seed=0x5EED0005)
-    r_seq, st_seq = srv.run(q, serving.SEQUENTIAL)
-    r_bat, st_bat = srv.run(q, serving.BATCHED)
-    same = bool(np.array_equal(r_seq["count"], r_bat["count"]) and np.array_equal(r_seq["checksum"], r_bat["checksum"]))
+
+    def at(rate, n):
+        q = serving.gen_queries(n, n_fact, rate, seed=0x5EED0005)
+        r_seq, st_seq = srv.run(q, serving.SEQUENTIAL)
+        r_bat, st_bat = srv.run(q, serving.BATCHED)
+        same = bool(np.array_equal(r_seq["count"], r_bat["count"]) and np.array_equal(r_seq["checksum"], r_bat["checksum"]))
+        return {"queries": n, "arrival_rate_qps": rate,
+                "batched": {k: st_bat[k] for k in ("qps", "p50_us", "p99_us", "mean_us", "batches", "max_inflight")},
+                "sequential": {k: st_seq[k] for k in ("qps", "p50_us", "p99_us", "mean_us")},
+                "qps_speedup": st_bat["qps"] / st_seq["qps"] if st_seq["qps"] else None, "results_identical": same}
+
+    res = at(rate_qps, nq)
+    res["slots"] = slots
+    res["rate_sweep"] = [at(r, int(min(nq, 4 * r))) for r in sweep]
     srv.close()
     for d in (fact, dim):
         for a in d.values():
             a.free()
-    return {"queries": nq, "arrival_rate_qps": rate_qps, "slots": slots,
-            "batched": {k: st_bat[k] for k in ("qps", "p50_us", "p99_us", "mean_us", "batches", "max_inflight")},
-            "sequential": {k: st_seq[k] for k in ("qps", "p50_us", "p99_us", "mean_us")},
-            "qps_speedup": st_bat["qps"] / st_seq["qps"] if st_seq["qps"] else None,
-            "results_identical": same}
+    return res
 
 
 def cpu_baseline_sort(sample_n=1 << 24, reps=3):
@@ -361,7 +371,7 @@ def main():
         # (no resident join index), to separate batching from the index
         log("bench: serving mixed stream, probe index")
         ctx.set_tune("serve_join_index", 0)
-        ops_out["serve_mixed_stream_probe_index"] = bench_serve(ctx)
+        ops_out["serve_mixed_stream_probe_index"] = bench_serve(ctx, sweep=())
         ctx.set_tune("serve_join_index", 1)
     cpu = None
     if not args.no_cpu:
