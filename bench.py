@@ -259,6 +259,8 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
     about 4 s of arrivals per rate)."""
     import numpy as np
     from pimdal_amd import datagen, serving
+    budget_before = ctx.mem_stats()["budget"]
+    ctx.set_budget(64 << 30)   # the serving stream runs under a fixed 64 GiB HBM budget
     fact = datagen.fact_table(ctx, n_fact, groups=n_dim)
     dim = datagen.dim_table(ctx, n_dim)
     srv = serving.Server(ctx, fact, dim, slots=slots, arena_bytes=1 << 30)
@@ -279,10 +281,13 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
     res = at(rate_qps, nq)
     res["slots"] = slots
     res["rate_sweep"] = [at(r, int(min(nq, 4 * r))) for r in sweep]
+    res["hbm_budget_bytes"] = 64 << 30
+    res["hbm_live_bytes"] = ctx.mem_stats()["live"]   # resident tables, server indexes and arenas
     srv.close()
     for d in (fact, dim):
         for a in d.values():
             a.free()
+    ctx.set_budget(budget_before)
     return res
 
 

@@ -71,30 +71,33 @@ void DevicePool::give_event(hipEvent_t e) {
 void* DevicePool::alloc(uint64_t bytes, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(mu_);
     const uint64_t sz = round_size(bytes);
-    // exact-size reuse: same stream, or the free has completed
-    auto range = cached_.equal_range(sz);
-    for (auto it = range.first; it != range.second; ++it) {
+    // reuse the smallest cached block of sz .. sz + sz/8 bytes whose free is
+    // on the same stream or has completed (requests of many sizes, as a
+    // serving stream makes, would otherwise each leave a block of their own)
+    for (auto it = cached_.lower_bound(sz); it != cached_.end() && it->first <= sz + sz / 8; ++it) {
         Block& b = it->second;
         bool ok = (b.stream == stream) || (hipEventQuery(b.ev) == hipSuccess);
         if (ok) {
             void* p = b.p;
+            const uint64_t bsz = b.size;
             give_event(b.ev);
             cached_.erase(it);
-            live_ += sz;
-            live_blocks_[p] = sz;
+            live_ += bsz;
+            live_blocks_[p] = bsz;
             peak_ = std::max(peak_, live_);
             return p;
         }
     }
     if (reserved_ + sz > budget_) {
-        // give cached memory back before failing
+        // give cached memory back, largest blocks first, until the request fits
         PD_HIP(hipDeviceSynchronize());
-        for (auto& kv : cached_) {
-            (void)hipFree(kv.second.p);
-            give_event(kv.second.ev);
-            reserved_ -= kv.second.size;
+        while (!cached_.empty() && reserved_ + sz > budget_) {
+            auto it = std::prev(cached_.end());
+            (void)hipFree(it->second.p);
+            give_event(it->second.ev);
+            reserved_ -= it->second.size;
+            cached_.erase(it);
         }
-        cached_.clear();
         if (reserved_ + sz > budget_)
             fail(PIMDAL_E_NOMEM, "HBM budget exceeded: need " + std::to_string(sz) + " B, reserved " +
                                      std::to_string(reserved_) + " B of " + std::to_string(budget_) + " B");
