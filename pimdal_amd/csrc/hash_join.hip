@@ -505,10 +505,19 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         return hash_join_global(c, bk, bp, nb, pk, pp, np, out_p, out_b, cap, s);
     PD_REQUIRE(nb < (1ull << 30) && np < (1ull << 30), PIMDAL_E_INVALID,
                "partitioned hash join supports up to 2^30-1 rows per side");
+    // partition count: knob hj_fanout, else about hj_rows_per_part build rows
+    // per partition; by default 2048 rows (a quarter of an LDS table, short
+    // probe chains) when that needs no more 8-bit partition passes than
+    // 4096 rows would and stays within the 16384 partitions a join takes,
+    // else 4096
     const int64_t tuned = c->tune_get("hj_fanout", 0);
-    uint32_t F = tuned > 0 ? next_pow2((uint64_t)tuned)
-                           : next_pow2(std::max<uint64_t>(1, ceil_div(nb, (uint64_t)c->tune_get("hj_rows_per_part", 4096))));
-    F = std::max<uint32_t>(F, 2);
+    const int64_t rpp = c->tune_get("hj_rows_per_part", 0);
+    auto fan = [&](uint64_t rows) { return std::max<uint32_t>(2, next_pow2(std::max<uint64_t>(1, ceil_div(nb, rows)))); };
+    auto passes = [](uint32_t f) { return ((uint32_t)__builtin_ctz(f) + 7) / 8; };
+    uint32_t F;
+    if (tuned > 0) F = std::max<uint32_t>(2, next_pow2((uint64_t)tuned));
+    else if (rpp > 0) F = fan((uint64_t)rpp);
+    else F = (fan(2048) <= 16384 && passes(fan(2048)) == passes(fan(4096))) ? fan(2048) : fan(4096);
     // probe rows per work item: 1.5x the average probe partition, so a
     // typical partition is one work item (one LDS table build) while a
     // skewed one is split over several blocks; at most np / (4 x CUs) rows,

@@ -141,9 +141,11 @@ void launch(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid
 // progress.  Persistent kernels that walk tiles in a fixed order (block b
 // takes tiles b, b + grid, ...) use it instead of a tile counter: one
 // device-wide atomic per tile serializes at about 11 ns each.
-// Returns false, with nothing launched, when the runtime refuses the grid
-// as too large to be resident at once (fewer CUs available than the device
-// reports, for example); other errors throw.
+// Returns false, with nothing launched, when the runtime refuses the grid:
+// too large to be resident at once (fewer CUs available than the device
+// reports, for example), or the device's cooperative dispatch not available
+// in its present state (seen while another process on the same GPU has run
+// cooperative launches).  Other errors throw.
 template <typename... KArgs, typename... Args>
 bool launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
                  uint32_t shmem, hipStream_t s, Args... args) {
@@ -157,7 +159,7 @@ bool launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3
         PD_HIP(hipEventRecord(e0, s));
     }
     const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shmem, s);
-    if (e == hipErrorCooperativeLaunchTooLarge) {
+    if (e == hipErrorCooperativeLaunchTooLarge || e == hipErrorIllegalState) {
         (void)hipGetLastError();   // clear the sticky launch error
         if (e0) c->event_pool.push_back(e0);
         return false;

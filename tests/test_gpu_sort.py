@@ -216,3 +216,29 @@ def test_sort_pass_variant_zipf_bits_payloads_and_hash_join(ctx, orc, pass_varia
     got = np.sort((op.numpy(cnt).astype(np.uint64) << np.uint64(32)) | ob.numpy(cnt))
     exp = np.sort((ep.astype(np.uint64) << np.uint64(32)) | eb)
     assert np.array_equal(got, exp)
+
+
+def test_sort_in_a_child_process_after_this_one_ran_persistent_passes(ctx, orc):
+    """This process has run co-resident (cooperative) passes; a second process
+    on the same GPU sorts too.  Where the runtime refuses that process's
+    cooperative grid, its passes take one block per tile and stay exact."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    k, v = orc.sort_workload(1 << 20, seed=91)
+    gk, gv = gpu_sort(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+    child = (
+        "import numpy as np, pimdal_amd\n"
+        "from pimdal_amd import operators as ops\n"
+        "from oracle import orc\n"
+        "k, v = orc.sort_workload(1 << 20, seed=92)\n"
+        "with pimdal_amd.Context(0) as c:\n"
+        "    ok, ov = ops.sort_pairs(c, c.to_device(k), c.to_device(v))\n"
+        "    ek, ev = orc.sort_pairs(k, v)\n"
+        "    assert np.array_equal(ok.numpy(), ek) and np.array_equal(ov.numpy(), ev)\n"
+        "print('child sort ok', flush=True)\n")
+    root = Path(__file__).resolve().parents[1]
+    res = subprocess.run([sys.executable, "-c", child], cwd=root, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and "child sort ok" in res.stdout, res.stderr[-2000:]
