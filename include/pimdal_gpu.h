@@ -181,6 +181,20 @@ int pimdal_select(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* payl
  * count.  Asynchronous. */
 int pimdal_select_dev(pimdal_ctx* ctx, pimdal_type key_type, const void* key, const uint32_t* payload, uint64_t n,
                       int64_t lo, int64_t hi, uint32_t* out_payload, uint32_t* out_rowid, uint64_t* d_count);
+/* No-allocation form of pimdal_select_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, outputs and result as pimdal_select_dev
+ * (always the one-pass kernel); its look-back status words live in a
+ * static device array of the library (1 MiB + 256 B per GPU, held until
+ * the process ends), so it calls no hipMalloc, hipFree or allocating
+ * routine.  Limit: n <= 2^29 rows, else PIMDAL_E_CAPACITY.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (two streams,
+ * two contexts, two host threads) race and must be serialised by the
+ * caller.  Asynchronous. */
+int pimdal_select_dev_noalloc(pimdal_ctx* ctx, pimdal_type key_type, const void* key, const uint32_t* payload,
+                              uint64_t n, int64_t lo, int64_t hi, uint32_t* out_payload, uint32_t* out_rowid,
+                              uint64_t* d_count);
 
 /* ---------------- sort ----------------
  * reference/db_operators/sort.cpp:42 sort(table, column) and

@@ -74,6 +74,7 @@ SIGNATURES: dict[str, list] = {
     "pimdal_col_export": [c_ctx_p, c_col_p, vp, vp],
     "pimdal_select": [c_ctx_p, c_col_p, c_col_p, i64, i64, C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_select_dev": [c_ctx_p, C.c_int, vp, vp, u64, i64, i64, vp, vp, vp],
+    "pimdal_select_dev_noalloc": [c_ctx_p, C.c_int, vp, vp, u64, i64, i64, vp, vp, vp],
     "pimdal_sort_pairs": [c_ctx_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p)],
     "pimdal_sort_pairs_dev": [c_ctx_p, vp, vp, u32, u64, vp, vp, C.c_int, C.c_int],
     "pimdal_hash_join": [c_ctx_p, c_col_p, c_col_p, c_col_p, c_col_p, C.POINTER(c_col_p), C.POINTER(c_col_p)],

@@ -8,6 +8,13 @@ namespace pd {
 void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
                   int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s);
 
+// The one-pass select with its status words in a static device array
+// instead of pool memory (no allocation); select_noalloc_init looks up that
+// array's address on the current device, once per device.
+void select_range_noalloc(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
+                          int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s);
+void select_noalloc_init(int device);
+
 void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t* kout,
                 void* vout, int begin_bit, int end_bit, hipStream_t s);
 

@@ -9,6 +9,7 @@
 
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
+#include "pd_ops.hpp"
 
 namespace pd {
 
@@ -307,6 +308,8 @@ extern "C" int pimdal_ctx_create(int device, uint64_t hbm_budget_bytes, pimdal_c
     PD_HIP(hipGetDeviceCount(&n));
     PD_REQUIRE(device >= 0 && device < n, PIMDAL_E_INVALID, "no such device");
     PD_HIP(hipSetDevice(device));
+    // device address of the no-allocation select's scratch array (once per device)
+    pd::select_noalloc_init(device);
     auto* c = new pimdal_ctx();
     c->device = device;
     hipDeviceProp_t prop;

@@ -10,6 +10,7 @@
 // already resident and the wait always ends), and writes the kept (payload,
 // row id) pairs from registers into their contiguous output range.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "pd_runtime.hpp"
@@ -335,31 +336,95 @@ __global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select_wr
     }
 }
 
-void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
-                  int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s) {
+// Host form of the predicate: [lo, hi) clamped to the key domain and
+// expressed as [ulo, ulo + wm1] in the unsigned order of the key type.
+// Returns false when no key value can satisfy it.
+static bool sel_bounds(pimdal_type kt, int64_t lo, int64_t hi, uint64_t& ulo, uint64_t& wm1) {
+    ulo = wm1 = 0;
+    if (hi <= lo) return false;
+    if (kt != PIMDAL_I64) {
+        const int64_t dmin = kt == PIMDAL_U32 ? 0 : -(1ll << 31);
+        const int64_t dmax = dmin + (1ll << 32);   // exclusive
+        const int64_t l2 = std::max(lo, dmin), h2 = std::min(hi, dmax);
+        if (h2 <= l2) return false;
+        ulo = (uint64_t)(l2 - dmin);
+        wm1 = (uint64_t)(h2 - l2 - 1);
+    } else {
+        ulo = (uint64_t)lo ^ (1ull << 63);
+        wm1 = (uint64_t)hi - (uint64_t)lo - 1;
+    }
+    return true;
+}
+
+static void sel_check_args(pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, const uint32_t* out_pay) {
     PD_REQUIRE(kt == PIMDAL_U32 || kt == PIMDAL_I32 || kt == PIMDAL_I64, PIMDAL_E_UNSUPPORTED,
                "select key must be uint32, int32 or int64");
     PD_REQUIRE(kt != PIMDAL_I64 || pay || !out_pay, PIMDAL_E_INVALID, "int64 keys need a 32-bit payload column");
     PD_REQUIRE(n < (1ull << 32), PIMDAL_E_INVALID, "select supports up to 2^32-1 rows (uint32 row ids)");
     PD_REQUIRE(key || n == 0, PIMDAL_E_INVALID, "key is NULL");
     PD_REQUIRE(!out_pay || pay, PIMDAL_E_INVALID, "payload is NULL but out_payload requested");
+}
+
+// One-pass block size for these columns: unaligned columns take the
+// 256-thread kernel with dword loads.
+static int sel_block(pimdal_ctx* c, bool vec) {
+    const int64_t bsel = c->tune_get("select_block", 1024);
+    return !vec ? 256 : bsel >= 1024 ? 1024 : bsel >= 512 ? 512 : 256;
+}
+
+static uint64_t sel_tiles(uint64_t n, int blk) { return ceil_div(n, (uint64_t)(blk / PD_WAVE) * SEL_WAVE_ROWS); }
+
+__global__ void k_sel_clear(uint64_t* __restrict__ p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// The one-pass select over `status` (ntiles + 1 words: a look-back status
+// per tile, then the tile counter), which it clears first.  PROFILED: the
+// launches go through the context's hipEvent timing; otherwise they are
+// plain launches, which create no events.
+template <bool PROFILED>
+static void select_one_pass(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n,
+                            uint64_t ulo, uint64_t wm1, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count,
+                            uint64_t* status, int BLK, bool vec, hipStream_t s) {
+    const uint64_t ntl = sel_tiles(n, BLK);
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(status + ntl);
+    auto go_launch = [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) {
+        if constexpr (PROFILED) {
+            launch(c, name, kernel, grid, block, 0, s, args...);
+        } else {
+            hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+            PD_HIP(hipGetLastError());
+        }
+    };
+    go_launch("fill_u64", k_sel_clear, dim3((uint32_t)std::min<uint64_t>(ceil_div(ntl + 1, 256), 4096)), dim3(256),
+              status, ntl + 1);
+    const int64_t per_cu = c->tune_get("select_blocks_per_cu", BLK == 256 ? 5 : BLK == 512 ? 3 : 1);
+    dim3 grid((uint32_t)std::min<uint64_t>(ntl, (uint64_t)c->cu_count * (uint64_t)per_cu));
+    const int64_t lbw = c->tune_get("select_lookback_width", 1);
+    auto go = [&](auto kp) {
+        using K = std::remove_const_t<std::remove_pointer_t<decltype(kp)>>;
+#define PD_SEL_LAUNCH(V, W, B) \
+    go_launch("select", k_select<K, V, W, B>, grid, dim3(B), kp, pay, n, ulo, wm1, out_pay, out_rid, status, ctr, d_count, ntl)
+        if (!vec) PD_SEL_LAUNCH(false, 1, 256);
+        else if (BLK == 1024) PD_SEL_LAUNCH(true, 1, 1024);
+        else if (BLK == 512) PD_SEL_LAUNCH(true, 1, 512);
+        else if (lbw >= 4) PD_SEL_LAUNCH(true, 4, 256);
+        else PD_SEL_LAUNCH(true, 1, 256);
+#undef PD_SEL_LAUNCH
+    };
+    if (kt == PIMDAL_U32) go(reinterpret_cast<const uint32_t*>(key));
+    else if (kt == PIMDAL_I32) go(reinterpret_cast<const int32_t*>(key));
+    else go(reinterpret_cast<const int64_t*>(key));
+}
+
+void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
+                  int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s) {
+    sel_check_args(kt, key, pay, n, out_pay);
     // without a payload column the key words stand in (read, never written out)
     if (!pay) pay = reinterpret_cast<const uint32_t*>(key);
-    // clamp [lo, hi) to the key domain and express it as [ulo, ulo + wm1] in
-    // the unsigned order of the key type
     uint64_t ulo = 0, wm1 = 0;
-    bool empty = hi <= lo;
-    if (!empty && kt != PIMDAL_I64) {
-        const int64_t dmin = kt == PIMDAL_U32 ? 0 : -(1ll << 31);
-        const int64_t dmax = dmin + (1ll << 32);   // exclusive
-        const int64_t l2 = std::max(lo, dmin), h2 = std::min(hi, dmax);
-        empty = h2 <= l2;
-        ulo = (uint64_t)(l2 - dmin);
-        wm1 = (uint64_t)(h2 - l2 - 1);
-    } else if (!empty) {
-        ulo = (uint64_t)lo ^ (1ull << 63);
-        wm1 = (uint64_t)hi - (uint64_t)lo - 1;
-    }
+    const bool empty = !sel_bounds(kt, lo, hi, ulo, wm1);
     if (n == 0 || empty) {
         fill_u64(c, d_count, 0, 1, s);
         return;
@@ -384,30 +449,63 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
         return;
     }
     // one pass: tile = block size / 64 waves x 1024 rows
-    const int64_t bsel = c->tune_get("select_block", 1024);
-    // unaligned columns take the 256-thread kernel with dword loads
-    const int BLK = !vec ? 256 : bsel >= 1024 ? 1024 : bsel >= 512 ? 512 : 256;
-    const uint64_t ntl = ceil_div(n, (uint64_t)(BLK / PD_WAVE) * SEL_WAVE_ROWS);
-    Scratch scratch(c, 8 * (ntl + 1), s);
-    uint64_t* status = scratch.as<uint64_t>();
-    uint32_t* ctr = reinterpret_cast<uint32_t*>(status + ntl);
-    fill_u64(c, status, 0, ntl + 1, s);
-    const int64_t per_cu = c->tune_get("select_blocks_per_cu", BLK == 256 ? 5 : BLK == 512 ? 3 : 1);
-    dim3 grid((uint32_t)std::min<uint64_t>(ntl, (uint64_t)c->cu_count * (uint64_t)per_cu));
-    const int64_t lbw = c->tune_get("select_lookback_width", 1);
-    auto go = [&](auto kp) {
-        using K = std::remove_const_t<std::remove_pointer_t<decltype(kp)>>;
-#define PD_SEL_LAUNCH(V, W, B) launch(c, "select", k_select<K, V, W, B>, grid, dim3(B), 0, s, kp, pay, n, ulo, wm1, out_pay, out_rid, status, ctr, d_count, ntl)
-        if (!vec) PD_SEL_LAUNCH(false, 1, 256);
-        else if (BLK == 1024) PD_SEL_LAUNCH(true, 1, 1024);
-        else if (BLK == 512) PD_SEL_LAUNCH(true, 1, 512);
-        else if (lbw >= 4) PD_SEL_LAUNCH(true, 4, 256);
-        else PD_SEL_LAUNCH(true, 1, 256);
-#undef PD_SEL_LAUNCH
-    };
-    if (kt == PIMDAL_U32) go(reinterpret_cast<const uint32_t*>(key));
-    else if (kt == PIMDAL_I32) go(reinterpret_cast<const int32_t*>(key));
-    else go(reinterpret_cast<const int64_t*>(key));
+    const int BLK = sel_block(c, vec);
+    Scratch scratch(c, 8 * (sel_tiles(n, BLK) + 1), s);
+    select_one_pass<true>(c, kt, key, pay, n, ulo, wm1, out_pay, out_rid, d_count, scratch.as<uint64_t>(), BLK, vec, s);
+}
+
+// ---- no-allocation select ----
+// The one-pass select's status words and tile counter live in `scratch`, a
+// file-scope device array of SEL_NA_WORDS words (1 MiB + 256 B, held by
+// every GPU the library runs on until the process ends), instead of
+// memory from the context's pool.  That bounds a call to 2^29 rows:
+// 2^17 tiles of the smallest (4096-row) tile shape plus the counter.
+constexpr uint64_t SEL_NA_WORDS = (1ull << 17) + 32;
+constexpr uint64_t SEL_NA_MAX_ROWS = 1ull << 29;
+static __device__ __align__(256) uint64_t scratch[SEL_NA_WORDS];
+
+// The array's device address on each device.  The name `scratch` in host
+// code is only the runtime's handle for the symbol, never a device address.
+constexpr int SEL_NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static uint64_t* g_na_ptr[SEL_NA_MAX_DEVICES] = {};
+
+// Looks up `scratch`'s address on `device`, which must be the current
+// device; once per device (later calls return at once).
+void select_noalloc_init(int device) {
+    PD_REQUIRE(device >= 0 && device < SEL_NA_MAX_DEVICES, PIMDAL_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (g_na_ptr[device]) return;
+    void* p = nullptr;
+    PD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PD_REQUIRE(p != nullptr, PIMDAL_E_HIP, "no device address for the select scratch array");
+    g_na_ptr[device] = static_cast<uint64_t*>(p);
+}
+
+static uint64_t* select_noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    return (device >= 0 && device < SEL_NA_MAX_DEVICES) ? g_na_ptr[device] : nullptr;
+}
+
+void select_range_noalloc(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
+                          int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s) {
+    sel_check_args(kt, key, pay, n, out_pay);
+    PD_REQUIRE(n <= SEL_NA_MAX_ROWS, PIMDAL_E_CAPACITY,
+               "pimdal_select_dev_noalloc takes at most 2^29 rows (its static scratch holds 2^17 tile statuses)");
+    uint64_t* status = select_noalloc_scratch(c->device);
+    PD_REQUIRE(status != nullptr, PIMDAL_E_INTERNAL, "select scratch address not set up for this device");
+    if (!pay) pay = reinterpret_cast<const uint32_t*>(key);
+    uint64_t ulo = 0, wm1 = 0;
+    const bool empty = !sel_bounds(kt, lo, hi, ulo, wm1);
+    if (n == 0 || empty) {
+        hipLaunchKernelGGL(k_sel_clear, dim3(1), dim3(64), 0, s, d_count, (uint64_t)1);
+        PD_HIP(hipGetLastError());
+        return;
+    }
+    const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)pay % 16 == 0);
+    const int BLK = sel_block(c, vec);
+    PD_REQUIRE(sel_tiles(n, BLK) + 1 <= SEL_NA_WORDS, PIMDAL_E_INTERNAL, "select scratch too small");
+    select_one_pass<false>(c, kt, key, pay, n, ulo, wm1, out_pay, out_rid, d_count, status, BLK, vec, s);
 }
 
 }  // namespace pd
@@ -420,6 +518,15 @@ extern "C" int pimdal_select_dev(pimdal_ctx* c, pimdal_type key_type, const void
     PD_API_BEGIN
     PD_REQUIRE(c && d_count, PIMDAL_E_INVALID, "NULL argument");
     select_range(c, key_type, key, payload, n, lo, hi, out_payload, out_rowid, d_count, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_select_dev_noalloc(pimdal_ctx* c, pimdal_type key_type, const void* key,
+                                         const uint32_t* payload, uint64_t n, int64_t lo, int64_t hi,
+                                         uint32_t* out_payload, uint32_t* out_rowid, uint64_t* d_count) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && d_count, PIMDAL_E_INVALID, "NULL argument");
+    select_range_noalloc(c, key_type, key, payload, n, lo, hi, out_payload, out_rowid, d_count, c->stream);
     PD_API_END
 }
 
