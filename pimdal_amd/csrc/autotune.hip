@@ -107,8 +107,12 @@ extern "C" int pimdal_autotune(pimdal_ctx* c, const char* op, uint64_t n, const 
         fill(c, bp.get(), nb, 6, 0);
         fill(c, pk.get(), n, 7, 0);
         fill(c, pp.get(), n, 8, 0);
-        knobs = {{"hj_mode", {0, 1}}, {"hj_block", {512, 1024}}, {"hj_rows_per_part", {2048, 4096}},
-                 {"hj_chunk", {16384, 32768, 65536, 131072}}};
+        // 0 / -1 stand for the size-aware defaults, so a cached value keeps
+        // adapting to the size of later calls; the one global table (hj_mode
+        // 1) is not a candidate: above 2^16 build rows its probe reads whole
+        // lines from HBM per key and loses to the partitioned join
+        knobs = {{"hj_mode", {-1, 0}}, {"hj_block", {512, 1024}}, {"hj_rows_per_part", {0, 4096}},
+                 {"hj_chunk", {0, 16384, 32768, 65536}}};
         tune(c, knobs, [&] {
             hash_join(c, bk.as<uint32_t>(), bp.as<uint32_t>(), nb, pk.as<uint32_t>(), pp.as<uint32_t>(), n,
                       o1.as<uint32_t>(), o2.as<uint32_t>(), n, s);

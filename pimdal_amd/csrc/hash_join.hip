@@ -28,6 +28,7 @@
 //     keys are detected and the join is re-run counting then writing all
 //     pairs.
 #include <algorithm>
+#include <vector>
 
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
@@ -491,8 +492,38 @@ static uint32_t next_pow2(uint64_t x) {
     return p;
 }
 
+// Largest partition count of the partitioned join (20 hash bits: three
+// onesweep passes; partition bounds and work items are 32-bit words).
+constexpr uint32_t HJ_MAX_FANOUT = 1u << 20;
+// Build rows one LDS table takes (7/8 of its slots).
+constexpr uint64_t HJ_LDS_ROWS = (uint64_t)HJ_TS * 7 / 8;
+
+// Partition count of the partitioned join for nb build rows: knob
+// hj_fanout, else about hj_rows_per_part build rows per partition; by
+// default 2048 rows (a quarter of an LDS table, short probe chains) when
+// that needs no more 8-bit partition passes than 4096 rows would, else
+// 4096.  At most HJ_MAX_FANOUT.
+static uint32_t hj_fanout(pimdal_ctx* c, uint64_t nb) {
+    const int64_t tuned = c->tune_get("hj_fanout", 0);
+    const int64_t rpp = c->tune_get("hj_rows_per_part", 0);
+    auto fan = [&](uint64_t rows) {
+        return std::min<uint32_t>(HJ_MAX_FANOUT, std::max<uint32_t>(2, next_pow2(std::max<uint64_t>(1, ceil_div(nb, rows)))));
+    };
+    auto passes = [](uint32_t f) { return ((uint32_t)__builtin_ctz(f) + 7) / 8; };
+    if (tuned > 0) return std::min<uint32_t>(HJ_MAX_FANOUT, std::max<uint32_t>(2, next_pow2((uint64_t)tuned)));
+    if (rpp > 0) return fan((uint64_t)rpp);
+    return passes(fan(2048)) == passes(fan(4096)) ? fan(2048) : fan(4096);
+}
+
 // Returns the number of matching pairs; the first min(count, cap) are in
 // out_p / out_b.
+//
+// A build partition larger than an LDS table (HJ_LDS_ROWS rows: heavy
+// duplicate build keys, which every partition count puts in one partition)
+// is not refused: the LDS kernels skip it, and after them it is joined on
+// its own through one global table in HBM (hash_join_global over the
+// partition's build and probe ranges), its pairs appended after the LDS
+// partitions' pairs.
 uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
                    const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap, hipStream_t s) {
     PD_REQUIRE(nb < (1ull << 32) && np < (1ull << 32), PIMDAL_E_INVALID, "join inputs must have < 2^32 rows");
@@ -505,19 +536,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         return hash_join_global(c, bk, bp, nb, pk, pp, np, out_p, out_b, cap, s);
     PD_REQUIRE(nb < (1ull << 30) && np < (1ull << 30), PIMDAL_E_INVALID,
                "partitioned hash join supports up to 2^30-1 rows per side");
-    // partition count: knob hj_fanout, else about hj_rows_per_part build rows
-    // per partition; by default 2048 rows (a quarter of an LDS table, short
-    // probe chains) when that needs no more 8-bit partition passes than
-    // 4096 rows would and stays within the 16384 partitions a join takes,
-    // else 4096
-    const int64_t tuned = c->tune_get("hj_fanout", 0);
-    const int64_t rpp = c->tune_get("hj_rows_per_part", 0);
-    auto fan = [&](uint64_t rows) { return std::max<uint32_t>(2, next_pow2(std::max<uint64_t>(1, ceil_div(nb, rows)))); };
-    auto passes = [](uint32_t f) { return ((uint32_t)__builtin_ctz(f) + 7) / 8; };
-    uint32_t F;
-    if (tuned > 0) F = std::max<uint32_t>(2, next_pow2((uint64_t)tuned));
-    else if (rpp > 0) F = fan((uint64_t)rpp);
-    else F = (fan(2048) <= 16384 && passes(fan(2048)) == passes(fan(4096))) ? fan(2048) : fan(4096);
+    uint32_t F = hj_fanout(c, nb);
     // probe rows per work item: 1.5x the average probe partition, so a
     // typical partition is one work item (one LDS table build) while a
     // skewed one is split over several blocks; at most np / (4 x CUs) rows,
@@ -542,9 +561,6 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
     uint32_t* p0 = direct ? out_p : tmp_p.as<uint32_t>();
     uint32_t* b0 = direct ? out_b : tmp_b.as<uint32_t>();
     for (int attempt = 0;; attempt++) {
-        PD_REQUIRE(F <= 16384, PIMDAL_E_UNSUPPORTED,
-                   "hash join: a build partition does not fit the LDS table (heavy duplicate build keys); "
-                   "use the sort-merge join");
         const uint32_t bits = (uint32_t)__builtin_ctz(F);
         Scratch HR(c, ((uint64_t)F + 1) * 4, s), HS(c, ((uint64_t)F + 1) * 4, s);
         hash_partition(c, bk, bp ? bp : bk, 4, nb, bits, rk.as<uint32_t>(), rp.get(), HR.as<uint32_t>(), s);
@@ -555,6 +571,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         launch(c, "hj_chunks", k_hj_plan, dim3(1), dim3(1024), 0, s, (const uint32_t*)HS.as<uint32_t>(), F, CH,
                wst.as<uint32_t>(), flags.as<uint32_t>(), wcnt.as<uint32_t>(), (uint32_t)(max_wi + 1));
         Scratch valid(c, direct_probe ? max_wi * vw * 8 : 0, s);
+        uint32_t fl[4] = {1, 0, 0, 0};   // {duplicate build keys, oversized build partition, pairs (u64)}
         if (direct_probe) {
             launch(c, "hj_join", k_hj_direct<1024, 8>, dim3((uint32_t)max_wi), dim3(1024), 0, s,
                    (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
@@ -562,13 +579,8 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
                    (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), F, CH,
                    (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), p0, b0, valid.as<uint64_t>(), vw,
                    flags.as<uint32_t>());
-            uint32_t fl[4];
             read_small(c, fl, flags.get(), 16, s);
-            if (fl[1]) {  // a build partition overflowed the table: more partitions
-                F *= 2;
-                continue;
-            }
-            if (!fl[0]) {
+            if (!fl[0] && !fl[1]) {
                 const uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
                 if (direct && total == np) return total;  // every probe row matched once: dense as written
                 exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
@@ -591,68 +603,96 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
                 }
                 return total;
             }
-            // duplicate build keys: the counting path below
-        }
+            // duplicate build keys or an oversized partition: the counting path below
+        } else {
 #define PD_HJ_UNIQUE(B, R)                                                                                     \
     launch(c, "hj_join", k_hj_join<HJ_UNIQUE, B, R>, dim3((uint32_t)max_wi), dim3(B), 0, s,                        \
            (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(), (const uint32_t*)HR.as<uint32_t>(), \
            GR, (const uint32_t*)sk.as<uint32_t>(), (const uint32_t*)sp.as<uint32_t>(),                             \
            (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), \
            (const uint32_t*)nullptr, p0, b0, (uint64_t)np, flags.as<uint32_t>())
-        uint32_t fl[4] = {1, 0, 0, 0};   // direct probe found duplicate build keys
-        if (!direct_probe) {
             if (c->tune_get("hj_block", 1024) == 1024) PD_HJ_UNIQUE(1024, 8);
             else PD_HJ_UNIQUE(512, 16);
-            read_small(c, fl, flags.get(), 16, s);
-        }
 #undef PD_HJ_UNIQUE
-        if (fl[1]) {  // a build partition overflowed the table: more partitions
-            F *= 2;
-            continue;
+            read_small(c, fl, flags.get(), 16, s);
+            if (!fl[0] && !fl[1]) {
+                const uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
+                if (direct && total == np) return total;  // every probe row matched once: already dense
+                // compact the per-work-item runs
+                exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
+                if (out_p && out_b && cap) {
+                    if (direct) {
+                        Scratch cp(c, total * 4 + 4, s), cb(c, total * 4 + 4, s);
+                        launch(c, "hj_compact", k_hj_compact, dim3((uint32_t)max_wi), dim3(256), 0, s,
+                               (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(),
+                               (const uint32_t*)wcnt.as<uint32_t>(), (const uint32_t*)woff.as<uint32_t>(),
+                               (const uint32_t*)p0, (const uint32_t*)b0, cp.as<uint32_t>(), cb.as<uint32_t>(), total);
+                        copy_d2d(c, out_p, cp.get(), std::min(total, cap) * 4, s);
+                        copy_d2d(c, out_b, cb.get(), std::min(total, cap) * 4, s);
+                        PD_HIP(hipStreamSynchronize(s));
+                    } else {
+                        launch(c, "hj_compact", k_hj_compact, dim3((uint32_t)max_wi), dim3(256), 0, s,
+                               (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(),
+                               (const uint32_t*)wcnt.as<uint32_t>(), (const uint32_t*)woff.as<uint32_t>(),
+                               (const uint32_t*)p0, (const uint32_t*)b0, out_p, out_b, cap);
+                    }
+                }
+                return total;
+            }
         }
-        uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
-        if (fl[0]) {
-            // duplicate build keys: count, scan, write every pair
-            fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
-            fill_u32(c, flags.get(), 0, 4, s);
-            launch(c, "hj_join_count", k_hj_join<HJ_COUNT>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
+        // ---- counting path: every pair of the partitions an LDS table holds ----
+        fill_u32(c, wcnt.get(), 0, max_wi + 1, s);
+        fill_u32(c, flags.get(), 0, 4, s);
+        launch(c, "hj_join_count", k_hj_join<HJ_COUNT>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
+               (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
+               (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
+               (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
+               (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), (const uint32_t*)nullptr,
+               (uint32_t*)nullptr, (uint32_t*)nullptr, (uint64_t)0, flags.as<uint32_t>());
+        exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
+        read_small(c, fl, flags.get(), 16, s);
+        // oversized build partitions: their row ranges, from the bounds
+        std::vector<uint32_t> big;
+        std::vector<uint32_t> hr, hs;
+        if (fl[1]) {
+            hr.resize((size_t)F + 1);
+            hs.resize((size_t)F + 1);
+            PD_HIP(hipMemcpyAsync(hr.data(), HR.get(), ((uint64_t)F + 1) * 4, hipMemcpyDeviceToHost, s));
+            PD_HIP(hipMemcpyAsync(hs.data(), HS.get(), ((uint64_t)F + 1) * 4, hipMemcpyDeviceToHost, s));
+            PD_HIP(hipStreamSynchronize(s));
+            for (uint32_t f = 0; f < F; f++)
+                if ((uint64_t)hr[f + 1] - hr[f] > HJ_LDS_ROWS) big.push_back(f);
+            // many oversized partitions mean too few partitions for the
+            // build side (a tuned fan-out), not a few heavy keys: twice the
+            // partitions
+            if (big.size() * 64 > F && F < HJ_MAX_FANOUT) {
+                F *= 2;
+                fill_u32(c, flags.get(), 0, 4, s);
+                continue;
+            }
+        }
+        const uint64_t lds_total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
+        PD_REQUIRE(lds_total < (1ull << 32), PIMDAL_E_UNSUPPORTED, "join output exceeds 2^32-1 pairs");
+        if (out_p && out_b && cap && lds_total)
+            launch(c, "hj_join_write", k_hj_join<HJ_WRITE>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
                    (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
                    (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
                    (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
-                   (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), (const uint32_t*)nullptr,
-                   (uint32_t*)nullptr, (uint32_t*)nullptr, (uint64_t)0, flags.as<uint32_t>());
-            exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
-            read_small(c, fl, flags.get(), 16, s);
-            total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
+                   (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(),
+                   (const uint32_t*)woff.as<uint32_t>(), out_p, out_b, cap, flags.as<uint32_t>());
+        // the oversized partitions, each through one global table, appended
+        uint64_t total = lds_total;
+        for (const uint32_t f : big) {
+            const uint64_t b0r = hr[f], nbf = (uint64_t)hr[f + 1] - hr[f];
+            const uint64_t s0r = hs[f], npf = (uint64_t)hs[f + 1] - hs[f];
+            if (!npf) continue;
+            const bool room = out_p && out_b && total < cap;
+            const uint64_t k = hash_join_global(c, rk.as<uint32_t>() + b0r, rp.as<uint32_t>() + b0r, nbf,
+                                                sk.as<uint32_t>() + s0r, sp.as<uint32_t>() + s0r, npf,
+                                                room ? out_p + total : nullptr, room ? out_b + total : nullptr,
+                                                room ? cap - total : 0, s);
+            total += k;
             PD_REQUIRE(total < (1ull << 32), PIMDAL_E_UNSUPPORTED, "join output exceeds 2^32-1 pairs");
-            if (out_p && out_b && cap)
-                launch(c, "hj_join_write", k_hj_join<HJ_WRITE>, dim3((uint32_t)max_wi), dim3(HJ_BLOCK), 0, s,
-                       (const uint32_t*)rk.as<uint32_t>(), (const uint32_t*)rp.as<uint32_t>(),
-                       (const uint32_t*)HR.as<uint32_t>(), GR, (const uint32_t*)sk.as<uint32_t>(),
-                       (const uint32_t*)sp.as<uint32_t>(), (const uint32_t*)HS.as<uint32_t>(), GS, F, CH,
-                       (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(),
-                       (const uint32_t*)woff.as<uint32_t>(), out_p, out_b, cap, flags.as<uint32_t>());
-            return total;
-        }
-        if (direct && total == np) return total;  // every probe row matched once: already dense
-        // compact the per-work-item runs
-        exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
-        if (out_p && out_b && cap) {
-            if (direct) {
-                Scratch cp(c, total * 4 + 4, s), cb(c, total * 4 + 4, s);
-                launch(c, "hj_compact", k_hj_compact, dim3((uint32_t)max_wi), dim3(256), 0, s,
-                       (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(),
-                       (const uint32_t*)wcnt.as<uint32_t>(), (const uint32_t*)woff.as<uint32_t>(),
-                       (const uint32_t*)p0, (const uint32_t*)b0, cp.as<uint32_t>(), cb.as<uint32_t>(), total);
-                copy_d2d(c, out_p, cp.get(), std::min(total, cap) * 4, s);
-                copy_d2d(c, out_b, cb.get(), std::min(total, cap) * 4, s);
-                PD_HIP(hipStreamSynchronize(s));
-            } else {
-                launch(c, "hj_compact", k_hj_compact, dim3((uint32_t)max_wi), dim3(256), 0, s,
-                       (const uint32_t*)HS.as<uint32_t>(), GS, F, CH, (const uint32_t*)wst.as<uint32_t>(),
-                       (const uint32_t*)wcnt.as<uint32_t>(), (const uint32_t*)woff.as<uint32_t>(),
-                       (const uint32_t*)p0, (const uint32_t*)b0, out_p, out_b, cap);
-            }
         }
         return total;
     }

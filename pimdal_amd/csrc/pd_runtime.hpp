@@ -43,8 +43,12 @@ public:
     uint64_t reserved() const { return reserved_; }
     uint64_t peak() const { return peak_; }
     uint64_t budget() const { return budget_; }
-    void set_budget(uint64_t b) { budget_ = b; }
+    // Takes the pool lock; when reserved bytes exceed the new budget,
+    // cached blocks are freed (largest first) until they fit, as far as
+    // cached memory allows (live blocks stay).
+    void set_budget(uint64_t b);
 private:
+    void trim_locked(uint64_t need);
     struct Block { void* p; uint64_t size; hipStream_t stream; hipEvent_t ev; };
     static uint64_t round_size(uint64_t b);
     hipEvent_t take_event();            // an idle event (created when none is idle)

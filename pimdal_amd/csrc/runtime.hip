@@ -69,6 +69,26 @@ void DevicePool::give_event(hipEvent_t e) {
     if (e) idle_events_.push_back(e);
 }
 
+// Frees cached blocks, largest first, until reserved + need fits the budget
+// (or nothing is cached).  Caller holds mu_.
+void DevicePool::trim_locked(uint64_t need) {
+    if (reserved_ + need <= budget_ || cached_.empty()) return;
+    PD_HIP(hipDeviceSynchronize());
+    while (!cached_.empty() && reserved_ + need > budget_) {
+        auto it = std::prev(cached_.end());
+        (void)hipFree(it->second.p);
+        give_event(it->second.ev);
+        reserved_ -= it->second.size;
+        cached_.erase(it);
+    }
+}
+
+void DevicePool::set_budget(uint64_t b) {
+    std::lock_guard<std::mutex> lk(mu_);
+    budget_ = b;
+    trim_locked(0);   // a lower budget gives cached memory back at once
+}
+
 void* DevicePool::alloc(uint64_t bytes, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(mu_);
     const uint64_t sz = round_size(bytes);
@@ -91,14 +111,7 @@ void* DevicePool::alloc(uint64_t bytes, hipStream_t stream) {
     }
     if (reserved_ + sz > budget_) {
         // give cached memory back, largest blocks first, until the request fits
-        PD_HIP(hipDeviceSynchronize());
-        while (!cached_.empty() && reserved_ + sz > budget_) {
-            auto it = std::prev(cached_.end());
-            (void)hipFree(it->second.p);
-            give_event(it->second.ev);
-            reserved_ -= it->second.size;
-            cached_.erase(it);
-        }
+        trim_locked(sz);
         if (reserved_ + sz > budget_)
             fail(PIMDAL_E_NOMEM, "HBM budget exceeded: need " + std::to_string(sz) + " B, reserved " +
                                      std::to_string(reserved_) + " B of " + std::to_string(budget_) + " B");

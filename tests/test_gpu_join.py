@@ -157,3 +157,91 @@ def test_join_probe_tile_spans_wide_build_range(ctx, orc, which):
         assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
     else:
         assert same_multiset((gp, gb), (ep, eb))
+
+
+def _heavy_dup_case(orc):
+    """2^17 unique build keys (100 .. 2^17 + 99) plus key 7 repeated 10000
+    times and key 13 repeated 9000 times: each heavy key's partition holds
+    more build rows than an LDS table (7168) at every partition count.
+    Probe: 2^19 keys over the unique range, 20 rows of key 7, 15 of key 13
+    and 64 misses."""
+    nu = 1 << 17
+    rk = np.concatenate([orc.gen_perm(41, 3, nu, nu) + np.uint32(100), np.full(10_000, 7, np.uint32),
+                         np.full(9_000, 13, np.uint32)])
+    rk = rk[np.argsort(orc.gen_u32(41, 9, rk.size), kind="stable")]
+    rp = orc.gen_u32(41, 5, rk.size)
+    sk = orc.gen_range(np.uint32, 41, 4, 1 << 19, 100, nu + 99)
+    pos = orc.gen_perm(41, 8, 1 << 19, 99)   # 99 distinct probe rows
+    sk[pos[:20]] = 7
+    sk[pos[20:35]] = 13
+    sk[pos[35:]] = np.uint32(1 << 30)
+    sp = orc.gen_u32(41, 6, sk.size)
+    return rk, rp, sk, sp
+
+
+@pytest.mark.parametrize("which", ["hash", "hash_part_rounds", "hash_global", "smj"])
+def test_join_heavy_duplicate_build_keys_beyond_lds_table(ctx, orc, which):
+    """Build partitions larger than an LDS table are joined through a global
+    table instead of being refused; every pair is emitted."""
+    rk, rp, sk, sp = _heavy_dup_case(orc)
+    gp, gb = run(ctx, join_fn(ctx, which), rk, rp, sk, sp)
+    ep, eb = orc.sort_merge_join(rk, rp, sk, sp) if which == "smj" else orc.hash_join(rk, rp, sk, sp)
+    n_exp = (1 << 19) - 99 + 20 * 10_000 + 15 * 9_000
+    assert len(ep) == n_exp
+    if which == "smj":
+        assert np.array_equal(gp, ep) and np.array_equal(gb, eb)
+    else:
+        assert same_multiset((gp, gb), (ep, eb))
+
+
+def test_join_heavy_duplicates_capacity_short(ctx, orc):
+    """Device form with a buffer shorter than the output: the full count comes
+    back with PIMDAL_E_CAPACITY and the pairs written are a subset of the
+    join."""
+    import ctypes as C
+    from pimdal_amd import _lib
+    rk, rp, sk, sp = _heavy_dup_case(orc)
+    d = [ctx.to_device(a) for a in (rk, rp, sk, sp)]
+    cap = 600_000
+    op, ob = ctx.empty(cap, np.uint32), ctx.empty(cap, np.uint32)
+    cnt = C.c_uint64()
+    rc = _lib.load().pimdal_hash_join_dev(ctx.handle, d[0].vp, d[1].vp, rk.size, d[2].vp, d[3].vp, sk.size, op.vp,
+                                          ob.vp, cap, C.byref(cnt))
+    ep, eb = orc.hash_join(rk, rp, sk, sp)
+    assert rc == -5 and cnt.value == len(ep)
+    exp = set(((ep.astype(np.uint64) << np.uint64(32)) | eb).tolist())
+    got = (op.numpy().astype(np.uint64) << np.uint64(32)) | ob.numpy()
+    assert set(got.tolist()) <= exp
+
+
+def test_join_tuned_fanout_too_small_doubles(ctx, orc):
+    """A tuned partition count far too small for the build side (2 partitions
+    of ~2^15 rows): the join doubles it until the partitions fit."""
+    rk, rp, sk, sp = orc.join_workload(1 << 16, 1 << 18)
+    ctx.set_tune("hj_mode", 0)
+    ctx.set_tune("hj_fanout", 2)
+    try:
+        gp, gb = run(ctx, ops.hash_join, rk, rp, sk, sp)
+    finally:
+        ctx.clear_tune("hj_mode")
+        ctx.clear_tune("hj_fanout")
+    ep, eb = orc.hash_join(rk, rp, sk, sp)
+    assert same_multiset((gp, gb), (ep, eb))
+
+
+def test_join_unique_build_above_2p26_rows(ctx):
+    """2^26 + 2^20 unique build keys (more than 16384 partitions of 4096 rows)
+    and 2^25 probe rows: joined, not refused; the output multiset is the
+    closed form from the inverse permutation."""
+    nr, ns = (1 << 26) + (1 << 20), 1 << 25
+    rk, rp, sk, sp = datagen.join_workload(ctx, nr, ns)
+    hk, hp, sks, sps = (a.numpy() for a in (rk, rp, sk, sp))
+    op, ob, k = ops.hash_join(ctx, rk, rp, sk, sp)
+    assert k == ns
+    inv = np.empty(nr, np.int64)
+    inv[hk] = np.arange(nr)
+    exp = np.sort((sps.astype(np.uint64) << np.uint64(32)) | hp[inv[sks]].astype(np.uint64))
+    got = np.sort((op.numpy(k).astype(np.uint64) << np.uint64(32)) | ob.numpy(k).astype(np.uint64))
+    for a in (rk, rp, sk, sp, op, ob):
+        a.free()
+    assert np.array_equal(got, exp)

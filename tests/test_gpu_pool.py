@@ -41,3 +41,20 @@ def test_budget_holds_over_many_sizes():
             c.empty(budget + MiB, np.uint8)                # larger than the whole budget
         st = c.mem_stats()
         assert st["live"] == 0 and st["budget"] == budget
+
+
+def test_lowering_the_budget_gives_cached_memory_back():
+    """Reserved bytes (live + cached) stay within the budget after it is
+    lowered: cached blocks are freed at once, live ones stay."""
+    with pimdal_amd.Context(0, 512 * MiB) as c:
+        keep = c.empty(64 * MiB, np.uint8)
+        for n in (100, 80, 60):
+            a = c.empty(n * MiB, np.uint8)
+            a.free()
+        c.sync()
+        st = c.mem_stats()
+        assert st["reserved"] >= (64 + 100 + 80 + 60) * MiB
+        c.set_budget(128 * MiB)
+        st = c.mem_stats()
+        assert st["budget"] == 128 * MiB and st["reserved"] <= 128 * MiB and st["live"] == 64 * MiB
+        keep.free()

@@ -221,7 +221,10 @@ def test_sort_pass_variant_zipf_bits_payloads_and_hash_join(ctx, orc, pass_varia
 def test_sort_in_a_child_process_after_this_one_ran_persistent_passes(ctx, orc):
     """This process has run co-resident (cooperative) passes; a second process
     on the same GPU sorts too.  Where the runtime refuses that process's
-    cooperative grid, its passes take one block per tile and stay exact."""
+    cooperative grid, its passes take one block per tile and stay exact.
+    The child reports which path its passes took (the fallback launches the
+    digit-offset scan that the co-resident passes do themselves); the
+    exactness check holds for either."""
     import subprocess
     import sys
     from pathlib import Path
@@ -235,10 +238,17 @@ def test_sort_in_a_child_process_after_this_one_ran_persistent_passes(ctx, orc):
         "from oracle import orc\n"
         "k, v = orc.sort_workload(1 << 20, seed=92)\n"
         "with pimdal_amd.Context(0) as c:\n"
+        "    c.set_profiling(True)\n"
         "    ok, ov = ops.sort_pairs(c, c.to_device(k), c.to_device(v))\n"
         "    ek, ev = orc.sort_pairs(k, v)\n"
         "    assert np.array_equal(ok.numpy(), ek) and np.array_equal(ov.numpy(), ev)\n"
+        "    n_scan = c.kernel_stat('radix_scan')[0]\n"
+        "    n_pass = c.kernel_stat('radix_onesweep')[0]\n"
+        "    assert n_pass >= 1\n"
+        "    print('child path:', 'one block per tile' if n_scan else 'co-resident', flush=True)\n"
         "print('child sort ok', flush=True)\n")
     root = Path(__file__).resolve().parents[1]
     res = subprocess.run([sys.executable, "-c", child], cwd=root, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0 and "child sort ok" in res.stdout, res.stderr[-2000:]
+    assert "child path:" in res.stdout
+    print(res.stdout.strip().splitlines()[0])
