@@ -322,6 +322,57 @@ int pimdal_q6_dev(pimdal_ctx* ctx, const int32_t* date, const int64_t* quantity,
                   const int64_t* discount, uint64_t n, int32_t date_lo, int32_t date_hi, int64_t disc_lo,
                   int64_t disc_hi, int64_t qty_lt, int64_t* revenue, uint64_t* n_rows);
 
+/* Asynchronous forms of the two calls above, for plans and for callers that
+ * keep results on the device: nothing is read back to the host, so they
+ * can be captured into a plan (pimdal_plan_begin).  Results go to device
+ * words: *d_n_groups (uint64) and *d_status (int32): 0, or
+ * PIMDAL_E_CAPACITY when there are more groups than `capacity` (the count
+ * is still exact, `capacity` groups are written), or PIMDAL_E_UNSUPPORTED
+ * when the data leaves what the asynchronous form covers: more than 8
+ * (flag1, flag2) combinations (flag form), or a measure beyond the packed
+ * widths (price beyond int32, quantity beyond int16, discount or tax beyond
+ * int8) or a hash partition whose groups overflow its LDS table (key form).
+ * Then the outputs are undefined and pimdal_filter_groupby_q1_dev gives the
+ * result.  The flag form's groups come out in ascending key order, as in
+ * pimdal_filter_groupby_q1_dev; the key form's in no particular order.
+ * pimdal_q6_dev_async leaves the revenue (int64) and kept-row count
+ * (uint64) in device words. */
+int pimdal_filter_groupby_q1_dev_async(pimdal_ctx* ctx, const int32_t* date, const uint8_t* flag1,
+                                       const uint8_t* flag2, const int32_t* key, const int64_t* quantity,
+                                       const int64_t* price, const int64_t* discount, const int64_t* tax, uint64_t n,
+                                       int32_t cutoff, const pimdal_q1_result* out, uint64_t capacity,
+                                       uint64_t* d_n_groups, int32_t* d_status);
+int pimdal_q6_dev_async(pimdal_ctx* ctx, const int32_t* date, const int64_t* quantity, const int64_t* price,
+                        const int64_t* discount, uint64_t n, int32_t date_lo, int32_t date_hi, int64_t disc_lo,
+                        int64_t disc_hi, int64_t qty_lt, int64_t* d_revenue, uint64_t* d_rows);
+
+/* ---------------- captured plans ----------------
+ * A plan records a sequence of asynchronous operator calls on the context's
+ * stream as one hipGraph and replays it: the query plans of
+ * pimdal/main/tpc_h/query1/host/host_q1.cpp:345-380 (filter + aggregation
+ * kernels launched one after another, the host waiting on each) become one
+ * graph launch.  Between pimdal_plan_begin and pimdal_plan_end the calls
+ * run as captured work, not at once.  Capturable: pimdal_select_dev,
+ * pimdal_select_dev_noalloc, pimdal_sort_pairs_dev with an explicit bit
+ * range, pimdal_date32_pack_dev, pimdal_bloom_build_dev,
+ * pimdal_bloom_probe_dev, pimdal_bloom_semi_join_dev, pimdal_dev_copy,
+ * pimdal_filter_groupby_q1_dev_async and pimdal_q6_dev_async.  Calls that
+ * wait for the device or return host results (joins, aggregations that
+ * return counts, column forms, host copies, pimdal_ctx_sync) are refused
+ * with PIMDAL_E_UNSUPPORTED inside a capture, and pimdal_plan_end then
+ * fails too (nothing to launch; destroy the plan).  While capturing, the
+ * radix passes use ordinary launches instead of cooperative ones and the
+ * scratch memory of every captured call is kept by the plan (charged to
+ * the context's budget) until pimdal_plan_destroy.  Launches are
+ * asynchronous on the context stream; they read the inputs and write the
+ * outputs at the addresses given at capture time.  Destroy plans before
+ * their context. */
+typedef struct pimdal_plan pimdal_plan;
+int pimdal_plan_begin(pimdal_ctx* ctx, pimdal_plan** out);
+int pimdal_plan_end(pimdal_plan* plan);
+int pimdal_plan_launch(pimdal_plan* plan);
+int pimdal_plan_destroy(pimdal_plan* plan);
+
 /* ---------------- serving ----------------
  * No counterpart in the reference (its drivers run one operator per
  * binary, pimdal/main/<operator>/host): a request loop over tables kept resident in

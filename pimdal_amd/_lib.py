@@ -91,6 +91,12 @@ SIGNATURES: dict[str, list] = {
     "pimdal_filter_groupby_q1_dev": [c_ctx_p, vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, u64, C.POINTER(u64)],
     "pimdal_q6_dev": [c_ctx_p, vp, vp, vp, vp, u64, i32, i32, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_int64),
                       C.POINTER(u64)],
+    "pimdal_filter_groupby_q1_dev_async": [c_ctx_p, vp, vp, vp, vp, vp, vp, vp, vp, u64, i32, vp, u64, vp, vp],
+    "pimdal_q6_dev_async": [c_ctx_p, vp, vp, vp, vp, u64, i32, i32, C.c_int64, C.c_int64, C.c_int64, vp, vp],
+    "pimdal_plan_begin": [c_ctx_p, C.POINTER(vp)],
+    "pimdal_plan_end": [vp],
+    "pimdal_plan_launch": [vp],
+    "pimdal_plan_destroy": [vp],
     "pimdal_server_create": [c_ctx_p, vp, C.c_int, u64, C.POINTER(vp)],
     "pimdal_server_run": [vp, vp, u64, C.c_int, vp, vp],
     "pimdal_server_destroy": [vp],

@@ -277,7 +277,8 @@ __global__ __launch_bounds__(QD_BLOCK) void k_q1_dense(const int32_t* __restrict
 __device__ void q1_final_rows(const unsigned __int128* __restrict__ sums, uint32_t G, const uint32_t* __restrict__ meta,
                               const uint8_t* __restrict__ inv, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
                               int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
-                              int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups);
+                              int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t cap,
+                              uint64_t* __restrict__ ngroups);
 
 // One block per group code: 128-bit sums of the block partials.  With
 // `done` set, the last block to finish (device-scope counter, zeroed
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(256) void k_q1_reduce(const uint64_t* __restrict__ 
                                                    int64_t* __restrict__ sq, int64_t* __restrict__ sp,
                                                    int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
                                                    int64_t* __restrict__ sch, int64_t* __restrict__ cnt,
-                                                   uint64_t* __restrict__ ngroups) {
+                                                   uint64_t cap, uint64_t* __restrict__ ngroups) {
     __shared__ unsigned __int128 s_v[256][6];
     __shared__ uint32_t s_last;
     const uint32_t g = blockIdx.x, t = threadIdx.x;
@@ -321,19 +322,24 @@ __global__ __launch_bounds__(256) void k_q1_reduce(const uint64_t* __restrict__ 
     __syncthreads();
     if (s_last && t == 0) {
         __threadfence();   // every block's sums before the rows
-        q1_final_rows(sums, G, meta, inv, okey, sq, sp, sd, sdp, sch, cnt, ngroups);
+        q1_final_rows(sums, G, meta, inv, okey, sq, sp, sd, sdp, sch, cnt, cap, ngroups);
     }
 }
 
 __device__ void q1_final_rows(const unsigned __int128* __restrict__ sums, uint32_t G, const uint32_t* __restrict__ meta,
                               const uint8_t* __restrict__ inv, int32_t* __restrict__ okey, int64_t* __restrict__ sq,
                               int64_t* __restrict__ sp, int64_t* __restrict__ sd, int64_t* __restrict__ sdp,
-                              int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t* __restrict__ ngroups) {
+                              int64_t* __restrict__ sch, int64_t* __restrict__ cnt, uint64_t cap,
+                              uint64_t* __restrict__ ngroups) {
     const uint32_t n1 = meta[0], n2 = meta[1];
     uint64_t k = 0;
     for (uint32_t g = 0; g < G && g < n1 * n2; g++) {
         const unsigned __int128* v = sums + g * 6;
         if (v[5] == 0) continue;
+        if (k >= cap) {   // counted, not written: the caller's arrays hold `cap` groups
+            k++;
+            continue;
+        }
         okey[k] = (int32_t)inv[g / n2] * 256 + (int32_t)inv[256 + g % n2];
         sq[k] = (int64_t)v[0];
         sp[k] = (int64_t)v[1];
@@ -1239,7 +1245,8 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
                        disc, tax, n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
             launch(c, "q1_reduce", k_q1_reduce, dim3(GG), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb,
                    (uint32_t)GG, sums.as<unsigned __int128>(), done_ctr, (const uint32_t*)meta, (const uint8_t*)inv,
-                   out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count, ng);
+                   out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count, cap,
+                   ng);
         };
         struct { uint64_t ng; uint32_t miss, pad, n1, n2; } hv;
         auto readback = [&]() {
@@ -1319,6 +1326,170 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
            out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count);
     PD_HIP(hipStreamSynchronize(s));
     return g;
+}
+
+// ---- asynchronous Q1 (no host reads: capturable into a plan) ----
+
+// Result of the asynchronous Q1 forms, from device words: *d_ng = groups,
+// *d_status = 0, PIMDAL_E_UNSUPPORTED when the data left what the form
+// covers (dense: more than 8 flag combinations; by key: a measure beyond the
+// packed widths, f0, or a partition beyond its LDS table, f1), or
+// PIMDAL_E_CAPACITY when there are more groups than `cap` (only `cap`
+// written).
+__global__ void k_q1_async_status(const uint32_t* __restrict__ dense_meta, const uint32_t* __restrict__ f0,
+                                  const uint32_t* __restrict__ f1, const uint64_t* __restrict__ groups, uint64_t cap,
+                                  uint64_t* __restrict__ d_ng, int32_t* __restrict__ d_status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    bool unsupported = false;
+    if (dense_meta) unsupported |= dense_meta[0] * dense_meta[1] > 8u;
+    if (f0) unsupported |= *f0 != 0;
+    if (f1) unsupported |= *f1 != 0;
+    const uint64_t g = groups ? *groups : 0;
+    *d_ng = unsupported ? 0 : g;
+    *d_status = unsupported ? PIMDAL_E_UNSUPPORTED : g > cap ? PIMDAL_E_CAPACITY : PIMDAL_OK;
+}
+
+// k_q1_unpack with the group count read on the device: the first
+// min(*count, cap) staged groups.
+__global__ void k_q1_unpack_dev(const uint64_t* __restrict__ count, uint64_t cap, const uint32_t* __restrict__ key,
+                                const uint64_t* __restrict__ f, uint64_t stride, int32_t* __restrict__ okey,
+                                int64_t* __restrict__ sq, int64_t* __restrict__ sp, int64_t* __restrict__ sd,
+                                int64_t* __restrict__ sdp, int64_t* __restrict__ sch, int64_t* __restrict__ cnt) {
+    const uint64_t g = min(*count, cap);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < g; i += (uint64_t)gridDim.x * blockDim.x) {
+        okey[i] = (int32_t)key[i];
+        sq[i] = (int64_t)f[i];
+        sp[i] = (int64_t)f[stride + i];
+        sd[i] = (int64_t)f[2 * stride + i];
+        sdp[i] = (int64_t)f[3 * stride + i];
+        sch[2 * i] = (int64_t)f[4 * stride + i];
+        sch[2 * i + 1] = (int64_t)f[5 * stride + i];
+        cnt[i] = (int64_t)f[6 * stride + i];
+    }
+}
+
+// Q1 by key without host reads: pack (overflow flag on the device),
+// partition by hash digits (ordinary launches while capturing), one LDS
+// table per partition.  Partitions beyond an LDS table and measures beyond
+// the packed widths are reported through the status word instead of taking
+// the general path.
+template <typename Agg>
+static void q1_key_async(pimdal_ctx* c, const int32_t* date, const int32_t* key, const int64_t* qty,
+                         const int64_t* price, const int64_t* disc, const int64_t* tax, uint64_t n, int32_t cutoff,
+                         const OutQ1& o, uint64_t cap, uint64_t* d_ng, int32_t* d_status, hipStream_t s) {
+    Scratch pk(c, n * 4, s), pw(c, n * 8, s), ovf(c, 8, s);
+    fill_u32(c, ovf.get(), 0, 2, s);
+    const bool vec = ((uintptr_t)date | (uintptr_t)key | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc |
+                      (uintptr_t)tax) % 16 == 0;
+    const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 4), (uint64_t)c->cu_count * 8));
+    if (vec)
+        launch(c, "q1_pack", k_q1_pack<true>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
+               qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
+    else
+        launch(c, "q1_pack", k_q1_pack<false>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
+               qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
+    const uint32_t bits = gb_part_bits(c, n, (int64_t)Agg::TS * 2);
+    const uint32_t F = 1u << bits;
+    Scratch qk(c, n * 4, s), qv(c, n * 8, s), bounds(c, ((uint64_t)F + 1) * 4, s);
+    hash_partition(c, pk.as<uint32_t>(), pw.get(), 8, n, bits, qk.as<uint32_t>(), qv.get(), bounds.as<uint32_t>(), s);
+    Scratch meta(c, 8 * 4, s), tog(c, (uint64_t)F * 4, s);
+    fill_u64(c, meta.get(), 0, 4, s);
+    fill_u32(c, tog.get(), 0, F, s);
+    const uint32_t max_rows = (uint32_t)c->tune_get("gb_lds_max_rows", 65536);
+    const RowsQ1Packed src{qk.as<uint32_t>(), (const uint64_t*)qv.get()};
+    const bool direct_out = cap >= n;
+    Scratch sk(c, direct_out ? 0 : n * 4, s), sf(c, direct_out ? 0 : n * 8 * Agg::NF, s);
+    if (direct_out) {
+        launch(c, "gb_lds", k_gb_lds<Agg, RowsQ1Packed, OutQ1>, dim3(F), dim3(GB_LDS_BLOCK), 0, s, src,
+               (const uint32_t*)bounds.as<uint32_t>(), 1u, F, max_rows, tog.as<uint32_t>(), o, meta.as<uint64_t>());
+    } else {
+        const OutFields stage{sk.as<uint32_t>(), sf.as<uint64_t>(), n};
+        launch(c, "gb_lds", k_gb_lds<Agg, RowsQ1Packed, OutFields>, dim3(F), dim3(GB_LDS_BLOCK), 0, s, src,
+               (const uint32_t*)bounds.as<uint32_t>(), 1u, F, max_rows, tog.as<uint32_t>(), stage, meta.as<uint64_t>());
+        launch(c, "q1_unpack", k_q1_unpack_dev, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(cap, 256), 4096))),
+               dim3(256), 0, s, (const uint64_t*)meta.as<uint64_t>(), cap, (const uint32_t*)sk.as<uint32_t>(),
+               (const uint64_t*)sf.as<uint64_t>(), n, o.key, o.sq, o.sp, o.sd, o.sdp, o.sch, o.cnt);
+    }
+    launch(c, "q1_status", k_q1_async_status, dim3(1), dim3(64), 0, s, (const uint32_t*)nullptr,
+           (const uint32_t*)ovf.as<uint32_t>(), (const uint32_t*)(meta.as<uint64_t>() + 1),
+           (const uint64_t*)meta.as<uint64_t>(), cap, d_ng, d_status);
+}
+
+// The asynchronous Q1 forms.  Flags (key == NULL): the codes of every row's
+// flag bytes (one pass over the two flag columns), then the dense pass for
+// up to 8 codes and the reduction, which writes the groups in ascending key
+// order; by key: q1_key_async.  Results and status come back in device
+// words (k_q1_async_status).
+void filter_groupby_q1_async(pimdal_ctx* c, const int32_t* date, const uint8_t* f1, const uint8_t* f2,
+                             const int32_t* key, const int64_t* qty, const int64_t* price, const int64_t* disc,
+                             const int64_t* tax, uint64_t n, int32_t cutoff, const pimdal_q1_result& out, uint64_t cap,
+                             uint64_t* d_ng, int32_t* d_status, hipStream_t s) {
+    PD_REQUIRE(date && qty && price && disc && tax && (key || (f1 && f2)), PIMDAL_E_INVALID, "missing input column");
+    PD_REQUIRE(out.key && out.sum_qty && out.sum_price && out.sum_disc && out.sum_disc_price && out.sum_charge &&
+                   out.count, PIMDAL_E_INVALID, "missing output column");
+    PD_REQUIRE(d_ng && d_status, PIMDAL_E_INVALID, "missing result word");
+    PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "group-by supports up to 2^30-1 rows per call");
+    if (n == 0) {
+        launch(c, "q1_status", k_q1_async_status, dim3(1), dim3(64), 0, s, (const uint32_t*)nullptr,
+               (const uint32_t*)nullptr, (const uint32_t*)nullptr, (const uint64_t*)nullptr, cap, d_ng, d_status);
+        return;
+    }
+    if (key) {
+        const OutQ1 o{out.key, out.sum_qty, out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count};
+        if (c->tune_get("gb_q1_slots", 1024) == 1024)
+            q1_key_async<AggQ1T<1024>>(c, date, key, qty, price, disc, tax, n, cutoff, o, cap, d_ng, d_status, s);
+        else
+            q1_key_async<AggQ1T<2048>>(c, date, key, qty, price, disc, tax, n, cutoff, o, cap, d_ng, d_status, s);
+        return;
+    }
+    // scratch: ng u64 | miss | reduction counter | meta[2] | bits[16] | lut[512] | inv[512]
+    Scratch small(c, 8 + 8 + 8 + 64 + 512 + 512, s);
+    uint64_t* ng = small.as<uint64_t>();
+    uint32_t* miss = reinterpret_cast<uint32_t*>(ng + 1);
+    uint32_t* done_ctr = miss + 1;
+    uint32_t* meta = miss + 2;
+    uint32_t* bits = meta + 2;
+    uint8_t* lut = reinterpret_cast<uint8_t*>(bits + 16);
+    uint8_t* inv = lut + 512;
+    constexpr int GG = 8;
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, QD_BLOCK * 16), (uint64_t)c->cu_count * 8));
+    Scratch part(c, (uint64_t)nb * 8 * 8 * 8, s), sums(c, 8 * 6 * 16, s);
+    const bool vec = ((uintptr_t)date | (uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)qty | (uintptr_t)price |
+                      (uintptr_t)disc | (uintptr_t)tax) % 16 == 0;
+    fill_u32(c, miss, 0, 4 + 16, s);   // miss, counter, meta[2], bits[16]
+    const uint32_t gd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64), (uint64_t)c->cu_count * 4));
+    launch(c, "q1_domain", k_flag_domain, dim3(gd), dim3(256), 0, s, f1, f2, n, bits);
+    launch(c, "q1_lut", k_flag_lut, dim3(1), dim3(64), 0, s, (const uint32_t*)bits, lut, inv, meta);
+    if (vec)
+        launch(c, "q1_dense", k_q1_dense<GG, true>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc, tax, n,
+               cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
+    else
+        launch(c, "q1_dense", k_q1_dense<GG, false>, dim3(nb), dim3(QD_BLOCK), 0, s, date, f1, f2, qty, price, disc, tax,
+               n, cutoff, (const uint8_t*)lut, (const uint32_t*)meta, part.as<uint64_t>(), miss);
+    launch(c, "q1_reduce", k_q1_reduce, dim3(GG), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb, (uint32_t)GG,
+           sums.as<unsigned __int128>(), done_ctr, (const uint32_t*)meta, (const uint8_t*)inv, out.key, out.sum_qty,
+           out.sum_price, out.sum_disc, out.sum_disc_price, out.sum_charge, out.count, cap, ng);
+    launch(c, "q1_status", k_q1_async_status, dim3(1), dim3(64), 0, s, (const uint32_t*)meta, (const uint32_t*)nullptr,
+           (const uint32_t*)nullptr, (const uint64_t*)ng, cap, d_ng, d_status);
+}
+
+// Q6 with its results left in device words (revenue, rows); no host read.
+void q6_async(pimdal_ctx* c, const int32_t* date, const int64_t* qty, const int64_t* price, const int64_t* disc,
+              uint64_t n, const Q6Args& a, int64_t* d_revenue, uint64_t* d_rows, hipStream_t s) {
+    PD_REQUIRE((date && qty && price && disc) || n == 0, PIMDAL_E_INVALID, "missing input column");
+    PD_REQUIRE(d_revenue && d_rows, PIMDAL_E_INVALID, "missing result word");
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, Q6_BLOCK * 4 * 4),
+                                                                           (uint64_t)c->cu_count * 8));
+    Scratch part(c, (uint64_t)nb * 16, s), res(c, 16, s);
+    const bool vec = ((uintptr_t)date | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc) % 16 == 0;
+    if (vec)
+        launch(c, "q6", k_q6<true>, dim3(nb), dim3(Q6_BLOCK), 0, s, date, qty, price, disc, n, a, part.as<uint64_t>());
+    else
+        launch(c, "q6", k_q6<false>, dim3(nb), dim3(Q6_BLOCK), 0, s, date, qty, price, disc, n, a, part.as<uint64_t>());
+    launch(c, "q6_final", k_q6_final, dim3(1), dim3(256), 0, s, (const uint64_t*)part.as<uint64_t>(), nb,
+           res.as<uint64_t>());
+    copy_d2d(c, d_revenue, res.get(), 8, s);
+    copy_d2d(c, d_rows, res.as<uint64_t>() + 1, 8, s);
 }
 
 void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n, uint8_t* lut_dev, uint32_t meta[2],
@@ -1437,6 +1608,7 @@ extern "C" int pimdal_hash_aggr_dev(pimdal_ctx* c, const uint32_t* key, const in
                                     uint64_t* n_groups) {
     PD_API_BEGIN
     PD_REQUIRE(c && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     *n_groups = hash_aggr(c, key, val, n, out_key, out_sum, out_count, capacity, c->stream);
     PD_API_END
 }
@@ -1446,6 +1618,7 @@ extern "C" int pimdal_sort_aggr_dev(pimdal_ctx* c, const uint32_t* key, const in
                                     uint64_t* n_groups) {
     PD_API_BEGIN
     PD_REQUIRE(c && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     *n_groups = sort_aggr(c, key, val, n, out_key, out_sum, out_count, capacity, c->stream);
     PD_API_END
 }
@@ -1457,6 +1630,7 @@ extern "C" int pimdal_filter_groupby_q1_dev(pimdal_ctx* c, const int32_t* date, 
                                             uint64_t capacity, uint64_t* n_groups) {
     PD_API_BEGIN
     PD_REQUIRE(c && out && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     *n_groups = filter_groupby_q1(c, date, flag1, flag2, key, quantity, price, discount, tax, n, cutoff, *out,
                                   capacity, c->stream);
     PD_API_END
@@ -1467,8 +1641,32 @@ extern "C" int pimdal_q6_dev(pimdal_ctx* c, const int32_t* date, const int64_t* 
                              int64_t disc_hi, int64_t qty_lt, int64_t* revenue, uint64_t* n_rows) {
     PD_API_BEGIN
     PD_REQUIRE(c && revenue && n_rows, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     const Q6Args a{date_lo, date_hi, disc_lo, disc_hi, qty_lt};
     *n_rows = q6(c, date, quantity, price, discount, n, a, revenue, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_filter_groupby_q1_dev_async(pimdal_ctx* c, const int32_t* date, const uint8_t* flag1,
+                                                  const uint8_t* flag2, const int32_t* key, const int64_t* quantity,
+                                                  const int64_t* price, const int64_t* discount, const int64_t* tax,
+                                                  uint64_t n, int32_t cutoff, const pimdal_q1_result* out,
+                                                  uint64_t capacity, uint64_t* d_n_groups, int32_t* d_status) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && out, PIMDAL_E_INVALID, "NULL argument");
+    filter_groupby_q1_async(c, date, flag1, flag2, key, quantity, price, discount, tax, n, cutoff, *out, capacity,
+                            d_n_groups, d_status, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_q6_dev_async(pimdal_ctx* c, const int32_t* date, const int64_t* quantity, const int64_t* price,
+                                   const int64_t* discount, uint64_t n, int32_t date_lo, int32_t date_hi,
+                                   int64_t disc_lo, int64_t disc_hi, int64_t qty_lt, int64_t* d_revenue,
+                                   uint64_t* d_rows) {
+    PD_API_BEGIN
+    PD_REQUIRE(c, PIMDAL_E_INVALID, "NULL argument");
+    const Q6Args a{date_lo, date_hi, disc_lo, disc_hi, qty_lt};
+    q6_async(c, date, quantity, price, discount, n, a, d_revenue, d_rows, c->stream);
     PD_API_END
 }
 
@@ -1483,6 +1681,7 @@ int aggr_cols(AggFn fn, pimdal_ctx* c, const pimdal_col* key, const pimdal_col* 
               pimdal_col** os, pimdal_col** oc) {
     PD_API_BEGIN
     PD_REQUIRE(c && key && val && ok && os && oc, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     PD_REQUIRE(type_width(key->type) == 4, PIMDAL_E_UNSUPPORTED, "group key must be 32-bit");
     PD_REQUIRE(val->type == PIMDAL_I64 || val->type == PIMDAL_U64, PIMDAL_E_UNSUPPORTED, "values must be 64-bit");
     PD_REQUIRE(key->length == val->length, PIMDAL_E_INVALID, "column lengths differ");

@@ -81,6 +81,7 @@ using namespace pd;
 extern "C" int pimdal_autotune(pimdal_ctx* c, const char* op, uint64_t n, const char* cache_path) {
     PD_API_BEGIN
     PD_REQUIRE(c && op && n > 0, PIMDAL_E_INVALID, "bad argument");
+    PD_NO_CAPTURE(c);
     hipStream_t s = c->stream;
     std::vector<Knob> knobs;
     const std::string o(op);

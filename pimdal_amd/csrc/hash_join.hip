@@ -708,6 +708,7 @@ extern "C" int pimdal_hash_join_dev(pimdal_ctx* c, const uint32_t* build_key, co
                                     uint64_t out_capacity, uint64_t* out_count) {
     PD_API_BEGIN
     PD_REQUIRE(c && out_count, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     const uint64_t k = hash_join(c, build_key, build_payload, n_build, probe_key, probe_payload, n_probe,
                                  out_probe_payload, out_build_payload, out_capacity, c->stream);
     PD_HIP(hipStreamSynchronize(c->stream));

@@ -433,6 +433,7 @@ extern "C" int pimdal_sort_merge_join_dev(pimdal_ctx* c, const uint32_t* build_k
                                           uint64_t out_capacity, uint64_t* out_count) {
     PD_API_BEGIN
     PD_REQUIRE(c && out_count, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     const uint64_t k = sort_merge_join(c, build_key, build_payload, n_build, probe_key, probe_payload, n_probe,
                                        out_probe_payload, out_build_payload, out_capacity, c->stream);
     *out_count = k;
@@ -451,6 +452,7 @@ int join_cols(JoinFn fn, pimdal_ctx* c, const pimdal_col* bk, const pimdal_col* 
               const pimdal_col* pp, pimdal_col** op, pimdal_col** ob) {
     PD_API_BEGIN
     PD_REQUIRE(c && bk && pk && op && ob, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     PD_REQUIRE(type_width(bk->type) == 4 && type_width(pk->type) == 4, PIMDAL_E_UNSUPPORTED, "join keys must be 32-bit");
     PD_REQUIRE(!bp || (bp->length == bk->length && type_width(bp->type) == 4), PIMDAL_E_INVALID, "bad build payload");
     PD_REQUIRE(!pp || (pp->length == pk->length && type_width(pp->type) == 4), PIMDAL_E_INVALID, "bad probe payload");

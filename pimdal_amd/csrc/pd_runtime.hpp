@@ -26,6 +26,9 @@ void hip_check(hipError_t e, const char* what, const char* file, int line);
 #define PD_REQUIRE(cond, code, msg) do { if (!(cond)) ::pd::fail((code), (msg)); } while (0)
 
 void set_last_error(const std::string& m);
+// Errors reported at the C ABI by this thread so far (a plan's capture
+// checks that no call inside it failed).
+uint64_t error_count();
 
 // Caching device allocator.  Blocks are rounded to 256 B (small) or 2 MiB
 // (large) and kept per rounded size after free; a cached block is reused on
@@ -65,6 +68,16 @@ struct KernelStat { uint64_t launches = 0; double total_ms = 0.0; };
 
 }  // namespace pd
 
+// A captured operator sequence (hipGraph) and the device memory its nodes
+// use, held until the plan is destroyed.
+struct pimdal_plan {
+    pimdal_ctx* ctx = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> blocks;       // pool blocks the captured kernels use
+    uint64_t errors_at_begin = 0;
+    bool capturing = false;
+};
+
 struct pimdal_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -82,6 +95,11 @@ struct pimdal_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, pd::KernelStat> stats;
+    // the plan being captured on `stream` (nullptr: eager).  While set,
+    // scratch memory belongs to the plan, launches carry no profiling events,
+    // host reads of device results are refused, and the radix passes use
+    // ordinary launches instead of cooperative ones.
+    pimdal_plan* capture = nullptr;
     // device properties
     int cu_count = 256;
     int xcd_count = 8;
@@ -107,11 +125,18 @@ uint32_t type_width(pimdal_type t);
 
 // Temporary device buffer from the context pool, freed on scope exit
 // (stream ordered on the context stream).
+// While a plan is being captured the buffer belongs to the plan (freed
+// when the plan is destroyed): the captured kernels use it at every replay.
 class Scratch {
 public:
     Scratch(pimdal_ctx* c, uint64_t bytes, hipStream_t s = nullptr)
-        : c_(c), s_(s ? s : c->stream), p_(bytes ? c->pool.alloc(bytes, s ? s : c->stream) : nullptr) {}
-    ~Scratch() { if (p_) c_->pool.free(p_, s_); }
+        : c_(c), s_(s ? s : c->stream), p_(bytes ? c->pool.alloc(bytes, s ? s : c->stream) : nullptr) {
+        if (p_ && c->capture) {
+            c->capture->blocks.push_back(p_);
+            plan_owned_ = true;
+        }
+    }
+    ~Scratch() { if (p_ && !plan_owned_) c_->pool.free(p_, s_); }
     Scratch(const Scratch&) = delete;
     Scratch& operator=(const Scratch&) = delete;
     template <typename T> T* as() const { return reinterpret_cast<T*>(p_); }
@@ -120,6 +145,7 @@ private:
     pimdal_ctx* c_;
     hipStream_t s_;
     void* p_;
+    bool plan_owned_ = false;
 };
 
 // Kernel launch with optional hipEvent timing on the launch stream.
@@ -130,14 +156,15 @@ template <typename... KArgs, typename... Args>
 void launch(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
             uint32_t shmem, hipStream_t s, Args... args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    const bool prof = c->profiling && !c->capture;   // no timing events inside a captured plan
     hipEvent_t e0 = nullptr;
-    if (c->profiling) {
+    if (prof) {
         e0 = take_event(c);
         PD_HIP(hipEventRecord(e0, s));
     }
     hipLaunchKernelGGL(kernel, grid, block, shmem, s, static_cast<KArgs>(args)...);
     PD_HIP(hipGetLastError());
-    if (c->profiling) record_launch(c, name, e0, s);
+    if (prof) record_launch(c, name, e0, s);
 }
 
 // Cooperative launch: the runtime starts the grid only if every block is
@@ -154,6 +181,7 @@ template <typename... KArgs, typename... Args>
 bool launch_coop(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
                  uint32_t shmem, hipStream_t s, Args... args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return true;
+    if (c->capture) return false;   // plans use ordinary launches: taken as a refusal
     std::tuple<KArgs...> vals(static_cast<KArgs>(args)...);
     void* ptrs[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1];
     std::apply([&](auto&... v) { int i = 0; ((ptrs[i++] = (void*)&v), ...); }, vals);
@@ -190,11 +218,19 @@ void copy_d2d(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes, hipStre
 
 uint64_t read_u64(pimdal_ctx* c, const void* dev, hipStream_t s);
 // Copies `bytes` (<= 64 through pinned memory) from the device and waits.
+// Refused (PIMDAL_E_UNSUPPORTED) while a plan is being captured.
 void read_small(pimdal_ctx* c, void* dst, const void* dev, uint64_t bytes, hipStream_t s);
 
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 }  // namespace pd
+
+// Entry points that wait for the device or return host results cannot be
+// recorded into a plan.
+#define PD_NO_CAPTURE(c)                                                                            \
+    PD_REQUIRE(!(c)->capture, PIMDAL_E_UNSUPPORTED,                                                 \
+               "this call waits for results on the host and cannot be captured into a plan; use "   \
+               "an asynchronous device form")
 
 // C-ABI boundary: run `body`, translate exceptions into return codes.
 #define PD_API_BEGIN try {

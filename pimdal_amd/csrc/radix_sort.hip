@@ -692,7 +692,9 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     const int64_t xs_knob = std::min<int64_t>(c->tune_get("sort_xcd_tiles", 16), 4096);
     const bool xp_shape = !early && !split && nt == 0 &&
                           ((block == 512 && items == 16) || (block == 256 && items == 16 && !ballot));
-    bool xp = xp_shape && xs_knob > 0;   // cleared if the runtime refuses the co-resident grid
+    // cleared if the runtime refuses the co-resident grid; never while a plan
+    // is captured (cooperative launches are not recorded into plans)
+    bool xp = xp_shape && xs_knob > 0 && !c->capture;
     auto xs_of = [&](uint32_t) -> uint32_t { return xp ? (uint32_t)xs_knob : 0u; };
     // status words: XCD-persistent passes scan their digit histogram
     // themselves and clear the next pass's statuses (two buffers in turn);
@@ -1109,6 +1111,7 @@ extern "C" int pimdal_sort_pairs(pimdal_ctx* c, const pimdal_col* key, const pim
                                  pimdal_col** out_key, pimdal_col** out_payload) {
     PD_API_BEGIN
     PD_REQUIRE(c && key && out_key, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     PD_REQUIRE(key->type == PIMDAL_U32, PIMDAL_E_UNSUPPORTED, "sort key must be uint32");
     PD_REQUIRE(!payload || payload->length == key->length, PIMDAL_E_INVALID, "payload length differs");
     PD_REQUIRE(!payload || out_payload, PIMDAL_E_INVALID, "out_payload is NULL");

@@ -15,7 +15,14 @@ namespace pd {
 
 static thread_local std::string g_last_error;
 
-void set_last_error(const std::string& m) { g_last_error = m; }
+static thread_local uint64_t g_error_count = 0;
+
+void set_last_error(const std::string& m) {
+    g_last_error = m;
+    g_error_count++;
+}
+
+uint64_t error_count() { return g_error_count; }
 
 [[noreturn]] void fail(int code, const std::string& msg) { throw Error(code, msg); }
 
@@ -238,6 +245,8 @@ uint64_t read_u64(pimdal_ctx* c, const void* dev, hipStream_t s) {
 // of the context: a copy into pageable memory goes through the runtime's
 // staging path, several times slower for a few bytes.
 void read_small(pimdal_ctx* c, void* dst, const void* dev, uint64_t bytes, hipStream_t s) {
+    PD_REQUIRE(!c->capture, PIMDAL_E_UNSUPPORTED,
+               "a result read back to the host cannot be captured into a plan; use an asynchronous device form");
     if (bytes > sizeof(c->readback_words)) {
         PD_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
@@ -341,6 +350,14 @@ extern "C" int pimdal_ctx_destroy(pimdal_ctx* c) {
     PD_API_BEGIN
     if (!c) return PIMDAL_OK;
     (void)hipSetDevice(c->device);
+    if (c->capture) {   // a capture left open: end it (the plan itself stays with the caller, empty)
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        c->capture->capturing = false;
+        c->capture = nullptr;
+    }
     (void)hipStreamSynchronize(c->stream);
     c->pool.destroy();
     for (int i = 0; i < 2; i++) {
@@ -358,6 +375,7 @@ extern "C" int pimdal_ctx_destroy(pimdal_ctx* c) {
 extern "C" int pimdal_ctx_sync(pimdal_ctx* c) {
     PD_API_BEGIN
     PD_REQUIRE(c, PIMDAL_E_INVALID, "ctx is NULL");
+    PD_NO_CAPTURE(c);
     PD_HIP(hipStreamSynchronize(c->stream));
     PD_API_END
 }
@@ -460,6 +478,7 @@ extern "C" int pimdal_dev_copy(pimdal_ctx* c, void* dst, const void* src, uint64
 extern "C" int pimdal_memcpy_h2d(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes) {
     PD_API_BEGIN
     PD_REQUIRE(c && (bytes == 0 || (dst && src)), PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     stage_h2d(c, dst, src, bytes);
     PD_API_END
 }
@@ -467,6 +486,7 @@ extern "C" int pimdal_memcpy_h2d(pimdal_ctx* c, void* dst, const void* src, uint
 extern "C" int pimdal_memcpy_d2h(pimdal_ctx* c, void* dst, const void* src, uint64_t bytes) {
     PD_API_BEGIN
     PD_REQUIRE(c && (bytes == 0 || (dst && src)), PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     stage_d2h(c, dst, src, bytes);
     PD_API_END
 }
@@ -500,6 +520,7 @@ extern "C" int pimdal_col_from_host(pimdal_ctx* c, pimdal_type type, const void*
 extern "C" int pimdal_col_wrap(pimdal_ctx* c, pimdal_type type, void* dptr, uint64_t length, pimdal_col** out) {
     PD_API_BEGIN
     PD_REQUIRE(c && out && (dptr || length == 0), PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     (void)type_width(type);
     *out = new pimdal_col{type, length, length, dptr, false};
     PD_API_END
@@ -508,6 +529,7 @@ extern "C" int pimdal_col_wrap(pimdal_ctx* c, pimdal_type type, void* dptr, uint
 extern "C" int pimdal_col_to_host(pimdal_ctx* c, const pimdal_col* col, void* dst, uint64_t dst_bytes) {
     PD_API_BEGIN
     PD_REQUIRE(c && col && (dst || col->length == 0), PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     const uint64_t b = col->length * type_width(col->type);
     PD_REQUIRE(dst_bytes >= b, PIMDAL_E_CAPACITY, "destination too small");
     stage_d2h(c, dst, col->data, b);
@@ -587,6 +609,7 @@ static const char* import_values(const struct ArrowArray* a, uint32_t w) {
 extern "C" int pimdal_col_import(pimdal_ctx* c, const struct ArrowArray* a, const struct ArrowSchema* s, pimdal_col** out) {
     PD_API_BEGIN
     PD_REQUIRE(c && a && s && out, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     return pimdal_col_import_chunks(c, &a, 1, s, out);
     PD_API_END
 }
@@ -595,6 +618,7 @@ extern "C" int pimdal_col_import_chunks(pimdal_ctx* c, const struct ArrowArray* 
                                         const struct ArrowSchema* s, pimdal_col** out) {
     PD_API_BEGIN
     PD_REQUIRE(c && s && out && (chunks || nchunks == 0), PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     const pimdal_type t = import_type(s);
     const uint32_t w = type_width(t);
     // every chunk is checked before anything is allocated or copied
@@ -642,6 +666,7 @@ void release_schema(struct ArrowSchema* s) {
 extern "C" int pimdal_col_export(pimdal_ctx* c, const pimdal_col* col, struct ArrowArray* out, struct ArrowSchema* os) {
     PD_API_BEGIN
     PD_REQUIRE(c && col && out && os, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     const uint64_t bytes = col->length * type_width(col->type);
     void* host = aligned_alloc(64, std::max<uint64_t>((bytes + 63) & ~63ull, 64));
     if (!host) throw std::bad_alloc();
@@ -674,5 +699,87 @@ extern "C" int pimdal_col_export(pimdal_ctx* c, const pimdal_col* col, struct Ar
     os->dictionary = nullptr;
     os->release = release_schema;
     os->private_data = nullptr;
+    PD_API_END
+}
+
+// ---------------- C ABI: captured plans ----------------
+
+extern "C" int pimdal_plan_begin(pimdal_ctx* c, pimdal_plan** out) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && out, PIMDAL_E_INVALID, "NULL argument");
+    PD_REQUIRE(!c->capture, PIMDAL_E_INVALID, "a plan is already being captured on this context");
+    auto* p = new pimdal_plan();
+    p->ctx = c;
+    // relaxed: the pool may take new device memory for the plan while capturing
+    const hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        delete p;
+        PD_HIP(e);
+    }
+    p->capturing = true;
+    p->errors_at_begin = pd::error_count();
+    c->capture = p;
+    *out = p;
+    PD_API_END
+}
+
+static void plan_release(pimdal_plan* p) {
+    pimdal_ctx* c = p->ctx;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+    for (void* b : p->blocks) c->pool.free(b, c->stream);
+    p->blocks.clear();
+}
+
+extern "C" int pimdal_plan_end(pimdal_plan* p) {
+    PD_API_BEGIN
+    PD_REQUIRE(p && p->capturing, PIMDAL_E_INVALID, "plan is not being captured");
+    pimdal_ctx* c = p->ctx;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    p->capturing = false;
+    c->capture = nullptr;
+    const bool failed = pd::error_count() != p->errors_at_begin;
+    if (e != hipSuccess || failed || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        plan_release(p);
+        PD_REQUIRE(!failed, PIMDAL_E_INVALID, "a call inside the capture failed; the plan holds nothing to launch");
+        PD_HIP(e);
+        pd::fail(PIMDAL_E_INTERNAL, "capture produced no graph");
+    }
+    const hipError_t ei = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        p->exec = nullptr;
+        plan_release(p);
+        PD_HIP(ei);
+    }
+    PD_API_END
+}
+
+extern "C" int pimdal_plan_launch(pimdal_plan* p) {
+    PD_API_BEGIN
+    PD_REQUIRE(p && p->exec && !p->capturing, PIMDAL_E_INVALID, "plan has no instantiated graph");
+    PD_REQUIRE(!p->ctx->capture, PIMDAL_E_INVALID, "cannot launch a plan while capturing another");
+    PD_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
+    PD_API_END
+}
+
+extern "C" int pimdal_plan_destroy(pimdal_plan* p) {
+    PD_API_BEGIN
+    if (!p) return PIMDAL_OK;
+    pimdal_ctx* c = p->ctx;
+    if (p->capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        c->capture = nullptr;
+        p->capturing = false;
+    }
+    (void)hipStreamSynchronize(c->stream);   // no replay still running on its memory
+    plan_release(p);
+    delete p;
     PD_API_END
 }

@@ -534,6 +534,7 @@ extern "C" int pimdal_select(pimdal_ctx* c, const pimdal_col* key, const pimdal_
                              pimdal_col** out_payload, pimdal_col** out_rowid) {
     PD_API_BEGIN
     PD_REQUIRE(c && key, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     PD_REQUIRE(!payload || payload->length == key->length, PIMDAL_E_INVALID, "payload length differs from key length");
     PD_REQUIRE(!payload || type_width(payload->type) == 4, PIMDAL_E_UNSUPPORTED, "payload must be 4 bytes wide");
     PD_REQUIRE(out_payload || out_rowid, PIMDAL_E_INVALID, "no output requested");

@@ -568,6 +568,7 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
                                     pimdal_server** out) {
     PD_API_BEGIN
     PD_REQUIRE(c && tables && out, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
     PD_REQUIRE(slots >= 1 && slots <= 16, PIMDAL_E_INVALID, "slots must be in [1, 16]");
     const pimdal_tables& t = *tables;
     PD_REQUIRE(t.orderkey && t.quantity && t.price && t.discount && t.tax && t.date && t.flag1 && t.flag2 && t.n_fact,
@@ -669,6 +670,7 @@ extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint6
                                  pimdal_query_result* res, pimdal_serve_stats* stats) {
     PD_API_BEGIN
     PD_REQUIRE(S && (qs || !nq) && (res || !nq), PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(S->ctx);
     PD_REQUIRE(mode == PIMDAL_SERVE_SEQUENTIAL || mode == PIMDAL_SERVE_BATCHED, PIMDAL_E_INVALID, "unknown mode");
     for (uint64_t i = 0; i < nq; i++) {
         const pimdal_query& q = qs[i];

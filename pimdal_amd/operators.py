@@ -214,3 +214,95 @@ def filter_groupby_q1(ctx: Context, t: dict, cutoff: int, by_key: bool = False, 
     ch = h["sum_charge"].reshape(k, 2)
     h["sum_charge"] = [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in ch]
     return h
+
+
+def _q1_outputs(ctx: Context, cap: int):
+    out = {f: ctx.empty(cap * (2 if f == "sum_charge" else 1), np.int32 if f == "key" else np.int64)
+           for f in Q1_FIELDS}
+    return out, _Q1Result(*[out[f].ptr for f in Q1_FIELDS])
+
+
+def filter_groupby_q1_async(ctx: Context, t: dict, cutoff: int, by_key: bool = False, capacity: int | None = None):
+    """The asynchronous Q1 form (pimdal_filter_groupby_q1_dev_async): enqueues
+    the work and returns (outputs, d_groups, d_status) as device arrays;
+    nothing is read back, so it can run inside a Plan."""
+    n = t["date"].n
+    cap = max(1, (64 if not by_key else n) if capacity is None else int(capacity))
+    out, res = _q1_outputs(ctx, cap)
+    d_ng, d_st = ctx.empty(1, np.uint64), ctx.empty(1, np.int32)
+    out["_res"] = res   # the descriptor is read at the call only; kept with the outputs for clarity
+    call("pimdal_filter_groupby_q1_dev_async", ctx.handle, t["date"].vp, t["flag1"].vp, t["flag2"].vp,
+         t["orderkey"].vp if by_key else None, t["quantity"].vp, t["price"].vp, t["discount"].vp, t["tax"].vp,
+         n, int(cutoff), C.byref(res), cap, d_ng.vp, d_st.vp)
+    return out, d_ng, d_st
+
+
+def q1_to_host(out: dict, d_ng: DeviceArray, d_st: DeviceArray):
+    """(status, host result) of an asynchronous Q1 call once its stream has run
+    (status 0, or the negative PIMDAL_E_* code the call left)."""
+    st = int(d_st.numpy()[0])
+    k = int(d_ng.numpy()[0])
+    cap = out["key"].n
+    kk = min(k, cap)
+    h = {f: out[f].numpy(2 * kk if f == "sum_charge" else kk) for f in Q1_FIELDS}
+    ch = h["sum_charge"].reshape(kk, 2)
+    h["sum_charge"] = [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in ch]
+    h["groups"] = k
+    return st, h
+
+
+def q6_async(ctx: Context, t: dict, date_lo: int = 731, date_hi: int = 1096, disc_lo: int = 5, disc_hi: int = 7,
+             qty_lt: int = 24):
+    """pimdal_q6_dev_async: (d_revenue int64[1], d_rows uint64[1]) device words."""
+    rev, rows = ctx.empty(1, np.int64), ctx.empty(1, np.uint64)
+    call("pimdal_q6_dev_async", ctx.handle, t["date"].vp, t["quantity"].vp, t["price"].vp, t["discount"].vp,
+         t["date"].n, int(date_lo), int(date_hi), int(disc_lo), int(disc_hi), int(qty_lt), rev.vp, rows.vp)
+    return rev, rows
+
+
+class Plan:
+    """A captured sequence of asynchronous device calls (pimdal_plan_*):
+
+        with Plan(ctx) as plan:       # capture
+            ...asynchronous calls...
+        plan.launch(); ctx.sync()     # replay
+
+    The calls inside the block are recorded, not run.  The plan keeps the
+    scratch memory of the captured calls until destroy()."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.handle = None
+
+    def __enter__(self):
+        h = C.c_void_p()
+        call("pimdal_plan_begin", self.ctx.handle, C.byref(h))
+        self.handle = h
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        if exc_type is None:
+            try:
+                call("pimdal_plan_end", self.handle)
+            except Exception:
+                self.destroy()
+                raise
+        else:
+            self.destroy()
+        return False
+
+    def launch(self) -> None:
+        call("pimdal_plan_launch", self.handle)
+
+    def destroy(self) -> None:
+        if self.handle:
+            call("pimdal_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                _lib.load().pimdal_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
