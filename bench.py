@@ -182,6 +182,40 @@ def bench_q1(ctx, steps, warmup, by_key, n=60_000_000):
     return res
 
 
+def bench_q1_plan(ctx, steps, warmup, by_key, n=60_000_000):
+    """The same pipeline captured once as a plan (one hipGraph) and replayed:
+    a step is one graph launch.  By key, the partition fan-out and table size
+    are autotuned at this size first (pimdal_autotune "q1")."""
+    from pimdal_amd import operators as ops, datagen
+    t = datagen.fact_table(ctx, n, groups=15_000_000)
+    tuned = {}
+    if by_key:
+        ctx.autotune("q1", n)
+        tuned = {k: ctx.get_tune(k) for k in ("gb_rows_per_part", "gb_q1_slots")}
+    with ops.Plan(ctx) as plan:
+        out, ng, st = ops.filter_groupby_q1_async(ctx, t, 2505, by_key=by_key)
+    for k in tuned:
+        ctx.clear_tune(k)   # baked into the captured graph; later operators take their defaults
+    for _ in range(warmup):
+        plan.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.launch()
+    ctx.sync()
+    tm = (time.perf_counter() - t0) / steps
+    status = int(st.numpy()[0])
+    groups = int(ng.numpy()[0])
+    plan.destroy()
+    alg = 42 * n + 88 * int(0.98 * n) + 52 * groups if by_key else 38 * n
+    res = {"rows": n, "groups": groups, "status": status, "s_per_step": tm, "rows_per_s": n / tm, "alg_bytes": alg,
+           "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK, "graph_launches_per_step": 1,
+           "autotuned": tuned}
+    for a in list(t.values()) + [a for k, a in out.items() if k != "_res"] + [ng, st]:
+        a.free()
+    return res
+
+
 def bench_date32(ctx, steps, warmup, n=60_000_000):
     """Arrow date32 column (days over 1992-1998) to the reference's packed dates, 6e7 values."""
     from pimdal_amd import operators as ops
@@ -364,6 +398,9 @@ def main():
         for by_key in (False, True):
             log(f"bench: filter+group-by 6e7 by_key={by_key}")
             ops_out["q1_orderkey" if by_key else "q1_flags"] = bench_q1(ctx, max(3, args.steps // 2), 1, by_key)
+            log(f"bench: filter+group-by 6e7 by_key={by_key}, one captured plan")
+            ops_out["q1_orderkey_plan" if by_key else "q1_flags_plan"] = bench_q1_plan(ctx, max(3, args.steps // 2), 1,
+                                                                                         by_key)
         log("bench: TPC-H Q6 shape 6e7")
         ops_out["q6"] = bench_q6(ctx, max(3, args.steps // 2), 1)
     if args.only in ("", "all", "ingest"):

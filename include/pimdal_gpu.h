@@ -104,7 +104,9 @@ int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
 /* Forgets a knob: the built-in default applies again. */
 int pimdal_ctx_clear_tune(pimdal_ctx* ctx, const char* key);
 /* Autotuner: times candidate knob values for `op` ("sort", "select",
- * "hash_join", "groupby") on synthetic device data of n rows, keeps the
+ * "hash_join", "groupby", "q1": the filter + group-by by key, its partition
+ * fan-out gb_rows_per_part and table size gb_q1_slots) on synthetic device
+ * data of n rows, keeps the
  * fastest in the context, and appends them to cache_path (may be NULL).
  * No counterpart in the reference (its sizes and fan-outs are compile-time
  * CMake definitions, pimdal/main/CMakeLists.txt:11-19). */

@@ -129,8 +129,30 @@ extern "C" int pimdal_autotune(pimdal_ctx* c, const char* op, uint64_t n, const 
             hash_aggr(c, k.as<uint32_t>(), v.as<int64_t>(), n, ok.as<uint32_t>(), os.as<int64_t>(), oc.as<int64_t>(), n, s);
         }, 2);
         c->tune["gb_groups_hint"] = hint;
+    } else if (o == "q1") {
+        // the filter + group-by by orderkey over a fact table of n rows (four
+        // rows per key, measures in their TPC-H ranges): partition fan-out
+        // (rows per partition) and LDS table size of the per-partition tables
+        PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "q1 tuning size too large");
+        Scratch ok(c, n * 4), qt(c, n * 8), pr(c, n * 8), di(c, n * 8), tx(c, n * 8), dt(c, n * 4);
+        const uint64_t seed = 0x5EED0004ull;
+        auto chk = [](int rc) { PD_REQUIRE(rc == PIMDAL_OK, rc, pimdal_last_error()); };
+        chk(pimdal_gen_orderkey(c, seed, n, std::max<uint64_t>(1, n / 4), ok.as<int32_t>()));
+        chk(pimdal_gen_range(c, PIMDAL_I64, seed, 12, n, 1, 50, qt.get()));
+        chk(pimdal_gen_range(c, PIMDAL_I64, seed, 13, n, 90000, 10500000, pr.get()));
+        chk(pimdal_gen_range(c, PIMDAL_I64, seed, 14, n, 0, 10, di.get()));
+        chk(pimdal_gen_range(c, PIMDAL_I64, seed, 15, n, 0, 8, tx.get()));
+        chk(pimdal_gen_range(c, PIMDAL_I32, seed, 16, n, 0, 2556, dt.get()));
+        Scratch rk(c, n * 4), r1(c, n * 8), r2(c, n * 8), r3(c, n * 8), r4(c, n * 8), r5(c, n * 16), r6(c, n * 8);
+        const pimdal_q1_result res{rk.as<int32_t>(), r1.as<int64_t>(), r2.as<int64_t>(), r3.as<int64_t>(),
+                                   r4.as<int64_t>(), r5.as<int64_t>(), r6.as<int64_t>()};
+        knobs = {{"gb_rows_per_part", {2048, 1024, 4096}}, {"gb_q1_slots", {1024, 2048}}};
+        tune(c, knobs, [&] {
+            filter_groupby_q1(c, dt.as<int32_t>(), nullptr, nullptr, ok.as<int32_t>(), qt.as<int64_t>(),
+                              pr.as<int64_t>(), di.as<int64_t>(), tx.as<int64_t>(), n, 2505, res, n, s);
+        }, 2);
     } else {
-        fail(PIMDAL_E_INVALID, "unknown operator '" + o + "' (sort, select, hash_join, groupby)");
+        fail(PIMDAL_E_INVALID, "unknown operator '" + o + "' (sort, select, hash_join, groupby, q1)");
     }
     if (cache_path && *cache_path) {
         FILE* f = fopen(cache_path, "a");
