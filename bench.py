@@ -297,7 +297,7 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
     ctx.set_budget(64 << 30)   # the serving stream runs under a fixed 64 GiB HBM budget
     fact = datagen.fact_table(ctx, n_fact, groups=n_dim)
     dim = datagen.dim_table(ctx, n_dim)
-    srv = serving.Server(ctx, fact, dim, slots=slots, arena_bytes=1 << 30)
+    srv = serving.Server(ctx, fact, dim, slots=slots, heap_bytes=4 << 30)
     wq = serving.gen_queries(warm, n_fact, rate_qps, seed=0x5EED0105)
     srv.run(wq, serving.BATCHED)
     srv.run(wq[:50], serving.SEQUENTIAL)
@@ -308,7 +308,8 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
         r_bat, st_bat = srv.run(q, serving.BATCHED)
         same = bool(np.array_equal(r_seq["count"], r_bat["count"]) and np.array_equal(r_seq["checksum"], r_bat["checksum"]))
         return {"queries": n, "arrival_rate_qps": rate,
-                "batched": {k: st_bat[k] for k in ("qps", "p50_us", "p99_us", "mean_us", "batches", "max_inflight")},
+                "batched": {k: st_bat[k] for k in ("qps", "p50_us", "p99_us", "mean_us", "batches", "max_inflight",
+                                                   "heap_peak_bytes")},
                 "sequential": {k: st_seq[k] for k in ("qps", "p50_us", "p99_us", "mean_us")},
                 "qps_speedup": st_bat["qps"] / st_seq["qps"] if st_seq["qps"] else None, "results_identical": same}
 
@@ -316,7 +317,8 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
     res["slots"] = slots
     res["rate_sweep"] = [at(r, int(min(nq, 4 * r))) for r in sweep]
     res["hbm_budget_bytes"] = 64 << 30
-    res["hbm_live_bytes"] = ctx.mem_stats()["live"]   # resident tables, server indexes and arenas
+    res["hbm_live_bytes"] = ctx.mem_stats()["live"]   # resident tables, server indexes and the serving heap
+    res["heap_bytes"] = 4 << 30
     srv.close()
     for d in (fact, dim):
         for a in d.values():

@@ -116,7 +116,11 @@ enum {
 #define PD_CK_SELECT 0x53454C4543540001ULL
 #define PD_CK_JOIN 0x4A4F494E00000002ULL
 #define PD_CK_GROUP 0x47524F5550000003ULL
-PD_GEN_FN uint64_t pd_ck_select(uint64_t fact_row) { return pd_mix64(fact_row ^ PD_CK_SELECT); }
+/* Selected fact row at output position pos (both < 2^32): the sum over a
+ * query's output depends on the order of the rows, not only on the set. */
+PD_GEN_FN uint64_t pd_ck_select(uint64_t fact_row, uint64_t pos) {
+    return pd_mix64(((pos << 32) | (fact_row & 0xFFFFFFFFull)) ^ PD_CK_SELECT);
+}
 PD_GEN_FN uint64_t pd_ck_join(uint64_t fact_row, uint64_t dim_row) {
     return pd_mix64(((fact_row << 32) | dim_row) ^ PD_CK_JOIN);
 }

@@ -387,7 +387,9 @@ int pimdal_plan_destroy(pimdal_plan* plan);
  * (include/pimdal_gen.h: pd_ck_select / pd_ck_join / pd_ck_group).
  * PIMDAL_SERVE_SEQUENTIAL runs one query at a time through the plain
  * operators; PIMDAL_SERVE_BATCHED coalesces arrived queries into batches on
- * `slots` streams (continuous batching, one hipGraph replay per batch). */
+ * `slots` streams (continuous batching, one hipGraph replay per batch).
+ * The select checksum depends on the output order (each row id with its
+ * output position); join and group-by checksums do not. */
 typedef struct pimdal_server pimdal_server;
 
 enum { PIMDAL_Q_SELECT = 1, PIMDAL_Q_JOIN = 2, PIMDAL_Q_GROUPBY = 3 };
@@ -411,7 +413,7 @@ typedef struct {
 
 typedef struct {
     int32_t kind;
-    uint32_t reserved;
+    uint32_t tenant;    /* admission shares the serving heap among the tenants with work */
     uint64_t offset;
     uint64_t rows;
     int64_t param;
@@ -427,13 +429,23 @@ typedef struct {
 typedef struct {
     double makespan_us, qps, p50_us, p99_us, mean_us;
     uint64_t batches, max_inflight;
+    uint64_t heap_bytes, heap_peak_bytes;   /* serving heap and its largest use during the run */
+    uint64_t tenants;
 } pimdal_serve_stats;
 
 /* Builds the resident dimension-key index (and, unless the context's
  * serve_join_index knob is 0, the foreign-key join index: the dimension row
- * and date of every fact row) and `slots` stream slots, each with an output
- * arena of slot_arena_bytes (0: 1 GiB) charged to the budget. */
-int pimdal_server_create(pimdal_ctx* ctx, const pimdal_tables* tables, int slots, uint64_t slot_arena_bytes,
+ * and date of every fact row, read by the joins of both modes), `slots`
+ * stream slots, and the serving heap of heap_bytes (0: half of what the
+ * context's budget leaves, at most 8 GiB), charged to the budget.  A
+ * batched query occupies its footprint in the heap while its batch runs:
+ * 4 bytes per slice row (select) or 8 (join) for the outputs, 8 per
+ * 4096-row tile for look-back statuses, 512 per 2^18-row chunk (group-by).
+ * Batches are admitted while their footprint fits the heap's largest free
+ * range; a query larger than the whole heap is refused (PIMDAL_E_NOMEM).
+ * Queries of each tenant are admitted in arrival order, tenants in turn;
+ * a tenant holds at most heap / (tenants with work) while others wait. */
+int pimdal_server_create(pimdal_ctx* ctx, const pimdal_tables* tables, int slots, uint64_t heap_bytes,
                          pimdal_server** out);
 int pimdal_server_run(pimdal_server* server, const pimdal_query* queries, uint64_t n, int mode,
                       pimdal_query_result* results, pimdal_serve_stats* stats);

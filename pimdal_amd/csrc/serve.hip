@@ -5,19 +5,24 @@
 //  * mode PIMDAL_SERVE_SEQUENTIAL — one query at a time through the plain
 //    operator entry points (the drop-in path): select_range over the price
 //    column; for a join, a select of the qualifying dimension rows followed
-//    by hash_join against the fact slice; the fused Q1 group-by over the
-//    slice.  Each query completes before the next starts.
+//    by hash_join against the fact slice (a select over the resident join
+//    index when there is one); the fused Q1 group-by over the slice.  Each query completes before the next starts.
 //  * mode PIMDAL_SERVE_BATCHED — continuous batching: arrivals queue up
-//    while earlier batches run; whenever one of `slots` stream slots is free
-//    and its output arena can hold them, every arrived query (FIFO) is
-//    coalesced into one batch.  A batch is one hipGraph replay per slot:
+//    (one FIFO per tenant) while earlier batches run; whenever one of
+//    `slots` stream slots is free, arrived queries are coalesced into one
+//    batch, taken round-robin over the tenants, as long as the batch's
+//    block of the serving heap (the footprint of its queries' outputs,
+//    statuses and partials) fits the heap's largest free range and each
+//    tenant stays within its share of the heap.  A batch is one hipGraph replay per slot:
 //    descriptor copy -> reset -> persistent select / join / group-by kernels
 //    that walk a device work list of (query, tile) items -> group-by
 //    finalisation -> result copy.  Joins read a foreign-key join index (the
 //    dimension row and date of every fact row, built once at server
 //    creation from a dimension-key index, since both tables stay resident),
 //    or with serve_join_index = 0 probe the dimension-key index per row.
-//    Slots run on their own streams, so batches overlap on the GPU.
+//    Slots run on their own streams, so batches overlap on the GPU.  The
+//    sequential mode's joins read the same join index (one select over it),
+//    so the two modes differ only in batching.
 // Both modes report per query the result count and an order-independent
 // checksum (include/pimdal_gen.h), so results can be compared across modes
 // and with the CPU restatement.
@@ -25,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -57,9 +63,13 @@ struct SvKind {
     SvQuery q[SV_MAXQ];
 };
 
+// One batch: its queries by kind, and the byte offsets of its block's
+// regions in the serving heap (select outputs, join outputs, look-back
+// status words, group-by partials), set at admission.
 struct SvBatch {
     SvKind sel, join, gb;
     uint64_t status_words;
+    uint64_t arena_off, status_off, partials_off;
 };
 
 struct SvResult {
@@ -100,9 +110,10 @@ __device__ __forceinline__ uint32_t sv_find_query(const SvKind& k, uint64_t wi) 
 // linear probing at load <= 0.5: one slot read per probe, and the date comes
 // with the slot, so the join predicate needs no second random read.
 
-__global__ void k_sv_reset(const SvBatch* __restrict__ b, uint64_t* __restrict__ status, SvResult* __restrict__ res,
+__global__ void k_sv_reset(const SvBatch* __restrict__ b, uint8_t* __restrict__ heap, SvResult* __restrict__ res,
                            uint32_t nres, uint32_t* __restrict__ ctr) {
     const uint64_t n = b->status_words;
+    uint64_t* status = reinterpret_cast<uint64_t*>(heap + b->status_off);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         status[i] = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nres; i += gridDim.x * blockDim.x) res[i] = SvResult{0, 0};
@@ -113,15 +124,17 @@ __global__ void k_sv_reset(const SvBatch* __restrict__ b, uint64_t* __restrict__
 // query: decoupled look-back over its own tiles (items are taken in order,
 // so a predecessor tile is always held by a running block), order-
 // preserving output, checksum added atomically (a sum, so the order of the
-// additions does not matter).
+// additions does not matter; a select's terms carry each row's output
+// position, so its checksum does depend on the output order).
 template <bool JOIN>
 __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict__ b, SvTables T,
-                                                      void* __restrict__ arena, uint64_t* __restrict__ status,
-                                                      SvResult* __restrict__ res, uint32_t* __restrict__ ctr) {
+                                                      uint8_t* __restrict__ heap, SvResult* __restrict__ res,
+                                                      uint32_t* __restrict__ ctr) {
+    void* arena = heap + b->arena_off;
+    uint64_t* status = reinterpret_cast<uint64_t*>(heap + b->status_off);
     __shared__ uint64_t s_wi;
     __shared__ uint32_t s_wave_tot[SV_BLOCK / PD_WAVE];
     __shared__ uint64_t s_excl;
-    __shared__ uint64_t s_ck[SV_BLOCK / PD_WAVE];
     const SvKind& K = JOIN ? b->join : b->sel;
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     constexpr int RPT = SV_TILE / SV_BLOCK;  // 16 rows per thread: row = tile_base + i*256 + t
@@ -193,20 +206,12 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
         }
         // order (i, wave, lane): per item i, counts per wave
         __shared__ uint32_t s_cnt[RPT][SV_BLOCK / PD_WAVE];
-        uint64_t ck = 0;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             const bool m = (hit >> i) & 1;
             const uint64_t bal = pd::ballot(m);
             if (l == 0) s_cnt[i][w] = pd::popc64(bal);
-            if (m) {
-                const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
-                if constexpr (JOIN) ck += pd_ck_join(r, dimrow[i]);
-                else ck += pd_ck_select(r);
-            }
         }
-        ck = pd::wave_sum64(ck);
-        if (l == 0) s_ck[w] = ck;
         __syncthreads();
         if (w == 0) {
             const uint32_t v = (&s_cnt[0][0])[l];
@@ -220,15 +225,13 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
             const uint64_t e = pd::lookback_single<4>(status + q.status_off, tile, total);
             if (l == 0) {
                 s_excl = e;
-                uint64_t cks = 0;
-                for (int ww = 0; ww < SV_BLOCK / PD_WAVE; ww++) cks += s_ck[ww];
-                if (cks) atomicAdd((unsigned long long*)&res[q.qid].checksum, (unsigned long long)cks);
                 if (tile == ntl - 1) res[q.qid].count = e + total;
             }
         }
         __syncthreads();
         const uint64_t excl = s_excl;
         const uint64_t lt = pd::lanemask_lt();
+        uint64_t ck = 0;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             const bool m = (hit >> i) & 1;
@@ -237,13 +240,18 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
                 const uint64_t o = excl + s_cnt[i][w] + pd::popc64(bal & lt);
                 const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
                 if (o < q.rows) {
-                    if constexpr (JOIN)
+                    if constexpr (JOIN) {
                         reinterpret_cast<uint2*>(arena)[q.out_off + o] = make_uint2((uint32_t)r, dimrow[i]);
-                    else
+                        ck += pd_ck_join(r, dimrow[i]);
+                    } else {
                         reinterpret_cast<uint32_t*>(arena)[q.out_off + o] = (uint32_t)r;
+                        ck += pd_ck_select(r, o);
+                    }
                 }
             }
         }
+        ck = pd::wave_sum64(ck);
+        if (l == 0 && ck) atomicAdd((unsigned long long*)&res[q.qid].checksum, (unsigned long long)ck);
         __syncthreads();
     }
 }
@@ -251,7 +259,8 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
 // Persistent dense group-by over (query, 2^18-row chunk) items: Q1 measures
 // in registers for up to 8 flag codes, one partial row per item.
 __global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restrict__ b, SvTables T,
-                                                         uint64_t* __restrict__ partials, uint32_t* __restrict__ ctr) {
+                                                         uint8_t* __restrict__ heap, uint32_t* __restrict__ ctr) {
+    uint64_t* partials = reinterpret_cast<uint64_t*>(heap + b->partials_off);
     __shared__ uint64_t s_wi;
     __shared__ uint64_t s_red[SV_BLOCK / PD_WAVE][SV_G][8];
     const SvKind& K = b->gb;
@@ -330,9 +339,10 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restri
 
 // One block per group-by query: 128-bit reduction of its items' partials,
 // then count and checksum of the non-empty groups (keys flag1*256+flag2).
-__global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint64_t* __restrict__ partials,
+__global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint8_t* __restrict__ heap,
                                    const uint8_t* __restrict__ inv, uint32_t n1, uint32_t n2,
                                    SvResult* __restrict__ res) {
+    const uint64_t* partials = reinterpret_cast<const uint64_t*>(heap + b->partials_off);
     const SvKind& K = b->gb;
     if (blockIdx.x >= K.nq) return;
     const SvQuery& q = K.q[blockIdx.x];
@@ -403,7 +413,8 @@ __global__ void k_fk_index_build(SvTables T, uint32_t* __restrict__ fk_row, int3
             x = T.dim_index[h];
         }
         fk_row[r] = x.y != 0 ? x.y - 1 : 0xFFFFFFFFu;
-        fk_date[r] = x.y != 0 ? (int32_t)x.z : 0;
+        // no dimension row: the largest date, so no date predicate keeps the row
+        fk_date[r] = x.y != 0 ? (int32_t)x.z : INT32_MAX;
     }
 }
 
@@ -411,16 +422,16 @@ __global__ void k_fk_index_build(SvTables T, uint32_t* __restrict__ fk_row, int3
 __global__ void k_ck_rows(const uint32_t* __restrict__ rows, uint64_t n, uint64_t base, uint64_t* __restrict__ out) {
     uint64_t s = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        s += pd_ck_select(base + rows[i]);
+        s += pd_ck_select(base + rows[i], i);
     s = pd::wave_sum64(s);
     if (pd::lane() == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
 }
 
 __global__ void k_ck_pairs(const uint32_t* __restrict__ fact, const uint32_t* __restrict__ dim, uint64_t n,
-                           uint64_t* __restrict__ out) {
+                           uint64_t fact_base, uint64_t* __restrict__ out) {
     uint64_t s = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        s += pd_ck_join(fact[i], dim[i]);
+        s += pd_ck_join(fact_base + fact[i], dim[i]);
     s = pd::wave_sum64(s);
     if (pd::lane() == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
 }
@@ -434,6 +445,63 @@ __global__ void k_iota64(uint32_t* __restrict__ out, uint64_t n, uint64_t start)
 
 using namespace pd;
 
+// First-fit allocator over the serving heap's byte range, 256-byte granules,
+// neighbouring free ranges merged.  Host-side bookkeeping only: the memory
+// is one pool allocation made when the server is created.
+class SvHeap {
+public:
+    void init(uint64_t bytes) {
+        free_.clear();
+        size_ = bytes / 256 * 256;
+        if (size_) free_[0] = size_;
+        used_ = peak_ = 0;
+    }
+    static uint64_t round(uint64_t n) { return (std::max<uint64_t>(n, 1) + 255) / 256 * 256; }
+    uint64_t largest() const {
+        uint64_t m = 0;
+        for (const auto& kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+    bool alloc(uint64_t n, uint64_t* off) {
+        n = round(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - n;
+            const uint64_t at = it->first + n;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used_ += n;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = round(n);
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint64_t size() const { return size_; }
+    uint64_t peak() const { return peak_; }
+    void reset_peak() { peak_ = used_; }
+private:
+    std::map<uint64_t, uint64_t> free_;
+    uint64_t size_ = 0, used_ = 0, peak_ = 0;
+};
+
 struct pimdal_server {
     pimdal_ctx* ctx = nullptr;
     pimdal_tables tab{};
@@ -445,7 +513,8 @@ struct pimdal_server {
     uint8_t h_inv[512];
     uint32_t n1 = 0, n2 = 0;
     int nslots = 0;
-    uint64_t arena_bytes = 0, status_words = 0, partial_items = 0;
+    uint8_t* heap = nullptr;  // serving heap: every batch's outputs, statuses and partials
+    SvHeap heap_alloc;
     struct Slot {
         hipStream_t s = nullptr;
         hipEvent_t ev = nullptr;
@@ -453,12 +522,10 @@ struct pimdal_server {
         SvBatch* d_batch = nullptr;
         SvResult* h_res = nullptr;    // pinned
         SvResult* d_res = nullptr;
-        void* arena = nullptr;
-        uint64_t* status = nullptr;
-        uint64_t* partials = nullptr;
         uint32_t* ctr = nullptr;
         hipGraphExec_t graph = nullptr;
         std::vector<uint32_t> qids;
+        uint64_t block_off = 0, block_bytes = 0;   // the batch's heap block
         bool busy = false;
     };
     std::vector<Slot> slots;
@@ -473,16 +540,16 @@ void sv_capture(pimdal_server* S, pimdal_server::Slot& sl) {
     hipGraph_t g;
     PD_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
     PD_HIP(hipMemcpyAsync(sl.d_batch, sl.h_batch, sizeof(SvBatch), hipMemcpyHostToDevice, sl.s));
-    hipLaunchKernelGGL(k_sv_reset, dim3(pgrid), dim3(256), 0, sl.s, (const SvBatch*)sl.d_batch, sl.status, sl.d_res,
+    hipLaunchKernelGGL(k_sv_reset, dim3(pgrid), dim3(256), 0, sl.s, (const SvBatch*)sl.d_batch, S->heap, sl.d_res,
                        nres, sl.ctr);
     hipLaunchKernelGGL(k_sv_scan<false>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
-                       sl.arena, sl.status, sl.d_res, sl.ctr);
+                       S->heap, sl.d_res, sl.ctr);
     hipLaunchKernelGGL(k_sv_scan<true>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
-                       sl.arena, sl.status, sl.d_res, sl.ctr);
-    hipLaunchKernelGGL(k_sv_groupby, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
-                       sl.partials, sl.ctr);
+                       S->heap, sl.d_res, sl.ctr);
+    hipLaunchKernelGGL(k_sv_groupby, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T, S->heap,
+                       sl.ctr);
     hipLaunchKernelGGL(k_sv_groupby_final, dim3(SV_MAXQ), dim3(64), 0, sl.s, (const SvBatch*)sl.d_batch,
-                       (const uint64_t*)sl.partials, (const uint8_t*)S->lut + 512, S->n1, S->n2, sl.d_res);
+                       (const uint8_t*)S->heap, (const uint8_t*)S->lut + 512, S->n1, S->n2, sl.d_res);
     PD_HIP(hipMemcpyAsync(sl.h_res, sl.d_res, sizeof(SvResult) * nres, hipMemcpyDeviceToHost, sl.s));
     PD_HIP(hipStreamEndCapture(sl.s, &g));
     PD_HIP(hipGraphInstantiate(&sl.graph, g, nullptr, nullptr, 0));
@@ -493,8 +560,13 @@ double now_us(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Heap bytes one query takes while its batch runs: its output rows (a
+// select's row ids, a join's (fact row, dim row) pairs, at most one per
+// slice row), one look-back status word per 4096-row tile, and for a
+// group-by one partial row per 2^18-row chunk.
 struct Footprint {
-    uint64_t arena_bytes, status_words, items;
+    uint64_t out_bytes, status_words, items;
+    uint64_t bytes() const { return out_bytes + status_words * 8 + items * SV_G * 8 * 8; }
 };
 
 Footprint footprint(const pimdal_query& q) {
@@ -519,8 +591,24 @@ void run_plain(pimdal_server* S, const pimdal_query& q, pimdal_query_result& r) 
                       (const uint32_t*)rows.as<uint32_t>(), k, q.offset, cnt.as<uint64_t>() + 1);
         r.count = k;
         r.checksum = read_u64(c, cnt.as<uint64_t>() + 1, s);
+    } else if (q.kind == PIMDAL_Q_JOIN && S->fk_row) {
+        // the resident join index of the two tables, as the batched path
+        // reads it: one select over the slice of (dimension date, dimension
+        // row) keeps the rows whose dimension row exists and has date < param
+        // (a fact row with no dimension row carries the largest date)
+        Scratch dr(c, std::max<uint64_t>(q.rows, 1) * 4), fr(c, std::max<uint64_t>(q.rows, 1) * 4), cnt(c, 16);
+        fill_u64(c, cnt.get(), 0, 2, s);
+        select_range(c, PIMDAL_I32, (const int32_t*)S->fk_date + q.offset, (const uint32_t*)S->fk_row + q.offset,
+                     q.rows, INT64_MIN, q.param, dr.as<uint32_t>(), fr.as<uint32_t>(), cnt.as<uint64_t>(), s);
+        const uint64_t k = read_u64(c, cnt.get(), s);
+        if (k) launch(c, "sv_ck", k_ck_pairs, dim3((uint32_t)std::min<uint64_t>(ceil_div(k, 256), 1024)), dim3(256), 0, s,
+                      (const uint32_t*)fr.as<uint32_t>(), (const uint32_t*)dr.as<uint32_t>(), k, q.offset,
+                      cnt.as<uint64_t>() + 1);
+        r.count = k;
+        r.checksum = read_u64(c, cnt.as<uint64_t>() + 1, s);
     } else if (q.kind == PIMDAL_Q_JOIN) {
-        // qualifying dimension rows: (key, row) with date < param
+        // no resident join index: qualifying dimension rows (key, row) with
+        // date < param, then the hash join with the fact slice
         Scratch dk(c, t.n_dim * 4), dr(c, t.n_dim * 4), cnt(c, 16);
         fill_u64(c, cnt.get(), 0, 2, s);
         select_range(c, PIMDAL_I32, t.dim_date, reinterpret_cast<const uint32_t*>(t.dim_key), t.n_dim, INT64_MIN,
@@ -534,7 +622,8 @@ void run_plain(pimdal_server* S, const pimdal_query& q, pimdal_query_result& r) 
                                      reinterpret_cast<const uint32_t*>(t.orderkey + q.offset), frow.as<uint32_t>(),
                                      q.rows, op.as<uint32_t>(), ob.as<uint32_t>(), q.rows, s);
         if (k) launch(c, "sv_ck", k_ck_pairs, dim3((uint32_t)std::min<uint64_t>(ceil_div(k, 256), 1024)), dim3(256), 0, s,
-                      (const uint32_t*)op.as<uint32_t>(), (const uint32_t*)ob.as<uint32_t>(), k, cnt.as<uint64_t>() + 1);
+                      (const uint32_t*)op.as<uint32_t>(), (const uint32_t*)ob.as<uint32_t>(), k, (uint64_t)0,
+                      cnt.as<uint64_t>() + 1);
         r.count = k;
         r.checksum = read_u64(c, cnt.as<uint64_t>() + 1, s);
     } else {
@@ -564,7 +653,7 @@ void run_plain(pimdal_server* S, const pimdal_query& q, pimdal_query_result& r) 
 
 }  // namespace
 
-extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, int slots, uint64_t slot_arena_bytes,
+extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, int slots, uint64_t heap_bytes,
                                     pimdal_server** out) {
     PD_API_BEGIN
     PD_REQUIRE(c && tables && out, PIMDAL_E_INVALID, "NULL argument");
@@ -605,8 +694,8 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
                         t.dim_date, (const uint4*)S->dim_index, cap - 1, (const uint8_t*)S->lut, S->n2,
                         nullptr, nullptr};
         // foreign-key join index of the resident pair (fact.orderkey ->
-        // dim.key): batched joins then stream 8 bytes per fact row instead
-        // of one random index probe per row
+        // dim.key): joins then stream 8 bytes per fact row instead of one
+        // random index probe per row (both run modes read it)
         if (c->tune_get("serve_join_index", 1)) {
             S->fk_row = c->pool.alloc(t.n_fact * 4, s);
             S->fk_date = c->pool.alloc(t.n_fact * 4, s);
@@ -615,11 +704,16 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
             S->T.fk_row = (const uint32_t*)S->fk_row;
             S->T.fk_date = (const int32_t*)S->fk_date;
         }
+        // the serving heap: by default half of what the budget leaves after
+        // the tables' indexes (at most 8 GiB, at least 64 MiB)
+        if (!heap_bytes) {
+            const uint64_t room = c->pool.budget() > c->pool.reserved() ? c->pool.budget() - c->pool.reserved() : 0;
+            heap_bytes = std::max<uint64_t>(64ull << 20, std::min<uint64_t>(room / 2, 8ull << 30));
+        }
+        S->heap = (uint8_t*)c->pool.alloc(heap_bytes, s);
+        S->heap_alloc.init(heap_bytes);
         // slots
         S->nslots = slots;
-        S->arena_bytes = slot_arena_bytes ? slot_arena_bytes : (1ull << 30);
-        S->status_words = S->arena_bytes / 4 / SV_TILE + 4 * SV_MAXQ;
-        S->partial_items = S->arena_bytes / 4 / SV_GB_CHUNK + 2 * SV_MAXQ;
         S->slots.resize(slots);
         for (auto& sl : S->slots) {
             PD_HIP(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
@@ -629,9 +723,6 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
             memset(sl.h_batch, 0, sizeof(SvBatch));
             sl.d_batch = (SvBatch*)c->pool.alloc(sizeof(SvBatch), s);
             sl.d_res = (SvResult*)c->pool.alloc(sizeof(SvResult) * 3 * SV_MAXQ, s);
-            sl.arena = c->pool.alloc(S->arena_bytes, s);
-            sl.status = (uint64_t*)c->pool.alloc(S->status_words * 8, s);
-            sl.partials = (uint64_t*)c->pool.alloc(S->partial_items * SV_G * 8 * 8, s);
             sl.ctr = (uint32_t*)c->pool.alloc(16, s);
         }
         PD_HIP(hipStreamSynchronize(s));
@@ -654,10 +745,11 @@ extern "C" int pimdal_server_destroy(pimdal_server* S) {
         if (sl.ev) (void)hipEventDestroy(sl.ev);
         if (sl.h_batch) (void)hipHostFree(sl.h_batch);
         if (sl.h_res) (void)hipHostFree(sl.h_res);
-        for (void* p : {(void*)sl.d_batch, (void*)sl.d_res, sl.arena, (void*)sl.status, (void*)sl.partials, (void*)sl.ctr})
+        for (void* p : {(void*)sl.d_batch, (void*)sl.d_res, (void*)sl.ctr})
             if (p) c->pool.free(p, c->stream);
         if (sl.s) (void)hipStreamDestroy(sl.s);
     }
+    if (S->heap) c->pool.free(S->heap, c->stream);
     if (S->dim_index) c->pool.free(S->dim_index, c->stream);
     if (S->fk_row) c->pool.free(S->fk_row, c->stream);
     if (S->fk_date) c->pool.free(S->fk_date, c->stream);
@@ -665,6 +757,115 @@ extern "C" int pimdal_server_destroy(pimdal_server* S) {
     delete S;
     PD_API_END
 }
+
+namespace {
+
+// Builds one batch into slot `sl` from the tenants' queues and admits it:
+// queries are taken round-robin over the tenants, FIFO within a tenant, as
+// long as the batch's heap block (its queries' footprints, each region
+// 256-byte aligned) fits the largest free range of the heap and the
+// tenant stays within its share of the heap (heap / tenants with work,
+// unless it is the only one).  A tenant whose oldest query does not fit
+// stops taking queries for this batch; others go on.  Returns the queries
+// admitted (0: nothing fits now).
+struct TenantQ {
+    std::deque<uint64_t> q;
+    uint64_t inflight_bytes = 0;
+};
+
+uint64_t admit_batch(pimdal_server* S, pimdal_server::Slot& sl, const pimdal_query* qs,
+                     std::map<uint32_t, TenantQ>& tenants, uint32_t& rr, std::vector<uint64_t>& qbytes,
+                     pimdal_query_result* res, double now) {
+    SvBatch& B = *sl.h_batch;
+    B.sel.nq = B.join.nq = B.gb.nq = 0;
+    B.sel.items = B.join.items = B.gb.items = 0;
+    const uint64_t limit = S->heap_alloc.largest();
+    uint32_t active = 0;
+    for (auto& kv : tenants) active += (!kv.second.q.empty() || kv.second.inflight_bytes) ? 1 : 0;
+    const uint64_t share = active > 1 ? S->heap_alloc.size() / active : S->heap_alloc.size();
+    // sizes of the four regions as queries join the batch
+    uint64_t sel_bytes = 0, join_bytes = 0, status = 0, gb_items = 0;
+    auto block_bytes = [&](uint64_t sb, uint64_t jb, uint64_t st, uint64_t gi) {
+        return SvHeap::round(sb) + SvHeap::round(jb) + SvHeap::round(st * 8) + SvHeap::round(gi * SV_G * 64);
+    };
+    std::map<uint32_t, uint64_t> taken;   // bytes per tenant in this batch
+    std::vector<uint64_t> picked;
+    std::vector<uint32_t> ids;
+    for (auto& kv : tenants) ids.push_back(kv.first);
+    std::vector<bool> blocked(ids.size(), false);
+    const size_t nt = ids.size();
+    size_t cur = nt ? rr % nt : 0;
+    uint32_t stuck = 0;   // consecutive tenants that could not add a query
+    while (nt && stuck < nt && picked.size() < (size_t)(3 * SV_MAXQ)) {
+        TenantQ& T = tenants[ids[cur]];
+        bool added = false;
+        if (!blocked[cur] && !T.q.empty()) {
+            const uint64_t qi = T.q.front();
+            const pimdal_query& q = qs[qi];
+            const Footprint f = footprint(q);
+            SvKind& K = q.kind == PIMDAL_Q_SELECT ? B.sel : q.kind == PIMDAL_Q_JOIN ? B.join : B.gb;
+            const uint64_t nb = block_bytes(sel_bytes + (q.kind == PIMDAL_Q_SELECT ? f.out_bytes : 0),
+                                            join_bytes + (q.kind == PIMDAL_Q_JOIN ? f.out_bytes : 0),
+                                            status + f.status_words, gb_items + f.items);
+            const bool fits = K.nq < (uint32_t)SV_MAXQ && nb <= limit &&
+                              T.inflight_bytes + taken[ids[cur]] + f.bytes() <= std::max(share, f.bytes());
+            if (fits) {
+                T.q.pop_front();
+                SvQuery& d = K.q[K.nq++];
+                d.param = q.param;
+                d.offset = q.offset;
+                d.rows = q.rows;
+                d.qid = (uint32_t)picked.size();
+                d.pad = 0;
+                d.item0 = K.items;
+                if (q.kind == PIMDAL_Q_GROUPBY) {
+                    d.out_off = 0;
+                    d.status_off = 0;
+                    K.items += f.items;
+                    gb_items += f.items;
+                } else {
+                    d.status_off = status;
+                    K.items += f.status_words;
+                    status += f.status_words;
+                    if (q.kind == PIMDAL_Q_SELECT) {
+                        d.out_off = sel_bytes / 4;
+                        sel_bytes += f.out_bytes;
+                    } else {
+                        d.out_off = join_bytes / 8;   // rebased past the select region below
+                        join_bytes += f.out_bytes;
+                    }
+                }
+                taken[ids[cur]] += f.bytes();
+                qbytes[qi] = f.bytes();
+                picked.push_back(qi);
+                res[qi].start_us = now;
+                added = true;
+            } else {
+                blocked[cur] = true;   // FIFO within a tenant: its later queries wait too
+            }
+        }
+        stuck = added ? 0 : stuck + 1;
+        cur = (cur + 1) % nt;
+    }
+    rr = nt ? (uint32_t)cur : 0;
+    if (picked.empty()) return 0;
+    const uint64_t bytes = block_bytes(sel_bytes, join_bytes, status, gb_items);
+    uint64_t off = 0;
+    PD_REQUIRE(S->heap_alloc.alloc(bytes, &off), PIMDAL_E_INTERNAL, "serving heap block vanished");
+    const uint64_t join_base = SvHeap::round(sel_bytes);
+    for (uint32_t i = 0; i < B.join.nq; i++) B.join.q[i].out_off += join_base / 8;
+    B.arena_off = off;
+    B.status_off = off + join_base + SvHeap::round(join_bytes);
+    B.partials_off = B.status_off + SvHeap::round(status * 8);
+    B.status_words = status;
+    for (uint64_t qi : picked) tenants[qs[qi].tenant].inflight_bytes += qbytes[qi];
+    sl.qids.assign(picked.begin(), picked.end());
+    sl.block_off = off;
+    sl.block_bytes = bytes;
+    return picked.size();
+}
+
+}  // namespace
 
 extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint64_t nq, int mode,
                                  pimdal_query_result* res, pimdal_serve_stats* stats) {
@@ -678,14 +879,17 @@ extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint6
         PD_REQUIRE(q.rows >= 1 && q.offset + q.rows <= S->tab.n_fact, PIMDAL_E_INVALID, "query slice out of range");
         PD_REQUIRE(i == 0 || qs[i - 1].arrival_us <= q.arrival_us, PIMDAL_E_INVALID, "queries must be in arrival order");
         const Footprint f = footprint(q);
-        PD_REQUIRE(f.arena_bytes <= S->arena_bytes / 2 && f.status_words + SV_MAXQ <= S->status_words &&
-                       f.items + SV_MAXQ <= S->partial_items,
-                   PIMDAL_E_NOMEM, "query larger than a slot can hold (raise slot_arena_bytes)");
+        PD_REQUIRE(SvHeap::round(f.out_bytes) + SvHeap::round(f.status_words * 8) + SvHeap::round(f.items * SV_G * 64) <=
+                       S->heap_alloc.size(),
+                   PIMDAL_E_NOMEM, "query larger than the serving heap (raise heap_bytes)");
     }
     pimdal_ctx* c = S->ctx;
     PD_HIP(hipStreamSynchronize(c->stream));
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t batches = 0, max_inflight = 0;
+    std::map<uint32_t, TenantQ> tenants;
+    for (uint64_t i = 0; i < nq; i++) tenants[qs[i].tenant];
+    S->heap_alloc.reset_peak();
     if (mode == PIMDAL_SERVE_SEQUENTIAL) {
         for (uint64_t i = 0; i < nq; i++) {
             while (now_us(t0) < qs[i].arrival_us) std::this_thread::yield();
@@ -699,93 +903,56 @@ extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint6
     } else {
         const bool dense_gb = S->n1 * S->n2 <= (uint32_t)SV_G;
         uint64_t next = 0, done = 0, inflight = 0;
-        std::deque<uint64_t> ready;
+        uint32_t rr = 0;
+        std::vector<uint64_t> qbytes(nq, 0);
         while (done < nq) {
             bool progress = false;
             const double now = now_us(t0);
-            while (next < nq && qs[next].arrival_us <= now) ready.push_back(next++);
-            // retire finished batches
+            while (next < nq && qs[next].arrival_us <= now) {
+                if (qs[next].kind == PIMDAL_Q_GROUPBY && !dense_gb) {
+                    // more than 8 flag groups: served by the plain operator, inline
+                    const uint64_t qi = next;
+                    res[qi].start_us = now_us(t0);
+                    run_plain(S, qs[qi], res[qi]);
+                    res[qi].end_us = now_us(t0);
+                    res[qi].latency_us = res[qi].end_us - qs[qi].arrival_us;
+                    done++;
+                } else {
+                    tenants[qs[next].tenant].q.push_back(next);
+                }
+                next++;
+                progress = true;
+            }
+            // retire finished batches: results, heap block, tenant shares
             for (auto& sl : S->slots) {
                 if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
                 const double t_end = now_us(t0);
                 for (size_t j = 0; j < sl.qids.size(); j++) {
-                    pimdal_query_result& r = res[sl.qids[j]];
+                    const uint64_t qi = sl.qids[j];
+                    pimdal_query_result& r = res[qi];
                     r.count = sl.h_res[j].count;
                     r.checksum = sl.h_res[j].checksum;
                     r.end_us = t_end;
-                    r.latency_us = t_end - qs[sl.qids[j]].arrival_us;
+                    r.latency_us = t_end - qs[qi].arrival_us;
+                    tenants[qs[qi].tenant].inflight_bytes -= qbytes[qi];
                 }
+                S->heap_alloc.release(sl.block_off, sl.block_bytes);
                 done += sl.qids.size();
                 inflight -= sl.qids.size();
                 sl.qids.clear();
                 sl.busy = false;
                 progress = true;
             }
-            // admit arrived queries into free slots
+            // admit arrived queries into free slots, by footprint against the heap
             for (auto& sl : S->slots) {
-                if (sl.busy || ready.empty()) continue;
-                SvBatch& B = *sl.h_batch;
-                B.sel.nq = B.join.nq = B.gb.nq = 0;
-                B.sel.items = B.join.items = B.gb.items = 0;
-                uint64_t sel_bytes = 0, join_bytes = 0, status = 0, gb_items = 0;
-                const uint64_t half = S->arena_bytes / 2;  // select outputs | join outputs
-                while (!ready.empty() && sl.qids.size() < (size_t)(3 * SV_MAXQ)) {
-                    const uint64_t qi = ready.front();
-                    const pimdal_query& q = qs[qi];
-                    if (q.kind == PIMDAL_Q_GROUPBY && !dense_gb) {
-                        // more than 8 flag groups: served by the plain operator, inline
-                        ready.pop_front();
-                        res[qi].start_us = now_us(t0);
-                        run_plain(S, q, res[qi]);
-                        res[qi].end_us = now_us(t0);
-                        res[qi].latency_us = res[qi].end_us - q.arrival_us;
-                        done++;
-                        progress = true;
-                        continue;
-                    }
-                    const Footprint f = footprint(q);
-                    SvKind& K = q.kind == PIMDAL_Q_SELECT ? B.sel : q.kind == PIMDAL_Q_JOIN ? B.join : B.gb;
-                    const bool fits = K.nq < (uint32_t)SV_MAXQ && status + f.status_words <= S->status_words &&
-                                      gb_items + f.items <= S->partial_items &&
-                                      (q.kind != PIMDAL_Q_SELECT || sel_bytes + f.arena_bytes <= half) &&
-                                      (q.kind != PIMDAL_Q_JOIN || join_bytes + f.arena_bytes <= half);
-                    if (!fits) break;  // FIFO: the rest waits for the next free slot
-                    ready.pop_front();
-                    SvQuery& d = K.q[K.nq++];
-                    d.param = q.param;
-                    d.offset = q.offset;
-                    d.rows = q.rows;
-                    d.qid = (uint32_t)sl.qids.size();
-                    d.pad = 0;
-                    if (q.kind == PIMDAL_Q_GROUPBY) {
-                        d.item0 = K.items;
-                        d.out_off = 0;
-                        d.status_off = 0;
-                        K.items += f.items;
-                        gb_items += f.items;
-                    } else {
-                        d.item0 = K.items;
-                        d.status_off = status;
-                        K.items += f.status_words;
-                        status += f.status_words;
-                        if (q.kind == PIMDAL_Q_SELECT) {
-                            d.out_off = sel_bytes / 4;
-                            sel_bytes += f.arena_bytes;
-                        } else {
-                            d.out_off = (half + join_bytes) / 8;
-                            join_bytes += f.arena_bytes;
-                        }
-                    }
-                    sl.qids.push_back((uint32_t)qi);
-                    res[qi].start_us = now_us(t0);
-                }
-                if (sl.qids.empty()) continue;
-                B.status_words = status;
+                if (sl.busy) continue;
+                const uint64_t k = admit_batch(S, sl, qs, tenants, rr, qbytes, res, now_us(t0));
+                if (!k) break;   // nothing fits (or nothing waits): wait for a batch to retire
                 PD_HIP(hipGraphLaunch(sl.graph, sl.s));
                 PD_HIP(hipEventRecord(sl.ev, sl.s));
                 sl.busy = true;
                 batches++;
-                inflight += sl.qids.size();
+                inflight += k;
                 max_inflight = std::max(max_inflight, inflight);
                 progress = true;
             }
@@ -809,6 +976,9 @@ extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint6
         stats->mean_us = nq ? sum / nq : 0.0;
         stats->batches = batches;
         stats->max_inflight = max_inflight;
+        stats->heap_bytes = S->heap_alloc.size();
+        stats->heap_peak_bytes = S->heap_alloc.peak();
+        stats->tenants = tenants.size();
     }
     PD_API_END
 }
