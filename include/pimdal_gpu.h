@@ -440,7 +440,7 @@ typedef struct {
  * context's budget leaves, at most 8 GiB), charged to the budget.  A
  * batched query occupies its footprint in the heap while its batch runs:
  * 4 bytes per slice row (select) or 8 (join) for the outputs, 8 per
- * 4096-row tile for look-back statuses, 512 per 2^18-row chunk (group-by).
+ * 16384-row tile for look-back statuses, 512 per 2^16-row chunk (group-by).
  * Batches are admitted while their footprint fits the heap's largest free
  * range; a query larger than the whole heap is refused (PIMDAL_E_NOMEM).
  * Queries of each tenant are admitted in arrival order, tenants in turn;

@@ -44,7 +44,7 @@ namespace pd {
 constexpr int SV_MAXQ = 512;          // queries per kind per batch
 constexpr int SV_BLOCK = 256;
 constexpr int SV_TILE = 4096;         // rows per select / join work item
-constexpr uint32_t SV_GB_CHUNK = 1u << 18;  // rows per group-by work item
+constexpr uint32_t SV_GB_CHUNK = 1u << 16;  // rows per group-by work item
 constexpr int SV_G = 8;               // dense group codes in served group-bys
 
 struct SvQuery {
@@ -138,11 +138,16 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
     const SvKind& K = JOIN ? b->join : b->sel;
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     constexpr int RPT = SV_TILE / SV_BLOCK;  // 16 rows per thread: row = tile_base + i*256 + t
+    uint32_t ck_qid = 0xFFFFFFFFu;   // the wave's checksum, flushed when its query changes
+    uint64_t ck_acc = 0;
     while (true) {
         if (t == 0) s_wi = atomicAdd(&ctr[JOIN ? 1 : 0], 1u);
         __syncthreads();
         const uint64_t wi = s_wi;
-        if (wi >= K.items) break;
+        if (wi >= K.items) {
+            if (l == 0 && ck_acc) atomicAdd((unsigned long long*)&res[ck_qid].checksum, (unsigned long long)ck_acc);
+            break;
+        }
         const uint32_t qi = sv_find_query(K, wi);
         const SvQuery& q = K.q[qi];
         const uint64_t tile = wi - q.item0;
@@ -251,12 +256,225 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
             }
         }
         ck = pd::wave_sum64(ck);
-        if (l == 0 && ck) atomicAdd((unsigned long long*)&res[q.qid].checksum, (unsigned long long)ck);
+        if (q.qid != ck_qid) {
+            if (l == 0 && ck_acc) atomicAdd((unsigned long long*)&res[ck_qid].checksum, (unsigned long long)ck_acc);
+            ck_acc = 0;
+            ck_qid = q.qid;
+        }
+        ck_acc += ck;
         __syncthreads();
     }
 }
 
-// Persistent dense group-by over (query, 2^18-row chunk) items: Q1 measures
+// ---- streaming select and index join: 16384-row tiles, 1024-thread blocks ----
+// The persistent scan above takes 4096-row tiles in 256-thread blocks and
+// waits on each tile's look-back before its next loads; measured on a
+// 2000-query stream it read 0.64 TB/s.  Here each block takes a tile
+// (wave w: rows wbase .. wbase + 1023, lane l: rows g*256 + 4l .. +3),
+// stages its kept rows compacted in the wave's LDS slice, publishes the
+// tile's count, then loads its next tile while the look-back of the held
+// one resolves, and writes it out (the pipeline of the library's one-pass
+// select).  Tiles of a query start at its offset rounded down to 4 rows,
+// so every 16-byte load is aligned; rows outside the slice are masked.
+// Select: price < param, output the fact row.  Join (resident join index):
+// dimension row present and its date < param, output (fact row, dim row).
+constexpr int SV2_BLOCK = 1024;
+constexpr int SV2_WAVES = SV2_BLOCK / PD_WAVE;
+constexpr int SV2_WROWS = 1024;
+constexpr uint64_t SV2_TILE = (uint64_t)SV2_WAVES * SV2_WROWS;
+
+__host__ __device__ inline uint64_t sv2_tiles(uint64_t offset, uint64_t rows) {
+    return ((offset & 3ull) + rows + SV2_TILE - 1) / SV2_TILE;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sv2_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <bool JOIN>
+__global__ __launch_bounds__(SV2_BLOCK) void k_sv_stream(const SvBatch* __restrict__ b, SvTables T,
+                                                         uint8_t* __restrict__ heap, SvResult* __restrict__ res,
+                                                         uint32_t* __restrict__ ctr) {
+    const SvKind& K = JOIN ? b->join : b->sel;
+    uint64_t* status = reinterpret_cast<uint64_t*>(heap + b->status_off);
+    uint32_t* out32 = reinterpret_cast<uint32_t*>(heap + b->arena_off);
+    uint2* out64 = reinterpret_cast<uint2*>(heap + b->arena_off);
+    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_wave_tot[SV2_WAVES];
+    __shared__ uint64_t s_excl;
+    __shared__ uint16_t s_idx[SV2_WAVES][SV2_WROWS];
+    __shared__ uint32_t s_pay[JOIN ? SV2_WAVES : 1][JOIN ? SV2_WROWS : 1];
+    const uint32_t t = threadIdx.x, l = pd::lane();
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint32_t nitems = (uint32_t)K.items;
+
+    // wave geometry of work item wi: query, tile, first row (absolute) and
+    // the range [vlo, vhi) of the wave's rows inside the query's slice
+    struct Geo { uint32_t qi; uint32_t tile; uint64_t wbase; uint32_t vlo, vhi; };
+    auto geo = [&](uint32_t wi) -> Geo {
+        Geo g;
+        g.qi = sv_find_query(K, wi);
+        const SvQuery& q = K.q[g.qi];
+        g.tile = (uint32_t)(wi - q.item0);
+        g.wbase = (q.offset & ~3ull) + (uint64_t)g.tile * SV2_TILE + (uint64_t)w * SV2_WROWS;
+        const uint64_t end = q.offset + q.rows;
+        g.vlo = g.wbase >= q.offset ? 0u : (uint32_t)min<uint64_t>(SV2_WROWS, q.offset - g.wbase);
+        g.vhi = end <= g.wbase ? 0u : (uint32_t)min<uint64_t>(SV2_WROWS, end - g.wbase);
+        return g;
+    };
+    // per lane: rows g*256 + 4l + e of the wave (g, e < 4)
+    int64_t kp[4][4];     // select: price
+    uint32_t kr[4][4];    // join: dimension row
+    int32_t kd[4][4];     // join: dimension date
+    auto load = [&](const Geo& g) {
+        const uint32_t l4 = l * 4;
+        if constexpr (JOIN) {
+            const __amdgpu_buffer_rsrc_t rr = sv2_rsrc(T.fk_row + g.wbase, g.vhi * 4u);
+            const __amdgpu_buffer_rsrc_t rd = sv2_rsrc(T.fk_date + g.wbase, g.vhi * 4u);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t r0 = q * 256 + l4;
+                const uint4 a = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rr, r0 * 4, 0, 0));
+                const uint4 d = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rd, r0 * 4, 0, 0));
+                kr[q][0] = a.x; kr[q][1] = a.y; kr[q][2] = a.z; kr[q][3] = a.w;
+                kd[q][0] = (int32_t)d.x; kd[q][1] = (int32_t)d.y; kd[q][2] = (int32_t)d.z; kd[q][3] = (int32_t)d.w;
+            }
+        } else {
+            const __amdgpu_buffer_rsrc_t rp = sv2_rsrc(T.price + g.wbase, g.vhi * 8u);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t r0 = q * 256 + l4;
+                const uint4 a = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rp, r0 * 8, 0, 0));
+                const uint4 c = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rp, r0 * 8 + 16, 0, 0));
+                kp[q][0] = (int64_t)(((uint64_t)a.y << 32) | a.x); kp[q][1] = (int64_t)(((uint64_t)a.w << 32) | a.z);
+                kp[q][2] = (int64_t)(((uint64_t)c.y << 32) | c.x); kp[q][3] = (int64_t)(((uint64_t)c.w << 32) | c.z);
+            }
+        }
+    };
+    // predicate, wave ranks (packed 16-bit group counts, one 64-bit scan),
+    // kept rows staged compacted in the wave's slice; returns the count
+    auto stage = [&](const Geo& g) -> uint32_t {
+        const int64_t param = K.q[g.qi].param;
+        const uint32_t l4 = l * 4;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t r = q * 256 + l4 + e;
+                bool keep = r >= g.vlo && r < g.vhi;
+                if constexpr (JOIN) keep = keep && kr[q][e] != 0xFFFFFFFFu && (int64_t)kd[q][e] < param;
+                else keep = keep && kp[q][e] < param;
+                bits |= (uint32_t)keep << (q * 4 + e);
+            }
+        uint64_t packed = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) packed |= (uint64_t)__popc((bits >> (q * 4)) & 0xF) << (16 * q);
+        const uint64_t incl = pd::wave_incl_scan64(packed);
+        const uint64_t tot = __shfl(incl, PD_WAVE - 1, PD_WAVE);
+        uint32_t wave_total = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t gb = (bits >> (q * 4)) & 0xF;
+            uint32_t pos = wave_total + ((uint32_t)(incl >> (16 * q)) & 0xFFFF) - __popc(gb);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if (gb & (1u << e)) {
+                    s_idx[w][pos] = (uint16_t)(q * 256 + l4 + e);
+                    if constexpr (JOIN) s_pay[w][pos] = kr[q][e];
+                    pos++;
+                }
+            }
+            wave_total += (uint32_t)(tot >> (16 * q)) & 0xFFFF;
+        }
+        return wave_total;
+    };
+    auto block_totals = [&](uint32_t& woff, uint32_t& ttot) {
+        woff = 0;
+        ttot = 0;
+#pragma unroll
+        for (int i = 0; i < SV2_WAVES; i++) {
+            const uint32_t v = s_wave_tot[i];
+            woff += (i < (int)w) ? v : 0;
+            ttot += v;
+        }
+    };
+
+    // prologue: the first tile counted, staged and published
+    if (t == 0) s_item = atomicAdd(&ctr[JOIN ? 1 : 0], 1u);
+    __syncthreads();
+    uint32_t ia = __builtin_amdgcn_readfirstlane(s_item);
+    if (ia >= nitems) return;
+    Geo A = geo(ia);
+    load(A);
+    uint32_t wt = stage(A);
+    if (l == 0) s_wave_tot[w] = wt;
+    __syncthreads();
+    uint32_t woff, ttot;
+    block_totals(woff, ttot);
+    if (w == 0) pd::lb_publish_agg(status + K.q[A.qi].status_off, A.tile, ttot);
+    // the wave's checksum of its current query, added to the query's word
+    // only when the wave moves to another query (same-word device atomics
+    // serialize: one per wave per tile would queue thousands on a query)
+    uint32_t ck_qid = 0xFFFFFFFFu;
+    uint64_t ck_acc = 0;
+    while (true) {
+        __syncthreads();   // every wave has read s_item and s_wave_tot
+        if (t == 0) s_item = atomicAdd(&ctr[JOIN ? 1 : 0], 1u);
+        __syncthreads();
+        const uint32_t ib = __builtin_amdgcn_readfirstlane(s_item);
+        Geo B{};
+        if (ib < nitems) {
+            B = geo(ib);
+            load(B);
+        }
+        const SvQuery& qa = K.q[A.qi];
+        if (w == 0) {
+            const uint64_t excl = pd::lb_resolve<1>(status + qa.status_off, A.tile, ttot);
+            if (l == 0) {
+                s_excl = excl;
+                if ((uint64_t)A.tile + 1 == sv2_tiles(qa.offset, qa.rows)) res[qa.qid].count = excl + ttot;
+            }
+        }
+        __syncthreads();
+        {
+            const uint64_t obase = s_excl + woff;
+            uint64_t ck = 0;
+            for (uint32_t i = l; i < wt; i += PD_WAVE) {
+                const uint64_t o = obase + i;
+                const uint32_t row = (uint32_t)(A.wbase + s_idx[w][i]);
+                if constexpr (JOIN) {
+                    const uint32_t dr = s_pay[w][i];
+                    out64[qa.out_off + o] = make_uint2(row, dr);
+                    ck += pd_ck_join(row, dr);
+                } else {
+                    out32[qa.out_off + o] = row;
+                    ck += pd_ck_select(row, o);
+                }
+            }
+            ck = pd::wave_sum64(ck);
+            if (qa.qid != ck_qid) {
+                if (l == 0 && ck_acc) atomicAdd((unsigned long long*)&res[ck_qid].checksum, (unsigned long long)ck_acc);
+                ck_acc = 0;
+                ck_qid = qa.qid;
+            }
+            ck_acc += ck;
+        }
+        if (ib >= nitems) {
+            if (l == 0 && ck_acc) atomicAdd((unsigned long long*)&res[ck_qid].checksum, (unsigned long long)ck_acc);
+            break;
+        }
+        // the wave's own LDS slice is free again (its reads were issued first)
+        wt = stage(B);
+        if (l == 0) s_wave_tot[w] = wt;
+        __syncthreads();
+        block_totals(woff, ttot);
+        A = B;
+        if (w == 0) pd::lb_publish_agg(status + K.q[A.qi].status_off, A.tile, ttot);
+    }
+}
+
+// Persistent dense group-by over (query, 2^16-row chunk) items: Q1 measures
 // in registers for up to 8 flag codes, one partial row per item.
 __global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restrict__ b, SvTables T,
                                                          uint8_t* __restrict__ heap, uint32_t* __restrict__ ctr) {
@@ -337,11 +555,18 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_groupby(const SvBatch* __restri
     }
 }
 
-// One block per group-by query: 128-bit reduction of its items' partials,
-// then count and checksum of the non-empty groups (keys flag1*256+flag2).
-__global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint8_t* __restrict__ heap,
-                                   const uint8_t* __restrict__ inv, uint32_t n1, uint32_t n2,
-                                   SvResult* __restrict__ res) {
+// One 512-thread block per group-by query, wave g = group code g: the
+// lanes sum the query's item partials (lane l: items l, l + 64, ...), then
+// the wave.  Every field is a 64-bit wrapping sum: the int64 measures keep
+// their low 64 bits, the count and the two 32-bit halves of the charge's
+// low word stay below 2^58 (2^16 rows x 2^32 per item, at most 2^10 items
+// below 2^31 rows), and the charge's high word enters the 128-bit total
+// shifted by 64, so only its low 64 bits count.  Then count and checksum of
+// the non-empty groups (keys flag1*256+flag2).
+__global__ __launch_bounds__(SV_G * PD_WAVE) void k_sv_groupby_final(const SvBatch* __restrict__ b,
+                                                                     const uint8_t* __restrict__ heap,
+                                                                     const uint8_t* __restrict__ inv, uint32_t n1,
+                                                                     uint32_t n2, SvResult* __restrict__ res) {
     const uint64_t* partials = reinterpret_cast<const uint64_t*>(heap + b->partials_off);
     const SvKind& K = b->gb;
     if (blockIdx.x >= K.nq) return;
@@ -349,32 +574,32 @@ __global__ void k_sv_groupby_final(const SvBatch* __restrict__ b, const uint8_t*
     const uint64_t items = (q.rows + SV_GB_CHUNK - 1) / SV_GB_CHUNK;
     __shared__ uint64_t s_ck[SV_G];
     __shared__ uint32_t s_n[SV_G];
-    const uint32_t g = threadIdx.x;
-    if (g < SV_G) {
+    const uint32_t g = threadIdx.x / PD_WAVE, l = pd::lane();
+    uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (g < n1 * n2) {
+        for (uint64_t it = l; it < items; it += PD_WAVE) {
+            const uint64_t* r = partials + ((q.item0 + it) * SV_G + g) * 8;
+#pragma unroll
+            for (int f = 0; f < 8; f++) a[f] += r[f];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 8; f++) a[f] = pd::wave_sum64(a[f]);
+    if (l == 0) {
         s_ck[g] = 0;
         s_n[g] = 0;
-        if (g < n1 * n2) {
-            unsigned __int128 a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (uint64_t it = 0; it < items; it++) {
-                const uint64_t* r = partials + ((q.item0 + it) * SV_G + g) * 8;
-#pragma unroll
-                for (int f = 0; f < 8; f++) {
-                    if (f == 4 || f == 6 || f == 7) a[f] += r[f];
-                    else a[f] += (__int128)(int64_t)r[f];
-                }
-            }
-            if (a[6]) {
-                const unsigned __int128 ch = a[4] + (a[7] << 32) + (a[5] << 64);
-                const int64_t v[7] = {(int64_t)a[0], (int64_t)a[1], (int64_t)a[2], (int64_t)a[3],
-                                      (int64_t)(uint64_t)ch, (int64_t)(uint64_t)(ch >> 64), (int64_t)a[6]};
-                const int64_t key = (int64_t)inv[g / n2] * 256 + inv[256 + g % n2];
-                s_ck[g] = pd_ck_group(key, v);
-                s_n[g] = 1;
-            }
+        if (g < n1 * n2 && a[6]) {
+            const unsigned __int128 ch = (unsigned __int128)a[4] + ((unsigned __int128)a[7] << 32) +
+                                         ((unsigned __int128)a[5] << 64);
+            const int64_t v[7] = {(int64_t)a[0], (int64_t)a[1], (int64_t)a[2], (int64_t)a[3],
+                                  (int64_t)(uint64_t)ch, (int64_t)(uint64_t)(ch >> 64), (int64_t)a[6]};
+            const int64_t key = (int64_t)inv[g / n2] * 256 + inv[256 + g % n2];
+            s_ck[g] = pd_ck_group(key, v);
+            s_n[g] = 1;
         }
     }
     __syncthreads();
-    if (g == 0) {
+    if (threadIdx.x == 0) {
         uint64_t ck = 0, n = 0;
         for (int i = 0; i < SV_G; i++) { ck += s_ck[i]; n += s_n[i]; }
         res[q.qid].count = n;
@@ -515,6 +740,10 @@ struct pimdal_server {
     int nslots = 0;
     uint8_t* heap = nullptr;  // serving heap: every batch's outputs, statuses and partials
     SvHeap heap_alloc;
+    // streaming kernels (16384-row tiles) for selects and index joins;
+    // otherwise the 4096-row scan (unaligned columns, probe-index joins)
+    bool stream_sel = false, stream_join = false;
+    uint32_t stream_grid = 0;
     struct Slot {
         hipStream_t s = nullptr;
         hipEvent_t ev = nullptr;
@@ -542,13 +771,23 @@ void sv_capture(pimdal_server* S, pimdal_server::Slot& sl) {
     PD_HIP(hipMemcpyAsync(sl.d_batch, sl.h_batch, sizeof(SvBatch), hipMemcpyHostToDevice, sl.s));
     hipLaunchKernelGGL(k_sv_reset, dim3(pgrid), dim3(256), 0, sl.s, (const SvBatch*)sl.d_batch, S->heap, sl.d_res,
                        nres, sl.ctr);
-    hipLaunchKernelGGL(k_sv_scan<false>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
-                       S->heap, sl.d_res, sl.ctr);
-    hipLaunchKernelGGL(k_sv_scan<true>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
-                       S->heap, sl.d_res, sl.ctr);
-    hipLaunchKernelGGL(k_sv_groupby, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T, S->heap,
+    if (S->stream_sel)
+        hipLaunchKernelGGL(k_sv_stream<false>, dim3(S->stream_grid), dim3(SV2_BLOCK), 0, sl.s,
+                           (const SvBatch*)sl.d_batch, S->T, S->heap, sl.d_res, sl.ctr);
+    else
+        hipLaunchKernelGGL(k_sv_scan<false>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
+                           S->heap, sl.d_res, sl.ctr);
+    if (S->stream_join)
+        hipLaunchKernelGGL(k_sv_stream<true>, dim3(S->stream_grid), dim3(SV2_BLOCK), 0, sl.s,
+                           (const SvBatch*)sl.d_batch, S->T, S->heap, sl.d_res, sl.ctr);
+    else
+        hipLaunchKernelGGL(k_sv_scan<true>, dim3(pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T,
+                           S->heap, sl.d_res, sl.ctr);
+    // the group-by's loads are 16-byte vectors with few in flight per thread:
+    // eight blocks per CU keep enough of them in flight
+    hipLaunchKernelGGL(k_sv_groupby, dim3(4 * pgrid), dim3(SV_BLOCK), 0, sl.s, (const SvBatch*)sl.d_batch, S->T, S->heap,
                        sl.ctr);
-    hipLaunchKernelGGL(k_sv_groupby_final, dim3(SV_MAXQ), dim3(64), 0, sl.s, (const SvBatch*)sl.d_batch,
+    hipLaunchKernelGGL(k_sv_groupby_final, dim3(SV_MAXQ), dim3(SV_G * PD_WAVE), 0, sl.s, (const SvBatch*)sl.d_batch,
                        (const uint8_t*)S->heap, (const uint8_t*)S->lut + 512, S->n1, S->n2, sl.d_res);
     PD_HIP(hipMemcpyAsync(sl.h_res, sl.d_res, sizeof(SvResult) * nres, hipMemcpyDeviceToHost, sl.s));
     PD_HIP(hipStreamEndCapture(sl.s, &g));
@@ -563,16 +802,16 @@ double now_us(std::chrono::steady_clock::time_point t0) {
 // Heap bytes one query takes while its batch runs: its output rows (a
 // select's row ids, a join's (fact row, dim row) pairs, at most one per
 // slice row), one look-back status word per 4096-row tile, and for a
-// group-by one partial row per 2^18-row chunk.
+// group-by one partial row per 2^16-row chunk.
 struct Footprint {
     uint64_t out_bytes, status_words, items;
     uint64_t bytes() const { return out_bytes + status_words * 8 + items * SV_G * 8 * 8; }
 };
 
-Footprint footprint(const pimdal_query& q) {
+Footprint footprint(const pimdal_server* S, const pimdal_query& q) {
     const uint64_t tiles = (q.rows + SV_TILE - 1) / SV_TILE;
-    if (q.kind == PIMDAL_Q_SELECT) return {q.rows * 4, tiles, 0};
-    if (q.kind == PIMDAL_Q_JOIN) return {q.rows * 8, tiles, 0};
+    if (q.kind == PIMDAL_Q_SELECT) return {q.rows * 4, S->stream_sel ? sv2_tiles(q.offset, q.rows) : tiles, 0};
+    if (q.kind == PIMDAL_Q_JOIN) return {q.rows * 8, S->stream_join ? sv2_tiles(q.offset, q.rows) : tiles, 0};
     return {0, 0, (q.rows + SV_GB_CHUNK - 1) / SV_GB_CHUNK};
 }
 
@@ -712,6 +951,10 @@ extern "C" int pimdal_server_create(pimdal_ctx* c, const pimdal_tables* tables, 
         }
         S->heap = (uint8_t*)c->pool.alloc(heap_bytes, s);
         S->heap_alloc.init(heap_bytes);
+        // the streaming kernels load 16 bytes at a time from 4-row-aligned rows
+        S->stream_sel = ((uintptr_t)t.price % 16) == 0;
+        S->stream_join = S->fk_row != nullptr;   // pool allocations: 256-byte aligned
+        S->stream_grid = blocks_per_cu(k_sv_stream<false>, SV2_BLOCK) * (uint32_t)c->cu_count;
         // slots
         S->nslots = slots;
         S->slots.resize(slots);
@@ -779,7 +1022,10 @@ uint64_t admit_batch(pimdal_server* S, pimdal_server::Slot& sl, const pimdal_que
     SvBatch& B = *sl.h_batch;
     B.sel.nq = B.join.nq = B.gb.nq = 0;
     B.sel.items = B.join.items = B.gb.items = 0;
+    // a batch takes at most its slot's share of the heap (unless one query
+    // needs more), so every slot can hold a batch and batches overlap
     const uint64_t limit = S->heap_alloc.largest();
+    const uint64_t batch_cap = S->heap_alloc.size() / (uint64_t)S->nslots;
     uint32_t active = 0;
     for (auto& kv : tenants) active += (!kv.second.q.empty() || kv.second.inflight_bytes) ? 1 : 0;
     const uint64_t share = active > 1 ? S->heap_alloc.size() / active : S->heap_alloc.size();
@@ -802,12 +1048,12 @@ uint64_t admit_batch(pimdal_server* S, pimdal_server::Slot& sl, const pimdal_que
         if (!blocked[cur] && !T.q.empty()) {
             const uint64_t qi = T.q.front();
             const pimdal_query& q = qs[qi];
-            const Footprint f = footprint(q);
+            const Footprint f = footprint(S, q);
             SvKind& K = q.kind == PIMDAL_Q_SELECT ? B.sel : q.kind == PIMDAL_Q_JOIN ? B.join : B.gb;
             const uint64_t nb = block_bytes(sel_bytes + (q.kind == PIMDAL_Q_SELECT ? f.out_bytes : 0),
                                             join_bytes + (q.kind == PIMDAL_Q_JOIN ? f.out_bytes : 0),
                                             status + f.status_words, gb_items + f.items);
-            const bool fits = K.nq < (uint32_t)SV_MAXQ && nb <= limit &&
+            const bool fits = K.nq < (uint32_t)SV_MAXQ && nb <= limit && (picked.empty() || nb <= batch_cap) &&
                               T.inflight_bytes + taken[ids[cur]] + f.bytes() <= std::max(share, f.bytes());
             if (fits) {
                 T.q.pop_front();
@@ -878,7 +1124,7 @@ extern "C" int pimdal_server_run(pimdal_server* S, const pimdal_query* qs, uint6
         PD_REQUIRE(q.kind >= PIMDAL_Q_SELECT && q.kind <= PIMDAL_Q_GROUPBY, PIMDAL_E_INVALID, "unknown query kind");
         PD_REQUIRE(q.rows >= 1 && q.offset + q.rows <= S->tab.n_fact, PIMDAL_E_INVALID, "query slice out of range");
         PD_REQUIRE(i == 0 || qs[i - 1].arrival_us <= q.arrival_us, PIMDAL_E_INVALID, "queries must be in arrival order");
-        const Footprint f = footprint(q);
+        const Footprint f = footprint(S, q);
         PD_REQUIRE(SvHeap::round(f.out_bytes) + SvHeap::round(f.status_words * 8) + SvHeap::round(f.items * SV_G * 64) <=
                        S->heap_alloc.size(),
                    PIMDAL_E_NOMEM, "query larger than the serving heap (raise heap_bytes)");
