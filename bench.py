@@ -9,7 +9,7 @@ operators' rows/s at their configured sizes, the dominant kernel's roofline
 a bounded sample on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--only sort|zipf|sortselect|join|bloom|groupby|ingest|serve]
+                  [--only sort|zipf|sortselect|join|bloom|groupby|q3|ingest|serve]
 """
 from __future__ import annotations
 
@@ -123,7 +123,8 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
-           "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64")
+           "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
+           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order")
 
 
 def kernel_breakdown(ctx, steps):
@@ -212,6 +213,35 @@ def bench_q1_plan(ctx, steps, warmup, by_key, n=60_000_000):
            "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK, "graph_launches_per_step": 1,
            "autotuned": tuned}
     for a in list(t.values()) + [a for k, a in out.items() if k != "_res"] + [ng, st]:
+        a.free()
+    return res
+
+
+def bench_q3(ctx, steps, warmup, n_lineitem=60_000_000):
+    """TPC-H Q3 shape at SF10's sizes (6e7 lineitems, 1.5e7 orders, 1.5e6
+    customers; host-generated synthetic tables copied to HBM before timing):
+    customer set, orders table, lineitem probe with the revenue summed in the
+    table, all groups in TPC-H order.  Algorithmic bytes: each input column
+    read once, the two order columns of the filter twice (count, then build)."""
+    import numpy as np
+    from pimdal_amd import operators as ops, datagen
+    t = datagen.q3_tables(n_lineitem)
+    d = {k: ctx.to_device(v) for k, v in t.items()}
+    date = int(datagen.pack_days(np.array([9204]))[0])
+    seg = datagen.Q3_SEGMENTS.index("BUILDING")
+    nc, no, nl = t["c_custkey"].size, t["o_orderkey"].size, n_lineitem
+    out = {}
+
+    def step():
+        out["r"] = ops.tpch_q3(ctx, d, seg, date, capacity=10, to_host=False)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    groups = out["r"][1]
+    alg = 5 * nc + 20 * no + 24 * nl
+    res = {"lineitem": nl, "orders": no, "customer": nc, "groups": groups, "s_per_step": tm,
+           "rows_per_s": (nc + no + nl) / tm, "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
+           "kernels": kernel_breakdown(ctx, steps)}
+    for a in list(d.values()) + list(out["r"][0].values()):
         a.free()
     return res
 
@@ -405,6 +435,9 @@ def main():
                                                                                          by_key)
         log("bench: TPC-H Q6 shape 6e7")
         ops_out["q6"] = bench_q6(ctx, max(3, args.steps // 2), 1)
+    if args.only in ("", "all", "q3"):
+        log("bench: TPC-H Q3 shape, 6e7 lineitems")
+        ops_out["q3_sf10"] = bench_q3(ctx, max(3, args.steps // 2), 1)
     if args.only in ("", "all", "ingest"):
         log("bench: date32 to packed dates 6e7")
         ops_out["date32_pack_6e7"] = bench_date32(ctx, args.steps, args.warmup)

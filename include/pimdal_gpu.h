@@ -348,6 +348,43 @@ int pimdal_q6_dev_async(pimdal_ctx* ctx, const int32_t* date, const int64_t* qua
                         const int64_t* discount, uint64_t n, int32_t date_lo, int32_t date_hi, int64_t disc_lo,
                         int64_t disc_hi, int64_t qty_lt, int64_t* d_revenue, uint64_t* d_rows);
 
+/* TPC-H Q3 shape: pimdal/main/tpc_h/query3/host/host_q3.cpp:471-519 with
+ * its kernels kernel_q3_1.c:116-137 (customers of a market segment),
+ * kernel_q3_2.c:146-190 (orders with o_orderdate < date), kernel_q3_3.c
+ * (join on custkey), kernel_q3_4.c:120-141 (lineitems with l_shipdate >
+ * date), kernel_q3_5.c (join on orderkey, revenue = l_extendedprice *
+ * (100 - l_discount) summed per (orderkey, orderdate, shippriority), top
+ * rows by revenue).  Dates are the packed dates of pimdal_date_pack; the
+ * segment is a dictionary code (the index of e.g. "BUILDING" in the
+ * c_mktsegment dictionary; the reference matches 16-byte strings).
+ * o_orderkey must be unique (PIMDAL_E_INVALID otherwise).  Writes the
+ * first min(groups, capacity) groups in TPC-H order (revenue descending,
+ * o_orderdate, then orderkey ascending); *n_groups = all groups.
+ * Revenue sums wrap modulo 2^64 like the reference's int64.  Synchronous. */
+typedef struct {
+    const int32_t* c_custkey;
+    const uint8_t* c_mktsegment;
+    uint64_t n_customer;
+    const int32_t* o_orderkey;
+    const int32_t* o_custkey;
+    const int32_t* o_orderdate;
+    const int32_t* o_shippriority;
+    uint64_t n_orders;
+    const int32_t* l_orderkey;
+    const int64_t* l_extendedprice;
+    const int64_t* l_discount;
+    const int32_t* l_shipdate;
+    uint64_t n_lineitem;
+} pimdal_q3_tables;
+typedef struct {
+    int32_t* orderkey;
+    int64_t* revenue;
+    int32_t* orderdate;
+    int32_t* shippriority;
+} pimdal_q3_result;
+int pimdal_tpch_q3_dev(pimdal_ctx* ctx, const pimdal_q3_tables* tables, uint32_t segment, int32_t date,
+                       const pimdal_q3_result* out, uint64_t capacity, uint64_t* n_groups);
+
 /* ---------------- captured plans ----------------
  * A plan records a sequence of asynchronous operator calls on the context's
  * stream as one hipGraph and replays it: the query plans of

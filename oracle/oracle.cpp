@@ -692,6 +692,63 @@ extern "C" uint64_t orc_bloom_probe(const uint32_t* keys, const uint32_t* pay, u
 
 extern "C" uint32_t orc_pim_hash(int which, uint32_t key) { return which == 0 ? pim_hash0(key) : pim_bloom_hash(which - 1, key); }
 
+/* ------------------------------------------------------------------ */
+/* TPC-H Q3 shape.                                                     */
+/* ------------------------------------------------------------------ */
+
+/* pimdal/main/tpc_h/query3/dpu/kernel_q3_1.c:116-137 (customers of one
+ * segment), kernel_q3_2.c:146-190 (orders with o_orderdate < date),
+ * kernel_q3_3.c:160-215 (join on custkey), kernel_q3_4.c:120-141 (lineitems
+ * with l_shipdate > date), kernel_q3_5.c:66-120, 170-262 (join on orderkey,
+ * revenue = extendedprice * (100 - discount) summed per (orderkey,
+ * orderdate, shippriority), sorted by revenue) and host_q3.cpp:300-345 (the
+ * top rows).  The segment is a dictionary code (the reference compares 16-
+ * byte strings); order: revenue descending, then o_orderdate, then orderkey
+ * ascending (the reference's quicksort leaves ties in no fixed order).
+ * Orderkeys must be unique (o_orderkey is the orders key); returns -1 when
+ * one repeats.  Writes at most cap groups; returns the group count. */
+extern "C" int64_t orc_q3(const int32_t* c_key, const uint8_t* c_seg, uint64_t nc, const int32_t* o_key,
+                          const int32_t* o_cust, const int32_t* o_date, const int32_t* o_prio, uint64_t no,
+                          const int32_t* l_key, const int64_t* l_price, const int64_t* l_disc, const int32_t* l_ship,
+                          uint64_t nl, uint32_t segment, int32_t date, uint64_t cap, int32_t* out_key,
+                          int64_t* out_rev, int32_t* out_date, int32_t* out_prio) {
+    std::unordered_map<int32_t, char> cust;
+    cust.reserve(nc / 4 + 16);
+    for (uint64_t i = 0; i < nc; i++)
+        if (c_seg[i] == segment) cust.emplace(c_key[i], 1);
+    std::unordered_map<int32_t, uint64_t> order_row;   // qualifying orderkey -> order row
+    for (uint64_t i = 0; i < no; i++) {
+        if (!(o_date[i] < date) || !cust.count(o_cust[i])) continue;
+        if (!order_row.emplace(o_key[i], i).second) return -1;
+    }
+    std::unordered_map<uint64_t, int64_t> rev;   // order row -> revenue (int64, wrapping like the reference)
+    std::unordered_map<uint64_t, char> seen;
+    for (uint64_t i = 0; i < nl; i++) {
+        if (!(l_ship[i] > date)) continue;
+        auto it = order_row.find(l_key[i]);
+        if (it == order_row.end()) continue;
+        const uint64_t r = it->second;
+        rev[r] = (int64_t)((uint64_t)rev[r] + (uint64_t)(l_price[i] * (100 - l_disc[i])));
+        seen[r] = 1;
+    }
+    std::vector<uint64_t> rows;
+    rows.reserve(seen.size());
+    for (const auto& kv : seen) rows.push_back(kv.first);
+    std::sort(rows.begin(), rows.end(), [&](uint64_t a, uint64_t b) {
+        if (rev[a] != rev[b]) return rev[a] > rev[b];
+        if (o_date[a] != o_date[b]) return o_date[a] < o_date[b];
+        return (uint32_t)o_key[a] < (uint32_t)o_key[b];
+    });
+    for (uint64_t j = 0; j < rows.size() && j < cap; j++) {
+        const uint64_t r = rows[j];
+        out_key[j] = o_key[r];
+        out_rev[j] = rev[r];
+        out_date[j] = o_date[r];
+        out_prio[j] = o_prio[r];
+    }
+    return (int64_t)rows.size();
+}
+
 extern "C" int orc_num_threads(void) { return omp_get_max_threads(); }
 extern "C" void orc_set_threads(int t) { omp_set_num_threads(t); }
 

@@ -46,6 +46,7 @@ _SIG = {
     "orc_bloom_build": (None, [vp, u64, u32, u32, vp]),
     "orc_bloom_probe": (u64, [vp, vp, u64, u32, u32, vp, vp, vp]),
     "orc_pim_hash": (u32, [C.c_int, u32]),
+    "orc_q3": (i64, [vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, u32, i32, u64, vp, vp, vp, vp]),
     "orc_num_threads": (C.c_int, []),
     "orc_set_threads": (None, [C.c_int]),
 }
@@ -358,3 +359,26 @@ def fact_table(n, seed=SEEDS["groupby"], groups=None):
         "flag1": gen_range(np.uint8, seed, COL["FLAG1"], n, 0, 2),
         "flag2": gen_range(np.uint8, seed, COL["FLAG2"], n, 0, 1),
     }
+
+
+Q3_COLS = {"customer": ("c_custkey", "c_mktsegment"),
+           "orders": ("o_orderkey", "o_custkey", "o_orderdate", "o_shippriority"),
+           "lineitem": ("l_orderkey", "l_extendedprice", "l_discount", "l_shipdate")}
+
+
+def q3(t, segment, date, cap=None):
+    """TPC-H Q3 groups in TPC-H order: dict of key, revenue, orderdate,
+    shippriority (numpy arrays) and the group count."""
+    c = {k: np.ascontiguousarray(t[k]) for cols in Q3_COLS.values() for k in cols}
+    nc, no, nl = c["c_custkey"].size, c["o_orderkey"].size, c["l_orderkey"].size
+    cap = no if cap is None else cap
+    o = {"key": np.empty(max(cap, 1), np.int32), "revenue": np.empty(max(cap, 1), np.int64),
+         "orderdate": np.empty(max(cap, 1), np.int32), "shippriority": np.empty(max(cap, 1), np.int32)}
+    g = lib().orc_q3(_p(c["c_custkey"]), _p(c["c_mktsegment"]), nc, _p(c["o_orderkey"]), _p(c["o_custkey"]),
+                     _p(c["o_orderdate"]), _p(c["o_shippriority"]), no, _p(c["l_orderkey"]), _p(c["l_extendedprice"]),
+                     _p(c["l_discount"]), _p(c["l_shipdate"]), nl, segment, date, cap, _p(o["key"]), _p(o["revenue"]),
+                     _p(o["orderdate"]), _p(o["shippriority"]))
+    if g < 0:
+        raise ValueError("o_orderkey values repeat")
+    k = min(g, cap)
+    return {f: v[:k] for f, v in o.items()}, g

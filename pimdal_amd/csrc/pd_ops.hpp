@@ -50,4 +50,9 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
 void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n, uint8_t* lut_dev, uint32_t meta[2],
                 uint8_t* inv_host, hipStream_t s);
 
+// TPC-H Q3 shape (q3.hip): returns the group count, the first min(count,
+// cap) groups written in TPC-H order.
+uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int32_t date, const pimdal_q3_result& out,
+                 uint64_t cap, hipStream_t s);
+
 }  // namespace pd

@@ -1,0 +1,351 @@
+// q3.hip — the TPC-H Q3 shape: two joins chained on the device with the
+// group-by fused into the second join's probe.
+//
+// Reference semantics: pimdal/main/tpc_h/query3/host/host_q3.cpp:471-519
+// (five DPU programs launched in turn, the host collecting and
+// redistributing rows between them): kernel_q3_1.c:116-137 selects the
+// customers of one market segment; kernel_q3_2.c:146-190 the orders with
+// o_orderdate < DATE; kernel_q3_3.c:160-215 hash-joins them on custkey;
+// kernel_q3_4.c:120-141 selects the lineitems with l_shipdate > DATE;
+// kernel_q3_5.c:66-120, 170-262 hash-joins those with the orders on
+// orderkey, computes revenue = l_extendedprice * (100 - l_discount), sums it
+// per (orderkey, orderdate, shippriority) and sorts for the top 10
+// (host_q3.cpp:300-345 merges the devices' top rows).
+//
+// Plan on gfx950 (nothing leaves the device between the steps):
+//  1. k_q3_cust    the qualifying customers' keys into a hash set in HBM
+//                  (a few hundred thousand keys: it stays in the caches);
+//  2. k_q3_osel / k_q3_orders   orders with o_orderdate < date whose
+//                  customer is in the set: listed (one pass over the orders),
+//                  then inserted into an open-addressing table (orderkey,
+//                  order row) sized for exactly that count;
+//  3. k_q3_lineitem   one streaming pass over the lineitems: shipdate >
+//                  date, probe the orders table, and add the revenue into
+//                  the order's slot (the join and the group-by in one pass:
+//                  the group key (orderkey, orderdate, shippriority) is the
+//                  order row, unique per orderkey);
+//  4. k_q3_emit    the slots that received a lineitem, compacted, with the
+//                  order's date and ship priority gathered;
+//  5. the TPC-H order (revenue descending, o_orderdate, then orderkey
+//     ascending) by four stable radix sorts of a row permutation, least
+//     significant key first; the first `capacity` groups are written.
+#include <algorithm>
+
+#include "pd_runtime.hpp"
+#include "pd_device.hpp"
+#include "pd_ops.hpp"
+
+namespace pd {
+
+__device__ __forceinline__ uint64_t q3_slot(uint32_t key, uint64_t mask) { return pd_mix64_dev(key) & mask; }
+
+// Customer set: slot word = 1 << 32 | custkey (0 = empty).
+__global__ void k_q3_cust(const int32_t* __restrict__ custkey, const uint8_t* __restrict__ segment, uint64_t n,
+                          uint32_t code, uint64_t* __restrict__ set, uint64_t mask) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (segment[i] != code) continue;
+        const uint32_t k = (uint32_t)custkey[i];
+        const uint64_t want = (1ull << 32) | k;
+        uint64_t h = q3_slot(k, mask);
+        while (true) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&set[h], 0ull, (unsigned long long)want);
+            if (prev == 0 || prev == want) break;
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+// Qualifying orders (o_orderdate < date, customer in the set) as a list of
+// order rows.  Block b takes the Q3_CHUNK rows from b * Q3_CHUNK, each
+// thread Q3_CHUNK / 256 of them (consecutive groups of 4, all probes of a
+// group in flight together); one counter atomic per block claims the
+// block's output range (same-word atomics serialize, so not one per wave).
+constexpr uint32_t Q3_CHUNK = 8192;
+
+__global__ __launch_bounds__(256) void k_q3_osel(const int32_t* __restrict__ odate, const int32_t* __restrict__ ocust,
+                                                 uint64_t n, int32_t date, const uint64_t* __restrict__ set,
+                                                 uint64_t mask, unsigned long long* __restrict__ ctr,
+                                                 uint32_t* __restrict__ rows) {
+    constexpr int PER = Q3_CHUNK / 256;   // 32 rows per thread
+    __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_base;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint64_t r0 = (uint64_t)blockIdx.x * Q3_CHUNK;
+    uint32_t bits = 0;   // bit j: row r0 + j * 256 + t qualifies
+#pragma unroll
+    for (int j0 = 0; j0 < PER; j0 += 4) {
+        bool cand[4];
+        uint32_t key[4];
+        uint64_t h[4], e[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t i = r0 + (uint64_t)(j0 + q) * 256 + t;
+            cand[q] = i < n && odate[i] < date;
+            key[q] = cand[q] ? (uint32_t)ocust[i] : 0u;
+            h[q] = q3_slot(key[q], mask);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) e[q] = cand[q] ? set[h[q]] : 0ull;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!cand[q]) continue;
+            const uint64_t want = (1ull << 32) | key[q];
+            while (e[q] != 0 && e[q] != want) {
+                h[q] = (h[q] + 1) & mask;
+                e[q] = set[h[q]];
+            }
+            if (e[q] == want) bits |= 1u << (j0 + q);
+        }
+    }
+    const uint32_t cnt = __popc(bits);
+    const uint32_t inc = pd::wave_incl_scan(cnt);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = inc - cnt, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        before += (i < (int)w) ? s_w[i] : 0u;
+        tot += s_w[i];
+    }
+    if (t == 0) s_base = tot ? atomicAdd(ctr, (unsigned long long)tot) : 0ull;
+    __syncthreads();
+    uint64_t o = s_base + before;
+    while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1;
+        rows[o++] = (uint32_t)(r0 + (uint64_t)j * 256 + t);
+    }
+}
+
+// Orders table from the qualifying rows: slot word = (order row + 1) << 32
+// | orderkey (0 = empty); a second orderkey equal to one already in the
+// table sets *dup.
+__global__ void k_q3_orders(const uint32_t* __restrict__ rows, uint64_t nq, const int32_t* __restrict__ okey,
+                            uint64_t* __restrict__ otab, uint64_t omask, uint32_t* __restrict__ dup) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nq; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = rows[j];
+        const uint32_t k = (uint32_t)okey[i];
+        const uint64_t want = ((i + 1) << 32) | k;
+        uint64_t h = q3_slot(k, omask);
+        while (true) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&otab[h], 0ull, (unsigned long long)want);
+            if (prev == 0) break;
+            if ((uint32_t)prev == k) { atomicOr(dup, 1u); break; }
+            h = (h + 1) & omask;
+        }
+    }
+}
+
+// One pass over the lineitems: revenue of every row with shipdate > date
+// whose order is in the table, added to the order's slot; the slot's hit
+// bit marks a group.  Four rows per thread, all probes in flight together.
+__global__ __launch_bounds__(256) void k_q3_lineitem(const int32_t* __restrict__ lkey,
+                                                     const int32_t* __restrict__ lship,
+                                                     const int64_t* __restrict__ lprice,
+                                                     const int64_t* __restrict__ ldisc, uint64_t n, int32_t date,
+                                                     const uint64_t* __restrict__ otab, uint64_t omask,
+                                                     unsigned long long* __restrict__ rev, uint32_t* __restrict__ hit) {
+    constexpr int R = 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; base < n; base += stride * R) {
+        uint32_t key[R];
+        bool want[R];
+        uint64_t h[R], e[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t i = base + (uint64_t)r * stride;
+            want[r] = i < n && lship[i] > date;
+            key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            h[r] = q3_slot(key[r], omask);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) e[r] = want[r] ? otab[h[r]] : 0ull;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!want[r]) continue;
+            while (e[r] != 0 && (uint32_t)e[r] != key[r]) {
+                h[r] = (h[r] + 1) & omask;
+                e[r] = otab[h[r]];
+            }
+            if (e[r] == 0) continue;
+            const uint64_t i = base + (uint64_t)r * stride;
+            const int64_t v = lprice[i] * (100 - ldisc[i]);
+            atomicAdd(&rev[h[r]], (unsigned long long)v);
+            const uint32_t bit = 1u << (h[r] & 31);
+            if (!(hit[h[r] >> 5] & bit)) atomicOr(&hit[h[r] >> 5], bit);
+        }
+    }
+}
+
+// Groups: the hit slots, compacted, with the order's key, date and ship
+// priority.  Block b takes the 8192 slots of hit words b * 256 .. + 255,
+// thread t one word; a block scan of the words' popcounts and one counter
+// atomic per block give every group its output position.
+__global__ __launch_bounds__(256) void k_q3_emit(const uint64_t* __restrict__ otab, const int64_t* __restrict__ rev,
+                                                 const uint32_t* __restrict__ hit, uint64_t words,
+                                                 const int32_t* __restrict__ odate,
+                                                 const int32_t* __restrict__ oprio,
+                                                 unsigned long long* __restrict__ ctr, uint32_t* __restrict__ g_key,
+                                                 int64_t* __restrict__ g_rev, int32_t* __restrict__ g_date,
+                                                 int32_t* __restrict__ g_prio) {
+    __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_base;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint64_t wd = (uint64_t)blockIdx.x * 256 + t;
+    uint32_t bits = wd < words ? hit[wd] : 0u;
+    const uint32_t cnt = __popc(bits);
+    const uint32_t inc = pd::wave_incl_scan(cnt);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = inc - cnt, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        before += (i < (int)w) ? s_w[i] : 0u;
+        tot += s_w[i];
+    }
+    if (t == 0) s_base = tot ? atomicAdd(ctr, (unsigned long long)tot) : 0ull;
+    __syncthreads();
+    uint64_t o = s_base + before;
+    while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1;
+        const uint64_t s = wd * 32 + (uint64_t)j;
+        const uint64_t e = otab[s];
+        const uint64_t row = (e >> 32) - 1;
+        g_key[o] = (uint32_t)e;
+        g_rev[o] = rev[s];
+        g_date[o] = odate[row];
+        g_prio[o] = oprio[row];
+        o++;
+    }
+}
+
+// Sort keys of the TPC-H order, gathered through the current permutation:
+// 0 orderkey, 1 orderdate (sign bit flipped), 2 / 3 low / high word of
+// ~(revenue with its sign bit flipped) (ascending = revenue descending).
+__global__ void k_q3_sortkey(uint32_t which, const uint32_t* __restrict__ perm, uint64_t n,
+                             const uint32_t* __restrict__ g_key, const int64_t* __restrict__ g_rev,
+                             const int32_t* __restrict__ g_date, uint32_t* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = perm[i];
+        uint32_t k;
+        if (which == 0) k = g_key[p];
+        else if (which == 1) k = (uint32_t)g_date[p] ^ 0x80000000u;
+        else {
+            const uint64_t r = ~((uint64_t)g_rev[p] ^ (1ull << 63));
+            k = which == 2 ? (uint32_t)r : (uint32_t)(r >> 32);
+        }
+        out[i] = k;
+    }
+}
+
+__global__ void k_q3_out(const uint32_t* __restrict__ perm, uint64_t n, const uint32_t* __restrict__ g_key,
+                         const int64_t* __restrict__ g_rev, const int32_t* __restrict__ g_date,
+                         const int32_t* __restrict__ g_prio, int32_t* __restrict__ o_key, int64_t* __restrict__ o_rev,
+                         int32_t* __restrict__ o_date, int32_t* __restrict__ o_prio) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = perm[i];
+        o_key[i] = (int32_t)g_key[p];
+        o_rev[i] = g_rev[p];
+        o_date[i] = g_date[p];
+        o_prio[i] = g_prio[p];
+    }
+}
+
+__global__ void k_q3_iota(uint32_t* __restrict__ out, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)i;
+}
+
+static uint64_t q3_pow2_slots(uint64_t n) {
+    uint64_t s = 1024;
+    while (s < 2 * n) s <<= 1;
+    return s;
+}
+
+static dim3 q3_grid(pimdal_ctx* c, uint64_t n) {
+    return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), (uint64_t)c->cu_count * 16)));
+}
+
+uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int32_t date, const pimdal_q3_result& out,
+                 uint64_t cap, hipStream_t s) {
+    PD_REQUIRE(t.c_custkey && t.c_mktsegment && t.o_orderkey && t.o_custkey && t.o_orderdate && t.o_shippriority &&
+                   t.l_orderkey && t.l_extendedprice && t.l_discount && t.l_shipdate,
+               PIMDAL_E_INVALID, "missing input column");
+    PD_REQUIRE(!cap || (out.orderkey && out.revenue && out.orderdate && out.shippriority), PIMDAL_E_INVALID,
+               "missing output column");
+    PD_REQUIRE(t.n_orders < (1ull << 30) && t.n_customer < (1ull << 31) && t.n_lineitem < (1ull << 40),
+               PIMDAL_E_INVALID, "tables too large (at most 2^30-1 orders)");
+    // 1. customer set
+    const uint64_t cslots = q3_pow2_slots(t.n_customer);
+    Scratch cset(c, cslots * 8, s), small(c, 64, s);
+    fill_u64(c, cset.get(), 0, cslots, s);
+    fill_u64(c, small.get(), 0, 8, s);
+    unsigned long long* cnt = small.as<unsigned long long>();   // [0] qualifying orders, [1] groups
+    uint32_t* dup = reinterpret_cast<uint32_t*>(small.as<uint64_t>() + 2);
+    launch(c, "q3_cust", k_q3_cust, q3_grid(c, t.n_customer), dim3(256), 0, s, t.c_custkey, t.c_mktsegment,
+           t.n_customer, segment, cset.as<uint64_t>(), cslots - 1);
+    // 2. qualifying orders: their rows listed, then a table sized for them
+    Scratch orows(c, std::max<uint64_t>(t.n_orders, 1) * 4, s);
+    launch(c, "q3_osel", k_q3_osel, dim3((uint32_t)std::max<uint64_t>(1, ceil_div(t.n_orders, Q3_CHUNK))), dim3(256), 0,
+           s, t.o_orderdate, t.o_custkey, t.n_orders, date, (const uint64_t*)cset.as<uint64_t>(), cslots - 1, cnt,
+           orows.as<uint32_t>());
+    uint64_t nq = 0;
+    read_small(c, &nq, cnt, 8, s);
+    const uint64_t oslots = q3_pow2_slots(nq);
+    Scratch otab(c, oslots * 8, s), rev(c, oslots * 8, s), hit(c, oslots / 8, s);
+    fill_u64(c, otab.get(), 0, oslots, s);
+    fill_u64(c, rev.get(), 0, oslots, s);
+    fill_u32(c, hit.get(), 0, oslots / 32, s);
+    launch(c, "q3_orders", k_q3_orders, q3_grid(c, nq), dim3(256), 0, s, (const uint32_t*)orows.as<uint32_t>(), nq,
+           t.o_orderkey, otab.as<uint64_t>(), oslots - 1, dup);
+    // 3. lineitems: probe and aggregate
+    launch(c, "q3_lineitem", k_q3_lineitem, q3_grid(c, ceil_div(t.n_lineitem, 4)), dim3(256), 0, s, t.l_orderkey,
+           t.l_shipdate, t.l_extendedprice, t.l_discount, t.n_lineitem, date, (const uint64_t*)otab.as<uint64_t>(),
+           oslots - 1, rev.as<unsigned long long>(), hit.as<uint32_t>());
+    // 4. groups
+    Scratch gk(c, std::max<uint64_t>(nq, 1) * 4, s), gr(c, std::max<uint64_t>(nq, 1) * 8, s),
+        gd(c, std::max<uint64_t>(nq, 1) * 4, s), gp(c, std::max<uint64_t>(nq, 1) * 4, s);
+    launch(c, "q3_emit", k_q3_emit, dim3((uint32_t)ceil_div(oslots / 32, 256)), dim3(256), 0, s,
+           (const uint64_t*)otab.as<uint64_t>(), (const int64_t*)rev.as<int64_t>(), (const uint32_t*)hit.as<uint32_t>(),
+           oslots / 32, t.o_orderdate,
+           t.o_shippriority, cnt + 1, gk.as<uint32_t>(), gr.as<int64_t>(), gd.as<int32_t>(), gp.as<int32_t>());
+    uint64_t hv[3] = {0, 0, 0};
+    read_small(c, hv, cnt, 24, s);
+    PD_REQUIRE((uint32_t)hv[2] == 0, PIMDAL_E_INVALID, "o_orderkey values must be unique");
+    const uint64_t g = hv[1];
+    if (g == 0 || cap == 0) return g;
+    // 5. TPC-H order: stable sorts of a permutation, least significant key first
+    Scratch pa(c, g * 4, s), pb(c, g * 4, s), ka(c, g * 4, s), kb(c, g * 4, s);
+    launch(c, "q3_order", k_q3_iota, q3_grid(c, g), dim3(256), 0, s, pa.as<uint32_t>(), g);
+    uint32_t* perm = pa.as<uint32_t>();
+    uint32_t* perm2 = pb.as<uint32_t>();
+    for (uint32_t which = 0; which < 4; which++) {
+        launch(c, "q3_order", k_q3_sortkey, q3_grid(c, g), dim3(256), 0, s, which, (const uint32_t*)perm, g,
+               (const uint32_t*)gk.as<uint32_t>(), (const int64_t*)gr.as<int64_t>(), (const int32_t*)gd.as<int32_t>(),
+               ka.as<uint32_t>());
+        // bits equal in every key are detected and skipped (dates and the
+        // revenue's high word vary in a few low bits only)
+        sort_pairs(c, ka.as<uint32_t>(), perm, 4, g, kb.as<uint32_t>(), perm2, -1, -1, s);
+        std::swap(perm, perm2);
+    }
+    const uint64_t k = std::min(g, cap);
+    launch(c, "q3_order", k_q3_out, q3_grid(c, k), dim3(256), 0, s, (const uint32_t*)perm, k,
+           (const uint32_t*)gk.as<uint32_t>(), (const int64_t*)gr.as<int64_t>(), (const int32_t*)gd.as<int32_t>(),
+           (const int32_t*)gp.as<int32_t>(), out.orderkey, out.revenue, out.orderdate, out.shippriority);
+    PD_HIP(hipStreamSynchronize(s));
+    return g;
+}
+
+}  // namespace pd
+
+using namespace pd;
+
+extern "C" int pimdal_tpch_q3_dev(pimdal_ctx* c, const pimdal_q3_tables* tables, uint32_t segment, int32_t date,
+                                  const pimdal_q3_result* out, uint64_t capacity, uint64_t* n_groups) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && tables && out && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
+    *n_groups = tpch_q3(c, *tables, segment, date, *out, capacity, c->stream);
+    PD_API_END
+}

@@ -693,8 +693,10 @@ static void run_passes(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_
     const bool xp_shape = !early && !split && nt == 0 &&
                           ((block == 512 && items == 16) || (block == 256 && items == 16 && !ballot));
     // cleared if the runtime refuses the co-resident grid; never while a plan
-    // is captured (cooperative launches are not recorded into plans)
-    bool xp = xp_shape && xs_knob > 0 && !c->capture;
+    // is captured (cooperative launches are not recorded into plans), nor
+    // for inputs of fewer tiles than the grid has blocks (a small sort
+    // would pay for launching the whole resident grid every pass)
+    bool xp = xp_shape && xs_knob > 0 && !c->capture && ntiles >= (uint64_t)c->cu_count * 2;
     auto xs_of = [&](uint32_t) -> uint32_t { return xp ? (uint32_t)xs_knob : 0u; };
     // status words: XCD-persistent passes scan their digit histogram
     // themselves and clear the next pass's statuses (two buffers in turn);

@@ -112,3 +112,48 @@ def dim_table(ctx: Context, n_dim: int, seed: int = SEEDS["groupby"]):
     call("pimdal_gen_dim_keys", ctx.handle, seed, n_dim, k.vp)
     return {"key": k, "date": rng_range(ctx, np.int32, seed, 20, n_dim, 0, 2556),
             "category": rng_range(ctx, np.uint8, seed, 21, n_dim, 0, 24)}
+
+
+# ---------------- TPC-H Q3 shape (host-generated, seeded) ----------------
+
+Q3_SEGMENTS = ("AUTOMOBILE", "BUILDING", "FURNITURE", "HOUSEHOLD", "MACHINERY")   # dictionary order
+
+
+def pack_days(days: np.ndarray) -> np.ndarray:
+    """Days since 1970-01-01 to the packed date day + (month << 5) +
+    ((year - 1900) << 9) (pimdal/main/tpc_h/reader/read_table.cpp:8-16)."""
+    d = np.datetime64("1970-01-01", "D") + np.asarray(days, dtype=np.int64).astype("timedelta64[D]")
+    y = d.astype("datetime64[Y]").astype(np.int64) + 1970
+    m = d.astype("datetime64[M]").astype(np.int64) % 12 + 1
+    day = (d - d.astype("datetime64[M]")).astype(np.int64) + 1
+    return (day + (m << 5) + ((y - 1900) << 9)).astype(np.int32)
+
+
+def q3_tables(n_lineitem: int, seed: int = 0x5EED0006) -> dict:
+    """Synthetic customer / orders / lineitem columns in TPC-H's shape: 40
+    lineitems per customer and 4 per order on average (SF10 at 6e7
+    lineitems); sparse unique orderkeys; order dates uniform over
+    1992-01-01 .. 1998-08-02; each lineitem ships 1..121 days after its order
+    (lineitems grouped by order); prices in cents, discount 0..10 percent;
+    segment codes index Q3_SEGMENTS.  Dates packed."""
+    rng = np.random.default_rng(seed)
+    nl = int(n_lineitem)
+    no = max(1, nl // 4)
+    nc = max(1, nl // 40)
+    start = 8035                      # 1992-01-01
+    span = 2405 - 151 + 1             # order dates up to 1998-08-02
+    c_key = (rng.permutation(nc) + 1).astype(np.int32)
+    c_seg = rng.integers(0, len(Q3_SEGMENTS), nc).astype(np.uint8)
+    o_key = (np.arange(no, dtype=np.int64) * 4 + rng.integers(0, 4, no)).astype(np.int32)
+    o_key = o_key[rng.permutation(no)]
+    o_cust = c_key[rng.integers(0, nc, no)]
+    o_days = start + rng.integers(0, span, no)
+    o_prio = rng.integers(0, 2, no).astype(np.int32)
+    l_orow = np.sort(rng.integers(0, no, nl))
+    l_days = o_days[l_orow] + rng.integers(1, 122, nl)
+    return {
+        "c_custkey": c_key, "c_mktsegment": c_seg,
+        "o_orderkey": o_key, "o_custkey": o_cust, "o_orderdate": pack_days(o_days), "o_shippriority": o_prio,
+        "l_orderkey": o_key[l_orow], "l_extendedprice": rng.integers(90000, 10500001, nl).astype(np.int64),
+        "l_discount": rng.integers(0, 11, nl).astype(np.int64), "l_shipdate": pack_days(l_days),
+    }

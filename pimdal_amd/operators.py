@@ -306,3 +306,36 @@ class Plan:
                 self.handle = None
         except Exception:
             pass
+
+
+class _Q3Tables(C.Structure):
+    _fields_ = [("c_custkey", C.c_void_p), ("c_mktsegment", C.c_void_p), ("n_customer", C.c_uint64),
+                ("o_orderkey", C.c_void_p), ("o_custkey", C.c_void_p), ("o_orderdate", C.c_void_p),
+                ("o_shippriority", C.c_void_p), ("n_orders", C.c_uint64), ("l_orderkey", C.c_void_p),
+                ("l_extendedprice", C.c_void_p), ("l_discount", C.c_void_p), ("l_shipdate", C.c_void_p),
+                ("n_lineitem", C.c_uint64)]
+
+
+class _Q3Result(C.Structure):
+    _fields_ = [("orderkey", C.c_void_p), ("revenue", C.c_void_p), ("orderdate", C.c_void_p),
+                ("shippriority", C.c_void_p)]
+
+
+def tpch_q3(ctx: Context, t: dict, segment: int, date: int, capacity: int = 10, to_host: bool = True):
+    """TPC-H Q3 shape over device columns `t` (customer, orders, lineitem):
+    the first `capacity` groups in TPC-H order and the group count.
+    pimdal/main/tpc_h/query3/host/host_q3.cpp:471-519."""
+    tab = _Q3Tables(t["c_custkey"].ptr, t["c_mktsegment"].ptr, t["c_custkey"].n, t["o_orderkey"].ptr,
+                    t["o_custkey"].ptr, t["o_orderdate"].ptr, t["o_shippriority"].ptr, t["o_orderkey"].n,
+                    t["l_orderkey"].ptr, t["l_extendedprice"].ptr, t["l_discount"].ptr, t["l_shipdate"].ptr,
+                    t["l_orderkey"].n)
+    cap = max(1, int(capacity))
+    out = {"key": ctx.empty(cap, np.int32), "revenue": ctx.empty(cap, np.int64),
+           "orderdate": ctx.empty(cap, np.int32), "shippriority": ctx.empty(cap, np.int32)}
+    res = _Q3Result(out["key"].ptr, out["revenue"].ptr, out["orderdate"].ptr, out["shippriority"].ptr)
+    g = C.c_uint64()
+    call("pimdal_tpch_q3_dev", ctx.handle, C.byref(tab), int(segment), int(date), C.byref(res), cap, C.byref(g))
+    if not to_host:
+        return out, g.value
+    k = min(g.value, cap)
+    return {f: a.numpy(k) for f, a in out.items()}, g.value

@@ -1,0 +1,82 @@
+"""TPC-H Q3 shape on the GPU (customer set -> orders table -> lineitem probe
+with the group-by in the table -> TPC-H order) against the CPU restatement:
+every group, in order, bit-exact."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from pimdal_amd import _lib, datagen, operators as ops
+
+pytestmark = pytest.mark.gpu
+
+DATE = int(datagen.pack_days(np.array([9204]))[0])   # 1995-03-15
+BUILDING = datagen.Q3_SEGMENTS.index("BUILDING")
+
+
+def _dev(ctx, t):
+    return {k: ctx.to_device(v) for k, v in t.items()}
+
+
+def _check(got, exp):
+    for f in ("key", "revenue", "orderdate", "shippriority"):
+        assert np.array_equal(got[f], exp[f]), f
+
+
+@pytest.mark.parametrize("n", [40, 1000, 1 << 20])
+def test_q3_all_groups_match_oracle(ctx, orc, n):
+    t = datagen.q3_tables(n, seed=11 + n)
+    exp, g_exp = orc.q3(t, BUILDING, DATE)
+    got, g = ops.tpch_q3(ctx, _dev(ctx, t), BUILDING, DATE, capacity=max(g_exp, 1))
+    assert g == g_exp
+    _check(got, exp)
+
+
+@pytest.mark.parametrize("segment,day", [(0, 8035 + 600), (4, 8035 + 2000), (2, 8035)])
+def test_q3_other_segments_and_dates(ctx, orc, segment, day):
+    t = datagen.q3_tables(300_000, seed=21)
+    date = int(datagen.pack_days(np.array([day]))[0])
+    exp, g_exp = orc.q3(t, segment, date)
+    got, g = ops.tpch_q3(ctx, _dev(ctx, t), segment, date, capacity=max(g_exp, 1))
+    assert g == g_exp
+    _check(got, exp)
+
+
+def test_q3_top10_at_6e7_lineitems(ctx, orc):
+    """SF10's shape: 6e7 lineitems, 1.5e7 orders, 1.5e6 customers; every
+    group compared in order, and the top 10 as the reference reports them."""
+    t = datagen.q3_tables(60_000_000, seed=31)
+    exp, g_exp = orc.q3(t, BUILDING, DATE)
+    d = _dev(ctx, t)
+    got, g = ops.tpch_q3(ctx, d, BUILDING, DATE, capacity=g_exp)
+    assert g == g_exp and g > 10_000
+    _check(got, exp)
+    top, g10 = ops.tpch_q3(ctx, d, BUILDING, DATE, capacity=10)
+    assert g10 == g_exp
+    _check(top, {f: v[:10] for f, v in exp.items()})
+
+
+def test_q3_edge_cases(ctx, orc):
+    """Negative prices (revenue below zero sorts last), keys at the int32
+    extremes, lineitems whose order is missing or filtered, a segment with no
+    customers."""
+    t = datagen.q3_tables(50_000, seed=41)
+    t["l_extendedprice"][::7] *= -1
+    t["o_orderkey"][0] = np.int32(-(1 << 31))
+    t["o_orderkey"][1] = np.int32((1 << 31) - 1)
+    t["l_orderkey"][:50] = t["o_orderkey"][0]
+    t["l_orderkey"][50:100] = t["o_orderkey"][1]
+    t["l_orderkey"][100:150] = 77_777_777          # no such order
+    for seg in (BUILDING, 9):
+        exp, g_exp = orc.q3(t, seg, DATE)
+        got, g = ops.tpch_q3(ctx, _dev(ctx, t), seg, DATE, capacity=max(g_exp, 1))
+        assert g == g_exp
+        _check(got, exp)
+
+
+def test_q3_repeated_orderkey_is_refused(ctx):
+    t = datagen.q3_tables(20_000, seed=51)
+    t["o_orderkey"][:] = t["o_orderkey"][0]
+    t["o_orderdate"][:] = t["o_orderdate"].min()
+    with pytest.raises(_lib.PimdalError, match="unique"):
+        ops.tpch_q3(ctx, _dev(ctx, t), BUILDING, DATE)

@@ -341,3 +341,94 @@ def test_bloom_partitions_are_isolated(orc):
     # probes that fall in another partition see an empty filter: nothing kept
     other = keys[:4000][part != 7]
     assert orc.bloom_probe(other, None, part_bits, shift, filt.ravel())[1].size == 0
+
+
+# ---------------- TPC-H Q3 shape ----------------
+
+def _q3_brute(t, segment, date):
+    """The SQL of the Q3 shape by nested loops over the rows (tiny inputs):
+    customers of the segment, their orders before `date`, those orders'
+    lineitems shipped after `date`; revenue summed per (orderkey,
+    orderdate, shippriority), ordered by revenue desc, orderdate, orderkey."""
+    groups = {}
+    for ci in range(len(t["c_custkey"])):
+        if int(t["c_mktsegment"][ci]) != segment:
+            continue
+        for oi in range(len(t["o_orderkey"])):
+            if int(t["o_custkey"][oi]) != int(t["c_custkey"][ci]) or not int(t["o_orderdate"][oi]) < date:
+                continue
+            for li in range(len(t["l_orderkey"])):
+                if int(t["l_orderkey"][li]) != int(t["o_orderkey"][oi]) or not int(t["l_shipdate"][li]) > date:
+                    continue
+                g = (int(t["o_orderkey"][oi]), int(t["o_orderdate"][oi]), int(t["o_shippriority"][oi]))
+                groups[g] = groups.get(g, 0) + int(t["l_extendedprice"][li]) * (100 - int(t["l_discount"][li]))
+    rows = sorted(groups.items(), key=lambda kv: (-kv[1], kv[0][1], kv[0][0] & 0xFFFFFFFF))
+    return rows
+
+
+def _q3_tiny(seed):
+    rng = np.random.default_rng(seed)
+    nc, no, nl = 12, 30, 90
+    t = {"c_custkey": rng.permutation(nc).astype(np.int32) * 3 - 5, "c_mktsegment": rng.integers(0, 3, nc).astype(np.uint8)}
+    t["o_orderkey"] = (rng.permutation(no) * 7 - 40).astype(np.int32)
+    t["o_custkey"] = t["c_custkey"][rng.integers(0, nc, no)]
+    t["o_custkey"][:3] = 9999                                   # no such customer
+    t["o_orderdate"] = rng.integers(100, 200, no).astype(np.int32)
+    t["o_shippriority"] = rng.integers(0, 3, no).astype(np.int32)
+    t["l_orderkey"] = t["o_orderkey"][rng.integers(0, no, nl)]
+    t["l_orderkey"][:4] = 12345                                  # no such order
+    t["l_extendedprice"] = rng.integers(-50, 1000, nl).astype(np.int64)
+    t["l_discount"] = rng.integers(0, 11, nl).astype(np.int64)
+    t["l_shipdate"] = rng.integers(120, 260, nl).astype(np.int32)
+    return t
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("segment,date", [(0, 150), (1, 170), (2, 130), (7, 150)])
+def test_q3_oracle_matches_brute_force(orc, seed, segment, date):
+    t = _q3_tiny(seed)
+    exp = _q3_brute(t, segment, date)
+    got, g = orc.q3(t, segment, date)
+    assert g == len(exp)
+    assert [(int(k), int(d), int(p)) for k, d, p in zip(got["key"], got["orderdate"], got["shippriority"])] == \
+        [e[0] for e in exp]
+    assert [int(r) for r in got["revenue"]] == [e[1] for e in exp]
+
+
+def test_q3_oracle_matches_pandas(orc):
+    """Medium tables: the oracle against pandas' merges and group-by (a
+    library's hash joins, as the reference checks its result with Acero,
+    pimdal/main/tpc_h/query3/host/host_q3.cpp:350-420)."""
+    import pandas as pd
+    from pimdal_amd import datagen
+    t = datagen.q3_tables(200_000, seed=5)
+    date = int(datagen.pack_days(np.array([9204]))[0])          # 1995-03-15
+    c = pd.DataFrame({"c_custkey": t["c_custkey"], "c_mktsegment": t["c_mktsegment"]})
+    o = pd.DataFrame({k: t[k] for k in ("o_orderkey", "o_custkey", "o_orderdate", "o_shippriority")})
+    li = pd.DataFrame({k: t[k] for k in ("l_orderkey", "l_extendedprice", "l_discount", "l_shipdate")})
+    m = c[c.c_mktsegment == 1].merge(o[o.o_orderdate < date], left_on="c_custkey", right_on="o_custkey")
+    m = m.merge(li[li.l_shipdate > date], left_on="o_orderkey", right_on="l_orderkey")
+    m["rev"] = m.l_extendedprice * (100 - m.l_discount)
+    g = m.groupby(["o_orderkey", "o_orderdate", "o_shippriority"], as_index=False)["rev"].sum()
+    g = g.sort_values(["rev", "o_orderdate", "o_orderkey"], ascending=[False, True, True])
+    got, n = orc.q3(t, 1, date)
+    assert n == len(g) and n > 100
+    assert np.array_equal(got["key"], g.o_orderkey.to_numpy()) and np.array_equal(got["revenue"], g.rev.to_numpy())
+    assert np.array_equal(got["orderdate"], g.o_orderdate.to_numpy())
+    assert np.array_equal(got["shippriority"], g.o_shippriority.to_numpy())
+
+
+def test_q3_oracle_rejects_repeated_orderkeys(orc):
+    t = _q3_tiny(9)
+    t["o_orderkey"][5] = t["o_orderkey"][6]
+    t["o_orderdate"][5] = t["o_orderdate"][6] = 100
+    cust = t["c_custkey"][t["c_mktsegment"] == 0][0]
+    t["o_custkey"][5] = t["o_custkey"][6] = cust
+    with pytest.raises(ValueError):
+        orc.q3(t, 0, 150)
+
+
+def test_packed_dates_of_the_generator_match_the_oracle(orc):
+    from pimdal_amd import datagen
+    days = np.concatenate([np.arange(8035, 8035 + 2600), np.array([-719162, 0, 11016, 2932896])]).astype(np.int32)
+    assert np.array_equal(datagen.pack_days(days), orc.date32_pack(days))
