@@ -124,7 +124,17 @@ int pimdal_memcpy_d2h(pimdal_ctx* ctx, void* dst, const void* src, uint64_t byte
 
 /* ---------------- columns ----------------
  * Replaces dist_table / get_buf (pimdal/support/transfer_helper.h:207-236):
- * a column is one device-resident, non-nullable, fixed-width array. */
+ * a column is one device-resident, fixed-width array, with an optional
+ * validity bitmap in Arrow's layout (bit i % 8 of byte i / 8, 1 = valid).
+ * The column forms of the operators (pimdal_select, pimdal_hash_join,
+ * pimdal_sort_merge_join, pimdal_hash_aggr, pimdal_sort_aggr) give NULLs
+ * SQL semantics, which the reference's non-nullable operators lack: a NULL
+ * key satisfies no select predicate and joins nothing; payload validity
+ * follows the kept rows and the joined pairs; in an aggregation the
+ * NULL-key rows form one group (the last, key NULL), SUM skips NULL values
+ * (NULL for a group with none) and COUNT counts the non-NULL values
+ * (groups then come in ascending key order).  The `*_dev` forms take no
+ * validity. */
 int pimdal_col_alloc(pimdal_ctx* ctx, pimdal_type type, uint64_t length, pimdal_col** out);
 int pimdal_col_from_host(pimdal_ctx* ctx, pimdal_type type, const void* data, uint64_t length, pimdal_col** out);
 /* Wraps caller-owned device memory (not freed by pimdal_col_free). */
@@ -132,10 +142,17 @@ int pimdal_col_wrap(pimdal_ctx* ctx, pimdal_type type, void* dptr, uint64_t leng
 int pimdal_col_to_host(pimdal_ctx* ctx, const pimdal_col* col, void* dst, uint64_t dst_bytes);
 int pimdal_col_info(const pimdal_col* col, pimdal_type* type, uint64_t* length, void** dptr);
 int pimdal_col_free(pimdal_ctx* ctx, pimdal_col* col);
-/* Arrow C Data Interface.  Import borrows the array until it returns and
- * rejects nullable (null_count != 0 with a validity buffer), non-primitive
- * or unsupported types with PIMDAL_E_UNSUPPORTED.  Export allocates host
- * buffers owned by `out`; the caller calls out->release and
+/* The column's device validity bitmap (NULL when it has no NULLs) and its
+ * NULL count. */
+int pimdal_col_validity(const pimdal_col* col, void** dptr, uint64_t* null_count);
+/* Sets the validity from a host bitmap of ceil(length / 8) bytes (Arrow
+ * layout); NULL, or a bitmap with every bit set, removes it. */
+int pimdal_col_set_validity(pimdal_ctx* ctx, pimdal_col* col, const uint8_t* bits);
+/* Arrow C Data Interface.  Import borrows the array until it returns,
+ * copies its validity bitmap (offset applied) when it has NULLs, and
+ * rejects non-primitive or unsupported types with PIMDAL_E_UNSUPPORTED.
+ * Export allocates host buffers owned by `out` (a validity buffer when the
+ * column has NULLs); the caller calls out->release and
  * out_schema->release. */
 int pimdal_col_import(pimdal_ctx* ctx, const struct ArrowArray* array, const struct ArrowSchema* schema, pimdal_col** out);
 /* Chunked column (an arrow::ChunkedArray, as a Parquet reader returns one

@@ -66,6 +66,8 @@ SIGNATURES: dict[str, list] = {
     "pimdal_col_to_host": [c_ctx_p, c_col_p, vp, u64],
     "pimdal_col_info": [c_col_p, C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(vp)],
     "pimdal_col_free": [c_ctx_p, c_col_p],
+    "pimdal_col_validity": [c_col_p, C.POINTER(vp), C.POINTER(u64)],
+    "pimdal_col_set_validity": [c_ctx_p, c_col_p, vp],
     "pimdal_col_import": [c_ctx_p, vp, vp, C.POINTER(c_col_p)],
     "pimdal_col_import_chunks": [c_ctx_p, vp, u32, vp, C.POINTER(c_col_p)],
     "pimdal_date_pack": [i32, i32, i32, C.POINTER(i32)],
@@ -112,7 +114,42 @@ SIGNATURES: dict[str, list] = {
     "pimdal_gen_orderkey": [c_ctx_p, u64, u64, u64, vp],
 }
 
-_RESTYPE = {"pimdal_last_error": C.c_char_p}
+_RESTYPE = {"pimdal_last_error": C.c_char_p, "pimdal_parquet_last_error": C.c_char_p}
+
+# libpimdal_parquet.so (include/pimdal_parquet.h)
+PARQUET_LIB_PATH = Path(__file__).resolve().parent / "libpimdal_parquet.so"
+PARQUET_SIGNATURES: dict[str, list] = {
+    "pimdal_parquet_last_error": [],
+    "pimdal_parquet_shape": [C.c_char_p, C.POINTER(u32), C.POINTER(u64)],
+    "pimdal_parquet_column_name": [C.c_char_p, u32, C.c_char_p, u64],
+    "pimdal_parquet_read_column": [c_ctx_p, C.c_char_p, C.c_char_p, i32, u32, C.POINTER(c_col_p), C.c_char_p, u64,
+                                   C.POINTER(u32)],
+}
+PARQUET_PACK_DATES, PARQUET_DICT_CODES = 1, 2
+_parquet: C.CDLL | None = None
+
+
+def load_parquet() -> C.CDLL:
+    """Load libpimdal_parquet.so (raises if it is missing)."""
+    global _parquet
+    if _parquet is not None:
+        return _parquet
+    load()
+    if not PARQUET_LIB_PATH.exists():
+        raise RuntimeError(f"{PARQUET_LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(PARQUET_LIB_PATH))
+    for name, args in PARQUET_SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, C.c_int)
+    _parquet = lib
+    return lib
+
+
+def call_parquet(name: str, *args) -> None:
+    rc = getattr(load_parquet(), name)(*args)
+    if rc != 0:
+        raise PimdalError(rc, (load_parquet().pimdal_parquet_last_error() or b"").decode())
 
 _lib: C.CDLL | None = None
 

@@ -1685,6 +1685,12 @@ int aggr_cols(AggFn fn, pimdal_ctx* c, const pimdal_col* key, const pimdal_col* 
     PD_REQUIRE(type_width(key->type) == 4, PIMDAL_E_UNSUPPORTED, "group key must be 32-bit");
     PD_REQUIRE(val->type == PIMDAL_I64 || val->type == PIMDAL_U64, PIMDAL_E_UNSUPPORTED, "values must be 64-bit");
     PD_REQUIRE(key->length == val->length, PIMDAL_E_INVALID, "column lengths differ");
+    if (key->validity || val->validity) {
+        // NULL keys form one group (last); SUM skips NULL values, COUNT counts
+        // the non-NULL ones; groups in ascending key order
+        aggr_cols_nullable(c, key, val, ok, os, oc);
+        return PIMDAL_OK;
+    }
     const uint64_t n = key->length, cap = std::max<uint64_t>(n, 1);
     pimdal_col *a = nullptr, *b = nullptr, *d = nullptr;
     try {

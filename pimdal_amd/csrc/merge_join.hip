@@ -445,8 +445,6 @@ extern "C" int pimdal_sort_merge_join_dev(pimdal_ctx* c, const uint32_t* build_k
 
 namespace {
 
-using JoinFn = uint64_t (*)(pimdal_ctx*, const uint32_t*, const uint32_t*, uint64_t, const uint32_t*,
-                            const uint32_t*, uint64_t, uint32_t*, uint32_t*, uint64_t, hipStream_t);
 
 int join_cols(JoinFn fn, pimdal_ctx* c, const pimdal_col* bk, const pimdal_col* bp, const pimdal_col* pk,
               const pimdal_col* pp, pimdal_col** op, pimdal_col** ob) {
@@ -456,6 +454,11 @@ int join_cols(JoinFn fn, pimdal_ctx* c, const pimdal_col* bk, const pimdal_col* 
     PD_REQUIRE(type_width(bk->type) == 4 && type_width(pk->type) == 4, PIMDAL_E_UNSUPPORTED, "join keys must be 32-bit");
     PD_REQUIRE(!bp || (bp->length == bk->length && type_width(bp->type) == 4), PIMDAL_E_INVALID, "bad build payload");
     PD_REQUIRE(!pp || (pp->length == pk->length && type_width(pp->type) == 4), PIMDAL_E_INVALID, "bad probe payload");
+    if (bk->validity || pk->validity || (bp && bp->validity) || (pp && pp->validity)) {
+        // NULL keys join nothing; payload validity follows the pairs
+        join_cols_nullable(fn, c, bk, bp, pk, pp, op, ob);
+        return PIMDAL_OK;
+    }
     // first call sizes the output (probe-sized buffers cover the unique-build case in one pass)
     uint64_t cap = pk->length;
     pimdal_col* a = nullptr;

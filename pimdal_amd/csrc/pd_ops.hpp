@@ -50,6 +50,21 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
 void flag_codes(pimdal_ctx* c, const uint8_t* f1, const uint8_t* f2, uint64_t n, uint8_t* lut_dev, uint32_t meta[2],
                 uint8_t* inv_host, hipStream_t s);
 
+// Device join entry (hash or sort-merge): pairs of (probe payload, build
+// payload); returns the count, writes min(count, cap).
+using JoinFn = uint64_t (*)(pimdal_ctx*, const uint32_t*, const uint32_t*, uint64_t, const uint32_t*,
+                            const uint32_t*, uint64_t, uint32_t*, uint32_t*, uint64_t, hipStream_t);
+
+// Column forms with validity bitmaps (nulls.hip): SQL null semantics over
+// the non-nullable kernels.
+uint64_t rows_with_bit(pimdal_ctx* c, const void* bits, uint64_t n, bool want, uint32_t* out_rows, hipStream_t s);
+void select_cols_nullable(pimdal_ctx* c, const pimdal_col* key, const pimdal_col* payload, int64_t lo, int64_t hi,
+                          pimdal_col** out_payload, pimdal_col** out_rowid);
+void join_cols_nullable(JoinFn fn, pimdal_ctx* c, const pimdal_col* bk, const pimdal_col* bp, const pimdal_col* pk,
+                        const pimdal_col* pp, pimdal_col** out_p, pimdal_col** out_b);
+void aggr_cols_nullable(pimdal_ctx* c, const pimdal_col* key, const pimdal_col* val, pimdal_col** ok, pimdal_col** os,
+                        pimdal_col** oc);
+
 // TPC-H Q3 shape (q3.hip): returns the group count, the first min(count,
 // cap) groups written in TPC-H order.
 uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int32_t date, const pimdal_q3_result& out,

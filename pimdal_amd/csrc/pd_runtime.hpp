@@ -117,11 +117,19 @@ struct pimdal_col {
     uint64_t capacity;
     void* data;      // device memory
     bool owned;      // allocated from the context pool
+    // validity bitmap (device, Arrow layout: bit i % 8 of byte i / 8, 1 =
+    // valid, ceil(length / 8) bytes rounded up to whole 32-bit words), pool
+    // memory owned by the column; nullptr = no nulls
+    void* validity = nullptr;
+    uint64_t null_count = 0;
 };
 
 namespace pd {
 
 uint32_t type_width(pimdal_type t);
+// Validity bitmaps: 32-bit words for n rows; set a column's from host words.
+uint64_t valid_words(uint64_t n);
+void set_validity_words(pimdal_ctx* c, pimdal_col* col, const uint32_t* bits, uint64_t nulls);
 
 // Temporary device buffer from the context pool, freed on scope exit
 // (stream ordered on the context stream).

@@ -6,6 +6,7 @@
 #include <string.h>
 #include <algorithm>
 #include <new>
+#include <vector>
 
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
@@ -550,6 +551,7 @@ extern "C" int pimdal_col_free(pimdal_ctx* c, pimdal_col* col) {
     PD_REQUIRE(c, PIMDAL_E_INVALID, "ctx is NULL");
     if (!col) return PIMDAL_OK;
     if (col->owned && col->data) c->pool.free(col->data, c->stream);
+    if (col->validity) c->pool.free(col->validity, c->stream);
     delete col;
     PD_API_END
 }
@@ -598,8 +600,8 @@ static const char* import_values(const struct ArrowArray* a, uint32_t w) {
     PD_REQUIRE(a->n_children == 0 && !a->dictionary, PIMDAL_E_UNSUPPORTED,
                "nested or dictionary arrays are not supported");
     PD_REQUIRE(a->n_buffers == 2, PIMDAL_E_UNSUPPORTED, "expected a primitive array with 2 buffers");
-    PD_REQUIRE(a->null_count == 0 || (a->null_count == -1 && a->buffers[0] == nullptr),
-               PIMDAL_E_UNSUPPORTED, "nullable columns are not supported (null_count != 0)");
+    PD_REQUIRE(a->null_count == 0 || a->buffers[0] != nullptr || a->null_count == -1, PIMDAL_E_INVALID,
+               "null_count > 0 without a validity buffer");
     PD_REQUIRE(a->length >= 0 && a->offset >= 0, PIMDAL_E_INVALID, "negative length or offset");
     const char* base = (const char*)a->buffers[1];
     PD_REQUIRE(base || a->length == 0, PIMDAL_E_INVALID, "missing data buffer");
@@ -637,6 +639,28 @@ extern "C" int pimdal_col_import_chunks(pimdal_ctx* c, const struct ArrowArray* 
             if (len) stage_h2d(c, (char*)col->data + at * w, import_values(chunks[i], w), len * w);
             at += len;
         }
+        // validity: the chunks' bitmaps (their offsets applied) concatenated
+        // at bit level into one bitmap from bit 0; chunks without one are all
+        // valid; no bitmap at all when nothing is null
+        bool any = false;
+        for (uint32_t i = 0; i < nchunks; i++) any |= chunks[i]->buffers[0] != nullptr && chunks[i]->null_count != 0;
+        if (any) {
+            std::vector<uint32_t> bits(valid_words(total), 0u);
+            uint64_t nulls = 0;
+            at = 0;
+            for (uint32_t i = 0; i < nchunks; i++) {
+                const struct ArrowArray* a = chunks[i];
+                const uint8_t* vb = (const uint8_t*)a->buffers[0];
+                const uint64_t len = (uint64_t)a->length, off = (uint64_t)a->offset;
+                for (uint64_t r = 0; r < len; r++) {
+                    const bool ok = !vb || a->null_count == 0 || ((vb[(off + r) >> 3] >> ((off + r) & 7)) & 1);
+                    if (ok) bits[(at + r) >> 5] |= 1u << ((at + r) & 31);
+                    else nulls++;
+                }
+                at += len;
+            }
+            if (nulls) set_validity_words(c, col, bits.data(), nulls);
+        }
     } catch (...) {
         pimdal_col_free(c, col);
         throw;
@@ -645,15 +669,62 @@ extern "C" int pimdal_col_import_chunks(pimdal_ctx* c, const struct ArrowArray* 
     PD_API_END
 }
 
+// 32-bit words of a validity bitmap of n rows.
+uint64_t pd::valid_words(uint64_t n) { return (n + 31) / 32; }
+
+// Gives `col` the validity of host bitmap words `bits` (valid_words(length)
+// words, bits past the length ignored) and its null count.
+void pd::set_validity_words(pimdal_ctx* c, pimdal_col* col, const uint32_t* bits, uint64_t nulls) {
+    const uint64_t nw = valid_words(col->length);
+    if (!col->validity) col->validity = c->pool.alloc(std::max<uint64_t>(nw, 1) * 4, c->stream);
+    stage_h2d(c, col->validity, bits, nw * 4);
+    col->null_count = nulls;
+}
+
+extern "C" int pimdal_col_validity(const pimdal_col* col, void** dptr, uint64_t* null_count) {
+    PD_API_BEGIN
+    PD_REQUIRE(col, PIMDAL_E_INVALID, "col is NULL");
+    if (dptr) *dptr = col->validity;
+    if (null_count) *null_count = col->null_count;
+    PD_API_END
+}
+
+extern "C" int pimdal_col_set_validity(pimdal_ctx* c, pimdal_col* col, const uint8_t* bits) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && col, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
+    if (!bits) {
+        if (col->validity) c->pool.free(col->validity, c->stream);
+        col->validity = nullptr;
+        col->null_count = 0;
+        return PIMDAL_OK;
+    }
+    std::vector<uint32_t> w(std::max<uint64_t>(valid_words(col->length), 1), 0u);
+    memcpy(w.data(), bits, (col->length + 7) / 8);
+    uint64_t nulls = 0;
+    for (uint64_t r = 0; r < col->length; r++) nulls += !((w[r >> 5] >> (r & 31)) & 1);
+    if (col->length % 32) w[col->length / 32] &= (1u << (col->length % 32)) - 1u;   // bits past the length: 0
+    if (!nulls) {
+        if (col->validity) c->pool.free(col->validity, c->stream);
+        col->validity = nullptr;
+        col->null_count = 0;
+        return PIMDAL_OK;
+    }
+    set_validity_words(c, col, w.data(), nulls);
+    PD_API_END
+}
+
 namespace {
 struct ExportPriv {
     void* data;
+    void* valid;
     const void* buffers[2];
 };
 void release_array(struct ArrowArray* a) {
     if (!a || !a->release) return;
     auto* p = (ExportPriv*)a->private_data;
     free(p->data);
+    free(p->valid);
     delete p;
     a->release = nullptr;
 }
@@ -676,12 +747,29 @@ extern "C" int pimdal_col_export(pimdal_ctx* c, const pimdal_col* col, struct Ar
         free(host);
         throw;
     }
+    void* vhost = nullptr;
+    if (col->validity) {
+        const uint64_t vb = valid_words(col->length) * 4;
+        vhost = aligned_alloc(64, std::max<uint64_t>((vb + 63) & ~63ull, 64));
+        if (!vhost) {
+            free(host);
+            throw std::bad_alloc();
+        }
+        try {
+            stage_d2h(c, vhost, col->validity, vb);
+        } catch (...) {
+            free(host);
+            free(vhost);
+            throw;
+        }
+    }
     auto* p = new ExportPriv;
     p->data = host;
-    p->buffers[0] = nullptr;
+    p->valid = vhost;
+    p->buffers[0] = vhost;
     p->buffers[1] = host;
     out->length = (int64_t)col->length;
-    out->null_count = 0;
+    out->null_count = (int64_t)col->null_count;
     out->offset = 0;
     out->n_buffers = 2;
     out->n_children = 0;

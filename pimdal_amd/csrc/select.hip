@@ -538,6 +538,11 @@ extern "C" int pimdal_select(pimdal_ctx* c, const pimdal_col* key, const pimdal_
     PD_REQUIRE(!payload || payload->length == key->length, PIMDAL_E_INVALID, "payload length differs from key length");
     PD_REQUIRE(!payload || type_width(payload->type) == 4, PIMDAL_E_UNSUPPORTED, "payload must be 4 bytes wide");
     PD_REQUIRE(out_payload || out_rowid, PIMDAL_E_INVALID, "no output requested");
+    if (key->validity || (payload && payload->validity)) {
+        // a NULL key satisfies no predicate; kept payloads keep their validity
+        select_cols_nullable(c, key, payload, lo, hi, out_payload, out_rowid);
+        return PIMDAL_OK;
+    }
     const uint64_t n = key->length;
     const pimdal_type pt = payload ? payload->type : key->type;
     pimdal_col* op = nullptr;

@@ -8,9 +8,15 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def declared():
+# header -> the library that exports its entry points
+LIBS = {"pimdal_parquet.h": "libpimdal_parquet.so"}
+
+
+def declared(lib="libpimdal_gpu.so"):
     names = set()
     for h in (ROOT / "include").glob("*.h"):
+        if LIBS.get(h.name, "libpimdal_gpu.so") != lib:
+            continue
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
         names |= set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(pimdal_\w+)\s*\(", text, flags=re.M))
     return names
@@ -21,19 +27,24 @@ def test_header_declares_entry_points():
     assert {"pimdal_ctx_create", "pimdal_select", "pimdal_sort_pairs"} <= d
 
 
-def test_library_exports_every_declared_symbol():
+@pytest.mark.parametrize("lib_name", ["libpimdal_gpu.so", "libpimdal_parquet.so"])
+def test_library_exports_every_declared_symbol(lib_name):
     import ctypes
-    lib_path = ROOT / "pimdal_amd" / "libpimdal_gpu.so"
+    lib_path = ROOT / "pimdal_amd" / lib_name
     if not lib_path.exists():
-        pytest.fail("libpimdal_gpu.so is not built (run __graft_entry__.build())")
+        pytest.fail(f"{lib_name} is not built (run __graft_entry__.build())")
+    ctypes.CDLL(str(ROOT / "pimdal_amd" / "libpimdal_gpu.so"))
     lib = ctypes.CDLL(str(lib_path))
-    missing = [n for n in sorted(declared()) if not hasattr(lib, n)]
+    names = declared(lib_name)
+    assert names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
 
 
 def test_python_binding_covers_header():
     from pimdal_amd import _lib
     assert declared() == set(_lib.SIGNATURES)
+    assert declared("libpimdal_parquet.so") == set(_lib.PARQUET_SIGNATURES)
 
 
 def test_load_fails_loudly_without_library(tmp_path, monkeypatch):

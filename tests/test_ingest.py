@@ -76,12 +76,18 @@ def test_import_chunked_array(ctx):
 
 
 @pytest.mark.gpu
-def test_import_refuses_nulls_and_other_types(ctx):
+def test_import_keeps_nulls_and_refuses_other_types(ctx):
+    """Nullable chunks (with offsets) import with their validity concatenated
+    at bit level and export back unchanged; strings are refused."""
     import pyarrow as pa
-    ok = pa.array(np.arange(10, dtype=np.int64))
-    with pytest.raises(pimdal_amd.PimdalError) as e:
-        ingest.import_arrow(ctx, pa.chunked_array([ok, pa.array([1, None, 3], type=pa.int64())]))
-    assert e.value.code == -2
+    vals = [1, None, 3, None, 5, 6, 7, None, 9, 10, None]
+    a = pa.array(vals, type=pa.int64())
+    ch = pa.chunked_array([pa.array(np.arange(13, dtype=np.int64)), a.slice(1), pa.array([None, 4], type=pa.int64()),
+                           a.slice(3, 5)])
+    col = ingest.import_arrow(ctx, ch)
+    back = ingest.column_to_arrow(ctx, col)
+    assert back.to_pylist() == ch.to_pylist() and back.null_count == ch.null_count
+    col.free()
     with pytest.raises(pimdal_amd.PimdalError) as e:
         ingest.import_arrow(ctx, pa.array(["a", "b"]))
     assert e.value.code == -2
@@ -117,3 +123,66 @@ def test_read_parquet_row_groups_and_dates(ctx, orc, tmp_path):
     assert np.array_equal((packed[:20000] >= lo) & (packed[:20000] < hi), ref)
     for c in list(cols.values()) + list(raw.values()):
         c.free()
+
+
+@pytest.mark.gpu
+def test_read_parquet_nullable_columns_across_row_groups(ctx, tmp_path):
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+    rng = np.random.default_rng(17)
+    n = 70_001
+    mask = rng.random(n) < 0.2
+    t = pa.table({"a": pa.array(rng.integers(-1 << 40, 1 << 40, n), mask=mask),
+                  "b": pa.array(rng.integers(0, 1 << 31, n).astype(np.int32), mask=rng.random(n) < 0.05),
+                  "c": pa.array(rng.integers(0, 1 << 32, n).astype(np.uint32))})
+    path = tmp_path / "nullable.parquet"
+    pq.write_table(t, path, row_group_size=8192)
+    cols = ingest.read_parquet(ctx, str(path))
+    for name in ("a", "b", "c"):
+        back = ingest.column_to_arrow(ctx, cols[name])
+        assert back.to_pylist() == t.column(name).to_pylist(), name
+        cols[name].free()
+
+
+@pytest.mark.gpu
+def test_q3_from_parquet_files(ctx, orc, tmp_path):
+    """customer / orders / lineitem written as Parquet (segments as strings,
+    dates as date32, several row groups), read by the C++ reader (strings
+    to sorted dictionary codes, dates packed on the device), then the Q3
+    chain: every group equals the oracle's on the original tables."""
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+    from pimdal_amd import datagen
+    t = datagen.q3_tables(400_000, seed=77)
+
+    def days(packed):
+        p = packed.astype(np.int64)
+        y, m, d = (p >> 9) + 1900, (p >> 5) & 15, p & 31
+        ym = (y - 1970) * 12 + (m - 1)
+        return (np.array(ym, dtype="datetime64[M]").astype("datetime64[D]") - np.datetime64("1970-01-01", "D")
+                ).astype(np.int64) + d - 1
+
+    segs = np.array(datagen.Q3_SEGMENTS)
+    files = {
+        "customer": pa.table({"c_custkey": t["c_custkey"], "c_mktsegment": pa.array(segs[t["c_mktsegment"]])}),
+        "orders": pa.table({"o_orderkey": t["o_orderkey"], "o_custkey": t["o_custkey"],
+                            "o_orderdate": pa.array(days(t["o_orderdate"]).astype(np.int32), type=pa.date32()),
+                            "o_shippriority": t["o_shippriority"]}),
+        "lineitem": pa.table({"l_orderkey": t["l_orderkey"], "l_extendedprice": t["l_extendedprice"],
+                              "l_discount": t["l_discount"],
+                              "l_shipdate": pa.array(days(t["l_shipdate"]).astype(np.int32), type=pa.date32())}),
+    }
+    d = {}
+    for name, tab in files.items():
+        path = tmp_path / f"{name}.parquet"
+        pq.write_table(tab, path, row_group_size=50_000)
+        d.update(ingest.read_parquet(ctx, str(path)))
+    assert d["c_mktsegment"].dictionary == sorted(datagen.Q3_SEGMENTS)
+    assert np.array_equal(d["o_orderdate"].numpy(), t["o_orderdate"])
+    seg = d["c_mktsegment"].dictionary.index("BUILDING")
+    date = ops.date_pack_str("1995-03-15")
+    exp, g_exp = orc.q3(t, datagen.Q3_SEGMENTS.index("BUILDING"), date)
+    got, g = ops.tpch_q3(ctx, d, seg, date, capacity=max(g_exp, 1))
+    assert g == g_exp and g > 0
+    for f in ("key", "revenue", "orderdate", "shippriority"):
+        assert np.array_equal(got[f], exp[f]), f
