@@ -9,7 +9,7 @@ operators' rows/s at their configured sizes, the dominant kernel's roofline
 a bounded sample on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--only sort|zipf|sortselect|join|bloom|groupby|q3|ingest|serve]
+                  [--only sort|zipf|sortselect|join|bloom|groupby|aggr|q3|ingest|serve]
 """
 from __future__ import annotations
 
@@ -124,7 +124,8 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
-           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order")
+           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "as_table", "as_emit",
+           "as_gather", "as_runs", "as_fix")
 
 
 def kernel_breakdown(ctx, steps):
@@ -315,6 +316,35 @@ def bench_q6(ctx, steps, warmup, n=60_000_000):
     return res
 
 
+AGGR_SHAPES = {"k50": (1, 50), "k4000": (1, 4000), "k25": (1, 25), "k2^24": (0, (1 << 24) - 1)}
+
+
+def bench_aggr(ctx, steps, warmup, which, shape, n=1 << 27):
+    """Hash or sort aggregation (SUM and COUNT per uint32 key) of n rows with
+    keys uniform over the shape's range and int64 values on [1, 10].
+    Algorithmic bytes: key and value read once, 20 bytes out per group."""
+    import ctypes as C
+    from pimdal_amd import datagen, _lib
+    lo, hi = AGGR_SHAPES[shape]
+    key, val = datagen.aggr_workload(ctx, n, lo, hi)
+    cap = min(n, hi - lo + 1)
+    ok, osum, ocnt = ctx.empty(cap, "uint32"), ctx.empty(cap, "int64"), ctx.empty(cap, "int64")
+    g = C.c_uint64()
+    fn = getattr(_lib.load(), f"pimdal_{which}_aggr_dev")
+
+    def step():
+        _lib.check(fn(ctx.handle, key.vp, val.vp, n, ok.vp, osum.vp, ocnt.vp, cap, C.byref(g)))
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    alg = 12 * n + 20 * g.value
+    res = {"rows": n, "groups": g.value, "key_range": [lo, hi], "s_per_step": tm, "rows_per_s": n / tm,
+           "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
+           "kernels": kernel_breakdown(ctx, steps)}
+    for a in (key, val, ok, osum, ocnt):
+        a.free()
+    return res
+
+
 def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, rate_qps=1e6, slots=4,
                 sweep=(2e3, 1e4, 1e5)):
     """Mixed select/join/group-by query stream over resident tables (6e7-row
@@ -357,20 +387,185 @@ def bench_serve(ctx, nq=10_000, warm=300, n_fact=60_000_000, n_dim=15_000_000, r
     return res
 
 
-def cpu_baseline_sort(sample_n=1 << 24, reps=3):
-    """The CPU restatement (oracle/) of the stable pair sort, all host threads."""
+# ---------------- CPU baseline: the oracle's restatement on this host ----------------
+#
+# Every operator line above is timed again on the host CPU, on a bounded
+# sample of the same seeded workload (host generators of the same recipes),
+# at the two thread settings of the reference's CPU runs: all cores, and 16
+# OpenMP threads on NUMA node 0 (reference/db_operators/profile.sh:6-13 runs
+# each operator under `numactl --cpunodebind=0 --membind=0` with
+# OMP_NUM_THREADS=16).  Each setting runs in a child process of this script
+# (`--cpu-child`, which never touches the GPU), started before the GPU is
+# initialised, so that the thread count and CPU binding hold for every
+# thread.  Per operator: inputs generated (not timed), then the median of
+# CPU_REPS timed runs; output allocation is inside the timing, as in
+# reference/db_operators/select.cpp:115-135.
+
+CPU_REPS = 5
+
+
+def _cpu_cases():
+    """(operator line, sample, rows counted, inputs factory, run) for every
+    operator; rows are counted as the GPU line counts them."""
     import numpy as np
     from oracle import orc
-    k, v = orc.sort_workload(sample_n)
-    orc.sort_pairs(k[:1024], v[:1024])
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        orc.sort_pairs(k, v)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    return {"value": sample_n / t, "unit": "rows/s", "cores": orc.num_threads(), "kind": "port",
-            "sample": f"2^{int(np.log2(sample_n))} uniform (u32,u32) pairs, stable sort, median of {reps}"}
+    from pimdal_amd import datagen   # host-side generators only (Q3 tables, packed dates)
+
+    def bloom_inputs(nb, npr):
+        bk = orc.gen_u32(0xB10, 1, nb)
+        pk, pp = orc.gen_u32(0xB10, 2, npr), orc.gen_u32(0xB10, 3, npr)
+        for q in range(0, npr, 4 * nb):
+            m = min(nb, npr - q)
+            pk[q:q + m] = bk[:m]
+        return bk, pk, pp
+
+    def bloom_run(bits):
+        def run(bk, pk, pp):
+            f = orc.bloom_build(bk, bits, 32)
+            return orc.bloom_probe(pk, pp, bits, 32, f)
+        return run
+
+    q3_date = int(datagen.pack_days(np.array([9204]))[0])
+    q3_seg = datagen.Q3_SEGMENTS.index("BUILDING")
+    cases = [
+        ("sort_2^28", "2^24 uniform (u32 key, u32 row id) pairs, stable sort", 1 << 24,
+         lambda: orc.sort_workload(1 << 24), orc.sort_pairs),
+        ("sort_zipf_2^28", "2^24 pairs, keys fmix32(Zipf 1.0 over 2^24 values), stable sort", 1 << 24,
+         lambda: orc.sort_workload(1 << 24, zipf=True), orc.sort_pairs),
+        ("select_2^27", "2^27 (u32 key, u32 payload) rows (the whole input), key < 429496730 (10 %)", 1 << 27,
+         lambda: orc.select_workload(1 << 27), lambda k, v: orc.select(k, v, 0, 429496730)),
+    ]
+    for which, fn in (("hash", orc.hash_join), ("smj", orc.sort_merge_join)):
+        for z in (False, True):
+            cases.append((f"{which}_join{'_zipf' if z else ''}",
+                          f"build 2^21 unique x probe 2^24 {'Zipf 1.0' if z else 'uniform'} FK (one eighth of each side)",
+                          (1 << 21) + (1 << 24), (lambda z=z: orc.join_workload(1 << 21, 1 << 24, zipf=z)), fn))
+    for nb, bits in ((1 << 16, 5), (1 << 20, 9)):
+        cases.append((f"bloom_semi_join_2^{nb.bit_length() - 1}x2^27",
+                      f"2^{nb.bit_length() - 1} build keys into {1 << bits} filters, probe 2^24 rows",
+                      nb + (1 << 24), (lambda nb=nb: bloom_inputs(nb, 1 << 24)), bloom_run(bits)))
+    # aggregation: the whole 2^27 rows where the oracle is fast, else the
+    # first rows of the same draw
+    for which, shape, rows in (("hash", "k50", 1 << 27), ("hash", "k4000", 1 << 27), ("hash", "k2^24", 1 << 20),
+                               ("sort", "k25", 1 << 24), ("sort", "k2^24", 1 << 24)):
+        lo, hi = AGGR_SHAPES[shape]
+        what = "the whole input" if rows == 1 << 27 else "the first rows of the 2^27-row draw"
+        cases.append((f"{which}_aggr_2^27_{shape}", f"2^{rows.bit_length() - 1} rows ({what}), keys uniform on "
+                      f"[{lo}, {hi}]", rows, (lambda lo=lo, hi=hi, rows=rows: orc.aggr_workload(rows, lo, hi)),
+                      orc.hash_aggr if which == "hash" else orc.sort_aggr))
+    fact = lambda: (orc.fact_table(60_000_000, groups=15_000_000),)
+    cases += [
+        ("q1_flags", "the whole 6e7-row fact table, date <= 2505, groups by (flag1, flag2)", 60_000_000, fact,
+         lambda t: orc.q1_flags(t, 2505)),
+        ("q1_orderkey", "2^20-row fact table with 2^18 orderkeys (4 rows per key as in 6e7 rows / 1.5e7 keys), "
+         "date <= 2505", 1 << 20, lambda: (orc.fact_table(1 << 20, groups=1 << 18),), lambda t: orc.q1_key(t, 2505)),
+        ("q6", "the whole 6e7-row fact table, TPC-H Q6 predicate", 60_000_000, fact, lambda t: orc.q6(t)),
+        ("q3_sf10", "7.5e6 lineitems, 1.875e6 orders, 1.875e5 customers (SF 1.25), top 10", 7_500_000 + 1_875_000 + 187_500,
+         lambda: (datagen.q3_tables(7_500_000),), lambda t: orc.q3(t, q3_seg, q3_date, cap=10)),
+        ("date32_pack_6e7", "2^20 date32 values over 1992-1998 (the C library's gmtime_r per value, which "
+         "takes glibc's time-zone lock)", 1 << 20,
+         lambda: (orc.gen_range(np.int32, 0xDA7E, 1, 1 << 20, 8035, 8035 + 2556),), orc.date32_pack),
+    ]
+    return cases
+
+
+def cpu_child(threads):
+    """Times every case at `threads` OpenMP threads; prints one JSON object."""
+    from oracle import orc
+    orc.set_threads(threads)
+    out = {}
+    for name, sample, rows, make, run in _cpu_cases():
+        args = make()
+        if not isinstance(args, tuple):
+            args = (args,)
+        times = []
+        for _ in range(CPU_REPS):
+            t0 = time.perf_counter()
+            run(*args)
+            times.append(time.perf_counter() - t0)
+        del args
+        t = statistics.median(times)
+        out[name] = {"rows_per_s": rows / t, "median_s": t, "min_s": min(times), "max_s": max(times),
+                     "rows": rows, "sample": sample}
+        log(f"cpu baseline ({threads} threads): {name} {rows / t / 1e6:.1f} M rows/s")
+    print(json.dumps({"threads": orc.num_threads(), "cases": out}), flush=True)
+
+
+def _cpulist(text):
+    cpus = set()
+    for part in text.strip().split(","):
+        if "-" in part:
+            a, b = part.split("-")
+            cpus.update(range(int(a), int(b) + 1))
+        elif part:
+            cpus.add(int(part))
+    return cpus
+
+
+def host_info():
+    """CPU model, NUMA nodes with their CPUs, this process's CPU affinity and
+    the cgroup CPU quota."""
+    info = {"cpu_model": None, "numa_nodes": {}, "affinity_cpus": len(os.sched_getaffinity(0)),
+            "online_cpus": os.cpu_count(), "cgroup_cpu_quota": None}
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    for d in sorted(Path("/sys/devices/system/node").glob("node[0-9]*")):
+        try:
+            info["numa_nodes"][d.name] = (d / "cpulist").read_text().strip()
+        except OSError:
+            pass
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if q != "max":
+            info["cgroup_cpu_quota"] = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_baselines(timeout_s=300):
+    """Runs the CPU child at both thread settings; returns the cpu_baseline
+    object: the headline sort at all cores as value, every operator at both
+    settings, and the host description."""
+    import math
+    import shutil
+    import subprocess
+    info = host_info()
+    allowed = os.sched_getaffinity(0)
+    n_all = len(allowed)
+    if info["cgroup_cpu_quota"]:
+        n_all = max(1, min(n_all, math.floor(info["cgroup_cpu_quota"])))
+    node0 = _cpulist(info["numa_nodes"].get("node0", "")) & allowed or allowed
+    numactl = shutil.which("numactl")
+    settings = {"all_cores": (n_all, None), "node0_16": (16, node0)}
+    res = {}
+    for name, (threads, cpus) in settings.items():
+        cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-child", str(threads)]
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        pre = None
+        if cpus is not None:
+            if numactl:
+                cmd = [numactl, "--cpunodebind=0", "--membind=0"] + cmd
+                binding = "numactl --cpunodebind=0 --membind=0"
+            else:
+                pre = lambda cpus=cpus: os.sched_setaffinity(0, cpus)
+                binding = (f"affinity to node0's {len(cpus)} CPUs (no numactl on this host: memory on node 0 "
+                           f"by first touch from threads bound there)")
+        else:
+            binding = f"none ({threads} threads over the {len(allowed)} CPUs this process may use)"
+        log(f"bench: cpu baseline, {name}")
+        p = subprocess.run(cmd, env=env, preexec_fn=pre, stdout=subprocess.PIPE, timeout=timeout_s, check=True)
+        r = json.loads(p.stdout.decode().strip().splitlines()[-1])
+        res[name] = {"threads": r["threads"], "binding": binding, "cases": r["cases"]}
+    head = res["all_cores"]["cases"]["sort_2^28"]
+    return {"value": head["rows_per_s"], "unit": "rows/s", "cores": res["all_cores"]["threads"], "kind": "port",
+            "sample": head["sample"] + f", median of {CPU_REPS}", "reps": CPU_REPS, "host": info,
+            "settings": res}
 
 
 def main():
@@ -383,11 +578,21 @@ def main():
     ap.add_argument("--tune", action="append", default=[], help="tuning knob key=value (repeatable)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device-to-device copy (the profiler's byte-counter reference)")
+    ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args.cpu_child)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path; "
               "run with --gpus 1 (how the operators would shard is documented with the design)", file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline runs first, in child processes, before this process
+    # initialises the GPU
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baselines()
 
     import pimdal_amd
     ctx = pimdal_amd.Context(0)
@@ -435,6 +640,11 @@ def main():
                                                                                          by_key)
         log("bench: TPC-H Q6 shape 6e7")
         ops_out["q6"] = bench_q6(ctx, max(3, args.steps // 2), 1)
+    if args.only in ("", "all", "aggr"):
+        for which, shape in (("hash", "k50"), ("hash", "k4000"), ("hash", "k2^24"), ("sort", "k25"),
+                             ("sort", "k2^24")):
+            log(f"bench: {which} aggregation 2^27 keys {AGGR_SHAPES[shape]}")
+            ops_out[f"{which}_aggr_2^27_{shape}"] = bench_aggr(ctx, max(3, args.steps // 2), 1, which, shape)
     if args.only in ("", "all", "q3"):
         log("bench: TPC-H Q3 shape, 6e7 lineitems")
         ops_out["q3_sf10"] = bench_q3(ctx, max(3, args.steps // 2), 1)
@@ -450,10 +660,13 @@ def main():
         ctx.set_tune("serve_join_index", 0)
         ops_out["serve_mixed_stream_probe_index"] = bench_serve(ctx, sweep=())
         ctx.set_tune("serve_join_index", 1)
-    cpu = None
-    if not args.no_cpu:
-        log("bench: cpu baseline")
-        cpu = cpu_baseline_sort()
+    if cpu:
+        for name, st in cpu["settings"].items():
+            for op, r in st["cases"].items():
+                tgt = ops_out.get(op) if op != "sort_2^28" else srt
+                if tgt is not None and tgt.get("rows_per_s"):
+                    tgt.setdefault("cpu_rows_per_s", {})[name] = r["rows_per_s"]
+                    tgt.setdefault("gpu_over_cpu", {})[name] = tgt["rows_per_s"] / r["rows_per_s"]
     pass_s = srt["pass_ms"] / 1e3
     achieved = srt["pass_alg_bytes"] / pass_s if pass_s else None
     line = {
@@ -473,7 +686,7 @@ def main():
                    "rows": srt["n"], "passes": 4, "digit_bits": 8,
                    "alg_bytes_per_row": 68, "sort_alg_GBps": srt["alg_GBps"], "sort_hbm_frac": srt["hbm_frac"],
                    "hist_ms": srt["hist_ms"], "count_ms": srt["count_ms"], "tscan_ms": srt["tscan_ms"],
-                   "keys_payload_GBps": srt["keys_payload_GBps"]},
+                   "keys_payload_GBps": srt["keys_payload_GBps"], "gpu_over_cpu": srt.get("gpu_over_cpu")},
         "roofline": {"kernel": "radix_onesweep", "bound": "hbm",
                      "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,

@@ -98,6 +98,8 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   decoupled look-back over per-tile match counts);
  *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,
  *   gb_q1_slots (1024 or 2048);
+ *   aggr_small (1: hash and sort aggregation first try per-block LDS tables
+ *   for up to 4096 keys per block and 32768 in all; 0: skip that path);
  *   serve_index_load_pct. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);

@@ -317,7 +317,8 @@ def set_threads(t: int) -> None:
 
 # ---------------- host versions of the workload recipes ----------------
 
-SEEDS = {"select": 0x5EED0001, "sort": 0x5EED0002, "join": 0x5EED0003, "groupby": 0x5EED0004, "serve": 0x5EED0005}
+SEEDS = {"select": 0x5EED0001, "sort": 0x5EED0002, "join": 0x5EED0003, "groupby": 0x5EED0004, "serve": 0x5EED0005,
+         "aggr": 0x5EED0007}
 COL = dict(KEY=1, PAYLOAD=2, BUILD_PERM=3, PROBE_FK=4, BUILD_PAY=5, PROBE_PAY=6, ZIPF=7, QTY=12, PRICE=13,
            DISC=14, TAX=15, DATE=16, FLAG1=17, FLAG2=18)
 
@@ -344,6 +345,10 @@ def join_workload(nr, ns, zipf=False, seed=SEEDS["join"]):
         sk = gen_range(np.uint32, seed, COL["PROBE_FK"], ns, 0, nr - 1)
     sp = gen_u32(seed, COL["PROBE_PAY"], ns)
     return rk, rp, sk, sp
+
+
+def aggr_workload(n, lo, hi, seed=SEEDS["aggr"]):
+    return gen_range(np.uint32, seed, COL["KEY"], n, lo, hi), gen_range(np.int64, seed, COL["PAYLOAD"], n, 1, 10)
 
 
 def fact_table(n, seed=SEEDS["groupby"], groups=None):

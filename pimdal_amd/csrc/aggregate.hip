@@ -1509,25 +1509,13 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
     if (n == 0) return 0;
     PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "hash aggregation supports up to 2^30-1 rows per call");
-    GbSource src{};
-    src.key = key;
-    src.in[0] = val;
+    // few groups: per-block LDS tables merged in one HBM table
+    const int64_t hint = c->tune_get("gb_groups_hint", 0);
     uint64_t g = 0;
-    {
-        // few groups: per-block LDS tables merged in one small HBM table
-        const uint64_t stage = std::min<uint64_t>(n, std::max<uint64_t>(cap, 1));
-        const int64_t hint = c->tune_get("gb_groups_hint", 0);
-        if (hint == 0 || hint <= (int64_t)(AggSum::TS / 4)) {
-            Scratch k32(c, stage * 4, s), fs(c, stage * 16, s);
-            if (group_by_direct<AggSum>(c, src, n, k32.as<uint32_t>(), fs.as<uint64_t>(), stage, &g, s)) {
-                PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-                copy_d2d(c, out_key, k32.get(), g * 4, s);
-                copy_d2d(c, out_sum, fs.get(), g * 8, s);
-                copy_d2d(c, out_cnt, fs.as<uint64_t>() + stage, g * 8, s);
-                PD_HIP(hipStreamSynchronize(s));
-                return g;
-            }
-        }
+    if (c->tune_get("aggr_small", 1) && hint <= 4096 &&
+        aggr_sum_small(c, key, val, n, out_key, out_sum, out_cnt, cap, false, &g, s)) {
+        PD_HIP(hipStreamSynchronize(s));
+        return g;
     }
     // otherwise onesweep partitions of (key, value) by hash digits and one
     // LDS table per partition, writing the outputs directly
@@ -1538,65 +1526,27 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
     return g;
 }
 
-// ===================== sort-based aggregation =====================
-
-__global__ void k_run_heads(const uint32_t* __restrict__ k, uint64_t n, uint32_t* __restrict__ head) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        head[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
-}
-
-// gid = exclusive scan of heads: row i belongs to group gid[i] + head[i] - 1.
-// Consecutive lanes of a wave hold consecutive rows, so each wave first
-// reduces its runs with a segmented scan and only the last lane of each run
-// adds to HBM.
-__global__ __launch_bounds__(256) void k_run_reduce(const uint32_t* __restrict__ k, const int64_t* __restrict__ v,
-                                                    uint64_t n, const uint32_t* __restrict__ head,
-                                                    const uint32_t* __restrict__ gid, uint32_t* __restrict__ okey,
-                                                    int64_t* __restrict__ osum, int64_t* __restrict__ ocnt) {
-    const uint32_t l = pd::lane();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x; i0 < n; i0 += stride) {
-        const uint64_t i = i0 + threadIdx.x;
-        const bool in = i < n;
-        const uint32_t g = in ? gid[i] + head[i] - 1 : 0xFFFFFFFFu;
-        int64_t s = in ? v[i] : 0;
-        uint64_t c = in ? 1 : 0;
-        if (in && head[i]) okey[g] = k[i];
-        // segmented inclusive scan over equal g
-#pragma unroll
-        for (int d = 1; d < PD_WAVE; d <<= 1) {
-            const int64_t ts = __shfl_up(s, d, PD_WAVE);
-            const uint64_t tc = __shfl_up(c, d, PD_WAVE);
-            const uint32_t tg = __shfl_up(g, d, PD_WAVE);
-            if (l >= (uint32_t)d && tg == g) { s += ts; c += tc; }
-        }
-        const uint32_t ng = __shfl_down(g, 1, PD_WAVE);
-        const bool last = in && (l == PD_WAVE - 1 || ng != g || i + 1 >= n);
-        if (last) {
-            atomicAdd((unsigned long long*)&osum[g], (unsigned long long)s);
-            atomicAdd((unsigned long long*)&ocnt[g], (unsigned long long)c);
-        }
-    }
-}
-
+// Sort aggregation: groups in ascending key order.  With few groups the
+// hash tables' groups are ordered by key (sorting the rows would only add
+// passes); otherwise the rows are radix sorted by key and reduced in one
+// pass over the sorted runs.
 uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
                    int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
     if (n == 0) return 0;
-    Scratch sk(c, n * 4, s), sv(c, n * 8, s), head(c, n * 4, s), gid(c, (n + 1) * 4, s), tot(c, 8, s);
+    PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "sort aggregation supports up to 2^30-1 rows per call");
+    const int64_t hint = c->tune_get("gb_groups_hint", 0);
+    uint64_t g = 0;
+    if (c->tune_get("aggr_small", 1) && hint <= 4096 &&
+        aggr_sum_small(c, key, val, n, out_key, out_sum, out_cnt, cap, true, &g, s)) {
+        PD_HIP(hipStreamSynchronize(s));
+        return g;
+    }
+    Scratch sk(c, n * 4, s), sv(c, n * 8, s);
     sort_pairs(c, key, val, 8, n, sk.as<uint32_t>(), sv.get(), -1, -1, s);
-    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), (uint64_t)c->cu_count * 16));
-    launch(c, "agg_heads", k_run_heads, dim3(g), dim3(256), 0, s, (const uint32_t*)sk.as<uint32_t>(), n, head.as<uint32_t>());
-    exclusive_scan_u32(c, head.as<uint32_t>(), gid.as<uint32_t>(), n, tot.as<uint64_t>(), s);
-    const uint64_t groups = read_u64(c, tot.get(), s);
-    PD_REQUIRE(groups <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-    fill_u64(c, out_sum, 0, groups, s);
-    fill_u64(c, out_cnt, 0, groups, s);
-    launch(c, "agg_run_reduce", k_run_reduce, dim3(g), dim3(256), 0, s, (const uint32_t*)sk.as<uint32_t>(),
-           (const int64_t*)sv.as<int64_t>(), n, (const uint32_t*)head.as<uint32_t>(), (const uint32_t*)gid.as<uint32_t>(),
-           out_key, out_sum, out_cnt);
+    g = aggr_sum_sorted(c, sk.as<uint32_t>(), sv.as<int64_t>(), n, out_key, out_sum, out_cnt, cap, s);
     PD_HIP(hipStreamSynchronize(s));
-    return groups;
+    return g;
 }
 
 }  // namespace pd

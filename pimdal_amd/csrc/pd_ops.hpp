@@ -40,6 +40,16 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
 uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
                    int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s);
 
+// SUM and COUNT per uint32 key (aggr_sum.hip).  aggr_sum_small: the
+// few-groups path, false (outputs untouched) when the groups do not fit its
+// tables; ordered = groups in ascending key order.  aggr_sum_sorted: rows
+// already sorted by key, groups in key order; both throw PIMDAL_E_CAPACITY
+// past cap groups.
+bool aggr_sum_small(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey, int64_t* osum,
+                    int64_t* ocnt, uint64_t cap, bool ordered, uint64_t* groups, hipStream_t s);
+uint64_t aggr_sum_sorted(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey,
+                         int64_t* osum, int64_t* ocnt, uint64_t cap, hipStream_t s);
+
 uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1, const uint8_t* f2,
                            const int32_t* key, const int64_t* qty, const int64_t* price, const int64_t* disc,
                            const int64_t* tax, uint64_t n, int32_t cutoff, const pimdal_q1_result& out,

@@ -27,11 +27,24 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t* __restri
     const uint64_t base = tile * SC_TILE + (uint64_t)t * SC_ITEMS;
     uint32_t v[SC_ITEMS];
     uint64_t sum = 0;
+    // 16-byte accesses where the thread's 16 elements are whole and aligned
+    const bool vec = base + SC_ITEMS <= n && ((uintptr_t)(in + base) % 16) == 0 && ((uintptr_t)(out + base) % 16) == 0;
+    if (vec) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
 #pragma unroll
-    for (int i = 0; i < SC_ITEMS; i++) {
-        v[i] = (base + i < n) ? in[base + i] : 0u;
-        sum += v[i];
+        for (int i = 0; i < SC_ITEMS / 4; i++) {
+            const uint4 x = p[i];
+            v[4 * i] = x.x;
+            v[4 * i + 1] = x.y;
+            v[4 * i + 2] = x.z;
+            v[4 * i + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SC_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0u;
     }
+#pragma unroll
+    for (int i = 0; i < SC_ITEMS; i++) sum += v[i];
     const uint64_t inc = pd::wave_incl_scan64(sum);
     if (l == PD_WAVE - 1) s_wsum[w] = inc;
     __syncthreads();
@@ -50,10 +63,20 @@ __global__ __launch_bounds__(SC_BLOCK) void k_scan_excl(const uint32_t* __restri
     }
     __syncthreads();
     uint64_t run = s_excl + wpre + inc - sum;
+    uint32_t o[SC_ITEMS];
 #pragma unroll
     for (int i = 0; i < SC_ITEMS; i++) {
-        if (base + i < n) out[base + i] = (uint32_t)run;
+        o[i] = (uint32_t)run;
         run += v[i];
+    }
+    if (vec) {
+        uint4* q = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int i = 0; i < SC_ITEMS / 4; i++) q[i] = make_uint4(o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < SC_ITEMS; i++)
+            if (base + i < n) out[base + i] = o[i];
     }
     if (tile == ntiles - 1 && t == SC_BLOCK - 1) out[n] = (uint32_t)(s_excl + tot);
 }

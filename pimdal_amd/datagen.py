@@ -14,7 +14,8 @@ from .device import Context, DeviceArray
 COL = dict(KEY=1, PAYLOAD=2, BUILD_PERM=3, PROBE_FK=4, BUILD_PAY=5, PROBE_PAY=6, ZIPF=7, QTY=12, PRICE=13,
            DISC=14, TAX=15, DATE=16, FLAG1=17, FLAG2=18)
 
-SEEDS = {"select": 0x5EED0001, "sort": 0x5EED0002, "join": 0x5EED0003, "groupby": 0x5EED0004, "serve": 0x5EED0005}
+SEEDS = {"select": 0x5EED0001, "sort": 0x5EED0002, "join": 0x5EED0003, "groupby": 0x5EED0004, "serve": 0x5EED0005,
+         "aggr": 0x5EED0007}
 
 SELECT_THRESHOLD = 429496730  # ceil(0.1 * 2^32)
 
@@ -85,6 +86,15 @@ def join_workload(ctx: Context, nr: int, ns: int, zipf: bool = False, seed: int 
         sk = rng_range(ctx, np.uint32, seed, COL["PROBE_FK"], ns, 0, nr - 1)
     sp = u32(ctx, seed, COL["PROBE_PAY"], ns)
     return rk, rp, sk, sp
+
+
+def aggr_workload(ctx: Context, n: int, lo: int, hi: int, seed: int = SEEDS["aggr"]):
+    """Group-by input: uint32 key uniform on [lo, hi], int64 value on [1, 10]
+    (reference/db_operators/aggregate.cpp:33-36 draws keys on [1, 50] and
+    values on [1, 10]; the PIM microbenchmarks use keys on [1, 4000] and
+    [1, 25])."""
+    return (rng_range(ctx, np.uint32, seed, COL["KEY"], n, lo, hi),
+            rng_range(ctx, np.int64, seed, COL["PAYLOAD"], n, 1, 10))
 
 
 def fact_table(ctx: Context, n: int, seed: int = SEEDS["groupby"], groups: int | None = None):

@@ -45,6 +45,127 @@ def test_sort_aggr_zipf_skew(ctx, orc):
         assert all(np.array_equal(a, b) for a, b in zip(got, exp))
 
 
+AGGR_SHAPES = {"k50": (1, 50), "k4000": (1, 4000), "k25": (1, 25), "k2^24": (0, (1 << 24) - 1)}
+
+
+@pytest.mark.parametrize("shape", sorted(AGGR_SHAPES))
+def test_aggr_bench_workloads_match_oracle(ctx, orc, shape):
+    """The benchmark's group-by inputs, generated on the device, equal the
+    host recipe; hash (canonical order) and sort (its own order) outputs
+    equal the oracle's at 2^22 rows."""
+    lo, hi = AGGR_SHAPES[shape]
+    n = 1 << 22
+    dk, dv = datagen.aggr_workload(ctx, n, lo, hi)
+    k, v = orc.aggr_workload(n, lo, hi)
+    assert np.array_equal(dk.numpy(), k) and np.array_equal(dv.numpy(), v)
+    exp = orc.sort_aggr(k, v)
+    for fn in (ops.hash_aggr, ops.sort_aggr):
+        ok, os_, oc, g = fn(ctx, dk, dv)
+        got = (ok.numpy(g), os_.numpy(g), oc.numpy(g))
+        if fn is ops.hash_aggr:
+            got = canon(*got)
+        assert all(np.array_equal(a, b) for a, b in zip(got, exp)), fn.__name__
+
+
+@pytest.mark.parametrize("shape", ["k50", "k2^24"])
+def test_aggr_2p27_hash_and_sort_agree(ctx, shape):
+    """At the benchmark's 2^27 rows (too slow for the oracle with 2^24 keys):
+    hash and sort outputs hold the same groups, sort's in strictly ascending
+    key order, counts summing to the row count and sums to the value total."""
+    lo, hi = AGGR_SHAPES[shape]
+    n = 1 << 27
+    dk, dv = datagen.aggr_workload(ctx, n, lo, hi)
+    total = int(dv.numpy().sum())
+    hk, hs, hc, hg = ops.hash_aggr(ctx, dk, dv)
+    sk, ss, sc, sg = ops.sort_aggr(ctx, dk, dv)
+    assert hg == sg
+    s = (sk.numpy(sg), ss.numpy(sg), sc.numpy(sg))
+    assert np.all(np.diff(s[0].astype(np.int64)) > 0)
+    assert int(s[2].sum()) == n and int(s[1].sum()) == total
+    h = canon(hk.numpy(hg), hs.numpy(hg), hc.numpy(hg))
+    assert all(np.array_equal(a, b) for a, b in zip(h, s))
+    for a in (dk, dv, hk, hs, hc, sk, ss, sc):
+        a.free()
+
+
+def _both_paths(ctx, orc, k, v, small_fits=None):
+    """hash and sort aggregation, with and without the few-groups LDS path,
+    against the oracle (sort output in its own order, hash canonical)."""
+    exp = orc.sort_aggr(k, v)
+    dk, dv = ctx.to_device(k), ctx.to_device(v)
+    try:
+        for small in (1, 0):
+            ctx.set_tune("aggr_small", small)
+            for fn in (ops.hash_aggr, ops.sort_aggr):
+                ok, os_, oc, g = fn(ctx, dk, dv)
+                got = (ok.numpy(g), os_.numpy(g), oc.numpy(g))
+                if fn is ops.hash_aggr:
+                    got = canon(*got)
+                assert g == exp[0].size, (fn.__name__, small)
+                assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (fn.__name__, small)
+    finally:
+        ctx.clear_tune("aggr_small")
+
+
+def test_aggr_paths_max_key_and_ragged(ctx, orc):
+    """Key 0xFFFFFFFF (the LDS table's empty marker) and 0 among 1000 keys,
+    row counts around the 8192-row round and tile."""
+    for n in (1, 2, 8191, 8192, 8193, 1_000_003):
+        k = orc.gen_range(np.uint32, 31, 1, n, 0, 999)
+        k[::7] = 0xFFFFFFFF
+        k[3::11] = 0
+        v = orc.gen_range(np.int64, 31, 2, n, -(1 << 40), 1 << 40)
+        _both_paths(ctx, orc, k, v)
+
+
+def test_aggr_paths_block_table_limit(ctx, orc):
+    """One block's rows (8192 rows: one block) with 4096 distinct keys fit its
+    table; 4097 do not and the many-groups path takes over."""
+    for distinct in (4096, 4097):
+        k = (np.arange(8192, dtype=np.uint64) % distinct).astype(np.uint32) * np.uint32(2654435761)
+        v = orc.gen_range(np.int64, 32, 2, 8192, 1, 10)
+        _both_paths(ctx, orc, k, v)
+
+
+def test_aggr_paths_hbm_table_limit(ctx, orc):
+    """Sorted keys: every block meets a few hundred keys, all blocks together
+    2^16 (past the HBM table's 32768): the many-groups path takes over; and
+    32768 keys exactly, which fit."""
+    for distinct in (1 << 16, 1 << 15):
+        n = 1 << 21
+        k = np.repeat(np.arange(distinct, dtype=np.uint32) * np.uint32(7919), n // distinct)
+        v = orc.gen_range(np.int64, 33, 2, n, -5, 5)
+        _both_paths(ctx, orc, k, v)
+
+
+def test_sort_aggr_runs_across_tiles(ctx, orc):
+    """Sorted-run reduce with runs spanning thousands of 8192-row tiles (few
+    keys, the LDS path off), Zipf runs, and a run ending on a tile edge."""
+    n = 1 << 22
+    for k in (orc.gen_range(np.uint32, 34, 1, n, 1, 25),
+              orc.sort_workload(n, zipf=True, distinct=1 << 16)[0],
+              np.repeat(np.arange(512, dtype=np.uint32), n // 512)):
+        v = orc.gen_range(np.int64, 34, 2, n, -(1 << 50), 1 << 50)
+        _both_paths(ctx, orc, k, v)
+
+
+def test_aggr_capacity_short(ctx, orc):
+    from pimdal_amd import _lib
+    k = orc.gen_range(np.uint32, 35, 1, 100_000, 0, 99)
+    v = orc.gen_range(np.int64, 35, 2, 100_000, 1, 10)
+    dk, dv = ctx.to_device(k), ctx.to_device(v)
+    try:
+        for small in (1, 0):
+            ctx.set_tune("aggr_small", small)
+            for fn in (ops.hash_aggr, ops.sort_aggr):
+                with pytest.raises(_lib.PimdalError, match="capacity"):
+                    fn(ctx, dk, dv, capacity=99)
+                ok, os_, oc, g = fn(ctx, dk, dv, capacity=100)
+                assert g == 100
+    finally:
+        ctx.clear_tune("aggr_small")
+
+
 def _cmp_q1(got, exp, ordered):
     keys = list(got["key"])
     if not ordered:
