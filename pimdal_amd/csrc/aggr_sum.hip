@@ -40,10 +40,15 @@ constexpr uint32_t AS_TS = 8192;             // LDS table slots
 constexpr uint32_t AS_LIMIT = AS_TS / 2;     // distinct keys a block may meet
 constexpr uint32_t AS_GT = 65536;            // HBM table slots
 constexpr uint32_t AS_GLIMIT = AS_GT / 2;    // distinct keys in all
+constexpr uint32_t AS_R = 8;                 // HBM table copies
 constexpr uint32_t AS_EMPTY = 0xFFFFFFFFu;
 
-// meta words: [0] gave up, [1] keys in the HBM table, [2] rows with key
-// 0xFFFFFFFF, [3] emit counter.
+// HBM tables: AS_R copies (block b adds into copy b % AS_R, so each key's
+// words take a few hundred device-scope atomic adds spread over 8 addresses
+// instead of on one), folded into copy 0 before the groups are emitted.
+// Copy r: gkey + r * AS_GT, gsum + r * (AS_GT + 1) (word AS_GT: the sum of
+// key 0xFFFFFFFF), gcnt + r * AS_GT.  meta words: [0] gave up, [2] rows with
+// key 0xFFFFFFFF, [3] emit counter, [4 + r] keys in copy r.
 
 __device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -96,6 +101,33 @@ __device__ uint32_t as_slot_slow(uint32_t* s_key, uint32_t* s_used, uint32_t* s_
     }
     lds_flag(s_over);
     return AS_TS + 1;
+}
+
+// Adds (sum, cnt) to key kk's entry of one HBM table copy, inserting it when
+// absent.  False when the copy is past AS_GLIMIT keys or full.
+__device__ bool as_ginsert(uint32_t* gkey, uint64_t* gsum, uint64_t* gcnt, uint64_t* nkeys, uint32_t kk, uint64_t sum,
+                           uint64_t cnt) {
+    uint32_t h = slot_hash(kk) & (AS_GT - 1);
+    for (uint32_t probes = 0; probes < AS_GT; probes++) {
+        const uint32_t cur = ld_relaxed(&gkey[h]);
+        bool here = cur == kk;
+        if (!here && cur == AS_EMPTY) {
+            const uint32_t prev = atomicCAS(&gkey[h], AS_EMPTY, kk);
+            if (prev == AS_EMPTY) {
+                if (atomicAdd((unsigned long long*)nkeys, 1ull) >= AS_GLIMIT) return false;
+                here = true;
+            } else {
+                here = prev == kk;
+            }
+        }
+        if (here) {
+            atomicAdd((unsigned long long*)&gsum[h], (unsigned long long)sum);
+            atomicAdd((unsigned long long*)&gcnt[h], (unsigned long long)cnt);
+            return true;
+        }
+        h = (h + 1) & (AS_GT - 1);
+    }
+    return false;
 }
 
 __global__ __launch_bounds__(AS_BLOCK) void k_as_table(const uint32_t* __restrict__ key,
@@ -178,39 +210,37 @@ __global__ __launch_bounds__(AS_BLOCK) void k_as_table(const uint32_t* __restric
         return;
     }
     if (ld_relaxed(&meta[0])) return;   // another block gave up: this result is not used
+    const uint32_t r = blockIdx.x % AS_R;
+    uint32_t* rkey = gkey + (uint64_t)r * AS_GT;
+    uint64_t* rsum = gsum + (uint64_t)r * (AS_GT + 1);
+    uint64_t* rcnt = gcnt + (uint64_t)r * AS_GT;
     for (uint32_t i = t; i < AS_TS; i += AS_BLOCK) {
         const uint32_t kk = s_key[i];
         if (kk == AS_EMPTY) continue;
-        uint32_t h = slot_hash(kk) & (AS_GT - 1);
-        uint32_t probes = 0;
-        bool ok = true;
-        while (true) {
-            const uint32_t cur = ld_relaxed(&gkey[h]);
-            if (cur == kk) break;
-            if (cur == AS_EMPTY) {
-                const uint32_t prev = atomicCAS(&gkey[h], AS_EMPTY, kk);
-                if (prev == AS_EMPTY) {
-                    if (atomicAdd((unsigned long long*)&meta[1], 1ull) >= AS_GLIMIT) st_relaxed(&meta[0], (uint64_t)1);
-                    break;
-                }
-                if (prev == kk) break;
-            }
-            h = (h + 1) & (AS_GT - 1);
-            if (++probes >= AS_GT) {
-                ok = false;
-                break;
-            }
-        }
-        if (!ok) {
-            st_relaxed(&meta[0], (uint64_t)1);
-            continue;
-        }
-        atomicAdd((unsigned long long*)&gsum[h], (unsigned long long)s_sum[i]);
-        atomicAdd((unsigned long long*)&gcnt[h], (unsigned long long)s_cnt[i]);
+        if (!as_ginsert(rkey, rsum, rcnt, &meta[4 + r], kk, s_sum[i], s_cnt[i])) st_relaxed(&meta[0], (uint64_t)1);
     }
     if (t == 0 && s_cnt[AS_TS]) {
-        atomicAdd((unsigned long long*)&gsum[AS_GT], (unsigned long long)s_sum[AS_TS]);
+        atomicAdd((unsigned long long*)&rsum[AS_GT], (unsigned long long)s_sum[AS_TS]);
         atomicAdd((unsigned long long*)&meta[2], (unsigned long long)s_cnt[AS_TS]);
+    }
+}
+
+// Folds table copies 1 .. AS_R - 1 into copy 0.
+__global__ __launch_bounds__(256) void k_as_fold(uint32_t* __restrict__ gkey, uint64_t* __restrict__ gsum,
+                                                 uint64_t* __restrict__ gcnt, uint64_t* __restrict__ meta) {
+    const uint64_t total = (uint64_t)(AS_R - 1) * AS_GT;
+    for (uint64_t x = blockIdx.x * 256ull + threadIdx.x; x < total; x += (uint64_t)gridDim.x * 256) {
+        const uint32_t r = 1 + (uint32_t)(x / AS_GT), sl = (uint32_t)(x % AS_GT);
+        const uint32_t kk = gkey[(uint64_t)r * AS_GT + sl];
+        if (kk == AS_EMPTY) continue;
+        if (!as_ginsert(gkey, gsum, gcnt, &meta[4], kk, gsum[(uint64_t)r * (AS_GT + 1) + sl],
+                        gcnt[(uint64_t)r * AS_GT + sl]))
+            st_relaxed(&meta[0], (uint64_t)1);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint64_t sp = 0;
+        for (uint32_t r = 1; r < AS_R; r++) sp += gsum[(uint64_t)r * (AS_GT + 1) + AS_GT];
+        atomicAdd((unsigned long long*)&gsum[AS_GT], (unsigned long long)sp);
     }
 }
 
@@ -260,22 +290,26 @@ __global__ __launch_bounds__(256) void k_as_gather(uint64_t g, const uint32_t* _
 
 bool aggr_sum_small(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey, int64_t* osum,
                     int64_t* ocnt, uint64_t cap, bool ordered, uint64_t* groups, hipStream_t s) {
-    // gsum[AS_GT + 1] | gcnt[AS_GT] | meta[4] | gkey[AS_GT]
-    const uint64_t words = (uint64_t)AS_GT + 1 + AS_GT + 4;
-    Scratch tab(c, words * 8 + (uint64_t)AS_GT * 4, s);
+    // gsum[AS_R][AS_GT + 1] | gcnt[AS_R][AS_GT] | meta[4 + AS_R] | gkey[AS_R][AS_GT]
+    const uint64_t words = (uint64_t)AS_R * (AS_GT + 1) + (uint64_t)AS_R * AS_GT + 4 + AS_R;
+    Scratch tab(c, words * 8 + (uint64_t)AS_R * AS_GT * 4, s);
     uint64_t* gsum = tab.as<uint64_t>();
-    uint64_t* gcnt = gsum + AS_GT + 1;
-    uint64_t* meta = gcnt + AS_GT;
-    uint32_t* gkey = reinterpret_cast<uint32_t*>(meta + 4);
+    uint64_t* gcnt = gsum + (uint64_t)AS_R * (AS_GT + 1);
+    uint64_t* meta = gcnt + (uint64_t)AS_R * AS_GT;
+    uint32_t* gkey = reinterpret_cast<uint32_t*>(meta + 4 + AS_R);
     fill_u64(c, gsum, 0, words, s);
-    fill_u32(c, gkey, AS_EMPTY, AS_GT, s);
+    fill_u32(c, gkey, AS_EMPTY, (uint64_t)AS_R * AS_GT, s);
     const uint32_t G = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>(ceil_div(n, (uint64_t)AS_BLOCK * AS_U), (uint64_t)c->cu_count));
     const uint64_t chunk = ceil_div(n, G);
     launch(c, "as_table", k_as_table, dim3(G), dim3(AS_BLOCK), 0, s, key, val, n, chunk, gkey, gsum, gcnt, meta);
-    uint64_t m[4];
-    read_small(c, m, meta, 32, s);
+    if (G > 1)
+        launch(c, "as_fold", k_as_fold, dim3(std::min<uint32_t>(G - 1, AS_R - 1) * 64), dim3(256), 0, s, gkey, gsum,
+               gcnt, meta);
+    uint64_t m[5];
+    read_small(c, m, meta, 40, s);
     if (m[0]) return false;
+    m[1] = m[4];   // keys in copy 0
     const uint64_t g = m[1] + (m[2] ? 1 : 0);
     PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     if (!ordered) {
@@ -309,8 +343,6 @@ bool aggr_sum_small(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint
 // ---------------- sorted rows: one reduce pass ----------------
 
 constexpr uint32_t AR_BLOCK = 1024;
-constexpr uint32_t AR_ITEMS = 8;
-constexpr uint32_t AR_TILE = AR_BLOCK * AR_ITEMS;
 
 // Segmented (sum, count) with a flag: a run head lies in the covered rows.
 struct Seg {
@@ -324,15 +356,40 @@ __device__ __forceinline__ Seg seg_join(const Seg& x, const Seg& y) {
     return y.f ? y : Seg{x.f, x.c + y.c, x.s + y.s};
 }
 
-__global__ __launch_bounds__(AR_BLOCK) void k_as_runs(const uint32_t* __restrict__ key, const int64_t* __restrict__ val,
-                                                      uint64_t n, uint64_t* __restrict__ status,
-                                                      uint32_t* __restrict__ tile_ctr, uint64_t ntiles,
-                                                      uint32_t* __restrict__ okey, int64_t* __restrict__ osum,
-                                                      int64_t* __restrict__ ocnt, uint64_t cap,
-                                                      int64_t* __restrict__ cont, uint64_t* __restrict__ total) {
+// Inclusive segmented scan over the wave's lanes.
+__device__ __forceinline__ Seg seg_wave_incl(Seg x) {
+    const uint32_t l = lane();
+#pragma unroll
+    for (int d = 1; d < PD_WAVE; d <<= 1) {
+        const uint32_t of = __shfl_up(x.f, d, PD_WAVE);
+        const uint32_t oc = __shfl_up(x.c, d, PD_WAVE);
+        const int64_t os = __shfl_up(x.s, d, PD_WAVE);
+        if (l >= (uint32_t)d) {
+            if (!x.f) {
+                x.c += oc;
+                x.s += os;
+            }
+            x.f |= of;
+        }
+    }
+    return x;
+}
+
+// AR_ITEMS rows per thread; MINW waves per SIMD asked of the register
+// allocator (8: two blocks per CU).
+template <uint32_t AR_ITEMS, int MINW>
+__global__ __launch_bounds__(AR_BLOCK, MINW) void k_as_runs(const uint32_t* __restrict__ key,
+                                                         const int64_t* __restrict__ val, uint64_t n,
+                                                         uint64_t* __restrict__ status, uint32_t* __restrict__ tile_ctr,
+                                                         uint64_t ntiles, uint32_t* __restrict__ okey,
+                                                         int64_t* __restrict__ osum, int64_t* __restrict__ ocnt,
+                                                         uint64_t cap, int64_t* __restrict__ cont,
+                                                         uint64_t* __restrict__ total) {
+    constexpr uint32_t NW = AR_BLOCK / PD_WAVE;
+    constexpr uint32_t AR_TILE = AR_BLOCK * AR_ITEMS;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wh[AR_BLOCK / PD_WAVE];    // heads per wave
-    __shared__ Seg s_ws[AR_BLOCK / PD_WAVE];         // segmented total per wave
+    __shared__ uint32_t s_wh[NW];    // heads per wave, then heads before the wave
+    __shared__ Seg s_ws[NW];         // segmented total per wave, then the carry into the wave
     __shared__ uint64_t s_gbase;
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = lane();
     if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
@@ -342,11 +399,16 @@ __global__ __launch_bounds__(AR_BLOCK) void k_as_runs(const uint32_t* __restrict
     uint32_t k[AR_ITEMS];
     int64_t v[AR_ITEMS];
     if (base + AR_ITEMS <= n) {
-        // 16-byte loads (key 32-byte and value 64-byte aligned per thread)
+        // 16-byte loads (a thread's rows start 16-byte aligned)
         const uint4* kp = reinterpret_cast<const uint4*>(key + base);
-        const uint4 k0 = kp[0], k1 = kp[1];
-        k[0] = k0.x; k[1] = k0.y; k[2] = k0.z; k[3] = k0.w;
-        k[4] = k1.x; k[5] = k1.y; k[6] = k1.z; k[7] = k1.w;
+#pragma unroll
+        for (uint32_t j = 0; j < AR_ITEMS / 4; j++) {
+            const uint4 x = kp[j];
+            k[4 * j] = x.x;
+            k[4 * j + 1] = x.y;
+            k[4 * j + 2] = x.z;
+            k[4 * j + 3] = x.w;
+        }
         const longlong2* vp = reinterpret_cast<const longlong2*>(val + base);
 #pragma unroll
         for (uint32_t j = 0; j < AR_ITEMS / 2; j++) {
@@ -379,49 +441,40 @@ __global__ __launch_bounds__(AR_BLOCK) void k_as_runs(const uint32_t* __restrict
         if (hmask >> j & 1) a = Seg{1, 1, v[j]};
         else a = Seg{a.f, a.c + 1, a.s + v[j]};
     }
-    // heads before this thread (block scan) and the tile's head total
     const uint32_t hc = (uint32_t)__popc(hmask);
     const uint32_t hinc = wave_incl_scan(hc);
-    // segmented inclusive scan over the wave
-    Seg inc = a;
-#pragma unroll
-    for (int d = 1; d < PD_WAVE; d <<= 1) {
-        const uint32_t of = __shfl_up(inc.f, d, PD_WAVE);
-        const uint32_t oc = __shfl_up(inc.c, d, PD_WAVE);
-        const int64_t os = __shfl_up(inc.s, d, PD_WAVE);
-        if (l >= (uint32_t)d && !inc.f) {
-            inc.c += oc;
-            inc.s += os;
-        }
-        if (l >= (uint32_t)d) inc.f |= of;
-    }
+    const Seg inc = seg_wave_incl(a);
     if (l == PD_WAVE - 1) {
         s_wh[w] = hinc;
         s_ws[w] = inc;
     }
     __syncthreads();
-    uint32_t hpre = hinc - hc, htot = 0;
-    Seg carry{0, 0, 0};
-#pragma unroll
-    for (uint32_t i = 0; i < AR_BLOCK / PD_WAVE; i++) {
-        if (i < w) {
-            hpre += s_wh[i];
-            carry = seg_join(carry, s_ws[i]);
-        }
-        htot += s_wh[i];
-    }
     if (w == 0) {
+        // wave 0: what comes before each wave, the tile's head total, the
+        // look-back over earlier tiles
+        const uint32_t wh = l < NW ? s_wh[l] : 0u;
+        const Seg ws = l < NW ? s_ws[l] : Seg{0, 0, 0};
+        const uint32_t whi = wave_incl_scan(wh);
+        const Seg wsi = seg_wave_incl(ws);
+        const uint32_t htot = __shfl(whi, NW - 1, PD_WAVE);
+        Seg wse{__shfl_up(wsi.f, 1, PD_WAVE), __shfl_up(wsi.c, 1, PD_WAVE), __shfl_up(wsi.s, 1, PD_WAVE)};
+        if (l == 0) wse = Seg{0, 0, 0};
+        if (l < NW) {
+            s_wh[l] = whi - wh;
+            s_ws[l] = wse;
+        }
         const uint64_t e = lookback_single<4>(status, tile, htot);
         if (l == 0) {
             s_gbase = e;
             if (tile == ntiles - 1) *total = e + htot;
         }
     }
-    // exclusive segmented value for this thread
+    // exclusive segmented value for this thread within its wave
     Seg ex{__shfl_up(inc.f, 1, PD_WAVE), __shfl_up(inc.c, 1, PD_WAVE), __shfl_up(inc.s, 1, PD_WAVE)};
     if (l == 0) ex = Seg{0, 0, 0};
-    ex = seg_join(carry, ex);
     __syncthreads();
+    ex = seg_join(s_ws[w], ex);
+    const uint32_t hpre = s_wh[w] + hinc - hc;
     const uint64_t gbase = s_gbase;
     // walk the rows; store at each run end
     uint64_t g = gbase + hpre;   // group number + 1 of the current row
@@ -468,17 +521,27 @@ __global__ __launch_bounds__(256) void k_as_fix(const int64_t* __restrict__ cont
 uint64_t aggr_sum_sorted(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey,
                          int64_t* osum, int64_t* ocnt, uint64_t cap, hipStream_t s) {
     if (n == 0) return 0;
-    PD_REQUIRE((uintptr_t)key % 32 == 0 && (uintptr_t)val % 64 == 0, PIMDAL_E_INTERNAL,
-               "sorted rows must be 32-byte (keys) and 64-byte (values) aligned");
-    const uint64_t ntiles = ceil_div(n, AR_TILE);
+    PD_REQUIRE((uintptr_t)key % 16 == 0 && (uintptr_t)val % 16 == 0, PIMDAL_E_INTERNAL,
+               "sorted rows must be 16-byte aligned");
+    const int64_t items = c->tune_get("as_runs_items", 8);
+    PD_REQUIRE(items == 4 || items == 8 || items == 16, PIMDAL_E_INVALID, "as_runs_items must be 4, 8 or 16");
+    const uint64_t ntiles = ceil_div(n, (uint64_t)AR_BLOCK * items);
     // status[ntiles] | ticket | total | cont[3 * ntiles]
     Scratch st(c, 8 * (ntiles + 2 + 3 * ntiles), s);
     uint64_t* status = st.as<uint64_t>();
     uint64_t* total = status + ntiles + 1;
     int64_t* cont = reinterpret_cast<int64_t*>(total + 1);
     fill_u64(c, status, 0, ntiles + 2 + 3 * ntiles, s);
-    launch(c, "as_runs", k_as_runs, dim3((uint32_t)ntiles), dim3(AR_BLOCK), 0, s, key, val, n, status,
-           reinterpret_cast<uint32_t*>(status + ntiles), ntiles, okey, osum, ocnt, cap, cont, total);
+    uint32_t* ticket = reinterpret_cast<uint32_t*>(status + ntiles);
+    if (items == 4)
+        launch(c, "as_runs", k_as_runs<4, 8>, dim3((uint32_t)ntiles), dim3(AR_BLOCK), 0, s, key, val, n, status,
+               ticket, ntiles, okey, osum, ocnt, cap, cont, total);
+    else if (items == 8)
+        launch(c, "as_runs", k_as_runs<8, 1>, dim3((uint32_t)ntiles), dim3(AR_BLOCK), 0, s, key, val, n, status,
+               ticket, ntiles, okey, osum, ocnt, cap, cont, total);
+    else
+        launch(c, "as_runs", k_as_runs<16, 1>, dim3((uint32_t)ntiles), dim3(AR_BLOCK), 0, s, key, val, n, status,
+               ticket, ntiles, okey, osum, ocnt, cap, cont, total);
     launch(c, "as_fix", k_as_fix, dim3((uint32_t)std::min<uint64_t>(ceil_div(ntiles, 256), 1024)), dim3(256), 0, s,
            (const int64_t*)cont, ntiles, osum, ocnt, cap);
     const uint64_t g = read_u64(c, total, s);

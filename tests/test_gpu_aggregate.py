@@ -138,15 +138,21 @@ def test_aggr_paths_hbm_table_limit(ctx, orc):
         _both_paths(ctx, orc, k, v)
 
 
-def test_sort_aggr_runs_across_tiles(ctx, orc):
-    """Sorted-run reduce with runs spanning thousands of 8192-row tiles (few
-    keys, the LDS path off), Zipf runs, and a run ending on a tile edge."""
-    n = 1 << 22
-    for k in (orc.gen_range(np.uint32, 34, 1, n, 1, 25),
-              orc.sort_workload(n, zipf=True, distinct=1 << 16)[0],
-              np.repeat(np.arange(512, dtype=np.uint32), n // 512)):
-        v = orc.gen_range(np.int64, 34, 2, n, -(1 << 50), 1 << 50)
-        _both_paths(ctx, orc, k, v)
+@pytest.mark.parametrize("items", [4, 8, 16])
+def test_sort_aggr_runs_across_tiles(ctx, orc, items):
+    """Sorted-run reduce (4, 8 or 16 rows per thread: tiles of 4096 to 16384
+    rows) with runs spanning thousands of tiles (few keys, the LDS path off),
+    Zipf runs, runs ending on tile edges, and a ragged last tile."""
+    n = (1 << 22) + 1000
+    ctx.set_tune("as_runs_items", items)
+    try:
+        for k in (orc.gen_range(np.uint32, 34, 1, n, 1, 25),
+                  orc.sort_workload(n, zipf=True, distinct=1 << 16)[0],
+                  np.repeat(np.arange(n // 4096 + 1, dtype=np.uint32), 4096)[:n]):
+            v = orc.gen_range(np.int64, 34, 2, n, -(1 << 50), 1 << 50)
+            _both_paths(ctx, orc, k, v)
+    finally:
+        ctx.clear_tune("as_runs_items")
 
 
 def test_aggr_capacity_short(ctx, orc):
