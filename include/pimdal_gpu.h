@@ -94,6 +94,10 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   hj_global_max_build, hj_probe (1 direct output slots + match bitmap,
  *   0 compacting rounds), hj_block (compacting probe block: 1024 threads x 8
  *   rows or 512 x 16), hj_fanout, hj_rows_per_part, hj_chunk;
+ *   hj_onepass (1: up to 2^24 build rows, one partition pass per side into at
+ *   most 1024 partitions probed through bucketed LDS tables; 0: 8-bit radix
+ *   partition passes), hj_csr_rows (8 or 16 probe rows per thread per
+ *   round), pw_items (16 or 8 rows per thread in the one-pass partition);
  *   smj_merge (1 direct output slots + match bitmap, 0 ticketed merge with
  *   decoupled look-back over per-tile match counts);
  *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,

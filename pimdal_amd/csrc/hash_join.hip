@@ -9,7 +9,18 @@
 // order (compare after a canonical sort, as the reference's own order
 // depends on its thread schedule).
 //
-// Plan on gfx950:
+// Plan on gfx950 (up to 2^24 build rows, knob hj_onepass):
+//  1. one partition pass per side (partition.hip) into F <= 1024 partitions
+//     of about 16384 build rows by the reference hash: per-tile counts, a
+//     scan over tiles, one scatter of (key, payload) rows in partition runs;
+//  2. work items of whole probe partitions (about 131072 rows; hot ones
+//     split);
+//  3. per item (k_hj_csr) the build partition bucketed in LDS by a counting
+//     sort (8192 buckets), every probe row's pair written at its own
+//     position with a match bitmap (packed afterwards unless every row
+//     matched).  Duplicate build keys or a partition past the table fall
+//     back to the general plan.
+// General plan:
 //  1. radix-partition both relations by the reference hash
 //     (hash_join.cpp:20-25) into F partitions with the onesweep passes of
 //     the radix sort run over hash digits (hash_partition): each pass
@@ -387,6 +398,163 @@ __global__ __launch_bounds__(BLOCK, BLOCK == 1024 ? 8 : 2) void k_hj_direct(
     }
 }
 
+// The direct probe for partitions of up to HC_RCAP build rows (one-pass
+// partitioning gives ~16384 rows per partition): the partition's rows are
+// bucketed in LDS by a counting sort (8192 buckets, 16-bit counts packed two
+// to a word, ranks from the atomic adds, an exclusive scan into bucket
+// starts), so the table holds exactly its rows and a probe walks one
+// bucket, about two rows.  Output, bitmap, counts and flags as k_hj_direct.
+constexpr uint32_t HC_RCAP = 16896;
+constexpr uint32_t HC_NB = 8192;
+constexpr int HC_BLOCK = 1024;
+
+__device__ __forceinline__ uint32_t hc_bucket(uint32_t k) { return pd::slot_hash(k) & (HC_NB - 1); }
+
+// HC_ROWS probe rows per thread per round (the next round's rows load
+// while this round probes).
+template <int HC_ROWS>
+__global__ __launch_bounds__(HC_BLOCK) void k_hj_csr(
+    const uint2* __restrict__ R, const uint32_t* __restrict__ HR, const uint2* __restrict__ S,
+    const uint32_t* __restrict__ HS, uint32_t F,
+    uint32_t CH, const uint32_t* __restrict__ wstart, uint32_t* __restrict__ wcount, uint32_t* __restrict__ out_p,
+    uint32_t* __restrict__ out_b, uint64_t* __restrict__ valid, uint32_t vw, uint32_t* __restrict__ flags) {
+    constexpr int WAVES = HC_BLOCK / PD_WAVE;
+    constexpr int BR = (HC_RCAP + HC_BLOCK - 1) / HC_BLOCK;   // build rows per thread
+    __shared__ uint2 tab[HC_RCAP];
+    __shared__ uint32_t bw[HC_NB / 2 + 1];   // 16-bit bucket counts, then starts; start[HC_NB] = rows
+    __shared__ uint32_t s_f, s_dup, s_cnt[WAVES];
+    const uint16_t* bst = reinterpret_cast<const uint16_t*>(bw);
+
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint32_t wi = blockIdx.x;
+    if (wi >= wstart[F]) return;
+    if (t == 0) {
+        uint32_t lo = 0, hi = F;   // partition: last f with wstart[f] <= wi
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (wstart[mid] <= wi) lo = mid; else hi = mid;
+        }
+        s_f = lo;
+        s_dup = 0;
+    }
+    for (uint32_t i = t; i < HC_NB / 2 + 1; i += HC_BLOCK) bw[i] = 0;
+    __syncthreads();
+    const uint32_t f = s_f;
+    const uint32_t ck = wi - wstart[f];
+    const uint64_t r0 = HR[f], r1 = HR[f + 1];
+    const uint64_t s0 = (uint64_t)HS[f] + (uint64_t)ck * CH, s1 = min((uint64_t)HS[f + 1], s0 + CH);
+    if (r1 - r0 > HC_RCAP) {
+        if (t == 0) {
+            atomicOr(&flags[1], 1u);
+            wcount[wi] = 0;
+        }
+        return;
+    }
+    // first round's probe rows, in flight during the build
+    uint2 row[HC_ROWS];
+#pragma unroll
+    for (int k = 0; k < HC_ROWS; k++) {
+        const uint64_t i = s0 + (uint64_t)k * HC_BLOCK + t;
+        row[k] = i < s1 ? S[i] : make_uint2(0u, 0u);
+    }
+    // ---- build: count per bucket (rank = the count before this row) ----
+    uint2 brow[BR];
+    uint32_t brank[BR];
+#pragma unroll
+    for (int k = 0; k < BR; k++) {
+        const uint64_t i = r0 + (uint64_t)k * HC_BLOCK + t;
+        brow[k] = i < r1 ? R[i] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < BR; k++) {
+        if (r0 + (uint64_t)k * HC_BLOCK + t >= r1) break;
+        const uint32_t b = hc_bucket(brow[k].x), sh = (b & 1) << 4;
+        brank[k] = (atomicAdd(&bw[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
+    }
+    __syncthreads();
+    // bucket starts: thread t scans buckets 8t .. 8t + 7 (words 4t .. 4t + 3)
+    {
+        uint32_t cnt[8], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t x = bw[4 * t + q];
+            cnt[2 * q] = x & 0xFFFFu;
+            cnt[2 * q + 1] = x >> 16;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) sum += cnt[q];
+        const uint32_t inc = pd::wave_incl_scan(sum);
+        if (l == PD_WAVE - 1) s_cnt[w] = inc;
+        __syncthreads();
+        uint32_t run = inc - sum;
+        for (int i = 0; i < (int)w; i++) run += s_cnt[i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t a = run, b = run + cnt[2 * q];
+            run = b + cnt[2 * q + 1];
+            bw[4 * t + q] = a | (b << 16);
+        }
+        if (t == HC_BLOCK - 1) bw[HC_NB / 2] = run;   // start[HC_NB] = rows
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BR; k++) {
+        if (r0 + (uint64_t)k * HC_BLOCK + t >= r1) break;
+        tab[bst[hc_bucket(brow[k].x)] + brank[k]] = brow[k];
+    }
+    __syncthreads();
+
+    // ---- probe ----
+    uint32_t matched = 0;
+    bool dup = false;
+    for (uint64_t rb = s0; rb < s1; rb += (uint64_t)HC_BLOCK * HC_ROWS) {
+        uint2 nrow[HC_ROWS];
+        const uint64_t nb = rb + (uint64_t)HC_BLOCK * HC_ROWS;
+#pragma unroll
+        for (int k = 0; k < HC_ROWS; k++) {
+            const uint64_t i = nb + (uint64_t)k * HC_BLOCK + t;
+            nrow[k] = i < s1 ? S[i] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < HC_ROWS; k++) {
+            const uint64_t i = rb + (uint64_t)k * HC_BLOCK + t;
+            uint32_t m = 0, first = 0;
+            if (i < s1) {
+                const uint32_t b = hc_bucket(row[k].x);
+                const uint32_t e = bst[b + 1];
+                for (uint32_t j = bst[b]; j < e; j++) {
+                    const uint2 r = tab[j];
+                    if (r.x == row[k].x) {
+                        if (m == 0) first = r.y;
+                        m++;
+                    }
+                }
+            }
+            dup |= m > 1;
+            if (m) {
+                out_p[i] = row[k].y;
+                out_b[i] = first;
+                matched++;
+            }
+            const uint64_t bb = pd::ballot(m != 0);
+            if (l == 0) valid[(uint64_t)wi * vw + (rb - s0 + (uint64_t)k * HC_BLOCK + (uint64_t)w * PD_WAVE) / PD_WAVE] = bb;
+        }
+#pragma unroll
+        for (int k = 0; k < HC_ROWS; k++) row[k] = nrow[k];
+    }
+    matched = pd::wave_sum(matched);
+    if (l == 0) s_cnt[w] = matched;
+    if (dup) s_dup = 1;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < WAVES; i++) tot += s_cnt[i];
+        wcount[wi] = tot;
+        if (s_dup) atomicOr(&flags[0], 1u);
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(flags + 2), (unsigned long long)tot);
+    }
+}
+
 // Packs the direct probe's output: work item wi's matched rows (bitmap
 // words valid[wi * vw ..]) go, in row order, to woff[wi] onwards.  One block
 // per work item; wave v takes words v, v + 4, ...; lane = bit.
@@ -398,7 +566,7 @@ __global__ __launch_bounds__(256) void k_hj_compact_bits(const uint32_t* __restr
                                                          const uint32_t* __restrict__ in_b, uint32_t* __restrict__ out_p,
                                                          uint32_t* __restrict__ out_b, uint64_t out_cap) {
     __shared__ uint32_t s_f, s_w[4];
-    __shared__ uint32_t s_wo[1024];   // exclusive popcount prefix per word (vw <= 1024)
+    __shared__ uint32_t s_wo[4096];   // exclusive popcount prefix per word (vw <= 4096)
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     const uint32_t wi = blockIdx.x;
     if (wi >= wstart[F]) return;
@@ -416,11 +584,11 @@ __global__ __launch_bounds__(256) void k_hj_compact_bits(const uint32_t* __restr
     const uint64_t s1 = min((uint64_t)HS[f + 1], s0 + CH);
     const uint32_t nw = (uint32_t)((s1 - s0 + PD_WAVE - 1) / PD_WAVE);
     const uint64_t* vb = valid + (uint64_t)wi * vw;
-    // block exclusive scan of the words' popcounts (4 words per thread)
-    uint32_t c[4], sum = 0;
+    // block exclusive scan of the words' popcounts (16 words per thread)
+    uint32_t c[16], sum = 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint32_t j = t * 4 + q;
+    for (int q = 0; q < 16; q++) {
+        const uint32_t j = t * 16 + q;
         c[q] = j < nw ? pd::popc64(vb[j]) : 0u;
         sum += c[q];
     }
@@ -430,8 +598,8 @@ __global__ __launch_bounds__(256) void k_hj_compact_bits(const uint32_t* __restr
     uint32_t run = inc - sum;
     for (uint32_t i = 0; i < w; i++) run += s_w[i];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        s_wo[t * 4 + q] = run;
+    for (int q = 0; q < 16; q++) {
+        s_wo[t * 16 + q] = run;
         run += c[q];
     }
     __syncthreads();
@@ -548,18 +716,94 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         while (CH > 8192 && (uint64_t)CH * 4 * (uint64_t)c->cu_count > np) CH >>= 1;
     }
     // probe (knob hj_probe): 1 = direct output slots + match bitmap (items of
-    // at most 65536 rows: the packing kernel scans 1024 bitmap words per
+    // at most 262144 rows: the packing kernel scans 4096 bitmap words per
     // item), 0 = compacting rounds
     const bool direct_probe = c->tune_get("hj_probe", 1) == 1;
-    if (direct_probe) CH = std::min<uint32_t>(std::max<uint32_t>(CH, 1), 65536);
+    if (direct_probe) CH = std::min<uint32_t>(std::max<uint32_t>(CH, 1), 262144);
     const uint32_t vw = (uint32_t)(ceil_div(CH, 8192) * 128);   // bitmap words per item (whole 8192-row rounds)
     constexpr uint32_t GR = 1, GS = 1;   // partition f = [H[f], H[f + 1])
-    Scratch rk(c, nb * 4, s), rp(c, nb * 4, s), sk(c, np * 4, s), sp(c, np * 4, s);
     Scratch flags(c, 16, s);
     const bool direct = cap >= np && out_p && out_b;
     Scratch tmp_p(c, direct ? 0 : np * 4, s), tmp_b(c, direct ? 0 : np * 4, s);
     uint32_t* p0 = direct ? out_p : tmp_p.as<uint32_t>();
     uint32_t* b0 = direct ? out_b : tmp_b.as<uint32_t>();
+    // the direct probe's pairs sit at their probe rows' positions, a bitmap
+    // per work item marking them: packed unless every probe row matched
+    auto finish_direct = [&](const uint32_t* HSp, uint32_t Fx, uint32_t CHx, const uint32_t* wstp, uint32_t* wcntp,
+                             uint32_t* woffp, const uint64_t* validp, uint32_t vwx, uint64_t nwi,
+                             uint64_t total) -> uint64_t {
+        if (direct && total == np) return total;   // dense as written
+        exclusive_scan_u32(c, wcntp, woffp, nwi, nullptr, s);
+        if (out_p && out_b && cap) {
+            if (direct) {
+                Scratch cp(c, total * 4 + 4, s), cb(c, total * 4 + 4, s);
+                launch(c, "hj_compact", k_hj_compact_bits, dim3((uint32_t)nwi), dim3(256), 0, s, HSp, Fx, CHx, wstp,
+                       (const uint32_t*)woffp, validp, vwx, (const uint32_t*)p0, (const uint32_t*)b0,
+                       cp.as<uint32_t>(), cb.as<uint32_t>(), total);
+                copy_d2d(c, out_p, cp.get(), std::min(total, cap) * 4, s);
+                copy_d2d(c, out_b, cb.get(), std::min(total, cap) * 4, s);
+                PD_HIP(hipStreamSynchronize(s));
+            } else {
+                launch(c, "hj_compact", k_hj_compact_bits, dim3((uint32_t)nwi), dim3(256), 0, s, HSp, Fx, CHx, wstp,
+                       (const uint32_t*)woffp, validp, vwx, (const uint32_t*)p0, (const uint32_t*)b0, out_p, out_b,
+                       cap);
+            }
+        }
+        return total;
+    };
+    // One partition pass per side (partition.hip) into at most 1024
+    // partitions of about 16384 build rows, probed through bucketed LDS
+    // tables (k_hj_csr): knob hj_onepass (default 1; off when hj_fanout or
+    // hj_rows_per_part is set).  Duplicate build keys or a partition past
+    // the table fall through to the general path below.
+    const uint32_t F1 = std::max<uint32_t>(2, next_pow2(ceil_div(nb, 16384)));
+    if (direct_probe && F1 <= 1024 && c->tune_get("hj_onepass", 1) == 1 && c->tune_get("hj_fanout", 0) <= 0 &&
+        c->tune_get("hj_rows_per_part", 0) <= 0) {
+        // probe rows per work item: the average partition plus 15 %, in whole
+        // 8192-row rounds (a typical partition is one item: one table
+        // build), halved while the items would be fewer than 4 per CU
+        uint32_t CH1 = (uint32_t)ch_tuned;
+        if (ch_tuned <= 0) {
+            const uint64_t avg = ceil_div(np, (uint64_t)F1);
+            CH1 = (uint32_t)std::min<uint64_t>(262144, std::max<uint64_t>(8192, ceil_div(avg * 23 / 20, 8192) * 8192));
+            while (CH1 > 8192 && (uint64_t)F1 * ceil_div(avg, (uint64_t)CH1) < 4 * (uint64_t)c->cu_count)
+                CH1 = (uint32_t)std::max<uint64_t>(8192, ceil_div(CH1 / 2, 8192) * 8192);
+        }
+        CH1 = std::min<uint32_t>(std::max<uint32_t>(CH1, 1), 262144);
+        const int64_t hc_rows = c->tune_get("hj_csr_rows", 8);
+        PD_REQUIRE(hc_rows == 8 || hc_rows == 16, PIMDAL_E_INVALID, "hj_csr_rows must be 8 or 16");
+        const uint64_t round1 = (uint64_t)HC_BLOCK * hc_rows;   // probe rows per block round
+        const uint32_t vw1 = (uint32_t)(ceil_div(CH1, round1) * (round1 / 64));
+        const uint32_t bits1 = (uint32_t)__builtin_ctz(F1);
+        Scratch HR(c, ((uint64_t)F1 + 1) * 4, s), HS(c, ((uint64_t)F1 + 1) * 4, s);
+        Scratch R2(c, nb * 8, s), S2(c, np * 8, s);   // partitioned (key, payload) rows
+        hash_partition_1p(c, bk, bp ? bp : bk, nb, bits1, R2.as<uint2>(), HR.as<uint32_t>(), s);
+        hash_partition_1p(c, pk, pp ? pp : pk, np, bits1, S2.as<uint2>(), HS.as<uint32_t>(), s);
+        Scratch wst(c, ((uint64_t)F1 + 1) * 4, s);
+        const uint64_t max_wi = F1 + ceil_div(np, CH1);
+        Scratch wcnt(c, (max_wi + 1) * 4, s), woff(c, (max_wi + 1) * 4, s);
+        launch(c, "hj_chunks", k_hj_plan, dim3(1), dim3(1024), 0, s, (const uint32_t*)HS.as<uint32_t>(), F1, CH1,
+               wst.as<uint32_t>(), flags.as<uint32_t>(), wcnt.as<uint32_t>(), (uint32_t)(max_wi + 1));
+        Scratch valid(c, max_wi * vw1 * 8, s);
+        if (hc_rows == 16)
+            launch(c, "hj_join", k_hj_csr<16>, dim3((uint32_t)max_wi), dim3(HC_BLOCK), 0, s, (const uint2*)R2.as<uint2>(),
+                   (const uint32_t*)HR.as<uint32_t>(), (const uint2*)S2.as<uint2>(), (const uint32_t*)HS.as<uint32_t>(),
+                   F1, CH1, (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), p0, b0, valid.as<uint64_t>(),
+                   vw1, flags.as<uint32_t>());
+        else
+            launch(c, "hj_join", k_hj_csr<8>, dim3((uint32_t)max_wi), dim3(HC_BLOCK), 0, s, (const uint2*)R2.as<uint2>(),
+                   (const uint32_t*)HR.as<uint32_t>(), (const uint2*)S2.as<uint2>(), (const uint32_t*)HS.as<uint32_t>(),
+                   F1, CH1, (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), p0, b0, valid.as<uint64_t>(),
+                   vw1, flags.as<uint32_t>());
+        uint32_t fl[4];
+        read_small(c, fl, flags.get(), 16, s);
+        if (!fl[0] && !fl[1])
+            return finish_direct(HS.as<uint32_t>(), F1, CH1, wst.as<uint32_t>(), wcnt.as<uint32_t>(),
+                                 woff.as<uint32_t>(), valid.as<uint64_t>(), vw1, max_wi,
+                                 (uint64_t)fl[2] | ((uint64_t)fl[3] << 32));
+        fill_u32(c, flags.get(), 0, 4, s);
+    }
+    Scratch rk(c, nb * 4, s), rp(c, nb * 4, s), sk(c, np * 4, s), sp(c, np * 4, s);
     for (int attempt = 0;; attempt++) {
         const uint32_t bits = (uint32_t)__builtin_ctz(F);
         Scratch HR(c, ((uint64_t)F + 1) * 4, s), HS(c, ((uint64_t)F + 1) * 4, s);
@@ -580,29 +824,10 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
                    (const uint32_t*)wst.as<uint32_t>(), wcnt.as<uint32_t>(), p0, b0, valid.as<uint64_t>(), vw,
                    flags.as<uint32_t>());
             read_small(c, fl, flags.get(), 16, s);
-            if (!fl[0] && !fl[1]) {
-                const uint64_t total = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32);
-                if (direct && total == np) return total;  // every probe row matched once: dense as written
-                exclusive_scan_u32(c, wcnt.as<uint32_t>(), woff.as<uint32_t>(), max_wi, nullptr, s);
-                if (out_p && out_b && cap) {
-                    if (direct) {
-                        Scratch cp(c, total * 4 + 4, s), cb(c, total * 4 + 4, s);
-                        launch(c, "hj_compact", k_hj_compact_bits, dim3((uint32_t)max_wi), dim3(256), 0, s,
-                               (const uint32_t*)HS.as<uint32_t>(), F, CH, (const uint32_t*)wst.as<uint32_t>(),
-                               (const uint32_t*)woff.as<uint32_t>(), (const uint64_t*)valid.as<uint64_t>(), vw,
-                               (const uint32_t*)p0, (const uint32_t*)b0, cp.as<uint32_t>(), cb.as<uint32_t>(), total);
-                        copy_d2d(c, out_p, cp.get(), std::min(total, cap) * 4, s);
-                        copy_d2d(c, out_b, cb.get(), std::min(total, cap) * 4, s);
-                        PD_HIP(hipStreamSynchronize(s));
-                    } else {
-                        launch(c, "hj_compact", k_hj_compact_bits, dim3((uint32_t)max_wi), dim3(256), 0, s,
-                               (const uint32_t*)HS.as<uint32_t>(), F, CH, (const uint32_t*)wst.as<uint32_t>(),
-                               (const uint32_t*)woff.as<uint32_t>(), (const uint64_t*)valid.as<uint64_t>(), vw,
-                               (const uint32_t*)p0, (const uint32_t*)b0, out_p, out_b, cap);
-                    }
-                }
-                return total;
-            }
+            if (!fl[0] && !fl[1])
+                return finish_direct(HS.as<uint32_t>(), F, CH, wst.as<uint32_t>(), wcnt.as<uint32_t>(),
+                                     woff.as<uint32_t>(), valid.as<uint64_t>(), vw, max_wi,
+                                     (uint64_t)fl[2] | ((uint64_t)fl[3] << 32));
             // duplicate build keys or an oversized partition: the counting path below
         } else {
 #define PD_HJ_UNIQUE(B, R)                                                                                     \

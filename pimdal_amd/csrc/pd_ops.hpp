@@ -27,6 +27,12 @@ void exclusive_scan_u32(pimdal_ctx* c, const uint32_t* in, uint32_t* out, uint64
 void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t bits,
                     uint32_t* kout, void* vout, uint32_t* bounds, hipStream_t s);
 
+// One pass into 2^bits <= 1024 partitions by ref_hash(key) & (2^bits - 1)
+// (partition.hip): (key, payload) rows out, order inside a partition
+// unspecified; bounds as hash_partition's.
+void hash_partition_1p(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uint64_t n, uint32_t bits, uint2* out,
+                       uint32_t* bounds, hipStream_t s);
+
 uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
                    const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap, hipStream_t s);
 

@@ -17,6 +17,7 @@ def run(ctx, fn, rk, rp, sk, sp):
 HJ_KNOBS = {
     "hash": {"hj_mode": -1},
     "hash_part": {"hj_mode": 0},
+    "hash_part_2pass": {"hj_mode": 0, "hj_onepass": 0},
     "hash_part_rounds": {"hj_mode": 0, "hj_probe": 0},
     "hash_part_rounds_512": {"hj_mode": 0, "hj_probe": 0, "hj_block": 512},
     "hash_global": {"hj_mode": 1},
@@ -26,8 +27,10 @@ HJ_KNOBS = {
 
 def join_fn(ctx, which):
     """hash = mode picked by build size (hj_mode -1); hash_part = radix-
-    partitioned LDS tables (hj_mode 0) with the direct-slot probe; hash_part_
-    rounds(_512) = the same with the compacting probe in 1024- (512-) thread
+    partitioned LDS tables (hj_mode 0) with the direct-slot probe (one
+    partition pass per side into bucketed tables up to 2^24 build rows);
+    hash_part_2pass = the same with the 8-bit radix partition passes and
+    linear-probing tables (hj_onepass 0); hash_part_rounds(_512) = the same with the compacting probe in 1024- (512-) thread
     blocks; hash_global = one global table (hj_mode 1); smj = sort-merge join
     (direct-slot merge); smj_lookback = the merge that compacts through a
     decoupled look-back."""
@@ -48,7 +51,7 @@ def join_fn(ctx, which):
 
 
 @pytest.mark.parametrize("zipf", [False, True])
-@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj",
+@pytest.mark.parametrize("which", ["hash", "hash_part", "hash_part_2pass", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj",
                                    "smj_lookback"])
 def test_join_pk_fk_2_20(ctx, orc, zipf, which):
     rk, rp, sk, sp = orc.join_workload(1 << 17, 1 << 20, zipf=zipf)
@@ -61,7 +64,7 @@ def test_join_pk_fk_2_20(ctx, orc, zipf, which):
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj",
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_2pass", "hash_part_rounds", "hash_part_rounds_512", "hash_global", "smj",
                                    "smj_lookback"])
 @pytest.mark.parametrize("nb,ns,kr,ks", [(1, 1, 1, 1), (3000, 5000, 1000, 2000), (100, 70_000, 60, 200),
                                          (50_000, 3, 10, 10), (4097, 8193, 1 << 31, 1 << 31)])
@@ -78,7 +81,7 @@ def test_join_duplicates_misses_edges(ctx, orc, which, nb, ns, kr, ks):
         assert same_multiset((gp, gb), (ep, eb))
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_part_rounds", "hash_global", "smj", "smj_lookback"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_2pass", "hash_part_rounds", "hash_global", "smj", "smj_lookback"])
 def test_join_unique_build_half_misses(ctx, orc, which):
     """Unique build keys (a permutation of [0, 2^18)) probed by keys uniform
     on [0, 2^19): about half the probe rows match nothing, so the direct
@@ -131,7 +134,7 @@ def _full_size_case(ctx, zipf):
     return _FULL[zipf]
 
 
-@pytest.mark.parametrize("which", ["hash_part", "hash_global", "smj"])
+@pytest.mark.parametrize("which", ["hash_part", "hash_part_2pass", "hash_global", "smj"])
 @pytest.mark.parametrize("zipf", [False, True])
 def test_join_2p24_build_2p27_probe_properties(ctx, zipf, which):
     (rk, rp, sk, sp), exp = _full_size_case(ctx, zipf)
@@ -245,3 +248,41 @@ def test_join_unique_build_above_2p26_rows(ctx):
     for a in (rk, rp, sk, sp, op, ob):
         a.free()
     assert np.array_equal(got, exp)
+
+
+def _ref_hash_low(keys, bits):
+    """ref_hash(key) & (2^bits - 1): low 32 bits of the byte-swapped 64-bit
+    product with 0x9E3779B185EBCA87 (the partition number)."""
+    p = keys.astype(np.uint64) * np.uint64(11400714785074694791)
+    return (p.byteswap() & np.uint64(0xFFFFFFFF)).astype(np.uint64) & np.uint64((1 << bits) - 1)
+
+
+def test_join_one_pass_partition_past_bucketed_table(ctx, orc):
+    """2^20 unique build keys (64 one-pass partitions) of which 17000 hash to
+    partition 5, past the 16896 rows a bucketed table holds: the join falls
+    through to the general path and still matches the oracle."""
+    cand = np.arange(1, 1 << 24, dtype=np.uint32)
+    hot = cand[_ref_hash_low(cand, 6) == 5][:17000]
+    rest = cand[_ref_hash_low(cand, 6) != 5][: (1 << 20) - hot.size]
+    rk = np.concatenate([hot, rest])
+    rk = rk[np.random.default_rng(3).permutation(rk.size)]
+    rp = np.arange(rk.size, dtype=np.uint32)
+    sk = rk[np.random.default_rng(4).integers(0, rk.size, 1 << 22)]
+    sp = np.arange(sk.size, dtype=np.uint32)
+    exp = orc.hash_join(rk, rp, sk, sp)
+    got = run(ctx, join_fn(ctx, "hash_part"), rk, rp, sk, sp)
+    assert same_multiset(got, exp)
+
+
+@pytest.mark.parametrize("nb", [1, 2, 16384, 16385, (1 << 24) + 1])
+def test_join_one_pass_fanout_edges(ctx, orc, nb):
+    """Build sizes at the one-pass partition counts' edges (2 partitions,
+    2 -> 4, and past 1024 partitions, where the radix passes take over)."""
+    rk = orc.gen_perm(71, 3, nb, nb)
+    rp = orc.gen_u32(71, 5, nb)
+    ns = 3 * nb + 7 if nb < (1 << 20) else 1 << 22
+    sk = orc.gen_range(np.uint32, 71, 4, ns, 0, nb + nb // 3)   # a quarter of the probe rows miss
+    sp = orc.gen_u32(71, 6, ns)
+    exp = orc.hash_join(rk, rp, sk, sp)
+    got = run(ctx, join_fn(ctx, "hash_part"), rk, rp, sk, sp)
+    assert same_multiset(got, exp)
