@@ -85,13 +85,20 @@ def bench_sort(ctx, steps, warmup, n=1 << 28, zipf=False):
     hist_ms, _ = kernel_avg_ms(ctx, "radix_hist")
     count_ms, count_launches = kernel_avg_ms(ctx, "radix_count")
     tscan_ms, _ = kernel_avg_ms(ctx, "radix_tscan")
+    bucket_ms, bucket_launches = kernel_avg_ms(ctx, "radix_bucket")
     for a in (keys, vals, ok, ov):
         a.free()
+    # trips through HBM per sort: one per onesweep pass, one for the bucket
+    # pass (which sorts the two low digits of every bucket in LDS)
+    per_step = lambda launches: launches / steps if launches else 0
+    trips = round(per_step(launches) + per_step(bucket_launches))
+    alg = (4 + 16 * trips) * n
     return {
-        "n": n, "s_per_step": t, "rows_per_s": n / t,
-        "alg_bytes": 68 * n, "alg_GBps": 68 * n / t / 1e9, "hbm_frac": 68 * n / t / HBM_PEAK,
+        "n": n, "s_per_step": t, "rows_per_s": n / t, "hbm_trips": trips,
+        "alg_bytes": alg, "alg_GBps": alg / t / 1e9, "hbm_frac": alg / t / HBM_PEAK,
         "keys_payload_GBps": 8 * n / t / 1e9,
         "pass_ms": pass_ms, "pass_launches": launches, "hist_ms": hist_ms,
+        "bucket_ms": bucket_ms, "bucket_launches": bucket_launches,
         "count_ms": count_ms, "count_launches": count_launches, "tscan_ms": tscan_ms,
         "pass_alg_bytes": 16 * n,
     }
@@ -682,10 +689,14 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "stable LSD radix sort of 2^28 (u32 key, u32 payload) pairs, keys uniform on [0,2^32)",
-                   "rows": srt["n"], "passes": 4, "digit_bits": 8,
-                   "alg_bytes_per_row": 68, "sort_alg_GBps": srt["alg_GBps"], "sort_hbm_frac": srt["hbm_frac"],
-                   "hist_ms": srt["hist_ms"], "count_ms": srt["count_ms"], "tscan_ms": srt["tscan_ms"],
+        "config": {"workload": "stable radix sort of 2^28 (u32 key, u32 payload) pairs, keys uniform on [0,2^32)",
+                   "rows": srt["n"], "hbm_trips": srt["hbm_trips"], "digit_bits": 8,
+                   "scheme": ("onesweep passes over the two high digits, then one pass sorting each bucket of "
+                              "equal high bits by the two low digits in LDS") if srt["bucket_launches"]
+                   else "one onesweep pass per digit",
+                   "alg_bytes_per_row": 4 + 16 * srt["hbm_trips"], "sort_alg_GBps": srt["alg_GBps"],
+                   "sort_hbm_frac": srt["hbm_frac"], "hist_ms": srt["hist_ms"], "bucket_ms": srt["bucket_ms"],
+                   "count_ms": srt["count_ms"], "tscan_ms": srt["tscan_ms"],
                    "keys_payload_GBps": srt["keys_payload_GBps"], "gpu_over_cpu": srt.get("gpu_over_cpu")},
         "roofline": {"kernel": "radix_onesweep", "bound": "hbm",
                      "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

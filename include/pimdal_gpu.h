@@ -86,7 +86,12 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   refused, for tests), sort_scheme (0 chained passes with decoupled
  *   look-back, 1 counted passes: per-tile digit counts and a scan, then a
  *   reorder with no look-back), sort_rts_order (tile order of counted
- *   passes: 0 block order, 1 ticket, 2 XCD-contiguous);
+ *   passes: 0 block order, 1 ticket, 2 XCD-contiguous), sort_bucket (1, the
+ *   default: for 4 varying digits whose buckets of the two high digits
+ *   hold 3072 to 6144 rows by a histogram estimate, as 2^28 uniform keys
+ *   do, global passes over the two high digits only, then one pass that
+ *   sorts each bucket by the two low digits in LDS; 2: that scheme for any
+ *   3 or 4 digits whose estimate fits; 0: one global pass per digit);
  *   select_mode (0 one pass with decoupled look-back, 1 count pass + write
  *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
  *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;

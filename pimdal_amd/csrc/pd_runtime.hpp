@@ -88,7 +88,7 @@ struct pimdal_ctx {
     uint64_t pinned_bytes = 0;
     // pinned words for small device-to-host reads (read_small)
     void* readback = nullptr;
-    uint64_t readback_words[8] = {};
+    static constexpr uint64_t READBACK_BYTES = 16384;   // pinned bytes for read_small
     // profiling: hipEvent pairs per kernel name on the stream it ran on
     bool profiling = false;
     struct Pending { std::string name; hipEvent_t e0, e1; };

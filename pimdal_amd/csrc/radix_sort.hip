@@ -22,6 +22,8 @@
 #include "pd_device.hpp"
 #include "pd_ops.hpp"
 
+#include <algorithm>
+
 namespace pd {
 
 constexpr int RS_ITEMS = 16;                   // keys per thread; tile = block * 16
@@ -891,6 +893,337 @@ static void sort_rts(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t 
     run_passes_rts<V, HV>(c, kin, vin, n, kout, vout, run, B, counted, s);
 }
 
+// ---- bucket pass: the last 1-2 low digits sorted in LDS per bucket ----
+// After stable passes over the two highest varying digits (a, then b), rows
+// with equal high bits (bucket = digit a | digit b << 8, 2^16 buckets in key
+// order) are contiguous and keep their input order.  For inputs whose
+// buckets fit a block's LDS (uniform 2^28 keys: about 4096 rows each), one
+// more read and write of the data sorts every bucket by the low digits in
+// LDS, instead of one global pass per low digit: three trips through HBM
+// instead of four for 32-bit keys.  A bucket larger than a block's capacity
+// sets an overflow word; the host then finishes with ordinary passes (the
+// rows with equal keys are still in input order, so stability holds).
+constexpr uint32_t RB_BLOCK = 512;
+constexpr uint32_t RB_ITEMS = 16;
+constexpr uint32_t RB_CAP = RB_BLOCK * RB_ITEMS;   // rows one block sorts at once
+constexpr uint32_t RB_NB = 1u << 16;               // buckets
+
+struct RbDesc {
+    uint32_t nlow;                 // low digits sorted in LDS (1 or 2), lowest first
+    uint32_t lshift[2], lmask[2];
+    uint32_t sa, sb, mb;           // the two high digits (a is 8 bits wide; b's mask mb)
+    uint32_t window;               // rows per block window
+};
+
+__device__ __forceinline__ uint32_t rb_bucket(uint32_t k, const RbDesc& d) {
+    return ((k >> d.sa) & 0xFFu) | (((k >> d.sb) & d.mb) << 8);
+}
+
+// bstart[u] = first row of bucket u in rows sorted by (b, a); bstart[2^16]
+// = n.  Block x = digit b (its run from the histogram of b), thread y =
+// digit a: binary search of the run, whose rows are ordered by digit a.
+__global__ __launch_bounds__(256) void k_rb_starts(const uint32_t* __restrict__ keys, uint32_t n,
+                                                   const uint32_t* __restrict__ hist_b, RbDesc d,
+                                                   uint32_t* __restrict__ bstart) {
+    __shared__ uint32_t s_w[256 / PD_WAVE];
+    const uint32_t t = threadIdx.x, db = blockIdx.x, l = lane(), w = t / PD_WAVE;
+    const uint32_t h = t < db ? hist_b[t] : 0u;
+    const uint32_t inc = wave_incl_scan(h);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 256 / PD_WAVE; i++) lo += s_w[i];
+    uint32_t hi = lo + hist_b[db];
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (((keys[mid] >> d.sa) & 0xFFu) >= t) hi = mid;
+        else lo = mid + 1;
+    }
+    bstart[db * 256 + t] = lo;
+    if (db == 255 && t == 0) bstart[RB_NB] = n;
+}
+
+// Chunks of whole buckets, at most RB_CAP rows and 256 buckets each.
+// Thread w takes the buckets that start in rows [w W, (w + 1) W) (W =
+// window): one chunk when they fit, else chunks formed greedily bucket by
+// bucket.  A bucket larger than RB_CAP sets *overflow and is left out.  A
+// chunk is (first row, end row, first bucket << 8 | bits of its last
+// bucket's offset from the first, 0 for one bucket);
+// chunks are appended to `list` (one counter atomic per wave, after a
+// counting walk and a wave scan).
+__device__ __forceinline__ uint32_t rb_first_ge(const uint32_t* __restrict__ bstart, uint64_t x) {
+    uint32_t lo = 0, hi = RB_NB;   // bstart is non-decreasing and bstart[2^16] = n >= x
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (bstart[mid] >= x) hi = mid;
+        else lo = mid + 1;
+    }
+    return bstart[lo];
+}
+
+__global__ __launch_bounds__(256) void k_rb_chunks(const uint32_t* __restrict__ keys, uint32_t n, uint32_t nw,
+                                                   const uint32_t* __restrict__ bstart, RbDesc d,
+                                                   uint4* __restrict__ list, uint32_t* __restrict__ count,
+                                                   uint32_t* __restrict__ overflow) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s = 0, e = 0;
+    if (w < nw) {
+        const uint64_t x0 = (uint64_t)w * d.window;
+        s = rb_first_ge(bstart, x0);
+        e = rb_first_ge(bstart, min((uint64_t)n, x0 + d.window));
+    }
+    // bits of a chunk's bucket offsets (0: one bucket)
+    auto span_bits = [](uint32_t span) -> uint32_t { return span ? 32u - (uint32_t)__clz(span) : 0u; };
+    // walk(emit): the chunks of [s, e); emit(first, end, info)
+    auto walk = [&](auto&& emit) {
+        if (s >= e) return;
+        const uint32_t u0 = rb_bucket(keys[s], d), u1 = rb_bucket(keys[e - 1], d);
+        if (e - s <= RB_CAP && u1 - u0 <= 255u) {
+            emit(s, e, (u0 << 8) | span_bits(u1 - u0));
+            return;
+        }
+        for (uint32_t p = s; p < e;) {
+            const uint32_t b0 = rb_bucket(keys[p], d);
+            const uint32_t e0 = bstart[b0 + 1];
+            if (e0 - p > RB_CAP) {
+                atomicOr(overflow, 1u);
+                p = e0;
+                continue;
+            }
+            const uint32_t lim = min(e, p + RB_CAP);
+            uint32_t u = b0;
+            while (u < b0 + 255u && u + 1 < RB_NB && bstart[u + 2] <= lim) u++;
+            const uint32_t q = bstart[u + 1];
+            emit(p, q, (b0 << 8) | (q > e0 ? span_bits(u - b0) : 0u));
+            p = q;
+        }
+    };
+    uint32_t cnt = 0;
+    walk([&](uint32_t, uint32_t, uint32_t) { cnt++; });
+    const uint32_t inc = wave_incl_scan(cnt);
+    uint32_t base = 0;
+    if (lane() == PD_WAVE - 1 && inc) base = atomicAdd(count, inc);
+    base = __shfl(base, PD_WAVE - 1) + inc - cnt;
+    walk([&](uint32_t a, uint32_t b, uint32_t info) { list[base++] = make_uint4(a, b, info, 0u); });
+}
+
+// Persistent: block b sorts chunks b, b + gridDim.x, ... of the list, in
+// place: each chunk is loaded, ranked stably by each low digit in turn,
+// then by bucket when it holds more than one, and written back (ranking by
+// per-wave LDS match masks, as the onesweep pass ranks; one ballot per
+// digit bit instead measured 2.22 against 1.68 ms at 2^28 pairs; 1024
+// threads x 8 rows, 64 VGPRs with spills, 1.87 ms).  Padding lanes take the
+// largest digit and rank after every real row.
+template <typename V, bool HV>
+__global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_sort(uint32_t* __restrict__ keys, V* __restrict__ vals,
+                                                        const uint4* __restrict__ list,
+                                                        const uint32_t* __restrict__ count, RbDesc d) {
+    constexpr uint32_t WAVES = RB_BLOCK / PD_WAVE;
+    static_assert(RB_CAP * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
+    __shared__ __align__(16) uint32_t s_keys[RB_CAP];
+    __shared__ __align__(16) V s_vals[HV ? RB_CAP : 1];
+    __shared__ uint16_t s_cnt[WAVES][RS_BINS];
+    __shared__ uint32_t s_off[RS_BINS];
+    __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
+    auto s_match = reinterpret_cast<uint64_t (*)[RS_BINS]>(s_keys);   // per-wave masks while ranking
+    const uint32_t t = threadIdx.x, l = lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint64_t lt = lanemask_lt();
+    const uint32_t nch = count[0];
+    if (blockIdx.x >= nch) return;
+    // rows per thread: a chunk is spread over all waves, wave w holding rows
+    // [w 64 ipt, (w + 1) 64 ipt); items past the wave's rows are skipped
+    // (wave-uniform), so only the last wave's last item holds padding lanes
+    uint32_t key[RB_ITEMS];
+    V val[HV ? RB_ITEMS : 1];
+    uint32_t p = 0, m = 0, info = 0, wbase = 0, wrows = 0;
+    auto load_chunk = [&](const uint4 c4) {
+        p = (uint32_t)__builtin_amdgcn_readfirstlane(c4.x);
+        m = (uint32_t)__builtin_amdgcn_readfirstlane(c4.y) - p;
+        info = (uint32_t)__builtin_amdgcn_readfirstlane(c4.z);
+        const uint32_t ipt = (m + RB_BLOCK - 1) / RB_BLOCK;
+        wbase = w * (PD_WAVE * ipt);
+        wrows = (uint32_t)__builtin_amdgcn_readfirstlane(m > wbase ? min(PD_WAVE * ipt, m - wbase) : 0u);
+        const __amdgpu_buffer_rsrc_t rk = buf_rsrc(keys + p + wbase, wrows * 4u);
+#pragma unroll
+        for (uint32_t i = 0; i < RB_ITEMS; i++)
+            key[i] = i * PD_WAVE < wrows ? __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0) : 0u;
+        if constexpr (HV) {
+            const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vals + p + wbase, wrows * (uint32_t)sizeof(V));
+#pragma unroll
+            for (uint32_t i = 0; i < RB_ITEMS; i++) {
+                if (i * PD_WAVE >= wrows) val[i] = V(0);
+                else if constexpr (sizeof(V) == 4)
+                    val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, (l + i * PD_WAVE) * 4u, 0, 0);
+                else
+                    val[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rv, (l + i * PD_WAVE) * 8u, 0, 0));
+            }
+        }
+    };
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        load_chunk(list[ci]);
+        const uint32_t u0 = info >> 8;
+        const uint32_t npass = d.nlow + ((info & 0xFFu) ? 1u : 0u);
+        for (uint32_t j = 0; j < npass; j++) {
+            const bool by_bucket = j >= d.nlow;
+            const uint32_t sh = by_bucket ? 0u : d.lshift[j], mk = by_bucket ? 0u : d.lmask[j];
+            const uint32_t dmax = RS_BINS - 1;   // padding lanes: the largest digit
+            for (uint32_t i = l; i < RS_BINS; i += PD_WAVE) {
+                s_match[w][i] = 0;
+                s_cnt[w][i] = 0;
+            }
+            __syncthreads();
+            uint32_t pos[RB_ITEMS];
+#pragma unroll
+            for (uint32_t i = 0; i < RB_ITEMS; i++) {
+                if (i * PD_WAVE >= wrows) break;
+                uint32_t kk = key[i];
+                asm volatile("" : "+v"(kk));
+                const uint32_t dg = (i * PD_WAVE + l) < wrows ? (by_bucket ? rb_bucket(kk, d) - u0 : (kk >> sh) & mk) : dmax;
+                // lanes holding the same digit: their bits ORed into the
+                // digit's LDS mask, read back
+                atomicOr((unsigned long long*)&s_match[w][dg], (unsigned long long)(1ull << l));
+                PD_COMPILER_BARRIER();
+                const uint64_t mm = s_match[w][dg];
+                const uint32_t old = s_cnt[w][dg];
+                const uint32_t below = popc64(mm & lt);
+                pos[i] = (old + below) | (dg << 16);
+                PD_COMPILER_BARRIER();
+                if (below == 0) {
+                    s_cnt[w][dg] = (uint16_t)(old + popc64(mm));
+                    s_match[w][dg] = 0;
+                }
+                PD_COMPILER_BARRIER();
+            }
+            __syncthreads();
+            // per digit: prefix over waves, then over digits
+            uint32_t cnt = 0, inc = 0;
+            if (t < RS_BINS) {
+#pragma unroll
+                for (uint32_t ww = 0; ww < WAVES; ww++) {
+                    const uint32_t c = s_cnt[ww][t];
+                    s_cnt[ww][t] = (uint16_t)cnt;
+                    cnt += c;
+                }
+                inc = wave_incl_scan(cnt);
+                if (l == PD_WAVE - 1) s_wsum[w] = inc;
+            }
+            __syncthreads();
+            if (t < RS_BINS) {
+                uint32_t add = 0;
+                for (uint32_t i = 0; i < w; i++) add += s_wsum[i];
+                s_off[t] = add + inc - cnt;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t i = 0; i < RB_ITEMS; i++) {
+                if (i * PD_WAVE >= wrows) break;
+                const uint32_t dg = pos[i] >> 16;
+                const uint32_t r = (pos[i] & 0xFFFFu) + s_off[dg] + s_cnt[w][dg];
+                s_keys[r] = key[i];
+                if constexpr (HV) s_vals[r] = val[i];
+            }
+            __syncthreads();
+            if (j + 1 < npass) {
+                // the next pass ranks the rows in their new order
+#pragma unroll
+                for (uint32_t i = 0; i < RB_ITEMS; i++) {
+                    if (i * PD_WAVE >= wrows) break;
+                    key[i] = s_keys[wbase + i * PD_WAVE + l];
+                    if constexpr (HV) val[i] = s_vals[wbase + i * PD_WAVE + l];
+                }
+                __syncthreads();   // every row is read before the masks reuse s_keys
+            }
+        }
+#pragma unroll 4
+        for (uint32_t k = 0; k < RB_ITEMS; k++) {
+            const uint32_t j = t + k * RB_BLOCK;
+            if (j >= m) break;
+            keys[p + j] = s_keys[j];
+            if constexpr (HV) vals[p + j] = s_vals[j];
+        }
+        __syncthreads();   // the next chunk reuses the LDS buffers
+    }
+}
+
+// Bucket scheme for the digits of `run` (knob sort_bucket).  The bucket
+// pass is worth one global pass only when it replaces two (four digits) and
+// its chunks are large: at 2^28 uniform pairs (buckets of about 4096 rows)
+// it takes 1.67 ms where two onesweep passes take 2.06 ms; at 2^27 (2048-row
+// buckets, two per chunk, a third LDS ranking) 1.31 ms against 1.06 ms.  So
+// by default (1) it runs for four varying digits and a largest bucket,
+// estimated from the histograms of the two high digits as if independent,
+// of 3072 to 3/4 of a block's capacity rows; 2 takes it for any 3 or 4
+// digits whose estimate fits (tests); 0 never.  `hh` holds the two
+// histograms.
+static bool use_bucket(pimdal_ctx* c, const RsPassDesc& run, uint64_t n, const uint32_t* hh) {
+    const int64_t mode = c->tune_get("sort_bucket", 1);
+    if (mode == 0 || c->capture || run.npass < 3) return false;
+    if (run.mask[run.npass - 2] != 0xFFu) return false;
+    uint32_t ma = 0, mb = 0;
+    for (int i = 0; i < RS_BINS; i++) {
+        ma = std::max(ma, hh[i]);
+        mb = std::max(mb, hh[RS_BINS + i]);
+    }
+    const double est = (double)ma * (double)mb / (double)n;
+    if (est > 0.75 * RB_CAP) return false;
+    return mode == 2 || (run.npass == 4 && est >= 3072.0);
+}
+
+template <typename V, bool HV>
+static void sort_bucket(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
+                        const RsPassDesc& run, const uint32_t* hist, uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
+    const uint32_t P = run.npass;
+    // the two high digits: in -> scratch -> out
+    RsPassDesc hi{};
+    hi.npass = 2;
+    for (int j = 0; j < 2; j++) {
+        hi.shift[j] = run.shift[P - 2 + j];
+        hi.mask[j] = run.mask[P - 2 + j];
+    }
+    run_passes<V, HV>(c, kin, vin, n, kout, vout, hi, hist + (P - 2) * RS_BINS, offs + (P - 2) * RS_BINS,
+                      ctrs + (P - 2), s);
+    RbDesc d{};
+    d.nlow = P - 2;
+    for (uint32_t j = 0; j < d.nlow; j++) {
+        d.lshift[j] = run.shift[j];
+        d.lmask[j] = run.mask[j];
+    }
+    d.sa = run.shift[P - 2];
+    d.sb = run.shift[P - 1];
+    d.mb = run.mask[P - 1];
+    d.window = RB_CAP / 2;
+    const uint32_t nw = (uint32_t)ceil_div(n, (uint64_t)d.window);
+    // bstart[2^16 + 1] | overflow | chunk count | chunk list (at most one
+    // chunk per window plus one per bucket)
+    const uint64_t maxch = (uint64_t)nw + RB_NB;
+    Scratch sc(c, (RB_NB + 1 + 3) * 4 + maxch * 16, s);
+    uint32_t* bstart = sc.as<uint32_t>();
+    uint32_t* ovf = bstart + RB_NB + 1;
+    uint32_t* cnt = ovf + 1;
+    uint4* list = reinterpret_cast<uint4*>(sc.as<char>() + (RB_NB + 1 + 3) * 4);
+    fill_u32(c, ovf, 0, 2, s);
+    launch(c, "radix_bstart", k_rb_starts, dim3(256), dim3(256), 0, s, (const uint32_t*)kout, (uint32_t)n,
+           hist + (P - 1) * RS_BINS, d, bstart);
+    launch(c, "radix_chunks", k_rb_chunks, dim3((nw + 255) / 256), dim3(256), 0, s, (const uint32_t*)kout, (uint32_t)n,
+           nw, (const uint32_t*)bstart, d, list, cnt, ovf);
+    const uint32_t grid = std::max<uint32_t>(1, blocks_per_cu(k_rb_sort<V, HV>, RB_BLOCK) * c->cu_count);
+    launch(c, "radix_bucket", k_rb_sort<V, HV>, dim3(grid), dim3(RB_BLOCK), 0, s, kout, vout, (const uint4*)list,
+           (const uint32_t*)cnt, d);
+    uint32_t of = 0;
+    read_small(c, &of, ovf, 4, s);
+    if (of) {
+        // a bucket did not fit: every digit again, by ordinary passes
+        Scratch tk(c, n * 4, s), tv(c, HV ? n * sizeof(V) : 0, s);
+        copy_d2d(c, tk.get(), kout, n * 4, s);
+        if (HV) copy_d2d(c, tv.get(), vout, n * sizeof(V), s);
+        fill_u32(c, ctrs, 0, 4, s);
+        run_passes<V, HV>(c, tk.as<uint32_t>(), HV ? tv.as<V>() : (const V*)nullptr, n, kout, vout, run, hist, offs,
+                          ctrs, s);
+    }
+}
+
 void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw, uint64_t n, uint32_t* kout,
                 void* vout, int begin_bit, int end_bit, hipStream_t s) {
     PD_REQUIRE(vw == 0 || vw == 4 || vw == 8, PIMDAL_E_UNSUPPORTED, "payload width must be 0, 4 or 8 bytes");
@@ -941,16 +1274,24 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
 
     RsPassDesc run = all;
+    // host copy of hist[4][256] | offs | andor, read once when the digit
+    // detection or the bucket scheme's check needs it
+    const bool bucket_ok = c->tune_get("sort_bucket", 1) != 0 && !c->capture;
+    std::vector<uint32_t> hh;
+    if (detect || (bucket_ok && all.npass >= 3)) {
+        hh.resize(4 * RS_BINS * 2 + 2);
+        read_small(c, hh.data(), hist, hh.size() * 4, s);
+    }
+    int orig[4] = {0, 1, 2, 3};   // run digit -> its histogram in hh
     if (detect) {
-        uint32_t ao[2];
-        read_small(c, ao, andor, 8, s);
-        const uint32_t varying = ao[0] ^ ao[1];
+        const uint32_t varying = hh[4 * RS_BINS * 2] ^ hh[4 * RS_BINS * 2 + 1];
         // keep only digits with a varying bit; move their histograms down
         run.npass = 0;
         for (int p = 0; p < 4; p++) {
             if ((varying >> (8 * p)) & 0xFF) {
                 if (run.npass != (uint32_t)p)
                     copy_d2d(c, hist + run.npass * RS_BINS, hist + p * RS_BINS, RS_BINS * 4, s);
+                orig[run.npass] = p;
                 run.shift[run.npass] = 8 * p;
                 run.mask[run.npass] = 0xFF;
                 run.npass++;
@@ -961,6 +1302,22 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         copy_d2d(c, kout, kin, n * 4, s);
         if (vw) copy_d2d(c, vout, vin, n * vw, s);
         return;
+    }
+    if (bucket_ok && run.npass >= 3) {
+        std::vector<uint32_t> h2(2 * RS_BINS);
+        std::copy_n(hh.begin() + orig[run.npass - 2] * RS_BINS, RS_BINS, h2.begin());
+        std::copy_n(hh.begin() + orig[run.npass - 1] * RS_BINS, RS_BINS, h2.begin() + RS_BINS);
+        if (use_bucket(c, run, n, h2.data())) {
+            if (vw == 0)
+                sort_bucket<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, s);
+            else if (vw == 4)
+                sort_bucket<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs,
+                                            ctrs, s);
+            else
+                sort_bucket<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs,
+                                            ctrs, s);
+            return;
+        }
     }
     if (vw == 0)
         run_passes<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, s);

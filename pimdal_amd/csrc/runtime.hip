@@ -248,12 +248,12 @@ uint64_t read_u64(pimdal_ctx* c, const void* dev, hipStream_t s) {
 void read_small(pimdal_ctx* c, void* dst, const void* dev, uint64_t bytes, hipStream_t s) {
     PD_REQUIRE(!c->capture, PIMDAL_E_UNSUPPORTED,
                "a result read back to the host cannot be captured into a plan; use an asynchronous device form");
-    if (bytes > sizeof(c->readback_words)) {
+    if (bytes > pimdal_ctx::READBACK_BYTES) {
         PD_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, s));
         PD_HIP(hipStreamSynchronize(s));
         return;
     }
-    if (!c->readback) PD_HIP(hipHostMalloc(&c->readback, sizeof(c->readback_words), hipHostMallocDefault));
+    if (!c->readback) PD_HIP(hipHostMalloc(&c->readback, pimdal_ctx::READBACK_BYTES, hipHostMallocDefault));
     PD_HIP(hipMemcpyAsync(c->readback, dev, bytes, hipMemcpyDeviceToHost, s));
     PD_HIP(hipStreamSynchronize(s));
     memcpy(dst, c->readback, bytes);

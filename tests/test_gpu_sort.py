@@ -69,7 +69,22 @@ def test_sort_2_28_properties(ctx, zipf):
     """Full size: sorted, stable, a permutation, and out_key[i] == in_key[out_row[i]]."""
     n = 1 << 28
     dk, dv = datagen.sort_workload(ctx, n, zipf=zipf)
+    ctx.set_profiling(True)
+    ctx.reset_stats()
     ok, ov = ops.sort_pairs(ctx, dk, dv)
+    # uniform keys take the bucket scheme by default, Zipf keys plain passes
+    assert ctx.kernel_stat("radix_bucket")[0] == (0 if zipf else 1)
+    ctx.set_profiling(False)
+    if not zipf:
+        # the same sort by one global pass per digit gives the same arrays
+        ctx.set_tune("sort_bucket", 0)
+        try:
+            pk, pv = ops.sort_pairs(ctx, dk, dv)
+        finally:
+            ctx.clear_tune("sort_bucket")
+        assert np.array_equal(pk.numpy(), ok.numpy()) and np.array_equal(pv.numpy(), ov.numpy())
+        pk.free()
+        pv.free()
     k = dk.numpy()
     dk.free()
     dv.free()
@@ -167,6 +182,9 @@ PASS_VARIANTS = {
     "counted_xcd_order": {"sort_scheme": 1},
     "counted_ticket_order": {"sort_scheme": 1, "sort_rts_order": 1},
     "counted_block1024": {"sort_scheme": 1, "sort_block": 1024},
+    "plain_passes_no_bucket": {"sort_bucket": 0},
+    "bucket_after_launch_order_tiles": {"sort_xcd_tiles": 0, "sort_bucket": 2},
+    "bucket_scheme": {"sort_bucket": 2},
 }
 
 
@@ -252,3 +270,95 @@ def test_sort_in_a_child_process_after_this_one_ran_persistent_passes(ctx, orc):
     assert res.returncode == 0 and "child sort ok" in res.stdout, res.stderr[-2000:]
     assert "child path:" in res.stdout
     print(res.stdout.strip().splitlines()[0])
+
+
+# ---- the bucket scheme: global passes over the two high digits, then each
+# bucket of equal high bits sorted by the low digits in LDS ----
+
+def _sort_stats(ctx, k, v, begin=-1, end=-1, bucket=2):
+    """Sort with sort_bucket = `bucket` (2: the bucket scheme wherever its
+    estimate fits, at sizes below the ones it is chosen for by default);
+    returns the bucket-pass and onesweep launch counts beside the result."""
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    ctx.set_tune("sort_bucket", bucket)
+    try:
+        gk, gv = gpu_sort(ctx, k, v, begin, end)
+        return gk, gv, ctx.kernel_stat("radix_bucket")[0], ctx.kernel_stat("radix_onesweep")[0]
+    finally:
+        ctx.clear_tune("sort_bucket")
+        ctx.set_profiling(False)
+
+
+@pytest.mark.parametrize("n", [1 << 20, (1 << 21) + 77, 1 << 22])
+def test_sort_bucket_scheme_uniform(ctx, orc, n):
+    k, v = orc.sort_workload(n, seed=300 + n)
+    gk, gv, nb, npass = _sort_stats(ctx, k, v)
+    assert nb == 1 and npass == 2, (nb, npass)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_bucket_scheme_keys_only_and_u64_payload(ctx, orc):
+    k, _ = orc.sort_workload(1 << 21, seed=31)
+    gk, _, nb, _ = _sort_stats(ctx, k, None)
+    assert nb == 1 and np.array_equal(gk, orc.sort_keys(k))
+    v = orc.gen_range(np.int64, 6, 3, 1 << 21, -(1 << 40), 1 << 40)
+    gk, gv, nb, _ = _sort_stats(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v.view(np.uint64))
+    assert nb == 1 and np.array_equal(gk, ek) and np.array_equal(gv.view(np.uint64), ev)
+
+
+def test_sort_bucket_scheme_24_bit_keys_and_bit_ranges(ctx, orc):
+    n = (1 << 21) + 5
+    v = np.arange(n, dtype=np.uint32)
+    k = orc.gen_range(np.uint32, 2, 7, n, 0, (1 << 24) - 1)   # three digits: one low digit in LDS
+    gk, gv, nb, npass = _sort_stats(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert nb == 1 and npass == 2 and np.array_equal(gk, ek) and np.array_equal(gv, ev)
+    k, _ = orc.sort_workload(n, seed=33)
+    for begin, end in ((0, 28), (3, 29), (0, 32)):
+        gk, gv, nb, _ = _sort_stats(ctx, k, v, begin, end)
+        mk = (k >> np.uint32(begin)) & np.uint32((1 << (end - begin)) - 1)
+        order = np.argsort(mk, kind="stable")
+        assert nb == 1 and np.array_equal(gv, v[order]) and np.array_equal(gk, k[order]), (begin, end)
+
+
+def test_sort_bucket_scheme_duplicate_keys_stay_stable(ctx, orc):
+    """2^16 distinct keys spread over 32 bits: every bucket holds runs of
+    equal keys, whose payloads (row ids) must keep input order."""
+    n = 1 << 21
+    r = orc.gen_range(np.uint32, 4, 1, n, 0, (1 << 16) - 1).astype(np.uint64)
+    k = ((r * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    v = np.arange(n, dtype=np.uint32)
+    gk, gv, nb, _ = _sort_stats(ctx, k, v)
+    ek, ev = orc.sort_pairs(k, v)
+    assert nb == 1 and np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_bucket_scheme_oversized_bucket_falls_back(ctx, orc):
+    """High byte == second byte: the histograms look uniform but only 256
+    buckets hold rows (16384 each, beyond a block's 8192), so the bucket pass
+    flags them and every digit is sorted again by ordinary passes."""
+    n = 1 << 22
+    b = orc.gen_range(np.uint32, 8, 1, n, 0, 255)
+    lo = orc.gen_range(np.uint32, 8, 2, n, 0, (1 << 16) - 1)
+    k = ((b << np.uint32(24)) | (b << np.uint32(16)) | lo).astype(np.uint32)
+    v = np.arange(n, dtype=np.uint32)
+    gk, gv, nb, npass = _sort_stats(ctx, k, v)
+    assert nb == 1 and npass == 2 + 4, (nb, npass)
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+def test_sort_bucket_scheme_off_small_and_zipf_take_plain_passes(ctx, orc):
+    k, v = orc.sort_workload(1 << 21, zipf=True)
+    gk, gv, nb, npass = _sort_stats(ctx, k, v)   # the estimate does not fit
+    assert nb == 0 and npass == 4
+    ek, ev = orc.sort_pairs(k, v)
+    assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+    k, v = orc.sort_workload(1 << 21, seed=35)
+    ek, ev = orc.sort_pairs(k, v)
+    for bucket in (0, 1):   # off; by default buckets of 32 rows are too small
+        gk, gv, nb, npass = _sort_stats(ctx, k, v, bucket=bucket)
+        assert nb == 0 and npass == 4 and np.array_equal(gk, ek) and np.array_equal(gv, ev)
