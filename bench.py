@@ -132,7 +132,9 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
            "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "as_table", "as_fold",
-           "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter")
+           "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",
+           "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "radix_bstart", "radix_chunks",
+           "radix_bucket", "q1_status")
 
 
 def kernel_breakdown(ctx, steps):
@@ -329,7 +331,10 @@ AGGR_SHAPES = {"k50": (1, 50), "k4000": (1, 4000), "k25": (1, 25), "k2^24": (0, 
 def bench_aggr(ctx, steps, warmup, which, shape, n=1 << 27):
     """Hash or sort aggregation (SUM and COUNT per uint32 key) of n rows with
     keys uniform over the shape's range and int64 values on [1, 10].
-    Algorithmic bytes: key and value read once, 20 bytes out per group."""
+    Algorithmic bytes: key and value read once, 20 bytes out per group, plus
+    24 bytes per row (12 read, 12 written) per partition pass that the path
+    taken makes (0 for few groups, 1 for the key-range path); the bound with
+    no partition pass is reported beside it."""
     import ctypes as C
     from pimdal_amd import datagen, _lib
     lo, hi = AGGR_SHAPES[shape]
@@ -343,8 +348,16 @@ def bench_aggr(ctx, steps, warmup, which, shape, n=1 << 27):
         _lib.check(fn(ctx.handle, key.vp, val.vp, n, ok.vp, osum.vp, ocnt.vp, cap, C.byref(g)))
 
     tm = timed_steps(ctx, step, steps, warmup)
-    alg = 12 * n + 20 * g.value
+    # partition passes per step: the key-range scatter, or the hashed onesweep
+    # passes, or the radix sort's passes of the sorted-runs path
+    passes = sum(ctx.kernel_stat(k)[0] for k in ("ad_scatter", "part_onesweep")) / steps
+    if ctx.kernel_stat("as_runs")[0]:
+        passes += ctx.kernel_stat("radix_onesweep")[0] / steps
+    p = round(passes)
+    alg0 = 12 * n + 20 * g.value
+    alg = alg0 + 24 * p * n
     res = {"rows": n, "groups": g.value, "key_range": [lo, hi], "s_per_step": tm, "rows_per_s": n / tm,
+           "partition_passes": p, "alg_bytes_no_pass": alg0, "hbm_frac_no_pass": alg0 / tm / HBM_PEAK,
            "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
            "kernels": kernel_breakdown(ctx, steps)}
     for a in (key, val, ok, osum, ocnt):

@@ -108,7 +108,10 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,
  *   gb_q1_slots (1024 or 2048);
  *   aggr_small (1: hash and sort aggregation first try per-block LDS tables
- *   for up to 4096 keys per block and 32768 in all; 0: skip that path);
+ *   for up to 4096 keys per block and 32768 in all; 0: skip that path),
+ *   aggr_dense_max (keys spanning at most this many values, default and
+ *   largest 2^24: one range-partition pass into 8192-key partitions summed
+ *   in key-indexed LDS arrays, groups in key order; 0: skip that path);
  *   serve_index_load_pct. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);

@@ -28,6 +28,9 @@
 // that starts in the tile stored by the thread that holds its last row
 // there.  The piece of a run carried over from an earlier tile is kept per
 // tile and added by a second kernel, after the first has stored every run.
+#include <algorithm>
+#include <vector>
+
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
 #include "pd_ops.hpp"
@@ -547,6 +550,390 @@ uint64_t aggr_sum_sorted(pimdal_ctx* c, const uint32_t* key, const int64_t* val,
     const uint64_t g = read_u64(c, total, s);
     PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     return g;
+}
+
+// ---------------- dense key range: one range-partition pass ----------------
+// When all keys lie within AD_MAXF * AD_SPAN (2^24, knob aggr_dense_max) of
+// the smallest, partition p holds keys [kmin + p AD_SPAN, kmin + (p + 1)
+// AD_SPAN).  A counting pass, a scan over tiles and one scatter move every
+// row once (the 8-bit hashed passes of the general path move it twice);
+// then one block per partition sums into LDS arrays indexed by the key's
+// offset (no hashing, no probing) and writes the partition's groups in key
+// order at its group offset, found by decoupled look-back over partitions
+// taken in ticket order.  Groups leave in key order, so the hash and the
+// sort aggregation both take this path.
+constexpr uint32_t AD_SPAN = 8192;                  // keys per partition
+constexpr uint32_t AD_SHIFT = 13;
+constexpr uint32_t AD_MAXF = 2048;                  // partitions
+constexpr uint32_t AD_BLOCK = 1024;
+constexpr uint32_t AD_ITEMS = 8;
+constexpr uint32_t AD_TILE = AD_BLOCK * AD_ITEMS;   // rows per tile
+constexpr uint32_t AD_GT = 16;                      // tiles per scan group
+constexpr uint32_t AD_FLAG_AGG = 1u << 30;
+constexpr uint32_t AD_FLAG_PRE = 2u << 30;
+constexpr uint32_t AD_VALUE = (1u << 30) - 1;
+
+// Per-block min and max of the keys (mm[2 b], mm[2 b + 1]); the host folds
+// them.  16-byte loads over the 16-byte-aligned middle.
+__global__ __launch_bounds__(256) void k_ad_minmax(const uint32_t* __restrict__ key, uint64_t n,
+                                                   uint32_t* __restrict__ mm) {
+    __shared__ uint32_t s_lo[256 / PD_WAVE], s_hi[256 / PD_WAVE];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t head = min(n, (uint64_t)(((16 - (uintptr_t)key % 16) % 16) / 4));
+    const uint64_t nv = (n - head) / 4;
+    for (uint64_t i = tid; i < head; i += stride) { lo = min(lo, key[i]); hi = max(hi, key[i]); }
+    const uint4* k4 = reinterpret_cast<const uint4*>(key + head);
+    for (uint64_t i = tid; i < nv; i += stride) {
+        const uint4 q = k4[i];
+        lo = min(min(lo, q.x), min(q.y, min(q.z, q.w)));
+        hi = max(max(hi, q.x), max(q.y, max(q.z, q.w)));
+    }
+    for (uint64_t i = head + nv * 4 + tid; i < n; i += stride) { lo = min(lo, key[i]); hi = max(hi, key[i]); }
+    for (int o = PD_WAVE / 2; o > 0; o /= 2) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+    const uint32_t w = threadIdx.x / PD_WAVE;
+    if (lane() == 0) { s_lo[w] = lo; s_hi[w] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 1; i < 256 / PD_WAVE; i++) { lo = min(lo, s_lo[i]); hi = max(hi, s_hi[i]); }
+        mm[2 * blockIdx.x] = lo;
+        mm[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// Rows per partition of each AD_TILE-row tile: cnt[tile * F + p].
+// Persistent: block b counts tiles b, b + gridDim.x, ...
+__global__ __launch_bounds__(512) void k_ad_count(const uint32_t* __restrict__ kin, uint64_t n, uint32_t kmin,
+                                                  uint32_t F, uint32_t* __restrict__ cnt) {
+    constexpr uint32_t Q = AD_TILE / 512;
+    __shared__ uint32_t h[AD_MAXF];
+    const uint32_t t = threadIdx.x, l = lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint64_t ntiles = (n + AD_TILE - 1) / AD_TILE;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        for (uint32_t i = t; i < F; i += 512) h[i] = 0;
+        const uint64_t wave_base = tile * AD_TILE + (uint64_t)w * (PD_WAVE * Q);
+        const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * Q), n - wave_base) : 0));
+        const __amdgpu_buffer_rsrc_t rk = buf_rsrc(kin + wave_base, wrows * 4u);
+        uint32_t q[Q];
+#pragma unroll
+        for (uint32_t i = 0; i < Q; i++) q[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < Q; i++)
+            if (l + i * PD_WAVE < wrows) atomicAdd(&h[(q[i] - kmin) >> AD_SHIFT], 1u);
+        __syncthreads();
+        for (uint32_t i = t; i < F; i += 512) cnt[tile * F + i] = h[i];
+        __syncthreads();
+    }
+}
+
+// In place: cnt[tile][p] becomes the rows of partition p in earlier tiles;
+// start[p] (p <= F) = rows of partitions below p.  Thread t takes
+// partitions t and t + 1024; a block takes the next group of AD_GT tiles
+// from a counter and resolves the group prefix by decoupled look-back.
+__global__ __launch_bounds__(1024) void k_ad_tscan(uint32_t* __restrict__ cnt, uint32_t ntiles, uint32_t F,
+                                                   uint32_t* __restrict__ gstatus, uint32_t* __restrict__ ctr,
+                                                   uint32_t* __restrict__ start) {
+    __shared__ uint32_t s_g, s_w[1024 / PD_WAVE];
+    const uint32_t t = threadIdx.x, l = lane(), w = t / PD_WAVE;
+    if (t == 0) s_g = atomicAdd(ctr, 1u);
+    __syncthreads();
+    const uint32_t g = s_g;
+    const uint32_t ngroups = (ntiles + AD_GT - 1) / AD_GT;
+    const uint32_t t0 = g * AD_GT, t1 = min(ntiles, t0 + AD_GT);
+    uint32_t tot[2] = {0, 0};
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        const uint32_t p = t + h * 1024;
+        if (p >= F) continue;
+        uint32_t cg[AD_GT], sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < AD_GT; i++) cg[i] = t0 + i < t1 ? cnt[(uint64_t)(t0 + i) * F + p] : 0u;
+#pragma unroll
+        for (uint32_t i = 0; i < AD_GT; i++) sum += cg[i];
+        st_relaxed(&gstatus[(uint64_t)g * F + p], (g == 0 ? AD_FLAG_PRE : AD_FLAG_AGG) | sum);
+        uint32_t excl = 0;
+        if (g > 0) {
+            int64_t j = (int64_t)g - 1;
+            bool done = false;
+            while (!done) {   // every earlier group is running (tickets), so each publishes
+                uint32_t st[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    st[q] = (j - q >= 0) ? ld_relaxed(&gstatus[(uint64_t)(j - q) * F + p]) : AD_FLAG_PRE;
+                bool stall = false;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (done || stall) continue;
+                    if ((st[q] >> 30) == 0) { stall = true; continue; }
+                    excl += st[q] & AD_VALUE;
+                    if ((st[q] >> 30) == 2) done = true;
+                    else j--;
+                }
+                if (stall) __builtin_amdgcn_s_sleep(1);
+            }
+            st_relaxed(&gstatus[(uint64_t)g * F + p], AD_FLAG_PRE | (excl + sum));
+        }
+        uint32_t run = excl;
+#pragma unroll
+        for (uint32_t i = 0; i < AD_GT; i++) {
+            if (t0 + i < t1) cnt[(uint64_t)(t0 + i) * F + p] = run;
+            run += cg[i];
+        }
+        tot[h] = run;
+    }
+    if (g == ngroups - 1) {
+        // tot = rows of partitions t and t + 1024: exclusive scan over partitions
+        uint32_t base = 0;
+#pragma unroll
+        for (uint32_t h = 0; h < 2; h++) {
+            const uint32_t v = tot[h];
+            const uint32_t inc = wave_incl_scan(v);
+            if (l == PD_WAVE - 1) s_w[w] = inc;
+            __syncthreads();
+            uint32_t add = 0, all = 0;
+            for (uint32_t i = 0; i < 1024 / PD_WAVE; i++) {
+                if (i < w) add += s_w[i];
+                all += s_w[i];
+            }
+            const uint32_t p = t + h * 1024;
+            if (p < F) start[p] = base + add + inc - v;
+            if (p == F - 1) start[F] = base + add + inc;   // every row
+            base += all;
+            __syncthreads();
+        }
+    }
+}
+
+// Block b takes tile (b % 8) * xt + b / 8 (neighbouring tiles on one XCD,
+// so partition runs of neighbouring tiles meet in its L2).  Rows are ranked
+// within their partition by LDS counters (order inside a partition is not
+// needed), staged in LDS by partition and written out in partition runs.
+__global__ __launch_bounds__(AD_BLOCK) void k_ad_scatter(const uint32_t* __restrict__ kin,
+                                                         const int64_t* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                         int64_t* __restrict__ vout, uint64_t n, uint32_t kmin,
+                                                         uint32_t F, const uint32_t* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ start, uint32_t xt) {
+    __shared__ __align__(16) uint32_t s_k[AD_TILE];
+    __shared__ __align__(16) int64_t s_v[AD_TILE];
+    __shared__ uint32_t s_c[AD_MAXF];   // tile rows per partition, then their start in the tile
+    __shared__ uint32_t s_g[AD_MAXF];   // output position of the partition's run minus its tile start
+    __shared__ uint32_t s_ws[AD_BLOCK / PD_WAVE];
+    const uint32_t t = threadIdx.x, l = lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint64_t tile = (uint64_t)(blockIdx.x % 8) * xt + blockIdx.x / 8;
+    const uint64_t tile_base = tile * AD_TILE;
+    if (tile_base >= n) return;   // grid rounded up to a multiple of 8
+    const uint32_t tile_rows = (uint32_t)min((uint64_t)AD_TILE, n - tile_base);
+    uint32_t gpos[2];
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        const uint32_t p = t + h * AD_BLOCK;
+        s_c[p] = 0;
+        gpos[h] = p < F ? start[p] + cnt[tile * F + p] : 0u;
+    }
+    const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * AD_ITEMS);
+    const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * AD_ITEMS), n - wave_base) : 0));
+    const __amdgpu_buffer_rsrc_t rk = buf_rsrc(kin + wave_base, wrows * 4u);
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vin + wave_base, wrows * 8u);
+    uint32_t key[AD_ITEMS];
+    int64_t val[AD_ITEMS];
+#pragma unroll
+    for (uint32_t i = 0; i < AD_ITEMS; i++) {
+        key[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0);
+        val[i] = (int64_t)__builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rv, (l + i * PD_WAVE) * 8u, 0, 0));
+    }
+    __syncthreads();
+    // rank within the partition (LDS counter), partition in the high bits
+    uint32_t rk_d[AD_ITEMS];
+#pragma unroll
+    for (uint32_t i = 0; i < AD_ITEMS; i++) {
+        rk_d[i] = 0xFFFFFFFFu;
+        if (l + i * PD_WAVE < wrows) {
+            const uint32_t p = (key[i] - kmin) >> AD_SHIFT;
+            rk_d[i] = atomicAdd(&s_c[p], 1u) | (p << 16);
+        }
+    }
+    __syncthreads();
+    // tile-local partition starts: exclusive scan over the counters
+    uint32_t base = 0;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        const uint32_t p = t + h * AD_BLOCK;
+        const uint32_t c = s_c[p];
+        const uint32_t inc = wave_incl_scan(c);
+        if (l == PD_WAVE - 1) s_ws[w] = inc;
+        __syncthreads();
+        uint32_t add = 0, all = 0;
+        for (uint32_t i = 0; i < AD_BLOCK / PD_WAVE; i++) {
+            if (i < w) add += s_ws[i];
+            all += s_ws[i];
+        }
+        const uint32_t st = base + add + inc - c;
+        base += all;
+        __syncthreads();   // s_ws is rewritten by the next half
+        s_c[p] = st;
+        s_g[p] = gpos[h] - st;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < AD_ITEMS; i++) {
+        if (rk_d[i] == 0xFFFFFFFFu) continue;
+        const uint32_t j = s_c[rk_d[i] >> 16] + (rk_d[i] & 0xFFFFu);
+        s_k[j] = key[i];
+        s_v[j] = val[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < AD_ITEMS; i++) {
+        const uint32_t j = i * AD_BLOCK + t;
+        if (j >= tile_rows) break;
+        const uint32_t k = s_k[j];
+        const uint32_t dst = s_g[(k - kmin) >> AD_SHIFT] + j;
+        kout[dst] = k;
+        vout[dst] = s_v[j];
+    }
+}
+
+// One block per partition, in ticket order: LDS count and sum arrays
+// indexed by key - kmin - p AD_SPAN, then the groups (count > 0) in key
+// order at the partition's group offset (decoupled look-back over
+// partitions).  Writes at most `cap` groups; *total = all groups.
+__global__ __launch_bounds__(AD_BLOCK) void k_ad_sum(const uint32_t* __restrict__ pk, const int64_t* __restrict__ pv,
+                                                     const uint32_t* __restrict__ start, uint32_t F, uint32_t kmin,
+                                                     uint32_t* __restrict__ ctr, uint64_t* __restrict__ status,
+                                                     uint32_t* __restrict__ okey, int64_t* __restrict__ osum,
+                                                     int64_t* __restrict__ ocnt, uint64_t cap,
+                                                     uint64_t* __restrict__ total) {
+    constexpr uint32_t SLOTS = AD_SPAN / AD_BLOCK;   // slots per thread at the emit
+    __shared__ uint32_t s_cnt[AD_SPAN];
+    __shared__ unsigned long long s_sum[AD_SPAN];
+    __shared__ uint32_t s_p, s_w[AD_BLOCK / PD_WAVE];
+    __shared__ uint64_t s_excl;
+    const uint32_t t = threadIdx.x, l = lane(), w = t / PD_WAVE;
+    if (t == 0) s_p = atomicAdd(ctr, 1u);
+#pragma unroll
+    for (uint32_t i = 0; i < SLOTS; i++) {
+        s_cnt[i * AD_BLOCK + t] = 0;
+        s_sum[i * AD_BLOCK + t] = 0;
+    }
+    __syncthreads();
+    const uint32_t p = s_p;
+    const uint32_t r0 = start[p], r1 = start[p + 1];
+    const uint32_t kbase = kmin + p * AD_SPAN;
+    // four rows per thread in flight per round
+    uint32_t i = r0 + t;
+    for (; i + 3 * AD_BLOCK < r1; i += 4 * AD_BLOCK) {
+        uint32_t k[4];
+        int64_t v[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            k[u] = pk[i + u * AD_BLOCK];
+            v[u] = pv[i + u * AD_BLOCK];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            atomicAdd(&s_cnt[k[u] - kbase], 1u);
+            atomicAdd(&s_sum[k[u] - kbase], (unsigned long long)v[u]);
+        }
+    }
+    for (; i < r1; i += AD_BLOCK) {
+        const uint32_t k = pk[i];
+        atomicAdd(&s_cnt[k - kbase], 1u);
+        atomicAdd(&s_sum[k - kbase], (unsigned long long)pv[i]);
+    }
+    __syncthreads();
+    // thread t owns slots [t SLOTS, (t + 1) SLOTS): groups counted, scanned
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < SLOTS; j++) c += s_cnt[t * SLOTS + j] ? 1u : 0u;
+    const uint32_t inc = wave_incl_scan(c);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t add = 0, gp = 0;
+    for (uint32_t j = 0; j < AD_BLOCK / PD_WAVE; j++) {
+        if (j < w) add += s_w[j];
+        gp += s_w[j];
+    }
+    if (w == 0) {
+        const uint64_t ex = lookback_single<1>(status, p, gp);
+        if (l == 0) s_excl = ex;
+    }
+    __syncthreads();
+    uint64_t off = s_excl + add + inc - c;
+#pragma unroll
+    for (uint32_t j = 0; j < SLOTS; j++) {
+        const uint32_t sl = t * SLOTS + j;
+        const uint32_t cc = s_cnt[sl];
+        if (!cc) continue;
+        if (off < cap) {
+            okey[off] = kbase + sl;
+            osum[off] = (int64_t)s_sum[sl];
+            ocnt[off] = cc;
+        }
+        off++;
+    }
+    if (p == F - 1 && t == 0) *total = s_excl + gp;
+}
+
+bool aggr_sum_dense(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey,
+                    int64_t* osum, int64_t* ocnt, uint64_t cap, uint64_t* groups, hipStream_t s) {
+    const int64_t maxr = std::min<int64_t>(c->tune_get("aggr_dense_max", (int64_t)AD_MAXF * AD_SPAN),
+                                           (int64_t)AD_MAXF * AD_SPAN);
+    if (maxr <= 0 || n == 0) return false;
+    const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64), (uint64_t)c->cu_count * 4));
+    uint32_t mm[2];
+    {
+        Scratch part(c, (uint64_t)G * 8, s);
+        launch(c, "ad_minmax", k_ad_minmax, dim3(G), dim3(256), 0, s, key, n, part.as<uint32_t>());
+        std::vector<uint32_t> h(2 * G);
+        read_small(c, h.data(), part.get(), (uint64_t)G * 8, s);
+        mm[0] = 0xFFFFFFFFu;
+        mm[1] = 0;
+        for (uint32_t b = 0; b < G; b++) {
+            mm[0] = std::min(mm[0], h[2 * b]);
+            mm[1] = std::max(mm[1], h[2 * b + 1]);
+        }
+    }
+    const uint64_t range = (uint64_t)mm[1] - mm[0] + 1;
+    if (range > (uint64_t)maxr) return false;
+    const uint32_t kmin = mm[0];
+    const uint32_t F = (uint32_t)ceil_div(range, AD_SPAN);
+    const uint32_t ntiles = (uint32_t)ceil_div(n, AD_TILE);
+    const uint32_t ngroups = (ntiles + AD_GT - 1) / AD_GT;
+    const uint32_t xt = (ntiles + 7) / 8;
+    // cnt[ntiles][F] | gstatus[ngroups][F] | start[F + 1] | 2 tickets | (8-aligned) status[F] | total
+    const uint64_t w32 = (uint64_t)ntiles * F + (uint64_t)ngroups * F + F + 1 + 2;
+    const uint64_t w32a = (w32 + 1) & ~1ull;
+    Scratch sc(c, w32a * 4 + ((uint64_t)F + 1) * 8, s);
+    uint32_t* cnt = sc.as<uint32_t>();
+    uint32_t* gst = cnt + (uint64_t)ntiles * F;
+    uint32_t* start = gst + (uint64_t)ngroups * F;
+    uint32_t* tick = start + F + 1;
+    uint64_t* pst = reinterpret_cast<uint64_t*>(cnt + w32a);
+    uint64_t* tot = pst + F;
+    fill_u32(c, gst, 0, (uint64_t)ngroups * F, s);
+    fill_u32(c, tick, 0, 2, s);
+    fill_u64(c, pst, 0, (uint64_t)F + 1, s);
+    Scratch pk(c, n * 4, s), pv(c, n * 8, s);
+    const uint32_t cgrid = std::min<uint32_t>(ntiles, (uint32_t)c->cu_count * 4);
+    launch(c, "ad_count", k_ad_count, dim3(cgrid), dim3(512), 0, s, key, n, kmin, F, cnt);
+    launch(c, "ad_tscan", k_ad_tscan, dim3(ngroups), dim3(1024), 0, s, cnt, ntiles, F, gst, tick, start);
+    launch(c, "ad_scatter", k_ad_scatter, dim3(8 * xt), dim3(AD_BLOCK), 0, s, key, val, pk.as<uint32_t>(),
+           pv.as<int64_t>(), n, kmin, F, (const uint32_t*)cnt, (const uint32_t*)start, xt);
+    launch(c, "ad_sum", k_ad_sum, dim3(F), dim3(AD_BLOCK), 0, s, (const uint32_t*)pk.as<uint32_t>(),
+           (const int64_t*)pv.as<int64_t>(), (const uint32_t*)start, F, kmin, tick + 1, pst, okey, osum, ocnt, cap,
+           tot);
+    const uint64_t g = read_u64(c, tot, s);
+    PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    *groups = g;
+    return true;
 }
 
 }  // namespace pd

@@ -1517,6 +1517,11 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
         PD_HIP(hipStreamSynchronize(s));
         return g;
     }
+    // keys within 2^24 of each other: one range-partition pass
+    if (aggr_sum_dense(c, key, val, n, out_key, out_sum, out_cnt, cap, &g, s)) {
+        PD_HIP(hipStreamSynchronize(s));
+        return g;
+    }
     // otherwise onesweep partitions of (key, value) by hash digits and one
     // LDS table per partition, writing the outputs directly
     g = group_by_onesweep<AggSum, RowsSum>(
@@ -1528,8 +1533,9 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
 
 // Sort aggregation: groups in ascending key order.  With few groups the
 // hash tables' groups are ordered by key (sorting the rows would only add
-// passes); otherwise the rows are radix sorted by key and reduced in one
-// pass over the sorted runs.
+// passes); with keys in a range of at most 2^24, range partitions summed
+// into key-indexed LDS arrays; otherwise the rows are radix sorted by key
+// and reduced in one pass over the sorted runs.
 uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
                    int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
@@ -1539,6 +1545,11 @@ uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
     uint64_t g = 0;
     if (c->tune_get("aggr_small", 1) && hint <= 4096 &&
         aggr_sum_small(c, key, val, n, out_key, out_sum, out_cnt, cap, true, &g, s)) {
+        PD_HIP(hipStreamSynchronize(s));
+        return g;
+    }
+    // keys within 2^24 of each other: range partitions give key order
+    if (aggr_sum_dense(c, key, val, n, out_key, out_sum, out_cnt, cap, &g, s)) {
         PD_HIP(hipStreamSynchronize(s));
         return g;
     }

@@ -90,21 +90,27 @@ def test_aggr_2p27_hash_and_sort_agree(ctx, shape):
 
 def _both_paths(ctx, orc, k, v, small_fits=None):
     """hash and sort aggregation, with and without the few-groups LDS path,
-    against the oracle (sort output in its own order, hash canonical)."""
+    and with and without the dense key-range path, against the oracle (sort
+    output in its own order, hash canonical)."""
     exp = orc.sort_aggr(k, v)
     dk, dv = ctx.to_device(k), ctx.to_device(v)
     try:
-        for small in (1, 0):
+        for small, dense in ((1, 1), (0, 1), (0, 0)):
             ctx.set_tune("aggr_small", small)
+            if dense:
+                ctx.clear_tune("aggr_dense_max")
+            else:
+                ctx.set_tune("aggr_dense_max", 0)
             for fn in (ops.hash_aggr, ops.sort_aggr):
                 ok, os_, oc, g = fn(ctx, dk, dv)
                 got = (ok.numpy(g), os_.numpy(g), oc.numpy(g))
                 if fn is ops.hash_aggr:
                     got = canon(*got)
-                assert g == exp[0].size, (fn.__name__, small)
-                assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (fn.__name__, small)
+                assert g == exp[0].size, (fn.__name__, small, dense)
+                assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (fn.__name__, small, dense)
     finally:
         ctx.clear_tune("aggr_small")
+        ctx.clear_tune("aggr_dense_max")
 
 
 def test_aggr_paths_max_key_and_ragged(ctx, orc):
@@ -153,6 +159,56 @@ def test_sort_aggr_runs_across_tiles(ctx, orc, items):
             _both_paths(ctx, orc, k, v)
     finally:
         ctx.clear_tune("as_runs_items")
+
+
+def _dense_ran(ctx, fn, dk, dv):
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    try:
+        ok, os_, oc, g = fn(ctx, dk, dv)
+        return (ok.numpy(g), os_.numpy(g), oc.numpy(g)), ctx.kernel_stat("ad_sum")[0] > 0
+    finally:
+        ctx.set_profiling(False)
+
+
+@pytest.mark.parametrize("case", ["span_2^24", "span_2^24_plus_1", "top_of_u32", "from_2^31", "ragged_8193",
+                                  "one_key_per_partition_edge"])
+def test_aggr_dense_key_range(ctx, orc, case):
+    """The key-range path: ranges of exactly 2^24 keys (2048 partitions) and
+    one more (the hashed path instead), keys ending at 0xFFFFFFFF, starting
+    at 2^31, a ragged row count, and keys on partition edges."""
+    n = 1 << 21
+    v = orc.gen_range(np.int64, 36, 2, n, -(1 << 45), 1 << 45)
+    if case == "span_2^24":
+        k = orc.gen_range(np.uint32, 36, 1, n, 5, 5 + (1 << 24) - 1)
+        k[0], k[1] = 5, 5 + (1 << 24) - 1
+    elif case == "span_2^24_plus_1":
+        k = orc.gen_range(np.uint32, 36, 1, n, 5, 5 + (1 << 24))
+        k[0], k[1] = 5, 5 + (1 << 24)
+    elif case == "top_of_u32":
+        k = orc.gen_range(np.uint32, 36, 1, n, 0xFFFFFFFF - (1 << 20), 0xFFFFFFFF)
+        k[::97] = 0xFFFFFFFF
+    elif case == "from_2^31":
+        k = orc.gen_range(np.uint32, 36, 1, n, 1 << 31, (1 << 31) + 300_000)
+    elif case == "ragged_8193":
+        n = 8193
+        k = orc.gen_range(np.uint32, 36, 1, n, 0, 1 << 20)
+        v = v[:n]
+    else:
+        base = np.arange(n, dtype=np.uint64) % 2048
+        k = (base * np.uint64(8192) + np.uint64(8191) * (np.arange(n) % 2).astype(np.uint64)).astype(np.uint32)
+    exp = orc.sort_aggr(k, v)
+    dk, dv = ctx.to_device(k), ctx.to_device(v)
+    ctx.set_tune("aggr_small", 0)   # straight to the many-groups paths
+    try:
+        for fn in (ops.hash_aggr, ops.sort_aggr):
+            got, dense = _dense_ran(ctx, fn, dk, dv)
+            assert dense == (case != "span_2^24_plus_1"), (case, fn.__name__)
+            if fn is ops.hash_aggr:
+                got = canon(*got)
+            assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (case, fn.__name__)
+    finally:
+        ctx.clear_tune("aggr_small")
 
 
 def test_aggr_capacity_short(ctx, orc):
@@ -317,6 +373,7 @@ def test_hash_aggr_many_partitions(orc):
     k = orc.gen_range(np.uint32, 41, 1, n, 0, n // 3)
     v = orc.gen_range(np.int64, 41, 2, n, -(1 << 40), 1 << 40)
     with pimdal_amd.Context(0) as c2:
+        c2.set_tune("aggr_dense_max", 0)   # the hashed partitions, not the key-range path
         c2.set_tune("gb_rows_per_part", 16)
         c2.set_tune("gb_groups_hint", n // 3)
         got = agg(c2, ops.hash_aggr, k, v)
@@ -333,6 +390,7 @@ def test_hash_aggr_partitions_over_the_row_limit_between_lds_ones(orc):
     k = orc.gen_range(np.uint32, 43, 1, n, 0, n // 2)
     v = orc.gen_range(np.int64, 43, 2, n, -(1 << 40), 1 << 40)
     with pimdal_amd.Context(0) as c2:
+        c2.set_tune("aggr_dense_max", 0)   # the hashed partitions, not the key-range path
         c2.set_tune("gb_rows_per_part", 16)
         c2.set_tune("gb_lds_max_rows", 20)
         c2.set_tune("gb_groups_hint", n // 2)
