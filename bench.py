@@ -131,7 +131,8 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
-           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "as_table", "as_fold",
+           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "q4_osel", "q4_orders",
+           "q4_lineitem", "q4_count", "q4_emit", "as_table", "as_fold",
            "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",
            "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "radix_bstart", "radix_chunks",
            "radix_bucket", "q1_status")
@@ -252,6 +253,33 @@ def bench_q3(ctx, steps, warmup, n_lineitem=60_000_000):
            "rows_per_s": (nc + no + nl) / tm, "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
            "kernels": kernel_breakdown(ctx, steps)}
     for a in list(d.values()) + list(out["r"][0].values()):
+        a.free()
+    return res
+
+
+def bench_q4(ctx, steps, warmup, n_lineitem=60_000_000):
+    """TPC-H Q4 shape at SF10's sizes (6e7 lineitems, 1.5e7 orders;
+    host-generated synthetic tables copied to HBM before timing): orders of
+    1993-07-01 .. 1993-09-30 in a table, one lineitem pass marking the
+    orders with a late lineitem, counts per priority.  Algorithmic bytes:
+    each input column read once (orders 9 B, lineitems 12 B per row)."""
+    import numpy as np
+    from pimdal_amd import operators as ops, datagen
+    t = datagen.q4_tables(n_lineitem)
+    d = {k: ctx.to_device(v) for k, v in t.items()}
+    lo, hi = (int(x) for x in datagen.pack_days(np.array([8582, 8674])))
+    no, nl = t["o_orderkey"].size, n_lineitem
+    out = {}
+
+    def step():
+        out["r"] = ops.tpch_q4(ctx, d, lo, hi)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    alg = 9 * no + 12 * nl
+    res = {"lineitem": nl, "orders": no, "groups": out["r"][2], "s_per_step": tm, "rows_per_s": (no + nl) / tm,
+           "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
+           "kernels": kernel_breakdown(ctx, steps)}
+    for a in d.values():
         a.free()
     return res
 
@@ -447,6 +475,7 @@ def _cpu_cases():
 
     q3_date = int(datagen.pack_days(np.array([9204]))[0])
     q3_seg = datagen.Q3_SEGMENTS.index("BUILDING")
+    q4_lo, q4_hi = (int(x) for x in datagen.pack_days(np.array([8582, 8674])))
     cases = [
         ("sort_2^28", "2^24 uniform (u32 key, u32 row id) pairs, stable sort", 1 << 24,
          lambda: orc.sort_workload(1 << 24), orc.sort_pairs),
@@ -482,6 +511,8 @@ def _cpu_cases():
         ("q6", "the whole 6e7-row fact table, TPC-H Q6 predicate", 60_000_000, fact, lambda t: orc.q6(t)),
         ("q3_sf10", "7.5e6 lineitems, 1.875e6 orders, 1.875e5 customers (SF 1.25), top 10", 7_500_000 + 1_875_000 + 187_500,
          lambda: (datagen.q3_tables(7_500_000),), lambda t: orc.q3(t, q3_seg, q3_date, cap=10)),
+        ("q4_sf10", "1.5e7 lineitems, 3.75e6 orders (SF 2.5), orders of 1993-07-01 .. 1993-09-30",
+         15_000_000 + 3_750_000, lambda: (datagen.q4_tables(15_000_000),), lambda t: orc.q4(t, q4_lo, q4_hi)),
         ("date32_pack_6e7", "2^20 date32 values over 1992-1998 (the C library's gmtime_r per value, which "
          "takes glibc's time-zone lock)", 1 << 20,
          lambda: (orc.gen_range(np.int32, 0xDA7E, 1, 1 << 20, 8035, 8035 + 2556),), orc.date32_pack),
@@ -593,7 +624,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", default="")
+    ap.add_argument("--only", default="", help="comma-separated operator groups (zipf, sortselect, join, bloom, "
+                    "groupby, aggr, q3, q4, ingest, serve); the 2^28 sort always runs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="append", default=[], help="tuning knob key=value (repeatable)")
     ap.add_argument("--calibrate", action="store_true",
@@ -614,6 +646,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baselines()
 
+    only = {o for o in args.only.split(",") if o}
+    sel = lambda *names: not only or "all" in only or any(n in only for n in names)
     import pimdal_amd
     ctx = pimdal_amd.Context(0)
     for kv in args.tune:
@@ -629,13 +663,13 @@ def main():
         a.free(); b.free()
     log("bench: sort 2^28 uniform")
     srt = bench_sort(ctx, args.steps, args.warmup)
-    if args.only in ("", "all", "zipf"):
+    if sel("zipf"):
         log("bench: sort 2^28 zipf")
         ops_out["sort_zipf_2^28"] = bench_sort(ctx, max(3, args.steps // 2), 1, zipf=True)
-    if args.only in ("", "all", "sortselect"):
+    if sel("sortselect"):
         log("bench: select 2^27")
         ops_out["select_2^27"] = bench_select(ctx, args.steps, args.warmup)
-    if args.only in ("", "all", "join"):
+    if sel("join"):
         for which in ("hash", "smj"):
             for z in (False, True):
                 log(f"bench: {which} join 2^24 x 2^27 zipf={z}")
@@ -647,11 +681,11 @@ def main():
             ops_out[f"hash_join_build2^20_mode{mode}"] = bench_join(ctx, max(3, args.steps // 2), 1, "hash", False,
                                                                     nr=1 << 20)
         ctx.set_tune("hj_mode", -1)
-    if args.only in ("", "all", "bloom"):
+    if sel("bloom"):
         for nb, bits in ((1 << 16, 5), (1 << 20, 9)):
             log(f"bench: bloom semi-join 2^{nb.bit_length() - 1} x 2^27, {1 << bits} filters")
             ops_out[f"bloom_semi_join_2^{nb.bit_length() - 1}x2^27"] = bench_bloom(ctx, args.steps, args.warmup, nb, bits)
-    if args.only in ("", "all", "groupby"):
+    if sel("groupby"):
         for by_key in (False, True):
             log(f"bench: filter+group-by 6e7 by_key={by_key}")
             ops_out["q1_orderkey" if by_key else "q1_flags"] = bench_q1(ctx, max(3, args.steps // 2), 1, by_key)
@@ -660,18 +694,21 @@ def main():
                                                                                          by_key)
         log("bench: TPC-H Q6 shape 6e7")
         ops_out["q6"] = bench_q6(ctx, max(3, args.steps // 2), 1)
-    if args.only in ("", "all", "aggr"):
+    if sel("aggr"):
         for which, shape in (("hash", "k50"), ("hash", "k4000"), ("hash", "k2^24"), ("sort", "k25"),
                              ("sort", "k2^24")):
             log(f"bench: {which} aggregation 2^27 keys {AGGR_SHAPES[shape]}")
             ops_out[f"{which}_aggr_2^27_{shape}"] = bench_aggr(ctx, max(3, args.steps // 2), 1, which, shape)
-    if args.only in ("", "all", "q3"):
+    if sel("q3"):
         log("bench: TPC-H Q3 shape, 6e7 lineitems")
         ops_out["q3_sf10"] = bench_q3(ctx, max(3, args.steps // 2), 1)
-    if args.only in ("", "all", "ingest"):
+    if sel("q3", "q4"):
+        log("bench: TPC-H Q4 shape, 6e7 lineitems")
+        ops_out["q4_sf10"] = bench_q4(ctx, max(3, args.steps // 2), 1)
+    if sel("ingest"):
         log("bench: date32 to packed dates 6e7")
         ops_out["date32_pack_6e7"] = bench_date32(ctx, args.steps, args.warmup)
-    if args.only in ("", "all", "serve"):
+    if sel("serve"):
         log("bench: serving mixed stream")
         ops_out["serve_mixed_stream"] = bench_serve(ctx)
         # the same stream with batched joins probing the key index per row

@@ -416,6 +416,30 @@ typedef struct {
 int pimdal_tpch_q3_dev(pimdal_ctx* ctx, const pimdal_q3_tables* tables, uint32_t segment, int32_t date,
                        const pimdal_q3_result* out, uint64_t capacity, uint64_t* n_groups);
 
+/* TPC-H Q4 shape: pimdal/main/tpc_h/query4/host/host_q4.cpp:303-330 with its
+ * kernels kernel_q4_1.c:161-201 (orders with date_start <= o_orderdate <
+ * date_end), kernel_q4_2.c:104-140 (lineitems with l_commitdate <
+ * l_receiptdate), kernel_q4_3.c:144-209 (join on orderkey, one row per
+ * order) and kernel_q4_4.c:72-99 with host_q4.cpp:211-231 (orders counted
+ * per o_orderpriority, ordered by priority).  Dates are packed dates; the
+ * priority is a dictionary code (the reference groups 16-byte strings; a
+ * sorted dictionary keeps their order).  o_orderkey must be unique among
+ * the qualifying orders (PIMDAL_E_INVALID otherwise).  Writes the first
+ * min(groups, capacity) (priority, order count) rows in priority order;
+ * *n_groups = priorities with at least one order.  Synchronous. */
+typedef struct {
+    const int32_t* o_orderkey;
+    const int32_t* o_orderdate;
+    const uint8_t* o_orderpriority;
+    uint64_t n_orders;
+    const int32_t* l_orderkey;
+    const int32_t* l_commitdate;
+    const int32_t* l_receiptdate;
+    uint64_t n_lineitem;
+} pimdal_q4_tables;
+int pimdal_tpch_q4_dev(pimdal_ctx* ctx, const pimdal_q4_tables* tables, int32_t date_start, int32_t date_end,
+                       uint8_t* out_priority, int64_t* out_count, uint64_t capacity, uint64_t* n_groups);
+
 /* ---------------- captured plans ----------------
  * A plan records a sequence of asynchronous operator calls on the context's
  * stream as one hipGraph and replays it: the query plans of

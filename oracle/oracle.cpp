@@ -749,6 +749,41 @@ extern "C" int64_t orc_q3(const int32_t* c_key, const uint8_t* c_seg, uint64_t n
     return (int64_t)rows.size();
 }
 
+/* ------------------------------------------------------------------ */
+/* TPC-H Q4 shape.                                                     */
+/* ------------------------------------------------------------------ */
+
+/* pimdal/main/tpc_h/query4/dpu/kernel_q4_1.c:161-164 (orders with
+ * date_start <= o_orderdate < date_end), kernel_q4_2.c:104-106 (lineitems
+ * with l_commitdate < l_receiptdate), kernel_q4_3.c:144-209 (join on
+ * orderkey, one row per order), kernel_q4_4.c:72-99 and
+ * host_q4.cpp:211-231 (orders counted per o_orderpriority, summed over the
+ * devices, ordered by priority).  The priority is a dictionary code (the
+ * reference groups 16-byte strings).  Orderkeys of the qualifying orders
+ * must be unique; returns -1 when one repeats.  count[code] = qualifying
+ * orders of that priority with at least one qualifying lineitem; returns
+ * the number of codes with a count. */
+extern "C" int64_t orc_q4(const int32_t* o_key, const int32_t* o_date, const uint8_t* o_prio, uint64_t no,
+                          const int32_t* l_key, const int32_t* l_commit, const int32_t* l_receipt, uint64_t nl,
+                          int32_t date_start, int32_t date_end, int64_t* count /* [256] */) {
+    std::unordered_map<int32_t, uint64_t> order_row;   // qualifying orderkey -> order row
+    for (uint64_t i = 0; i < no; i++) {
+        if (!(o_date[i] >= date_start && o_date[i] < date_end)) continue;
+        if (!order_row.emplace(o_key[i], i).second) return -1;
+    }
+    std::unordered_map<uint64_t, char> exists;
+    for (uint64_t i = 0; i < nl; i++) {
+        if (!(l_commit[i] < l_receipt[i])) continue;
+        auto it = order_row.find(l_key[i]);
+        if (it != order_row.end()) exists[it->second] = 1;
+    }
+    for (int c = 0; c < 256; c++) count[c] = 0;
+    for (const auto& kv : exists) count[o_prio[kv.first]]++;
+    int64_t g = 0;
+    for (int c = 0; c < 256; c++) g += count[c] ? 1 : 0;
+    return g;
+}
+
 extern "C" int orc_num_threads(void) { return omp_get_max_threads(); }
 extern "C" void orc_set_threads(int t) { omp_set_num_threads(t); }
 

@@ -47,6 +47,7 @@ _SIG = {
     "orc_bloom_probe": (u64, [vp, vp, u64, u32, u32, vp, vp, vp]),
     "orc_pim_hash": (u32, [C.c_int, u32]),
     "orc_q3": (i64, [vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, u32, i32, u64, vp, vp, vp, vp]),
+    "orc_q4": (i64, [vp, vp, vp, u64, vp, vp, vp, u64, i32, i32, vp]),
     "orc_num_threads": (C.c_int, []),
     "orc_set_threads": (None, [C.c_int]),
 }
@@ -387,3 +388,21 @@ def q3(t, segment, date, cap=None):
         raise ValueError("o_orderkey values repeat")
     k = min(g, cap)
     return {f: v[:k] for f, v in o.items()}, g
+
+
+Q4_COLS = {"orders": ("o_orderkey", "o_orderdate", "o_orderpriority"),
+           "lineitem": ("l_orderkey", "l_commitdate", "l_receiptdate")}
+
+
+def q4(t, date_start, date_end):
+    """TPC-H Q4: (priority codes with a count, in code order; their order
+    counts)."""
+    c = {k: np.ascontiguousarray(t[k]) for cols in Q4_COLS.values() for k in cols}
+    cnt = np.zeros(256, np.int64)
+    g = lib().orc_q4(_p(c["o_orderkey"]), _p(c["o_orderdate"]), _p(c["o_orderpriority"]), c["o_orderkey"].size,
+                     _p(c["l_orderkey"]), _p(c["l_commitdate"]), _p(c["l_receiptdate"]), c["l_orderkey"].size,
+                     int(date_start), int(date_end), _p(cnt))
+    if g < 0:
+        raise ValueError("o_orderkey values repeat")
+    codes = np.nonzero(cnt)[0].astype(np.uint8)
+    return codes, cnt[codes]

@@ -90,5 +90,7 @@ void aggr_cols_nullable(pimdal_ctx* c, const pimdal_col* key, const pimdal_col* 
 // cap) groups written in TPC-H order.
 uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int32_t date, const pimdal_q3_result& out,
                  uint64_t cap, hipStream_t s);
+uint64_t tpch_q4(pimdal_ctx* c, const pimdal_q4_tables& t, int32_t date_start, int32_t date_end, uint8_t* out_prio,
+                 int64_t* out_count, uint64_t cap, hipStream_t s);
 
 }  // namespace pd

@@ -1,5 +1,6 @@
 // q3.hip — the TPC-H Q3 shape: two joins chained on the device with the
-// group-by fused into the second join's probe.
+// group-by fused into the second join's probe; and the TPC-H Q4 shape (a
+// semi-join of orders with lineitems, counted per priority), further down.
 //
 // Reference semantics: pimdal/main/tpc_h/query3/host/host_q3.cpp:471-519
 // (five DPU programs launched in turn, the host collecting and
@@ -337,6 +338,185 @@ uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int
     return g;
 }
 
+// ===================== TPC-H Q4 shape =====================
+// Reference: pimdal/main/tpc_h/query4/host/host_q4.cpp:303-330 runs four DPU
+// programs with the host collecting and redistributing rows between them:
+// kernel_q4_1.c:161-201 selects the orders with date_start <= o_orderdate <
+// date_end, kernel_q4_2.c:104-140 the lineitems with l_commitdate <
+// l_receiptdate, kernel_q4_3.c:144-209 hash-joins them on orderkey and keeps
+// one row per order (group by orderkey), kernel_q4_4.c:72-99 counts per
+// o_orderpriority and host_q4.cpp:211-231 sums the devices' counts and
+// orders them by priority.  Plan on gfx950:
+//  1. k_q4_osel      the qualifying orders' rows listed (one pass);
+//  2. k_q3_orders    the orders table (orderkey, order row) sized for them;
+//  3. k_q4_lineitem  one pass over the lineitems: commitdate < receiptdate,
+//                    probe, and a hit sets the slot's hit bit (the EXISTS:
+//                    an order is counted once however many lineitems match);
+//  4. k_q4_count     hit slots counted per priority code (LDS counters, one
+//                    device atomic per code per block);
+//  5. k_q4_emit      the codes with a count, in code order (a sorted
+//                    dictionary keeps the priority strings' order).
+
+__global__ __launch_bounds__(256) void k_q4_osel(const int32_t* __restrict__ odate, uint64_t n, int32_t lo, int32_t hi,
+                                                 unsigned long long* __restrict__ ctr, uint32_t* __restrict__ rows) {
+    constexpr int PER = Q3_CHUNK / 256;   // 32 rows per thread, bit j: row r0 + j * 256 + t
+    __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_base;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint64_t r0 = (uint64_t)blockIdx.x * Q3_CHUNK;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint64_t i = r0 + (uint64_t)j * 256 + t;
+        if (i < n) {
+            const int32_t d = odate[i];
+            if (d >= lo && d < hi) bits |= 1u << j;
+        }
+    }
+    const uint32_t cnt = __popc(bits);
+    const uint32_t inc = pd::wave_incl_scan(cnt);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = inc - cnt, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        before += (i < (int)w) ? s_w[i] : 0u;
+        tot += s_w[i];
+    }
+    if (t == 0) s_base = tot ? atomicAdd(ctr, (unsigned long long)tot) : 0ull;
+    __syncthreads();
+    uint64_t o = s_base + before;
+    while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1;
+        rows[o++] = (uint32_t)(r0 + (uint64_t)j * 256 + t);
+    }
+}
+
+// Lineitems with l_commitdate < l_receiptdate whose order is in the table
+// set the slot's hit bit.  Four rows per thread, all probes in flight.
+__global__ __launch_bounds__(256) void k_q4_lineitem(const int32_t* __restrict__ lkey,
+                                                     const int32_t* __restrict__ lcommit,
+                                                     const int32_t* __restrict__ lreceipt, uint64_t n,
+                                                     const uint64_t* __restrict__ otab, uint64_t omask,
+                                                     uint32_t* __restrict__ hit) {
+    constexpr int R = 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; base < n; base += stride * R) {
+        uint32_t key[R];
+        bool want[R];
+        uint64_t h[R], e[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t i = base + (uint64_t)r * stride;
+            want[r] = i < n && lcommit[i] < lreceipt[i];
+            key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            h[r] = q3_slot(key[r], omask);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) e[r] = want[r] ? otab[h[r]] : 0ull;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!want[r]) continue;
+            while (e[r] != 0 && (uint32_t)e[r] != key[r]) {
+                h[r] = (h[r] + 1) & omask;
+                e[r] = otab[h[r]];
+            }
+            if (e[r] == 0) continue;
+            const uint32_t bit = 1u << (h[r] & 31);
+            if (!(hit[h[r] >> 5] & bit)) atomicOr(&hit[h[r] >> 5], bit);
+        }
+    }
+}
+
+// Hit slots per priority code: thread = one hit word; LDS counters, then
+// one device add per code with a count in the block.
+__global__ __launch_bounds__(256) void k_q4_count(const uint64_t* __restrict__ otab, const uint32_t* __restrict__ hit,
+                                                  uint64_t words, const uint8_t* __restrict__ oprio,
+                                                  unsigned long long* __restrict__ cnt) {
+    __shared__ uint32_t s_c[256];
+    const uint32_t t = threadIdx.x;
+    s_c[t] = 0;
+    __syncthreads();
+    const uint64_t wd = (uint64_t)blockIdx.x * 256 + t;
+    uint32_t bits = wd < words ? hit[wd] : 0u;
+    while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1;
+        const uint64_t e = otab[wd * 32 + (uint64_t)j];
+        atomicAdd(&s_c[oprio[(e >> 32) - 1]], 1u);
+    }
+    __syncthreads();
+    if (s_c[t]) atomicAdd(&cnt[t], (unsigned long long)s_c[t]);
+}
+
+// Codes with a count, in code order: thread t = code t; at most cap rows
+// written; meta[0] = the number of such codes.
+__global__ __launch_bounds__(256) void k_q4_emit(const unsigned long long* __restrict__ cnt, uint8_t* __restrict__ oprio,
+                                                 int64_t* __restrict__ ocount, uint64_t cap,
+                                                 unsigned long long* __restrict__ meta) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const unsigned long long c = cnt[t];
+    const uint32_t f = c ? 1u : 0u;
+    const uint32_t inc = pd::wave_incl_scan(f);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = inc - f, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        before += (i < (int)w) ? s_w[i] : 0u;
+        tot += s_w[i];
+    }
+    if (f && before < cap) {
+        oprio[before] = (uint8_t)t;
+        ocount[before] = (int64_t)c;
+    }
+    if (t == 0) meta[0] = tot;
+}
+
+uint64_t tpch_q4(pimdal_ctx* c, const pimdal_q4_tables& t, int32_t date_start, int32_t date_end, uint8_t* out_prio,
+                 int64_t* out_count, uint64_t cap, hipStream_t s) {
+    PD_REQUIRE(t.o_orderkey && t.o_orderdate && t.o_orderpriority && t.l_orderkey && t.l_commitdate &&
+                   t.l_receiptdate,
+               PIMDAL_E_INVALID, "missing input column");
+    PD_REQUIRE(!cap || (out_prio && out_count), PIMDAL_E_INVALID, "missing output column");
+    PD_REQUIRE(t.n_orders < (1ull << 30) && t.n_lineitem < (1ull << 40), PIMDAL_E_INVALID,
+               "tables too large (at most 2^30-1 orders)");
+    // small: [0] qualifying orders, [1] emitted codes, [2] repeated-key flag, [3..259) counts per code
+    Scratch small(c, (3 + 256) * 8, s);
+    fill_u64(c, small.get(), 0, 3 + 256, s);
+    unsigned long long* meta = small.as<unsigned long long>();
+    uint32_t* dup = reinterpret_cast<uint32_t*>(meta + 2);
+    unsigned long long* cnt = meta + 3;
+    // 1. qualifying orders, 2. their table
+    Scratch orows(c, std::max<uint64_t>(t.n_orders, 1) * 4, s);
+    launch(c, "q4_osel", k_q4_osel, dim3((uint32_t)std::max<uint64_t>(1, ceil_div(t.n_orders, Q3_CHUNK))), dim3(256), 0,
+           s, t.o_orderdate, t.n_orders, date_start, date_end, meta, orows.as<uint32_t>());
+    uint64_t nq = 0;
+    read_small(c, &nq, meta, 8, s);
+    const uint64_t oslots = q3_pow2_slots(nq);
+    Scratch otab(c, oslots * 8, s), hit(c, oslots / 8, s);
+    fill_u64(c, otab.get(), 0, oslots, s);
+    fill_u32(c, hit.get(), 0, oslots / 32, s);
+    launch(c, "q4_orders", k_q3_orders, q3_grid(c, nq), dim3(256), 0, s, (const uint32_t*)orows.as<uint32_t>(), nq,
+           t.o_orderkey, otab.as<uint64_t>(), oslots - 1, dup);
+    // 3. lineitems mark their orders
+    launch(c, "q4_lineitem", k_q4_lineitem, q3_grid(c, ceil_div(t.n_lineitem, 4)), dim3(256), 0, s, t.l_orderkey,
+           t.l_commitdate, t.l_receiptdate, t.n_lineitem, (const uint64_t*)otab.as<uint64_t>(), oslots - 1,
+           hit.as<uint32_t>());
+    // 4. counts per priority, 5. in priority order
+    launch(c, "q4_count", k_q4_count, dim3((uint32_t)ceil_div(oslots / 32, 256)), dim3(256), 0, s,
+           (const uint64_t*)otab.as<uint64_t>(), (const uint32_t*)hit.as<uint32_t>(), oslots / 32, t.o_orderpriority,
+           cnt);
+    launch(c, "q4_emit", k_q4_emit, dim3(1), dim3(256), 0, s, (const unsigned long long*)cnt, out_prio, out_count, cap,
+           meta + 1);
+    uint64_t hv[3] = {0, 0, 0};
+    read_small(c, hv, meta, 24, s);
+    PD_REQUIRE((uint32_t)hv[2] == 0, PIMDAL_E_INVALID, "o_orderkey values must be unique");
+    return hv[1];
+}
+
 }  // namespace pd
 
 using namespace pd;
@@ -347,5 +527,14 @@ extern "C" int pimdal_tpch_q3_dev(pimdal_ctx* c, const pimdal_q3_tables* tables,
     PD_REQUIRE(c && tables && out && n_groups, PIMDAL_E_INVALID, "NULL argument");
     PD_NO_CAPTURE(c);
     *n_groups = tpch_q3(c, *tables, segment, date, *out, capacity, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_tpch_q4_dev(pimdal_ctx* c, const pimdal_q4_tables* tables, int32_t date_start, int32_t date_end,
+                                  uint8_t* out_priority, int64_t* out_count, uint64_t capacity, uint64_t* n_groups) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && tables && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
+    *n_groups = tpch_q4(c, *tables, date_start, date_end, out_priority, out_count, capacity, c->stream);
     PD_API_END
 }

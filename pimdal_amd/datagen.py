@@ -167,3 +167,31 @@ def q3_tables(n_lineitem: int, seed: int = 0x5EED0006) -> dict:
         "l_orderkey": o_key[l_orow], "l_extendedprice": rng.integers(90000, 10500001, nl).astype(np.int64),
         "l_discount": rng.integers(0, 11, nl).astype(np.int64), "l_shipdate": pack_days(l_days),
     }
+
+
+Q4_PRIORITIES = ("1-URGENT", "2-HIGH", "3-MEDIUM", "4-NOT SPECIFIED", "5-LOW")   # dictionary order
+
+
+def q4_tables(n_lineitem: int, seed: int = 0x5EED0007) -> dict:
+    """Synthetic orders / lineitem columns in TPC-H's shape for Q4: 4
+    lineitems per order on average (SF10 at 6e7 lineitems), sparse unique
+    orderkeys, order dates uniform over 1992-01-01 .. 1998-08-02, commit
+    dates 30..90 days after the order, receipt dates 1..30 days after a ship
+    date 1..121 days after the order (dbgen's ranges), priority codes
+    indexing Q4_PRIORITIES.  Dates packed."""
+    rng = np.random.default_rng(seed)
+    nl = int(n_lineitem)
+    no = max(1, nl // 4)
+    start = 8035                      # 1992-01-01
+    span = 2405 - 151 + 1
+    o_key = (np.arange(no, dtype=np.int64) * 4 + rng.integers(0, 4, no)).astype(np.int32)
+    o_key = o_key[rng.permutation(no)]
+    o_days = start + rng.integers(0, span, no)
+    l_orow = np.sort(rng.integers(0, no, nl))
+    ship = o_days[l_orow] + rng.integers(1, 122, nl)
+    return {
+        "o_orderkey": o_key, "o_orderdate": pack_days(o_days),
+        "o_orderpriority": rng.integers(0, len(Q4_PRIORITIES), no).astype(np.uint8),
+        "l_orderkey": o_key[l_orow], "l_commitdate": pack_days(o_days[l_orow] + rng.integers(30, 91, nl)),
+        "l_receiptdate": pack_days(ship + rng.integers(1, 31, nl)),
+    }

@@ -339,3 +339,24 @@ def tpch_q3(ctx: Context, t: dict, segment: int, date: int, capacity: int = 10, 
         return out, g.value
     k = min(g.value, cap)
     return {f: a.numpy(k) for f, a in out.items()}, g.value
+
+
+class _Q4Tables(C.Structure):
+    _fields_ = [("o_orderkey", C.c_void_p), ("o_orderdate", C.c_void_p), ("o_orderpriority", C.c_void_p),
+                ("n_orders", C.c_uint64), ("l_orderkey", C.c_void_p), ("l_commitdate", C.c_void_p),
+                ("l_receiptdate", C.c_void_p), ("n_lineitem", C.c_uint64)]
+
+
+def tpch_q4(ctx: Context, t: dict, date_start: int, date_end: int, capacity: int = 256):
+    """TPC-H Q4 shape over device columns `t` (orders, lineitem): priority
+    codes with at least one order, in code order, their order counts, and
+    the group count.  pimdal/main/tpc_h/query4/host/host_q4.cpp:303-330."""
+    tab = _Q4Tables(t["o_orderkey"].ptr, t["o_orderdate"].ptr, t["o_orderpriority"].ptr, t["o_orderkey"].n,
+                    t["l_orderkey"].ptr, t["l_commitdate"].ptr, t["l_receiptdate"].ptr, t["l_orderkey"].n)
+    cap = max(1, int(capacity))
+    prio, cnt = ctx.empty(cap, np.uint8), ctx.empty(cap, np.int64)
+    g = C.c_uint64()
+    call("pimdal_tpch_q4_dev", ctx.handle, C.byref(tab), int(date_start), int(date_end), prio.ptr, cnt.ptr, cap,
+         C.byref(g))
+    k = min(g.value, cap)
+    return prio.numpy(k), cnt.numpy(k), g.value

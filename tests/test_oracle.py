@@ -432,3 +432,66 @@ def test_packed_dates_of_the_generator_match_the_oracle(orc):
     from pimdal_amd import datagen
     days = np.concatenate([np.arange(8035, 8035 + 2600), np.array([-719162, 0, 11016, 2932896])]).astype(np.int32)
     assert np.array_equal(datagen.pack_days(days), orc.date32_pack(days))
+
+
+def _q4_brute(t, lo, hi):
+    """The SQL of the Q4 shape by nested loops (tiny inputs): orders with
+    lo <= o_orderdate < hi for which EXISTS a lineitem of that order with
+    l_commitdate < l_receiptdate, counted per o_orderpriority, in priority
+    order."""
+    counts = {}
+    for oi in range(len(t["o_orderkey"])):
+        if not (lo <= int(t["o_orderdate"][oi]) < hi):
+            continue
+        if any(int(t["l_orderkey"][li]) == int(t["o_orderkey"][oi]) and
+               int(t["l_commitdate"][li]) < int(t["l_receiptdate"][li]) for li in range(len(t["l_orderkey"]))):
+            p = int(t["o_orderpriority"][oi])
+            counts[p] = counts.get(p, 0) + 1
+    return sorted(counts.items())
+
+
+def _q4_tiny(seed):
+    rng = np.random.default_rng(seed)
+    no, nl = 40, 120
+    t = {"o_orderkey": (rng.permutation(no) * 7 - 40).astype(np.int32),
+         "o_orderdate": rng.integers(100, 200, no).astype(np.int32),
+         "o_orderpriority": rng.integers(0, 5, no).astype(np.uint8)}
+    t["l_orderkey"] = t["o_orderkey"][rng.integers(0, no, nl)]
+    t["l_orderkey"][:4] = 12345                                  # no such order
+    t["l_commitdate"] = rng.integers(100, 200, nl).astype(np.int32)
+    t["l_receiptdate"] = rng.integers(100, 200, nl).astype(np.int32)
+    t["l_receiptdate"][4:8] = t["l_commitdate"][4:8]             # equal dates do not qualify
+    return t
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("lo,hi", [(100, 200), (120, 150), (150, 151), (300, 400)])
+def test_q4_oracle_matches_brute_force(orc, seed, lo, hi):
+    t = _q4_tiny(seed)
+    exp = _q4_brute(t, lo, hi)
+    codes, counts = orc.q4(t, lo, hi)
+    assert [(int(c), int(n)) for c, n in zip(codes, counts)] == exp
+
+
+def test_q4_oracle_matches_pandas(orc):
+    """Medium tables: the oracle against pandas' semi-join and group-by."""
+    import pandas as pd
+    from pimdal_amd import datagen
+    t = datagen.q4_tables(200_000, seed=6)
+    lo, hi = (int(x) for x in datagen.pack_days(np.array([8582, 8674])))   # 1993-07-01, 1993-10-01
+    o = pd.DataFrame({k: t[k] for k in ("o_orderkey", "o_orderdate", "o_orderpriority")})
+    li = pd.DataFrame({k: t[k] for k in ("l_orderkey", "l_commitdate", "l_receiptdate")})
+    keys = li[li.l_commitdate < li.l_receiptdate].l_orderkey.unique()
+    q = o[(o.o_orderdate >= lo) & (o.o_orderdate < hi) & o.o_orderkey.isin(keys)]
+    g = q.groupby("o_orderpriority").size().sort_index()
+    codes, counts = orc.q4(t, lo, hi)
+    assert len(g) == 5 and counts.sum() > 1000
+    assert np.array_equal(codes, g.index.to_numpy().astype(np.uint8)) and np.array_equal(counts, g.to_numpy())
+
+
+def test_q4_oracle_refuses_repeated_orderkeys(orc):
+    t = _q4_tiny(1)
+    t["o_orderkey"][1] = t["o_orderkey"][0]
+    t["o_orderdate"][:2] = 150
+    with pytest.raises(ValueError):
+        orc.q4(t, 100, 200)
