@@ -132,7 +132,7 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
            "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "q4_osel", "q4_orders",
-           "q4_lineitem", "q4_count", "q4_emit", "as_table", "as_fold",
+           "q4_bits", "q4_lineitem", "q4_count", "q4_emit", "as_table", "as_fold",
            "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",
            "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "radix_bstart", "radix_chunks",
            "radix_bucket", "q1_status")

@@ -349,9 +349,11 @@ uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int
 // orders them by priority.  Plan on gfx950:
 //  1. k_q4_osel      the qualifying orders' rows listed (one pass);
 //  2. k_q3_orders    the orders table (orderkey, order row) sized for them;
-//  3. k_q4_lineitem  one pass over the lineitems: commitdate < receiptdate,
-//                    probe, and a hit sets the slot's hit bit (the EXISTS:
-//                    an order is counted once however many lineitems match);
+//  3. k_q4_bits / k_q4_lineitem  a filter bitmap of the qualifying orders'
+//                    keys (at most 2 MB, L2-resident), then one pass over the
+//                    lineitems: commitdate < receiptdate, the filter bit, the
+//                    table probe, and a hit sets the slot's hit bit (the
+//                    EXISTS: an order counts once however many lineitems match);
 //  4. k_q4_count     hit slots counted per priority code (LDS counters, one
 //                    device atomic per code per block);
 //  5. k_q4_emit      the codes with a count, in code order (a sorted
@@ -393,17 +395,37 @@ __global__ __launch_bounds__(256) void k_q4_osel(const int32_t* __restrict__ oda
     }
 }
 
+// Filter bit of an orderkey (the high half of the table's hash, so it is
+// independent of the slot).
+__device__ __forceinline__ uint32_t q4_bit(uint32_t key, uint32_t bmask) {
+    return (uint32_t)(pd_mix64_dev(key) >> 32) & bmask;
+}
+
+// The qualifying orders' filter bits (a bitmap of at most 2 MB that stays
+// in L2, in front of the table).
+__global__ void k_q4_bits(const uint32_t* __restrict__ rows, uint64_t nq, const int32_t* __restrict__ okey,
+                          uint32_t* __restrict__ bits, uint32_t bmask) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nq; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = q4_bit((uint32_t)okey[rows[j]], bmask);
+        atomicOr(&bits[b >> 5], 1u << (b & 31));
+    }
+}
+
 // Lineitems with l_commitdate < l_receiptdate whose order is in the table
-// set the slot's hit bit.  Four rows per thread, all probes in flight.
+// set the slot's hit bit.  Most lineitems' orders do not qualify; the
+// filter bitmap (an L2 hit) turns those away before the table (a random
+// line from MALL each).  Four rows per thread, all loads of a step in
+// flight together.
 __global__ __launch_bounds__(256) void k_q4_lineitem(const int32_t* __restrict__ lkey,
                                                      const int32_t* __restrict__ lcommit,
                                                      const int32_t* __restrict__ lreceipt, uint64_t n,
+                                                     const uint32_t* __restrict__ bits, uint32_t bmask,
                                                      const uint64_t* __restrict__ otab, uint64_t omask,
                                                      uint32_t* __restrict__ hit) {
     constexpr int R = 4;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; base < n; base += stride * R) {
-        uint32_t key[R];
+        uint32_t key[R], fb[R], fw[R];
         bool want[R];
         uint64_t h[R], e[R];
 #pragma unroll
@@ -411,6 +433,13 @@ __global__ __launch_bounds__(256) void k_q4_lineitem(const int32_t* __restrict__
             const uint64_t i = base + (uint64_t)r * stride;
             want[r] = i < n && lcommit[i] < lreceipt[i];
             key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            fb[r] = q4_bit(key[r], bmask);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) fw[r] = want[r] ? bits[fb[r] >> 5] : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            want[r] = want[r] && ((fw[r] >> (fb[r] & 31)) & 1u);
             h[r] = q3_slot(key[r], omask);
         }
 #pragma unroll
@@ -496,15 +525,21 @@ uint64_t tpch_q4(pimdal_ctx* c, const pimdal_q4_tables& t, int32_t date_start, i
     uint64_t nq = 0;
     read_small(c, &nq, meta, 8, s);
     const uint64_t oslots = q3_pow2_slots(nq);
-    Scratch otab(c, oslots * 8, s), hit(c, oslots / 8, s);
+    // filter bits: about 32 per qualifying order, 2^16 to 2^24 (2 MB)
+    uint64_t nbits = 1ull << 16;
+    while (nbits < 32 * nq && nbits < (1ull << 24)) nbits <<= 1;
+    Scratch otab(c, oslots * 8, s), hit(c, oslots / 8, s), fbits(c, nbits / 8, s);
     fill_u64(c, otab.get(), 0, oslots, s);
     fill_u32(c, hit.get(), 0, oslots / 32, s);
+    fill_u32(c, fbits.get(), 0, nbits / 32, s);
     launch(c, "q4_orders", k_q3_orders, q3_grid(c, nq), dim3(256), 0, s, (const uint32_t*)orows.as<uint32_t>(), nq,
            t.o_orderkey, otab.as<uint64_t>(), oslots - 1, dup);
+    launch(c, "q4_bits", k_q4_bits, q3_grid(c, nq), dim3(256), 0, s, (const uint32_t*)orows.as<uint32_t>(), nq,
+           t.o_orderkey, fbits.as<uint32_t>(), (uint32_t)(nbits - 1));
     // 3. lineitems mark their orders
     launch(c, "q4_lineitem", k_q4_lineitem, q3_grid(c, ceil_div(t.n_lineitem, 4)), dim3(256), 0, s, t.l_orderkey,
-           t.l_commitdate, t.l_receiptdate, t.n_lineitem, (const uint64_t*)otab.as<uint64_t>(), oslots - 1,
-           hit.as<uint32_t>());
+           t.l_commitdate, t.l_receiptdate, t.n_lineitem, (const uint32_t*)fbits.as<uint32_t>(),
+           (uint32_t)(nbits - 1), (const uint64_t*)otab.as<uint64_t>(), oslots - 1, hit.as<uint32_t>());
     // 4. counts per priority, 5. in priority order
     launch(c, "q4_count", k_q4_count, dim3((uint32_t)ceil_div(oslots / 32, 256)), dim3(256), 0, s,
            (const uint64_t*)otab.as<uint64_t>(), (const uint32_t*)hit.as<uint32_t>(), oslots / 32, t.o_orderpriority,
