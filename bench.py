@@ -132,7 +132,8 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
            "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "q4_osel", "q4_orders",
-           "q4_bits", "q4_lineitem", "q4_count", "q4_emit", "as_table", "as_fold",
+           "q4_bits", "q4_lineitem", "q4_count", "q4_emit", "q5_nations", "q5_cust", "q5_osel", "q5_orders",
+           "q5_supp", "q5_lineitem", "q5_emit", "as_table", "as_fold",
            "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",
            "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "radix_bstart", "radix_chunks",
            "radix_bucket", "q1_status")
@@ -279,6 +280,34 @@ def bench_q4(ctx, steps, warmup, n_lineitem=60_000_000):
     res = {"lineitem": nl, "orders": no, "groups": out["r"][2], "s_per_step": tm, "rows_per_s": (no + nl) / tm,
            "alg_bytes": alg, "alg_GBps": alg / tm / 1e9, "hbm_frac": alg / tm / HBM_PEAK,
            "kernels": kernel_breakdown(ctx, steps)}
+    for a in d.values():
+        a.free()
+    return res
+
+
+def bench_q5(ctx, steps, warmup, n_lineitem=60_000_000):
+    """TPC-H Q5 shape at SF10's sizes (6e7 lineitems, 1.5e7 orders, 1.5e6
+    customers, 1e5 suppliers; host-generated synthetic tables copied to HBM
+    before timing): region ASIA, orders of 1994.  Algorithmic bytes: each
+    input column read once (customers 8 B, orders 12 B, lineitems 24 B,
+    suppliers 8 B per row)."""
+    import numpy as np
+    from pimdal_amd import operators as ops, datagen
+    t = datagen.q5_tables(n_lineitem)
+    d = {k: ctx.to_device(v) for k, v in t.items()}
+    lo, hi = (int(x) for x in datagen.pack_days(np.array([8766, 9131])))
+    asia = datagen.Q5_REGIONS.index("ASIA")
+    nc, no, ns, nl = t["c_custkey"].size, t["o_orderkey"].size, t["s_suppkey"].size, n_lineitem
+    out = {}
+
+    def step():
+        out["r"] = ops.tpch_q5(ctx, d, asia, lo, hi)
+
+    tm = timed_steps(ctx, step, steps, warmup)
+    alg = 8 * nc + 12 * no + 24 * nl + 8 * ns
+    res = {"lineitem": nl, "orders": no, "customer": nc, "supplier": ns, "groups": out["r"][2], "s_per_step": tm,
+           "rows_per_s": (nc + no + nl + ns) / tm, "alg_bytes": alg, "alg_GBps": alg / tm / 1e9,
+           "hbm_frac": alg / tm / HBM_PEAK, "kernels": kernel_breakdown(ctx, steps)}
     for a in d.values():
         a.free()
     return res
@@ -476,6 +505,7 @@ def _cpu_cases():
     q3_date = int(datagen.pack_days(np.array([9204]))[0])
     q3_seg = datagen.Q3_SEGMENTS.index("BUILDING")
     q4_lo, q4_hi = (int(x) for x in datagen.pack_days(np.array([8582, 8674])))
+    q5_lo, q5_hi = (int(x) for x in datagen.pack_days(np.array([8766, 9131])))
     cases = [
         ("sort_2^28", "2^24 uniform (u32 key, u32 row id) pairs, stable sort", 1 << 24,
          lambda: orc.sort_workload(1 << 24), orc.sort_pairs),
@@ -513,6 +543,9 @@ def _cpu_cases():
          lambda: (datagen.q3_tables(7_500_000),), lambda t: orc.q3(t, q3_seg, q3_date, cap=10)),
         ("q4_sf10", "1.5e7 lineitems, 3.75e6 orders (SF 2.5), orders of 1993-07-01 .. 1993-09-30",
          15_000_000 + 3_750_000, lambda: (datagen.q4_tables(15_000_000),), lambda t: orc.q4(t, q4_lo, q4_hi)),
+        ("q5_sf10", "1.5e7 lineitems, 3.75e6 orders, 3.75e5 customers, 2.5e4 suppliers (SF 2.5), region ASIA, "
+         "orders of 1994", 15_000_000 + 3_750_000 + 375_000 + 25_000, lambda: (datagen.q5_tables(15_000_000),),
+         lambda t: orc.q5(t, datagen.Q5_REGIONS.index("ASIA"), q5_lo, q5_hi)),
         ("date32_pack_6e7", "2^20 date32 values over 1992-1998 (the C library's gmtime_r per value, which "
          "takes glibc's time-zone lock)", 1 << 20,
          lambda: (orc.gen_range(np.int32, 0xDA7E, 1, 1 << 20, 8035, 8035 + 2556),), orc.date32_pack),
@@ -702,9 +735,11 @@ def main():
     if sel("q3"):
         log("bench: TPC-H Q3 shape, 6e7 lineitems")
         ops_out["q3_sf10"] = bench_q3(ctx, max(3, args.steps // 2), 1)
-    if sel("q3", "q4"):
+    if sel("q3", "q4", "q5"):
         log("bench: TPC-H Q4 shape, 6e7 lineitems")
         ops_out["q4_sf10"] = bench_q4(ctx, max(3, args.steps // 2), 1)
+        log("bench: TPC-H Q5 shape, 6e7 lineitems")
+        ops_out["q5_sf10"] = bench_q5(ctx, max(3, args.steps // 2), 1)
     if sel("ingest"):
         log("bench: date32 to packed dates 6e7")
         ops_out["date32_pack_6e7"] = bench_date32(ctx, args.steps, args.warmup)

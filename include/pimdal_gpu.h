@@ -440,6 +440,49 @@ typedef struct {
 int pimdal_tpch_q4_dev(pimdal_ctx* ctx, const pimdal_q4_tables* tables, int32_t date_start, int32_t date_end,
                        uint8_t* out_priority, int64_t* out_count, uint64_t capacity, uint64_t* n_groups);
 
+/* TPC-H Q5 shape: pimdal/main/tpc_h/query5/host/host_q5.cpp:566-640 with its
+ * kernels kernel_q5_1.c (the region of r_name, its nations, their
+ * customers), kernel_q5_2.c (orders with date_start <= o_orderdate <
+ * date_end), kernel_q5_3.c (customers join orders on custkey),
+ * kernel_q5_4.c / kernel_q5_5.c (orders join lineitems on orderkey),
+ * kernel_q5_6.c / kernel_q5_7.c (join suppliers on suppkey and nation,
+ * revenue = l_extendedprice * (100 - l_discount) summed per nation) and
+ * host_q5.cpp:459-500 (sums over the devices, revenue descending).  The
+ * region name is a dictionary code (the reference matches the substring
+ * "ASIA" in 16-byte names); nation keys must lie in [0, 1024)
+ * (PIMDAL_E_UNSUPPORTED otherwise); custkey, orderkey and suppkey must be
+ * unique among the qualifying customers, orders and the suppliers
+ * (PIMDAL_E_INVALID).  Writes the first min(groups, capacity) (nationkey,
+ * revenue) rows, revenue descending then nationkey ascending; *n_groups =
+ * nations with at least one matching lineitem.  Revenue sums wrap modulo
+ * 2^64 like the reference's int64.  Synchronous. */
+typedef struct {
+    const int32_t* r_regionkey;
+    const uint8_t* r_name;
+    uint64_t n_region;
+    const int32_t* n_nationkey;
+    const int32_t* n_regionkey;
+    uint64_t n_nation;
+    const int32_t* c_custkey;
+    const int32_t* c_nationkey;
+    uint64_t n_customer;
+    const int32_t* o_orderkey;
+    const int32_t* o_custkey;
+    const int32_t* o_orderdate;
+    uint64_t n_orders;
+    const int32_t* l_orderkey;
+    const int32_t* l_suppkey;
+    const int64_t* l_extendedprice;
+    const int64_t* l_discount;
+    uint64_t n_lineitem;
+    const int32_t* s_suppkey;
+    const int32_t* s_nationkey;
+    uint64_t n_supplier;
+} pimdal_q5_tables;
+int pimdal_tpch_q5_dev(pimdal_ctx* ctx, const pimdal_q5_tables* tables, uint32_t region_name, int32_t date_start,
+                       int32_t date_end, int32_t* out_nationkey, int64_t* out_revenue, uint64_t capacity,
+                       uint64_t* n_groups);
+
 /* ---------------- captured plans ----------------
  * A plan records a sequence of asynchronous operator calls on the context's
  * stream as one hipGraph and replays it: the query plans of

@@ -784,6 +784,61 @@ extern "C" int64_t orc_q4(const int32_t* o_key, const int32_t* o_date, const uin
     return g;
 }
 
+/* ------------------------------------------------------------------ */
+/* TPC-H Q5 shape.                                                     */
+/* ------------------------------------------------------------------ */
+
+/* pimdal/main/tpc_h/query5/dpu/kernel_q5_1.c:210-212 (the region by name,
+ * then its nations and their customers), kernel_q5_2.c:123-146 (orders with
+ * date_start <= o_orderdate < date_end), kernel_q5_3.c (customers join
+ * orders on custkey), kernel_q5_5.c (join lineitems on orderkey),
+ * kernel_q5_7.c:64-100 (join suppliers on suppkey and nationkey, revenue =
+ * l_extendedprice * (100 - l_discount)), host_q5.cpp:459-500 (summed per
+ * nation, revenue descending; ties here by nationkey).  The region name is
+ * a dictionary code.  Returns -1 when a qualifying custkey or orderkey, or a
+ * suppkey, repeats; writes at most cap rows; returns the nation count. */
+extern "C" int64_t orc_q5(const int32_t* r_key, const uint8_t* r_name, uint64_t nr, const int32_t* n_key,
+                          const int32_t* n_reg, uint64_t nn, const int32_t* c_key, const int32_t* c_nat, uint64_t nc,
+                          const int32_t* o_key, const int32_t* o_cust, const int32_t* o_date, uint64_t no,
+                          const int32_t* l_key, const int32_t* l_supp, const int64_t* l_price, const int64_t* l_disc,
+                          uint64_t nl, const int32_t* s_key, const int32_t* s_nat, uint64_t ns, uint32_t region_name,
+                          int32_t date_start, int32_t date_end, uint64_t cap, int32_t* out_nat, int64_t* out_rev) {
+    std::unordered_map<int32_t, char> nations;
+    for (uint64_t i = 0; i < nn; i++)
+        for (uint64_t r = 0; r < nr; r++)
+            if (r_name[r] == region_name && r_key[r] == n_reg[i]) nations.emplace(n_key[i], 1);
+    std::unordered_map<int32_t, int32_t> cust;   // custkey -> nation
+    for (uint64_t i = 0; i < nc; i++)
+        if (nations.count(c_nat[i]) && !cust.emplace(c_key[i], c_nat[i]).second) return -1;
+    std::unordered_map<int32_t, int32_t> ord;    // orderkey -> customer's nation
+    for (uint64_t i = 0; i < no; i++) {
+        if (!(o_date[i] >= date_start && o_date[i] < date_end)) continue;
+        auto it = cust.find(o_cust[i]);
+        if (it == cust.end()) continue;
+        if (!ord.emplace(o_key[i], it->second).second) return -1;
+    }
+    std::unordered_map<int32_t, int32_t> supp;   // suppkey -> nation
+    for (uint64_t i = 0; i < ns; i++)
+        if (!supp.emplace(s_key[i], s_nat[i]).second) return -1;
+    std::unordered_map<int32_t, int64_t> rev;
+    for (uint64_t i = 0; i < nl; i++) {
+        auto o = ord.find(l_key[i]);
+        if (o == ord.end()) continue;
+        auto sp = supp.find(l_supp[i]);
+        if (sp == supp.end() || sp->second != o->second) continue;
+        rev[o->second] = (int64_t)((uint64_t)rev[o->second] + (uint64_t)(l_price[i] * (100 - l_disc[i])));
+    }
+    std::vector<std::pair<int32_t, int64_t>> rows(rev.begin(), rev.end());
+    std::sort(rows.begin(), rows.end(), [](const std::pair<int32_t, int64_t>& a, const std::pair<int32_t, int64_t>& b) {
+        return a.second != b.second ? a.second > b.second : a.first < b.first;
+    });
+    for (uint64_t j = 0; j < rows.size() && j < cap; j++) {
+        out_nat[j] = rows[j].first;
+        out_rev[j] = rows[j].second;
+    }
+    return (int64_t)rows.size();
+}
+
 extern "C" int orc_num_threads(void) { return omp_get_max_threads(); }
 extern "C" void orc_set_threads(int t) { omp_set_num_threads(t); }
 

@@ -48,6 +48,8 @@ _SIG = {
     "orc_pim_hash": (u32, [C.c_int, u32]),
     "orc_q3": (i64, [vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, u32, i32, u64, vp, vp, vp, vp]),
     "orc_q4": (i64, [vp, vp, vp, u64, vp, vp, vp, u64, i32, i32, vp]),
+    "orc_q5": (i64, [vp, vp, u64, vp, vp, u64, vp, vp, u64, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp, u64, u32,
+                     i32, i32, u64, vp, vp]),
     "orc_num_threads": (C.c_int, []),
     "orc_set_threads": (None, [C.c_int]),
 }
@@ -406,3 +408,26 @@ def q4(t, date_start, date_end):
         raise ValueError("o_orderkey values repeat")
     codes = np.nonzero(cnt)[0].astype(np.uint8)
     return codes, cnt[codes]
+
+
+Q5_COLS = {"region": ("r_regionkey", "r_name"), "nation": ("n_nationkey", "n_regionkey"),
+           "customer": ("c_custkey", "c_nationkey"), "orders": ("o_orderkey", "o_custkey", "o_orderdate"),
+           "lineitem": ("l_orderkey", "l_suppkey", "l_extendedprice", "l_discount"),
+           "supplier": ("s_suppkey", "s_nationkey")}
+
+
+def q5(t, region_name, date_start, date_end, cap=1024):
+    """TPC-H Q5: (nation keys, revenues), revenue descending then key."""
+    c = {k: np.ascontiguousarray(t[k]) for cols in Q5_COLS.values() for k in cols}
+    onat, orev = np.empty(max(cap, 1), np.int32), np.empty(max(cap, 1), np.int64)
+    n = lambda k: c[k].size
+    g = lib().orc_q5(_p(c["r_regionkey"]), _p(c["r_name"]), n("r_regionkey"), _p(c["n_nationkey"]),
+                     _p(c["n_regionkey"]), n("n_nationkey"), _p(c["c_custkey"]), _p(c["c_nationkey"]), n("c_custkey"),
+                     _p(c["o_orderkey"]), _p(c["o_custkey"]), _p(c["o_orderdate"]), n("o_orderkey"),
+                     _p(c["l_orderkey"]), _p(c["l_suppkey"]), _p(c["l_extendedprice"]), _p(c["l_discount"]),
+                     n("l_orderkey"), _p(c["s_suppkey"]), _p(c["s_nationkey"]), n("s_suppkey"), int(region_name),
+                     int(date_start), int(date_end), cap, _p(onat), _p(orev))
+    if g < 0:
+        raise ValueError("a key repeats")
+    k = min(g, cap)
+    return onat[:k], orev[:k]

@@ -97,6 +97,7 @@ SIGNATURES: dict[str, list] = {
     "pimdal_q6_dev_async": [c_ctx_p, vp, vp, vp, vp, u64, i32, i32, C.c_int64, C.c_int64, C.c_int64, vp, vp],
     "pimdal_tpch_q3_dev": [c_ctx_p, vp, u32, i32, vp, u64, C.POINTER(u64)],
     "pimdal_tpch_q4_dev": [c_ctx_p, vp, i32, i32, vp, vp, u64, C.POINTER(u64)],
+    "pimdal_tpch_q5_dev": [c_ctx_p, vp, u32, i32, i32, vp, vp, u64, C.POINTER(u64)],
     "pimdal_plan_begin": [c_ctx_p, C.POINTER(vp)],
     "pimdal_plan_end": [vp],
     "pimdal_plan_launch": [vp],

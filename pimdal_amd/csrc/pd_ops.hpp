@@ -92,5 +92,7 @@ uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int
                  uint64_t cap, hipStream_t s);
 uint64_t tpch_q4(pimdal_ctx* c, const pimdal_q4_tables& t, int32_t date_start, int32_t date_end, uint8_t* out_prio,
                  int64_t* out_count, uint64_t cap, hipStream_t s);
+uint64_t tpch_q5(pimdal_ctx* c, const pimdal_q5_tables& t, uint32_t region_name, int32_t date_start, int32_t date_end,
+                 int32_t* out_nation, int64_t* out_rev, uint64_t cap, hipStream_t s);
 
 }  // namespace pd

@@ -1,6 +1,7 @@
 // q3.hip — the TPC-H Q3 shape: two joins chained on the device with the
-// group-by fused into the second join's probe; and the TPC-H Q4 shape (a
-// semi-join of orders with lineitems, counted per priority), further down.
+// group-by fused into the second join's probe; further down the TPC-H Q4
+// shape (a semi-join of orders with lineitems, counted per priority) and
+// the TPC-H Q5 shape (five tables joined, revenue per nation).
 //
 // Reference semantics: pimdal/main/tpc_h/query3/host/host_q3.cpp:471-519
 // (five DPU programs launched in turn, the host collecting and
@@ -552,6 +553,304 @@ uint64_t tpch_q4(pimdal_ctx* c, const pimdal_q4_tables& t, int32_t date_start, i
     return hv[1];
 }
 
+// ===================== TPC-H Q5 shape =====================
+// Reference: pimdal/main/tpc_h/query5/host/host_q5.cpp:566-640 runs seven
+// DPU programs with three host redistributions: kernel_q5_1.c selects the
+// region by name, its nations and their customers (custkey, nationkey);
+// kernel_q5_2.c the orders of [date_start, date_end); kernel_q5_3.c joins
+// them on custkey (orderkey, nationkey); kernel_q5_4.c / kernel_q5_5.c join
+// the lineitems on orderkey; kernel_q5_6.c / kernel_q5_7.c join the
+// suppliers on (suppkey, nationkey) and sum l_extendedprice * (100 -
+// l_discount) per nation; host_q5.cpp:459-500 sums the devices' rows and
+// orders by revenue.  Plan on gfx950:
+//  1. k_q5_nations  the region's nations as flags (nation keys < 1024);
+//  2. k_q5_cust     the region's customers in a table custkey -> nation;
+//  3. k_q5_osel / k_q5_orders   orders of the date range whose customer is
+//                   in the table, listed, then tabled orderkey -> nation,
+//                   with a filter bitmap of their keys (as Q4's);
+//  4. k_q5_supp     suppliers tabled suppkey -> nation (all of them);
+//  5. k_q5_lineitem one pass: filter bit, orders table, suppliers table,
+//                   and when the two nations agree the revenue and a row
+//                   count added per nation in LDS, one device add per
+//                   nation per block;
+//  6. k_q5_emit     nations with a row, by revenue descending then key.
+constexpr uint32_t Q5_NATIONS = 1024;
+
+// flags[k] = 1 for the nations of a region named `name`; flags[Q5_NATIONS]
+// = 1 when a nation key is out of range.  One block.
+__global__ __launch_bounds__(256) void k_q5_nations(const int32_t* __restrict__ rkey, const uint8_t* __restrict__ rname,
+                                                    uint64_t nr, const int32_t* __restrict__ nkey,
+                                                    const int32_t* __restrict__ nreg, uint64_t nn, uint32_t name,
+                                                    uint32_t* __restrict__ flags) {
+    for (uint64_t i = threadIdx.x; i < nn; i += blockDim.x) {
+        const int32_t k = nkey[i];
+        if (k < 0 || k >= (int32_t)Q5_NATIONS) {
+            flags[Q5_NATIONS] = 1;
+            continue;
+        }
+        bool in = false;
+        for (uint64_t r = 0; r < nr && !in; r++) in = rname[r] == name && rkey[r] == nreg[i];
+        if (in) flags[k] = 1;
+    }
+}
+
+// Table slot word: (nation + 1) << 32 | key (0 = empty).  Repeated keys
+// set *dup.
+__device__ __forceinline__ void q5_insert(uint64_t* tab, uint64_t mask, uint32_t key, uint32_t nation, uint32_t* dup) {
+    const uint64_t want = ((uint64_t)(nation + 1) << 32) | key;
+    uint64_t h = q3_slot(key, mask);
+    while (true) {
+        const uint64_t prev = atomicCAS((unsigned long long*)&tab[h], 0ull, (unsigned long long)want);
+        if (prev == 0) break;
+        if ((uint32_t)prev == key) {
+            atomicOr(dup, 1u);
+            break;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// The nation of `key` in a table, or -1.
+__device__ __forceinline__ int32_t q5_find(const uint64_t* tab, uint64_t mask, uint32_t key) {
+    uint64_t h = q3_slot(key, mask);
+    while (true) {
+        const uint64_t e = tab[h];
+        if (e == 0) return -1;
+        if ((uint32_t)e == key) return (int32_t)(e >> 32) - 1;
+        h = (h + 1) & mask;
+    }
+}
+
+__global__ void k_q5_cust(const int32_t* __restrict__ ckey, const int32_t* __restrict__ cnat, uint64_t n,
+                          const uint32_t* __restrict__ flags, uint64_t* __restrict__ tab, uint64_t mask,
+                          uint32_t* __restrict__ dup) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int32_t nat = cnat[i];
+        if (nat < 0 || nat >= (int32_t)Q5_NATIONS || !flags[nat]) continue;
+        q5_insert(tab, mask, (uint32_t)ckey[i], (uint32_t)nat, dup);
+    }
+}
+
+// Orders of [lo, hi) whose customer is in the table: their rows listed
+// (one counter atomic per 8192-row block, as k_q3_osel).
+__global__ __launch_bounds__(256) void k_q5_osel(const int32_t* __restrict__ odate, const int32_t* __restrict__ ocust,
+                                                 uint64_t n, int32_t lo, int32_t hi, const uint64_t* __restrict__ ctab,
+                                                 uint64_t cmask, unsigned long long* __restrict__ ctr,
+                                                 uint32_t* __restrict__ rows) {
+    constexpr int PER = Q3_CHUNK / 256;
+    __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_base;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    const uint64_t r0 = (uint64_t)blockIdx.x * Q3_CHUNK;
+    uint32_t bits = 0;
+#pragma unroll 4
+    for (int j = 0; j < PER; j++) {
+        const uint64_t i = r0 + (uint64_t)j * 256 + t;
+        if (i < n) {
+            const int32_t d = odate[i];
+            if (d >= lo && d < hi && q5_find(ctab, cmask, (uint32_t)ocust[i]) >= 0) bits |= 1u << j;
+        }
+    }
+    const uint32_t cnt = __popc(bits);
+    const uint32_t inc = pd::wave_incl_scan(cnt);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = inc - cnt, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        before += (i < (int)w) ? s_w[i] : 0u;
+        tot += s_w[i];
+    }
+    if (t == 0) s_base = tot ? atomicAdd(ctr, (unsigned long long)tot) : 0ull;
+    __syncthreads();
+    uint64_t o = s_base + before;
+    while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1;
+        rows[o++] = (uint32_t)(r0 + (uint64_t)j * 256 + t);
+    }
+}
+
+// The listed orders tabled orderkey -> their customer's nation, with their
+// filter bits.
+__global__ void k_q5_orders(const uint32_t* __restrict__ rows, uint64_t nq, const int32_t* __restrict__ okey,
+                            const int32_t* __restrict__ ocust, const uint64_t* __restrict__ ctab, uint64_t cmask,
+                            uint64_t* __restrict__ otab, uint64_t omask, uint32_t* __restrict__ bits, uint32_t bmask,
+                            uint32_t* __restrict__ dup) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nq; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = rows[j];
+        const uint32_t k = (uint32_t)okey[i];
+        const int32_t nat = q5_find(ctab, cmask, (uint32_t)ocust[i]);
+        q5_insert(otab, omask, k, (uint32_t)nat, dup);
+        const uint32_t b = q4_bit(k, bmask);
+        atomicOr(&bits[b >> 5], 1u << (b & 31));
+    }
+}
+
+__global__ void k_q5_supp(const int32_t* __restrict__ skey, const int32_t* __restrict__ snat, uint64_t n,
+                          uint64_t* __restrict__ tab, uint64_t mask, uint32_t* __restrict__ dup) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int32_t nat = snat[i];
+        // a supplier outside the nation range matches no order's nation
+        q5_insert(tab, mask, (uint32_t)skey[i], nat >= 0 && nat < (int32_t)Q5_NATIONS ? (uint32_t)nat : Q5_NATIONS, dup);
+    }
+}
+
+// Lineitems: filter bit, the order's nation, the supplier's nation; equal
+// nations add price * (100 - discount) and one row to the nation's LDS
+// accumulators.  Four rows per thread, the loads of a step in flight.
+__global__ __launch_bounds__(256) void k_q5_lineitem(const int32_t* __restrict__ lkey,
+                                                     const int32_t* __restrict__ lsupp,
+                                                     const int64_t* __restrict__ lprice,
+                                                     const int64_t* __restrict__ ldisc, uint64_t n,
+                                                     const uint32_t* __restrict__ bits, uint32_t bmask,
+                                                     const uint64_t* __restrict__ otab, uint64_t omask,
+                                                     const uint64_t* __restrict__ stab, uint64_t smask,
+                                                     unsigned long long* __restrict__ rev,
+                                                     unsigned long long* __restrict__ rows) {
+    constexpr int R = 4;
+    __shared__ unsigned long long s_rev[Q5_NATIONS];
+    __shared__ uint32_t s_rows[Q5_NATIONS];
+    for (uint32_t i = threadIdx.x; i < Q5_NATIONS; i += 256) {
+        s_rev[i] = 0;
+        s_rows[i] = 0;
+    }
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; base < n; base += stride * R) {
+        uint32_t key[R], fb[R], fw[R];
+        bool want[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t i = base + (uint64_t)r * stride;
+            want[r] = i < n;
+            key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            fb[r] = q4_bit(key[r], bmask);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) fw[r] = want[r] ? bits[fb[r] >> 5] : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!want[r] || !((fw[r] >> (fb[r] & 31)) & 1u)) continue;
+            const int32_t on = q5_find(otab, omask, key[r]);
+            if (on < 0) continue;
+            const uint64_t i = base + (uint64_t)r * stride;
+            const int32_t sn = q5_find(stab, smask, (uint32_t)lsupp[i]);
+            if (sn != on) continue;
+            atomicAdd(&s_rev[on], (unsigned long long)(lprice[i] * (100 - ldisc[i])));
+            atomicAdd(&s_rows[on], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < Q5_NATIONS; i += 256) {
+        if (s_rows[i]) {
+            atomicAdd(&rev[i], s_rev[i]);
+            atomicAdd(&rows[i], (unsigned long long)s_rows[i]);
+        }
+    }
+}
+
+// Nations with a row, revenue descending then nationkey ascending: the
+// nations with a row are listed (any order), then each one's rank is the
+// number of listed nations before it.  One block of 1024.
+__global__ __launch_bounds__(1024) void k_q5_emit(const unsigned long long* __restrict__ rev,
+                                                  const unsigned long long* __restrict__ rows, int32_t* __restrict__ onat,
+                                                  int64_t* __restrict__ orev, uint64_t cap,
+                                                  unsigned long long* __restrict__ meta) {
+    __shared__ int64_t s_rev[Q5_NATIONS];
+    __shared__ uint32_t s_key[Q5_NATIONS];
+    __shared__ uint32_t s_n;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) s_n = 0;
+    __syncthreads();
+    if (rows[t]) {
+        const uint32_t j = atomicAdd(&s_n, 1u);
+        s_rev[j] = (int64_t)rev[t];
+        s_key[j] = t;
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    if (t < n) {
+        const int64_t r = s_rev[t];
+        const uint32_t k = s_key[t];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < n; j++) rank += (s_rev[j] > r || (s_rev[j] == r && s_key[j] < k)) ? 1u : 0u;
+        if (rank < cap) {
+            onat[rank] = (int32_t)k;
+            orev[rank] = r;
+        }
+    }
+    if (t == 0) meta[0] = n;
+}
+
+uint64_t tpch_q5(pimdal_ctx* c, const pimdal_q5_tables& t, uint32_t region_name, int32_t date_start, int32_t date_end,
+                 int32_t* out_nation, int64_t* out_rev, uint64_t cap, hipStream_t s) {
+    PD_REQUIRE(t.r_regionkey && t.r_name && t.n_nationkey && t.n_regionkey && t.c_custkey && t.c_nationkey &&
+                   t.o_orderkey && t.o_custkey && t.o_orderdate && t.l_orderkey && t.l_suppkey && t.l_extendedprice &&
+                   t.l_discount && t.s_suppkey && t.s_nationkey,
+               PIMDAL_E_INVALID, "missing input column");
+    PD_REQUIRE(!cap || (out_nation && out_rev), PIMDAL_E_INVALID, "missing output column");
+    PD_REQUIRE(t.n_orders < (1ull << 30) && t.n_customer < (1ull << 30) && t.n_supplier < (1ull << 30) &&
+                   t.n_lineitem < (1ull << 40) && t.n_region <= 4096 && t.n_nation <= 4096,
+               PIMDAL_E_INVALID, "tables too large");
+    // small: flags[Q5_NATIONS + 1] u32 | dup u32 | pad | counters: [0] listed orders, [1] nations out,
+    // rev[Q5_NATIONS], rows[Q5_NATIONS]
+    const uint64_t fw = Q5_NATIONS + 2;
+    Scratch small(c, fw * 4 + (2 + 2 * (uint64_t)Q5_NATIONS) * 8 + 8, s);
+    fill_u64(c, small.get(), 0, (fw * 4 + (2 + 2 * (uint64_t)Q5_NATIONS) * 8 + 8) / 8, s);
+    uint32_t* flags = small.as<uint32_t>();
+    uint32_t* dup = flags + Q5_NATIONS + 1;
+    unsigned long long* meta = reinterpret_cast<unsigned long long*>(small.as<char>() + ((fw * 4 + 7) & ~7ull));
+    unsigned long long* rev = meta + 2;
+    unsigned long long* rows = rev + Q5_NATIONS;
+    // 1. nations, 2. customers
+    launch(c, "q5_nations", k_q5_nations, dim3(1), dim3(256), 0, s, t.r_regionkey, t.r_name, t.n_region, t.n_nationkey,
+           t.n_regionkey, t.n_nation, region_name, flags);
+    const uint64_t cslots = q3_pow2_slots(t.n_customer);
+    Scratch ctab(c, cslots * 8, s);
+    fill_u64(c, ctab.get(), 0, cslots, s);
+    launch(c, "q5_cust", k_q5_cust, q3_grid(c, t.n_customer), dim3(256), 0, s, t.c_custkey, t.c_nationkey, t.n_customer,
+           (const uint32_t*)flags, ctab.as<uint64_t>(), cslots - 1, dup);
+    // 3. orders of the range with a customer of the region
+    Scratch orows(c, std::max<uint64_t>(t.n_orders, 1) * 4, s);
+    launch(c, "q5_osel", k_q5_osel, dim3((uint32_t)std::max<uint64_t>(1, ceil_div(t.n_orders, Q3_CHUNK))), dim3(256), 0,
+           s, t.o_orderdate, t.o_custkey, t.n_orders, date_start, date_end, (const uint64_t*)ctab.as<uint64_t>(),
+           cslots - 1, meta, orows.as<uint32_t>());
+    uint32_t hflags[2] = {0, 0};
+    read_small(c, hflags, flags + Q5_NATIONS, 8, s);
+    PD_REQUIRE(hflags[0] == 0, PIMDAL_E_UNSUPPORTED, "nation keys must lie in [0, 1024)");
+    PD_REQUIRE(hflags[1] == 0, PIMDAL_E_INVALID, "c_custkey values of the region's customers must be unique");
+    uint64_t nq = 0;
+    read_small(c, &nq, meta, 8, s);
+    const uint64_t oslots = q3_pow2_slots(nq);
+    uint64_t nbits = 1ull << 16;
+    while (nbits < 32 * nq && nbits < (1ull << 24)) nbits <<= 1;
+    const uint64_t sslots = q3_pow2_slots(t.n_supplier);
+    Scratch otab(c, oslots * 8, s), fbits(c, nbits / 8, s), stab(c, sslots * 8, s);
+    fill_u64(c, otab.get(), 0, oslots, s);
+    fill_u32(c, fbits.get(), 0, nbits / 32, s);
+    fill_u64(c, stab.get(), 0, sslots, s);
+    launch(c, "q5_orders", k_q5_orders, q3_grid(c, nq), dim3(256), 0, s, (const uint32_t*)orows.as<uint32_t>(), nq,
+           t.o_orderkey, t.o_custkey, (const uint64_t*)ctab.as<uint64_t>(), cslots - 1, otab.as<uint64_t>(), oslots - 1,
+           fbits.as<uint32_t>(), (uint32_t)(nbits - 1), dup);
+    // 4. suppliers, 5. lineitems
+    launch(c, "q5_supp", k_q5_supp, q3_grid(c, t.n_supplier), dim3(256), 0, s, t.s_suppkey, t.s_nationkey, t.n_supplier,
+           stab.as<uint64_t>(), sslots - 1, dup);
+    launch(c, "q5_lineitem", k_q5_lineitem, q3_grid(c, ceil_div(t.n_lineitem, 4)), dim3(256), 0, s, t.l_orderkey,
+           t.l_suppkey, t.l_extendedprice, t.l_discount, t.n_lineitem, (const uint32_t*)fbits.as<uint32_t>(),
+           (uint32_t)(nbits - 1), (const uint64_t*)otab.as<uint64_t>(), oslots - 1,
+           (const uint64_t*)stab.as<uint64_t>(), sslots - 1, rev, rows);
+    // 6. nations by revenue
+    launch(c, "q5_emit", k_q5_emit, dim3(1), dim3(Q5_NATIONS), 0, s, (const unsigned long long*)rev,
+           (const unsigned long long*)rows, out_nation, out_rev, cap, meta + 1);
+    uint64_t g = 0;
+    read_small(c, &g, meta + 1, 8, s);
+    uint32_t d2 = 0;
+    read_small(c, &d2, dup, 4, s);
+    PD_REQUIRE(d2 == 0, PIMDAL_E_INVALID, "orderkey and suppkey values must be unique");
+    return g;
+}
+
 }  // namespace pd
 
 using namespace pd;
@@ -571,5 +870,15 @@ extern "C" int pimdal_tpch_q4_dev(pimdal_ctx* c, const pimdal_q4_tables* tables,
     PD_REQUIRE(c && tables && n_groups, PIMDAL_E_INVALID, "NULL argument");
     PD_NO_CAPTURE(c);
     *n_groups = tpch_q4(c, *tables, date_start, date_end, out_priority, out_count, capacity, c->stream);
+    PD_API_END
+}
+
+extern "C" int pimdal_tpch_q5_dev(pimdal_ctx* c, const pimdal_q5_tables* tables, uint32_t region_name,
+                                  int32_t date_start, int32_t date_end, int32_t* out_nationkey, int64_t* out_revenue,
+                                  uint64_t capacity, uint64_t* n_groups) {
+    PD_API_BEGIN
+    PD_REQUIRE(c && tables && n_groups, PIMDAL_E_INVALID, "NULL argument");
+    PD_NO_CAPTURE(c);
+    *n_groups = tpch_q5(c, *tables, region_name, date_start, date_end, out_nationkey, out_revenue, capacity, c->stream);
     PD_API_END
 }

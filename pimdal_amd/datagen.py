@@ -195,3 +195,36 @@ def q4_tables(n_lineitem: int, seed: int = 0x5EED0007) -> dict:
         "l_orderkey": o_key[l_orow], "l_commitdate": pack_days(o_days[l_orow] + rng.integers(30, 91, nl)),
         "l_receiptdate": pack_days(ship + rng.integers(1, 31, nl)),
     }
+
+
+Q5_REGIONS = ("AFRICA", "AMERICA", "ASIA", "EUROPE", "MIDDLE EAST")   # dictionary order
+# TPC-H's 25 nations by key and their region keys
+Q5_NATION_REGION = (0, 1, 1, 1, 4, 0, 3, 3, 2, 2, 4, 4, 2, 4, 0, 0, 0, 1, 2, 3, 4, 2, 3, 3, 1)
+
+
+def q5_tables(n_lineitem: int, seed: int = 0x5EED0008) -> dict:
+    """Synthetic region / nation / customer / orders / lineitem / supplier
+    columns in TPC-H's shape for Q5: 25 nations in 5 regions (dbgen's
+    mapping), 40 lineitems per customer, 4 per order, 600 per supplier
+    (SF10 at 6e7 lineitems: 1.5e6 customers, 1e5 suppliers); order dates
+    uniform over 1992-01-01 .. 1998-08-02; prices in cents, discount 0..10
+    percent; region names as codes into Q5_REGIONS.  Dates packed."""
+    rng = np.random.default_rng(seed)
+    nl = int(n_lineitem)
+    no, nc, ns = max(1, nl // 4), max(1, nl // 40), max(1, nl // 600)
+    o_key = (np.arange(no, dtype=np.int64) * 4 + rng.integers(0, 4, no)).astype(np.int32)
+    o_key = o_key[rng.permutation(no)]
+    c_key = (rng.permutation(nc) + 1).astype(np.int32)
+    s_key = (rng.permutation(ns) + 1).astype(np.int32)
+    o_days = 8035 + rng.integers(0, 2405 - 151 + 1, no)
+    l_orow = np.sort(rng.integers(0, no, nl))
+    return {
+        "r_regionkey": np.arange(5, dtype=np.int32), "r_name": np.arange(5, dtype=np.uint8),
+        "n_nationkey": np.arange(25, dtype=np.int32), "n_regionkey": np.array(Q5_NATION_REGION, np.int32),
+        "c_custkey": c_key, "c_nationkey": rng.integers(0, 25, nc).astype(np.int32),
+        "o_orderkey": o_key, "o_custkey": c_key[rng.integers(0, nc, no)], "o_orderdate": pack_days(o_days),
+        "l_orderkey": o_key[l_orow], "l_suppkey": s_key[rng.integers(0, ns, nl)],
+        "l_extendedprice": rng.integers(90000, 10500001, nl).astype(np.int64),
+        "l_discount": rng.integers(0, 11, nl).astype(np.int64),
+        "s_suppkey": s_key, "s_nationkey": rng.integers(0, 25, ns).astype(np.int32),
+    }

@@ -360,3 +360,33 @@ def tpch_q4(ctx: Context, t: dict, date_start: int, date_end: int, capacity: int
          C.byref(g))
     k = min(g.value, cap)
     return prio.numpy(k), cnt.numpy(k), g.value
+
+
+class _Q5Tables(C.Structure):
+    _fields_ = [("r_regionkey", C.c_void_p), ("r_name", C.c_void_p), ("n_region", C.c_uint64),
+                ("n_nationkey", C.c_void_p), ("n_regionkey", C.c_void_p), ("n_nation", C.c_uint64),
+                ("c_custkey", C.c_void_p), ("c_nationkey", C.c_void_p), ("n_customer", C.c_uint64),
+                ("o_orderkey", C.c_void_p), ("o_custkey", C.c_void_p), ("o_orderdate", C.c_void_p),
+                ("n_orders", C.c_uint64), ("l_orderkey", C.c_void_p), ("l_suppkey", C.c_void_p),
+                ("l_extendedprice", C.c_void_p), ("l_discount", C.c_void_p), ("n_lineitem", C.c_uint64),
+                ("s_suppkey", C.c_void_p), ("s_nationkey", C.c_void_p), ("n_supplier", C.c_uint64)]
+
+
+def tpch_q5(ctx: Context, t: dict, region_name: int, date_start: int, date_end: int, capacity: int = 1024):
+    """TPC-H Q5 shape over device columns `t` (region, nation, customer,
+    orders, lineitem, supplier): nation keys and revenues, revenue
+    descending, and the nation count.
+    pimdal/main/tpc_h/query5/host/host_q5.cpp:566-640."""
+    tab = _Q5Tables(t["r_regionkey"].ptr, t["r_name"].ptr, t["r_regionkey"].n, t["n_nationkey"].ptr,
+                    t["n_regionkey"].ptr, t["n_nationkey"].n, t["c_custkey"].ptr, t["c_nationkey"].ptr,
+                    t["c_custkey"].n, t["o_orderkey"].ptr, t["o_custkey"].ptr, t["o_orderdate"].ptr,
+                    t["o_orderkey"].n, t["l_orderkey"].ptr, t["l_suppkey"].ptr, t["l_extendedprice"].ptr,
+                    t["l_discount"].ptr, t["l_orderkey"].n, t["s_suppkey"].ptr, t["s_nationkey"].ptr,
+                    t["s_suppkey"].n)
+    cap = max(1, int(capacity))
+    nat, rev = ctx.empty(cap, np.int32), ctx.empty(cap, np.int64)
+    g = C.c_uint64()
+    call("pimdal_tpch_q5_dev", ctx.handle, C.byref(tab), int(region_name), int(date_start), int(date_end), nat.ptr,
+         rev.ptr, cap, C.byref(g))
+    k = min(g.value, cap)
+    return nat.numpy(k), rev.numpy(k), g.value

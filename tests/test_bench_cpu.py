@@ -29,7 +29,7 @@ def test_cpu_cases_name_gpu_lines():
     assert len(names) == len(set(names))
     gpu_lines = {"sort_2^28", "sort_zipf_2^28", "select_2^27", "hash_join", "hash_join_zipf", "smj_join",
                  "smj_join_zipf", "bloom_semi_join_2^16x2^27", "bloom_semi_join_2^20x2^27", "q1_flags",
-                 "q1_orderkey", "q6", "q3_sf10", "date32_pack_6e7"}
+                 "q1_orderkey", "q6", "q3_sf10", "q4_sf10", "q5_sf10", "date32_pack_6e7"}
     gpu_lines |= {f"{w}_aggr_2^27_{s}" for w, s in (("hash", "k50"), ("hash", "k4000"), ("hash", "k2^24"),
                                                      ("sort", "k25"), ("sort", "k2^24"))}
     assert set(names) == gpu_lines

@@ -1,0 +1,75 @@
+"""TPC-H Q5 shape on the GPU (region's nations -> customers -> orders of the
+year -> lineitems -> suppliers of the same nation -> revenue per nation)
+against the CPU restatement, bit-exact and in order."""
+import numpy as np
+import pytest
+
+from pimdal_amd import _lib, datagen, operators as ops
+
+pytestmark = pytest.mark.gpu
+
+ASIA = datagen.Q5_REGIONS.index("ASIA")
+LO, HI = (int(x) for x in datagen.pack_days(np.array([8766, 9131])))   # 1994-01-01, 1995-01-01
+
+
+def _dev(ctx, t):
+    return {k: ctx.to_device(v) for k, v in t.items()}
+
+
+def _check(ctx, orc, t, name, lo, hi, capacity=1024):
+    en, er = orc.q5(t, name, lo, hi)
+    gn, gr, g = ops.tpch_q5(ctx, _dev(ctx, t), name, lo, hi, capacity=capacity)
+    assert g == en.size
+    k = min(g, capacity)
+    assert np.array_equal(gn, en[:k]) and np.array_equal(gr, er[:k])
+    return g
+
+
+@pytest.mark.parametrize("n", [600, 20_000, 1 << 20])
+def test_q5_matches_oracle(ctx, orc, n):
+    t = datagen.q5_tables(n, seed=5 + n)
+    _check(ctx, orc, t, ASIA, LO, HI)
+
+
+@pytest.mark.parametrize("region", range(5))
+def test_q5_every_region_and_date_ranges(ctx, orc, region):
+    t = datagen.q5_tables(300_000, seed=25)
+    _check(ctx, orc, t, region, LO, HI)
+    lo2, hi2 = (int(x) for x in datagen.pack_days(np.array([8035, 10800])))
+    _check(ctx, orc, t, region, lo2, hi2)
+    _check(ctx, orc, t, region, HI, LO)          # empty range
+
+
+def test_q5_edge_cases(ctx, orc):
+    """Negative prices, nation keys off the nation table, customers and
+    suppliers outside every region, lineitems of missing orders and
+    suppliers, a region name no row has, and a short capacity."""
+    t = datagen.q5_tables(60_000, seed=45)
+    t["l_extendedprice"][::7] *= -1
+    t["c_nationkey"][::11] = 1000                 # in range, not a nation
+    t["s_nationkey"][::13] = -3                   # out of range: matches nothing
+    t["l_orderkey"][:100] = 77_777_777
+    t["l_suppkey"][100:200] = 88_888_888
+    g = _check(ctx, orc, t, ASIA, LO, HI)
+    assert g == 5
+    _check(ctx, orc, t, ASIA, LO, HI, capacity=2)
+    _check(ctx, orc, t, 9, LO, HI)
+
+
+def test_q5_sf10_lineitems(ctx, orc):
+    """SF10's shape: 6e7 lineitems, 1.5e7 orders, 1.5e6 customers, 1e5 suppliers."""
+    t = datagen.q5_tables(60_000_000, seed=35)
+    assert _check(ctx, orc, t, ASIA, LO, HI) == 5
+
+
+def test_q5_refusals(ctx):
+    t = datagen.q5_tables(20_000, seed=55)
+    bad = dict(t)
+    bad["n_nationkey"] = t["n_nationkey"].copy()
+    bad["n_nationkey"][3] = 5000
+    with pytest.raises(_lib.PimdalError, match="nation keys"):
+        ops.tpch_q5(ctx, _dev(ctx, bad), ASIA, LO, HI)
+    bad = dict(t)
+    bad["s_suppkey"] = np.full_like(t["s_suppkey"], t["s_suppkey"][0])
+    with pytest.raises(_lib.PimdalError, match="unique"):
+        ops.tpch_q5(ctx, _dev(ctx, bad), ASIA, LO, HI)

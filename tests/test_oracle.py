@@ -495,3 +495,83 @@ def test_q4_oracle_refuses_repeated_orderkeys(orc):
     t["o_orderdate"][:2] = 150
     with pytest.raises(ValueError):
         orc.q4(t, 100, 200)
+
+
+def _q5_brute(t, name, lo, hi):
+    """The SQL of the Q5 shape by nested loops (tiny inputs): customer,
+    orders, lineitem, supplier, nation, region joined on their keys, the
+    supplier's nation equal to the customer's, the region named `name`,
+    lo <= o_orderdate < hi; revenue per nation, descending (ties by key)."""
+    region_keys = {int(k) for k, n in zip(t["r_regionkey"], t["r_name"]) if int(n) == name}
+    nations = {int(k) for k, r in zip(t["n_nationkey"], t["n_regionkey"]) if int(r) in region_keys}
+    rev = {}
+    for ci in range(len(t["c_custkey"])):
+        cn = int(t["c_nationkey"][ci])
+        if cn not in nations:
+            continue
+        for oi in range(len(t["o_orderkey"])):
+            if int(t["o_custkey"][oi]) != int(t["c_custkey"][ci]) or not lo <= int(t["o_orderdate"][oi]) < hi:
+                continue
+            for li in range(len(t["l_orderkey"])):
+                if int(t["l_orderkey"][li]) != int(t["o_orderkey"][oi]):
+                    continue
+                for si in range(len(t["s_suppkey"])):
+                    if int(t["s_suppkey"][si]) == int(t["l_suppkey"][li]) and int(t["s_nationkey"][si]) == cn:
+                        rev[cn] = rev.get(cn, 0) + int(t["l_extendedprice"][li]) * (100 - int(t["l_discount"][li]))
+    return sorted(rev.items(), key=lambda kv: (-kv[1], kv[0]))
+
+
+def _q5_tiny(seed):
+    rng = np.random.default_rng(seed)
+    nc, no, nl, ns = 15, 40, 120, 8
+    t = {"r_regionkey": np.array([3, 1, 2], np.int32), "r_name": np.array([0, 1, 2], np.uint8),
+         "n_nationkey": np.arange(6, dtype=np.int32) * 2, "n_regionkey": np.array([3, 1, 2, 1, 3, 2], np.int32)}
+    t["c_custkey"] = (rng.permutation(nc) * 3 - 5).astype(np.int32)
+    t["c_nationkey"] = (rng.integers(0, 7, nc) * 2).astype(np.int32)          # 12: no such nation
+    t["o_orderkey"] = (rng.permutation(no) * 7 - 40).astype(np.int32)
+    t["o_custkey"] = t["c_custkey"][rng.integers(0, nc, no)]
+    t["o_custkey"][:3] = 9999                                                 # no such customer
+    t["o_orderdate"] = rng.integers(100, 200, no).astype(np.int32)
+    t["l_orderkey"] = t["o_orderkey"][rng.integers(0, no, nl)]
+    t["s_suppkey"] = (rng.permutation(ns) + 1).astype(np.int32)
+    t["s_nationkey"] = (rng.integers(0, 6, ns) * 2).astype(np.int32)
+    t["l_suppkey"] = t["s_suppkey"][rng.integers(0, ns, nl)]
+    t["l_suppkey"][:4] = 555                                                  # no such supplier
+    t["l_extendedprice"] = rng.integers(-50, 1000, nl).astype(np.int64)
+    t["l_discount"] = rng.integers(0, 11, nl).astype(np.int64)
+    return t
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("name,lo,hi", [(0, 100, 200), (1, 120, 170), (2, 100, 200), (2, 150, 150), (7, 100, 200)])
+def test_q5_oracle_matches_brute_force(orc, seed, name, lo, hi):
+    t = _q5_tiny(seed)
+    exp = _q5_brute(t, name, lo, hi)
+    nat, rev = orc.q5(t, name, lo, hi)
+    assert [(int(a), int(b)) for a, b in zip(nat, rev)] == exp
+
+
+def test_q5_oracle_matches_pandas(orc):
+    """Medium tables: the oracle against pandas' merges (customer -> orders ->
+    lineitem -> supplier on (suppkey, nation)) and group-by."""
+    import pandas as pd
+    from pimdal_amd import datagen
+    t = datagen.q5_tables(400_000, seed=7)
+    asia = datagen.Q5_REGIONS.index("ASIA")
+    lo, hi = (int(x) for x in datagen.pack_days(np.array([8766, 9131])))   # 1994-01-01, 1995-01-01
+    df = lambda *cols: pd.DataFrame({k: t[k] for k in cols})
+    reg = df("r_regionkey", "r_name")
+    nat = df("n_nationkey", "n_regionkey").merge(reg[reg.r_name == asia], left_on="n_regionkey", right_on="r_regionkey")
+    cu = df("c_custkey", "c_nationkey").merge(nat, left_on="c_nationkey", right_on="n_nationkey")
+    o = df("o_orderkey", "o_custkey", "o_orderdate")
+    m = cu.merge(o[(o.o_orderdate >= lo) & (o.o_orderdate < hi)], left_on="c_custkey", right_on="o_custkey")
+    m = m.merge(df("l_orderkey", "l_suppkey", "l_extendedprice", "l_discount"), left_on="o_orderkey",
+                right_on="l_orderkey")
+    m = m.merge(df("s_suppkey", "s_nationkey"), left_on=["l_suppkey", "c_nationkey"],
+                right_on=["s_suppkey", "s_nationkey"])
+    m["rev"] = m.l_extendedprice * (100 - m.l_discount)
+    g = m.groupby("c_nationkey", as_index=False)["rev"].sum().sort_values(["rev", "c_nationkey"],
+                                                                          ascending=[False, True])
+    nat_o, rev_o = orc.q5(t, asia, lo, hi)
+    assert len(g) == 5
+    assert np.array_equal(nat_o, g.c_nationkey.to_numpy()) and np.array_equal(rev_o, g.rev.to_numpy())
