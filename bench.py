@@ -131,7 +131,7 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
-           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "q4_osel", "q4_orders",
+           "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "q3_topk", "q4_osel", "q4_orders",
            "q4_bits", "q4_lineitem", "q4_count", "q4_emit", "q5_nations", "q5_cust", "q5_osel", "q5_orders",
            "q5_supp", "q5_lineitem", "q5_emit", "as_table", "as_fold",
            "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",

@@ -259,6 +259,157 @@ __global__ void k_q3_iota(uint32_t* __restrict__ out, uint64_t n) {
         out[i] = (uint32_t)i;
 }
 
+// ---- the first `cap` groups when cap is small (the reference's top 10) ----
+// Order key of a group: r = ~(revenue ^ 2^63) (ascending = revenue
+// descending), then orderdate ^ 2^31, then orderkey.  A histogram of r over
+// its range finds the bins holding the first cap groups; those candidates
+// (every group in a bin at or below the bin of the cap-th one, ties
+// included) are sorted whole in one block.
+constexpr uint32_t Q3_TBINS = 4096;
+constexpr uint32_t Q3_TCAND = 4096;   // candidates one block sorts
+
+__device__ __forceinline__ uint64_t q3_rkey(int64_t rev) { return ~((uint64_t)rev ^ (1ull << 63)); }
+
+// mm[0] = min r, mm[1] = max r (device atomics, one per block).
+__global__ __launch_bounds__(256) void k_q3_tminmax(const int64_t* __restrict__ g_rev, uint64_t g,
+                                                   unsigned long long* __restrict__ mm) {
+    uint64_t lo = ~0ull, hi = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < g; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = q3_rkey(g_rev[i]);
+        lo = min(lo, r);
+        hi = max(hi, r);
+    }
+    for (int o = PD_WAVE / 2; o > 0; o /= 2) {
+        lo = min(lo, (uint64_t)__shfl_xor((unsigned long long)lo, o));
+        hi = max(hi, (uint64_t)__shfl_xor((unsigned long long)hi, o));
+    }
+    if (pd::lane() == 0) {
+        atomicMin(&mm[0], (unsigned long long)lo);
+        atomicMax(&mm[1], (unsigned long long)hi);
+    }
+}
+
+__device__ __forceinline__ uint32_t q3_tshift(uint64_t span) {
+    // smallest shift with span >> shift < Q3_TBINS
+    uint32_t sh = 0;
+    while ((span >> sh) >= Q3_TBINS) sh++;
+    return sh;
+}
+
+__global__ __launch_bounds__(256) void k_q3_thist(const int64_t* __restrict__ g_rev, uint64_t g,
+                                                 const unsigned long long* __restrict__ mm,
+                                                 uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[Q3_TBINS];
+    for (uint32_t i = threadIdx.x; i < Q3_TBINS; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint64_t lo = mm[0];
+    const uint32_t sh = q3_tshift(mm[1] - lo);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < g; i += (uint64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[(q3_rkey(g_rev[i]) - lo) >> sh], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < Q3_TBINS; i += 256)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// The bin b of the cap-th group and the groups in bins <= b: thr[0] = b,
+// thr[1] = that count.  One block: thread t sums bins [16t, 16t + 16).
+__global__ __launch_bounds__(256) void k_q3_tthresh(const uint32_t* __restrict__ hist, uint64_t cap,
+                                                   uint32_t* __restrict__ thr) {
+    __shared__ uint32_t s_w[4];
+    constexpr uint32_t PER = Q3_TBINS / 256;
+    const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) sum += hist[t * PER + j];
+    const uint32_t inc = pd::wave_incl_scan(sum);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (uint32_t i = 0; i < w; i++) run += s_w[i];
+    // the thread whose bins reach cap finds the bin
+    if (run < cap && run + sum >= cap) {
+        for (uint32_t j = 0; j < PER; j++) {
+            run += hist[t * PER + j];
+            if (run >= cap) {
+                thr[0] = t * PER + j;
+                thr[1] = run;
+                break;
+            }
+        }
+    }
+}
+
+// Candidates: the groups whose bin is at most thr[0], listed (any order).
+__global__ __launch_bounds__(256) void k_q3_tcand(const int64_t* __restrict__ g_rev, uint64_t g,
+                                                 const unsigned long long* __restrict__ mm,
+                                                 const uint32_t* __restrict__ thr, uint32_t* __restrict__ ctr,
+                                                 uint32_t* __restrict__ cand) {
+    const uint64_t lo = mm[0];
+    const uint32_t sh = q3_tshift(mm[1] - lo), b = thr[0];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // whole waves step together, so the ballot and the wave's one counter
+    // add see every lane
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x; base < g; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        const bool in = i < g && ((q3_rkey(g_rev[i]) - lo) >> sh) <= b;
+        const uint64_t m = pd::ballot(in);
+        uint32_t o = 0;
+        if (pd::lane() == 0 && m) o = atomicAdd(ctr, (uint32_t)__popcll(m));
+        o = __shfl(o, 0) + pd::popc64(m & pd::lanemask_lt());
+        if (in) cand[o] = (uint32_t)i;
+    }
+}
+
+// Sorts the nc candidates by (r, date, orderkey) in one block (bitonic over
+// the next power of two, padding last) and writes the first k groups.
+__global__ __launch_bounds__(1024) void k_q3_tsort(const uint32_t* __restrict__ cand, uint32_t nc, uint32_t k,
+                                                  const uint32_t* __restrict__ g_key, const int64_t* __restrict__ g_rev,
+                                                  const int32_t* __restrict__ g_date, const int32_t* __restrict__ g_prio,
+                                                  int32_t* __restrict__ o_key, int64_t* __restrict__ o_rev,
+                                                  int32_t* __restrict__ o_date, int32_t* __restrict__ o_prio) {
+    __shared__ uint64_t s_r[Q3_TCAND];
+    __shared__ uint64_t s_dk[Q3_TCAND];   // date ^ 2^31 << 32 | orderkey
+    __shared__ uint32_t s_i[Q3_TCAND];
+    uint32_t N = 2;
+    while (N < nc) N <<= 1;
+    for (uint32_t j = threadIdx.x; j < N; j += 1024) {
+        if (j < nc) {
+            const uint32_t i = cand[j];
+            s_r[j] = q3_rkey(g_rev[i]);
+            s_dk[j] = ((uint64_t)((uint32_t)g_date[i] ^ 0x80000000u) << 32) | g_key[i];
+            s_i[j] = i;
+        } else {
+            s_r[j] = ~0ull;
+            s_dk[j] = ~0ull;
+            s_i[j] = 0xFFFFFFFFu;
+        }
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= N; size <<= 1) {
+        for (uint32_t stride = size / 2; stride > 0; stride >>= 1) {
+            for (uint32_t j = threadIdx.x; j < N / 2; j += 1024) {
+                const uint32_t a = 2 * j - (j & (stride - 1)), bb = a + stride;
+                const bool up = (a & size) == 0;
+                const bool gt = s_r[a] > s_r[bb] || (s_r[a] == s_r[bb] && s_dk[a] > s_dk[bb]);
+                if (gt == up) {
+                    const uint64_t r = s_r[a], dk = s_dk[a];
+                    const uint32_t ii = s_i[a];
+                    s_r[a] = s_r[bb]; s_dk[a] = s_dk[bb]; s_i[a] = s_i[bb];
+                    s_r[bb] = r; s_dk[bb] = dk; s_i[bb] = ii;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < k; j += 1024) {
+        const uint32_t i = s_i[j];
+        o_key[j] = (int32_t)g_key[i];
+        o_rev[j] = g_rev[i];
+        o_date[j] = g_date[i];
+        o_prio[j] = g_prio[i];
+    }
+}
+
 static uint64_t q3_pow2_slots(uint64_t n) {
     uint64_t s = 1024;
     while (s < 2 * n) s <<= 1;
@@ -317,6 +468,33 @@ uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int
     PD_REQUIRE((uint32_t)hv[2] == 0, PIMDAL_E_INVALID, "o_orderkey values must be unique");
     const uint64_t g = hv[1];
     if (g == 0 || cap == 0) return g;
+    // 5a. a few groups asked for: the candidates of the first `cap`, sorted in one block
+    if (cap < g && cap <= Q3_TCAND && c->tune_get("q3_topk", 1)) {
+        Scratch ts(c, 16 + 16 + Q3_TBINS * 4 + (uint64_t)g * 4, s);
+        unsigned long long* mm = ts.as<unsigned long long>();
+        uint32_t* thr = reinterpret_cast<uint32_t*>(mm + 2);   // [0] bin, [1] candidates, [2] listed
+        uint32_t* hist = thr + 4;
+        uint32_t* cand = hist + Q3_TBINS;
+        fill_u64(c, mm, ~0ull, 1, s);
+        fill_u32(c, reinterpret_cast<uint32_t*>(mm + 1), 0, 2 + 4 + Q3_TBINS, s);
+        const dim3 gg = dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(g, 256), 1024)));
+        launch(c, "q3_topk", k_q3_tminmax, gg, dim3(256), 0, s, (const int64_t*)gr.as<int64_t>(), g, mm);
+        launch(c, "q3_topk", k_q3_thist, gg, dim3(256), 0, s, (const int64_t*)gr.as<int64_t>(), g,
+               (const unsigned long long*)mm, hist);
+        launch(c, "q3_topk", k_q3_tthresh, dim3(1), dim3(256), 0, s, (const uint32_t*)hist, cap, thr);
+        uint32_t th[2] = {0, 0};
+        read_small(c, th, thr, 8, s);
+        if (th[1] <= Q3_TCAND) {
+            launch(c, "q3_topk", k_q3_tcand, gg, dim3(256), 0, s, (const int64_t*)gr.as<int64_t>(), g,
+                   (const unsigned long long*)mm, (const uint32_t*)thr, thr + 2, cand);
+            launch(c, "q3_topk", k_q3_tsort, dim3(1), dim3(1024), 0, s, (const uint32_t*)cand, th[1], (uint32_t)cap,
+                   (const uint32_t*)gk.as<uint32_t>(), (const int64_t*)gr.as<int64_t>(),
+                   (const int32_t*)gd.as<int32_t>(), (const int32_t*)gp.as<int32_t>(), out.orderkey, out.revenue,
+                   out.orderdate, out.shippriority);
+            PD_HIP(hipStreamSynchronize(s));
+            return g;
+        }
+    }
     // 5. TPC-H order: stable sorts of a permutation, least significant key first
     Scratch pa(c, g * 4, s), pb(c, g * 4, s), ka(c, g * 4, s), kb(c, g * 4, s);
     launch(c, "q3_order", k_q3_iota, q3_grid(c, g), dim3(256), 0, s, pa.as<uint32_t>(), g);

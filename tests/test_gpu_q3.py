@@ -80,3 +80,30 @@ def test_q3_repeated_orderkey_is_refused(ctx):
     t["o_orderdate"][:] = t["o_orderdate"].min()
     with pytest.raises(_lib.PimdalError, match="unique"):
         ops.tpch_q3(ctx, _dev(ctx, t), BUILDING, DATE)
+
+
+@pytest.mark.parametrize("cap", [1, 2, 10, 100, 4096, 4097])
+def test_q3_first_groups_match_oracle(ctx, orc, cap):
+    """A capacity below the group count: the revenue histogram finds the
+    candidates of the first `cap` groups, sorted in one block (up to 4096);
+    4097 takes the full sort."""
+    t = datagen.q3_tables(1 << 21, seed=61)
+    exp, g_exp = orc.q3(t, BUILDING, DATE, cap=cap)
+    got, g = ops.tpch_q3(ctx, _dev(ctx, t), BUILDING, DATE, capacity=cap)
+    assert g == g_exp and g > cap
+    _check(got, exp)
+
+
+def test_q3_first_groups_with_tied_and_negative_revenue(ctx, orc):
+    """Revenues tied over thousands of groups (more candidates than one block
+    sorts: the full sort) and a few negative ones; ties ordered by date,
+    then orderkey."""
+    t = datagen.q3_tables(1 << 20, seed=62)
+    t["l_extendedprice"][:] = 1000
+    t["l_discount"][:] = 0
+    t["l_extendedprice"][::97] = -5
+    for cap in (10, 3000):
+        exp, g_exp = orc.q3(t, BUILDING, DATE, cap=cap)
+        got, g = ops.tpch_q3(ctx, _dev(ctx, t), BUILDING, DATE, capacity=cap)
+        assert g == g_exp
+        _check(got, exp)
