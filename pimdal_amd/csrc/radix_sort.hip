@@ -1021,8 +1021,13 @@ __global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_sort(uint32_t* __restrict__ 
                                                         const uint32_t* __restrict__ count, RbDesc d) {
     constexpr uint32_t WAVES = RB_BLOCK / PD_WAVE;
     static_assert(RB_CAP * 4 >= WAVES * RS_BINS * 8, "match masks must fit in the key buffer");
-    __shared__ __align__(16) uint32_t s_keys[RB_CAP];
-    __shared__ __align__(16) V s_vals[HV ? RB_CAP : 1];
+    // 4-byte payloads travel with their keys as one 8-byte LDS word (one
+    // LDS store per row at the scatter and one load at the reload and the
+    // write-out, instead of two)
+    constexpr bool PACK = HV && sizeof(V) == 4;
+    __shared__ __align__(16) uint32_t s_keys[PACK ? 2 * RB_CAP : RB_CAP];
+    __shared__ __align__(16) V s_vals[HV && !PACK ? RB_CAP : 1];
+    uint2* s_kv = reinterpret_cast<uint2*>(s_keys);
     __shared__ uint16_t s_cnt[WAVES][RS_BINS];
     __shared__ uint32_t s_off[RS_BINS];
     __shared__ uint32_t s_wsum[RS_BINS / PD_WAVE];
@@ -1121,8 +1126,12 @@ __global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_sort(uint32_t* __restrict__ 
                 if (i * PD_WAVE >= wrows) break;
                 const uint32_t dg = pos[i] >> 16;
                 const uint32_t r = (pos[i] & 0xFFFFu) + s_off[dg] + s_cnt[w][dg];
-                s_keys[r] = key[i];
-                if constexpr (HV) s_vals[r] = val[i];
+                if constexpr (PACK) {
+                    s_kv[r] = make_uint2(key[i], (uint32_t)val[i]);
+                } else {
+                    s_keys[r] = key[i];
+                    if constexpr (HV) s_vals[r] = val[i];
+                }
             }
             __syncthreads();
             if (j + 1 < npass) {
@@ -1130,8 +1139,14 @@ __global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_sort(uint32_t* __restrict__ 
 #pragma unroll
                 for (uint32_t i = 0; i < RB_ITEMS; i++) {
                     if (i * PD_WAVE >= wrows) break;
-                    key[i] = s_keys[wbase + i * PD_WAVE + l];
-                    if constexpr (HV) val[i] = s_vals[wbase + i * PD_WAVE + l];
+                    if constexpr (PACK) {
+                        const uint2 e = s_kv[wbase + i * PD_WAVE + l];
+                        key[i] = e.x;
+                        val[i] = (V)e.y;
+                    } else {
+                        key[i] = s_keys[wbase + i * PD_WAVE + l];
+                        if constexpr (HV) val[i] = s_vals[wbase + i * PD_WAVE + l];
+                    }
                 }
                 __syncthreads();   // every row is read before the masks reuse s_keys
             }
@@ -1140,8 +1155,14 @@ __global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_sort(uint32_t* __restrict__ 
         for (uint32_t k = 0; k < RB_ITEMS; k++) {
             const uint32_t j = t + k * RB_BLOCK;
             if (j >= m) break;
-            keys[p + j] = s_keys[j];
-            if constexpr (HV) vals[p + j] = s_vals[j];
+            if constexpr (PACK) {
+                const uint2 e = s_kv[j];
+                keys[p + j] = e.x;
+                vals[p + j] = (V)e.y;
+            } else {
+                keys[p + j] = s_keys[j];
+                if constexpr (HV) vals[p + j] = s_vals[j];
+            }
         }
         __syncthreads();   // the next chunk reuses the LDS buffers
     }
