@@ -135,7 +135,7 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "q4_bits", "q4_lineitem", "q4_count", "q4_emit", "q5_nations", "q5_cust", "q5_osel", "q5_orders",
            "q5_supp", "q5_lineitem", "q5_emit", "as_table", "as_fold",
            "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",
-           "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "radix_bstart", "radix_chunks",
+           "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "ad_direct", "radix_bstart", "radix_chunks",
            "radix_bucket", "q1_status")
 
 

@@ -109,6 +109,9 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   gb_q1_slots (1024 or 2048);
  *   aggr_small (1: hash and sort aggregation first try per-block LDS tables
  *   for up to 4096 keys per block and 32768 in all; 0: skip that path),
+ *   aggr_direct (1: hash and sort aggregation first try per-block LDS arrays
+ *   indexed by key when a sample of the first 2^16 keys spans at most 8192
+ *   values and every key falls in the window around it; 0: skip that path),
  *   aggr_dense_max (keys spanning at most this many values, default and
  *   largest 2^24: one range-partition pass into 8192-key partitions summed
  *   in key-indexed LDS arrays, groups in key order; 0: skip that path);

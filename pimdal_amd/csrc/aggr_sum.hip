@@ -584,8 +584,19 @@ __global__ __launch_bounds__(256) void k_ad_minmax(const uint32_t* __restrict__ 
     const uint64_t nv = (n - head) / 4;
     for (uint64_t i = tid; i < head; i += stride) { lo = min(lo, key[i]); hi = max(hi, key[i]); }
     const uint4* k4 = reinterpret_cast<const uint4*>(key + head);
-    for (uint64_t i = tid; i < nv; i += stride) {
-        const uint4 q = k4[i];
+    uint64_t i4 = tid;
+    for (; i4 + 3 * stride < nv; i4 += 4 * stride) {   // four loads in flight
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = k4[i4 + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            lo = min(min(lo, q[u].x), min(q[u].y, min(q[u].z, q[u].w)));
+            hi = max(max(hi, q[u].x), max(q[u].y, max(q[u].z, q[u].w)));
+        }
+    }
+    for (; i4 < nv; i4 += stride) {
+        const uint4 q = k4[i4];
         lo = min(min(lo, q.x), min(q.y, min(q.z, q.w)));
         hi = max(max(hi, q.x), max(q.y, max(q.z, q.w)));
     }
@@ -931,6 +942,191 @@ bool aggr_sum_dense(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint
            (const int64_t*)pv.as<int64_t>(), (const uint32_t*)start, F, kmin, tick + 1, pst, okey, osum, ocnt, cap,
            tot);
     const uint64_t g = read_u64(c, tot, s);
+    PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+    *groups = g;
+    return true;
+}
+
+// ---------------- direct key range: no table, no partition pass ----------------
+// When a sample of the keys (the first 2^16 rows) spans at most AD_SPAN
+// values, every block sums its rows into LDS arrays indexed by key - base,
+// with no hashing and no probing.  The window [base, base + W) holds the
+// sample's range with room either side (W a power of two, 512 <= W <=
+// AD_SPAN); the LDS arrays hold AD_SPAN / W copies of it (at most one per
+// wave), so few keys are not added by every wave into the same words.  The
+// blocks' arrays are added into AD_DR HBM copies and the groups emitted in
+// key order.  The window is worked out on the device from the sample's
+// per-block minima and maxima, so the host reads back one word: the number
+// of groups, or AD_NONE when the sample's span is too wide or a key fell
+// outside the window (the caller then takes the table paths).
+constexpr uint32_t AD_DR = 4;            // HBM copies (block b adds into copy b % AD_DR)
+constexpr uint32_t AD_DU = 8;            // rows per thread per round
+constexpr uint32_t AD_WMIN = 512;        // smallest window
+constexpr uint32_t AD_SG = 16;           // blocks of the sample's min/max
+constexpr uint64_t AD_NONE = ~0ull;
+
+// The window from the sample's AD_SG (min, max) pairs: false when the span
+// is wider than AD_SPAN.  The window stays within u32.
+__device__ bool ad_window(const uint32_t* __restrict__ mm, uint32_t& base, uint32_t& wbits) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < AD_SG; b++) {
+        lo = min(lo, mm[2 * b]);
+        hi = max(hi, mm[2 * b + 1]);
+    }
+    const uint64_t span = (uint64_t)hi - lo + 1;
+    if (span > AD_SPAN) return false;
+    wbits = 0;
+    while ((1u << wbits) < AD_WMIN || ((1ull << wbits) < 2 * span && (1u << wbits) < AD_SPAN)) wbits++;
+    const uint64_t W = 1ull << wbits;
+    uint64_t b = lo - min((uint64_t)lo, (W - span) / 2);
+    b = min(b, 0x100000000ull - W);
+    base = (uint32_t)b;
+    return true;
+}
+
+__global__ __launch_bounds__(AD_BLOCK) void k_ad_direct(const uint32_t* __restrict__ key,
+                                                        const int64_t* __restrict__ val, uint64_t n, uint64_t chunk,
+                                                        const uint32_t* __restrict__ mm,
+                                                        unsigned long long* __restrict__ gcnt,
+                                                        unsigned long long* __restrict__ gsum,
+                                                        uint32_t* __restrict__ flag) {
+    // slot AD_SPAN takes the rows outside the window (their sums are not used)
+    __shared__ uint32_t s_cnt[AD_SPAN + 1];
+    __shared__ unsigned long long s_sum[AD_SPAN + 1];
+    const uint32_t t = threadIdx.x;
+    uint32_t base = 0, wbits = 0;
+    if (!ad_window(mm, base, wbits)) {
+        if (blockIdx.x == 0 && t == 0) atomicOr(flag, 1u);
+        return;
+    }
+    const uint32_t W = 1u << wbits;
+    const uint32_t copies = min(AD_SPAN >> wbits, AD_BLOCK / PD_WAVE);
+    const uint32_t my = ((t / PD_WAVE) % copies) << wbits;
+    for (uint32_t i = t; i < AD_SPAN + 1; i += AD_BLOCK) {
+        s_cnt[i] = 0;
+        s_sum[i] = 0;
+    }
+    __syncthreads();
+    const uint64_t start = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = min(n, start + chunk);
+    bool out = false;
+    for (uint64_t i0 = start; i0 < end; i0 += (uint64_t)AD_BLOCK * AD_DU) {
+        uint32_t k[AD_DU];
+        int64_t v[AD_DU];
+        bool in[AD_DU];
+        // loads clamped to the block's last row, so all of them are in
+        // flight together
+#pragma unroll
+        for (uint32_t u = 0; u < AD_DU; u++) {
+            const uint64_t i = i0 + (uint64_t)u * AD_BLOCK + t;
+            in[u] = i < end;
+            const uint64_t j = in[u] ? i : end - 1;
+            k[u] = key[j] - base;
+            v[u] = val[j];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < AD_DU; u++) {
+            if (!in[u]) continue;   // past the block's rows
+            out |= k[u] >= W;
+            const uint32_t sl = k[u] < W ? my + k[u] : AD_SPAN;
+            atomicAdd(&s_cnt[sl], 1u);
+            atomicAdd(&s_sum[sl], (unsigned long long)v[u]);
+        }
+    }
+    if (__syncthreads_or(out)) {
+        if (t == 0) atomicOr(flag, 1u);
+        return;
+    }
+    const uint32_t r = blockIdx.x % AD_DR;
+    for (uint32_t i = t; i < W; i += AD_BLOCK) {
+        uint32_t c = 0;
+        unsigned long long sm = 0;
+        for (uint32_t j = 0; j < copies; j++) {
+            c += s_cnt[(j << wbits) + i];
+            sm += s_sum[(j << wbits) + i];
+        }
+        if (!c) continue;
+        atomicAdd(&gcnt[(uint64_t)r * AD_SPAN + i], (unsigned long long)c);
+        atomicAdd(&gsum[(uint64_t)r * AD_SPAN + i], sm);
+    }
+}
+
+// The copies added up and the groups (count > 0) written in key order: one
+// block, thread t owning slots [8t, 8t + 8).  Writes at most cap groups;
+// *total = all groups.
+__global__ __launch_bounds__(AD_BLOCK) void k_ad_direct_emit(const unsigned long long* __restrict__ gcnt,
+                                                             const unsigned long long* __restrict__ gsum,
+                                                             const uint32_t* __restrict__ mm,
+                                                             const uint32_t* __restrict__ flag,
+                                                             uint32_t* __restrict__ okey, int64_t* __restrict__ osum,
+                                                             int64_t* __restrict__ ocnt, uint64_t cap,
+                                                             uint64_t* __restrict__ total) {
+    constexpr uint32_t SLOTS = AD_SPAN / AD_BLOCK;
+    __shared__ uint32_t s_w[AD_BLOCK / PD_WAVE];
+    const uint32_t t = threadIdx.x, l = lane(), w = t / PD_WAVE;
+    uint32_t base = 0, wbits = 0;
+    if (*flag || !ad_window(mm, base, wbits)) {
+        if (t == 0) *total = AD_NONE;
+        return;
+    }
+    uint64_t c[SLOTS], sm[SLOTS];
+    uint32_t nz = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < SLOTS; j++) {
+        const uint32_t sl = t * SLOTS + j;
+        c[j] = 0;
+        sm[j] = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < AD_DR; r++) {
+            c[j] += gcnt[(uint64_t)r * AD_SPAN + sl];
+            sm[j] += gsum[(uint64_t)r * AD_SPAN + sl];
+        }
+        nz += c[j] ? 1u : 0u;
+    }
+    const uint32_t inc = wave_incl_scan(nz);
+    if (l == PD_WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    uint64_t off = inc - nz, all = 0;
+    for (uint32_t i = 0; i < AD_BLOCK / PD_WAVE; i++) {
+        if (i < w) off += s_w[i];
+        all += s_w[i];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < SLOTS; j++) {
+        if (!c[j]) continue;
+        if (off < cap) {
+            okey[off] = base + t * SLOTS + j;
+            osum[off] = (int64_t)sm[j];
+            ocnt[off] = (int64_t)c[j];
+        }
+        off++;
+    }
+    if (t == 0) *total = all;
+}
+
+bool aggr_sum_direct(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey,
+                     int64_t* osum, int64_t* ocnt, uint64_t cap, uint64_t* groups, hipStream_t s) {
+    if (n == 0 || c->tune_get("aggr_direct", 1) == 0) return false;
+    // gcnt[AD_DR][AD_SPAN] | gsum[AD_DR][AD_SPAN] | total | flag | mm[2 AD_SG]
+    const uint64_t words = 2ull * AD_DR * AD_SPAN + 2 + AD_SG;
+    Scratch sc(c, words * 8, s);
+    unsigned long long* gcnt = sc.as<unsigned long long>();
+    unsigned long long* gsum = gcnt + (uint64_t)AD_DR * AD_SPAN;
+    uint64_t* tot = reinterpret_cast<uint64_t*>(gsum + (uint64_t)AD_DR * AD_SPAN);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(tot + 1);
+    uint32_t* mm = reinterpret_cast<uint32_t*>(tot + 2);
+    fill_u64(c, gcnt, 0, words - AD_SG, s);
+    launch(c, "ad_minmax", k_ad_minmax, dim3(AD_SG), dim3(256), 0, s, key, std::min<uint64_t>(n, 1ull << 16), mm);
+    const uint32_t G = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(ceil_div(n, (uint64_t)AD_BLOCK * AD_DU), (uint64_t)c->cu_count));
+    const uint64_t chunk = ceil_div(n, G);
+    launch(c, "ad_direct", k_ad_direct, dim3(G), dim3(AD_BLOCK), 0, s, key, val, n, chunk, (const uint32_t*)mm, gcnt,
+           gsum, flag);
+    launch(c, "ad_direct", k_ad_direct_emit, dim3(1), dim3(AD_BLOCK), 0, s, (const unsigned long long*)gcnt,
+           (const unsigned long long*)gsum, (const uint32_t*)mm, (const uint32_t*)flag, okey, osum, ocnt, cap, tot);
+    const uint64_t g = read_u64(c, tot, s);
+    if (g == AD_NONE) return false;
     PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
     *groups = g;
     return true;

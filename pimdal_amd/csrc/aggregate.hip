@@ -1509,9 +1509,14 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
     if (n == 0) return 0;
     PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "hash aggregation supports up to 2^30-1 rows per call");
+    // keys within 8192 values: key-indexed LDS arrays
+    uint64_t g = 0;
+    if (aggr_sum_direct(c, key, val, n, out_key, out_sum, out_cnt, cap, &g, s)) {
+        PD_HIP(hipStreamSynchronize(s));
+        return g;
+    }
     // few groups: per-block LDS tables merged in one HBM table
     const int64_t hint = c->tune_get("gb_groups_hint", 0);
-    uint64_t g = 0;
     if (c->tune_get("aggr_small", 1) && hint <= 4096 &&
         aggr_sum_small(c, key, val, n, out_key, out_sum, out_cnt, cap, false, &g, s)) {
         PD_HIP(hipStreamSynchronize(s));
@@ -1531,18 +1536,23 @@ uint64_t hash_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint6
     return g;
 }
 
-// Sort aggregation: groups in ascending key order.  With few groups the
-// hash tables' groups are ordered by key (sorting the rows would only add
-// passes); with keys in a range of at most 2^24, range partitions summed
-// into key-indexed LDS arrays; otherwise the rows are radix sorted by key
-// and reduced in one pass over the sorted runs.
+// Sort aggregation: groups in ascending key order.  Keys within 8192
+// values sum into key-indexed LDS arrays; with few groups the hash tables'
+// groups are ordered by key (sorting the rows would only add passes); with
+// keys in a range of at most 2^24, range partitions summed into key-indexed
+// LDS arrays; otherwise the rows are radix sorted by key and reduced in one
+// pass over the sorted runs.
 uint64_t sort_aggr(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* out_key,
                    int64_t* out_sum, int64_t* out_cnt, uint64_t cap, hipStream_t s) {
     PD_REQUIRE((key && val) || n == 0, PIMDAL_E_INVALID, "missing input column");
     if (n == 0) return 0;
     PD_REQUIRE(n < (1ull << 30), PIMDAL_E_INVALID, "sort aggregation supports up to 2^30-1 rows per call");
-    const int64_t hint = c->tune_get("gb_groups_hint", 0);
     uint64_t g = 0;
+    if (aggr_sum_direct(c, key, val, n, out_key, out_sum, out_cnt, cap, &g, s)) {
+        PD_HIP(hipStreamSynchronize(s));
+        return g;
+    }
+    const int64_t hint = c->tune_get("gb_groups_hint", 0);
     if (c->tune_get("aggr_small", 1) && hint <= 4096 &&
         aggr_sum_small(c, key, val, n, out_key, out_sum, out_cnt, cap, true, &g, s)) {
         PD_HIP(hipStreamSynchronize(s));

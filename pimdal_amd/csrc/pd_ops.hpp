@@ -60,6 +60,12 @@ uint64_t aggr_sum_sorted(pimdal_ctx* c, const uint32_t* key, const int64_t* val,
 // False (outputs untouched) when the key range is wider.
 bool aggr_sum_dense(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey,
                     int64_t* osum, int64_t* ocnt, uint64_t cap, uint64_t* groups, hipStream_t s);
+// Keys within 8192 values of each other (by a sample of the first 2^16 rows;
+// knob aggr_direct): per-block LDS arrays indexed by key, no table and no
+// partition pass; groups in key order.  False (outputs untouched) when the
+// sample or any key falls outside.
+bool aggr_sum_direct(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint64_t n, uint32_t* okey,
+                     int64_t* osum, int64_t* ocnt, uint64_t cap, uint64_t* groups, hipStream_t s);
 
 uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1, const uint8_t* f2,
                            const int32_t* key, const int64_t* qty, const int64_t* price, const int64_t* disc,

@@ -95,7 +95,8 @@ def _both_paths(ctx, orc, k, v, small_fits=None):
     exp = orc.sort_aggr(k, v)
     dk, dv = ctx.to_device(k), ctx.to_device(v)
     try:
-        for small, dense in ((1, 1), (0, 1), (0, 0)):
+        for direct, small, dense in ((1, 1, 1), (0, 1, 1), (0, 0, 1), (0, 0, 0)):
+            ctx.set_tune("aggr_direct", direct)
             ctx.set_tune("aggr_small", small)
             if dense:
                 ctx.clear_tune("aggr_dense_max")
@@ -106,9 +107,10 @@ def _both_paths(ctx, orc, k, v, small_fits=None):
                 got = (ok.numpy(g), os_.numpy(g), oc.numpy(g))
                 if fn is ops.hash_aggr:
                     got = canon(*got)
-                assert g == exp[0].size, (fn.__name__, small, dense)
-                assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (fn.__name__, small, dense)
+                assert g == exp[0].size, (fn.__name__, direct, small, dense)
+                assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (fn.__name__, direct, small, dense)
     finally:
+        ctx.clear_tune("aggr_direct")
         ctx.clear_tune("aggr_small")
         ctx.clear_tune("aggr_dense_max")
 
@@ -498,3 +500,55 @@ def test_q6_6e7_rows_full_size(ctx, orc):
     d = datagen.fact_table(ctx, n)
     t = {c: d[c].numpy() for c in ("date", "quantity", "price", "discount")}
     assert ops.q6(ctx, d) == orc.q6(t)
+
+
+def _direct_ran(ctx, fn, dk, dv):
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    try:
+        ok, os_, oc, g = fn(ctx, dk, dv)
+        return (ok.numpy(g), os_.numpy(g), oc.numpy(g)), ctx.kernel_stat("ad_direct")[0]
+    finally:
+        ctx.set_profiling(False)
+
+
+@pytest.mark.parametrize("case", ["k50", "k4000", "span_8192", "top_of_u32", "below_sample_window",
+                                  "beyond_sample_window", "beyond_small_window", "few_rows"])
+def test_aggr_direct_key_window(ctx, orc, case):
+    """The key-indexed path: a sample of the first 2^16 keys places a window
+    of 512 to 8192 keys (at least twice the sample's span, one LDS copy per
+    wave while 16 fit); keys outside it (met later in the rows) send the
+    call to the table paths.  Sort output in its own order, hash canonical."""
+    n = (1 << 21) + 3
+    v = orc.gen_range(np.int64, 37, 2, n, -(1 << 45), 1 << 45)
+    if case == "k50":
+        k = orc.gen_range(np.uint32, 37, 1, n, 1, 50)
+    elif case == "k4000":
+        k = orc.gen_range(np.uint32, 37, 1, n, 1, 4000)
+    elif case == "span_8192":
+        k = orc.gen_range(np.uint32, 37, 1, n, 100, 100 + 8191)
+        k[0], k[1] = 100, 100 + 8191
+    elif case == "top_of_u32":
+        k = orc.gen_range(np.uint32, 37, 1, n, 0xFFFFFFFF - 3000, 0xFFFFFFFF)
+        k[::5] = 0xFFFFFFFF
+    elif case == "below_sample_window":
+        k = orc.gen_range(np.uint32, 37, 1, n, 20_000, 20_100)
+        k[-7] = 5                       # past the sample, below the window
+    elif case == "beyond_sample_window":
+        k = orc.gen_range(np.uint32, 37, 1, n, 0, 100)
+        k[(1 << 16) + 5] = 50_000       # past the sample, above the window
+    elif case == "beyond_small_window":
+        k = orc.gen_range(np.uint32, 37, 1, n, 1000, 1100)
+        k[(1 << 16) + 9] = 1000 + 700   # within 8192 of the sample, past its 512-key window
+    else:
+        n = 1000
+        k = orc.gen_range(np.uint32, 37, 1, n, 7, 300)
+        v = v[:n]
+    exp = orc.sort_aggr(k, v)
+    dk, dv = ctx.to_device(k), ctx.to_device(v)
+    for fn in (ops.hash_aggr, ops.sort_aggr):
+        got, launches = _direct_ran(ctx, fn, dk, dv)
+        assert launches >= 1, (case, fn.__name__)
+        if fn is ops.hash_aggr:
+            got = canon(*got)
+        assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (case, fn.__name__)
