@@ -136,7 +136,7 @@ KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count
            "q5_supp", "q5_lineitem", "q5_emit", "as_table", "as_fold",
            "as_emit", "as_gather", "as_runs", "as_fix", "pw_count", "pw_tscan", "pw_last", "pw_scatter",
            "ad_minmax", "ad_count", "ad_tscan", "ad_scatter", "ad_sum", "ad_direct", "radix_bstart", "radix_chunks",
-           "radix_bucket", "q1_status")
+           "radix_bucket", "radix_bucket_retry", "q1_status")
 
 
 def kernel_breakdown(ctx, steps):

@@ -73,6 +73,7 @@ __global__ __launch_bounds__(256) void k_q3_osel(const int32_t* __restrict__ oda
     __shared__ unsigned long long s_base;
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     const uint64_t r0 = (uint64_t)blockIdx.x * Q3_CHUNK;
+    if (r0 >= n) return;   // no rows (the loads below are clamped to row n - 1)
     uint32_t bits = 0;   // bit j: row r0 + j * 256 + t qualifies
 #pragma unroll
     for (int j0 = 0; j0 < PER; j0 += 4) {
@@ -81,9 +82,13 @@ __global__ __launch_bounds__(256) void k_q3_osel(const int32_t* __restrict__ oda
         uint64_t h[4], e[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
+            // loads clamped to the last row, so none waits behind a branch
             const uint64_t i = r0 + (uint64_t)(j0 + q) * 256 + t;
-            cand[q] = i < n && odate[i] < date;
-            key[q] = cand[q] ? (uint32_t)ocust[i] : 0u;
+            const uint64_t ci = i < n ? i : n - 1;
+            const int32_t d = odate[ci];
+            const uint32_t cu = (uint32_t)ocust[ci];
+            cand[q] = i < n && d < date;
+            key[q] = cand[q] ? cu : 0u;
             h[q] = q3_slot(key[q], mask);
         }
 #pragma unroll
@@ -155,9 +160,13 @@ __global__ __launch_bounds__(256) void k_q3_lineitem(const int32_t* __restrict__
         uint64_t h[R], e[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            // loads clamped to the last row, so none waits behind a branch
             const uint64_t i = base + (uint64_t)r * stride;
-            want[r] = i < n && lship[i] > date;
-            key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            const uint64_t ci = i < n ? i : n - 1;
+            const int32_t d = lship[ci];
+            const uint32_t k = (uint32_t)lkey[ci];
+            want[r] = i < n && d > date;
+            key[r] = want[r] ? k : 0u;
             h[r] = q3_slot(key[r], omask);
         }
 #pragma unroll
@@ -545,14 +554,14 @@ __global__ __launch_bounds__(256) void k_q4_osel(const int32_t* __restrict__ oda
     __shared__ unsigned long long s_base;
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     const uint64_t r0 = (uint64_t)blockIdx.x * Q3_CHUNK;
+    if (r0 >= n) return;   // no rows (the loads below are clamped to row n - 1)
     uint32_t bits = 0;
 #pragma unroll
     for (int j = 0; j < PER; j++) {
+        // loads clamped to the last row, so all are in flight together
         const uint64_t i = r0 + (uint64_t)j * 256 + t;
-        if (i < n) {
-            const int32_t d = odate[i];
-            if (d >= lo && d < hi) bits |= 1u << j;
-        }
+        const int32_t d = odate[i < n ? i : n - 1];
+        if (i < n && d >= lo && d < hi) bits |= 1u << j;
     }
     const uint32_t cnt = __popc(bits);
     const uint32_t inc = pd::wave_incl_scan(cnt);
@@ -609,9 +618,13 @@ __global__ __launch_bounds__(256) void k_q4_lineitem(const int32_t* __restrict__
         uint64_t h[R], e[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            // loads clamped to the last row, so none waits behind a branch
             const uint64_t i = base + (uint64_t)r * stride;
-            want[r] = i < n && lcommit[i] < lreceipt[i];
-            key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            const uint64_t ci = i < n ? i : n - 1;
+            const int32_t dc = lcommit[ci], dr = lreceipt[ci];
+            const uint32_t k = (uint32_t)lkey[ci];
+            want[r] = i < n && dc < dr;
+            key[r] = want[r] ? k : 0u;
             fb[r] = q4_bit(key[r], bmask);
         }
 #pragma unroll
@@ -820,13 +833,34 @@ __global__ __launch_bounds__(256) void k_q5_osel(const int32_t* __restrict__ oda
     __shared__ unsigned long long s_base;
     const uint32_t t = threadIdx.x, w = t / PD_WAVE, l = pd::lane();
     const uint64_t r0 = (uint64_t)blockIdx.x * Q3_CHUNK;
+    if (r0 >= n) return;   // no rows (the loads below are clamped to row n - 1)
     uint32_t bits = 0;
-#pragma unroll 4
-    for (int j = 0; j < PER; j++) {
-        const uint64_t i = r0 + (uint64_t)j * 256 + t;
-        if (i < n) {
-            const int32_t d = odate[i];
-            if (d >= lo && d < hi && q5_find(ctab, cmask, (uint32_t)ocust[i]) >= 0) bits |= 1u << j;
+#pragma unroll
+    for (int j0 = 0; j0 < PER; j0 += 4) {
+        bool cand[4];
+        uint32_t key[4];
+        uint64_t h[4], e[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            // loads clamped to the last row, so none waits behind a branch
+            const uint64_t i = r0 + (uint64_t)(j0 + q) * 256 + t;
+            const uint64_t ci = i < n ? i : n - 1;
+            const int32_t d = odate[ci];
+            const uint32_t cu = (uint32_t)ocust[ci];
+            cand[q] = i < n && d >= lo && d < hi;
+            key[q] = cu;
+            h[q] = q3_slot(cu, cmask);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) e[q] = cand[q] ? ctab[h[q]] : 0ull;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!cand[q]) continue;
+            while (e[q] != 0 && (uint32_t)e[q] != key[q]) {
+                h[q] = (h[q] + 1) & cmask;
+                e[q] = ctab[h[q]];
+            }
+            if (e[q] != 0) bits |= 1u << (j0 + q);
         }
     }
     const uint32_t cnt = __popc(bits);
@@ -902,7 +936,7 @@ __global__ __launch_bounds__(256) void k_q5_lineitem(const int32_t* __restrict__
         for (int r = 0; r < R; r++) {
             const uint64_t i = base + (uint64_t)r * stride;
             want[r] = i < n;
-            key[r] = want[r] ? (uint32_t)lkey[i] : 0u;
+            key[r] = (uint32_t)lkey[want[r] ? i : n - 1];   // clamped: no load waits behind a branch
             fb[r] = q4_bit(key[r], bmask);
         }
 #pragma unroll

@@ -907,6 +907,13 @@ constexpr uint32_t RB_BLOCK = 512;
 constexpr uint32_t RB_ITEMS = 16;
 constexpr uint32_t RB_CAP = RB_BLOCK * RB_ITEMS;   // rows one block sorts at once
 constexpr uint32_t RB_NB = 1u << 16;               // buckets
+// 32-bit keys (buckets = the high 16 bits): one LDS counting pass over
+// RB_FBINS coarse bins (the bucket's offset in the chunk, then the top bits
+// of the low 16), each bin then sorted by (low 16 bits, row in the chunk);
+// a bin above RB_FIXMAX rows sends the chunk to the stable ranking instead
+constexpr uint32_t RB_FBITS = 11;
+constexpr uint32_t RB_FBINS = 1u << RB_FBITS;
+constexpr uint32_t RB_FIXMAX = 32;
 
 struct RbDesc {
     uint32_t nlow;                 // low digits sorted in LDS (1 or 2), lowest first
@@ -1168,6 +1175,153 @@ __global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_sort(uint32_t* __restrict__ 
     }
 }
 
+// Counting form of the bucket pass, for 32-bit keys whose buckets are the
+// high 16 bits (four varying digits).  Persistent over the chunk list as
+// k_rb_sort.  Every row takes a rank in its coarse bin (the bucket's offset
+// in the chunk << tb | the low 16 bits >> (16 - tb), tb = RB_FBITS - bits
+// of the chunk's bucket span: at most RB_FBINS bins) from one LDS atomic;
+// a block scan gives the bins' starts; each row's sort word (low 16 bits
+// << 16 | row in the chunk) is placed in its bin; a row's final place is
+// its bin's start plus the words of the bin below its own (about three
+// independent reads), which orders equal keys by input row, so the sort
+// stays stable.  Rows then go from registers to their places in LDS and
+// out in order.  One LDS atomic, one placement, a few reads and one store
+// per row, against two ranking rounds of match masks.  A chunk with a bin
+// above RB_FIXMAX rows is left untouched and listed in `retry` for
+// k_rb_sort.
+template <typename V, bool HV>
+__global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_count(uint32_t* __restrict__ keys, V* __restrict__ vals,
+                                                         const uint4* __restrict__ list,
+                                                         const uint32_t* __restrict__ count,
+                                                         uint4* __restrict__ retry, uint32_t* __restrict__ nretry) {
+    constexpr uint32_t WAVES = RB_BLOCK / PD_WAVE;
+    constexpr bool PACK = HV && sizeof(V) == 4;
+    static_assert(RB_FBINS == 4 * RB_BLOCK, "four bins per thread");
+    static_assert(RB_CAP <= 65536, "rows in a chunk fit 16 bits");
+    __shared__ __align__(16) uint32_t s_pk[PACK ? 2 * RB_CAP : RB_CAP];
+    __shared__ __align__(16) V s_val[HV && !PACK ? RB_CAP : 1];
+    __shared__ __align__(16) uint32_t s_bin[RB_FBINS + 4];   // counts, then starts; [RB_FBINS] = rows
+    __shared__ uint32_t s_fw[WAVES];
+    uint2* s_kv = reinterpret_cast<uint2*>(s_pk);
+    auto e_put = [&](uint32_t r, uint32_t pk, V v) {
+        if constexpr (PACK) {
+            s_kv[r] = make_uint2(pk, (uint32_t)v);
+        } else {
+            s_pk[r] = pk;
+            if constexpr (HV) s_val[r] = v;
+        }
+    };
+    auto e_pk = [&](uint32_t r) -> uint32_t {
+        if constexpr (PACK) return s_kv[r].x;
+        else return s_pk[r];
+    };
+    auto e_val = [&](uint32_t r) -> V {
+        if constexpr (PACK) return (V)s_kv[r].y;
+        else if constexpr (HV) return s_val[r];
+        else return V(0);
+    };
+    const uint32_t t = threadIdx.x, l = lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint32_t nch = count[0];
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        const uint4 c4 = list[ci];
+        const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane(c4.x);
+        const uint32_t m = (uint32_t)__builtin_amdgcn_readfirstlane(c4.y) - p;
+        const uint32_t info = (uint32_t)__builtin_amdgcn_readfirstlane(c4.z);
+        const uint32_t u0 = info >> 8, tb = RB_FBITS - (info & 0xFFu);
+        // wave w holds rows [w 64 ipt, (w + 1) 64 ipt), as k_rb_sort
+        const uint32_t ipt = (m + RB_BLOCK - 1) / RB_BLOCK;
+        const uint32_t wbase = w * (PD_WAVE * ipt);
+        const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(m > wbase ? min(PD_WAVE * ipt, m - wbase) : 0u);
+        for (uint32_t i = t; i < RB_FBINS + 4; i += RB_BLOCK) s_bin[i] = 0;
+        const __amdgpu_buffer_rsrc_t rk = buf_rsrc(keys + p + wbase, wrows * 4u);
+        uint32_t key[RB_ITEMS];
+#pragma unroll
+        for (uint32_t i = 0; i < RB_ITEMS; i++)
+            key[i] = i * PD_WAVE < wrows ? __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0) : 0u;
+        V val[HV ? RB_ITEMS : 1];
+        if constexpr (HV) {
+            const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vals + p + wbase, wrows * (uint32_t)sizeof(V));
+#pragma unroll
+            for (uint32_t i = 0; i < RB_ITEMS; i++) {
+                if (i * PD_WAVE >= wrows) val[i] = V(0);
+                else if constexpr (sizeof(V) == 4)
+                    val[i] = __builtin_amdgcn_raw_buffer_load_b32(rv, (l + i * PD_WAVE) * 4u, 0, 0);
+                else
+                    val[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rv, (l + i * PD_WAVE) * 8u, 0, 0));
+            }
+        }
+        __syncthreads();
+        uint32_t br[RB_ITEMS];   // bin << 16 | rank in the bin
+#pragma unroll
+        for (uint32_t i = 0; i < RB_ITEMS; i++) {
+            if (i * PD_WAVE >= wrows) break;
+            const uint32_t b = (((key[i] >> 16) - u0) << tb) | ((key[i] & 0xFFFFu) >> (16 - tb));
+            if (i * PD_WAVE + l < wrows) br[i] = (b << 16) | atomicAdd(&s_bin[b], 1u);
+        }
+        __syncthreads();
+        // exclusive scan of the counts: thread t holds bins [4t, 4t + 4)
+        const uint4 n4 = reinterpret_cast<const uint4*>(s_bin)[t];
+        const uint32_t tsum = n4.x + n4.y + n4.z + n4.w;
+        const uint32_t tinc = wave_incl_scan(tsum);
+        if (l == PD_WAVE - 1) s_fw[w] = tinc;
+        __syncthreads();
+        uint32_t st = tinc - tsum;
+        for (uint32_t i = 0; i < w; i++) st += s_fw[i];
+        reinterpret_cast<uint4*>(s_bin)[t] = make_uint4(st, st + n4.x, st + n4.x + n4.y, st + n4.x + n4.y + n4.z);
+        if (t == RB_BLOCK - 1) s_bin[RB_FBINS] = st + tsum;
+        __syncthreads();
+        // the sort words (low 16 bits << 16 | row) placed by bin
+#pragma unroll
+        for (uint32_t i = 0; i < RB_ITEMS; i++) {
+            if (i * PD_WAVE >= wrows) break;
+            if (i * PD_WAVE + l < wrows) {
+                const uint32_t b = br[i] >> 16;
+                const uint32_t b0 = s_bin[b], b1 = s_bin[b + 1];
+                s_pk[b0 + (br[i] & 0xFFFFu)] = (key[i] << 16) | (wbase + i * PD_WAVE + l);
+                br[i] = (b0 << 16) | (b1 - b0);   // the bin's start and size
+            }
+        }
+        __syncthreads();
+        // a row's place: its bin's start + the bin's words below its own
+        // (independent reads, so they overlap)
+        bool big = false;
+#pragma unroll
+        for (uint32_t i = 0; i < RB_ITEMS; i++) {
+            if (i * PD_WAVE >= wrows) break;
+            if (i * PD_WAVE + l < wrows) {
+                const uint32_t b0 = br[i] >> 16, c = br[i] & 0xFFFFu;
+                const uint32_t mine = (key[i] << 16) | (wbase + i * PD_WAVE + l);
+                uint32_t below = 0;
+                if (c > RB_FIXMAX) {
+                    big = true;
+                } else {
+                    for (uint32_t q = 0; q < c; q++) below += s_pk[b0 + q] < mine ? 1u : 0u;
+                }
+                br[i] = b0 + below;
+            }
+        }
+        if (__syncthreads_or(big)) {
+            if (t == 0) retry[atomicAdd(nretry, 1u)] = c4;
+            continue;   // the chunk is untouched; the barrier above ended every LDS read
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < RB_ITEMS; i++) {
+            if (i * PD_WAVE >= wrows) break;
+            if (i * PD_WAVE + l < wrows) e_put(br[i], key[i], val[HV ? i : 0]);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t k = 0; k < RB_ITEMS; k++) {
+            const uint32_t j = t + k * RB_BLOCK;
+            if (j >= m) break;
+            keys[p + j] = e_pk(j);
+            if constexpr (HV) vals[p + j] = e_val(j);
+        }
+        __syncthreads();   // the next chunk reuses the LDS buffers
+    }
+}
+
 // Bucket scheme for the digits of `run` (knob sort_bucket).  The bucket
 // pass is worth one global pass only when it replaces two (four digits) and
 // its chunks are large: at 2^28 uniform pairs (buckets of about 4096 rows)
@@ -1218,20 +1372,39 @@ static void sort_bucket(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64
     const uint32_t nw = (uint32_t)ceil_div(n, (uint64_t)d.window);
     // bstart[2^16 + 1] | overflow | chunk count | chunk list (at most one
     // chunk per window plus one per bucket)
+    // the counting form needs 32-bit keys bucketed by their high 16 bits
+    // (knob sort_bucket_count, default 1)
+    const bool counting = c->tune_get("sort_bucket_count", 1) != 0 && d.nlow == 2 && d.lshift[0] == 0 &&
+                          d.lshift[1] == 8 && d.lmask[0] == 0xFFu && d.lmask[1] == 0xFFu && d.sa == 16 &&
+                          d.sb == 24 && d.mb == 0xFFu;
+    // bstart[2^16 + 1] | overflow | chunk count | retry count | chunk list
+    // (at most one chunk per window plus one per bucket) | retry list
     const uint64_t maxch = (uint64_t)nw + RB_NB;
-    Scratch sc(c, (RB_NB + 1 + 3) * 4 + maxch * 16, s);
+    const uint64_t head = (RB_NB + 1 + 3) * 4;
+    Scratch sc(c, head + maxch * 16 * (counting ? 2 : 1), s);
     uint32_t* bstart = sc.as<uint32_t>();
     uint32_t* ovf = bstart + RB_NB + 1;
     uint32_t* cnt = ovf + 1;
-    uint4* list = reinterpret_cast<uint4*>(sc.as<char>() + (RB_NB + 1 + 3) * 4);
-    fill_u32(c, ovf, 0, 2, s);
+    uint32_t* nretry = ovf + 2;
+    uint4* list = reinterpret_cast<uint4*>(sc.as<char>() + head);
+    uint4* retry = list + maxch;
+    fill_u32(c, ovf, 0, 3, s);
     launch(c, "radix_bstart", k_rb_starts, dim3(256), dim3(256), 0, s, (const uint32_t*)kout, (uint32_t)n,
            hist + (P - 1) * RS_BINS, d, bstart);
     launch(c, "radix_chunks", k_rb_chunks, dim3((nw + 255) / 256), dim3(256), 0, s, (const uint32_t*)kout, (uint32_t)n,
            nw, (const uint32_t*)bstart, d, list, cnt, ovf);
     const uint32_t grid = std::max<uint32_t>(1, blocks_per_cu(k_rb_sort<V, HV>, RB_BLOCK) * c->cu_count);
-    launch(c, "radix_bucket", k_rb_sort<V, HV>, dim3(grid), dim3(RB_BLOCK), 0, s, kout, vout, (const uint4*)list,
-           (const uint32_t*)cnt, d);
+    if (counting) {
+        // chunks with a large bin (many equal low bits) go to the ranking
+        const uint32_t cgrid = std::max<uint32_t>(1, blocks_per_cu(k_rb_count<V, HV>, RB_BLOCK) * c->cu_count);
+        launch(c, "radix_bucket", k_rb_count<V, HV>, dim3(cgrid), dim3(RB_BLOCK), 0, s, kout, vout,
+               (const uint4*)list, (const uint32_t*)cnt, retry, nretry);
+        launch(c, "radix_bucket_retry", k_rb_sort<V, HV>, dim3(grid), dim3(RB_BLOCK), 0, s, kout, vout,
+               (const uint4*)retry, (const uint32_t*)nretry, d);
+    } else {
+        launch(c, "radix_bucket", k_rb_sort<V, HV>, dim3(grid), dim3(RB_BLOCK), 0, s, kout, vout, (const uint4*)list,
+               (const uint32_t*)cnt, d);
+    }
     uint32_t of = 0;
     read_small(c, &of, ovf, 4, s);
     if (of) {

@@ -161,9 +161,13 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
             // resident join index: the match and its date per fact row
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
+                // loads clamped to the query's last row (r0 < rend): none
+                // waits behind a branch
                 const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
-                dimrow[i] = r < rend ? T.fk_row[r] : 0xFFFFFFFFu;
-                const int32_t dd = r < rend ? T.fk_date[r] : 0;
+                const uint64_t rc = r < rend ? r : rend - 1;
+                const uint32_t dr = T.fk_row[rc];
+                const int32_t dd = T.fk_date[rc];
+                dimrow[i] = r < rend ? dr : 0xFFFFFFFFu;
                 const bool m = dimrow[i] != 0xFFFFFFFFu && (int64_t)dd < q.param;
                 hit |= (uint32_t)m << i;
             }
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
                 const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
-                key[i] = r < rend ? T.orderkey[r] : 0;
+                key[i] = T.orderkey[r < rend ? r : rend - 1];   // clamped (r0 < rend)
             }
             uint4 e[RPT];
 #pragma unroll
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sv_scan(const SvBatch* __restrict_
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
                 const uint64_t r = r0 + (uint64_t)i * SV_BLOCK + t;
-                const bool m = r < rend && T.price[r] < q.param;
+                const bool m = r < rend && T.price[r < rend ? r : rend - 1] < q.param;   // clamped (r0 < rend)
                 hit |= (uint32_t)m << i;
             }
         }

@@ -336,6 +336,54 @@ def test_sort_bucket_scheme_duplicate_keys_stay_stable(ctx, orc):
     assert nb == 1 and np.array_equal(gk, ek) and np.array_equal(gv, ev)
 
 
+@pytest.mark.parametrize("counting", [0, 1])
+def test_sort_bucket_counting_form_matches_ranking(ctx, orc, counting):
+    """Four varying digits take the counting form of the bucket pass (one LDS
+    atomic per row into 2048 coarse bins, each bin ordered by (low bits,
+    row)); knob sort_bucket_count=0 takes the two ranking rounds.  Both
+    against the oracle, with 2^16 repeated keys so equal keys share bins."""
+    n = (1 << 22) + 333
+    k, _ = orc.sort_workload(n, seed=41)
+    k[::7] = k[: (n + 6) // 7][::-1]        # repeats, met out of order
+    v = np.arange(n, dtype=np.uint32)
+    ctx.set_tune("sort_bucket_count", counting)
+    try:
+        gk, gv, nb, npass = _sort_stats(ctx, k, v)
+    finally:
+        ctx.clear_tune("sort_bucket_count")
+    ek, ev = orc.sort_pairs(k, v)
+    assert nb == 1 and npass == 2 and np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+@pytest.mark.parametrize("payload", ["u32", "none", "u64"])
+def test_sort_bucket_counting_large_bins_go_to_ranking(ctx, orc, payload):
+    """Buckets whose rows share their low 16 bits put more than 32 rows in
+    one coarse bin: those chunks are handed, untouched, to the ranking
+    kernel (radix_bucket_retry); the others stay in the counting form."""
+    n = 1 << 22
+    k, _ = orc.sort_workload(n, seed=43)
+    hot = orc.gen_range(np.uint32, 9, 1, 300, 0, 0xFFFFFFFF)
+    k[1::5] = hot[np.arange(k[1::5].size) % hot.size]      # 300 keys about 2800 times each
+    lo = np.uint32(0xBEEF)
+    k[2::11] = (k[2::11] & np.uint32(0xFFFF0000)) | lo    # equal low bits across buckets
+    v = {"u32": np.arange(n, dtype=np.uint32), "none": None,
+         "u64": orc.gen_range(np.int64, 9, 2, n, -(1 << 50), 1 << 50).view(np.uint64)}[payload]
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    ctx.set_tune("sort_bucket", 2)
+    try:
+        gk, gv = gpu_sort(ctx, k, v)
+        assert ctx.kernel_stat("radix_bucket")[0] == 1 and ctx.kernel_stat("radix_bucket_retry")[0] == 1
+    finally:
+        ctx.clear_tune("sort_bucket")
+        ctx.set_profiling(False)
+    if v is None:
+        assert np.array_equal(gk, orc.sort_keys(k))
+    else:
+        ek, ev = orc.sort_pairs(k, v)
+        assert np.array_equal(gk, ek) and np.array_equal(gv.view(ev.dtype), ev)
+
+
 def test_sort_bucket_scheme_oversized_bucket_falls_back(ctx, orc):
     """High byte == second byte: the histograms look uniform but only 256
     buckets hold rows (16384 each, beyond a block's 8192), so the bucket pass
