@@ -1284,7 +1284,9 @@ __global__ __launch_bounds__(RB_BLOCK, 4) void k_rb_count(uint32_t* __restrict__
         }
         __syncthreads();
         // a row's place: its bin's start + the bin's words below its own
-        // (independent reads, so they overlap)
+        // (one pass over a bin of about three words per row; reading word q
+        // of every row's bin in one step took 1.66 ms against 1.38 ms, since
+        // every step then issues a read for each item)
         bool big = false;
 #pragma unroll
         for (uint32_t i = 0; i < RB_ITEMS; i++) {
@@ -1348,7 +1350,8 @@ static bool use_bucket(pimdal_ctx* c, const RsPassDesc& run, uint64_t n, const u
 
 template <typename V, bool HV>
 static void sort_bucket(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64_t n, uint32_t* kout, V* vout,
-                        const RsPassDesc& run, const uint32_t* hist, uint32_t* offs, uint32_t* ctrs, hipStream_t s) {
+                        const RsPassDesc& run, uint32_t* hist, uint32_t* offs, uint32_t* ctrs, bool low_counted,
+                        hipStream_t s) {
     const uint32_t P = run.npass;
     // the two high digits: in -> scratch -> out
     RsPassDesc hi{};
@@ -1413,6 +1416,20 @@ static void sort_bucket(pimdal_ctx* c, const uint32_t* kin, const V* vin, uint64
         copy_d2d(c, tk.get(), kout, n * 4, s);
         if (HV) copy_d2d(c, tv.get(), vout, n * sizeof(V), s);
         fill_u32(c, ctrs, 0, 4, s);
+        if (!low_counted) {
+            // only the high digits were counted: the low ones now (four
+            // digits at shifts 0, 8, 16, 24)
+            RsPassDesc lo2{};
+            lo2.npass = 2;
+            for (int j = 0; j < 2; j++) {
+                lo2.shift[j] = 8 * j;
+                lo2.mask[j] = 0xFF;
+            }
+            const uint32_t hg = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
+            // (ctrs - 2: the AND/OR words, which the same keys leave as they are)
+            launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hg)), dim3(512), 0, s,
+                   (const uint32_t*)tk.as<uint32_t>(), n, lo2, hist, ctrs - 2);
+        }
         run_passes<V, HV>(c, tk.as<uint32_t>(), HV ? tv.as<V>() : (const V*)nullptr, n, kout, vout, run, hist, offs,
                           ctrs, s);
     }
@@ -1464,19 +1481,60 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
     fill_u32(c, andor, 0xFFFFFFFFu, 1, s);
     fill_u32(c, andor + 1, 0, 1 + 4, s);
     const uint32_t hist_grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 512 * 16), (uint64_t)c->cu_count * 4);
-    if (all.npass)
+    const bool bucket_ok = c->tune_get("sort_bucket", 1) != 0 && !c->capture;
+    // Where the bucket scheme is all but certain (four digits, buckets of
+    // at least n / 2^16 >= 3072 rows), the first read counts only the two
+    // high digits, which are all it uses; the low digits are counted later
+    // if ordinary passes turn out to need them.
+    bool std4 = all.npass == 4;
+    for (int p = 0; p < 4 && std4; p++) std4 = all.shift[p] == 8u * p && all.mask[p] == 0xFFu;
+    const bool hi_first = bucket_ok && std4 && c->tune_get("sort_bucket", 1) == 1 && n >= 3072ull * RB_NB;
+    RsPassDesc lo2{}, hi2{};
+    lo2.npass = hi2.npass = 2;
+    for (int j = 0; j < 2; j++) {
+        lo2.shift[j] = 8 * j;
+        hi2.shift[j] = 16 + 8 * j;
+        lo2.mask[j] = hi2.mask[j] = 0xFF;
+    }
+    auto count_low = [&](const uint32_t* keys) {
+        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, keys, n, lo2, hist,
+               andor);
+    };
+    if (hi_first)
+        launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, hi2,
+               hist + 2 * RS_BINS, andor);
+    else if (all.npass)
         launch(c, "radix_hist", k_rs_hist<false>, dim3(std::max(1u, hist_grid)), dim3(512), 0, s, kin, n, all, hist, andor);
 
     RsPassDesc run = all;
     // host copy of hist[4][256] | offs | andor, read once when the digit
     // detection or the bucket scheme's check needs it
-    const bool bucket_ok = c->tune_get("sort_bucket", 1) != 0 && !c->capture;
     std::vector<uint32_t> hh;
     if (detect || (bucket_ok && all.npass >= 3)) {
         hh.resize(4 * RS_BINS * 2 + 2);
         read_small(c, hh.data(), hist, hh.size() * 4, s);
     }
     int orig[4] = {0, 1, 2, 3};   // run digit -> its histogram in hh
+    if (hi_first) {
+        // the bucket scheme now, or the low digits counted for ordinary passes
+        // with the digits detected, all four must vary
+        const uint32_t varying = hh[4 * RS_BINS * 2] ^ hh[4 * RS_BINS * 2 + 1];
+        bool four = true;
+        for (int p = 0; p < 4; p++) four = four && (!detect || ((varying >> (8 * p)) & 0xFF) != 0);
+        if (four && use_bucket(c, all, n, hh.data() + 2 * RS_BINS)) {
+            if (vw == 0)
+                sort_bucket<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, all, hist, offs, ctrs, false, s);
+            else if (vw == 4)
+                sort_bucket<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, all, hist, offs,
+                                            ctrs, false, s);
+            else
+                sort_bucket<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, all, hist, offs,
+                                            ctrs, false, s);
+            return;
+        }
+        count_low(kin);
+        read_small(c, hh.data(), hist, 2 * RS_BINS * 4, s);
+    }
     if (detect) {
         const uint32_t varying = hh[4 * RS_BINS * 2] ^ hh[4 * RS_BINS * 2 + 1];
         // keep only digits with a varying bit; move their histograms down
@@ -1503,13 +1561,13 @@ void sort_pairs(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_t vw
         std::copy_n(hh.begin() + orig[run.npass - 1] * RS_BINS, RS_BINS, h2.begin() + RS_BINS);
         if (use_bucket(c, run, n, h2.data())) {
             if (vw == 0)
-                sort_bucket<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, s);
+                sort_bucket<uint32_t, false>(c, kin, nullptr, n, kout, nullptr, run, hist, offs, ctrs, true, s);
             else if (vw == 4)
                 sort_bucket<uint32_t, true>(c, kin, (const uint32_t*)vin, n, kout, (uint32_t*)vout, run, hist, offs,
-                                            ctrs, s);
+                                            ctrs, true, s);
             else
                 sort_bucket<uint64_t, true>(c, kin, (const uint64_t*)vin, n, kout, (uint64_t*)vout, run, hist, offs,
-                                            ctrs, s);
+                                            ctrs, true, s);
             return;
         }
     }

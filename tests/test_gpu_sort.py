@@ -102,6 +102,43 @@ def test_sort_2_28_properties(ctx, zipf):
     assert np.all(gv[:-1][ties] < gv[1:][ties])
 
 
+@pytest.mark.parametrize("bits", [(-1, -1), (0, 32)])
+def test_sort_2_28_oversized_buckets_count_low_digits_late(ctx, orc, bits):
+    """At 2^28 the first read counts only the two high digits.  Keys whose
+    high byte equals the second fill 256 buckets of about 2^20 rows: the
+    bucket pass flags them, the low digits are counted then, and every digit
+    is sorted by ordinary passes.  Row ids as payloads; sorted, stable, a
+    permutation, out_key[i] == in_key[out_row[i]]."""
+    n = 1 << 28
+    b = orc.gen_range(np.uint32, 12, 1, n, 0, 255)
+    k = (b << np.uint32(24)) | (b << np.uint32(16)) | orc.gen_range(np.uint32, 12, 2, n, 0, (1 << 16) - 1)
+    del b
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    try:
+        dk = ctx.to_device(k)
+        dv = ctx.to_device(np.arange(n, dtype=np.uint32))
+        ok, ov = ops.sort_pairs(ctx, dk, dv, *bits)
+        assert ctx.kernel_stat("radix_bucket")[0] == 1
+        assert ctx.kernel_stat("radix_hist")[0] == 2 and ctx.kernel_stat("radix_onesweep")[0] == 2 + 4
+    finally:
+        ctx.set_profiling(False)
+    dk.free()
+    dv.free()
+    gk = ok.numpy()
+    ok.free()
+    gv = ov.numpy()
+    ov.free()
+    assert np.all(gk[:-1] <= gk[1:])
+    seen = np.zeros(n, np.bool_)
+    seen[gv] = True
+    assert seen.all()
+    del seen
+    assert np.array_equal(k[gv], gk)
+    ties = gk[:-1] == gk[1:]
+    assert np.all(gv[:-1][ties] < gv[1:][ties])
+
+
 def test_autotune_sets_knobs_and_results_stay_exact(ctx, orc, tmp_path):
     cache = str(tmp_path / "tune.txt")
     ctx.autotune("sort", 1 << 22, cache)
