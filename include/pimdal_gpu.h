@@ -91,7 +91,11 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   hold 3072 to 6144 rows by a histogram estimate, as 2^28 uniform keys
  *   do, global passes over the two high digits only, then one pass that
  *   sorts each bucket by the two low digits in LDS; 2: that scheme for any
- *   3 or 4 digits whose estimate fits; 0: one global pass per digit);
+ *   3 or 4 digits whose estimate fits; 0: one global pass per digit),
+ *   sort_bucket_count (1, the default: for 32-bit keys the bucket pass
+ *   ranks rows by one LDS atomic into 2048 coarse bins and orders each bin
+ *   by (low 16 bits, row), chunks with a bin above 32 rows going to the
+ *   match-mask ranking; 0: the match-mask ranking for every chunk);
  *   select_mode (0 one pass with decoupled look-back, 1 count pass + write
  *   pass), select_block (256, 512, 1024 threads; 1024 rows per wave per tile),
  *   select_lookback_width (1 or 4 status words per lane), select_blocks_per_cu;
