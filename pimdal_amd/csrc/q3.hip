@@ -15,8 +15,9 @@
 // (host_q3.cpp:300-345 merges the devices' top rows).
 //
 // Plan on gfx950 (nothing leaves the device between the steps):
-//  1. k_q3_cust    the qualifying customers' keys into a hash set in HBM
-//                  (a few hundred thousand keys: it stays in the caches);
+//  1. k_q3_cust    the qualifying customers' keys: those below 2^25 as
+//                  bits of a 4 MB bitmap (an L2 read per probe for dense
+//                  custkeys), larger ones into a hash set in HBM;
 //  2. k_q3_osel / k_q3_orders   orders with o_orderdate < date whose
 //                  customer is in the set: listed (one pass over the orders),
 //                  then inserted into an open-addressing table (orderkey,
@@ -41,12 +42,22 @@ namespace pd {
 
 __device__ __forceinline__ uint64_t q3_slot(uint32_t key, uint64_t mask) { return pd_mix64_dev(key) & mask; }
 
-// Customer set: slot word = 1 << 32 | custkey (0 = empty).
+// Customer set: keys below 2^Q3_CBITS as bits of a 4 MB bitmap (TPC-H's
+// dense custkeys touch 1.5e6 bits at SF10, which stay in L2, where a probe
+// of the 32 MB hash set is a random MALL read), larger keys in the hash set
+// (slot word = 1 << 32 | custkey, 0 = empty).
+constexpr uint32_t Q3_CBITS = 25;   // the largest bitmap: 2^25 keys, 4 MB
+
 __global__ void k_q3_cust(const int32_t* __restrict__ custkey, const uint8_t* __restrict__ segment, uint64_t n,
-                          uint32_t code, uint64_t* __restrict__ set, uint64_t mask) {
+                          uint32_t code, uint32_t* __restrict__ cbits, uint32_t dn, uint64_t* __restrict__ set,
+                          uint64_t mask) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         if (segment[i] != code) continue;
         const uint32_t k = (uint32_t)custkey[i];
+        if (k < dn) {
+            atomicOr(&cbits[k >> 5], 1u << (k & 31));
+            continue;
+        }
         const uint64_t want = (1ull << 32) | k;
         uint64_t h = q3_slot(k, mask);
         while (true) {
@@ -65,9 +76,9 @@ __global__ void k_q3_cust(const int32_t* __restrict__ custkey, const uint8_t* __
 constexpr uint32_t Q3_CHUNK = 8192;
 
 __global__ __launch_bounds__(256) void k_q3_osel(const int32_t* __restrict__ odate, const int32_t* __restrict__ ocust,
-                                                 uint64_t n, int32_t date, const uint64_t* __restrict__ set,
-                                                 uint64_t mask, unsigned long long* __restrict__ ctr,
-                                                 uint32_t* __restrict__ rows) {
+                                                 uint64_t n, int32_t date, const uint32_t* __restrict__ cbits,
+                                                 uint32_t dn, const uint64_t* __restrict__ set, uint64_t mask,
+                                                 unsigned long long* __restrict__ ctr, uint32_t* __restrict__ rows) {
     constexpr int PER = Q3_CHUNK / 256;   // 32 rows per thread
     __shared__ uint32_t s_w[4];
     __shared__ unsigned long long s_base;
@@ -91,11 +102,21 @@ __global__ __launch_bounds__(256) void k_q3_osel(const int32_t* __restrict__ oda
             key[q] = cand[q] ? cu : 0u;
             h[q] = q3_slot(key[q], mask);
         }
+        bool small[4];
+        uint32_t cw[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) e[q] = cand[q] ? set[h[q]] : 0ull;
+        for (int q = 0; q < 4; q++) {
+            small[q] = key[q] < dn;
+            cw[q] = cand[q] && small[q] ? cbits[key[q] >> 5] : 0u;
+            e[q] = cand[q] && !small[q] ? set[h[q]] : 0ull;
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (!cand[q]) continue;
+            if (small[q]) {
+                if ((cw[q] >> (key[q] & 31)) & 1u) bits |= 1u << (j0 + q);
+                continue;
+            }
             const uint64_t want = (1ull << 32) | key[q];
             while (e[q] != 0 && e[q] != want) {
                 h[q] = (h[q] + 1) & mask;
@@ -425,6 +446,15 @@ static uint64_t q3_pow2_slots(uint64_t n) {
     return s;
 }
 
+// Keys a direct customer structure covers: the power of two at or above
+// n_customer + 1 (TPC-H custkeys are 1 .. n_customer), from 2^16 up to
+// 2^maxbits; larger keys go to the hash tables.
+static uint32_t q3_direct_keys(uint64_t n_customer, uint32_t maxbits) {
+    uint64_t d = 1ull << 16;
+    while (d < n_customer + 1 && d < (1ull << maxbits)) d <<= 1;
+    return (uint32_t)d;
+}
+
 static dim3 q3_grid(pimdal_ctx* c, uint64_t n) {
     return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), (uint64_t)c->cu_count * 16)));
 }
@@ -440,18 +470,20 @@ uint64_t tpch_q3(pimdal_ctx* c, const pimdal_q3_tables& t, uint32_t segment, int
                PIMDAL_E_INVALID, "tables too large (at most 2^30-1 orders)");
     // 1. customer set
     const uint64_t cslots = q3_pow2_slots(t.n_customer);
-    Scratch cset(c, cslots * 8, s), small(c, 64, s);
+    const uint32_t dn = q3_direct_keys(t.n_customer, Q3_CBITS);
+    Scratch cset(c, cslots * 8, s), cbits(c, dn / 8, s), small(c, 64, s);
     fill_u64(c, cset.get(), 0, cslots, s);
+    fill_u32(c, cbits.get(), 0, dn / 32, s);
     fill_u64(c, small.get(), 0, 8, s);
     unsigned long long* cnt = small.as<unsigned long long>();   // [0] qualifying orders, [1] groups
     uint32_t* dup = reinterpret_cast<uint32_t*>(small.as<uint64_t>() + 2);
     launch(c, "q3_cust", k_q3_cust, q3_grid(c, t.n_customer), dim3(256), 0, s, t.c_custkey, t.c_mktsegment,
-           t.n_customer, segment, cset.as<uint64_t>(), cslots - 1);
+           t.n_customer, segment, cbits.as<uint32_t>(), dn, cset.as<uint64_t>(), cslots - 1);
     // 2. qualifying orders: their rows listed, then a table sized for them
     Scratch orows(c, std::max<uint64_t>(t.n_orders, 1) * 4, s);
     launch(c, "q3_osel", k_q3_osel, dim3((uint32_t)std::max<uint64_t>(1, ceil_div(t.n_orders, Q3_CHUNK))), dim3(256), 0,
-           s, t.o_orderdate, t.o_custkey, t.n_orders, date, (const uint64_t*)cset.as<uint64_t>(), cslots - 1, cnt,
-           orows.as<uint32_t>());
+           s, t.o_orderdate, t.o_custkey, t.n_orders, date, (const uint32_t*)cbits.as<uint32_t>(), dn,
+           (const uint64_t*)cset.as<uint64_t>(), cslots - 1, cnt, orows.as<uint32_t>());
     uint64_t nq = 0;
     read_small(c, &nq, cnt, 8, s);
     const uint64_t oslots = q3_pow2_slots(nq);
@@ -812,22 +844,37 @@ __device__ __forceinline__ int32_t q5_find(const uint64_t* tab, uint64_t mask, u
     }
 }
 
+// The region's customers: keys below `dn` in a direct array (entry = nation
+// + 1, 0 = none; dense TPC-H custkeys make it a few MB that the caches
+// hold, where the hash table is a random MALL read per probe), larger keys
+// in the hash table.  A key met twice sets *dup either way.
 __global__ void k_q5_cust(const int32_t* __restrict__ ckey, const int32_t* __restrict__ cnat, uint64_t n,
-                          const uint32_t* __restrict__ flags, uint64_t* __restrict__ tab, uint64_t mask,
-                          uint32_t* __restrict__ dup) {
+                          const uint32_t* __restrict__ flags, uint32_t* __restrict__ cdir, uint32_t dn,
+                          uint64_t* __restrict__ tab, uint64_t mask, uint32_t* __restrict__ dup) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const int32_t nat = cnat[i];
         if (nat < 0 || nat >= (int32_t)Q5_NATIONS || !flags[nat]) continue;
-        q5_insert(tab, mask, (uint32_t)ckey[i], (uint32_t)nat, dup);
+        const uint32_t k = (uint32_t)ckey[i];
+        if (k < dn) {
+            if (atomicCAS(&cdir[k], 0u, (uint32_t)nat + 1) != 0) atomicOr(dup, 1u);
+            continue;
+        }
+        q5_insert(tab, mask, k, (uint32_t)nat, dup);
     }
+}
+
+// The nation of customer `key`, or -1.
+__device__ __forceinline__ int32_t q5_cust_nation(const uint32_t* cdir, uint32_t dn, const uint64_t* tab,
+                                                  uint64_t mask, uint32_t key) {
+    return key < dn ? (int32_t)cdir[key] - 1 : q5_find(tab, mask, key);
 }
 
 // Orders of [lo, hi) whose customer is in the table: their rows listed
 // (one counter atomic per 8192-row block, as k_q3_osel).
 __global__ __launch_bounds__(256) void k_q5_osel(const int32_t* __restrict__ odate, const int32_t* __restrict__ ocust,
-                                                 uint64_t n, int32_t lo, int32_t hi, const uint64_t* __restrict__ ctab,
-                                                 uint64_t cmask, unsigned long long* __restrict__ ctr,
-                                                 uint32_t* __restrict__ rows) {
+                                                 uint64_t n, int32_t lo, int32_t hi, const uint32_t* __restrict__ cdir,
+                                                 uint32_t dn, const uint64_t* __restrict__ ctab, uint64_t cmask,
+                                                 unsigned long long* __restrict__ ctr, uint32_t* __restrict__ rows) {
     constexpr int PER = Q3_CHUNK / 256;
     __shared__ uint32_t s_w[4];
     __shared__ unsigned long long s_base;
@@ -851,11 +898,19 @@ __global__ __launch_bounds__(256) void k_q5_osel(const int32_t* __restrict__ oda
             key[q] = cu;
             h[q] = q3_slot(cu, cmask);
         }
+        uint32_t dv[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) e[q] = cand[q] ? ctab[h[q]] : 0ull;
+        for (int q = 0; q < 4; q++) {
+            dv[q] = cand[q] && key[q] < dn ? cdir[key[q]] : 0u;
+            e[q] = cand[q] && key[q] >= dn ? ctab[h[q]] : 0ull;
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (!cand[q]) continue;
+            if (key[q] < dn) {
+                if (dv[q] != 0) bits |= 1u << (j0 + q);
+                continue;
+            }
             while (e[q] != 0 && (uint32_t)e[q] != key[q]) {
                 h[q] = (h[q] + 1) & cmask;
                 e[q] = ctab[h[q]];
@@ -886,13 +941,13 @@ __global__ __launch_bounds__(256) void k_q5_osel(const int32_t* __restrict__ oda
 // The listed orders tabled orderkey -> their customer's nation, with their
 // filter bits.
 __global__ void k_q5_orders(const uint32_t* __restrict__ rows, uint64_t nq, const int32_t* __restrict__ okey,
-                            const int32_t* __restrict__ ocust, const uint64_t* __restrict__ ctab, uint64_t cmask,
-                            uint64_t* __restrict__ otab, uint64_t omask, uint32_t* __restrict__ bits, uint32_t bmask,
-                            uint32_t* __restrict__ dup) {
+                            const int32_t* __restrict__ ocust, const uint32_t* __restrict__ cdir, uint32_t dn,
+                            const uint64_t* __restrict__ ctab, uint64_t cmask, uint64_t* __restrict__ otab,
+                            uint64_t omask, uint32_t* __restrict__ bits, uint32_t bmask, uint32_t* __restrict__ dup) {
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nq; j += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = rows[j];
         const uint32_t k = (uint32_t)okey[i];
-        const int32_t nat = q5_find(ctab, cmask, (uint32_t)ocust[i]);
+        const int32_t nat = q5_cust_nation(cdir, dn, ctab, cmask, (uint32_t)ocust[i]);
         q5_insert(otab, omask, k, (uint32_t)nat, dup);
         const uint32_t b = q4_bit(k, bmask);
         atomicOr(&bits[b >> 5], 1u << (b & 31));
@@ -1019,15 +1074,17 @@ uint64_t tpch_q5(pimdal_ctx* c, const pimdal_q5_tables& t, uint32_t region_name,
     launch(c, "q5_nations", k_q5_nations, dim3(1), dim3(256), 0, s, t.r_regionkey, t.r_name, t.n_region, t.n_nationkey,
            t.n_regionkey, t.n_nation, region_name, flags);
     const uint64_t cslots = q3_pow2_slots(t.n_customer);
-    Scratch ctab(c, cslots * 8, s);
+    const uint32_t dn = q3_direct_keys(t.n_customer, 22);
+    Scratch ctab(c, cslots * 8, s), cdir(c, (uint64_t)dn * 4, s);
     fill_u64(c, ctab.get(), 0, cslots, s);
+    fill_u32(c, cdir.get(), 0, dn, s);
     launch(c, "q5_cust", k_q5_cust, q3_grid(c, t.n_customer), dim3(256), 0, s, t.c_custkey, t.c_nationkey, t.n_customer,
-           (const uint32_t*)flags, ctab.as<uint64_t>(), cslots - 1, dup);
+           (const uint32_t*)flags, cdir.as<uint32_t>(), dn, ctab.as<uint64_t>(), cslots - 1, dup);
     // 3. orders of the range with a customer of the region
     Scratch orows(c, std::max<uint64_t>(t.n_orders, 1) * 4, s);
     launch(c, "q5_osel", k_q5_osel, dim3((uint32_t)std::max<uint64_t>(1, ceil_div(t.n_orders, Q3_CHUNK))), dim3(256), 0,
-           s, t.o_orderdate, t.o_custkey, t.n_orders, date_start, date_end, (const uint64_t*)ctab.as<uint64_t>(),
-           cslots - 1, meta, orows.as<uint32_t>());
+           s, t.o_orderdate, t.o_custkey, t.n_orders, date_start, date_end, (const uint32_t*)cdir.as<uint32_t>(), dn,
+           (const uint64_t*)ctab.as<uint64_t>(), cslots - 1, meta, orows.as<uint32_t>());
     uint32_t hflags[2] = {0, 0};
     read_small(c, hflags, flags + Q5_NATIONS, 8, s);
     PD_REQUIRE(hflags[0] == 0, PIMDAL_E_UNSUPPORTED, "nation keys must lie in [0, 1024)");
@@ -1043,8 +1100,8 @@ uint64_t tpch_q5(pimdal_ctx* c, const pimdal_q5_tables& t, uint32_t region_name,
     fill_u32(c, fbits.get(), 0, nbits / 32, s);
     fill_u64(c, stab.get(), 0, sslots, s);
     launch(c, "q5_orders", k_q5_orders, q3_grid(c, nq), dim3(256), 0, s, (const uint32_t*)orows.as<uint32_t>(), nq,
-           t.o_orderkey, t.o_custkey, (const uint64_t*)ctab.as<uint64_t>(), cslots - 1, otab.as<uint64_t>(), oslots - 1,
-           fbits.as<uint32_t>(), (uint32_t)(nbits - 1), dup);
+           t.o_orderkey, t.o_custkey, (const uint32_t*)cdir.as<uint32_t>(), dn, (const uint64_t*)ctab.as<uint64_t>(),
+           cslots - 1, otab.as<uint64_t>(), oslots - 1, fbits.as<uint32_t>(), (uint32_t)(nbits - 1), dup);
     // 4. suppliers, 5. lineitems
     launch(c, "q5_supp", k_q5_supp, q3_grid(c, t.n_supplier), dim3(256), 0, s, t.s_suppkey, t.s_nationkey, t.n_supplier,
            stab.as<uint64_t>(), sslots - 1, dup);

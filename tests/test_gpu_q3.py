@@ -74,6 +74,27 @@ def test_q3_edge_cases(ctx, orc):
         _check(got, exp)
 
 
+def test_q3_custkeys_beyond_the_bitmap(ctx, orc):
+    """Customer keys below the bitmap's size (the power of two above the
+    customer count, 2^16 here) go to the bitmap, the others (up to 2^31 - 1,
+    and negative ones, 2^31 and up as unsigned) to the hash set; keys on
+    both sides of the boundary."""
+    t = datagen.q3_tables(200_000, seed=61)
+    old = t["c_custkey"].astype(np.int64)
+    new = np.where(old % 3 == 0, old + (1 << 25) + 10, np.where(old % 3 == 1, -old, old))
+    new[:6] = [65535, 65536, (1 << 25) - 1, 1 << 25, (1 << 31) - 1, -(1 << 31)]
+    assert np.unique(new).size == new.size
+    remap = np.zeros(old.max() + 1, np.int64)
+    remap[old] = new
+    t["c_custkey"] = new.astype(np.int32)
+    t["o_custkey"] = remap[t["o_custkey"]].astype(np.int32)
+    for seg in range(5):
+        exp, g_exp = orc.q3(t, seg, DATE)
+        got, g = ops.tpch_q3(ctx, _dev(ctx, t), seg, DATE, capacity=max(g_exp, 1))
+        assert g == g_exp and g > 0
+        _check(got, exp)
+
+
 def test_q3_repeated_orderkey_is_refused(ctx):
     t = datagen.q3_tables(20_000, seed=51)
     t["o_orderkey"][:] = t["o_orderkey"][0]

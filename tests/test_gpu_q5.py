@@ -62,6 +62,23 @@ def test_q5_sf10_lineitems(ctx, orc):
     assert _check(ctx, orc, t, ASIA, LO, HI) == 5
 
 
+def test_q5_custkeys_beyond_the_direct_array(ctx, orc):
+    """Customer keys below the direct array's size (the power of two above
+    the customer count, 2^16 here) are looked up there, larger and negative
+    keys in the hash table; keys on both sides of the boundary."""
+    t = datagen.q5_tables(100_000, seed=65)
+    old = t["c_custkey"].astype(np.int64)
+    new = np.where(old % 3 == 0, old + (1 << 25) + 10, np.where(old % 3 == 1, -old, old))
+    new[:4] = [65535, 65536, (1 << 31) - 1, -(1 << 31)]
+    assert np.unique(new).size == new.size
+    remap = np.zeros(old.max() + 1, np.int64)
+    remap[old] = new
+    t["c_custkey"] = new.astype(np.int32)
+    t["o_custkey"] = remap[t["o_custkey"]].astype(np.int32)
+    for region in range(5):
+        assert _check(ctx, orc, t, region, LO, HI) > 0
+
+
 def test_q5_refusals(ctx):
     t = datagen.q5_tables(20_000, seed=55)
     bad = dict(t)
@@ -73,3 +90,13 @@ def test_q5_refusals(ctx):
     bad["s_suppkey"] = np.full_like(t["s_suppkey"], t["s_suppkey"][0])
     with pytest.raises(_lib.PimdalError, match="unique"):
         ops.tpch_q5(ctx, _dev(ctx, bad), ASIA, LO, HI)
+    # a region's customer key twice: in the direct array, and in the hash table
+    for key in (7, 1 << 30):
+        bad = dict(t)
+        bad["c_custkey"] = t["c_custkey"].copy()
+        bad["c_nationkey"] = t["c_nationkey"].copy()
+        bad["c_custkey"][:2] = key
+        rk = t["r_regionkey"][t["r_name"] == ASIA][0]
+        bad["c_nationkey"][:2] = t["n_nationkey"][t["n_regionkey"] == rk][0]
+        with pytest.raises(_lib.PimdalError, match="c_custkey"):
+            ops.tpch_q5(ctx, _dev(ctx, bad), ASIA, LO, HI)
