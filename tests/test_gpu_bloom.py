@@ -22,7 +22,7 @@ def test_bloom_filters_match_oracle(ctx, orc, part_bits, shift, nb):
     assert np.array_equal(filt.view(np.uint8), orc.bloom_build(keys, part_bits, shift))
 
 
-@pytest.mark.parametrize("part_bits,shift", [(0, 32), (5, 32), (9, 30)])
+@pytest.mark.parametrize("part_bits,shift", [(0, 32), (5, 32), (6, 32), (7, 20), (9, 30), (10, 24)])
 @pytest.mark.parametrize("np_", [0, 1, 4097, 1 << 21])
 @pytest.mark.parametrize("with_payload", [True, False])
 def test_bloom_probe_matches_oracle(ctx, orc, part_bits, shift, np_, with_payload):
