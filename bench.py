@@ -125,7 +125,7 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
     return out
 
 
-KERNELS = ("bloom_build", "bloom_flags", "date32_pack", "part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
+KERNELS = ("bloom_build", "bloom_flags", "bloom_group", "bloom_or", "date32_pack", "part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
            "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
            "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",

@@ -19,11 +19,19 @@
 //                   (filters staged in LDS when they fit, else read through
 //                   L2/MALL); the kept rows' payloads and row indices are
 //                   then compacted in row order by the selection pass
+//   k_bloom_group   64 to 1024 filters: groups of 64 filters held in LDS,
+//                   each group's blocks streaming every probe key, kept rows
+//                   marked in one bitmap per group
+//   k_bloom_or      keep flags from the groups' bitmaps, for the selection
 #include "pd_runtime.hpp"
 #include "pd_device.hpp"
 #include "pd_ops.hpp"
 
 namespace pd {
+
+// Compiler-only barrier: LDS accesses of one wave execute in issue order, so
+// the queue's writes and reads need ordering against code motion only.
+#define PD_BG_BARRIER() __asm__ volatile("" ::: "memory")
 
 constexpr uint32_t BLOOM_BITS = 16384;             // hash_steps.c:15 FILTER_SIZE
 constexpr uint32_t BLOOM_WORDS = BLOOM_BITS / 32;  // 512 uint32 words per filter
@@ -122,6 +130,158 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_flags(const uint32_t* __restric
     for (uint64_t i = n4 * 4 + blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += stride) flag[i] = keep(keys[i]);
 }
 
+// ---- probe with the filters split into groups held in LDS ----
+// 64 filters (128 KB) per group: partition p is filter p % 64 of group
+// p / 64.  Block b holds group b % G in LDS (with G = 8 each XCD's blocks
+// hold one group) and streams the probe keys in chunks of 16384 or 32768
+// rows: chunks j, j + B, ... (j = b / G) for the B = gridDim.x / G blocks of
+// its group, so every group reads every key once (G reads of the key
+// column in all).  Each wave tests the group of 64 rows at a time (one
+// mixer), appends the rows of its own group to a per-wave LDS queue (key,
+// row in the chunk and filter), and tests the four filter bits of 64 queued
+// rows at a time, so the bit tests run on full waves whatever the group's
+// share of the rows.  Kept rows set their bit in an LDS bitmap of the chunk,
+// written out per group: bm[g][chunk * (chunk rows / 32) + word].  With the
+// filters in HBM every bit test is a random L2 read (about 275 G reads/s on
+// the chip); here it is an LDS read, and what remains is the G-fold read of
+// the keys and the per-key group test (512 filters x 2^27 keys: 1.29 ms
+// against 1.95 ms; 128 filters: 0.54 against 1.78 ms).
+constexpr uint32_t BG_FILTERS = 64;
+constexpr uint32_t BG_BLOCK = 1024;
+constexpr uint32_t BG_WAVES = BG_BLOCK / PD_WAVE;
+constexpr uint32_t BG_Q = 128;                        // queue entries per wave
+
+// ITEMS rows per thread per chunk (chunk = 1024 ITEMS rows); the block's
+// next D chunks are loading while it tests one.
+template <uint32_t ITEMS, int D>
+__global__ __launch_bounds__(BG_BLOCK) void k_bloom_group(const uint32_t* __restrict__ keys, uint64_t n,
+                                                          uint32_t part_bits, uint32_t shift, uint32_t G,
+                                                          const uint32_t* __restrict__ filt,
+                                                          uint32_t* __restrict__ bm) {
+    constexpr uint32_t CHUNK = BG_BLOCK * ITEMS;
+    constexpr uint32_t CW = CHUNK / 32;                       // bitmap words per chunk
+    constexpr uint32_t RB = ITEMS == 32 ? 15 : 14;            // bits of a row in the chunk
+    static_assert((1u << RB) == CHUNK, "chunk rows");
+    __shared__ __align__(16) uint32_t s_f[BG_FILTERS * BLOOM_WORDS];
+    __shared__ uint32_t s_bm[CW];
+    __shared__ uint32_t s_qk[BG_WAVES][BG_Q];
+    __shared__ uint32_t s_qr[BG_WAVES][BG_Q];   // row in the chunk | filter in the group << RB
+    const uint32_t t = threadIdx.x, l = lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint32_t g = blockIdx.x % G, B = gridDim.x / G;
+    const uint32_t* fg = filt + (uint64_t)g * BG_FILTERS * BLOOM_WORDS;
+    if (((uintptr_t)fg & 15) == 0) {
+        for (uint32_t i = t; i < BG_FILTERS * BLOOM_WORDS / 4; i += BG_BLOCK)
+            reinterpret_cast<uint4*>(s_f)[i] = reinterpret_cast<const uint4*>(fg)[i];
+    } else {
+        for (uint32_t i = t; i < BG_FILTERS * BLOOM_WORDS; i += BG_BLOCK) s_f[i] = fg[i];
+    }
+    for (uint32_t i = t; i < CW; i += BG_BLOCK) s_bm[i] = 0;
+    const uint64_t nchunks = (n + CHUNK - 1) / CHUNK;
+    uint32_t* out = bm + (uint64_t)g * nchunks * CW;
+    auto load = [&](uint64_t ch, uint32_t (&k)[ITEMS], uint32_t& wrows) {
+        const uint64_t wave_base = ch * CHUNK + (uint64_t)w * (PD_WAVE * ITEMS);
+        wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * ITEMS), n - wave_base) : 0));
+        const __amdgpu_buffer_rsrc_t rk = buf_rsrc(keys + wave_base, wrows * 4u);
+#pragma unroll
+        for (uint32_t i = 0; i < ITEMS; i++) k[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0);
+    };
+    // the queue's first entry and length (wave-uniform)
+    uint32_t qh = 0, qn = 0;
+    // test `cnt` queued rows from the head (lane i takes entry head + i)
+    auto drain = [&](uint32_t cnt) {
+        PD_BG_BARRIER();
+        if (l < cnt) {
+            const uint32_t e = (qh + l) & (BG_Q - 1);
+            const uint32_t k = s_qk[w][e], r = s_qr[w][e];
+            uint32_t b[4];
+            bloom_bits(k, b);
+            const uint32_t* f = s_f + (r >> RB) * BLOOM_WORDS;
+            uint32_t hit = 1;
+#pragma unroll
+            for (int h = 0; h < 4; h++) hit &= (f[b[h] >> 5] >> (b[h] & 31)) & 1u;
+            if (hit) atomicOr(&s_bm[(r & (CHUNK - 1)) >> 5], 1u << (r & 31));
+        }
+        PD_BG_BARRIER();
+        qh = (qh + cnt) & (BG_Q - 1);
+        qn -= cnt;
+    };
+    uint32_t key[D + 1][ITEMS], rows[D + 1];
+    // this block's chunks: j0 + i B for i < K
+    const uint64_t j0 = blockIdx.x / G;
+    const uint64_t K = j0 < nchunks ? (nchunks - j0 + B - 1) / B : 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        rows[d] = 0;
+        if ((uint64_t)d < K) load(j0 + (uint64_t)d * B, key[d], rows[d]);
+    }
+    __syncthreads();
+    for (uint64_t i = 0; i < K; i++) {
+        const uint64_t ch = j0 + i * B;
+        if (i + D < K) load(j0 + (i + D) * B, key[D], rows[D]);
+#pragma unroll
+        for (uint32_t i = 0; i < ITEMS; i++) {
+            const uint32_t part = bloom_part(key[0][i], part_bits, shift);
+            const bool in = l + i * PD_WAVE < rows[0] && (part / BG_FILTERS) == g;
+            const uint64_t m = ballot(in);
+            if (in) {
+                const uint32_t e = (qh + qn + popc64(m & lanemask_lt())) & (BG_Q - 1);
+                s_qk[w][e] = key[0][i];
+                s_qr[w][e] = (w * (PD_WAVE * ITEMS) + i * PD_WAVE + l) | ((part % BG_FILTERS) << RB);
+            }
+            qn += popc64(m);
+            if (qn >= PD_WAVE) drain(PD_WAVE);
+        }
+        if (qn) drain(qn);
+        __syncthreads();
+        for (uint32_t i = t; i < CW; i += BG_BLOCK) {
+            out[ch * CW + i] = s_bm[i];
+            s_bm[i] = 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+#pragma unroll
+            for (uint32_t i = 0; i < ITEMS; i++) key[d][i] = key[d + 1][i];
+            rows[d] = rows[d + 1];
+        }
+    }
+}
+
+// Keep flags from the groups' bitmaps: flag[row] = 1 when row's bit is set
+// in any group's bitmap (each row's bit is set in its own group's only).
+// A wave takes 64 bitmap words (2048 rows): lane l ORs word l over the
+// groups, then the wave writes the 2048 flags as 8 rounds of 64 contiguous
+// 16-byte stores, each lane taking its four rows' bits from the lane that
+// holds their word.
+__global__ __launch_bounds__(256) void k_bloom_or(const uint32_t* __restrict__ bm, uint32_t G, uint64_t gstride,
+                                                  uint64_t n, uint32_t* __restrict__ flag) {
+    const uint32_t l = lane();
+    const uint64_t nw = (n + 31) / 32;
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / PD_WAVE;
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / PD_WAVE;
+    const bool vec = ((uintptr_t)flag % 16) == 0;
+    for (uint64_t w0 = wave * PD_WAVE; w0 < nw; w0 += nwaves * PD_WAVE) {
+        uint32_t x = 0;
+        if (w0 + l < nw)
+            for (uint32_t gg = 0; gg < G; gg++) x |= bm[gg * gstride + w0 + l];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) {
+            const uint32_t j = q * PD_WAVE + l;               // 4-row group of this wave
+            const uint32_t xw = (uint32_t)__shfl((int)x, (int)(j / 8), PD_WAVE);
+            const uint32_t sh = (j % 8) * 4;
+            const uint64_t r0 = w0 * 32 + (uint64_t)j * 4;
+            if (vec && r0 + 4 <= n) {
+                reinterpret_cast<uint4*>(flag + r0)[0] =
+                    make_uint4((xw >> sh) & 1u, (xw >> (sh + 1)) & 1u, (xw >> (sh + 2)) & 1u, (xw >> (sh + 3)) & 1u);
+            } else {
+                for (uint32_t i = 0; i < 4 && r0 + i < n; i++) flag[r0 + i] = (xw >> (sh + i)) & 1u;
+            }
+        }
+    }
+}
+
 static void check_bloom_args(uint32_t part_bits, uint32_t shift) {
     PD_REQUIRE(part_bits <= 16, PIMDAL_E_INVALID, "part_bits above 16 (filters of 2^part_bits x 2 KB)");
     PD_REQUIRE(shift <= 32 && shift >= part_bits, PIMDAL_E_INVALID,
@@ -151,6 +311,35 @@ static void bloom_probe(pimdal_ctx* c, const uint32_t* keys, const uint32_t* pay
     }
     Scratch flags(c, n * 4, s);
     const bool lds = part_bits <= BLOOM_LDS_MAX_BITS;
+    // filter groups in LDS (knob bloom_groups, default on): 64 to 1024
+    // filters, probe sides of at least bloom_group_min_rows rows
+    if (!lds && part_bits <= 10 && c->tune_get("bloom_groups", 1) != 0 &&
+        n >= (uint64_t)c->tune_get("bloom_group_min_rows", 1 << 16)) {
+        const uint32_t G = 1u << (part_bits - 6);
+        // chunk shape (knob bloom_group_shape): 1 = 32768-row chunks, the
+        // next one loading (default; 1.29 ms for 512 filters x 2^27 keys);
+        // 0 = 16384-row chunks, next loading (1.39 ms); 2 = 16384-row
+        // chunks, the next two loading (1.31 ms)
+        const int64_t shape = c->tune_get("bloom_group_shape", 1);
+        PD_REQUIRE(shape >= 0 && shape <= 2, PIMDAL_E_INVALID, "bloom_group_shape must be 0, 1 or 2");
+        const uint64_t chunk = shape == 1 ? 32768 : 16384;
+        const uint64_t nchunks = ceil_div(n, chunk);
+        const uint64_t gstride = nchunks * (chunk / 32);
+        Scratch bm(c, (uint64_t)G * gstride * 4, s);
+        const uint32_t grid = std::max<uint32_t>(G, ((uint32_t)c->cu_count / G) * G);
+#define PD_BG(IT, D)                                                                                             \
+    launch(c, "bloom_group", k_bloom_group<IT, D>, dim3(grid), dim3(BG_BLOCK), 0, s, keys, n, part_bits, shift, G, \
+           filt, bm.as<uint32_t>())
+        if (shape == 1) PD_BG(32, 1);
+        else if (shape == 2) PD_BG(16, 2);
+        else PD_BG(16, 1);
+#undef PD_BG
+        const uint64_t nw = ceil_div(n, 32);   // a wave per 64 bitmap words, four waves per block
+        launch(c, "bloom_or", k_bloom_or, dim3((uint32_t)std::min<uint64_t>(ceil_div(nw, 256), 16384)), dim3(256), 0, s,
+               (const uint32_t*)bm.as<uint32_t>(), G, gstride, n, flags.as<uint32_t>());
+        select_range(c, PIMDAL_U32, flags.get(), pay ? pay : keys, n, 1, 2, out_pay, out_row, d_count, s);
+        return;
+    }
     const bool vec = (uintptr_t)keys % 16 == 0;   // the flags are pool memory, 256-byte aligned
     const uint64_t work = vec ? ceil_div(n, 4) : n;
     // LDS: two 1024-thread blocks per CU, each copying the filters once;

@@ -48,3 +48,30 @@ def test_bloom_rejects_bad_partitions(ctx):
         with pytest.raises(pimdal_amd.PimdalError) as e:
             ops.bloom_build(ctx, k, bits, shift) if bits <= 16 else ops.bloom_semi_join(ctx, k, k, None, bits, shift)
         assert e.value.code == -1
+
+
+@pytest.mark.parametrize("part_bits,shift", [(6, 32), (8, 27), (9, 32), (10, 24)])
+@pytest.mark.parametrize("np_", [1 << 16, (1 << 16) + 1, (1 << 22) + 12345])
+def test_bloom_filter_groups_match_oracle(ctx, orc, part_bits, shift, np_):
+    """The probe through filter groups held in LDS (64 filters a group, a
+    block per group streaming every chunk; 16384- and 32768-row chunks, one
+    or two chunks loading ahead) and the probe reading the filters through
+    L2 give the oracle's kept rows, in row order; sizes cover a ragged last
+    chunk and one row past the group path's minimum."""
+    build = _keys(17 + part_bits, 2000 << part_bits)
+    probe = _keys(18, np_)
+    probe[::4] = build[np.arange(probe[::4].size) % build.size]
+    pay = _keys(19, np_)
+    f = orc.bloom_build(build, part_bits, shift)
+    ep, er = orc.bloom_probe(probe, pay, part_bits, shift, f)
+    df, dk, dp = ctx.to_device(f.view(np.uint32)), ctx.to_device(probe), ctx.to_device(pay)
+    try:
+        for groups, shape in ((1, 0), (1, 1), (1, 2), (0, 0)):
+            ctx.set_tune("bloom_groups", groups)
+            ctx.set_tune("bloom_group_shape", shape)
+            gp, gr, cnt = ops.bloom_probe(ctx, dk, dp, part_bits, shift, df)
+            assert cnt == ep.size, (groups, shape)
+            assert np.array_equal(gp.numpy(cnt), ep) and np.array_equal(gr.numpy(cnt), er), (groups, shape)
+    finally:
+        ctx.clear_tune("bloom_groups")
+        ctx.clear_tune("bloom_group_shape")
