@@ -212,8 +212,9 @@ int pimdal_date32_pack_dev(pimdal_ctx* ctx, const int32_t* days, uint64_t n, int
  * predicate form of select.cpp:33-36, bounds as arguments), order
  * preserving.  Outputs the payload and the row index of every kept row;
  * pass the key column as payload to get the reference's compacted keys.
- * key type: PIMDAL_U32, PIMDAL_I32 or PIMDAL_I64 (compared as signed 64-bit
- * against lo/hi); the payload is 4 bytes wide.
+ * key type: PIMDAL_U8, PIMDAL_U32, PIMDAL_I32 or PIMDAL_I64 (compared as
+ * signed 64-bit against lo/hi); the payload is 4 bytes wide.  Without a
+ * payload column only row ids are output.
  * out_payload / out_rowid may be NULL to skip them. */
 int pimdal_select(pimdal_ctx* ctx, const pimdal_col* key, const pimdal_col* payload, int64_t lo, int64_t hi,
                   pimdal_col** out_payload, pimdal_col** out_rowid);

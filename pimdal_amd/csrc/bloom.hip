@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_bloom_build(const uint32_t* __restrict_
 template <bool LDS, bool VEC, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_bloom_flags(const uint32_t* __restrict__ keys, uint64_t n,
                                                        uint32_t part_bits, uint32_t shift,
-                                                       const uint32_t* __restrict__ filt, uint32_t* __restrict__ flag) {
+                                                       const uint32_t* __restrict__ filt, uint8_t* __restrict__ flag) {
     __shared__ uint32_t s_f[LDS ? (BLOOM_WORDS << BLOOM_LDS_MAX_BITS) : 1];
     const uint32_t* f = filt;
     if constexpr (LDS) {
@@ -120,14 +120,10 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_flags(const uint32_t* __restric
     const uint64_t n4 = VEC ? n / 4 : 0;
     for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n4; v += stride) {
         const uint4 k = reinterpret_cast<const uint4*>(keys)[v];
-        uint4 o;
-        o.x = keep(k.x);
-        o.y = keep(k.y);
-        o.z = keep(k.z);
-        o.w = keep(k.w);
-        reinterpret_cast<uint4*>(flag)[v] = o;
+        reinterpret_cast<uint32_t*>(flag)[v] = keep(k.x) | (keep(k.y) << 8) | (keep(k.z) << 16) | (keep(k.w) << 24);
     }
-    for (uint64_t i = n4 * 4 + blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += stride) flag[i] = keep(keys[i]);
+    for (uint64_t i = n4 * 4 + blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += stride)
+        flag[i] = (uint8_t)keep(keys[i]);
 }
 
 // ---- probe with the filters split into groups held in LDS ----
@@ -252,11 +248,15 @@ __global__ __launch_bounds__(BG_BLOCK) void k_bloom_group(const uint32_t* __rest
 // Keep flags from the groups' bitmaps: flag[row] = 1 when row's bit is set
 // in any group's bitmap (each row's bit is set in its own group's only).
 // A wave takes 64 bitmap words (2048 rows): lane l ORs word l over the
-// groups, then the wave writes the 2048 flags as 8 rounds of 64 contiguous
-// 16-byte stores, each lane taking its four rows' bits from the lane that
-// holds their word.
+// groups, then the wave writes the 2048 one-byte flags as 2 rounds of 64
+// contiguous 16-byte stores, each lane taking its 16 rows' bits from the
+// lane that holds their word.
+__device__ __forceinline__ uint32_t bloom_spread4(uint32_t nib) {   // 4 bits -> 4 bytes of 0 / 1
+    return (nib * 0x204081u) & 0x01010101u;
+}
+
 __global__ __launch_bounds__(256) void k_bloom_or(const uint32_t* __restrict__ bm, uint32_t G, uint64_t gstride,
-                                                  uint64_t n, uint32_t* __restrict__ flag) {
+                                                  uint64_t n, uint8_t* __restrict__ flag) {
     const uint32_t l = lane();
     const uint64_t nw = (n + 31) / 32;
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / PD_WAVE;
@@ -267,16 +267,16 @@ __global__ __launch_bounds__(256) void k_bloom_or(const uint32_t* __restrict__ b
         if (w0 + l < nw)
             for (uint32_t gg = 0; gg < G; gg++) x |= bm[gg * gstride + w0 + l];
 #pragma unroll
-        for (uint32_t q = 0; q < 8; q++) {
-            const uint32_t j = q * PD_WAVE + l;               // 4-row group of this wave
-            const uint32_t xw = (uint32_t)__shfl((int)x, (int)(j / 8), PD_WAVE);
-            const uint32_t sh = (j % 8) * 4;
-            const uint64_t r0 = w0 * 32 + (uint64_t)j * 4;
-            if (vec && r0 + 4 <= n) {
-                reinterpret_cast<uint4*>(flag + r0)[0] =
-                    make_uint4((xw >> sh) & 1u, (xw >> (sh + 1)) & 1u, (xw >> (sh + 2)) & 1u, (xw >> (sh + 3)) & 1u);
+        for (uint32_t q = 0; q < 2; q++) {
+            const uint32_t j = q * PD_WAVE + l;               // 16-row group of this wave
+            const uint32_t xw = (uint32_t)__shfl((int)x, (int)(j / 2), PD_WAVE);
+            const uint32_t h = (xw >> ((j % 2) * 16)) & 0xFFFFu;
+            const uint64_t r0 = w0 * 32 + (uint64_t)j * 16;
+            if (vec && r0 + 16 <= n) {
+                reinterpret_cast<uint4*>(flag + r0)[0] = make_uint4(bloom_spread4(h & 15u), bloom_spread4((h >> 4) & 15u),
+                                                                    bloom_spread4((h >> 8) & 15u), bloom_spread4(h >> 12));
             } else {
-                for (uint32_t i = 0; i < 4 && r0 + i < n; i++) flag[r0 + i] = (xw >> (sh + i)) & 1u;
+                for (uint32_t i = 0; i < 16 && r0 + i < n; i++) flag[r0 + i] = (uint8_t)((h >> i) & 1u);
             }
         }
     }
@@ -309,7 +309,7 @@ static void bloom_probe(pimdal_ctx* c, const uint32_t* keys, const uint32_t* pay
         fill_u64(c, d_count, 0, 1, s);
         return;
     }
-    Scratch flags(c, n * 4, s);
+    Scratch flags(c, n, s);   // one byte per row
     const bool lds = part_bits <= BLOOM_LDS_MAX_BITS;
     // filter groups in LDS (knob bloom_groups, default on): 64 to 1024
     // filters, probe sides of at least bloom_group_min_rows rows
@@ -336,8 +336,8 @@ static void bloom_probe(pimdal_ctx* c, const uint32_t* keys, const uint32_t* pay
 #undef PD_BG
         const uint64_t nw = ceil_div(n, 32);   // a wave per 64 bitmap words, four waves per block
         launch(c, "bloom_or", k_bloom_or, dim3((uint32_t)std::min<uint64_t>(ceil_div(nw, 256), 16384)), dim3(256), 0, s,
-               (const uint32_t*)bm.as<uint32_t>(), G, gstride, n, flags.as<uint32_t>());
-        select_range(c, PIMDAL_U32, flags.get(), pay ? pay : keys, n, 1, 2, out_pay, out_row, d_count, s);
+               (const uint32_t*)bm.as<uint32_t>(), G, gstride, n, flags.as<uint8_t>());
+        select_range(c, PIMDAL_U8, flags.get(), pay ? pay : keys, n, 1, 2, out_pay, out_row, d_count, s);
         return;
     }
     const bool vec = (uintptr_t)keys % 16 == 0;   // the flags are pool memory, 256-byte aligned
@@ -349,14 +349,14 @@ static void bloom_probe(pimdal_ctx* c, const uint32_t* keys, const uint32_t* pay
     const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(work, block), cap)));
 #define PD_BLOOM_FLAGS(L, V, B)                                                                                \
     launch(c, "bloom_flags", k_bloom_flags<L, V, B>, grid, dim3(B), 0, s, keys, n, part_bits, shift, filt,    \
-           flags.as<uint32_t>())
+           flags.as<uint8_t>())
     if (lds && vec) PD_BLOOM_FLAGS(true, true, 1024);
     else if (lds) PD_BLOOM_FLAGS(true, false, 1024);
     else if (vec) PD_BLOOM_FLAGS(false, true, 256);
     else PD_BLOOM_FLAGS(false, false, 256);
 #undef PD_BLOOM_FLAGS
     // kept rows (flag in [1, 2)) in row order: payload (the key when there is none) and row index
-    select_range(c, PIMDAL_U32, flags.get(), pay ? pay : keys, n, 1, 2, out_pay, out_row, d_count, s);
+    select_range(c, PIMDAL_U8, flags.get(), pay ? pay : keys, n, 1, 2, out_pay, out_row, d_count, s);
 }
 
 }  // namespace pd

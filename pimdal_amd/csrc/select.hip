@@ -37,12 +37,33 @@ template <typename K, bool VEC>
 __device__ __forceinline__ void sel_load(const K* kb, const uint32_t* pb, uint32_t valid,
                                          K (&k)[SEL_GROUPS][4], uint32_t (&p)[SEL_GROUPS][4]) {
     const __amdgpu_buffer_rsrc_t rk = sel_rsrc(kb, valid * (uint32_t)sizeof(K));
-    const __amdgpu_buffer_rsrc_t rp = sel_rsrc(pb, valid * 4u);
+    // no payload column: a descriptor of no bytes (every load returns 0 and
+    // touches no memory)
+    const __amdgpu_buffer_rsrc_t rp = sel_rsrc(pb, pb ? valid * 4u : 0u);
     const uint32_t l4 = pd::lane() * 4;
 #pragma unroll
     for (int g = 0; g < SEL_GROUPS; g++) {
         const uint32_t r0 = g * 256 + l4;
-        if constexpr (VEC) {
+        if constexpr (sizeof(K) == 1) {
+            // one dword per lane holds its four keys; a wave whose rows end
+            // inside a dword loads bytes (the range check would return 0 for
+            // the whole dword)
+            if (VEC && valid == (uint32_t)SEL_WAVE_ROWS) {
+                const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rk, r0, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; e++) k[g][e] = (K)(v >> (8 * e));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) k[g][e] = (K)__builtin_amdgcn_raw_buffer_load_b8(rk, r0 + e, 0, 0);
+            }
+            if constexpr (VEC) {
+                const uint4 q = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rp, r0 * 4, 0, 0));
+                p[g][0] = q.x; p[g][1] = q.y; p[g][2] = q.z; p[g][3] = q.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) p[g][e] = __builtin_amdgcn_raw_buffer_load_b32(rp, (r0 + e) * 4, 0, 0);
+            }
+        } else if constexpr (VEC) {
             if constexpr (sizeof(K) == 8) {
                 const uint4 a = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rk, r0 * 8, 0, 0));
                 const uint4 b = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rk, r0 * 8 + 16, 0, 0));
@@ -174,7 +195,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(K) == 8 || BLOCK == 1024) ? 4 : 5) v
     };
     auto load = [&](uint32_t tl, K (&k)[SEL_GROUPS][4], uint32_t (&p)[SEL_GROUPS][4]) {
         const uint64_t wb = (uint64_t)tl * TILE + (uint64_t)w * SEL_WAVE_ROWS;
-        sel_load<K, VEC>(key + wb, pay + wb, wave_rows(tl), k, p);
+        sel_load<K, VEC>(key + wb, pay ? pay + wb : nullptr, wave_rows(tl), k, p);
     };
     K k[SEL_GROUPS][4];
     uint32_t p[SEL_GROUPS][4];
@@ -254,7 +275,16 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select_count(const K* __restrict_
         for (int g = 0; g < SEL_GROUPS; g++) {
             const uint32_t r0 = g * 256 + l4;
             K kk[4];
-            if constexpr (VEC && sizeof(K) == 4) {
+            if constexpr (sizeof(K) == 1) {
+                if (VEC && valid == (uint32_t)SEL_WAVE_ROWS) {
+                    const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rk, r0, 0, 0);
+#pragma unroll
+                    for (int e = 0; e < 4; e++) kk[e] = (K)(v >> (8 * e));
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) kk[e] = (K)__builtin_amdgcn_raw_buffer_load_b8(rk, r0 + e, 0, 0);
+                }
+            } else if constexpr (VEC && sizeof(K) == 4) {
                 const uint4 v = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rk, r0 * 4, 0, 0));
                 kk[0] = (K)v.x; kk[1] = (K)v.y; kk[2] = (K)v.z; kk[3] = (K)v.w;
             } else if constexpr (VEC) {
@@ -309,7 +339,7 @@ __global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select_wr
     if (tile0 * SEL_TILE >= n) return;
     {
         const uint64_t wb = tile0 * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
-        sel_load<K, VEC>(key + wb, pay + wb, wave_rows(tile0), k, p);
+        sel_load<K, VEC>(key + wb, pay ? pay + wb : nullptr, wave_rows(tile0), k, p);
     }
     for (uint32_t j = 0; j < SEL2_CHUNK_TILES; j++) {
         const uint64_t tl = tile0 + j;
@@ -320,7 +350,7 @@ __global__ __launch_bounds__(SEL_BLOCK, sizeof(K) == 8 ? 4 : 5) void k_select_wr
         const uint64_t nt = tl + 1;
         if (j + 1 < SEL2_CHUNK_TILES && nt * SEL_TILE < n) {
             const uint64_t wb = nt * SEL_TILE + (uint64_t)w * SEL_WAVE_ROWS;
-            sel_load<K, VEC>(key + wb, pay + wb, wave_rows(nt), k, p);
+            sel_load<K, VEC>(key + wb, pay ? pay + wb : nullptr, wave_rows(nt), k, p);
         }
         uint32_t woff = 0, ttot = 0;
 #pragma unroll
@@ -343,8 +373,8 @@ static bool sel_bounds(pimdal_type kt, int64_t lo, int64_t hi, uint64_t& ulo, ui
     ulo = wm1 = 0;
     if (hi <= lo) return false;
     if (kt != PIMDAL_I64) {
-        const int64_t dmin = kt == PIMDAL_U32 ? 0 : -(1ll << 31);
-        const int64_t dmax = dmin + (1ll << 32);   // exclusive
+        const int64_t dmin = (kt == PIMDAL_U32 || kt == PIMDAL_U8) ? 0 : -(1ll << 31);
+        const int64_t dmax = dmin + (kt == PIMDAL_U8 ? 256 : (1ll << 32));   // exclusive
         const int64_t l2 = std::max(lo, dmin), h2 = std::min(hi, dmax);
         if (h2 <= l2) return false;
         ulo = (uint64_t)(l2 - dmin);
@@ -357,9 +387,10 @@ static bool sel_bounds(pimdal_type kt, int64_t lo, int64_t hi, uint64_t& ulo, ui
 }
 
 static void sel_check_args(pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, const uint32_t* out_pay) {
-    PD_REQUIRE(kt == PIMDAL_U32 || kt == PIMDAL_I32 || kt == PIMDAL_I64, PIMDAL_E_UNSUPPORTED,
-               "select key must be uint32, int32 or int64");
+    PD_REQUIRE(kt == PIMDAL_U32 || kt == PIMDAL_I32 || kt == PIMDAL_I64 || kt == PIMDAL_U8, PIMDAL_E_UNSUPPORTED,
+               "select key must be uint8, uint32, int32 or int64");
     PD_REQUIRE(kt != PIMDAL_I64 || pay || !out_pay, PIMDAL_E_INVALID, "int64 keys need a 32-bit payload column");
+
     PD_REQUIRE(n < (1ull << 32), PIMDAL_E_INVALID, "select supports up to 2^32-1 rows (uint32 row ids)");
     PD_REQUIRE(key || n == 0, PIMDAL_E_INVALID, "key is NULL");
     PD_REQUIRE(!out_pay || pay, PIMDAL_E_INVALID, "payload is NULL but out_payload requested");
@@ -415,14 +446,16 @@ static void select_one_pass(pimdal_ctx* c, pimdal_type kt, const void* key, cons
     };
     if (kt == PIMDAL_U32) go(reinterpret_cast<const uint32_t*>(key));
     else if (kt == PIMDAL_I32) go(reinterpret_cast<const int32_t*>(key));
+    else if (kt == PIMDAL_U8) go(reinterpret_cast<const uint8_t*>(key));
     else go(reinterpret_cast<const int64_t*>(key));
 }
 
 void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t* pay, uint64_t n, int64_t lo,
                   int64_t hi, uint32_t* out_pay, uint32_t* out_rid, uint64_t* d_count, hipStream_t s) {
     sel_check_args(kt, key, pay, n, out_pay);
-    // without a payload column the key words stand in (read, never written out)
-    if (!pay) pay = reinterpret_cast<const uint32_t*>(key);
+    // without a payload column the key words stand in (read, never written
+    // out); uint8 keys, a quarter of a payload's width, read none instead
+    if (!pay && kt != PIMDAL_U8) pay = reinterpret_cast<const uint32_t*>(key);
     uint64_t ulo = 0, wm1 = 0;
     const bool empty = !sel_bounds(kt, lo, hi, ulo, wm1);
     if (n == 0 || empty) {
@@ -445,6 +478,7 @@ void select_range(pimdal_ctx* c, pimdal_type kt, const void* key, const uint32_t
         };
         if (kt == PIMDAL_U32) go2(reinterpret_cast<const uint32_t*>(key));
         else if (kt == PIMDAL_I32) go2(reinterpret_cast<const int32_t*>(key));
+        else if (kt == PIMDAL_U8) go2(reinterpret_cast<const uint8_t*>(key));
         else go2(reinterpret_cast<const int64_t*>(key));
         return;
     }
@@ -494,7 +528,7 @@ void select_range_noalloc(pimdal_ctx* c, pimdal_type kt, const void* key, const 
                "pimdal_select_dev_noalloc takes at most 2^29 rows (its static scratch holds 2^17 tile statuses)");
     uint64_t* status = select_noalloc_scratch(c->device);
     PD_REQUIRE(status != nullptr, PIMDAL_E_INTERNAL, "select scratch address not set up for this device");
-    if (!pay) pay = reinterpret_cast<const uint32_t*>(key);
+    if (!pay && kt != PIMDAL_U8) pay = reinterpret_cast<const uint32_t*>(key);
     uint64_t ulo = 0, wm1 = 0;
     const bool empty = !sel_bounds(kt, lo, hi, ulo, wm1);
     if (n == 0 || empty) {

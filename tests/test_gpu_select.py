@@ -118,3 +118,35 @@ def test_select_int64_full_domain_bounds(ctx, orc, lo, hi):
     if -(1 << 63) <= lo and hi <= (1 << 63) - 1:   # inside int64: the oracle agrees too
         op2, or2 = orc.select(k, p, lo, hi)
         assert np.array_equal(or2, er)
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 1021, 1024, 4097, 100_003])
+@pytest.mark.parametrize("lo,hi", [(1, 2), (0, 256), (-5, 7), (200, 1000), (300, 400)])
+@pytest.mark.parametrize("with_payload", [True, False])
+def test_select_uint8_keys(ctx, orc, n, lo, hi, with_payload):
+    """uint8 predicate columns (the flag columns of the fact table, the Bloom
+    probe's keep flags): the same rows as the oracle's uint32 selection over
+    the same values with the range clamped to [0, 256); ragged row counts
+    (within a dword, within a wave) and no payload column (row ids only)."""
+    rng = np.random.default_rng(700 + n)
+    k = rng.integers(0, 256, n, dtype=np.uint16).astype(np.uint8)
+    p = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    ep, er = orc.select(k.astype(np.uint32), p, max(lo, 0), min(hi, 256))
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    op, orow, cnt = ops.select(ctx, dk, dp if with_payload else None, lo, hi, want_payload=with_payload)
+    assert cnt == er.size
+    assert np.array_equal(orow.numpy(cnt), er)
+    if with_payload:
+        assert np.array_equal(op.numpy(cnt), ep)
+
+
+def test_select_uint8_unaligned(ctx, orc):
+    """A uint8 column starting one byte into its allocation (dword loads off)."""
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 256, 70_001, dtype=np.uint16).astype(np.uint8)
+    p = rng.integers(0, 1 << 32, 70_001, dtype=np.uint64).astype(np.uint32)
+    dk, dp = ctx.to_device(k), ctx.to_device(p)
+    vk, vp = dk.view(1, 70_000), dp.view(1, 70_000)
+    op, orow, cnt = ops.select(ctx, vk, vp, 10, 90)
+    ep, er = orc.select(k[1:].astype(np.uint32), p[1:], 10, 90)
+    assert np.array_equal(op.numpy(cnt), ep) and np.array_equal(orow.numpy(cnt), er)
