@@ -119,6 +119,12 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   aggr_dense_max (keys spanning at most this many values, default and
  *   largest 2^24: one range-partition pass into 8192-key partitions summed
  *   in key-indexed LDS arrays, groups in key order; 0: skip that path);
+ *   bloom_groups (1: probes with 64 to 1024 filters hold them in LDS in
+ *   groups of 64, each group's blocks reading every probe key; 0: every
+ *   filter bit read through L2), bloom_group_min_rows (smallest probe side
+ *   that takes the groups, default 2^16), bloom_group_shape (1: 32768-row
+ *   chunks, the next one loading; 0: 16384-row chunks; 2: 16384-row
+ *   chunks, two loading ahead);
  *   serve_index_load_pct. */
 int pimdal_ctx_set_tune(pimdal_ctx* ctx, const char* key, int64_t value);
 int pimdal_ctx_get_tune(pimdal_ctx* ctx, const char* key, int64_t* value);
