@@ -129,3 +129,19 @@ def test_aggregation_with_nulls_tiny_brute_force(ctx):
         groups[g] = (s + (int(v[i]) if vv[i] else 0), c + (1 if vv[i] else 0))
     exp = [(g, s if c else None, c) for g, (s, c) in sorted(groups.items(), key=lambda t: (t[0] is None, t[0] or 0))]
     assert list(zip(_list(ctx, ok), _list(ctx, os_), _list(ctx, oc))) == exp
+
+
+@pytest.mark.parametrize("n,p_null", [(1, 0.5), (4097, 0.3), (300_001, 0.1)])
+def test_select_uint8_key_column_with_nulls(ctx, orc, n, p_null):
+    """A nullable uint8 key column (an Arrow uint8 array, such as a flag
+    column) through the column select: the oracle's uint32 selection over
+    the same values, NULL keys kept out, payload validity following the
+    rows."""
+    rng = np.random.default_rng(90 + n)
+    k = rng.integers(0, 256, n, dtype=np.uint16).astype(np.uint8)
+    p = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    kv, pv = _valid(n, 7, p_null), _valid(n, 8, 0.2)
+    kc, pc = _imp(ctx, _nullable(k, kv)), _imp(ctx, _nullable(p, pv))
+    op, orow = ingest.col_select(ctx, kc, pc, 3, 200)
+    exp_p, exp_r = _select_expected(orc, k.astype(np.uint32), kv, p, pv, 3, 200)
+    assert _list(ctx, orow) == exp_r and _list(ctx, op) == exp_p
