@@ -128,7 +128,7 @@ def bench_select(ctx, steps, warmup, n=1 << 27):
 KERNELS = ("bloom_build", "bloom_flags", "bloom_group", "bloom_or", "date32_pack", "part_hist", "part_count", "part_tscan", "radix_count", "radix_tscan", "part_onesweep", "part_bounds", "scan", "hj_chunks", "hj_join", "hj_join_count",
            "hj_join_write", "hj_compact", "hg_build", "hg_probe", "hg_probe_count", "hg_probe_write", "radix_hist",
            "radix_scan", "radix_onesweep", "smj_dupcheck", "smj_ranges", "smj_merge", "smj_expand", "q1_domain",
-           "q1_lut", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
+           "q1_lut", "q1_count", "q1_dense", "q1_reduce", "q1_final", "gb_direct", "gb_direct_emit", "gb_chunks", "gb_hist",
            "gb_scatter", "gb_lds", "gb_global", "gb_emit_global", "q1_unpack", "agg_heads", "agg_run_reduce",
            "select", "select_count", "select_write", "q1_pack", "q6", "q6_final", "smj_pack", "fill_u32", "fill_u64",
            "q3_cust", "q3_osel", "q3_orders", "q3_lineitem", "q3_emit", "q3_order", "q3_topk", "q4_osel", "q4_orders",

@@ -527,6 +527,192 @@ __global__ __launch_bounds__(256) void k_q1_pack(const int32_t* __restrict__ dat
     if (pd::ballot(bad) && pd::lane() == 0) atomicOr(overflow, 1u);
 }
 
+// Filter + pack fused with the first partition pass of the Q1 group-by on
+// a key column (knob gb_q1_fused, default 1): the rows move to partitions
+// of the low lo_bits hash bits as they are packed, so the packed rows are
+// written once instead of written by k_q1_pack and read back by the first
+// onesweep pass (whose key histogram pass goes too).  8192-row tiles:
+//  k_q1p_count    rows per low-bit partition of every tile (the keys
+//                 alone: filtered rows are kept, marked Q1_SKIP), into
+//                 cnt[tile][1024], and the histogram of the next hash digit
+//                 (bits [lo_bits, lo_bits + 8)) for the pass that follows;
+//  (partition_tile_scan: per-tile offsets and partition starts)
+//  k_q1p_scatter  per tile: the six columns loaded, filtered and packed,
+//                 rows ranked in their partition by LDS counters (order
+//                 inside a partition does not matter: the next pass is the
+//                 stable one), staged in LDS by partition and written in
+//                 partition runs; tiles dealt to the XCDs in contiguous runs.
+constexpr uint32_t Q1P_BLOCK = 1024;
+constexpr uint32_t Q1P_ITEMS = 8;
+constexpr uint32_t Q1P_TILE = Q1P_BLOCK * Q1P_ITEMS;   // 8192 rows
+constexpr uint32_t Q1P_CSTRIDE = 1024;                 // count row width (partition_tile_scan's)
+
+__global__ __launch_bounds__(512) void k_q1p_count(const uint32_t* __restrict__ key, uint64_t n, uint32_t lo_bits,
+                                                   uint32_t* __restrict__ cnt, uint32_t* __restrict__ hist_hi) {
+    constexpr uint32_t Q = Q1P_TILE / 512;   // rows per thread
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t hh[256];
+    const uint32_t t = threadIdx.x, l = pd::lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint32_t lo_mask = (1u << lo_bits) - 1;
+    const uint64_t ntiles = (n + Q1P_TILE - 1) / Q1P_TILE;
+    if (t < 256) hh[t] = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        if (t < 256) h[t] = 0;
+        const uint64_t wave_base = tile * Q1P_TILE + (uint64_t)w * (PD_WAVE * Q);
+        const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * Q), n - wave_base) : 0));
+        const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(key + wave_base, wrows * 4u);
+        uint32_t q[Q];
+#pragma unroll
+        for (uint32_t i = 0; i < Q; i++) q[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < Q; i++) {
+            if (l + i * PD_WAVE < wrows) {
+                const uint32_t hv = pd::ref_hash(q[i]);
+                atomicAdd(&h[hv & lo_mask], 1u);
+                atomicAdd(&hh[(hv >> lo_bits) & 0xFFu], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < Q1P_CSTRIDE; i += 512) cnt[tile * Q1P_CSTRIDE + i] = i <= lo_mask ? h[i] : 0u;
+        __syncthreads();
+    }
+    if (t < 256 && hh[t]) atomicAdd(&hist_hi[t], hh[t]);
+}
+
+__global__ __launch_bounds__(Q1P_BLOCK) void k_q1p_scatter(
+    const int32_t* __restrict__ date, const uint32_t* __restrict__ key, const int64_t* __restrict__ qty,
+    const int64_t* __restrict__ price, const int64_t* __restrict__ disc, const int64_t* __restrict__ tax, uint64_t n,
+    int32_t cutoff, uint32_t lo_bits, const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ start, uint32_t xt,
+    uint32_t* __restrict__ okey, uint64_t* __restrict__ ow, uint32_t* __restrict__ overflow) {
+    __shared__ __align__(16) uint32_t s_k[Q1P_TILE];
+    __shared__ __align__(16) uint64_t s_v[Q1P_TILE];
+    __shared__ uint32_t s_c[256];   // tile rows per partition, then their start in the tile
+    __shared__ uint32_t s_g[256];   // output position of the partition's run minus its tile start
+    __shared__ uint32_t s_ws[Q1P_BLOCK / PD_WAVE];
+    const uint32_t t = threadIdx.x, l = pd::lane();
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane(t / PD_WAVE);
+    const uint32_t lo_mask = (1u << lo_bits) - 1;
+    const uint64_t tile = (uint64_t)(blockIdx.x % 8) * xt + blockIdx.x / 8;
+    const uint64_t tile_base = tile * Q1P_TILE;
+    if (tile_base >= n) return;   // grid rounded up to a multiple of 8
+    const uint32_t tile_rows = (uint32_t)min((uint64_t)Q1P_TILE, n - tile_base);
+    uint32_t gpos = 0;
+    if (t < 256) {
+        s_c[t] = 0;
+        gpos = t <= lo_mask ? start[t] + cnt[tile * Q1P_CSTRIDE + t] : 0u;
+    }
+    // wave w: rows [tile_base + w * 512, + 512), item i = row w * 512 + i * 64 + lane
+    const uint64_t wave_base = tile_base + (uint64_t)w * (PD_WAVE * Q1P_ITEMS);
+    const uint32_t wrows = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (uint32_t)(n > wave_base ? min((uint64_t)(PD_WAVE * Q1P_ITEMS), n - wave_base) : 0));
+    const __amdgpu_buffer_rsrc_t rd = pd::buf_rsrc(date + wave_base, wrows * 4u);
+    const __amdgpu_buffer_rsrc_t rk = pd::buf_rsrc(key + wave_base, wrows * 4u);
+    const __amdgpu_buffer_rsrc_t rq = pd::buf_rsrc(qty + wave_base, wrows * 8u);
+    const __amdgpu_buffer_rsrc_t rp = pd::buf_rsrc(price + wave_base, wrows * 8u);
+    const __amdgpu_buffer_rsrc_t rs = pd::buf_rsrc(disc + wave_base, wrows * 8u);
+    const __amdgpu_buffer_rsrc_t rt = pd::buf_rsrc(tax + wave_base, wrows * 8u);
+    // every column's loads issued before any row is packed
+    uint32_t kk[Q1P_ITEMS];
+    int32_t dd[Q1P_ITEMS];
+    int64_t qq[Q1P_ITEMS], pp[Q1P_ITEMS], ss[Q1P_ITEMS], tt[Q1P_ITEMS];
+#pragma unroll
+    for (uint32_t i = 0; i < Q1P_ITEMS; i++) {
+        const uint32_t o = l + i * PD_WAVE;
+        kk[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, o * 4u, 0, 0);
+        dd[i] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rd, o * 4u, 0, 0);
+        qq[i] = (int64_t)__builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rq, o * 8u, 0, 0));
+        pp[i] = (int64_t)__builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rp, o * 8u, 0, 0));
+        ss[i] = (int64_t)__builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rs, o * 8u, 0, 0));
+        tt[i] = (int64_t)__builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rt, o * 8u, 0, 0));
+    }
+    uint64_t pw[Q1P_ITEMS];
+    bool bad = false;
+#pragma unroll
+    for (uint32_t i = 0; i < Q1P_ITEMS; i++) {
+        if (dd[i] > cutoff) {
+            pw[i] = Q1_SKIP;
+        } else {
+            bad |= l + i * PD_WAVE < wrows && !q1_packable(qq[i], pp[i], ss[i], tt[i]);
+            pw[i] = q1_pack(qq[i], pp[i], ss[i], tt[i]);
+        }
+    }
+    __syncthreads();
+    uint32_t rk_d[Q1P_ITEMS];   // rank in the partition | partition << 16
+#pragma unroll
+    for (uint32_t i = 0; i < Q1P_ITEMS; i++) {
+        rk_d[i] = 0xFFFFFFFFu;
+        if (l + i * PD_WAVE < wrows) {
+            const uint32_t d = pd::ref_hash(kk[i]) & lo_mask;
+            rk_d[i] = atomicAdd(&s_c[d], 1u) | (d << 16);
+        }
+    }
+    __syncthreads();
+    // tile-local partition starts (exclusive scan over the 256 counters, waves 0-3)
+    const uint32_t c = t < 256 ? s_c[t] : 0u;
+    const uint32_t inc = pd::wave_incl_scan(c);
+    if (l == PD_WAVE - 1) s_ws[w] = inc;
+    __syncthreads();
+    if (t < 256) {
+        uint32_t add = 0;
+        for (uint32_t i = 0; i < w; i++) add += s_ws[i];
+        const uint32_t st = add + inc - c;
+        s_c[t] = st;
+        s_g[t] = gpos - st;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < Q1P_ITEMS; i++) {
+        if (rk_d[i] == 0xFFFFFFFFu) continue;
+        const uint32_t j = s_c[rk_d[i] >> 16] + (rk_d[i] & 0xFFFFu);
+        s_k[j] = kk[i];
+        s_v[j] = pw[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < Q1P_ITEMS; i++) {
+        const uint32_t j = i * Q1P_BLOCK + t;
+        if (j >= tile_rows) break;
+        const uint32_t k = s_k[j];
+        const uint64_t dst = (uint64_t)s_g[pd::ref_hash(k) & lo_mask] + j;
+        okey[dst] = k;
+        ow[dst] = s_v[j];
+    }
+    if (pd::ballot(bad) && l == 0) atomicOr(overflow, 1u);
+}
+
+// Filter + pack + the Q1 group-by's hash partition of n rows into 2^bits
+// partitions (9 <= bits <= 16): the fused first pass over the low bits - 8
+// bits, then hash_partition_finish over the top 8.  (qk, qv): the
+// partitioned rows; bounds as hash_partition's; *ovf set when a kept row's
+// measures do not fit the packed widths.
+static void q1_pack_partition(pimdal_ctx* c, const int32_t* date, const uint32_t* key, const int64_t* qty,
+                              const int64_t* price, const int64_t* disc, const int64_t* tax, uint64_t n,
+                              int32_t cutoff, uint32_t bits, uint32_t* qk, uint64_t* qv, uint32_t* bounds,
+                              uint32_t* ovf, hipStream_t s) {
+    PD_REQUIRE(bits >= 9 && bits <= 16 && n >= 1 && n < (1ull << 30), PIMDAL_E_INTERNAL, "fused Q1 partition shape");
+    const uint32_t lo_bits = bits - 8;
+    const uint32_t ntiles = (uint32_t)ceil_div(n, (uint64_t)Q1P_TILE);
+    const uint32_t xt = (ntiles + 7) / 8;
+    // cnt[ntiles][1024] | start[1025] | hist of the top digit[256]
+    Scratch sc(c, ((uint64_t)ntiles * Q1P_CSTRIDE + 1025 + 256) * 4, s);
+    uint32_t* cnt = sc.as<uint32_t>();
+    uint32_t* start = cnt + (uint64_t)ntiles * Q1P_CSTRIDE;
+    uint32_t* hist_hi = start + 1025;
+    fill_u32(c, hist_hi, 0, 256, s);
+    const uint32_t cgrid = std::min<uint32_t>(ntiles, (uint32_t)c->cu_count * 4);
+    launch(c, "q1_count", k_q1p_count, dim3(cgrid), dim3(512), 0, s, key, n, lo_bits, cnt, hist_hi);
+    partition_tile_scan(c, cnt, ntiles, 1u << lo_bits, start, s);
+    Scratch pk(c, n * 4, s), pv(c, n * 8, s);
+    // (buffer loads: any 4- and 8-byte aligned columns)
+    launch(c, "q1_pack", k_q1p_scatter, dim3(8 * xt), dim3(Q1P_BLOCK), 0, s, date, key, qty, price, disc, tax, n,
+           cutoff, lo_bits, (const uint32_t*)cnt, (const uint32_t*)start, xt, pk.as<uint32_t>(), pv.as<uint64_t>(),
+           ovf);
+    hash_partition_finish(c, pk.as<uint32_t>(), pv.as<uint64_t>(), n, bits, lo_bits, hist_hi, qk, qv, bounds, s);
+}
+
 // Output policies of the per-partition kernels: put(o, key, fields).
 struct OutFields {   // key array + NF u64 field columns of stride `stride`
     uint32_t* key;
@@ -1023,6 +1209,45 @@ uint64_t group_by_onesweep(pimdal_ctx* c, const uint32_t* key, const void* paylo
     return aggregate_parts<Agg>(c, make_src(pk.as<uint32_t>(), pv.get()), bounds.as<uint32_t>(), 1u, F, n, out, cap, s);
 }
 
+// Q1 by key, packed rows: filter + pack, partition by hash digits, one LDS
+// table per partition.  Fused (knob gb_q1_fused, default 1, partitions of
+// 9 to 16 bits): q1_pack_partition; otherwise k_q1_pack into (key, 8-byte
+// measures) and hash_partition.  *ovf set when a kept row's measures do not
+// fit the packed widths (the outputs are then not valid).
+static void q1_pack_rows(pimdal_ctx* c, const int32_t* date, const uint32_t* key, const int64_t* qty,
+                         const int64_t* price, const int64_t* disc, const int64_t* tax, uint64_t n, int32_t cutoff,
+                         uint32_t bits, uint32_t* qk, uint64_t* qv, uint32_t* bounds, uint32_t* ovf, hipStream_t s) {
+    if (bits >= 9 && bits <= 16 && c->tune_get("gb_q1_fused", 1)) {
+        q1_pack_partition(c, date, key, qty, price, disc, tax, n, cutoff, bits, qk, qv, bounds, ovf, s);
+        return;
+    }
+    Scratch pk(c, n * 4, s), pw(c, n * 8, s);
+    const bool vec = ((uintptr_t)date | (uintptr_t)key | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc |
+                      (uintptr_t)tax) % 16 == 0;
+    const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 4), (uint64_t)c->cu_count * 8));
+    if (vec)
+        launch(c, "q1_pack", k_q1_pack<true>, dim3(gp), dim3(256), 0, s, date, key, qty, price, disc, tax, n, cutoff,
+               pk.as<uint32_t>(), pw.as<uint64_t>(), ovf);
+    else
+        launch(c, "q1_pack", k_q1_pack<false>, dim3(gp), dim3(256), 0, s, date, key, qty, price, disc, tax, n, cutoff,
+               pk.as<uint32_t>(), pw.as<uint64_t>(), ovf);
+    hash_partition(c, pk.as<uint32_t>(), pw.get(), 8, n, bits, qk, qv, bounds, s);
+}
+
+template <typename Agg>
+static uint64_t q1_key_packed(pimdal_ctx* c, const int32_t* date, const uint32_t* key, const int64_t* qty,
+                              const int64_t* price, const int64_t* disc, const int64_t* tax, uint64_t n, int32_t cutoff,
+                              const OutQ1& o, uint64_t cap, uint32_t* ovf, hipStream_t s) {
+    if (n == 0) return 0;
+    const uint32_t bits = gb_part_bits(c, n, (int64_t)Agg::TS * 2);
+    const uint32_t F = 1u << bits;
+    Scratch qk(c, n * 4, s), qv(c, n * 8, s), bounds(c, ((uint64_t)F + 1) * 4, s);
+    q1_pack_rows(c, date, key, qty, price, disc, tax, n, cutoff, bits, qk.as<uint32_t>(), qv.as<uint64_t>(),
+                 bounds.as<uint32_t>(), ovf, s);
+    return aggregate_parts<Agg>(c, RowsQ1Packed{qk.as<uint32_t>(), (const uint64_t*)qv.get()}, bounds.as<uint32_t>(),
+                                1u, F, n, o, cap, s);
+}
+
 // Low-cardinality attempt: per-block LDS tables merged into one small HBM
 // table.  Returns false (nothing written) when a table filled up; skipped
 // when the group-count hint says the groups cannot fit.
@@ -1290,17 +1515,8 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         // onesweep partitions and per-partition LDS tables writing the Q1
         // arrays directly; measures beyond the packed widths take the
         // general path below
-        Scratch pk(c, n * 4, s), pw(c, n * 8, s), ovf(c, 8, s);
+        Scratch ovf(c, 8, s);
         fill_u32(c, ovf.get(), 0, 1, s);
-        const bool vec = ((uintptr_t)date | (uintptr_t)key | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc |
-                          (uintptr_t)tax) % 16 == 0;
-        const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 4), (uint64_t)c->cu_count * 8));
-        if (vec)
-            launch(c, "q1_pack", k_q1_pack<true>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
-                   qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
-        else
-            launch(c, "q1_pack", k_q1_pack<false>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
-                   qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
         // The overflow flag is read after the packed path has run (its group
         // count depends on the keys only); when set, the general path below
         // recomputes every output.
@@ -1308,11 +1524,12 @@ uint64_t filter_groupby_q1(pimdal_ctx* c, const int32_t* date, const uint8_t* f1
         // LDS table per partition: 1024 slots (64 KB, two blocks per CU;
         // default) or 2048 slots (128 KB, one block per CU, half the
         // partitions)
-        auto mk = [](uint32_t* k, void* v) { return RowsQ1Packed{k, (const uint64_t*)v}; };
         const uint64_t g =
             c->tune_get("gb_q1_slots", 1024) == 1024
-                ? group_by_onesweep<AggQ1T<1024>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s)
-                : group_by_onesweep<AggQ1T<2048>, RowsQ1Packed>(c, pk.as<uint32_t>(), pw.get(), 8, n, mk, o, cap, s);
+                ? q1_key_packed<AggQ1T<1024>>(c, date, reinterpret_cast<const uint32_t*>(key), qty, price, disc, tax,
+                                              n, cutoff, o, cap, ovf.as<uint32_t>(), s)
+                : q1_key_packed<AggQ1T<2048>>(c, date, reinterpret_cast<const uint32_t*>(key), qty, price, disc, tax,
+                                              n, cutoff, o, cap, ovf.as<uint32_t>(), s);
         uint32_t overflow = 0;
         read_small(c, &overflow, ovf.get(), 4, s);
         if (!overflow) return g;
@@ -1377,21 +1594,13 @@ template <typename Agg>
 static void q1_key_async(pimdal_ctx* c, const int32_t* date, const int32_t* key, const int64_t* qty,
                          const int64_t* price, const int64_t* disc, const int64_t* tax, uint64_t n, int32_t cutoff,
                          const OutQ1& o, uint64_t cap, uint64_t* d_ng, int32_t* d_status, hipStream_t s) {
-    Scratch pk(c, n * 4, s), pw(c, n * 8, s), ovf(c, 8, s);
+    Scratch ovf(c, 8, s);
     fill_u32(c, ovf.get(), 0, 2, s);
-    const bool vec = ((uintptr_t)date | (uintptr_t)key | (uintptr_t)qty | (uintptr_t)price | (uintptr_t)disc |
-                      (uintptr_t)tax) % 16 == 0;
-    const uint32_t gp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 4), (uint64_t)c->cu_count * 8));
-    if (vec)
-        launch(c, "q1_pack", k_q1_pack<true>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
-               qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
-    else
-        launch(c, "q1_pack", k_q1_pack<false>, dim3(gp), dim3(256), 0, s, date, reinterpret_cast<const uint32_t*>(key),
-               qty, price, disc, tax, n, cutoff, pk.as<uint32_t>(), pw.as<uint64_t>(), ovf.as<uint32_t>());
     const uint32_t bits = gb_part_bits(c, n, (int64_t)Agg::TS * 2);
     const uint32_t F = 1u << bits;
     Scratch qk(c, n * 4, s), qv(c, n * 8, s), bounds(c, ((uint64_t)F + 1) * 4, s);
-    hash_partition(c, pk.as<uint32_t>(), pw.get(), 8, n, bits, qk.as<uint32_t>(), qv.get(), bounds.as<uint32_t>(), s);
+    q1_pack_rows(c, date, reinterpret_cast<const uint32_t*>(key), qty, price, disc, tax, n, cutoff, bits,
+                 qk.as<uint32_t>(), qv.as<uint64_t>(), bounds.as<uint32_t>(), ovf.as<uint32_t>(), s);
     Scratch meta(c, 8 * 4, s), tog(c, (uint64_t)F * 4, s);
     fill_u64(c, meta.get(), 0, 4, s);
     fill_u32(c, tog.get(), 0, F, s);

@@ -201,6 +201,15 @@ __global__ __launch_bounds__(PW_BLOCK) void k_pw_scatter(const uint32_t* __restr
     }
 }
 
+void partition_tile_scan(pimdal_ctx* c, uint32_t* cnt, uint32_t ntiles, uint32_t F, uint32_t* start, hipStream_t s) {
+    PD_REQUIRE(F >= 1 && F <= PW_BINS, PIMDAL_E_INVALID, "tile scan: 1 to 1024 partitions");
+    const uint32_t ngroups = (ntiles + PW_GT - 1) / PW_GT;
+    Scratch gst(c, ((uint64_t)ngroups * PW_BINS + 1) * 4, s);
+    fill_u32(c, gst.get(), 0, (uint64_t)ngroups * PW_BINS + 1, s);
+    launch(c, "pw_tscan", k_pw_tscan, dim3(ngroups), dim3(PW_BINS), 0, s, cnt, ntiles, gst.as<uint32_t>(),
+           gst.as<uint32_t>() + (uint64_t)ngroups * PW_BINS, start, F);
+}
+
 void hash_partition_1p(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uint64_t n, uint32_t bits, uint2* out,
                        uint32_t* bounds, hipStream_t s) {
     PD_REQUIRE(bits >= 1 && bits <= 10, PIMDAL_E_INVALID, "one-pass partition: 1 to 10 bits");

@@ -33,6 +33,18 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
 void hash_partition_1p(pimdal_ctx* c, const uint32_t* kin, const uint32_t* vin, uint64_t n, uint32_t bits, uint2* out,
                        uint32_t* bounds, hipStream_t s);
 
+// Per-tile partition counts cnt[tile][1024] (columns below F used) to
+// per-tile offsets in place, and start[p] (p <= F) = rows of partitions
+// below p (partition.hip).
+void partition_tile_scan(pimdal_ctx* c, uint32_t* cnt, uint32_t ntiles, uint32_t F, uint32_t* start, hipStream_t s);
+
+// The last pass of hash_partition for rows already partitioned by the low
+// lo_bits hash bits: one stable pass over bits [lo_bits, bits) from that
+// digit's histogram (8-byte payloads), then the bounds (radix_sort.hip).
+void hash_partition_finish(pimdal_ctx* c, const uint32_t* kin, const uint64_t* vin, uint64_t n, uint32_t bits,
+                           uint32_t lo_bits, const uint32_t* hist_hi, uint32_t* kout, uint64_t* vout, uint32_t* bounds,
+                           hipStream_t s);
+
 uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64_t nb, const uint32_t* pk,
                    const uint32_t* pp, uint64_t np, uint32_t* out_p, uint32_t* out_b, uint64_t cap, hipStream_t s);
 

@@ -1706,6 +1706,30 @@ void hash_partition(pimdal_ctx* c, const uint32_t* kin, const void* vin, uint32_
            s, (const uint32_t*)kout, n, F, bounds);
 }
 
+// The last pass of hash_partition over rows already partitioned by hash
+// bits [0, lo_bits) (in any order inside those partitions): one stable
+// onesweep pass over bits [lo_bits, bits) from that digit's histogram
+// (hist_hi, RS_BINS words), then the partition bounds.
+void hash_partition_finish(pimdal_ctx* c, const uint32_t* kin, const uint64_t* vin, uint64_t n, uint32_t bits,
+                           uint32_t lo_bits, const uint32_t* hist_hi, uint32_t* kout, uint64_t* vout, uint32_t* bounds,
+                           hipStream_t s) {
+    PD_REQUIRE(lo_bits >= 1 && bits > lo_bits && bits - lo_bits <= 8 && bits <= 24, PIMDAL_E_INVALID,
+               "partition finish: one digit of at most 8 bits above the low bits");
+    PD_REQUIRE(n >= 1 && n < (1ull << 30), PIMDAL_E_INVALID, "partition finish: 1 to 2^30-1 rows");
+    const uint32_t F = 1u << bits;
+    RsPassDesc run{};
+    run.npass = 1;
+    run.shift[0] = lo_bits;
+    run.mask[0] = (1u << (bits - lo_bits)) - 1;
+    Scratch small(c, (RS_BINS + 8) * 4, s);   // offs | ctrs
+    uint32_t* offs = small.as<uint32_t>();
+    uint32_t* ctrs = offs + RS_BINS;
+    fill_u32(c, ctrs, 0, 8, s);
+    run_passes<uint64_t, true, true>(c, kin, vin, n, kout, vout, run, hist_hi, offs, ctrs, s);
+    launch(c, "part_bounds", k_part_bounds, dim3((uint32_t)std::min<uint64_t>(ceil_div(F + 1, 256), 1024)), dim3(256), 0,
+           s, (const uint32_t*)kout, n, F, bounds);
+}
+
 }  // namespace pd
 
 using namespace pd;

@@ -459,6 +459,43 @@ def test_q1_by_orderkey_2048_slot_tables(ctx, orc, n, groups):
     _cmp_q1(got, orc.q1_key(t, 2505), ordered=False)
 
 
+@pytest.mark.parametrize("n,rows_per_part,bad_row", [((1 << 21) + 4097, 2048, None), (3_000_001, 64, None),
+                                                   (700_003, 2048, None), (1_000_003, 512, ("price", 1 << 31))])
+def test_q1_by_orderkey_fused_first_partition_pass(ctx, orc, n, rows_per_part, bad_row):
+    """Filter + pack fused with the first hash partition pass (partitions of
+    9 to 16 bits: 8192-row tiles placed by the low bits, then the stable
+    pass over the top 8) and the separate pack + partition passes give the
+    oracle's groups, through the synchronous and the device-status forms;
+    ragged sizes, 16 partition bits, columns starting one element into
+    their allocations, and a kept row beyond the packed widths (general
+    path; the device-status form reports unsupported)."""
+    t = orc.fact_table(n + 1, groups=max(1, n // 4))
+    if bad_row:
+        t["date"][4321] = 0
+        t[bad_row[0]][4321] = bad_row[1]
+    full = {c: ctx.to_device(v) for c, v in t.items()}
+    d = {c: v.view(1, n) for c, v in full.items()}          # 4- and 8-byte columns off 16-byte alignment
+    h = {c: v[1:] for c, v in t.items()}
+    exp = orc.q1_key(h, 2505)
+    ctx.set_tune("gb_rows_per_part", rows_per_part)
+    try:
+        for fused in (1, 0):
+            ctx.set_tune("gb_q1_fused", fused)
+            _cmp_q1(ops.filter_groupby_q1(ctx, d, 2505, by_key=True), exp, ordered=False)
+            out, ng, st = ops.filter_groupby_q1_async(ctx, d, 2505, by_key=True)
+            ctx.sync()
+            status, got = ops.q1_to_host(out, ng, st)
+            if bad_row:
+                assert status == -2, fused          # PIMDAL_E_UNSUPPORTED
+            else:
+                assert status == 0, fused
+                got.pop("groups")
+                _cmp_q1(got, exp, ordered=False)
+    finally:
+        ctx.clear_tune("gb_q1_fused")
+        ctx.clear_tune("gb_rows_per_part")
+
+
 # ---------------- TPC-H Q6 shape ----------------
 
 @pytest.mark.gpu
