@@ -110,7 +110,9 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   smj_merge (1 direct output slots + match bitmap, 0 ticketed merge with
  *   decoupled look-back over per-tile match counts);
  *   gb_groups_hint, gb_rows_per_part, gb_lds_max_rows, gb_packed,
- *   gb_q1_slots (1024 or 2048);
+ *   gb_q1_slots (1024 or 2048), gb_q1_fused (1: Q1 by key filters and
+ *   packs rows inside its first hash partition pass; 0: a pack map, then
+ *   the partition passes);
  *   aggr_small (1: hash and sort aggregation first try per-block LDS tables
  *   for up to 4096 keys per block and 32768 in all; 0: skip that path),
  *   aggr_direct (1: hash and sort aggregation first try per-block LDS arrays
