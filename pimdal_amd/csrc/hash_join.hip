@@ -777,7 +777,12 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         const uint32_t bits1 = (uint32_t)__builtin_ctz(F1);
         Scratch HR(c, ((uint64_t)F1 + 1) * 4, s), HS(c, ((uint64_t)F1 + 1) * 4, s);
         Scratch R2(c, nb * 8, s), S2(c, np * 8, s);   // partitioned (key, payload) rows
-        hash_partition_1p(c, bk, bp ? bp : bk, nb, bits1, R2.as<uint2>(), HR.as<uint32_t>(), s);
+        // the build side's partition pass on the side stream, beside the
+        // probe side's (knob hj_overlap, default 1; not while capturing)
+        const bool overlap = c->tune_get("hj_overlap", 1) == 1 && !c->capture;
+        hipStream_t sb = overlap ? side_fork(c, s) : s;
+        SideJoin build_done(c, s, overlap);
+        hash_partition_1p(c, bk, bp ? bp : bk, nb, bits1, R2.as<uint2>(), HR.as<uint32_t>(), sb);
         hash_partition_1p(c, pk, pp ? pp : pk, np, bits1, S2.as<uint2>(), HS.as<uint32_t>(), s);
         Scratch wst(c, ((uint64_t)F1 + 1) * 4, s);
         const uint64_t max_wi = F1 + ceil_div(np, CH1);
@@ -785,6 +790,7 @@ uint64_t hash_join(pimdal_ctx* c, const uint32_t* bk, const uint32_t* bp, uint64
         launch(c, "hj_chunks", k_hj_plan, dim3(1), dim3(1024), 0, s, (const uint32_t*)HS.as<uint32_t>(), F1, CH1,
                wst.as<uint32_t>(), flags.as<uint32_t>(), wcnt.as<uint32_t>(), (uint32_t)(max_wi + 1));
         Scratch valid(c, max_wi * vw1 * 8, s);
+        build_done.join();
         if (hc_rows == 16)
             launch(c, "hj_join", k_hj_csr<16>, dim3((uint32_t)max_wi), dim3(HC_BLOCK), 0, s, (const uint2*)R2.as<uint2>(),
                    (const uint32_t*)HR.as<uint32_t>(), (const uint2*)S2.as<uint2>(), (const uint32_t*)HS.as<uint32_t>(),

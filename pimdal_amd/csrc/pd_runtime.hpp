@@ -95,6 +95,10 @@ struct pimdal_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, pd::KernelStat> stats;
+    // a second stream for independent work of one call (created on first
+    // use, default priority) and its fork / join events
+    hipStream_t side = nullptr;
+    hipEvent_t side_ev[2] = {nullptr, nullptr};
     // the plan being captured on `stream` (nullptr: eager).  While set,
     // scratch memory belongs to the plan, launches carry no profiling events,
     // host reads of device results are refused, and the radix passes use
@@ -159,6 +163,33 @@ private:
 // Kernel launch with optional hipEvent timing on the launch stream.
 hipEvent_t take_event(pimdal_ctx* c);
 void record_launch(pimdal_ctx* c, const char* name, hipEvent_t e0, hipStream_t s);
+// The context's side stream, started after the work enqueued so far on s
+// (fork); join makes s wait for everything enqueued on the side stream.
+hipStream_t side_fork(pimdal_ctx* c, hipStream_t s);
+void side_join(pimdal_ctx* c, hipStream_t s);
+// Joins the side stream into s at join() or, on an error path, when it goes
+// out of scope: buffers declared before it are freed on s only after the
+// side stream's work.
+class SideJoin {
+public:
+    SideJoin(pimdal_ctx* c, hipStream_t s, bool on) : c_(c), s_(s), on_(on) {}
+    ~SideJoin() {
+        if (on_) {
+            (void)hipEventRecord(c_->side_ev[1], c_->side);
+            (void)hipStreamWaitEvent(s_, c_->side_ev[1], 0);
+        }
+    }
+    void join() {
+        if (on_) side_join(c_, s_);
+        on_ = false;
+    }
+    SideJoin(const SideJoin&) = delete;
+    SideJoin& operator=(const SideJoin&) = delete;
+private:
+    pimdal_ctx* c_;
+    hipStream_t s_;
+    bool on_;
+};
 
 template <typename... KArgs, typename... Args>
 void launch(pimdal_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,

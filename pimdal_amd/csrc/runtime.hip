@@ -193,6 +193,21 @@ void record_launch(pimdal_ctx* c, const char* name, hipEvent_t e0, hipStream_t s
     c->pending.push_back({name, e0, e1});
 }
 
+hipStream_t side_fork(pimdal_ctx* c, hipStream_t s) {
+    if (!c->side) {
+        PD_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        for (hipEvent_t& e : c->side_ev) PD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    PD_HIP(hipEventRecord(c->side_ev[0], s));
+    PD_HIP(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
+    return c->side;
+}
+
+void side_join(pimdal_ctx* c, hipStream_t s) {
+    PD_HIP(hipEventRecord(c->side_ev[1], c->side));
+    PD_HIP(hipStreamWaitEvent(s, c->side_ev[1], 0));
+}
+
 static void resolve_pending(pimdal_ctx* c) {
     for (auto& p : c->pending) {
         PD_HIP(hipEventSynchronize(p.e1));
@@ -368,6 +383,11 @@ extern "C" int pimdal_ctx_destroy(pimdal_ctx* c) {
     if (c->readback) (void)hipHostFree(c->readback);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.e0); (void)hipEventDestroy(p.e1); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+        for (hipEvent_t e : c->side_ev) (void)hipEventDestroy(e);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
     PD_API_END
