@@ -105,7 +105,9 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   rows or 512 x 16), hj_fanout, hj_rows_per_part, hj_chunk;
  *   hj_onepass (1: up to 2^24 build rows, one partition pass per side into at
  *   most 1024 partitions probed through bucketed LDS tables; 0: 8-bit radix
- *   partition passes), hj_csr_rows (8 or 16 probe rows per thread per
+ *   partition passes), hj_overlap (1: the build side's partition pass runs
+ *   on a second stream of the context beside the probe side's; 0: both on
+ *   the context stream), hj_csr_rows (8 or 16 probe rows per thread per
  *   round), pw_items (16 or 8 rows per thread in the one-pass partition);
  *   smj_merge (1 direct output slots + match bitmap, 0 ticketed merge with
  *   decoupled look-back over per-tile match counts);
