@@ -615,10 +615,41 @@ __global__ __launch_bounds__(256) void k_ad_minmax(const uint32_t* __restrict__ 
     }
 }
 
-// Rows per partition of each AD_TILE-row tile: cnt[tile * F + p].
+// Min and max of a sample of the keys, rows i * stride (i < 2^16) spread
+// over the whole input: per-block pairs mm[2 b], mm[2 b + 1]; 64 blocks.
+constexpr uint32_t AD_SAMPLE = 1u << 16;
+__global__ __launch_bounds__(256) void k_ad_sample_minmax(const uint32_t* __restrict__ key, uint64_t n,
+                                                          uint64_t stride, uint32_t* __restrict__ mm) {
+    __shared__ uint32_t s_lo[256 / PD_WAVE], s_hi[256 / PD_WAVE];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    uint32_t q[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+        const uint64_t row = ((uint64_t)(blockIdx.x * 4 + u) * 256 + threadIdx.x) * stride;
+        q[u] = row < n ? key[row] : key[0];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) { lo = min(lo, q[u]); hi = max(hi, q[u]); }
+    for (int o = PD_WAVE / 2; o > 0; o /= 2) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+    const uint32_t w = threadIdx.x / PD_WAVE;
+    if (lane() == 0) { s_lo[w] = lo; s_hi[w] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 1; i < 256 / PD_WAVE; i++) { lo = min(lo, s_lo[i]); hi = max(hi, s_hi[i]); }
+        mm[2 * blockIdx.x] = lo;
+        mm[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// Rows per partition of each AD_TILE-row tile: cnt[tile * F + p].  A key
+// outside [kmin, kmin + F * AD_SPAN) is not counted and sets *outside.
 // Persistent: block b counts tiles b, b + gridDim.x, ...
 __global__ __launch_bounds__(512) void k_ad_count(const uint32_t* __restrict__ kin, uint64_t n, uint32_t kmin,
-                                                  uint32_t F, uint32_t* __restrict__ cnt) {
+                                                  uint32_t F, uint32_t* __restrict__ cnt,
+                                                  uint32_t* __restrict__ outside) {
     constexpr uint32_t Q = AD_TILE / 512;
     __shared__ uint32_t h[AD_MAXF];
     const uint32_t t = threadIdx.x, l = lane();
@@ -634,9 +665,16 @@ __global__ __launch_bounds__(512) void k_ad_count(const uint32_t* __restrict__ k
 #pragma unroll
         for (uint32_t i = 0; i < Q; i++) q[i] = __builtin_amdgcn_raw_buffer_load_b32(rk, (l + i * PD_WAVE) * 4u, 0, 0);
         __syncthreads();
+        bool out = false;
 #pragma unroll
-        for (uint32_t i = 0; i < Q; i++)
-            if (l + i * PD_WAVE < wrows) atomicAdd(&h[(q[i] - kmin) >> AD_SHIFT], 1u);
+        for (uint32_t i = 0; i < Q; i++) {
+            const uint32_t p = (q[i] - kmin) >> AD_SHIFT;
+            if (l + i * PD_WAVE < wrows) {
+                if (q[i] >= kmin && p < F) atomicAdd(&h[p], 1u);
+                else out = true;
+            }
+        }
+        if (pd::ballot(out) && l == 0) atomicOr(outside, 1u);
         __syncthreads();
         for (uint32_t i = t; i < F; i += 512) cnt[tile * F + i] = h[i];
         __syncthreads();
@@ -898,11 +936,18 @@ bool aggr_sum_dense(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint
     const int64_t maxr = std::min<int64_t>(c->tune_get("aggr_dense_max", (int64_t)AD_MAXF * AD_SPAN),
                                            (int64_t)AD_MAXF * AD_SPAN);
     if (maxr <= 0 || n == 0) return false;
-    const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64), (uint64_t)c->cu_count * 4));
-    uint32_t mm[2];
-    {
+    // (min, max) of the keys: every key (`exact`), or a 2^16-key sample
+    // spread over the input whose window the count pass then checks
+    auto key_range = [&](bool exact, uint32_t* mm) {
+        const uint32_t G = exact ? (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 64),
+                                                                                      (uint64_t)c->cu_count * 4))
+                                 : AD_SAMPLE / 1024;
         Scratch part(c, (uint64_t)G * 8, s);
-        launch(c, "ad_minmax", k_ad_minmax, dim3(G), dim3(256), 0, s, key, n, part.as<uint32_t>());
+        if (exact)
+            launch(c, "ad_minmax", k_ad_minmax, dim3(G), dim3(256), 0, s, key, n, part.as<uint32_t>());
+        else
+            launch(c, "ad_minmax", k_ad_sample_minmax, dim3(G), dim3(256), 0, s, key, n,
+                   std::max<uint64_t>(1, n / AD_SAMPLE), part.as<uint32_t>());
         std::vector<uint32_t> h(2 * G);
         read_small(c, h.data(), part.get(), (uint64_t)G * 8, s);
         mm[0] = 0xFFFFFFFFu;
@@ -911,40 +956,74 @@ bool aggr_sum_dense(pimdal_ctx* c, const uint32_t* key, const int64_t* val, uint
             mm[0] = std::min(mm[0], h[2 * b]);
             mm[1] = std::max(mm[1], h[2 * b + 1]);
         }
+    };
+    // knob aggr_dense_sample (default 1): the window from the sample, the
+    // sample's span plus 1/64 of it and one partition either side, from 0
+    // when that margin reaches it, and at most the widest window (still
+    // holding the sample's keys); a key outside it sends the call to the
+    // exact min/max
+    const bool sampled = c->tune_get("aggr_dense_sample", 1) == 1 && n > AD_SAMPLE;
+    uint32_t mm[2];
+    key_range(!sampled, mm);
+    const uint64_t span = (uint64_t)mm[1] - mm[0] + 1;
+    if (span > (uint64_t)maxr) return false;   // a sample's span is at most the keys'
+    uint64_t lo = mm[0], hi = mm[1];
+    if (sampled) {
+        const uint64_t margin = span / 64 + AD_SPAN;
+        lo = mm[0] >= margin ? mm[0] - margin : 0;
+        hi = std::min<uint64_t>(0xFFFFFFFFull, (uint64_t)mm[1] + margin);
+        if (hi - lo + 1 > (uint64_t)maxr) {   // widest window, from lo where it still holds the sample's max
+            if ((uint64_t)mm[1] + 1 > (uint64_t)maxr) lo = std::max<uint64_t>(lo, (uint64_t)mm[1] + 1 - (uint64_t)maxr);
+            hi = std::min<uint64_t>(0xFFFFFFFFull, lo + (uint64_t)maxr - 1);
+        }
     }
-    const uint64_t range = (uint64_t)mm[1] - mm[0] + 1;
-    if (range > (uint64_t)maxr) return false;
-    const uint32_t kmin = mm[0];
-    const uint32_t F = (uint32_t)ceil_div(range, AD_SPAN);
-    const uint32_t ntiles = (uint32_t)ceil_div(n, AD_TILE);
-    const uint32_t ngroups = (ntiles + AD_GT - 1) / AD_GT;
-    const uint32_t xt = (ntiles + 7) / 8;
-    // cnt[ntiles][F] | gstatus[ngroups][F] | start[F + 1] | 2 tickets | (8-aligned) status[F] | total
-    const uint64_t w32 = (uint64_t)ntiles * F + (uint64_t)ngroups * F + F + 1 + 2;
-    const uint64_t w32a = (w32 + 1) & ~1ull;
-    Scratch sc(c, w32a * 4 + ((uint64_t)F + 1) * 8, s);
-    uint32_t* cnt = sc.as<uint32_t>();
-    uint32_t* gst = cnt + (uint64_t)ntiles * F;
-    uint32_t* start = gst + (uint64_t)ngroups * F;
-    uint32_t* tick = start + F + 1;
-    uint64_t* pst = reinterpret_cast<uint64_t*>(cnt + w32a);
-    uint64_t* tot = pst + F;
-    fill_u32(c, gst, 0, (uint64_t)ngroups * F, s);
-    fill_u32(c, tick, 0, 2, s);
-    fill_u64(c, pst, 0, (uint64_t)F + 1, s);
-    Scratch pk(c, n * 4, s), pv(c, n * 8, s);
-    const uint32_t cgrid = std::min<uint32_t>(ntiles, (uint32_t)c->cu_count * 4);
-    launch(c, "ad_count", k_ad_count, dim3(cgrid), dim3(512), 0, s, key, n, kmin, F, cnt);
-    launch(c, "ad_tscan", k_ad_tscan, dim3(ngroups), dim3(1024), 0, s, cnt, ntiles, F, gst, tick, start);
-    launch(c, "ad_scatter", k_ad_scatter, dim3(8 * xt), dim3(AD_BLOCK), 0, s, key, val, pk.as<uint32_t>(),
-           pv.as<int64_t>(), n, kmin, F, (const uint32_t*)cnt, (const uint32_t*)start, xt);
-    launch(c, "ad_sum", k_ad_sum, dim3(F), dim3(AD_BLOCK), 0, s, (const uint32_t*)pk.as<uint32_t>(),
-           (const int64_t*)pv.as<int64_t>(), (const uint32_t*)start, F, kmin, tick + 1, pst, okey, osum, ocnt, cap,
-           tot);
-    const uint64_t g = read_u64(c, tot, s);
-    PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
-    *groups = g;
-    return true;
+    Scratch flag(c, 8, s);
+    for (bool check = sampled;; check = false) {
+        const uint64_t range = hi - lo + 1;
+        const uint32_t kmin = (uint32_t)lo;
+        const uint32_t F = (uint32_t)ceil_div(range, AD_SPAN);
+        const uint32_t ntiles = (uint32_t)ceil_div(n, AD_TILE);
+        const uint32_t ngroups = (ntiles + AD_GT - 1) / AD_GT;
+        const uint32_t xt = (ntiles + 7) / 8;
+        // cnt[ntiles][F] | gstatus[ngroups][F] | start[F + 1] | 2 tickets | (8-aligned) status[F] | total
+        const uint64_t w32 = (uint64_t)ntiles * F + (uint64_t)ngroups * F + F + 1 + 2;
+        const uint64_t w32a = (w32 + 1) & ~1ull;
+        Scratch sc(c, w32a * 4 + ((uint64_t)F + 1) * 8, s);
+        uint32_t* cnt = sc.as<uint32_t>();
+        uint32_t* gst = cnt + (uint64_t)ntiles * F;
+        uint32_t* start = gst + (uint64_t)ngroups * F;
+        uint32_t* tick = start + F + 1;
+        uint64_t* pst = reinterpret_cast<uint64_t*>(cnt + w32a);
+        uint64_t* tot = pst + F;
+        fill_u32(c, flag.get(), 0, 1, s);
+        const uint32_t cgrid = std::min<uint32_t>(ntiles, (uint32_t)c->cu_count * 4);
+        launch(c, "ad_count", k_ad_count, dim3(cgrid), dim3(512), 0, s, key, n, kmin, F, cnt, flag.as<uint32_t>());
+        if (check) {
+            uint32_t outside = 0;
+            read_small(c, &outside, flag.get(), 4, s);
+            if (outside) {   // the sample missed keys beyond its window
+                key_range(true, mm);
+                if ((uint64_t)mm[1] - mm[0] + 1 > (uint64_t)maxr) return false;
+                lo = mm[0];
+                hi = mm[1];
+                continue;
+            }
+        }
+        fill_u32(c, gst, 0, (uint64_t)ngroups * F, s);
+        fill_u32(c, tick, 0, 2, s);
+        fill_u64(c, pst, 0, (uint64_t)F + 1, s);
+        Scratch pk(c, n * 4, s), pv(c, n * 8, s);
+        launch(c, "ad_tscan", k_ad_tscan, dim3(ngroups), dim3(1024), 0, s, cnt, ntiles, F, gst, tick, start);
+        launch(c, "ad_scatter", k_ad_scatter, dim3(8 * xt), dim3(AD_BLOCK), 0, s, key, val, pk.as<uint32_t>(),
+               pv.as<int64_t>(), n, kmin, F, (const uint32_t*)cnt, (const uint32_t*)start, xt);
+        launch(c, "ad_sum", k_ad_sum, dim3(F), dim3(AD_BLOCK), 0, s, (const uint32_t*)pk.as<uint32_t>(),
+               (const int64_t*)pv.as<int64_t>(), (const uint32_t*)start, F, kmin, tick + 1, pst, okey, osum, ocnt,
+               cap, tot);
+        const uint64_t g = read_u64(c, tot, s);
+        PD_REQUIRE(g <= cap, PIMDAL_E_CAPACITY, "group-by output larger than capacity");
+        *groups = g;
+        return true;
+    }
 }
 
 // ---------------- direct key range: no table, no partition pass ----------------

@@ -174,11 +174,16 @@ def _dense_ran(ctx, fn, dk, dv):
 
 
 @pytest.mark.parametrize("case", ["span_2^24", "span_2^24_plus_1", "top_of_u32", "from_2^31", "ragged_8193",
-                                  "one_key_per_partition_edge"])
+                                  "one_key_per_partition_edge", "outlier_below_sample", "wrap_low_outlier"])
 def test_aggr_dense_key_range(ctx, orc, case):
     """The key-range path: ranges of exactly 2^24 keys (2048 partitions) and
     one more (the hashed path instead), keys ending at 0xFFFFFFFF, starting
-    at 2^31, a ragged row count, and keys on partition edges."""
+    at 2^31, a ragged row count, and keys on partition edges.  The window
+    comes from a 2^16-key sample (rows 0, 32, 64, ... here): a key the sample
+    missed below its window sends the call to the exact min/max (still the
+    key-range path), and so does a key near 0 under a window near the top
+    of the key space whose partitions run past 2^32 (then the hashed path:
+    the span is too wide)."""
     n = 1 << 21
     v = orc.gen_range(np.int64, 36, 2, n, -(1 << 45), 1 << 45)
     if case == "span_2^24":
@@ -196,6 +201,12 @@ def test_aggr_dense_key_range(ctx, orc, case):
         n = 8193
         k = orc.gen_range(np.uint32, 36, 1, n, 0, 1 << 20)
         v = v[:n]
+    elif case == "outlier_below_sample":
+        k = orc.gen_range(np.uint32, 36, 1, n, 1 << 20, 1 << 21)
+        k[1] = 100
+    elif case == "wrap_low_outlier":
+        k = orc.gen_range(np.uint32, 36, 1, n, 0xFFFFFFFF - (1 << 20), 0xFFFFFFFF)
+        k[1] = 3
     else:
         base = np.arange(n, dtype=np.uint64) % 2048
         k = (base * np.uint64(8192) + np.uint64(8191) * (np.arange(n) % 2).astype(np.uint64)).astype(np.uint32)
@@ -205,7 +216,7 @@ def test_aggr_dense_key_range(ctx, orc, case):
     try:
         for fn in (ops.hash_aggr, ops.sort_aggr):
             got, dense = _dense_ran(ctx, fn, dk, dv)
-            assert dense == (case != "span_2^24_plus_1"), (case, fn.__name__)
+            assert dense == (case not in ("span_2^24_plus_1", "wrap_low_outlier")), (case, fn.__name__)
             if fn is ops.hash_aggr:
                 got = canon(*got)
             assert all(np.array_equal(a, b) for a, b in zip(got, exp)), (case, fn.__name__)
