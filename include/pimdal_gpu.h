@@ -122,7 +122,10 @@ int pimdal_ctx_reset_stats(pimdal_ctx* ctx);
  *   values and every key falls in the window around it; 0: skip that path),
  *   aggr_dense_max (keys spanning at most this many values, default and
  *   largest 2^24: one range-partition pass into 8192-key partitions summed
- *   in key-indexed LDS arrays, groups in key order; 0: skip that path);
+ *   in key-indexed LDS arrays, groups in key order; 0: skip that path),
+ *   aggr_dense_sample (1: that path's key window from 2^16 keys sampled
+ *   across the input, widened, the exact min/max only when a key falls
+ *   outside it; 0: the exact min/max of every key);
  *   bloom_groups (1: probes with 64 to 1024 filters hold them in LDS in
  *   groups of 64, each group's blocks reading every probe key; 0: every
  *   filter bit read through L2), bloom_group_min_rows (smallest probe side
